@@ -1,0 +1,81 @@
+// ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// A C entry point around the reference's own StripedSmithWaterman::Aligner,
+// compiled together with the reference sources where they lie
+// (src/ssw/ssw.cpp, src/ssw/ssw_core.cpp) by oracle/Makefile into
+// oracle/_ref/libssw_ref.so.  It makes exactly klign's call
+// (src/klign.cpp:251-273):
+//   Align(query = SSW-query, ref = SSW-ref, |ref|,
+//         Filter(report_begin_position=true, report_cigar=false, 0, 32767),
+//         &aln, maskLen = max(|ref| / 2, 15))
+// and returns score, ref_begin/end and query_begin/end per pair.  Used to
+// record golden vectors and, on the host cores of the GPU machine, as the
+// "reference" CPU baseline.
+#include <pthread.h>
+
+#include <algorithm>
+#include <atomic>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ssw.hpp"
+
+extern "C" {
+typedef struct {
+  int32_t match, mismatch, gap_open, gap_ext, ambiguity;
+} ssw_ref_scoring;
+
+typedef struct {
+  int32_t score, ref_begin, ref_end, query_begin, query_end;
+} ssw_ref_result;
+
+int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                        const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
+                        int nthreads);
+}
+
+int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                        const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
+                        int nthreads) {
+  if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+  std::atomic<int64_t> next(0);
+  auto work = [&]() {
+    StripedSmithWaterman::Aligner aligner((uint8_t)sc->match, (uint8_t)sc->mismatch, (uint8_t)sc->gap_open,
+                                          (uint8_t)sc->gap_ext, (uint8_t)sc->ambiguity);
+    StripedSmithWaterman::Filter filter(true, false, 0, 32767);
+    std::string q, r;
+    for (;;) {
+      int64_t p0 = next.fetch_add(16);
+      if (p0 >= npairs) break;
+      int64_t p1 = std::min<int64_t>(p0 + 16, npairs);
+      for (int64_t p = p0; p < p1; p++) {
+        r.assign(a + a_off[p], (size_t)a_len[p]);
+        q.assign(b + b_off[p], (size_t)b_len[p]);
+        StripedSmithWaterman::Alignment aln;
+        aln.Clear();
+        int mask_len = std::max((int)(r.size() / 2), 15);
+        bool ok = aligner.Align(q.c_str(), r.c_str(), (int)r.size(), filter, &aln, mask_len);
+        ssw_ref_result &o = out[p];
+        if (!ok) {
+          o.score = -1;
+          o.ref_begin = o.ref_end = o.query_begin = o.query_end = -2;
+          continue;
+        }
+        o.score = aln.sw_score;
+        o.ref_begin = aln.ref_begin;
+        o.ref_end = aln.ref_end;
+        o.query_begin = aln.query_begin;
+        o.query_end = aln.query_end;
+      }
+    }
+  };
+  if (nthreads == 1) {
+    work();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+  }
+  return 0;
+}

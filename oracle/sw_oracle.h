@@ -1,0 +1,63 @@
+/*
+ * ORACLE -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Scalar, int32, O(m*n) restatement of the result semantics of the reference
+ * CPU aligner (StripedSmithWaterman / SSW, src/ssw/ssw_core.cpp and
+ * src/ssw/ssw.cpp of mhm2).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or, for the CPU baseline, as the thing timed on host cores).  The product
+ * path (mhm2_staging_amd) never links or calls it.
+ *
+ * Pinning (what checks this restatement independently of its own formula):
+ *   - golden vectors produced by the reference SSW itself, compiled from
+ *     /root/reference/src/ssw by oracle/Makefile into oracle/_ref/ and recorded
+ *     under tests/golden/ by tests/golden/make_golden.py;
+ *   - the 18 known-answer cases of test/ssw-test.cpp:151-238;
+ *   - brute-force path enumeration on tiny inputs (tests/test_oracle_bruteforce.py),
+ *     which defines the optimum and both tie-breaks from first principles.
+ */
+#ifndef MHMSW_ORACLE_H
+#define MHMSW_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t match;      /* > 0 */
+  int32_t mismatch;   /* cost, >= 0, used as -mismatch */
+  int32_t gap_open;   /* cost of a gap of length 1 */
+  int32_t gap_ext;    /* cost of each further gap position */
+  int32_t ambiguity;  /* cost of any comparison involving code 4 (N) */
+} oracle_scoring;
+
+/* Result in SSW naming: "ref" is SSW's outer sequence, "query" its striped
+ * inner sequence (ssw_core.hpp:46-56, s_align). */
+typedef struct {
+  int32_t score;
+  int32_t ref_begin, ref_end;
+  int32_t query_begin, query_end;
+} oracle_result;
+
+/* ASCII -> code 0..4, as kBaseTranslation (ssw.cpp:13-26): ACGT/acgt -> 0..3,
+ * U/u -> 0, every other byte -> 4. */
+int oracle_translate_base(int c);
+void oracle_translate(const char *s, int n, int8_t *out);
+
+/* One pair on translated codes.  ref (length m) is SSW-ref, query (length n)
+ * is SSW-query. */
+void oracle_align_codes(const int8_t *ref, int m, const int8_t *query, int n,
+                        const oracle_scoring *sc, oracle_result *out);
+
+/* Batch on ASCII: pair p aligns SSW-ref = a[a_off[p] .. a_off[p]+a_len[p])
+ * against SSW-query = b[b_off[p] .. +b_len[p]).  nthreads <= 0 means all
+ * hardware threads.  Returns 0. */
+int oracle_align_batch(int64_t npairs,
+                       const char *a, const int64_t *a_off, const int32_t *a_len,
+                       const char *b, const int64_t *b_off, const int32_t *b_len,
+                       const oracle_scoring *sc, oracle_result *out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

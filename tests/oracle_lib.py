@@ -1,0 +1,95 @@
+"""ctypes access to the oracle (test infrastructure only).
+
+``liboracle.so`` is the scalar C restatement in ``oracle/sw_oracle.c``;
+``_ref/libssw_ref.so`` is the reference SSW compiled from its own sources by
+``oracle/Makefile``.  Both take the same batch arguments: pair p aligns
+SSW-ref ``a[a_off[p]:a_off[p]+a_len[p]]`` against SSW-query ``b[...]``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libssw_ref.so")
+
+RESULT_DTYPE = np.dtype([("score", "<i4"), ("ref_begin", "<i4"), ("ref_end", "<i4"),
+                         ("query_begin", "<i4"), ("query_end", "<i4")])
+
+
+class Scoring(ctypes.Structure):
+    _fields_ = [("match", ctypes.c_int32), ("mismatch", ctypes.c_int32), ("gap_open", ctypes.c_int32),
+                ("gap_ext", ctypes.c_int32), ("ambiguity", ctypes.c_int32)]
+
+
+_libs = {}
+
+
+def _load(path, fn):
+    if path not in _libs:
+        if not os.path.exists(path):
+            return None
+        lib = ctypes.CDLL(path)
+        f = getattr(lib, fn)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Scoring), ctypes.c_void_p, ctypes.c_int]
+        _libs[path] = f
+    return _libs[path]
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def pack_strings(seqs):
+    """Concatenate byte strings; return (buffer, offsets int64, lengths int32)."""
+    seqs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    lens = np.array([len(s) for s in seqs], dtype=np.int32)
+    offs = np.zeros(len(seqs), dtype=np.int64)
+    if len(seqs) > 1:
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    buf = np.frombuffer(b"".join(seqs) + b"\0", dtype=np.uint8).copy()
+    return buf, offs, lens
+
+
+def _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads):
+    n = len(a_len)
+    out = np.zeros(n, dtype=RESULT_DTYPE)
+    sc = Scoring(*scoring)
+    a_buf = np.ascontiguousarray(a_buf, dtype=np.uint8)
+    b_buf = np.ascontiguousarray(b_buf, dtype=np.uint8)
+    a_off = np.ascontiguousarray(a_off, dtype=np.int64)
+    b_off = np.ascontiguousarray(b_off, dtype=np.int64)
+    a_len = np.ascontiguousarray(a_len, dtype=np.int32)
+    b_len = np.ascontiguousarray(b_len, dtype=np.int32)
+    fn(n, a_buf.ctypes.data, a_off.ctypes.data, a_len.ctypes.data, b_buf.ctypes.data, b_off.ctypes.data,
+       b_len.ctypes.data, ctypes.byref(sc), out.ctypes.data, nthreads)
+    return out
+
+
+def oracle_batch(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads=0):
+    fn = _load(ORACLE_SO, "oracle_align_batch")
+    if fn is None:
+        raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+    return _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads)
+
+
+def ref_batch(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads=0):
+    fn = _load(REF_SO, "ssw_ref_align_batch")
+    if fn is None:
+        raise RuntimeError("oracle/_ref/libssw_ref.so is not built")
+    return _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads)
+
+
+def oracle_pairs(refs, queries, scoring, nthreads=0):
+    a = pack_strings(refs)
+    b = pack_strings(queries)
+    return oracle_batch(*a, *b, scoring, nthreads)
+
+
+def ref_pairs(refs, queries, scoring, nthreads=0):
+    a = pack_strings(refs)
+    b = pack_strings(queries)
+    return ref_batch(*a, *b, scoring, nthreads)
