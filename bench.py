@@ -1,0 +1,180 @@
+"""Throughput benchmark of the batched Smith-Waterman engine on one MI355X.
+
+One step = one whole job over the staged batch (unpack, forward pass, 32-bit
+reruns, reverse pass, finalize) on inputs already resident in HBM.  Prints one
+JSON line.  Default workload: the uneven batched launch (1M pairs, reads
+U{100..250}, windows U{150..1024}, 0.5% N, 2% substitutions, 0.2% indels, 10%
+unrelated pairs) with mhm2's 1/3/5/2/2 scoring.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "batched SW GCUPS + pairs/s, bit-exact vs SSW; p99 latency at fixed QPS"
+WORKLOADS = {
+    1: "plain SW: 10k pairs, 150 bp reads vs 350 bp windows, 1% subs, 0.1% indels",
+    2: "uneven batched launch: 1M pairs, reads U{100..250} vs windows U{150..1024}, 0.5% N, 2% subs, 0.2% indels, 10% unrelated",
+    3: "length-skewed batch: 4M pairs, windows lognormal(median 400) in [150,1024], reads 150/250 (70/30), 25% chimeric tails",
+    4: "overflow-exact: 500k pairs, 250 bp reads vs 1024 bp tandem-repeat windows, 5% 4000 bp reads",
+}
+VALU_OPS_PER_CELL = 10  # t=H-gO, E, F (2 each), d=Hdiag+s, H=max of 4, best=max(best,H)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(batch, scoring, seconds_target=12.0):
+    """Reference SSW (oracle/_ref) or, when absent, the oracle port, timed on host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    kind = "reference" if oracle_lib.have_ref() else "port"
+    fn = oracle_lib.ref_batch if kind == "reference" else oracle_lib.oracle_batch
+    cores = min(os.cpu_count() or 1, 16)
+    n = batch.npairs
+    # calibrate on a small slice, then run a slice sized for ~seconds_target
+    k = min(n, 2000)
+    t = time.perf_counter()
+    fn(batch.reads, batch.read_off[:k], batch.read_len[:k], batch.ctgs, batch.ctg_off[:k], batch.ctg_len[:k],
+       scoring, cores)
+    dt = max(time.perf_counter() - t, 1e-6)
+    k2 = int(min(n, max(k, k * seconds_target / dt)))
+    sl = slice(0, k2)
+    cells = int(np.dot(batch.read_len[sl].astype(np.int64), batch.ctg_len[sl].astype(np.int64)))
+    t = time.perf_counter()
+    res = fn(batch.reads, batch.read_off[sl], batch.read_len[sl], batch.ctgs, batch.ctg_off[sl], batch.ctg_len[sl],
+             scoring, cores)
+    dt = time.perf_counter() - t
+    return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": kind,
+            "pairs_per_s": round(k2 / dt, 1),
+            "sample": f"first {k2} of {n} pairs of the same workload ({cells} cells), {dt:.1f} s"}, res, k2
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
+    ap.add_argument("--npairs", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--check", type=int, default=3000, help="pairs checked against the CPU aligner")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the path shards by pairs "
+              "with no exchange, so multi-GPU throughput is N times the one-GPU value by construction.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mhm2_staging_amd as m
+    from mhm2_staging_amd import _native
+
+    scoring = m.MHM2_SCORING
+    t0 = time.perf_counter()
+    batch = m.generate(args.config, npairs=args.npairs)
+    log(f"[bench] generated config {args.config}: {batch.npairs} pairs, {batch.cells():.3e} cells "
+        f"in {time.perf_counter() - t0:.1f}s")
+    al = m.Aligner(device=0, scoring=scoring, read_is_ssw_ref=True, mode=m.MODE_FAST)
+    t0 = time.perf_counter()
+    al.stage(batch)
+    log(f"[bench] staged (pack + H2D) in {time.perf_counter() - t0:.2f}s")
+    for _ in range(args.warmup):
+        al.align_staged()
+        al.wait()
+    log("[bench] warmup done")
+    job_ms, class_ms = [], []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        al.align_staged()
+        al.wait()
+        st = al.stats()
+        job_ms.append(st["job_ms"])
+        class_ms.append(st["class_ms"])
+    elapsed = time.perf_counter() - t
+    res = al.results()
+    cells = batch.cells()
+    gcups = cells * args.steps / elapsed / 1e9
+    log(f"[bench] fast form: {gcups:.1f} GCUPS, {elapsed / args.steps * 1e3:.1f} ms/step, stats {st['job_ms']:.1f} ms")
+
+    # dominant kernel: the forward class with the most device time
+    cm = np.mean(np.array(class_ms), 0)
+    k = int(np.argmax(cm))
+    kcells = st["class_cells"][k]
+    ops = VALU_OPS_PER_CELL * kcells
+    achieved = ops / (cm[k] * 1e-3) / 1e12
+    dev_peak = None
+    try:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+
+        class Prop(ctypes.Structure):
+            _fields_ = [("raw", ctypes.c_char * 4096)]
+        cus = ctypes.c_int(0)
+        clk = ctypes.c_int(0)
+        # hipDeviceAttributeMultiprocessorCount / hipDeviceAttributeClockRate
+        hip.hipDeviceGetAttribute(ctypes.byref(cus), 63, 0)
+        hip.hipDeviceGetAttribute(ctypes.byref(clk), 5, 0)
+        if cus.value > 0 and clk.value > 0:
+            dev_peak = cus.value * 64 * 2 * clk.value * 1e3 / 1e12
+    except Exception:
+        dev_peak = None
+    peak = 78.6  # 256 CUs x 64 lanes x 2 int16 per lane x 2.4 GHz, T lane-ops/s
+    roofline = {"bound": "valu", "kernel": _native.class_name(k), "achieved": round(achieved, 2), "peak": peak,
+                "unit": "TOP/s (int16 lane-ops)", "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel_ms": round(float(cm[k]), 3), "kernel_cells": int(kcells),
+                "ops_per_cell": VALU_OPS_PER_CELL}
+    if dev_peak:
+        roofline["peak_from_device_clock"] = round(dev_peak, 1)
+
+    out = {"metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "i16 (packed, exact i32 rerun)",
+           "data": "synthetic (seeded generator)",
+           "config": {"workload": WORKLOADS[args.config], "npairs": batch.npairs, "cells": cells,
+                      "scoring": "match 1, mismatch 3, gap open 5, gap extend 2, N 2", "form": "fast"},
+           "pairs_per_s": round(batch.npairs * args.steps / elapsed, 1),
+           "device_job_ms": round(float(np.mean(job_ms)), 3),
+           "stats": {kk: v for kk, v in st.items() if not kk.startswith("class_")},
+           "classes": {_native.class_name(i): {"pairs": st["class_pairs"][i], "ms": round(float(cm[i]), 3)}
+                       for i in range(_native.NUM_CLASSES) if st["class_pairs"][i]},
+           "roofline": roofline}
+
+    if not args.no_plain:
+        al.set_mode(m.MODE_PLAIN)
+        al.align_staged()
+        al.wait()
+        t = time.perf_counter()
+        al.align_staged()
+        al.wait()
+        pe = time.perf_counter() - t
+        pres = al.results()
+        plain_gcups = cells / pe / 1e9
+        out["plain_form"] = {"value": round(plain_gcups, 2), "unit": "GCUPS",
+                             "fast_over_plain": round(gcups / plain_gcups, 2),
+                             "identical_results": bool((pres == res).all())}
+        al.set_mode(m.MODE_FAST)
+        log(f"[bench] plain form: {plain_gcups:.1f} GCUPS")
+
+    if not args.no_cpu:
+        cb, cres, kk = cpu_baseline(batch, scoring)
+        out["cpu_baseline"] = cb
+        got = np.stack([res[c][:kk] for c in ("score", "read_begin", "read_end", "ctg_begin", "ctg_end")], 1)
+        want = np.stack([cres[c] for c in ("score", "ref_begin", "ref_end", "query_begin", "query_end")], 1)
+        out["bit_exact_check"] = {"pairs": int(kk), "mismatches": int((got != want).any(1).sum()),
+                                  "against": cb["kind"]}
+        log(f"[bench] cpu baseline {cb}")
+    al.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

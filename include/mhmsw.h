@@ -1,0 +1,123 @@
+/*
+ * mhmsw -- batched Smith-Waterman for mhm2's klign read-to-contig stage on
+ * AMD Instinct MI355X (gfx950).  C ABI: plain pointers and sizes only.
+ *
+ * Results follow the reference CPU aligner's semantics exactly (SSW,
+ * src/ssw/ssw_core.cpp:1104-1200 as klign calls it at src/klign.cpp:251-273):
+ * local alignment with affine gaps (a gap of length L costs
+ * gap_open + (L-1)*gap_ext), SSW's N handling and SSW's tie-breaks.
+ *
+ * Every entry point returns MHMSW_OK (0) or a negative MHMSW_E* code; the
+ * message of the last failure on a context is mhmsw_last_error(ctx).
+ * A context is not thread-safe: one caller thread at a time, at most one
+ * batch in flight, as with adept_sw::GPUDriver (src/klign.cpp:586-591).
+ */
+#ifndef MHMSW_H
+#define MHMSW_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHMSW_OK 0
+#define MHMSW_E_ARG (-1)       /* bad argument */
+#define MHMSW_E_HIP (-2)       /* HIP runtime error (message has the HIP error string) */
+#define MHMSW_E_STATE (-3)     /* call out of sequence (e.g. results before a batch) */
+#define MHMSW_E_NODEVICE (-4)  /* no usable GPU */
+#define MHMSW_E_SCORING (-5)   /* scoring outside what SSW semantics support */
+#define MHMSW_E_NOMEM (-6)     /* allocation failed */
+#define MHMSW_E_INTERNAL (-7)  /* an internal consistency check failed */
+
+/* Execution forms.  FAST: packed 16-bit persistent length-bucketed kernels with
+ * an exact 32-bit rerun for any pair whose score could overflow.  PLAIN: one
+ * 32-bit launch per batch, one wavefront per pair (the straightforward form). */
+#define MHMSW_MODE_FAST 0
+#define MHMSW_MODE_PLAIN 1
+
+typedef struct mhmsw_ctx mhmsw_ctx;
+
+/* Costs are positive magnitudes, as in mhm2's AlnScoring (include/klign.hpp:53-62). */
+typedef struct {
+  int32_t match;      /* 1..127 */
+  int32_t mismatch;   /* 0..127 */
+  int32_t gap_open;   /* 1..127, >= gap_ext */
+  int32_t gap_ext;    /* 1..127 */
+  int32_t ambiguity;  /* 0..127, cost of any comparison with N */
+} mhmsw_scoring;
+
+/* One alignment, 0-based inclusive coordinates.  A pair with score 0 reports
+ * read/ctg begin and end as SSW does for its SSW-ref (-1, -1) and SSW-query
+ * (0, 0) roles. */
+typedef struct {
+  int32_t score;
+  int32_t read_begin, read_end;
+  int32_t ctg_begin, ctg_end;
+} mhmsw_result;
+
+/* Work classes of the forward pass: 0..7 packed 16-bit with 16-lane groups
+ * and P = 1..8 register pairs per lane (rows <= 32*P), 8..15 packed 16-bit
+ * with 64-lane groups (rows <= 128*P), 16 the 32-bit form. */
+#define MHMSW_NUM_CLASSES 17
+
+/* Per-batch counters of the last completed batch. */
+typedef struct {
+  int64_t npairs;
+  int64_t cells;            /* forward cells, sum of |read| * |ctg| */
+  int64_t reverse_cells;    /* cells of the reverse (begin) pass */
+  int64_t fast_pairs;       /* pairs whose forward pass ran packed 16-bit */
+  int64_t plain_pairs;      /* pairs that ran 32-bit from the start */
+  int64_t rerun_pairs;      /* pairs rerun in 32-bit after the 16-bit pass */
+  double job_ms;            /* device time of the whole job (events on the context stream) */
+  double fwd_fast_ms;       /* device time of the forward packed kernels */
+  double fwd_plain_ms;      /* device time of the forward 32-bit kernels (incl. reruns) */
+  double rev_ms;            /* device time of the reverse pass */
+  double prep_ms;           /* unpack + scheduling kernels */
+  /* forward pass per work class: pairs, cells, device time of its launch */
+  int64_t class_pairs[MHMSW_NUM_CLASSES];
+  int64_t class_cells[MHMSW_NUM_CLASSES];
+  double class_ms[MHMSW_NUM_CLASSES];
+} mhmsw_stats;
+
+/* --- devices (replace adept_sw free functions, src/adept-sw/driver.cpp:28-77) --- */
+int mhmsw_device_count(int *count);
+int mhmsw_device_total_mem(int device_id, size_t *bytes);
+int mhmsw_device_name(int device_id, char *buf, size_t buflen);
+
+/* --- context (replaces GPUDriver::init / ~GPUDriver, driver.cpp:198-268) --- */
+int mhmsw_create(int device_id, mhmsw_ctx **out);
+int mhmsw_destroy(mhmsw_ctx *ctx);
+const char *mhmsw_last_error(const mhmsw_ctx *ctx);
+/* Scoring incl. ambiguity (GPUDriver::init takes no ambiguity, driver.hpp:45-46). */
+int mhmsw_set_scoring(mhmsw_ctx *ctx, const mhmsw_scoring *scoring);
+/* 1 (default, klign's CPU call klign.cpp:257): the read is SSW's outer
+ * sequence ("ref") and the contig window its striped sequence ("query").
+ * 0: the opposite roles.  Tie-breaks depend on this choice. */
+int mhmsw_set_orientation(mhmsw_ctx *ctx, int read_is_ssw_ref);
+int mhmsw_set_mode(mhmsw_ctx *ctx, int mode);
+
+/* --- batches (replace run_kernel_forwards / run_kernel_backwards,
+ *     kernel_is_done / kernel_block and get_aln_results, driver.cpp:270-417) ---
+ * Pair p aligns reads[read_off[p] .. +read_len[p]) with
+ * ctgs[ctg_off[p] .. +ctg_len[p]) (ASCII; A/C/G/T/U any case, anything else is N).
+ * mhmsw_stage packs and copies the batch to the device and returns when the
+ * host buffers may be reused; mhmsw_align_staged starts the job on the staged
+ * batch (it may be called again on the same staged batch);
+ * mhmsw_submit = stage + align_staged.  Results stay valid until the next
+ * stage/submit. */
+int mhmsw_stage(mhmsw_ctx *ctx, int64_t npairs, const char *reads, const int64_t *read_off, const int32_t *read_len,
+                const char *ctgs, const int64_t *ctg_off, const int32_t *ctg_len);
+int mhmsw_align_staged(mhmsw_ctx *ctx);
+int mhmsw_submit(mhmsw_ctx *ctx, int64_t npairs, const char *reads, const int64_t *read_off, const int32_t *read_len,
+                 const char *ctgs, const int64_t *ctg_off, const int32_t *ctg_len);
+/* 1 when the job has finished, 0 while it runs, < 0 on error. */
+int mhmsw_query(mhmsw_ctx *ctx);
+int mhmsw_wait(mhmsw_ctx *ctx);
+/* Copies npairs results (npairs must equal the batch size) to host memory. */
+int mhmsw_results(mhmsw_ctx *ctx, mhmsw_result *out, int64_t npairs);
+int mhmsw_get_stats(mhmsw_ctx *ctx, mhmsw_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""ctypes binding of libmhmsw.so (include/mhmsw.h, include/mhmsw_synth.h).
+
+The library is built in-tree by ``make -C mhm2_staging_amd``.  There is no
+fallback: every entry point raises if the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmhmsw.so")
+
+MHMSW_OK = 0
+MODE_FAST = 0
+MODE_PLAIN = 1
+NUM_CLASSES = 17
+
+
+def class_name(k):
+    """Work class k of the forward pass (include/mhmsw.h)."""
+    if k < 8:
+        return f"fast16_P{k + 1}"
+    if k < 16:
+        return f"fast64_P{k - 7}"
+    return "plain32"
+
+RESULT_DTYPE = np.dtype([("score", "<i4"), ("read_begin", "<i4"), ("read_end", "<i4"),
+                         ("ctg_begin", "<i4"), ("ctg_end", "<i4")])
+
+
+class Scoring(ctypes.Structure):
+    _fields_ = [("match", ctypes.c_int32), ("mismatch", ctypes.c_int32), ("gap_open", ctypes.c_int32),
+                ("gap_ext", ctypes.c_int32), ("ambiguity", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_int64), ("cells", ctypes.c_int64), ("reverse_cells", ctypes.c_int64),
+                ("fast_pairs", ctypes.c_int64), ("plain_pairs", ctypes.c_int64), ("rerun_pairs", ctypes.c_int64),
+                ("job_ms", ctypes.c_double), ("fwd_fast_ms", ctypes.c_double), ("fwd_plain_ms", ctypes.c_double),
+                ("rev_ms", ctypes.c_double), ("prep_ms", ctypes.c_double),
+                ("class_pairs", ctypes.c_int64 * 17), ("class_cells", ctypes.c_int64 * 17),
+                ("class_ms", ctypes.c_double * 17)]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name.startswith("class_") else v
+        return d
+
+
+class SynthBatch(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_int64), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("ctgs", ctypes.c_void_p), ("ctg_off", ctypes.c_void_p),
+                ("ctg_len", ctypes.c_void_p), ("reads_bytes", ctypes.c_int64), ("ctgs_bytes", ctypes.c_int64)]
+
+
+# every symbol include/mhmsw.h and include/mhmsw_synth.h declare
+EXPORTS = [
+    "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
+    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_stage",
+    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_get_stats",
+    "mhmsw_synth_generate", "mhmsw_synth_free",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libmhmsw.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C mhm2_staging_amd` "
+                           "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    sigs = {
+        "mhmsw_device_count": [ctypes.POINTER(ctypes.c_int)],
+        "mhmsw_device_total_mem": [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)],
+        "mhmsw_device_name": [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t],
+        "mhmsw_create": [ctypes.c_int, ctypes.POINTER(vp)],
+        "mhmsw_destroy": [vp],
+        "mhmsw_set_scoring": [vp, ctypes.POINTER(Scoring)],
+        "mhmsw_set_orientation": [vp, ctypes.c_int],
+        "mhmsw_set_mode": [vp, ctypes.c_int],
+        "mhmsw_stage": [vp, i64, vp, vp, vp, vp, vp, vp],
+        "mhmsw_align_staged": [vp],
+        "mhmsw_submit": [vp, i64, vp, vp, vp, vp, vp, vp],
+        "mhmsw_query": [vp],
+        "mhmsw_wait": [vp],
+        "mhmsw_results": [vp, vp, i64],
+        "mhmsw_get_stats": [vp, ctypes.POINTER(Stats)],
+        "mhmsw_synth_generate": [ctypes.c_int, i64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(SynthBatch)],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.mhmsw_last_error.argtypes = [vp]
+    L.mhmsw_last_error.restype = ctypes.c_char_p
+    L.mhmsw_synth_free.argtypes = [ctypes.POINTER(SynthBatch)]
+    L.mhmsw_synth_free.restype = None
+    _lib = L
+    return L
+
+
+class MhmswError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mhmsw error {code}: {msg}")
+        self.code = code

@@ -1,0 +1,197 @@
+"""Host-side mirror of mhm2's GPU Smith-Waterman interface.
+
+``Aligner`` is a thin owner of one mhmsw context (include/mhmsw.h).
+``GPUDriver`` mirrors adept_sw::GPUDriver (src/adept-sw/driver.hpp:39-55):
+same method names, argument meaning (penalties passed negative to ``init``)
+and call protocol; results come back as int16 arrays named as in
+AlignmentResults, where "ref" is the contig window and "query" the read.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native
+from .synth import PairBatch, from_strings
+
+# mhm2's alignment scoring, BASELINE "previous defaults" (CMakeDefinitions.txt:118-122)
+MHM2_SCORING = (1, 3, 5, 2, 2)
+# the values active in the reference tree (CMakeDefinitions.txt:113-117)
+MHM2_TREE_SCORING = (1, 1, 1, 1, 1)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Aligner:
+    """One device context: scoring, orientation, mode, and one batch at a time."""
+
+    def __init__(self, device=0, scoring=MHM2_SCORING, read_is_ssw_ref=True, mode=_native.MODE_FAST):
+        self._L = _native.lib()
+        self._ctx = ctypes.c_void_p()
+        rc = self._L.mhmsw_create(int(device), ctypes.byref(self._ctx))
+        if rc != 0:
+            raise _native.MhmswError(rc, f"mhmsw_create(device={device}) failed")
+        self._batch = None
+        self.set_scoring(scoring)
+        self.set_orientation(read_is_ssw_ref)
+        self.set_mode(mode)
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self._L.mhmsw_last_error(self._ctx)
+            raise _native.MhmswError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    def close(self):
+        if self._ctx:
+            self._L.mhmsw_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_scoring(self, scoring):
+        sc = _native.Scoring(*[int(x) for x in scoring])
+        self._check(self._L.mhmsw_set_scoring(self._ctx, ctypes.byref(sc)), "set_scoring")
+        self.scoring = tuple(int(x) for x in scoring)
+
+    def set_orientation(self, read_is_ssw_ref):
+        self._check(self._L.mhmsw_set_orientation(self._ctx, 1 if read_is_ssw_ref else 0), "set_orientation")
+
+    def set_mode(self, mode):
+        self._check(self._L.mhmsw_set_mode(self._ctx, int(mode)), "set_mode")
+
+    def stage(self, batch: PairBatch):
+        self._batch = batch  # keep the host buffers alive while referenced
+        self._check(self._L.mhmsw_stage(self._ctx, batch.npairs, _ptr(batch.reads), _ptr(batch.read_off),
+                                        _ptr(batch.read_len), _ptr(batch.ctgs), _ptr(batch.ctg_off),
+                                        _ptr(batch.ctg_len)), "stage")
+
+    def align_staged(self):
+        self._check(self._L.mhmsw_align_staged(self._ctx), "align_staged")
+
+    def submit(self, batch: PairBatch):
+        self.stage(batch)
+        self.align_staged()
+
+    def query(self):
+        return self._check(self._L.mhmsw_query(self._ctx), "query") == 1
+
+    def wait(self):
+        self._check(self._L.mhmsw_wait(self._ctx), "wait")
+
+    def results(self):
+        n = self._batch.npairs if self._batch is not None else 0
+        out = np.zeros(n, dtype=_native.RESULT_DTYPE)
+        self._check(self._L.mhmsw_results(self._ctx, _ptr(out), n), "results")
+        return out
+
+    def stats(self):
+        st = _native.Stats()
+        self._check(self._L.mhmsw_get_stats(self._ctx, ctypes.byref(st)), "get_stats")
+        return st.as_dict()
+
+    def align(self, batch: PairBatch):
+        self.submit(batch)
+        self.wait()
+        return self.results()
+
+
+class AlignmentResults:
+    """int16 arrays as in adept_sw::AlignmentResults (driver.hpp:17-23)."""
+
+    def __init__(self, res):
+        self.ref_begin = res["ctg_begin"].astype(np.int16)
+        self.query_begin = res["read_begin"].astype(np.int16)
+        self.ref_end = res["ctg_end"].astype(np.int16)
+        self.query_end = res["read_end"].astype(np.int16)
+        self.top_scores = res["score"].astype(np.int16)
+
+
+class GPUDriver:
+    """Python mirror of adept_sw::GPUDriver (src/adept-sw/driver.hpp:39-55)."""
+
+    def __init__(self):
+        self._al = None
+        self._results = None
+        self._submitted = False
+
+    def init(self, upcxx_rank_me, upcxx_rank_n, match_score, mismatch_score, gap_opening_score,
+             gap_extending_score, rlen_limit, ambiguity=1):
+        t = time.perf_counter()
+        ndev = get_num_node_gpus()
+        if ndev <= 0:
+            raise RuntimeError("GPU assert no GPU device available")
+        scoring = (match_score, -mismatch_score, -gap_opening_score, -gap_extending_score, ambiguity)
+        self._al = Aligner(device=upcxx_rank_me % ndev, scoring=scoring, read_is_ssw_ref=True)
+        return time.perf_counter() - t
+
+    def run_kernel_forwards(self, reads, contigs, max_read_size=0, max_contig_size=0):
+        if self._al is None:
+            raise RuntimeError("GPU assert run_kernel_forwards before init")
+        self._al.submit(from_strings(reads, contigs))
+        self._submitted = True
+        self._results = None
+        self._n = len(reads)
+
+    def run_kernel_backwards(self, reads, contigs, max_read_size=0, max_contig_size=0):
+        if not self._submitted:
+            raise RuntimeError("GPU assert run_kernel_backwards before run_kernel_forwards")
+        if len(reads) != self._n or len(contigs) != self._n:
+            raise RuntimeError("GPU assert run_kernel_backwards with a different batch")
+
+    def kernel_is_done(self):
+        if not self._submitted:
+            return True
+        if not self._al.query():
+            return False
+        self._fetch()
+        return True
+
+    def kernel_block(self):
+        if not self._submitted:
+            return
+        self._al.wait()
+        self._fetch()
+
+    def _fetch(self):
+        if self._results is None:
+            self._results = AlignmentResults(self._al.results())
+
+    def get_aln_results(self):
+        return self._results
+
+
+def get_num_node_gpus():
+    L = _native.lib()
+    n = ctypes.c_int(0)
+    if L.mhmsw_device_count(ctypes.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def get_tot_gpu_mem():
+    L = _native.lib()
+    b = ctypes.c_size_t(0)
+    if L.mhmsw_device_total_mem(0, ctypes.byref(b)) != 0:
+        return 0
+    return b.value
+
+
+def get_avail_gpu_mem_per_rank(tot_ranks, num_devices=0):
+    if num_devices == 0:
+        num_devices = get_num_node_gpus()
+    if not num_devices:
+        return 0
+    return int(get_tot_gpu_mem() * 0.8) // max(1, tot_ranks // num_devices)

@@ -1,0 +1,238 @@
+// adept_sw::GPUDriver over the mhmsw C ABI.
+//
+// Keeps the reference's call protocol (src/klign.cpp:361-367,
+// test/ssw-test.cpp:53-58): init once; per batch run_kernel_forwards ->
+// kernel_block -> run_kernel_backwards -> kernel_block -> get_aln_results.
+// run_kernel_forwards starts the whole job (both passes run on the device
+// without a host round trip); run_kernel_backwards only checks the sequence.
+// Errors throw std::runtime_error("GPU assert ...") like cudaErrchk
+// (src/adept-sw/driver.cpp:17-26).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/adept_sw/driver.hpp"
+#include "../../include/mhmsw.h"
+
+#ifndef ALN_AMBIGUITY_COST
+#define ALN_AMBIGUITY_COST 1
+#endif
+
+namespace adept_sw {
+
+struct DriverState {
+  mhmsw_ctx *ctx = nullptr;
+  mhmsw_scoring scoring{};
+  size_t cap = 0;
+  std::vector<short> store;  // 5 * cap
+  std::vector<mhmsw_result> res;
+  std::vector<char> rbuf, cbuf;
+  std::vector<int64_t> roff, coff;
+  std::vector<int32_t> rlen, clen;
+  int64_t npairs = 0;
+  bool submitted = false;
+  bool fetched = false;
+  bool backwards_called = false;
+};
+
+static void throw_ctx(DriverState *s, int rc, const char *what, const char *file, int line) {
+  std::ostringstream os;
+  os << "GPU assert " << what << " (mhmsw " << rc << ": " << (s && s->ctx ? mhmsw_last_error(s->ctx) : "") << ") "
+     << file << ":" << line << "\n";
+  throw std::runtime_error(os.str());
+}
+
+#define MHMSW_CHECK(s, expr)                                    \
+  do {                                                          \
+    int _rc = (expr);                                           \
+    if (_rc < 0) throw_ctx((s), _rc, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+size_t get_tot_gpu_mem() {
+  size_t b = 0;
+  if (mhmsw_device_total_mem(0, &b) != MHMSW_OK) return 0;  // device 0, as driver.cpp:40-44
+  return b;
+}
+
+size_t get_avail_gpu_mem_per_rank(int totRanks, int num_devices) {
+  if (num_devices == 0) num_devices = get_num_node_gpus();
+  if (!num_devices) return 0;
+  int ranksPerDevice = std::max(1, totRanks / num_devices);
+  return (size_t)(get_tot_gpu_mem() * 0.8) / ranksPerDevice;
+}
+
+std::string get_device_name(int device_id) {
+  char buf[256];
+  int rc = mhmsw_device_name(device_id, buf, sizeof(buf));
+  if (rc != MHMSW_OK) {
+    std::ostringstream os;
+    os << "GPU assert mhmsw_device_name(" << device_id << ") failed: " << rc << "\n";
+    throw std::runtime_error(os.str());
+  }
+  return std::string(buf);
+}
+
+int get_num_node_gpus() {
+  int n = 0;
+  if (mhmsw_device_count(&n) != MHMSW_OK) return 0;
+  return n;
+}
+
+bool initialize_gpu(double &time_to_initialize, int &device_count, size_t &total_mem) {
+  auto t = std::chrono::high_resolution_clock::now();
+  device_count = get_num_node_gpus();
+  if (device_count > 0) {
+    total_mem = get_tot_gpu_mem();
+    void *first_touch = nullptr;
+    if (hipHostMalloc(&first_touch, sizeof(double), hipHostMallocDefault) == hipSuccess) (void)hipHostFree(first_touch);
+  }
+  std::chrono::duration<double> elapsed = std::chrono::high_resolution_clock::now() - t;
+  time_to_initialize = elapsed.count();
+  return device_count > 0;
+}
+
+bool initialize_gpu() {
+  double t;
+  int c;
+  size_t m;
+  return initialize_gpu(t, c, m);
+}
+
+GPUDriver::~GPUDriver() {
+  if (driver_state) {
+    if (driver_state->ctx) mhmsw_destroy(driver_state->ctx);
+    delete driver_state;
+    driver_state = nullptr;
+  }
+}
+
+double GPUDriver::init(int upcxx_rank_me, int upcxx_rank_n, short match_score, short mismatch_score,
+                       short gap_opening_score, short gap_extending_score, int rlen_limit) {
+  (void)upcxx_rank_n;
+  (void)rlen_limit;  // buffers are sized per batch by actual bytes
+  auto t = std::chrono::high_resolution_clock::now();
+  if (driver_state) {
+    if (driver_state->ctx) mhmsw_destroy(driver_state->ctx);
+    delete driver_state;
+  }
+  driver_state = new DriverState();
+  int ndev = get_num_node_gpus();
+  if (ndev <= 0) throw std::runtime_error("GPU assert no GPU device available\n");
+  int device = upcxx_rank_me % ndev;  // as driver.cpp:220
+  MHMSW_CHECK(driver_state, mhmsw_create(device, &driver_state->ctx));
+  driver_state->scoring.match = match_score;
+  driver_state->scoring.mismatch = -mismatch_score;
+  driver_state->scoring.gap_open = -gap_opening_score;
+  driver_state->scoring.gap_ext = -gap_extending_score;
+  driver_state->scoring.ambiguity = ALN_AMBIGUITY_COST;
+  MHMSW_CHECK(driver_state, mhmsw_set_scoring(driver_state->ctx, &driver_state->scoring));
+  MHMSW_CHECK(driver_state, mhmsw_set_orientation(driver_state->ctx, 1));
+  std::chrono::duration<double> elapsed = std::chrono::high_resolution_clock::now() - t;
+  return elapsed.count();
+}
+
+void GPUDriver::set_ambiguity(short ambiguity_cost) {
+  if (!driver_state) throw std::runtime_error("GPU assert set_ambiguity before init\n");
+  driver_state->scoring.ambiguity = ambiguity_cost;
+  MHMSW_CHECK(driver_state, mhmsw_set_scoring(driver_state->ctx, &driver_state->scoring));
+}
+
+void GPUDriver::set_read_is_ssw_ref(bool read_is_ref) {
+  if (!driver_state) throw std::runtime_error("GPU assert set_read_is_ssw_ref before init\n");
+  MHMSW_CHECK(driver_state, mhmsw_set_orientation(driver_state->ctx, read_is_ref ? 1 : 0));
+}
+
+static void flatten(const std::vector<std::string> &v, std::vector<char> &buf, std::vector<int64_t> &off,
+                    std::vector<int32_t> &len) {
+  size_t tot = 0;
+  for (auto &s : v) tot += s.size();
+  buf.resize(tot + 1);
+  off.resize(v.size());
+  len.resize(v.size());
+  size_t o = 0;
+  for (size_t i = 0; i < v.size(); i++) {
+    off[i] = (int64_t)o;
+    len[i] = (int32_t)v[i].size();
+    std::copy(v[i].begin(), v[i].end(), buf.begin() + o);
+    o += v[i].size();
+  }
+  buf[tot] = 0;
+}
+
+void GPUDriver::run_kernel_forwards(std::vector<std::string> &reads, std::vector<std::string> &contigs,
+                                    unsigned maxReadSize, unsigned maxContigSize) {
+  (void)maxReadSize;
+  (void)maxContigSize;
+  DriverState *s = driver_state;
+  if (!s) throw std::runtime_error("GPU assert run_kernel_forwards before init\n");
+  if (reads.size() != contigs.size()) throw std::runtime_error("GPU assert reads and contigs differ in count\n");
+  flatten(reads, s->rbuf, s->roff, s->rlen);
+  flatten(contigs, s->cbuf, s->coff, s->clen);
+  s->npairs = (int64_t)reads.size();
+  MHMSW_CHECK(s, mhmsw_submit(s->ctx, s->npairs, s->rbuf.data(), s->roff.data(), s->rlen.data(), s->cbuf.data(),
+                              s->coff.data(), s->clen.data()));
+  s->submitted = true;
+  s->fetched = false;
+  s->backwards_called = false;
+}
+
+void GPUDriver::run_kernel_backwards(std::vector<std::string> &reads, std::vector<std::string> &contigs,
+                                     unsigned maxReadSize, unsigned maxContigSize) {
+  (void)maxReadSize;
+  (void)maxContigSize;
+  DriverState *s = driver_state;
+  if (!s || !s->submitted) throw std::runtime_error("GPU assert run_kernel_backwards before run_kernel_forwards\n");
+  if ((int64_t)reads.size() != s->npairs || (int64_t)contigs.size() != s->npairs)
+    throw std::runtime_error("GPU assert run_kernel_backwards with a different batch\n");
+  s->backwards_called = true;  // the begin pass was queued with the forward pass
+}
+
+static void fetch(DriverState *s, AlignmentResults &a) {
+  if (s->fetched) return;
+  const size_t n = (size_t)s->npairs;
+  s->res.resize(std::max<size_t>(n, 1));
+  MHMSW_CHECK(s, mhmsw_results(s->ctx, s->res.data(), s->npairs));
+  if (n > s->cap) {
+    s->cap = std::max<size_t>(n, KLIGN_GPU_BLOCK_SIZE);
+    s->store.assign(5 * s->cap, 0);
+  }
+  short *base = s->store.data();
+  a.ref_begin = base;
+  a.query_begin = base + s->cap;
+  a.ref_end = base + 2 * s->cap;
+  a.query_end = base + 3 * s->cap;
+  a.top_scores = base + 4 * s->cap;
+  for (size_t i = 0; i < n; i++) {
+    const mhmsw_result &r = s->res[i];
+    a.ref_begin[i] = (short)r.ctg_begin;
+    a.ref_end[i] = (short)r.ctg_end;
+    a.query_begin[i] = (short)r.read_begin;
+    a.query_end[i] = (short)r.read_end;
+    a.top_scores[i] = (short)r.score;
+  }
+  s->fetched = true;
+}
+
+bool GPUDriver::kernel_is_done() {
+  DriverState *s = driver_state;
+  if (!s || !s->submitted) return true;
+  int rc = mhmsw_query(s->ctx);
+  if (rc < 0) throw_ctx(s, rc, "mhmsw_query", __FILE__, __LINE__);
+  if (rc == 0) return false;
+  fetch(s, alignments);
+  return true;
+}
+
+void GPUDriver::kernel_block() {
+  DriverState *s = driver_state;
+  if (!s || !s->submitted) return;
+  MHMSW_CHECK(s, mhmsw_wait(s->ctx));
+  fetch(s, alignments);
+}
+
+}  // namespace adept_sw

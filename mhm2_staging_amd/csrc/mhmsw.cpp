@@ -1,0 +1,617 @@
+// Host runtime of the mhmsw C ABI (include/mhmsw.h).
+//
+// One context = one device, one HIP stream, device buffers that grow with the
+// largest batch seen (as adept_sw::GPUDriver keeps its buffers for the life of
+// the driver, src/adept-sw/driver.cpp:198-247, but sized by actual bytes
+// instead of 2*rlen_limit*20000).  A batch is packed on the host to 2-bit codes
+// plus an N bitmap, copied once, and then aligned entirely on the device:
+// unpack -> forward pass -> 32-bit rerun of flagged pairs -> reverse pass ->
+// finalize, with no host round trip between the passes (the reference needs
+// one to size its reverse launch, driver.cpp:395).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/mhmsw.h"
+#include "sw_common.h"
+#include "sw_launch.h"
+
+using namespace mhmsw;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t cap = 0;  // elements
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(n + n / 4, 64);
+    hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+template <class T>
+struct HostBuf {
+  T* ptr = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(n + n / 4, 64);
+    hipError_t e = hipHostMalloc((void**)&ptr, want * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+inline uint32_t base_code(unsigned char c) {
+  switch (c) {
+    case 'A': case 'a': case 'U': case 'u': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+  }
+}
+
+constexpr int kEvents = 6;
+
+}  // namespace
+
+struct mhmsw_ctx {
+  int device = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  mhmsw_scoring scoring{1, 3, 5, 2, 2};
+  ScoreParams sp{};
+  int mode = MHMSW_MODE_FAST;
+  int read_is_ref = 1;
+
+  // staged batch
+  int64_t npairs = 0;
+  int64_t total = 0;
+  int64_t cells = 0;
+  int max_plain_cols = 0;
+  bool fast_class_used[kNumClasses] = {};
+  bool staged = false;
+  bool in_flight = false;
+  bool have_results = false;
+
+  HostBuf<uint32_t> h_packed2, h_nmask;
+  HostBuf<int64_t> h_base;
+  HostBuf<int32_t> h_len;
+  DevBuf<uint32_t> d_packed2, d_nmask, d_codes;
+  DevBuf<int64_t> d_base;
+  DevBuf<int32_t> d_len, d_bucket, d_rank, d_errors;
+  DevBuf<unsigned long long> d_counters;
+  int64_t n_fast = 0, n_plain = 0;
+  DevBuf<PairEnd> d_fwd, d_rev;
+  DevBuf<SwTask> d_tasks;
+  DevBuf<PassSched> d_sched;
+  DevBuf<AlnResult> d_results;
+  hipEvent_t ev[kEvents] = {};
+  hipEvent_t cev[kNumClasses][2] = {};
+  bool class_launched[kNumClasses] = {};
+  int64_t class_pairs[kNumClasses] = {};
+  int64_t class_cells[kNumClasses] = {};
+  mhmsw_stats stats{};
+  int32_t h_errors[4] = {};
+};
+
+static int fail(mhmsw_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIP_OR_FAIL(c, expr)                                                                        \
+  do {                                                                                              \
+    hipError_t _e = (expr);                                                                         \
+    if (_e != hipSuccess) return fail((c), MHMSW_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                                      __FILE__, __LINE__);                                          \
+  } while (0)
+
+static void compute_score_params(mhmsw_ctx* c) {
+  const mhmsw_scoring& s = c->scoring;
+  ScoreParams& p = c->sp;
+  p.match = s.match;
+  p.mismatch = s.mismatch;
+  p.gap_open = s.gap_open;
+  p.gap_ext = s.gap_ext;
+  p.ambiguity = s.ambiguity;
+  p.read_is_ref = c->read_is_ref;
+  // packed kernel: column and row codes select one byte of an 8-byte table
+  // holding 16*(s + gap_open): bytes 0..3 = match, mismatch x3 (selector a^b),
+  // bytes 4..7 = ambiguity (selector has the N bit).  Needs every entry in
+  // [0, 255] and a 16*score range that fits 16 bits for realistic lengths.
+  const int m16 = 16 * (s.match + s.gap_open);
+  const int x16 = 16 * (s.gap_open - s.mismatch);
+  const int n16 = 16 * (s.gap_open - s.ambiguity);
+  p.fast_ok = (m16 <= 255 && x16 >= 0 && n16 >= 0) ? 1 : 0;
+  if (p.fast_ok) {
+    p.perm_lo = (uint32_t)m16 | ((uint32_t)x16 << 8) | ((uint32_t)x16 << 16) | ((uint32_t)x16 << 24);
+    p.perm_hi = (uint32_t)n16 * 0x01010101u;
+  } else {
+    p.perm_lo = p.perm_hi = 0;
+  }
+}
+
+// Host copy of the device's routing (make_task in sw_kernels.hip) for the
+// forward pass; used to launch only the classes that have work.
+static int fwd_class(const mhmsw_ctx* c, int32_t lref, int32_t lqry) {
+  if (lref <= 0 || lqry <= 0) return -1;
+  const bool wide = c->mode == MHMSW_MODE_PLAIN || !c->sp.fast_ok;
+  const int32_t shorter = std::min(lref, lqry), longer = std::max(lref, lqry);
+  if (!wide && shorter <= kFast16MaxRows && longer <= kFastMaxCols) return kClassFast16 + (shorter + 31) / 32 - 1;
+  if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) return kClassFast64 + (shorter + 127) / 128 - 1;
+  return kClassPlain;
+}
+
+extern "C" {
+
+int mhmsw_device_count(int* count) {
+  if (!count) return MHMSW_E_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return MHMSW_OK;
+}
+
+int mhmsw_device_total_mem(int device_id, size_t* bytes) {
+  if (!bytes) return MHMSW_E_ARG;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device_id) != hipSuccess) return MHMSW_E_NODEVICE;
+  *bytes = prop.totalGlobalMem;
+  return MHMSW_OK;
+}
+
+int mhmsw_device_name(int device_id, char* buf, size_t buflen) {
+  if (!buf || buflen == 0) return MHMSW_E_ARG;
+  if (hipDeviceGetPCIBusId(buf, (int)buflen, device_id) != hipSuccess) return MHMSW_E_NODEVICE;
+  return MHMSW_OK;
+}
+
+int mhmsw_create(int device_id, mhmsw_ctx** out) {
+  if (!out) return MHMSW_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MHMSW_E_NODEVICE;
+  if (device_id < 0 || device_id >= n) return MHMSW_E_ARG;
+  mhmsw_ctx* c = new (std::nothrow) mhmsw_ctx();
+  if (!c) return MHMSW_E_NOMEM;
+  c->device = device_id;
+  if (hipSetDevice(device_id) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MHMSW_E_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device_id) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MHMSW_E_HIP;
+  }
+  c->cus = prop.multiProcessorCount;
+  for (int i = 0; i < kEvents; i++) {
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+      mhmsw_destroy(c);
+      return MHMSW_E_HIP;
+    }
+  }
+  for (int i = 0; i < kNumClasses; i++) {
+    if (hipEventCreate(&c->cev[i][0]) != hipSuccess || hipEventCreate(&c->cev[i][1]) != hipSuccess) {
+      mhmsw_destroy(c);
+      return MHMSW_E_HIP;
+    }
+  }
+  if (c->d_sched.ensure(1) != hipSuccess || c->d_errors.ensure(4) != hipSuccess ||
+      c->d_counters.ensure(2) != hipSuccess) {
+    mhmsw_destroy(c);
+    return MHMSW_E_NOMEM;
+  }
+  compute_score_params(c);
+  *out = c;
+  return MHMSW_OK;
+}
+
+int mhmsw_destroy(mhmsw_ctx* c) {
+  if (!c) return MHMSW_E_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->h_packed2.release();
+  c->h_nmask.release();
+  c->h_base.release();
+  c->h_len.release();
+  c->d_packed2.release();
+  c->d_nmask.release();
+  c->d_codes.release();
+  c->d_base.release();
+  c->d_len.release();
+  c->d_bucket.release();
+  c->d_rank.release();
+  c->d_errors.release();
+  c->d_counters.release();
+  c->d_fwd.release();
+  c->d_rev.release();
+  c->d_tasks.release();
+  c->d_sched.release();
+  c->d_results.release();
+  for (int i = 0; i < kEvents; i++)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < kNumClasses; i++)
+    for (int k = 0; k < 2; k++)
+      if (c->cev[i][k]) (void)hipEventDestroy(c->cev[i][k]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MHMSW_OK;
+}
+
+const char* mhmsw_last_error(const mhmsw_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mhmsw_set_scoring(mhmsw_ctx* c, const mhmsw_scoring* s) {
+  if (!c || !s) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "scoring changed while a batch is in flight");
+  if (s->match < 1 || s->match > 127 || s->mismatch < 0 || s->mismatch > 127 || s->gap_open < 1 ||
+      s->gap_open > 127 || s->gap_ext < 1 || s->gap_ext > 127 || s->ambiguity < 0 || s->ambiguity > 127)
+    return fail(c, MHMSW_E_SCORING, "scoring out of range (match 1..127, costs 0..127, gaps 1..127)");
+  if (s->gap_ext > s->gap_open) return fail(c, MHMSW_E_SCORING, "gap_ext must not exceed gap_open");
+  // SSW's lazy-F loop never lets a gap be followed directly by the opposite
+  // gap (ssw_core.cpp:381).  Results equal textbook Gotoh only when such a
+  // path can never beat mismatches plus one gap.
+  const int worst = std::max(s->mismatch, s->ambiguity);
+  if (s->gap_open + s->gap_ext < worst || 2 * s->gap_ext < worst)
+    return fail(c, MHMSW_E_SCORING,
+                "scoring where a gap followed by the opposite gap beats mismatches (gap_open+gap_ext < %d or "
+                "2*gap_ext < %d) has no exact SSW equivalent",
+                worst, worst);
+  c->scoring = *s;
+  compute_score_params(c);
+  return MHMSW_OK;
+}
+
+int mhmsw_set_orientation(mhmsw_ctx* c, int read_is_ssw_ref) {
+  if (!c) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "orientation changed while a batch is in flight");
+  c->read_is_ref = read_is_ssw_ref ? 1 : 0;
+  compute_score_params(c);
+  return MHMSW_OK;
+}
+
+int mhmsw_set_mode(mhmsw_ctx* c, int mode) {
+  if (!c) return MHMSW_E_ARG;
+  if (mode != MHMSW_MODE_FAST && mode != MHMSW_MODE_PLAIN) return fail(c, MHMSW_E_ARG, "unknown mode %d", mode);
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "mode changed while a batch is in flight");
+  c->mode = mode;
+  return MHMSW_OK;
+}
+
+int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* read_off, const int32_t* read_len,
+                const char* ctgs, const int64_t* ctg_off, const int32_t* ctg_len) {
+  if (!c) return MHMSW_E_ARG;
+  if (npairs < 0 || (npairs > 0 && (!reads || !read_off || !read_len || !ctgs || !ctg_off || !ctg_len)))
+    return fail(c, MHMSW_E_ARG, "bad batch arguments");
+  if (npairs > (int64_t)1 << 30) return fail(c, MHMSW_E_ARG, "batch of %lld pairs is too large", (long long)npairs);
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  HIP_OR_FAIL(c, hipSetDevice(c->device));
+  c->staged = false;
+  c->have_results = false;
+  const int64_t nseq = 2 * npairs;
+  HIP_OR_FAIL(c, c->h_base.ensure((size_t)std::max<int64_t>(nseq, 1)));
+  HIP_OR_FAIL(c, c->h_len.ensure((size_t)std::max<int64_t>(nseq, 1)));
+  int64_t acc = kPadBases;
+  int64_t cells = 0;
+  int maxplain = 0;
+  bool used[kNumClasses] = {};
+  int64_t nfast = 0, nplain = 0;
+  int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
+  for (int64_t p = 0; p < npairs; p++) {
+    const int32_t rl = read_len[p], cl = ctg_len[p];
+    if (rl < 0 || cl < 0) return fail(c, MHMSW_E_ARG, "negative length at pair %lld", (long long)p);
+    if (std::min(rl, cl) > kPlainMaxCols)
+      return fail(c, MHMSW_E_ARG, "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl,
+                  cl);
+    if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20))
+      return fail(c, MHMSW_E_ARG, "pair %lld: sequence longer than 2^20 bases", (long long)p);
+    c->h_len.ptr[2 * p] = rl;
+    c->h_len.ptr[2 * p + 1] = cl;
+    c->h_base.ptr[2 * p] = acc;
+    acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+    c->h_base.ptr[2 * p + 1] = acc;
+    acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+    cells += (int64_t)rl * cl;
+    maxplain = std::max(maxplain, (int)std::min(rl, cl));
+    const int32_t lref = c->read_is_ref ? rl : cl, lqry = c->read_is_ref ? cl : rl;
+    const int cls = fwd_class(c, lref, lqry);
+    if (cls >= 0) {
+      used[cls] = true;
+      cpairs[cls]++;
+      ccells[cls] += (int64_t)rl * cl;
+      if (cls == kClassPlain) {
+        nplain++;
+      } else {
+        nfast++;
+      }
+    }
+  }
+  c->n_fast = nfast;
+  c->n_plain = nplain;
+  std::memcpy(c->class_pairs, cpairs, sizeof(cpairs));
+  std::memcpy(c->class_cells, ccells, sizeof(ccells));
+  acc += kPadBases;
+  const int64_t total = (acc + 31) / 32 * 32;
+  c->npairs = npairs;
+  c->total = total;
+  c->cells = cells;
+  c->max_plain_cols = maxplain;
+  std::memcpy(c->fast_class_used, used, sizeof(used));
+
+  // pack: 2-bit codes (16 per dword) and the N bitmap (32 per dword)
+  const size_t n2 = (size_t)(total / 16), nm = (size_t)(total / 32);
+  HIP_OR_FAIL(c, c->h_packed2.ensure(n2));
+  HIP_OR_FAIL(c, c->h_nmask.ensure(nm));
+  std::memset(c->h_packed2.ptr, 0, n2 * sizeof(uint32_t));
+  std::memset(c->h_nmask.ptr, 0, nm * sizeof(uint32_t));
+  {
+    int nth = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (nseq < 4096) nth = 1;
+    auto work = [&](int tid) {
+      for (int64_t s = tid; s < nseq; s += nth) {
+        const char* src = (s & 1) ? ctgs + ctg_off[s >> 1] : reads + read_off[s >> 1];
+        const int32_t len = c->h_len.ptr[s];
+        const int64_t b0 = c->h_base.ptr[s];  // multiple of 32: whole dwords of both arrays
+        uint32_t* two = c->h_packed2.ptr + b0 / 16;
+        uint32_t* nb = c->h_nmask.ptr + b0 / 32;
+        for (int32_t x = 0; x < len; x++) {
+          const uint32_t code = base_code((unsigned char)src[x]);
+          if (code == 4u) {
+            nb[x >> 5] |= 1u << (x & 31);
+          } else {
+            two[x >> 4] |= code << (2 * (x & 15));
+          }
+        }
+      }
+    };
+    if (nth == 1) {
+      work(0);
+    } else {
+      std::vector<std::thread> th;
+      for (int t = 0; t < nth; t++) th.emplace_back(work, t);
+      for (auto& t : th) t.join();
+    }
+  }
+  const size_t np1 = (size_t)std::max<int64_t>(npairs, 1);
+  HIP_OR_FAIL(c, c->d_packed2.ensure(n2));
+  HIP_OR_FAIL(c, c->d_nmask.ensure(nm));
+  HIP_OR_FAIL(c, c->d_codes.ensure((size_t)(2 * total / 8 + 64)));
+  HIP_OR_FAIL(c, c->d_base.ensure((size_t)std::max<int64_t>(nseq, 1)));
+  HIP_OR_FAIL(c, c->d_len.ensure((size_t)std::max<int64_t>(nseq, 1)));
+  HIP_OR_FAIL(c, c->d_bucket.ensure(np1));
+  HIP_OR_FAIL(c, c->d_rank.ensure(np1));
+  HIP_OR_FAIL(c, c->d_fwd.ensure(np1));
+  HIP_OR_FAIL(c, c->d_rev.ensure(np1));
+  HIP_OR_FAIL(c, c->d_results.ensure(np1));
+  HIP_OR_FAIL(c, c->d_tasks.ensure(np1 + 4 * kNumClasses + 64));
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr, c->h_packed2.ptr, n2 * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr, c->h_nmask.ptr, nm * 4, hipMemcpyHostToDevice, c->stream));
+  if (nseq > 0) {
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, c->h_base.ptr, (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, c->h_len.ptr, (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  c->staged = true;
+  return MHMSW_OK;
+}
+
+static BatchView make_view(mhmsw_ctx* c) {
+  BatchView b;
+  b.npairs = c->npairs;
+  b.total = c->total;
+  b.seq_base = c->d_base.ptr;
+  b.seq_len = c->d_len.ptr;
+  b.packed2 = c->d_packed2.ptr;
+  b.nmask = c->d_nmask.ptr;
+  b.codes = c->d_codes.ptr;
+  b.fwd = c->d_fwd.ptr;
+  b.rev = c->d_rev.ptr;
+  b.sched = c->d_sched.ptr;
+  b.tasks = c->d_tasks.ptr;
+  b.bucket = c->d_bucket.ptr;
+  b.rank = c->d_rank.ptr;
+  b.results = c->d_results.ptr;
+  b.errors = c->d_errors.ptr;
+  b.counters = c->d_counters.ptr;
+  b.sc = c->sp;
+  b.mode = c->mode;
+  return b;
+}
+
+int mhmsw_align_staged(mhmsw_ctx* c) {
+  if (!c) return MHMSW_E_ARG;
+  if (!c->staged) return fail(c, MHMSW_E_STATE, "no staged batch");
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  HIP_OR_FAIL(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  BatchView b = make_view(c);
+  const int fast_grid = std::max(1, c->cus * 8);
+  const int plain_grid = std::max(1, c->cus * 8);
+  const int plain_cols = std::max(1, c->max_plain_cols);
+  HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
+  HIP_OR_FAIL(c, launch_clear_counters(b, st));
+  if (c->npairs > 0) {
+    HIP_OR_FAIL(c, launch_unpack(b, st));
+    HIP_OR_FAIL(c, launch_schedule(b, kPassFwd, st));
+  }
+  HIP_OR_FAIL(c, hipEventRecord(c->ev[1], st));
+  int maxp16 = 0, maxp64 = 0;
+  for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = false;
+  if (c->npairs > 0) {
+    for (int P = kMaxP; P >= 1; P--) {
+      const int k = kClassFast64 + P - 1;
+      if (c->fast_class_used[k]) {
+        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
+        HIP_OR_FAIL(c, launch_fast(b, 64, P, fast_grid, st));
+        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
+        c->class_launched[k] = true;
+        maxp64 = std::max(maxp64, P);
+      }
+    }
+    for (int P = kMaxP; P >= 1; P--) {
+      const int k = kClassFast16 + P - 1;
+      if (c->fast_class_used[k]) {
+        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
+        HIP_OR_FAIL(c, launch_fast(b, 16, P, fast_grid, st));
+        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
+        c->class_launched[k] = true;
+        maxp16 = std::max(maxp16, P);
+      }
+    }
+  }
+  HIP_OR_FAIL(c, hipEventRecord(c->ev[2], st));
+  const bool any_fast = maxp16 > 0 || maxp64 > 0;
+  if (c->npairs > 0) {
+    if (c->fast_class_used[kClassPlain]) {
+      HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][0], st));
+      HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
+      HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][1], st));
+      c->class_launched[kClassPlain] = true;
+    }
+    if (any_fast) {
+      HIP_OR_FAIL(c, launch_schedule(b, kPassRerun, st));
+      HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
+    }
+  }
+  HIP_OR_FAIL(c, hipEventRecord(c->ev[3], st));
+  if (c->npairs > 0) {
+    HIP_OR_FAIL(c, launch_schedule(b, kPassRev, st));
+    if (any_fast) {
+      for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, fast_grid, st));
+      for (int P = kMaxP; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, fast_grid, st));
+    }
+    HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
+  }
+  HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
+  if (c->npairs > 0) HIP_OR_FAIL(c, launch_finalize(b, st));
+  HIP_OR_FAIL(c, hipEventRecord(c->ev[5], st));
+  c->in_flight = true;
+  c->have_results = false;
+  return MHMSW_OK;
+}
+
+int mhmsw_submit(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* read_off, const int32_t* read_len,
+                 const char* ctgs, const int64_t* ctg_off, const int32_t* ctg_len) {
+  int rc = mhmsw_stage(c, npairs, reads, read_off, read_len, ctgs, ctg_off, ctg_len);
+  if (rc != MHMSW_OK) return rc;
+  return mhmsw_align_staged(c);
+}
+
+static int complete(mhmsw_ctx* c) {
+  c->in_flight = false;
+  HIP_OR_FAIL(c, hipMemcpy(c->h_errors, c->d_errors.ptr, sizeof(c->h_errors), hipMemcpyDeviceToHost));
+  unsigned long long counters[2] = {0, 0};
+  HIP_OR_FAIL(c, hipMemcpy(counters, c->d_counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+  float ms[kEvents] = {};
+  for (int i = 1; i < kEvents; i++) HIP_OR_FAIL(c, hipEventElapsedTime(&ms[i], c->ev[i - 1], c->ev[i]));
+  mhmsw_stats& s = c->stats;
+  s.npairs = c->npairs;
+  s.cells = c->cells;
+  s.reverse_cells = (int64_t)counters[0];
+  s.fast_pairs = c->n_fast;
+  s.plain_pairs = c->n_plain;
+  s.rerun_pairs = c->h_errors[1];
+  for (int k = 0; k < kNumClasses; k++) {
+    s.class_pairs[k] = c->class_pairs[k];
+    s.class_cells[k] = c->class_cells[k];
+    float cm = 0.f;
+    if (c->class_launched[k]) HIP_OR_FAIL(c, hipEventElapsedTime(&cm, c->cev[k][0], c->cev[k][1]));
+    s.class_ms[k] = cm;
+  }
+  s.prep_ms = ms[1];
+  s.fwd_fast_ms = ms[2];
+  s.fwd_plain_ms = ms[3];
+  s.rev_ms = ms[4] + ms[5];
+  s.job_ms = ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+  c->have_results = true;
+  if (c->h_errors[0] != 0 || c->h_errors[2] != 0)
+    return fail(c, MHMSW_E_INTERNAL, "%d pairs: reverse pass did not reproduce the forward score; %d end points "
+                "outside their pair", c->h_errors[0], c->h_errors[2]);
+  return MHMSW_OK;
+}
+
+int mhmsw_query(mhmsw_ctx* c) {
+  if (!c) return MHMSW_E_ARG;
+  if (!c->in_flight) return c->have_results ? 1 : fail(c, MHMSW_E_STATE, "no batch submitted");
+  hipError_t e = hipEventQuery(c->ev[kEvents - 1]);
+  if (e == hipErrorNotReady) return 0;
+  if (e != hipSuccess) return fail(c, MHMSW_E_HIP, "hipEventQuery: %s", hipGetErrorString(e));
+  int rc = complete(c);
+  return rc == MHMSW_OK ? 1 : rc;
+}
+
+int mhmsw_wait(mhmsw_ctx* c) {
+  if (!c) return MHMSW_E_ARG;
+  if (!c->in_flight) return c->have_results ? MHMSW_OK : fail(c, MHMSW_E_STATE, "no batch submitted");
+  HIP_OR_FAIL(c, hipSetDevice(c->device));
+  HIP_OR_FAIL(c, hipEventSynchronize(c->ev[kEvents - 1]));
+  return complete(c);
+}
+
+int mhmsw_results(mhmsw_ctx* c, mhmsw_result* out, int64_t npairs) {
+  if (!c || (!out && npairs > 0)) return MHMSW_E_ARG;
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  if (!c->have_results) return fail(c, MHMSW_E_STATE, "no results: submit a batch first");
+  if (npairs != c->npairs) return fail(c, MHMSW_E_ARG, "results requested for %lld pairs, batch has %lld",
+                                       (long long)npairs, (long long)c->npairs);
+  static_assert(sizeof(mhmsw_result) == sizeof(AlnResult), "result layout");
+  if (npairs > 0)
+    HIP_OR_FAIL(c, hipMemcpy(out, c->d_results.ptr, (size_t)npairs * sizeof(AlnResult), hipMemcpyDeviceToHost));
+  if (c->h_errors[0] != 0 || c->h_errors[2] != 0)
+    return fail(c, MHMSW_E_INTERNAL, "%d pairs: reverse pass did not reproduce the forward score; %d end points "
+                "outside their pair", c->h_errors[0], c->h_errors[2]);
+  return MHMSW_OK;
+}
+
+int mhmsw_get_stats(mhmsw_ctx* c, mhmsw_stats* out) {
+  if (!c || !out) return MHMSW_E_ARG;
+  if (!c->have_results) return fail(c, MHMSW_E_STATE, "no completed batch");
+  *out = c->stats;
+  return MHMSW_OK;
+}
+
+}  // extern "C"

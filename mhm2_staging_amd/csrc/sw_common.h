@@ -1,0 +1,83 @@
+// Layouts shared by the HIP kernels (sw_kernels.hip) and the host runtime (mhmsw.cpp).
+#pragma once
+#include <stdint.h>
+
+namespace mhmsw {
+
+// Sequence codes, one 4-bit nibble per base, 8 per dword, lowest nibble first:
+// 0..3 = A,C,G,T, 4 = N (any non-ACGTU byte, as kBaseTranslation does in
+// src/ssw/ssw.cpp:13-26).
+constexpr int kCodeN = 4;
+
+// Every sequence starts on a 32-base boundary of the unpacked buffers, after a
+// front pad; kPadBases of slack sit before the first and after the last one so
+// that kernels may read a few dwords outside a sequence.
+constexpr int kSeqAlign = 32;
+constexpr int64_t kPadBases = 1024;
+
+// Classes of work (one persistent kernel launch each).
+constexpr int kMaxP = 8;                 // packed registers per lane (2 rows each)
+constexpr int kClassFast16 = 0;          // + (P-1): 16-lane groups, rows <= 32*P
+constexpr int kClassFast64 = kMaxP;      // + (P-1): 64-lane groups, rows <= 128*P
+constexpr int kClassPlain = 2 * kMaxP;   // 32-bit, one wavefront per pair
+constexpr int kNumClasses = 2 * kMaxP + 1;
+constexpr int kNumBins = 128;            // column-length bins of 32 columns
+constexpr int kNumBuckets = kNumClasses * kNumBins;
+
+constexpr int kFast16MaxRows = 32 * kMaxP;   // 256
+constexpr int kFast64MaxRows = 128 * kMaxP;  // 1024
+constexpr int kFastMaxCols = 60000;          // step counter stays below 2^16
+constexpr int kPlainMaxCols = 8192;          // LDS boundary row of the 32-bit kernel
+
+// Task flags.
+constexpr int32_t kTaskRowsAreRef = 1;  // rows hold SSW-ref (outer) positions
+constexpr int32_t kTaskReverse = 2;     // result goes to the reverse (begin) table
+
+struct SwTask {
+  int32_t pair;       // -1: empty slot
+  int32_t flags;
+  int64_t row_start;  // nibble index of row 0 in the code buffer
+  int64_t col_start;  // nibble index of column 0
+  int32_t row_len;
+  int32_t col_len;
+};
+
+// Per-pair end point of one pass: the lexicographic optimum
+// (max score, then least SSW-ref index i, then least SSW-query index j).
+struct PairEnd {
+  int32_t score;
+  int32_t i;
+  int32_t j;
+  int32_t flags;  // kEndOverflow: the packed pass may have saturated
+};
+constexpr int32_t kEndOverflow = 1;
+
+// Scheduling state of one pass (device memory).
+struct PassSched {
+  int32_t bucket_count[kNumBuckets];
+  int32_t bucket_base[kNumBuckets];
+  int32_t class_count[kNumClasses];   // real tasks
+  int32_t class_base[kNumClasses];    // first slot
+  int32_t class_slots[kNumClasses];   // padded slot count
+  int32_t class_next[kNumClasses];    // persistent work counter
+  int32_t npairs_routed;
+};
+
+// Final per-pair record, same layout as mhmsw_result in include/mhmsw.h.
+struct AlnResult {
+  int32_t score;
+  int32_t read_begin, read_end;
+  int32_t ctg_begin, ctg_end;
+};
+
+constexpr int32_t kEndWide = 2;  // the pair's forward pass ran in 32-bit (reverse must too)
+
+// Per-batch scoring, pre-scaled for the packed kernel.
+struct ScoreParams {
+  int32_t match, mismatch, gap_open, gap_ext, ambiguity;
+  uint32_t perm_lo, perm_hi;  // v_perm table of (s + gap_open) * 16 per class
+  int32_t fast_ok;            // scoring fits the packed kernel
+  int32_t read_is_ref;        // orientation
+};
+
+}  // namespace mhmsw

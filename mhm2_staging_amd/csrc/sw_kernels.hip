@@ -1,0 +1,724 @@
+// HIP kernels for batched Smith-Waterman with SSW result semantics, gfx950.
+//
+// Replaces gpu_bsw::sequence_dna_kernel / sequence_dna_reverse
+// (src/adept-sw/kernel.cpp:160-509) and the host-side packing and reverse
+// sizing of adept_sw::GPUDriver (src/adept-sw/driver.cpp:281-417).
+//
+// Recurrence (local Gotoh, SSW's, ssw_core.cpp:321-379):
+//   E(i,j) = max(E(i-1,j) - gE, H(i-1,j) - gO)
+//   F(i,j) = max(F(i,j-1) - gE, H(i,j-1) - gO)
+//   H(i,j) = max(0, H(i-1,j-1) + s(i,j), E(i,j), F(i,j))
+// End point: the lexicographic optimum (max H, least SSW-ref index i, least
+// SSW-query index j), which is what ssw_core.cpp:464-506 selects.  Begin point:
+// the same optimum on the reversed prefixes ending at the end point
+// (ssw_core.cpp:1162-1178).
+//
+// Two forms:
+//  * fast (packed 16-bit): a wavefront is split into 64/G groups of G lanes,
+//    each group aligns one pair.  Lane l of a group holds 2P consecutive rows
+//    as P packed registers (low half: rows 2Pl .. 2Pl+P-1, high half: rows
+//    2Pl+P .. 2Pl+2P-1).  The halves are two stages of a systolic array: at
+//    step t the low half of lane l works on column t-2l and the high half on
+//    column t-2l-1, so every dependency crosses one stage per step and is
+//    carried by one DPP row/wave shift.  Scores are kept multiplied by 16; the
+//    low 4 bits of a running per-row maximum hold 15 - (t mod 16), so the
+//    maximum also names the first step that reached it.  Every 16 steps the
+//    per-row maxima are folded into 32-bit (score, first step) records.
+//  * plain (32-bit): one wavefront per pair, one row per lane, rows in tiles
+//    of 64, the last row of a tile passed to the next through LDS.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sw_common.h"
+#include "sw_launch.h"
+
+namespace mhmsw {
+
+// ----------------------------------------------------------------------------
+// small helpers
+// ----------------------------------------------------------------------------
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t u32(v2i16 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ v2i16 vi(uint32_t x) { return __builtin_bit_cast(v2i16, x); }
+__device__ __forceinline__ v2u16 vu(uint32_t x) { return __builtin_bit_cast(v2u16, x); }
+__device__ __forceinline__ uint32_t u32u(v2u16 x) { return __builtin_bit_cast(uint32_t, x); }
+
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) { return u32(__builtin_elementwise_max(vi(a), vi(b))); }
+__device__ __forceinline__ uint32_t pk_maxu(uint32_t a, uint32_t b) {
+  return u32u(__builtin_elementwise_max(vu(a), vu(b)));
+}
+__device__ __forceinline__ uint32_t pk_adds(uint32_t a, uint32_t b) {
+  return u32(__builtin_elementwise_add_sat(vi(a), vi(b)));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return u32(vi(a) - vi(b)); }
+
+// Substitution selector: for 16-bit halves of codes a (rows) and b (column),
+// bits 0-1 become a^b and every other bit a|b.  bitop3 truth table with
+// src0=0xF0, src1=0xCC, src2=0xAA: src2 ? (src0^src1) : (src0|src1) = 0x7C.
+__device__ __forceinline__ uint32_t subst_sel(uint32_t a, uint32_t b) {
+  return __builtin_amdgcn_bitop3_b32(a, b, 0x00030003u, 0x7C);
+}
+
+constexpr uint32_t kKillCode = 0x1010u;  // column code outside the pair: selector >= 13 -> s' = 0xFFFF
+constexpr uint32_t kKillRow = 0x1C10u;   // row code of a padding row (same effect)
+constexpr uint32_t kRowHi = 0x0C00u;     // high selector byte of a row code: 0x0C -> byte 0x00
+
+__device__ __forceinline__ uint32_t code_at(const uint32_t* __restrict__ codes, int64_t idx) {
+  return (codes[idx >> 3] >> ((uint32_t)(idx & 7) * 4u)) & 0xFu;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t shift_in(uint32_t old, uint32_t src) {
+  // lane l receives lane l-1 of its group; lane 0 of a group keeps `old`
+  if constexpr (G == 16) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, 0x111, 0xF, 0xF, false);  // row_shr:1
+  } else {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, 0x138, 0xF, 0xF, false);  // wave_shr:1
+  }
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    int w = __shfl_xor(v, o, 64);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    int w = __shfl_xor(v, o, 64);
+    v = v < w ? v : w;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t end_key(int32_t score, int32_t i, int32_t j) {
+  return ((uint64_t)(uint32_t)score << 42) | ((uint64_t)(0x1FFFFFu - (uint32_t)i) << 21) |
+         (uint64_t)(0x1FFFFFu - (uint32_t)j);
+}
+__device__ __forceinline__ void key_decode(uint64_t k, int32_t& score, int32_t& i, int32_t& j) {
+  score = (int32_t)(k >> 42);
+  i = (int32_t)(0x1FFFFFu - (uint32_t)((k >> 21) & 0x1FFFFFu));
+  j = (int32_t)(0x1FFFFFu - (uint32_t)(k & 0x1FFFFFu));
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m, int w) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = (uint32_t)__shfl_xor((int)lo, m, w);
+  hi = (uint32_t)__shfl_xor((int)hi, m, w);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// ----------------------------------------------------------------------------
+// unpacking: 2-bit + N mask -> nibble codes (forward), then reversed copies
+// ----------------------------------------------------------------------------
+__global__ void unpack_kernel(BatchView b) {
+  const int64_t ndw = b.total >> 3;  // total is a multiple of 32
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndw; d += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t two = (b.packed2[d >> 1] >> ((uint32_t)(d & 1) * 16u)) & 0xFFFFu;
+    uint32_t nm = (b.nmask[d >> 2] >> ((uint32_t)(d & 3) * 8u)) & 0xFFu;
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      uint32_t c = ((nm >> k) & 1u) ? (uint32_t)kCodeN : (two >> (2 * k)) & 3u;
+      out |= c << (4 * k);
+    }
+    b.codes[d] = out;
+  }
+}
+
+// one block per sequence: reversed copy at [total + base, total + base + len)
+__global__ void reverse_kernel(BatchView b) {
+  const int64_t s = blockIdx.x;
+  if (s >= 2 * b.npairs) return;
+  const int64_t base = b.seq_base[s];
+  const int32_t len = b.seq_len[s];
+  const int32_t ndw = (len + 7) >> 3;
+  const int64_t out0 = (b.total + base) >> 3;
+  for (int32_t w = threadIdx.x; w < ndw; w += blockDim.x) {
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      int32_t x = w * 8 + k;
+      if (x < len) out |= code_at(b.codes, base + (len - 1 - x)) << (4 * k);
+    }
+    b.codes[out0 + w] = out;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// scheduling: classify -> schedule -> scatter
+// ----------------------------------------------------------------------------
+// Geometry of pair p in pass `pass`; false when the pair has no task.
+__device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, int& cls, int& bin) {
+  const int64_t sref = b.sc.read_is_ref ? 2 * p : 2 * p + 1;  // SSW-ref sequence
+  const int64_t sqry = b.sc.read_is_ref ? 2 * p + 1 : 2 * p;  // SSW-query sequence
+  const int32_t lref = b.seq_len[sref], lqry = b.seq_len[sqry];
+  const PairEnd fe = b.fwd[p];
+  int32_t ref_len, qry_len;
+  int64_t ref_start, qry_start;
+  bool wide = b.mode == 1 || !b.sc.fast_ok;
+  if (pass == kPassFwd) {
+    if (lref <= 0 || lqry <= 0) return false;
+    ref_len = lref;
+    qry_len = lqry;
+    ref_start = b.seq_base[sref];
+    qry_start = b.seq_base[sqry];
+  } else if (pass == kPassRerun) {
+    if (!(fe.flags & kEndOverflow)) return false;
+    wide = true;
+    ref_len = lref;
+    qry_len = lqry;
+    ref_start = b.seq_base[sref];
+    qry_start = b.seq_base[sqry];
+  } else {
+    if (fe.score <= 0) return false;
+    if (fe.i < 0 || fe.i >= lref || fe.j < 0 || fe.j >= lqry) {
+      atomicAdd(&b.errors[2], 1);  // an end point outside the pair: never reverse from it
+      return false;
+    }
+    if (fe.flags & kEndWide) wide = true;
+    // reversed prefixes [0..i] and [0..j]: R'[k] = R[i-k] = rev[len-1-i+k]
+    ref_len = fe.i + 1;
+    qry_len = fe.j + 1;
+    ref_start = b.total + b.seq_base[sref] + (lref - 1 - fe.i);
+    qry_start = b.total + b.seq_base[sqry] + (lqry - 1 - fe.j);
+  }
+  t.pair = (int32_t)p;
+  t.flags = pass == kPassRev ? kTaskReverse : 0;
+  const int32_t shorter = ref_len <= qry_len ? ref_len : qry_len;
+  const int32_t longer = ref_len <= qry_len ? qry_len : ref_len;
+  const bool ref_rows_if_short = ref_len <= qry_len;
+  if (!wide && shorter <= kFast16MaxRows && longer <= kFastMaxCols) {
+    const int P = (shorter + 31) / 32;
+    cls = kClassFast16 + P - 1;
+  } else if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) {
+    const int P = (shorter + 127) / 128;
+    cls = kClassFast64 + P - 1;
+  } else {
+    cls = kClassPlain;
+  }
+  bool rows_ref;
+  if (cls == kClassPlain) {
+    rows_ref = !ref_rows_if_short;  // rows = the longer sequence, columns (LDS row) = the shorter
+    if (ref_len == qry_len) rows_ref = true;
+  } else {
+    rows_ref = ref_rows_if_short;  // rows = the shorter sequence
+  }
+  if (rows_ref) {
+    t.flags |= kTaskRowsAreRef;
+    t.row_start = ref_start;
+    t.row_len = ref_len;
+    t.col_start = qry_start;
+    t.col_len = qry_len;
+  } else {
+    t.row_start = qry_start;
+    t.row_len = qry_len;
+    t.col_start = ref_start;
+    t.col_len = ref_len;
+  }
+  int bn = t.col_len / 32;
+  bin = bn < kNumBins ? bn : kNumBins - 1;
+  return true;
+}
+
+__global__ void clear_counters_kernel(BatchView b) {
+  if (threadIdx.x < 4) b.errors[threadIdx.x] = 0;
+  if (threadIdx.x < 2) b.counters[threadIdx.x] = 0;
+}
+
+__global__ void clear_sched_kernel(PassSched* s) {
+  int32_t* w = reinterpret_cast<int32_t*>(s);
+  const int n = (int)(sizeof(PassSched) / sizeof(int32_t));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0;
+}
+
+__global__ void classify_kernel(BatchView b, int pass) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < b.npairs; p += (int64_t)gridDim.x * blockDim.x) {
+    SwTask t;
+    int cls = 0, bin = 0;
+    bool has = make_task(b, pass, p, t, cls, bin);
+    if (!has) {
+      b.bucket[p] = -1;
+      if (pass == kPassFwd) b.fwd[p] = PairEnd{0, -1, 0, 0};
+      if (pass == kPassRev) b.rev[p] = PairEnd{0, 0, 0, 0};
+      continue;
+    }
+    const int bk = cls * kNumBins + bin;
+    b.bucket[p] = bk;
+    if (pass == kPassRerun) atomicAdd(&b.errors[1], 1);
+    if (pass == kPassRev) atomicAdd(&b.counters[0], (unsigned long long)t.row_len * (unsigned long long)t.col_len);
+    b.rank[p] = atomicAdd(&b.sched->bucket_count[bk], 1);
+  }
+}
+
+// single block: class totals, slot bases (bins in descending order inside a
+// class, so the longest columns are taken first), padding slots
+__global__ void schedule_kernel(BatchView b) {
+  PassSched* s = b.sched;
+  const int c = threadIdx.x;
+  __shared__ int32_t slots[kNumClasses];
+  if (c < kNumClasses) {
+    int32_t tot = 0;
+    for (int bn = kNumBins - 1; bn >= 0; bn--) tot += s->bucket_count[c * kNumBins + bn];
+    s->class_count[c] = tot;
+    int32_t padded = tot;
+    if (c < kClassFast64) padded = (tot + 3) & ~3;  // 4 groups per wavefront
+    slots[c] = padded;
+    s->class_slots[c] = padded;
+    s->class_next[c] = 0;
+  }
+  __syncthreads();
+  if (c < kNumClasses) {
+    int32_t base = 0;
+    for (int k = 0; k < c; k++) base += slots[k];
+    s->class_base[c] = base;
+    int32_t run = base;
+    for (int bn = kNumBins - 1; bn >= 0; bn--) {
+      s->bucket_base[c * kNumBins + bn] = run;
+      run += s->bucket_count[c * kNumBins + bn];
+    }
+    for (int32_t k = run; k < base + slots[c]; k++) {
+      SwTask e;
+      e.pair = -1;
+      e.flags = 0;
+      e.row_start = kPadBases;
+      e.col_start = kPadBases;
+      e.row_len = 0;
+      e.col_len = 0;
+      b.tasks[k] = e;
+    }
+  }
+}
+
+__global__ void scatter_kernel(BatchView b, int pass) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < b.npairs; p += (int64_t)gridDim.x * blockDim.x) {
+    const int bk = b.bucket[p];
+    if (bk < 0) continue;
+    SwTask t;
+    int cls = 0, bin = 0;
+    make_task(b, pass, p, t, cls, bin);
+    b.tasks[b.sched->bucket_base[bk] + b.rank[p]] = t;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// fast packed 16-bit kernel
+// ----------------------------------------------------------------------------
+template <int G, int P>
+struct FastState {
+  uint32_t rb[P];      // row codes (selector form), packed
+  uint32_t ho[P];      // 16*(H(r, c_last) - gO), packed
+  uint32_t hg[P];      // 16*F(r, c_last + 1), packed (gap along the columns)
+  uint32_t best[P];    // per-row running max of 16*H + tag, packed u16
+  uint32_t rec_lo[P];  // (H << 16) | (0xFFFF - first step), low-half rows
+  uint32_t rec_hi[P];  // same, high-half rows
+  uint32_t xa, xb;     // exchanged 16*(H-gO) of the row above, 2 steps / 1 step ago
+  uint32_t vg;         // exchanged 16*E of the first row
+  uint32_t q;          // column codes (low: this step, high: last step)
+  uint32_t w;          // next 8 column codes
+  uint32_t dcur, dnext;  // code dwords feeding the next refill
+};
+
+template <int G, int P, bool EDGE, int S>
+__device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, uint32_t tlo, uint32_t thi,
+                                          uint32_t go, uint32_t ge, uint32_t neg) {
+  uint32_t code = st.w & 0xFu;
+  st.w >>= 4;
+  if constexpr (EDGE) {
+    const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
+    code = kill ? kKillCode : code;
+  }
+  st.q = (st.q << 16) | code;
+  constexpr uint32_t tag = (uint32_t)(15 - S) * 0x00010001u;
+  uint32_t diag = st.xa;
+  uint32_t vg = st.vg;
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    const uint32_t sel = subst_sel(st.rb[p], st.q);
+    const uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
+    const uint32_t d = pk_adds(diag, sp);
+    uint32_t h = pk_max(d, st.hg[p]);
+    h = pk_max(h, vg);
+    h = pk_max(h, 0u);
+    diag = st.ho[p];
+    const uint32_t ho = pk_sub(h, go);
+    st.ho[p] = ho;
+    st.hg[p] = pk_max(pk_sub(st.hg[p], ge), ho);
+    vg = pk_max(pk_sub(vg, ge), ho);
+    st.best[p] = pk_maxu(st.best[p], h | tag);
+  }
+  const uint32_t last = st.ho[P - 1];
+  const uint32_t xnew = __builtin_amdgcn_alignbit(last, shift_in<G>(neg, last), 16);
+  const uint32_t vnew = __builtin_amdgcn_alignbit(vg, shift_in<G>(neg, vg), 16);
+  st.xa = st.xb;
+  st.xb = xnew;
+  st.vg = vnew;
+}
+
+template <int G, int P, bool EDGE>
+__device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, const uint32_t* __restrict__ codes,
+                                           int64_t& dw, uint32_t sh, uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge,
+                                           uint32_t neg) {
+  // w takes the 8 codes from dcur:dnext (loaded one refill earlier); the
+  // dword after them is requested now and used at the next refill.
+#define MHMSW_REFILL()                                            \
+  {                                                               \
+    st.w = __builtin_amdgcn_alignbit(st.dnext, st.dcur, sh);      \
+    st.dcur = st.dnext;                                           \
+    dw++;                                                         \
+    st.dnext = codes[dw];                                         \
+  }
+  MHMSW_REFILL();
+  fast_step<G, P, EDGE, 0>(st, t0 + 0, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 1>(st, t0 + 1, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 2>(st, t0 + 2, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 3>(st, t0 + 3, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 4>(st, t0 + 4, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 5>(st, t0 + 5, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 6>(st, t0 + 6, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 7>(st, t0 + 7, l, col_len, tlo, thi, go, ge, neg);
+  MHMSW_REFILL();
+  fast_step<G, P, EDGE, 8>(st, t0 + 8, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 9>(st, t0 + 9, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 10>(st, t0 + 10, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 11>(st, t0 + 11, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 12>(st, t0 + 12, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 13>(st, t0 + 13, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 14>(st, t0 + 14, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, 15>(st, t0 + 15, l, col_len, tlo, thi, go, ge, neg);
+#undef MHMSW_REFILL
+  // fold the chunk's per-row maxima into the (score, first step) records
+  const uint32_t cbase = 0xFFF0u - (uint32_t)t0;  // 0xFFFF - (t0 + 15 - tag) = cbase + tag
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    const uint32_t k = st.best[p];
+    const uint32_t klo = k & 0xFFFFu, khi = k >> 16;
+    const uint32_t clo = ((klo & 0xFFF0u) << 12) | ((klo & 0xFu) | cbase);
+    const uint32_t chi = ((khi & 0xFFF0u) << 12) | ((khi & 0xFu) | cbase);
+    st.rec_lo[p] = st.rec_lo[p] > clo ? st.rec_lo[p] : clo;
+    st.rec_hi[p] = st.rec_hi[p] > chi ? st.rec_hi[p] : chi;
+  }
+}
+
+template <int G, int P>
+__device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
+  const bool active = task.pair >= 0;
+  const int32_t row_len = active ? task.row_len : 0;
+  const int32_t col_len = active ? task.col_len : 0;
+  const int nsteps_need = wave_max_i32(active ? col_len + 2 * G - 1 : 0);
+  if (nsteps_need <= 0) return;
+  const int mincols = wave_min_i32(active ? col_len : 0x7FFFFFFF);
+  const int nsteps = (nsteps_need + 15) & ~15;
+
+  const uint32_t* __restrict__ codes = b.codes;
+  const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
+  const uint32_t go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
+  const uint32_t ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
+  const uint32_t neg = (uint32_t)(uint16_t)(-16 * b.sc.gap_open) * 0x00010001u;
+
+  FastState<G, P> st;
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    const int rlo = 2 * P * l + p, rhi = 2 * P * l + P + p;
+    const uint32_t clo = rlo < row_len ? (code_at(codes, task.row_start + rlo) | kRowHi) : kKillRow;
+    const uint32_t chi = rhi < row_len ? (code_at(codes, task.row_start + rhi) | kRowHi) : kKillRow;
+    st.rb[p] = clo | (chi << 16);
+    st.ho[p] = neg;
+    st.hg[p] = neg;
+    st.best[p] = 0;
+    st.rec_lo[p] = 0;
+    st.rec_hi[p] = 0;
+  }
+  st.xa = neg;
+  st.xb = neg;
+  st.vg = neg;
+  st.q = (kKillCode << 16) | kKillCode;
+  // column stream: at step t the low half reads column t - 2l
+  const int64_t cb = task.col_start - 2 * l;  // >= kPadBases - 2G >= 0
+  int64_t dw = (cb >> 3) + 1;  // index of st.dnext
+  const uint32_t sh = (uint32_t)(cb & 7) * 4u;
+  st.dcur = codes[dw - 1];
+  st.dnext = codes[dw];
+  st.w = 0;
+
+  for (int t0 = 0; t0 < nsteps; t0 += 16) {
+    const bool middle = t0 >= 2 * G && t0 + 16 <= mincols;
+    if (middle) {
+      fast_chunk<G, P, false>(st, t0, l, col_len, codes, dw, sh, tlo, thi, go, ge, neg);
+    } else {
+      fast_chunk<G, P, true>(st, t0, l, col_len, codes, dw, sh, tlo, thi, go, ge, neg);
+    }
+  }
+
+  // per-lane best (score, i, j), then the group's
+  uint64_t bestk = 0;
+  int32_t hmax = 0;
+  const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      const uint32_t rec = half ? st.rec_hi[p] : st.rec_lo[p];
+      const int32_t h = (int32_t)(rec >> 16);
+      hmax = h > hmax ? h : hmax;
+      const int32_t tf = 0xFFFF - (int32_t)(rec & 0xFFFFu);
+      const int32_t r = 2 * P * l + half * P + p;
+      const int32_t col = tf - (2 * l + half);
+      if (r < row_len && h > 0 && col >= 0 && col < col_len) {
+        const uint64_t k = rows_ref ? end_key(h, r, col) : end_key(h, col, r);
+        bestk = k > bestk ? k : bestk;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1) {
+    const uint64_t other = shfl_xor_u64(bestk, o, G);
+    bestk = other > bestk ? other : bestk;
+    const int32_t ho2 = __shfl_xor(hmax, o, G);
+    hmax = ho2 > hmax ? ho2 : hmax;
+  }
+  if (active && l == 0) {
+    PairEnd e;
+    if (bestk == 0) {
+      e = PairEnd{0, -1, 0, 0};
+    } else {
+      key_decode(bestk, e.score, e.i, e.j);
+      e.flags = 0;
+    }
+    if (hmax >= 2047) e.flags |= kEndOverflow;  // 16*H may have saturated: rerun in 32-bit
+    if (task.flags & kTaskReverse) {
+      b.rev[task.pair] = e;
+    } else {
+      b.fwd[task.pair] = e;
+    }
+  }
+}
+
+template <int G, int P>
+__global__ __launch_bounds__(256) void sw_fast_kernel(BatchView b) {
+  constexpr int GPW = 64 / G;
+  const int cls = (G == 16 ? kClassFast16 : kClassFast64) + P - 1;
+  const int lane = threadIdx.x & 63;
+  const int l = lane & (G - 1);
+  const int grp = lane / G;
+  PassSched* s = b.sched;
+  const int32_t nwt = s->class_slots[cls] / GPW;
+  const int32_t base = s->class_base[cls];
+  for (;;) {
+    int w = 0;
+    if (lane == 0) w = atomicAdd(&s->class_next[cls], 1);
+    w = __shfl(w, 0, 64);
+    if (w >= nwt) break;
+    const SwTask task = b.tasks[base + w * GPW + grp];
+    fast_pair<G, P>(b, task, l);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// plain 32-bit kernel: one wavefront per pair, rows in tiles of 64
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void sw_plain_kernel(BatchView b) {
+  extern __shared__ int2 bnd[];  // last row of the previous tile: (H, E) per column
+  const int k = threadIdx.x;
+  PassSched* s = b.sched;
+  const int32_t n = s->class_count[kClassPlain];
+  const int32_t base = s->class_base[kClassPlain];
+  const int32_t M = b.sc.match, X = b.sc.mismatch, A = b.sc.ambiguity, gO = b.sc.gap_open, gE = b.sc.gap_ext;
+  const int32_t NEG = -(1 << 28);
+  const uint32_t* __restrict__ codes = b.codes;
+  for (;;) {
+    int w = 0;
+    if (k == 0) w = atomicAdd(&s->class_next[kClassPlain], 1);
+    w = __shfl(w, 0, 64);
+    if (w >= n) break;
+    const SwTask task = b.tasks[base + w];
+    const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
+    const int32_t rows = task.row_len, cols = task.col_len;
+    uint64_t pair_best = 0;
+    const int ntiles = (rows + 63) / 64;
+    for (int tile = 0; tile < ntiles; tile++) {
+      const int32_t r = tile * 64 + k;
+      const bool row_ok = r < rows;
+      const uint32_t rc = row_ok ? code_at(codes, task.row_start + r) : 0u;
+      int32_t hprev = 0, hg = NEG, best = 0, bestc = 0;
+      int32_t up_h_prev = 0;  // H(r-1, c-1)
+      int32_t out_h = 0, out_e = NEG;
+      const int64_t cb = task.col_start - k;  // column c = t - k
+      for (int t = 0; t < cols + 63; t++) {
+        const int32_t c = t - k;
+        const bool col_ok = c >= 0 && c < cols;
+        // inputs from the row above at column c
+        int32_t top_h = 0, top_e = NEG;
+        if (k == 0 && tile > 0 && col_ok) {
+          const int2 v = bnd[c];
+          top_h = v.x;
+          top_e = v.y;
+        }
+        const int32_t up_h = __builtin_amdgcn_update_dpp(top_h, out_h, 0x138, 0xF, 0xF, false);
+        const int32_t up_e = __builtin_amdgcn_update_dpp(top_e, out_e, 0x138, 0xF, 0xF, false);
+        const uint32_t q = col_ok ? code_at(codes, cb + t) : 0u;
+        const int32_t sub = ((rc | q) & 4u) ? -A : (rc == q ? M : -X);
+        const int32_t e = max(up_e - gE, up_h - gO);     // gap along the rows
+        const int32_t f = max(hg - gE, hprev - gO);      // gap along the columns
+        int32_t h = max(max(up_h_prev + sub, e), max(f, 0));
+        const bool ok = row_ok && col_ok;
+        h = ok ? h : 0;
+        const int32_t f_keep = ok ? f : NEG;
+        const int32_t e_out = ok ? e : NEG;
+        if (ok && h > best) {
+          best = h;
+          bestc = c;
+        }
+        if (k == 63 && col_ok) bnd[c] = make_int2(h, e_out);
+        up_h_prev = up_h;
+        hprev = h;
+        hg = f_keep;
+        out_h = h;
+        out_e = e_out;
+      }
+      uint64_t key = 0;
+      if (row_ok && best > 0) key = rows_ref ? end_key(best, r, bestc) : end_key(best, bestc, r);
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const uint64_t other = shfl_xor_u64(key, o, 64);
+        key = other > key ? other : key;
+      }
+      pair_best = key > pair_best ? key : pair_best;
+      __syncthreads();
+    }
+    if (k == 0) {
+      PairEnd e;
+      if (pair_best == 0) {
+        e = PairEnd{0, -1, 0, kEndWide};
+      } else {
+        key_decode(pair_best, e.score, e.i, e.j);
+        e.flags = kEndWide;
+      }
+      if (task.flags & kTaskReverse) {
+        b.rev[task.pair] = e;
+      } else {
+        b.fwd[task.pair] = e;
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// results in klign's terms
+// ----------------------------------------------------------------------------
+__global__ void finalize_kernel(BatchView b) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < b.npairs; p += (int64_t)gridDim.x * blockDim.x) {
+    const PairEnd f = b.fwd[p];
+    int32_t rb = -1, re = -1, qb = 0, qe = 0, score = 0;
+    if (f.score > 0) {
+      const PairEnd r = b.rev[p];
+      score = f.score;
+      re = f.i;
+      qe = f.j;
+      rb = f.i - r.i;
+      qb = f.j - r.j;
+      if (r.score != f.score) atomicAdd(&b.errors[0], 1);
+    }
+    AlnResult o;
+    o.score = score;
+    if (b.sc.read_is_ref) {
+      o.read_begin = rb;
+      o.read_end = re;
+      o.ctg_begin = qb;
+      o.ctg_end = qe;
+    } else {
+      o.read_begin = qb;
+      o.read_end = qe;
+      o.ctg_begin = rb;
+      o.ctg_end = re;
+    }
+    b.results[p] = o;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// host-side launchers
+// ----------------------------------------------------------------------------
+static int grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 65535) g = 65535;
+  return (int)g;
+}
+
+hipError_t launch_clear_counters(const BatchView& b, hipStream_t st) {
+  hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const BatchView& b, hipStream_t st) {
+  hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 3, 256)), dim3(256), 0, st, b);
+  if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3((unsigned)(2 * b.npairs)), dim3(64), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
+  hipLaunchKernelGGL(clear_sched_kernel, dim3(8), dim3(256), 0, st, b.sched);
+  hipLaunchKernelGGL(classify_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, b, pass);
+  hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, st, b);
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, b, pass);
+  return hipGetLastError();
+}
+
+template <int G, int P>
+static hipError_t launch_fast_one(const BatchView& b, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((sw_fast_kernel<G, P>), dim3(grid), dim3(256), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st) {
+  if (group == 16) {
+    switch (P) {
+      case 1: return launch_fast_one<16, 1>(b, grid, st);
+      case 2: return launch_fast_one<16, 2>(b, grid, st);
+      case 3: return launch_fast_one<16, 3>(b, grid, st);
+      case 4: return launch_fast_one<16, 4>(b, grid, st);
+      case 5: return launch_fast_one<16, 5>(b, grid, st);
+      case 6: return launch_fast_one<16, 6>(b, grid, st);
+      case 7: return launch_fast_one<16, 7>(b, grid, st);
+      case 8: return launch_fast_one<16, 8>(b, grid, st);
+    }
+  } else {
+    switch (P) {
+      case 1: return launch_fast_one<64, 1>(b, grid, st);
+      case 2: return launch_fast_one<64, 2>(b, grid, st);
+      case 3: return launch_fast_one<64, 3>(b, grid, st);
+      case 4: return launch_fast_one<64, 4>(b, grid, st);
+      case 5: return launch_fast_one<64, 5>(b, grid, st);
+      case 6: return launch_fast_one<64, 6>(b, grid, st);
+      case 7: return launch_fast_one<64, 7>(b, grid, st);
+      case 8: return launch_fast_one<64, 8>(b, grid, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st) {
+  const size_t lds = (size_t)(max_cols > 0 ? max_cols : 1) * sizeof(int2);
+  hipLaunchKernelGGL(sw_plain_kernel, dim3(grid), dim3(64), lds, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const BatchView& b, hipStream_t st) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu) {
+  // all fast kernels use 256-thread blocks; P = 8 has the largest register file
+  (void)group;
+  (void)P;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sw_fast_kernel<16, 8>, 256, 0);
+}
+
+}  // namespace mhmsw

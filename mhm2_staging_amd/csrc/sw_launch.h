@@ -1,0 +1,43 @@
+// Host-callable launchers of the kernels in sw_kernels.hip.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "sw_common.h"
+
+namespace mhmsw {
+
+constexpr int kPassFwd = 0;    // forward pass: end points
+constexpr int kPassRerun = 1;  // 32-bit forward rerun of pairs the packed pass flagged
+constexpr int kPassRev = 2;    // reverse pass: begin points
+
+struct BatchView {
+  int64_t npairs;
+  int64_t total;            // padded bases of one region (forward codes at [0,total), reversed at [total,2*total))
+  const int64_t* seq_base;  // [2*npairs] first base of sequence s (read of pair p: 2p, contig window: 2p+1)
+  const int32_t* seq_len;   // [2*npairs]
+  const uint32_t* packed2;  // 2-bit codes, 16 per dword
+  const uint32_t* nmask;    // N flags, 32 per dword
+  uint32_t* codes;          // nibble codes, 8 per dword, 2*total nibbles plus slack
+  PairEnd* fwd;
+  PairEnd* rev;
+  PassSched* sched;
+  SwTask* tasks;
+  int32_t* bucket;  // [npairs]
+  int32_t* rank;    // [npairs]
+  AlnResult* results;
+  int32_t* errors;  // [0] reverse score mismatches, [1] pairs rerun in 32-bit
+  unsigned long long* counters;  // [0] cells of the reverse pass
+  ScoreParams sc;
+  int32_t mode;     // 0 fast, 1 plain
+};
+
+hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
+hipError_t launch_unpack(const BatchView& b, hipStream_t st);
+hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
+hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
+hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
+hipError_t launch_finalize(const BatchView& b, hipStream_t st);
+hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu);
+
+}  // namespace mhmsw

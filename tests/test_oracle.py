@@ -1,0 +1,103 @@
+"""The oracle (oracle/sw_oracle.c) against the reference's own answers.
+
+* golden vectors recorded from the reference SSW compiled from its sources
+  (tests/golden/make_golden.py);
+* the known-answer expectations of test/ssw-test.cpp:151-238;
+* when oracle/_ref is built (this container), fresh seeded corpora against
+  the reference directly.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from golden.kat_cases import BASELINE_SCORING, KAT_CASES, KAT_SCORINGS
+from oracle_lib import have_ref, oracle_batch, oracle_pairs, pack_strings, ref_batch
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FIELDS = ("score", "ref_begin", "ref_end", "query_begin", "query_end")
+
+
+def table(res):
+    return np.stack([res[k] for k in FIELDS], 1).astype(np.int32)
+
+
+def load_small():
+    z = np.load(os.path.join(GOLD, "golden_small.npz"))
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", ["ssw_default", "mhm2_tree", "cigar", "baseline"])
+def test_oracle_matches_reference_golden_small(name):
+    g = load_small()
+    sc = tuple(int(x) for x in g[f"scoring_{name}"])
+    res = oracle_batch(g["ref_buf"], g["ref_off"], g["ref_len"], g["qry_buf"], g["qry_off"], g["qry_len"], sc)
+    np.testing.assert_array_equal(table(res), g[f"res_{name}"])
+
+
+def test_oracle_matches_reference_kat_json():
+    with open(os.path.join(GOLD, "golden_kat.json")) as f:
+        kat = json.load(f)
+    for name, sc in kat["scorings"].items():
+        res = oracle_pairs([c[1] for c in KAT_CASES], [c[0] for c in KAT_CASES], tuple(sc))
+        assert table(res).tolist() == kat["results"][name], name
+        res2 = oracle_pairs([c[0] for c in KAT_CASES], [c[1] for c in KAT_CASES], tuple(sc))
+        assert table(res2).tolist() == kat["results"][name + "_swapped"], name
+
+
+@pytest.mark.parametrize("name", list(KAT_SCORINGS))
+def test_oracle_satisfies_reference_kat_expectations(name):
+    """The assertions of test/ssw-test.cpp:129-149 for the scorings that test uses."""
+    sc = KAT_SCORINGS[name]
+    res = oracle_pairs([c[1] for c in KAT_CASES], [c[0] for c in KAT_CASES], sc)
+    for case, r in zip(KAT_CASES, res):
+        q, ref, qs, qe, rs, re_ = case
+        if qs is None:  # check_not_alns
+            assert r["ref_end"] - r["ref_begin"] <= 2
+            assert r["query_end"] - r["query_begin"] <= 2
+            assert r["score"] <= 4
+        else:
+            assert (r["query_begin"], r["query_end"], r["ref_begin"], r["ref_end"]) == (qs, qe, rs, re_), (name, case)
+
+
+def test_oracle_kat_one_deletion_under_baseline_scoring():
+    """Under 1/3/5/2/2 the '1 del' case prefers the ungapped 11-base run (score 11 > 15-5)."""
+    res = oracle_pairs(["AAAATTTTCCCCGGGG"], ["AAAATTTCCCCGGGG"], BASELINE_SCORING)
+    assert res["score"][0] == 11
+    assert (res["ref_begin"][0], res["ref_end"][0]) == (5, 15)
+    assert (res["query_begin"][0], res["query_end"][0]) == (4, 14)
+
+
+def test_oracle_golden_config1_hash():
+    import hashlib
+    z = np.load(os.path.join(GOLD, "golden_config1.npz"))
+    t = z["results"].astype("<i4")
+    assert hashlib.sha256(np.ascontiguousarray(t).tobytes()).hexdigest() == str(z["sha256"])
+    # closed-form facts of config 1: 150 bp reads inside 350 bp windows
+    assert (t[:, 0] <= 150).all() and (t[:, 0] > 100).mean() > 0.95
+    assert ((t[:, 1] <= t[:, 2]) & (t[:, 3] <= t[:, 4])).all()
+
+
+def test_oracle_config1_subset_matches_golden():
+    import mhm2_staging_amd as m
+    z = np.load(os.path.join(GOLD, "golden_config1.npz"))
+    b = m.generate(1)
+    idx = np.arange(0, b.npairs, 7)
+    res = oracle_batch(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
+                       BASELINE_SCORING)
+    np.testing.assert_array_equal(table(res), z["results"][idx])
+
+
+@pytest.mark.skipif(not have_ref(), reason="oracle/_ref (reference SSW build) not present")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_matches_reference_fresh_corpus(seed):
+    rng = np.random.default_rng(seed)
+    refs, qrys = [], []
+    for i in range(1500):
+        alph = ["ACGT", "ACGTN", "AC", "GGGGT"][i % 4]
+        refs.append("".join(rng.choice(list(alph), int(rng.integers(1, 120)))))
+        qrys.append("".join(rng.choice(list(alph), int(rng.integers(1, 120)))))
+    a, b = pack_strings(refs), pack_strings(qrys)
+    for sc in [BASELINE_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1), (3, 2, 6, 1, 2)]:
+        np.testing.assert_array_equal(table(oracle_batch(*a, *b, sc)), table(ref_batch(*a, *b, sc)))
