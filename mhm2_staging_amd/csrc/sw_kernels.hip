@@ -96,6 +96,17 @@ __device__ __forceinline__ int wave_min_i32(int v) {
   return v;
 }
 
+// Wave-uniform copies (SGPR) of values every lane holds or that lane 0 holds.
+// Loop bounds and work indices must be uniform so that the wavefront never
+// splits around the persistent work loop.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t from_lane0(int32_t v) { return __builtin_amdgcn_readlane(v, 0); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ uint64_t end_key(int32_t score, int32_t i, int32_t j) {
   return ((uint64_t)(uint32_t)score << 42) | ((uint64_t)(0x1FFFFFu - (uint32_t)i) << 21) |
          (uint64_t)(0x1FFFFFu - (uint32_t)j);
@@ -410,9 +421,9 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const bool active = task.pair >= 0;
   const int32_t row_len = active ? task.row_len : 0;
   const int32_t col_len = active ? task.col_len : 0;
-  const int nsteps_need = wave_max_i32(active ? col_len + 2 * G - 1 : 0);
+  const int nsteps_need = uni(wave_max_i32(active ? col_len + 2 * G - 1 : 0));
   if (nsteps_need <= 0) return;
-  const int mincols = wave_min_i32(active ? col_len : 0x7FFFFFFF);
+  const int mincols = uni(wave_min_i32(active ? col_len : 0x7FFFFFFF));
   const int nsteps = (nsteps_need + 15) & ~15;
 
   const uint32_t* __restrict__ codes = b.codes;
@@ -510,9 +521,9 @@ __global__ __launch_bounds__(256) void sw_fast_kernel(BatchView b) {
   const int32_t nwt = s->class_slots[cls] / GPW;
   const int32_t base = s->class_base[cls];
   for (;;) {
-    int w = 0;
-    if (lane == 0) w = atomicAdd(&s->class_next[cls], 1);
-    w = __shfl(w, 0, 64);
+    // every lane takes part in the atomic (lane 0 adds 1, the rest 0): no
+    // divergent branch in the loop header, so the wavefront stays whole
+    const int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
     if (w >= nwt) break;
     const SwTask task = b.tasks[base + w * GPW + grp];
     fast_pair<G, P>(b, task, l);
@@ -532,11 +543,16 @@ __global__ __launch_bounds__(64) void sw_plain_kernel(BatchView b) {
   const int32_t NEG = -(1 << 28);
   const uint32_t* __restrict__ codes = b.codes;
   for (;;) {
-    int w = 0;
-    if (k == 0) w = atomicAdd(&s->class_next[kClassPlain], 1);
-    w = __shfl(w, 0, 64);
+    const int w = from_lane0(atomicAdd(&s->class_next[kClassPlain], k == 0 ? 1 : 0));
     if (w >= n) break;
-    const SwTask task = b.tasks[base + w];
+    const SwTask* tp = &b.tasks[base + w];
+    SwTask task;
+    task.pair = uni(tp->pair);
+    task.flags = uni(tp->flags);
+    task.row_start = uni64(tp->row_start);
+    task.col_start = uni64(tp->col_start);
+    task.row_len = uni(tp->row_len);
+    task.col_len = uni(tp->col_len);
     const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
     const int32_t rows = task.row_len, cols = task.col_len;
     uint64_t pair_best = 0;
