@@ -67,8 +67,19 @@ def main():
     ap.add_argument("--npairs", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--check", type=int, default=3000, help="pairs checked against the CPU aligner")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (3 times) to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the path shards by pairs "
               "with no exchange, so multi-GPU throughput is N times the one-GPU value by construction.",
