@@ -145,6 +145,14 @@ def main():
                 "ops_per_cell": VALU_OPS_PER_CELL}
     if dev_peak:
         roofline["peak_from_device_clock"] = round(dev_peak, 1)
+    # HBM bytes per launch come from rocprofv3 PMC runs recorded under profiles/
+    tfile = os.path.join(ROOT, "profiles", "traffic_uneven_1M.json")
+    if args.config == 2 and not args.npairs and os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        if roofline["kernel"] in tr.get("per_launch_bytes", {}):
+            roofline["traffic"] = tr["per_launch_bytes"][roofline["kernel"]]
+            roofline["traffic_source"] = tr["source"]
 
     out = {"metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
