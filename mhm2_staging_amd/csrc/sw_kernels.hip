@@ -23,7 +23,10 @@
 //    carried by one DPP row/wave shift.  Scores are kept multiplied by 16; the
 //    low 4 bits of a running per-row maximum hold 15 - (t mod 16), so the
 //    maximum also names the first step that reached it.  Every 16 steps the
-//    per-row maxima are folded into 32-bit (score, first step) records.
+//    per-row maxima are folded into 32-bit (score, first step) records.  Gap
+//    values are clamped at 0 with unsigned saturating subtracts (as SSW's
+//    are), so H = max(d, E, F) needs no extra max with 0: 12 VALU ops per
+//    packed register per step, i.e. 6 per cell.
 //  * plain (32-bit): one wavefront per pair, one row per lane, rows in tiles
 //    of 64, the last row of a tile passed to the next through LDS.
 #include <hip/hip_runtime.h>
@@ -53,6 +56,10 @@ __device__ __forceinline__ uint32_t pk_adds(uint32_t a, uint32_t b) {
   return u32(__builtin_elementwise_add_sat(vi(a), vi(b)));
 }
 __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return u32(vi(a) - vi(b)); }
+// unsigned saturating subtract: gap values are kept >= 0 (as SSW's are)
+__device__ __forceinline__ uint32_t pk_subu_sat(uint32_t a, uint32_t b) {
+  return u32u(__builtin_elementwise_sub_sat(vu(a), vu(b)));
+}
 
 // Substitution selector: for 16-bit halves of codes a (rows) and b (column),
 // bits 0-1 become a^b and every other bit a|b.  bitop3 truth table with
@@ -142,7 +149,10 @@ __global__ void unpack_kernel(BatchView b) {
   }
 }
 
-// one block per sequence: reversed copy at [total + base, total + base + len)
+// one block per sequence: reversed copy at [total + base, total + base + len).
+// Output dword w holds reversed positions 8w..8w+7 = source positions
+// len-1-8w down to len-8-8w: one funnel shift of two source dwords, then a
+// nibble reversal (byte permute + nibble swap).  Nibbles past len are 0.
 __global__ void reverse_kernel(BatchView b) {
   const int64_t s = blockIdx.x;
   if (s >= 2 * b.npairs) return;
@@ -151,12 +161,13 @@ __global__ void reverse_kernel(BatchView b) {
   const int32_t ndw = (len + 7) >> 3;
   const int64_t out0 = (b.total + base) >> 3;
   for (int32_t w = threadIdx.x; w < ndw; w += blockDim.x) {
-    uint32_t out = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      int32_t x = w * 8 + k;
-      if (x < len) out |= code_at(b.codes, base + (len - 1 - x)) << (4 * k);
-    }
+    const int64_t s0 = base + len - 8 - 8 * (int64_t)w;  // >= base - 8 >= 0 (front pad)
+    const uint32_t d0 = b.codes[s0 >> 3], d1 = b.codes[(s0 >> 3) + 1];
+    const uint32_t v = __builtin_amdgcn_alignbit(d1, d0, (uint32_t)(s0 & 7) * 4u);
+    const uint32_t y = __builtin_amdgcn_perm(v, v, 0x00010203u);  // byte reverse
+    uint32_t out = ((y >> 4) & 0x0F0F0F0Fu) | ((y << 4) & 0xF0F0F0F0u);
+    const int32_t keep = len - 8 * w;  // valid nibbles in this dword
+    if (keep < 8) out &= (1u << (4 * keep)) - 1u;
     b.codes[out0 + w] = out;
   }
 }
@@ -196,6 +207,22 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     // reversed prefixes [0..i] and [0..j]: R'[k] = R[i-k] = rev[len-1-i+k]
     ref_len = fe.i + 1;
     qry_len = fe.j + 1;
+    // Only cells with H' = score matter, and every alignment reaching the
+    // score in the reversed rectangle starts at its corner (the forward end
+    // is the only cell holding the score there).  An alignment over k+1 rows
+    // and l+1 columns scores at most match*min(k+1,l+1) minus the cost of
+    // |k-l| gap positions, so |k-l| <= D; rows and columns beyond the other
+    // side plus D cannot reach the score and are dropped (exact: DP values
+    // inside the kept rectangle do not depend on the dropped cells).
+    {
+      const int32_t L = ref_len < qry_len ? ref_len : qry_len;
+      const int64_t slack = (int64_t)b.sc.match * L - fe.score;
+      int64_t D = 0;
+      if (slack >= b.sc.gap_open) D = 1 + (slack - b.sc.gap_open) / b.sc.gap_ext;
+      const int64_t rcap = (int64_t)qry_len + D, qcap = (int64_t)ref_len + D;
+      if (ref_len > rcap) ref_len = (int32_t)rcap;
+      if (qry_len > qcap) qry_len = (int32_t)qcap;
+    }
     ref_start = b.total + b.seq_base[sref] + (lref - 1 - fe.i);
     qry_start = b.total + b.seq_base[sqry] + (lqry - 1 - fe.j);
   }
@@ -324,12 +351,13 @@ template <int G, int P>
 struct FastState {
   uint32_t rb[P];      // row codes (selector form), packed
   uint32_t ho[P];      // 16*(H(r, c_last) - gO), packed
-  uint32_t hg[P];      // 16*F(r, c_last + 1), packed (gap along the columns)
+  uint32_t hg[P];      // 16*F(r, c_last + 1), packed (gap along the columns), clamped >= 0
   uint32_t best[P];    // per-row running max of 16*H + tag, packed u16
   uint32_t rec_lo[P];  // (H << 16) | (0xFFFF - first step), low-half rows
   uint32_t rec_hi[P];  // same, high-half rows
   uint32_t xa, xb;     // exchanged 16*(H-gO) of the row above, 2 steps / 1 step ago
-  uint32_t vg;         // exchanged 16*E of the first row
+  uint32_t vg;         // exchanged 16*E of the first row, clamped >= 0
+  uint32_t xs, vs;     // DPP shift results; lane 0 of a group never receives and keeps its boundary
   uint32_t q;          // column codes (low: this step, high: last step)
   uint32_t w;          // next 8 column codes
   uint32_t dcur, dnext;  // code dwords feeding the next refill
@@ -353,19 +381,21 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
     const uint32_t sel = subst_sel(st.rb[p], st.q);
     const uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
     const uint32_t d = pk_adds(diag, sp);
+    // E, F >= 0, so H = max(d, E, F) needs no separate max with 0
     uint32_t h = pk_max(d, st.hg[p]);
     h = pk_max(h, vg);
-    h = pk_max(h, 0u);
     diag = st.ho[p];
     const uint32_t ho = pk_sub(h, go);
     st.ho[p] = ho;
-    st.hg[p] = pk_max(pk_sub(st.hg[p], ge), ho);
-    vg = pk_max(pk_sub(vg, ge), ho);
+    st.hg[p] = pk_max(pk_subu_sat(st.hg[p], ge), ho);
+    vg = pk_max(pk_subu_sat(vg, ge), ho);
     st.best[p] = pk_maxu(st.best[p], h | tag);
   }
   const uint32_t last = st.ho[P - 1];
-  const uint32_t xnew = __builtin_amdgcn_alignbit(last, shift_in<G>(neg, last), 16);
-  const uint32_t vnew = __builtin_amdgcn_alignbit(vg, shift_in<G>(neg, vg), 16);
+  st.xs = shift_in<G>(st.xs, last);
+  st.vs = shift_in<G>(st.vs, vg);
+  const uint32_t xnew = __builtin_amdgcn_alignbit(last, st.xs, 16);
+  const uint32_t vnew = __builtin_amdgcn_alignbit(vg, st.vs, 16);
   st.xa = st.xb;
   st.xb = xnew;
   st.vg = vnew;
@@ -440,14 +470,16 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
     const uint32_t chi = rhi < row_len ? (code_at(codes, task.row_start + rhi) | kRowHi) : kKillRow;
     st.rb[p] = clo | (chi << 16);
     st.ho[p] = neg;
-    st.hg[p] = neg;
+    st.hg[p] = 0;
     st.best[p] = 0;
     st.rec_lo[p] = 0;
     st.rec_hi[p] = 0;
   }
-  st.xa = neg;
+  st.xa = neg;  // boundary row: H = 0, stored as 16*(0 - gO)
   st.xb = neg;
-  st.vg = neg;
+  st.vg = 0;    // boundary E, clamped
+  st.xs = neg;
+  st.vs = 0;
   st.q = (kKillCode << 16) | kKillCode;
   // column stream: at step t the low half reads column t - 2l
   const int64_t cb = task.col_start - 2 * l;  // >= kPadBases - 2G >= 0
