@@ -95,6 +95,10 @@ int mhmsw_set_scoring(mhmsw_ctx *ctx, const mhmsw_scoring *scoring);
  * 0: the opposite roles.  Tie-breaks depend on this choice. */
 int mhmsw_set_orientation(mhmsw_ctx *ctx, int read_is_ssw_ref);
 int mhmsw_set_mode(mhmsw_ctx *ctx, int mode);
+/* 1 (default): the job of a staged batch (unpack, forward, reruns, reverse,
+ * finalize: ~40 launches) is captured once as a hipGraph and replayed while
+ * the staged geometry stays the same.  0: launches are issued one by one. */
+int mhmsw_set_graphs(mhmsw_ctx *ctx, int enable);
 
 /* --- batches (replace run_kernel_forwards / run_kernel_backwards,
  *     kernel_is_done / kernel_block and get_aln_results, driver.cpp:270-417) ---

@@ -59,7 +59,8 @@ class SynthBatch(ctypes.Structure):
 # every symbol include/mhmsw.h and include/mhmsw_synth.h declare
 EXPORTS = [
     "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
-    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_stage",
+    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_graphs",
+    "mhmsw_stage",
     "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
 ]
@@ -86,6 +87,7 @@ def lib():
         "mhmsw_set_scoring": [vp, ctypes.POINTER(Scoring)],
         "mhmsw_set_orientation": [vp, ctypes.c_int],
         "mhmsw_set_mode": [vp, ctypes.c_int],
+        "mhmsw_set_graphs": [vp, ctypes.c_int],
         "mhmsw_stage": [vp, i64, vp, vp, vp, vp, vp, vp],
         "mhmsw_align_staged": [vp],
         "mhmsw_submit": [vp, i64, vp, vp, vp, vp, vp, vp],

@@ -72,6 +72,9 @@ class Aligner:
     def set_mode(self, mode):
         self._check(self._L.mhmsw_set_mode(self._ctx, int(mode)), "set_mode")
 
+    def set_graphs(self, enable):
+        self._check(self._L.mhmsw_set_graphs(self._ctx, 1 if enable else 0), "set_graphs")
+
     def stage(self, batch: PairBatch):
         self._batch = batch  # keep the host buffers alive while referenced
         self._check(self._L.mhmsw_stage(self._ctx, batch.npairs, _ptr(batch.reads), _ptr(batch.read_off),

@@ -114,6 +114,12 @@ struct mhmsw_ctx {
   DevBuf<SwTask> d_tasks;
   DevBuf<PassSched> d_sched;
   DevBuf<AlnResult> d_results;
+  // hipGraph of the whole job, replayed while the staged geometry is unchanged
+  bool use_graphs = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_key;
+  int64_t graph_captures = 0, graph_replays = 0;
   hipEvent_t ev[kEvents] = {};
   hipEvent_t cev[kNumClasses][2] = {};
   bool class_launched[kNumClasses] = {};
@@ -269,6 +275,8 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   for (int i = 0; i < kNumClasses; i++)
     for (int k = 0; k < 2; k++)
       if (c->cev[i][k]) (void)hipEventDestroy(c->cev[i][k]);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MHMSW_OK;
@@ -435,6 +443,7 @@ int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* 
 
 static BatchView make_view(mhmsw_ctx* c) {
   BatchView b;
+  std::memset(&b, 0, sizeof(b));  // the graph cache compares views bytewise
   b.npairs = c->npairs;
   b.total = c->total;
   b.seq_base = c->d_base.ptr;
@@ -456,16 +465,7 @@ static BatchView make_view(mhmsw_ctx* c) {
   return b;
 }
 
-int mhmsw_align_staged(mhmsw_ctx* c) {
-  if (!c) return MHMSW_E_ARG;
-  if (!c->staged) return fail(c, MHMSW_E_STATE, "no staged batch");
-  if (c->in_flight) {
-    int rc = mhmsw_wait(c);
-    if (rc != MHMSW_OK) return rc;
-  }
-  HIP_OR_FAIL(c, hipSetDevice(c->device));
-  hipStream_t st = c->stream;
-  BatchView b = make_view(c);
+static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, int* maxp16_out, int* maxp64_out) {
   const int fast_grid = std::max(1, c->cus * 8);
   const int plain_grid = std::max(1, c->cus * 8);
   const int plain_cols = std::max(1, c->max_plain_cols);
@@ -477,7 +477,6 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   }
   HIP_OR_FAIL(c, hipEventRecord(c->ev[1], st));
   int maxp16 = 0, maxp64 = 0;
-  for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = false;
   if (c->npairs > 0) {
     for (int P = kMaxP; P >= 1; P--) {
       const int k = kClassFast64 + P - 1;
@@ -485,7 +484,6 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
         HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
         HIP_OR_FAIL(c, launch_fast(b, 64, P, fast_grid, st));
         HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
-        c->class_launched[k] = true;
         maxp64 = std::max(maxp64, P);
       }
     }
@@ -495,7 +493,6 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
         HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
         HIP_OR_FAIL(c, launch_fast(b, 16, P, fast_grid, st));
         HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
-        c->class_launched[k] = true;
         maxp16 = std::max(maxp16, P);
       }
     }
@@ -507,7 +504,6 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
       HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][0], st));
       HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
       HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][1], st));
-      c->class_launched[kClassPlain] = true;
     }
     if (any_fast) {
       HIP_OR_FAIL(c, launch_schedule(b, kPassRerun, st));
@@ -526,8 +522,77 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
   if (c->npairs > 0) HIP_OR_FAIL(c, launch_finalize(b, st));
   HIP_OR_FAIL(c, hipEventRecord(c->ev[5], st));
+  *maxp16_out = maxp16;
+  *maxp64_out = maxp64;
+  return MHMSW_OK;
+}
+
+static void drop_graph(mhmsw_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key.clear();
+}
+
+int mhmsw_align_staged(mhmsw_ctx* c) {
+  if (!c) return MHMSW_E_ARG;
+  if (!c->staged) return fail(c, MHMSW_E_STATE, "no staged batch");
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  HIP_OR_FAIL(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  BatchView b = make_view(c);
+  for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = c->npairs > 0 && c->fast_class_used[k];
+  int maxp16 = 0, maxp64 = 0;
+  if (!c->use_graphs) {
+    int rc = enqueue_job(c, b, st, &maxp16, &maxp64);
+    if (rc != MHMSW_OK) return rc;
+  } else {
+    // the job depends only on the view, the classes used, the mode and the
+    // plain LDS size: replay the captured graph while those are unchanged
+    std::vector<unsigned char> key(sizeof(BatchView) + sizeof(c->fast_class_used) + 2 * sizeof(int));
+    unsigned char* kp = key.data();
+    std::memcpy(kp, &b, sizeof(BatchView));
+    std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
+    const int extra[2] = {c->max_plain_cols, c->mode};
+    std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
+    if (!c->graph_exec || key != c->graph_key) {
+      drop_graph(c);
+      HIP_OR_FAIL(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_job(c, b, st, &maxp16, &maxp64);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc != MHMSW_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(c, MHMSW_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      c->graph = g;
+      e = hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        drop_graph(c);
+        return fail(c, MHMSW_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      }
+      c->graph_key = key;
+      c->graph_captures++;
+    } else {
+      c->graph_replays++;
+    }
+    HIP_OR_FAIL(c, hipGraphLaunch(c->graph_exec, st));
+  }
   c->in_flight = true;
   c->have_results = false;
+  return MHMSW_OK;
+}
+
+int mhmsw_set_graphs(mhmsw_ctx* c, int enable) {
+  if (!c) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "graph mode changed while a batch is in flight");
+  c->use_graphs = enable != 0;
+  if (!c->use_graphs) drop_graph(c);
   return MHMSW_OK;
 }
 

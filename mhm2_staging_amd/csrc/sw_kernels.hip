@@ -213,8 +213,9 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     // and l+1 columns scores at most match*min(k+1,l+1) minus the cost of
     // |k-l| gap positions, so |k-l| <= D; rows and columns beyond the other
     // side plus D cannot reach the score and are dropped (exact: DP values
-    // inside the kept rectangle do not depend on the dropped cells).
-    {
+    // inside the kept rectangle do not depend on the dropped cells).  The
+    // plain form keeps the reference's full prefix rectangle (ssw_core.cpp:1167-1173).
+    if (b.mode != 1) {
       const int32_t L = ref_len < qry_len ? ref_len : qry_len;
       const int64_t slack = (int64_t)b.sc.match * L - fe.score;
       int64_t D = 0;

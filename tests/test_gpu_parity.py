@@ -182,3 +182,41 @@ def test_full_size_config2_properties(gpu_aligner):
     idx = np.random.default_rng(0).choice(b.npairs, 3000, replace=False)
     sub = b.subset(idx)
     assert_same(fast[idx], oracle_table(sub, BASELINE_SCORING), sub)
+
+
+def test_graph_replays_match_eager_launches(gpu_aligner):
+    """The captured job graph, replayed with eager work in between, matches eager launches and the oracle."""
+    a = m.generate(3, npairs=3000, seed=31)
+    bb = m.generate(2, npairs=2000, seed=32)
+    al = gpu_aligner
+    al.set_scoring(BASELINE_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    try:
+        al.set_graphs(True)
+        outs = []
+        al.stage(a)
+        for _ in range(3):  # capture, then replays
+            al.align_staged()
+            al.wait()
+            outs.append(gpu_table(al.results()))
+        al.set_graphs(False)
+        al.align_staged()
+        al.wait()
+        outs.append(gpu_table(al.results()))
+        al.stage(bb)  # eager work on another batch
+        al.align_staged()
+        al.wait()
+        other = gpu_table(al.results())
+        al.set_graphs(True)
+        al.stage(a)
+        for _ in range(2):
+            al.align_staged()
+            al.wait()
+            outs.append(gpu_table(al.results()))
+        want = oracle_table(a, BASELINE_SCORING)
+        for o in outs:
+            assert_same(o, want, a)
+        assert_same(other, oracle_table(bb, BASELINE_SCORING), bb)
+    finally:
+        al.set_graphs(True)
