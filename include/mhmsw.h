@@ -69,6 +69,8 @@ typedef struct {
   int64_t plain_pairs;      /* pairs that ran 32-bit from the start */
   int64_t rerun_pairs;      /* pairs rerun in 32-bit after the 16-bit pass */
   double job_ms;            /* device time of the whole job (events on the context stream) */
+  /* The per-phase and per-class times below are measured for jobs launched
+   * eagerly (mhmsw_set_graphs(ctx, 0)); a replayed graph reports job_ms only. */
   double fwd_fast_ms;       /* device time of the forward packed kernels */
   double fwd_plain_ms;      /* device time of the forward 32-bit kernels (incl. reruns) */
   double rev_ms;            /* device time of the reverse pass */

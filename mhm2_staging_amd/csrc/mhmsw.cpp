@@ -116,6 +116,7 @@ struct mhmsw_ctx {
   DevBuf<AlnResult> d_results;
   // hipGraph of the whole job, replayed while the staged geometry is unchanged
   bool use_graphs = true;
+  bool last_job_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_key;
@@ -465,52 +466,54 @@ static BatchView make_view(mhmsw_ctx* c) {
   return b;
 }
 
-static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, int* maxp16_out, int* maxp64_out) {
+// ev: record phase events (eager launches only; events recorded inside a
+// captured graph cannot be timed)
+static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev, int* maxp16_out, int* maxp64_out) {
   const int fast_grid = std::max(1, c->cus * 8);
   const int plain_grid = std::max(1, c->cus * 8);
   const int plain_cols = std::max(1, c->max_plain_cols);
-  HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
   HIP_OR_FAIL(c, launch_clear_counters(b, st));
   if (c->npairs > 0) {
     HIP_OR_FAIL(c, launch_unpack(b, st));
     HIP_OR_FAIL(c, launch_schedule(b, kPassFwd, st));
   }
-  HIP_OR_FAIL(c, hipEventRecord(c->ev[1], st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[1], st));
   int maxp16 = 0, maxp64 = 0;
   if (c->npairs > 0) {
     for (int P = kMaxP; P >= 1; P--) {
       const int k = kClassFast64 + P - 1;
       if (c->fast_class_used[k]) {
-        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
+        if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
         HIP_OR_FAIL(c, launch_fast(b, 64, P, fast_grid, st));
-        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
+        if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp64 = std::max(maxp64, P);
       }
     }
     for (int P = kMaxP; P >= 1; P--) {
       const int k = kClassFast16 + P - 1;
       if (c->fast_class_used[k]) {
-        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
+        if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
         HIP_OR_FAIL(c, launch_fast(b, 16, P, fast_grid, st));
-        HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
+        if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp16 = std::max(maxp16, P);
       }
     }
   }
-  HIP_OR_FAIL(c, hipEventRecord(c->ev[2], st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[2], st));
   const bool any_fast = maxp16 > 0 || maxp64 > 0;
   if (c->npairs > 0) {
     if (c->fast_class_used[kClassPlain]) {
-      HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][0], st));
+      if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][0], st));
       HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
-      HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][1], st));
+      if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][1], st));
     }
     if (any_fast) {
       HIP_OR_FAIL(c, launch_schedule(b, kPassRerun, st));
       HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
     }
   }
-  HIP_OR_FAIL(c, hipEventRecord(c->ev[3], st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[3], st));
   if (c->npairs > 0) {
     HIP_OR_FAIL(c, launch_schedule(b, kPassRev, st));
     if (any_fast) {
@@ -519,9 +522,9 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, int* ma
     }
     HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
   }
-  HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
   if (c->npairs > 0) HIP_OR_FAIL(c, launch_finalize(b, st));
-  HIP_OR_FAIL(c, hipEventRecord(c->ev[5], st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[5], st));
   *maxp16_out = maxp16;
   *maxp64_out = maxp64;
   return MHMSW_OK;
@@ -547,8 +550,9 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   BatchView b = make_view(c);
   for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = c->npairs > 0 && c->fast_class_used[k];
   int maxp16 = 0, maxp64 = 0;
+  c->last_job_graph = c->use_graphs;
   if (!c->use_graphs) {
-    int rc = enqueue_job(c, b, st, &maxp16, &maxp64);
+    int rc = enqueue_job(c, b, st, true, &maxp16, &maxp64);
     if (rc != MHMSW_OK) return rc;
   } else {
     // the job depends only on the view, the classes used, the mode and the
@@ -562,7 +566,7 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     if (!c->graph_exec || key != c->graph_key) {
       drop_graph(c);
       HIP_OR_FAIL(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_job(c, b, st, &maxp16, &maxp64);
+      int rc = enqueue_job(c, b, st, false, &maxp16, &maxp64);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(st, &g);
       if (rc != MHMSW_OK) {
@@ -581,7 +585,9 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     } else {
       c->graph_replays++;
     }
+    HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
     HIP_OR_FAIL(c, hipGraphLaunch(c->graph_exec, st));
+    HIP_OR_FAIL(c, hipEventRecord(c->ev[kEvents - 1], st));
   }
   c->in_flight = true;
   c->have_results = false;
@@ -609,7 +615,13 @@ static int complete(mhmsw_ctx* c) {
   unsigned long long counters[2] = {0, 0};
   HIP_OR_FAIL(c, hipMemcpy(counters, c->d_counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
   float ms[kEvents] = {};
-  for (int i = 1; i < kEvents; i++) HIP_OR_FAIL(c, hipEventElapsedTime(&ms[i], c->ev[i - 1], c->ev[i]));
+  float job = 0.f;
+  if (c->last_job_graph) {
+    HIP_OR_FAIL(c, hipEventElapsedTime(&job, c->ev[0], c->ev[kEvents - 1]));
+  } else {
+    for (int i = 1; i < kEvents; i++) HIP_OR_FAIL(c, hipEventElapsedTime(&ms[i], c->ev[i - 1], c->ev[i]));
+    for (int i = 1; i < kEvents; i++) job += ms[i];
+  }
   mhmsw_stats& s = c->stats;
   s.npairs = c->npairs;
   s.cells = c->cells;
@@ -621,14 +633,15 @@ static int complete(mhmsw_ctx* c) {
     s.class_pairs[k] = c->class_pairs[k];
     s.class_cells[k] = c->class_cells[k];
     float cm = 0.f;
-    if (c->class_launched[k]) HIP_OR_FAIL(c, hipEventElapsedTime(&cm, c->cev[k][0], c->cev[k][1]));
+    if (c->class_launched[k] && !c->last_job_graph)
+      HIP_OR_FAIL(c, hipEventElapsedTime(&cm, c->cev[k][0], c->cev[k][1]));
     s.class_ms[k] = cm;
   }
   s.prep_ms = ms[1];
   s.fwd_fast_ms = ms[2];
   s.fwd_plain_ms = ms[3];
   s.rev_ms = ms[4] + ms[5];
-  s.job_ms = ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+  s.job_ms = job;
   c->have_results = true;
   if (c->h_errors[0] != 0 || c->h_errors[2] != 0)
     return fail(c, MHMSW_E_INTERNAL, "%d pairs: reverse pass did not reproduce the forward score; %d end points "
