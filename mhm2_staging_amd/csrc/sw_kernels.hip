@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "sw_common.h"
 #include "sw_launch.h"
 
@@ -153,14 +155,21 @@ __global__ void unpack_kernel(BatchView b) {
 // Output dword w holds reversed positions 8w..8w+7 = source positions
 // len-1-8w down to len-8-8w: one funnel shift of two source dwords, then a
 // nibble reversal (byte permute + nibble swap).  Nibbles past len are 0.
+__device__ void reverse_one_impl(BatchView b, int64_t s, int lane);
+__device__ __forceinline__ void reverse_one(const BatchView& b, int64_t s, int lane) { reverse_one_impl(b, s, lane); }
+
 __global__ void reverse_kernel(BatchView b) {
-  const int64_t s = blockIdx.x;
-  if (s >= 2 * b.npairs) return;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; s < 2 * b.npairs; s += nwaves)
+    reverse_one(b, s, threadIdx.x & 63);
+}
+
+__device__ void reverse_one_impl(BatchView b, int64_t s, int lane) {
   const int64_t base = b.seq_base[s];
   const int32_t len = b.seq_len[s];
   const int32_t ndw = (len + 7) >> 3;
   const int64_t out0 = (b.total + base) >> 3;
-  for (int32_t w = threadIdx.x; w < ndw; w += blockDim.x) {
+  for (int32_t w = lane; w < ndw; w += 64) {
     const int64_t s0 = base + len - 8 - 8 * (int64_t)w;  // >= base - 8 >= 0 (front pad)
     const uint32_t d0 = b.codes[s0 >> 3], d1 = b.codes[(s0 >> 3) + 1];
     const uint32_t v = __builtin_amdgcn_alignbit(d1, d0, (uint32_t)(s0 & 7) * 4u);
@@ -276,22 +285,49 @@ __global__ void clear_sched_kernel(PassSched* s) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0;
 }
 
-__global__ void classify_kernel(BatchView b, int pass) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < b.npairs; p += (int64_t)gridDim.x * blockDim.x) {
+// Each 256-thread block classifies a contiguous run of kClassifyPerThread*256
+// pairs: ranks inside the block come from LDS atomics, then one global
+// atomic per used bucket reserves the block's range (far less contention on
+// the few buckets most pairs fall into).
+constexpr int kClassifyPerThread = 8;
+
+__global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
+  __shared__ int32_t hist[kNumBuckets];
+  for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x * kClassifyPerThread;
+  int32_t bk[kClassifyPerThread], lr[kClassifyPerThread];
+#pragma unroll
+  for (int k = 0; k < kClassifyPerThread; k++) {
+    const int64_t p = p0 + (int64_t)k * blockDim.x + threadIdx.x;
+    bk[k] = -1;
+    lr[k] = 0;
+    if (p >= b.npairs) continue;
     SwTask t;
     int cls = 0, bin = 0;
-    bool has = make_task(b, pass, p, t, cls, bin);
-    if (!has) {
+    if (!make_task(b, pass, p, t, cls, bin)) {
       b.bucket[p] = -1;
       if (pass == kPassFwd) b.fwd[p] = PairEnd{0, -1, 0, 0};
       if (pass == kPassRev) b.rev[p] = PairEnd{0, 0, 0, 0};
       continue;
     }
-    const int bk = cls * kNumBins + bin;
-    b.bucket[p] = bk;
+    bk[k] = cls * kNumBins + bin;
+    lr[k] = atomicAdd(&hist[bk[k]], 1);
     if (pass == kPassRerun) atomicAdd(&b.errors[1], 1);
     if (pass == kPassRev) atomicAdd(&b.counters[0], (unsigned long long)t.row_len * (unsigned long long)t.col_len);
-    b.rank[p] = atomicAdd(&b.sched->bucket_count[bk], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) {
+    const int32_t n = hist[i];
+    hist[i] = n ? atomicAdd(&b.sched->bucket_count[i], n) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kClassifyPerThread; k++) {
+    const int64_t p = p0 + (int64_t)k * blockDim.x + threadIdx.x;
+    if (p >= b.npairs || bk[k] < 0) continue;
+    b.bucket[p] = bk[k];
+    b.rank[p] = hist[bk[k]] + lr[k];
   }
 }
 
@@ -452,9 +488,20 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const bool active = task.pair >= 0;
   const int32_t row_len = active ? task.row_len : 0;
   const int32_t col_len = active ? task.col_len : 0;
-  const int nsteps_need = uni(wave_max_i32(active ? col_len + 2 * G - 1 : 0));
+  // per-group values are uniform within a group: combine the groups' lane 0
+  int nsteps_need = 0, mincols = 0x7FFFFFFF;
+  {
+    const int need = active ? col_len + 2 * G - 1 : 0;
+    const int cmin = active ? col_len : 0x7FFFFFFF;
+#pragma unroll
+    for (int g = 0; g < 64 / G; g++) {
+      const int a = __builtin_amdgcn_readlane(need, g * G);
+      const int c = __builtin_amdgcn_readlane(cmin, g * G);
+      nsteps_need = a > nsteps_need ? a : nsteps_need;
+      mincols = c < mincols ? c : mincols;
+    }
+  }
   if (nsteps_need <= 0) return;
-  const int mincols = uni(wave_min_i32(active ? col_len : 0x7FFFFFFF));
   const int nsteps = (nsteps_need + 15) & ~15;
 
   const uint32_t* __restrict__ codes = b.codes;
@@ -464,11 +511,18 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const uint32_t neg = (uint32_t)(uint16_t)(-16 * b.sc.gap_open) * 0x00010001u;
 
   FastState<G, P> st;
+  // this lane's 2P row codes are consecutive nibbles: three dword loads
+  const int64_t r0 = task.row_start + 2 * P * l;
+  const uint32_t rsh = (uint32_t)(r0 & 7) * 4u;
+  const uint32_t a0 = codes[r0 >> 3], a1 = codes[(r0 >> 3) + 1], a2 = codes[(r0 >> 3) + 2];
+  const uint32_t rw0 = __builtin_amdgcn_alignbit(a1, a0, rsh), rw1 = __builtin_amdgcn_alignbit(a2, a1, rsh);
 #pragma unroll
   for (int p = 0; p < P; p++) {
     const int rlo = 2 * P * l + p, rhi = 2 * P * l + P + p;
-    const uint32_t clo = rlo < row_len ? (code_at(codes, task.row_start + rlo) | kRowHi) : kKillRow;
-    const uint32_t chi = rhi < row_len ? (code_at(codes, task.row_start + rhi) | kRowHi) : kKillRow;
+    const uint32_t nlo = ((p < 8 ? rw0 : rw1) >> (4 * (p & 7))) & 0xFu;
+    const uint32_t nhi = ((P + p < 8 ? rw0 : rw1) >> (4 * ((P + p) & 7))) & 0xFu;
+    const uint32_t clo = rlo < row_len ? (nlo | kRowHi) : kKillRow;
+    const uint32_t chi = rhi < row_len ? (nhi | kRowHi) : kKillRow;
     st.rb[p] = clo | (chi << 16);
     st.ho[p] = neg;
     st.hg[p] = 0;
@@ -553,13 +607,19 @@ __global__ __launch_bounds__(256) void sw_fast_kernel(BatchView b) {
   PassSched* s = b.sched;
   const int32_t nwt = s->class_slots[cls] / GPW;
   const int32_t base = s->class_base[cls];
-  for (;;) {
-    // every lane takes part in the atomic (lane 0 adds 1, the rest 0): no
-    // divergent branch in the loop header, so the wavefront stays whole
-    const int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
-    if (w >= nwt) break;
-    const SwTask task = b.tasks[base + w * GPW + grp];
+  // every lane takes part in the atomic (lane 0 adds 1, the rest 0): no
+  // divergent branch in the loop, so the wavefront stays whole.  The next
+  // task is fetched before the current one is aligned.
+  int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+  SwTask task;
+  if (w < nwt) task = b.tasks[base + w * GPW + grp];
+  while (w < nwt) {
+    const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+    SwTask next;
+    if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
     fast_pair<G, P>(b, task, l);
+    task = next;
+    w = wn;
   }
 }
 
@@ -707,13 +767,17 @@ hipError_t launch_clear_counters(const BatchView& b, hipStream_t st) {
 
 hipError_t launch_unpack(const BatchView& b, hipStream_t st) {
   hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 3, 256)), dim3(256), 0, st, b);
-  if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3((unsigned)(2 * b.npairs)), dim3(64), 0, st, b);
+  if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3(grid_for(2 * b.npairs, 4)), dim3(256), 0, st, b);
   return hipGetLastError();
 }
 
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
   hipLaunchKernelGGL(clear_sched_kernel, dim3(8), dim3(256), 0, st, b.sched);
-  hipLaunchKernelGGL(classify_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, b, pass);
+  {
+    const int64_t per_block = 256 * kClassifyPerThread;
+    const unsigned nblk = (unsigned)std::max<int64_t>(1, (b.npairs + per_block - 1) / per_block);
+    hipLaunchKernelGGL(classify_kernel, dim3(nblk), dim3(256), 0, st, b, pass);
+  }
   hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, st, b);
   hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, b, pass);
   return hipGetLastError();
