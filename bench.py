@@ -58,6 +58,54 @@ def cpu_baseline(batch, scoring, seconds_target=12.0):
             "sample": f"first {k2} of {n} pairs of the same workload ({cells} cells), {dt:.1f} s"}, res, k2
 
 
+def serving_bench(qps_list, duration, slots=4, seed=5):
+    """Config 5: 64k resident contigs, Poisson requests of log-uniform 1e3..1e5 pairs."""
+    import mhm2_staging_amd as m
+    rng = np.random.default_rng(seed)
+    cs = m.contigs(nctgs=64000, seed=seed)
+    res = m.ResidentContigs(cs, device=0)
+    # a pool of distinct requests, resubmitted cyclically (generation is not timed)
+    pool = []
+    for k in range(48):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
+        pool.append(m.window_pairs(cs, n, seed=1000 + k))
+    srv = m.Server(res, scoring=m.MHM2_SCORING, slots=slots)
+    rid = srv.submit(pool[0])  # warm up every code path once
+    srv.results(rid)
+    points = []
+    for qps in qps_list:
+        gaps = rng.exponential(1.0 / qps, size=int(qps * duration) + 1)
+        arrivals = np.cumsum(gaps)
+        arrivals = arrivals[arrivals < duration]
+        t0 = time.perf_counter()
+        ids = []
+        for i, a in enumerate(arrivals):
+            d = t0 + a - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            b = pool[i % len(pool)]
+            ids.append((srv.submit(b), b.cells(), b.npairs))
+        lat, cells, first, last, pairs = [], 0, None, None, 0
+        for rid, c, n in ids:
+            _, t = srv.results(rid)
+            lat.append(t["done_s"] - t["submit_s"])
+            cells += c
+            pairs += n
+            first = t["submit_s"] if first is None else min(first, t["submit_s"])
+            last = t["done_s"] if last is None else max(last, t["done_s"])
+        lat = np.array(lat) * 1e3
+        span = last - first
+        points.append({"qps": qps, "requests": len(ids), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                       "sustained_gcups": round(cells / span / 1e9, 1), "pairs_per_s": round(pairs / span, 1)})
+        log(f"[bench] serving {points[-1]}")
+    srv.close()
+    res.close()
+    return {"workload": "64k resident contigs (log-uniform 500..20000 bp), Poisson requests of log-uniform "
+                        "1e3..1e5 pairs (150 bp reads, 1% error) vs U[300,1024] windows",
+            "slots": slots, "duration_s_per_point": duration, "points": points}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +115,9 @@ def main():
     ap.add_argument("--npairs", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
+    ap.add_argument("--serving-qps", type=str, default="50,400", help="comma-separated request rates")
+    ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times) to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -209,6 +260,8 @@ def main():
                                   "against": cb["kind"]}
         log(f"[bench] cpu baseline {cb}")
     al.close()
+    if not args.no_serving:
+        out["serving"] = serving_bench([float(x) for x in args.serving_qps.split(",")], args.serving_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -123,6 +123,52 @@ int mhmsw_wait(mhmsw_ctx *ctx);
 int mhmsw_results(mhmsw_ctx *ctx, mhmsw_result *out, int64_t npairs);
 int mhmsw_get_stats(mhmsw_ctx *ctx, mhmsw_stats *out);
 
+/* --- device-resident contigs (serving; no counterpart in the reference,
+ *     whose windows are copied per batch as strings, klign.cpp:691-706) ---
+ * Contigs are packed once to 2-bit codes plus an N bitmap and kept in HBM;
+ * batches then name windows as (contig id, offset, length) and the window
+ * codes are extracted on the device. */
+typedef struct mhmsw_contigs mhmsw_contigs;
+int mhmsw_contigs_load(int device_id, int64_t nctgs, const char *seqs, const int64_t *off, const int32_t *len,
+                       mhmsw_contigs **out);
+int mhmsw_contigs_free(mhmsw_contigs *contigs);
+int64_t mhmsw_contigs_bytes(const mhmsw_contigs *contigs); /* device bytes held */
+
+/* Stage a batch whose pair p aligns reads[read_off[p] .. +read_len[p]) with
+ * contig ctg_id[p] at [win_off[p], win_off[p] + win_len[p]).  The contigs
+ * must live on the context's device. */
+int mhmsw_stage_windows(mhmsw_ctx *ctx, const mhmsw_contigs *contigs, int64_t npairs, const char *reads,
+                        const int64_t *read_off, const int32_t *read_len, const int32_t *ctg_id, const int32_t *win_off,
+                        const int32_t *win_len);
+
+/* --- serving loop: continuous batching of alignment requests ---
+ * A server owns `slots` contexts (one HIP stream each) on one device and a
+ * dispatcher thread.  Requests are admitted in arrival order as soon as a
+ * slot is free and their device-memory estimate fits hbm_budget (bytes,
+ * 0 = 80% of the device); admitted requests run concurrently on the device.
+ * Results are identical to aligning the request alone. */
+typedef struct mhmsw_server mhmsw_server;
+typedef struct {
+  double submit_s;  /* host steady clock, seconds */
+  double admit_s;
+  double done_s;
+  double device_ms; /* device time of the request's job */
+} mhmsw_req_times;
+int mhmsw_server_create(int device_id, const mhmsw_scoring *scoring, const mhmsw_contigs *contigs, int slots,
+                        int64_t hbm_budget, mhmsw_server **out);
+/* Copies the request's host data; returns at once with *req_id. */
+int mhmsw_server_submit(mhmsw_server *srv, int64_t npairs, const char *reads, const int64_t *read_off,
+                        const int32_t *read_len, const int32_t *ctg_id, const int32_t *win_off, const int32_t *win_len,
+                        uint64_t *req_id);
+/* 0 queued, 1 running, 2 done, < 0 failed. */
+int mhmsw_server_status(mhmsw_server *srv, uint64_t req_id);
+int mhmsw_server_wait(mhmsw_server *srv, uint64_t req_id);
+/* Waits, copies npairs results and the request's times, and forgets the request. */
+int mhmsw_server_results(mhmsw_server *srv, uint64_t req_id, mhmsw_result *out, int64_t npairs, mhmsw_req_times *times);
+double mhmsw_server_clock(void); /* the clock of mhmsw_req_times */
+const char *mhmsw_server_last_error(const mhmsw_server *srv);
+int mhmsw_server_destroy(mhmsw_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,22 @@ class Stats(ctypes.Structure):
         return d
 
 
+class ReqTimes(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("device_ms", ctypes.c_double)]
+
+
+class SynthContigs(ctypes.Structure):
+    _fields_ = [("nctgs", ctypes.c_int64), ("seqs", ctypes.c_void_p), ("off", ctypes.c_void_p),
+                ("len", ctypes.c_void_p), ("bytes", ctypes.c_int64)]
+
+
+class SynthWindows(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_int64), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
+                ("read_len", ctypes.c_void_p), ("ctg_id", ctypes.c_void_p), ("win_off", ctypes.c_void_p),
+                ("win_len", ctypes.c_void_p), ("reads_bytes", ctypes.c_int64)]
+
+
 class SynthBatch(ctypes.Structure):
     _fields_ = [("npairs", ctypes.c_int64), ("reads", ctypes.c_void_p), ("read_off", ctypes.c_void_p),
                 ("read_len", ctypes.c_void_p), ("ctgs", ctypes.c_void_p), ("ctg_off", ctypes.c_void_p),
@@ -63,6 +79,10 @@ EXPORTS = [
     "mhmsw_stage",
     "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
+    "mhmsw_contigs_load", "mhmsw_contigs_free", "mhmsw_contigs_bytes", "mhmsw_stage_windows",
+    "mhmsw_server_create", "mhmsw_server_submit", "mhmsw_server_status", "mhmsw_server_wait",
+    "mhmsw_server_results", "mhmsw_server_clock", "mhmsw_server_last_error", "mhmsw_server_destroy",
+    "mhmsw_synth_contigs", "mhmsw_synth_contigs_free", "mhmsw_synth_window_pairs", "mhmsw_synth_window_pairs_free",
 ]
 
 _lib = None
@@ -96,6 +116,18 @@ def lib():
         "mhmsw_results": [vp, vp, i64],
         "mhmsw_get_stats": [vp, ctypes.POINTER(Stats)],
         "mhmsw_synth_generate": [ctypes.c_int, i64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(SynthBatch)],
+        "mhmsw_contigs_load": [ctypes.c_int, i64, vp, vp, vp, ctypes.POINTER(vp)],
+        "mhmsw_contigs_free": [vp],
+        "mhmsw_stage_windows": [vp, vp, i64, vp, vp, vp, vp, vp, vp],
+        "mhmsw_server_create": [ctypes.c_int, ctypes.POINTER(Scoring), vp, ctypes.c_int, i64, ctypes.POINTER(vp)],
+        "mhmsw_server_submit": [vp, i64, vp, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)],
+        "mhmsw_server_status": [vp, ctypes.c_uint64],
+        "mhmsw_server_wait": [vp, ctypes.c_uint64],
+        "mhmsw_server_results": [vp, ctypes.c_uint64, vp, i64, ctypes.POINTER(ReqTimes)],
+        "mhmsw_server_destroy": [vp],
+        "mhmsw_synth_contigs": [ctypes.c_uint64, i64, i32, i32, ctypes.c_int, ctypes.POINTER(SynthContigs)],
+        "mhmsw_synth_window_pairs": [ctypes.POINTER(SynthContigs), ctypes.c_uint64, i64, ctypes.c_int,
+                                     ctypes.POINTER(SynthWindows)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -105,6 +137,16 @@ def lib():
     L.mhmsw_last_error.restype = ctypes.c_char_p
     L.mhmsw_synth_free.argtypes = [ctypes.POINTER(SynthBatch)]
     L.mhmsw_synth_free.restype = None
+    L.mhmsw_synth_contigs_free.argtypes = [ctypes.POINTER(SynthContigs)]
+    L.mhmsw_synth_contigs_free.restype = None
+    L.mhmsw_synth_window_pairs_free.argtypes = [ctypes.POINTER(SynthWindows)]
+    L.mhmsw_synth_window_pairs_free.restype = None
+    L.mhmsw_contigs_bytes.argtypes = [vp]
+    L.mhmsw_contigs_bytes.restype = ctypes.c_int64
+    L.mhmsw_server_clock.argtypes = []
+    L.mhmsw_server_clock.restype = ctypes.c_double
+    L.mhmsw_server_last_error.argtypes = [vp]
+    L.mhmsw_server_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 

@@ -12,7 +12,7 @@ import time
 import numpy as np
 
 from . import _native
-from .synth import PairBatch, from_strings
+from .synth import ContigSet, PairBatch, WindowBatch, from_strings
 
 # mhm2's alignment scoring, BASELINE "previous defaults" (CMakeDefinitions.txt:118-122)
 MHM2_SCORING = (1, 3, 5, 2, 2)
@@ -80,6 +80,13 @@ class Aligner:
         self._check(self._L.mhmsw_stage(self._ctx, batch.npairs, _ptr(batch.reads), _ptr(batch.read_off),
                                         _ptr(batch.read_len), _ptr(batch.ctgs), _ptr(batch.ctg_off),
                                         _ptr(batch.ctg_len)), "stage")
+
+    def stage_windows(self, contigs, batch: WindowBatch):
+        """Stage pairs whose windows are ranges of device-resident contigs."""
+        self._batch = batch
+        self._check(self._L.mhmsw_stage_windows(self._ctx, contigs.handle, batch.npairs, _ptr(batch.reads),
+                                                _ptr(batch.read_off), _ptr(batch.read_len), _ptr(batch.ctg_id),
+                                                _ptr(batch.win_off), _ptr(batch.win_len)), "stage_windows")
 
     def align_staged(self):
         self._check(self._L.mhmsw_align_staged(self._ctx), "align_staged")
@@ -198,3 +205,82 @@ def get_avail_gpu_mem_per_rank(tot_ranks, num_devices=0):
     if not num_devices:
         return 0
     return int(get_tot_gpu_mem() * 0.8) // max(1, tot_ranks // num_devices)
+
+
+class ResidentContigs:
+    """Contigs packed to 2-bit codes + N bitmap and kept in device memory."""
+
+    def __init__(self, cs: ContigSet, device=0):
+        self._L = _native.lib()
+        self.handle = ctypes.c_void_p()
+        self.nctgs = cs.nctgs
+        rc = self._L.mhmsw_contigs_load(int(device), cs.nctgs, _ptr(cs.seqs), _ptr(cs.off), _ptr(cs.len),
+                                        ctypes.byref(self.handle))
+        if rc != 0:
+            raise _native.MhmswError(rc, "mhmsw_contigs_load failed")
+        self.device = device
+        self.bytes = self._L.mhmsw_contigs_bytes(self.handle)
+
+    def close(self):
+        if self.handle:
+            self._L.mhmsw_contigs_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching serving loop over device-resident contigs (mhmsw_server_*)."""
+
+    def __init__(self, contigs: ResidentContigs, scoring=MHM2_SCORING, slots=4, hbm_budget=0):
+        self._L = _native.lib()
+        self._h = ctypes.c_void_p()
+        self._contigs = contigs
+        sc = _native.Scoring(*[int(x) for x in scoring])
+        rc = self._L.mhmsw_server_create(int(contigs.device), ctypes.byref(sc), contigs.handle, int(slots),
+                                         int(hbm_budget), ctypes.byref(self._h))
+        if rc != 0:
+            raise _native.MhmswError(rc, "mhmsw_server_create failed")
+        self._pending = {}
+
+    def submit(self, batch: WindowBatch):
+        rid = ctypes.c_uint64(0)
+        rc = self._L.mhmsw_server_submit(self._h, batch.npairs, _ptr(batch.reads), _ptr(batch.read_off),
+                                         _ptr(batch.read_len), _ptr(batch.ctg_id), _ptr(batch.win_off),
+                                         _ptr(batch.win_len), ctypes.byref(rid))
+        if rc != 0:
+            raise _native.MhmswError(rc, "submit")
+        self._pending[rid.value] = batch.npairs
+        return rid.value
+
+    def status(self, rid):
+        return self._L.mhmsw_server_status(self._h, ctypes.c_uint64(rid))
+
+    def results(self, rid):
+        """Wait for request rid; return (results, times dict)."""
+        n = self._pending.pop(rid)
+        out = np.zeros(n, dtype=_native.RESULT_DTYPE)
+        t = _native.ReqTimes()
+        rc = self._L.mhmsw_server_results(self._h, ctypes.c_uint64(rid), _ptr(out), n, ctypes.byref(t))
+        if rc != 0:
+            msg = self._L.mhmsw_server_last_error(self._h)
+            raise _native.MhmswError(rc, f"request {rid}: {msg.decode() if msg else ''}")
+        return out, {"submit_s": t.submit_s, "admit_s": t.admit_s, "done_s": t.done_s, "device_ms": t.device_ms}
+
+    def clock(self):
+        return self._L.mhmsw_server_clock()
+
+    def close(self):
+        if self._h:
+            self._L.mhmsw_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

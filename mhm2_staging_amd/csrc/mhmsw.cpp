@@ -82,6 +82,16 @@ constexpr int kEvents = 6;
 
 }  // namespace
 
+struct mhmsw_contigs {
+  int device = 0;
+  int64_t n = 0;
+  std::vector<int32_t> len;
+  uint32_t* d2 = nullptr;     // 2-bit codes
+  uint32_t* dN = nullptr;     // N bitmap
+  int64_t* dbase = nullptr;   // first base of each contig (multiple of 32)
+  int64_t bytes = 0;
+};
+
 struct mhmsw_ctx {
   int device = 0;
   int cus = 0;
@@ -109,6 +119,8 @@ struct mhmsw_ctx {
   DevBuf<int64_t> d_base;
   DevBuf<int32_t> d_len, d_bucket, d_rank, d_errors;
   DevBuf<unsigned long long> d_counters;
+  DevBuf<int32_t> d_win_ctg, d_win_off;
+  const mhmsw_contigs* res = nullptr;  // resident contigs of the staged batch (windowed batches)
   int64_t n_fast = 0, n_plain = 0;
   DevBuf<PairEnd> d_fwd, d_rev;
   DevBuf<SwTask> d_tasks;
@@ -266,6 +278,8 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   c->d_rank.release();
   c->d_errors.release();
   c->d_counters.release();
+  c->d_win_ctg.release();
+  c->d_win_off.release();
   c->d_fwd.release();
   c->d_rev.release();
   c->d_tasks.release();
@@ -322,11 +336,38 @@ int mhmsw_set_mode(mhmsw_ctx* c, int mode) {
   return MHMSW_OK;
 }
 
-int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* read_off, const int32_t* read_len,
-                const char* ctgs, const int64_t* ctg_off, const int32_t* ctg_len) {
-  if (!c) return MHMSW_E_ARG;
-  if (npairs < 0 || (npairs > 0 && (!reads || !read_off || !read_len || !ctgs || !ctg_off || !ctg_len)))
+}  // extern "C"
+
+// A batch's contig windows come either as strings (ctgs/ctg_off/ctg_len) or
+// as ranges of resident contigs (res/ctg_id/win_off/win_len).
+struct StageInput {
+  int64_t npairs = 0;
+  const char* reads = nullptr;
+  const int64_t* read_off = nullptr;
+  const int32_t* read_len = nullptr;
+  const char* ctgs = nullptr;
+  const int64_t* ctg_off = nullptr;
+  const int32_t* ctg_len = nullptr;
+  const mhmsw_contigs* res = nullptr;
+  const int32_t* ctg_id = nullptr;
+  const int32_t* win_off = nullptr;
+  const int32_t* win_len = nullptr;
+};
+
+static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
+  const int64_t npairs = in.npairs;
+  const char* reads = in.reads;
+  const int64_t* read_off = in.read_off;
+  const int32_t* read_len = in.read_len;
+  const char* ctgs = in.ctgs;
+  const int64_t* ctg_off = in.ctg_off;
+  const bool windowed = in.res != nullptr;
+  if (npairs < 0 || (npairs > 0 && (!reads || !read_off || !read_len)) ||
+      (npairs > 0 && !windowed && (!ctgs || !ctg_off || !in.ctg_len)) ||
+      (npairs > 0 && windowed && (!in.ctg_id || !in.win_off || !in.win_len)))
     return fail(c, MHMSW_E_ARG, "bad batch arguments");
+  if (windowed && in.res->device != c->device)
+    return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", in.res->device, c->device);
   if (npairs > (int64_t)1 << 30) return fail(c, MHMSW_E_ARG, "batch of %lld pairs is too large", (long long)npairs);
   if (c->in_flight) {
     int rc = mhmsw_wait(c);
@@ -345,8 +386,13 @@ int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* 
   int64_t nfast = 0, nplain = 0;
   int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
   for (int64_t p = 0; p < npairs; p++) {
-    const int32_t rl = read_len[p], cl = ctg_len[p];
+    const int32_t rl = read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
     if (rl < 0 || cl < 0) return fail(c, MHMSW_E_ARG, "negative length at pair %lld", (long long)p);
+    if (windowed) {
+      const int32_t id = in.ctg_id[p];
+      if (id < 0 || id >= in.res->n || in.win_off[p] < 0 || (int64_t)in.win_off[p] + cl > in.res->len[(size_t)id])
+        return fail(c, MHMSW_E_ARG, "pair %lld: window outside its resident contig", (long long)p);
+    }
     if (std::min(rl, cl) > kPlainMaxCols)
       return fail(c, MHMSW_E_ARG, "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl,
                   cl);
@@ -396,6 +442,7 @@ int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* 
     if (nseq < 4096) nth = 1;
     auto work = [&](int tid) {
       for (int64_t s = tid; s < nseq; s += nth) {
+        if ((s & 1) && windowed) continue;  // extracted on the device
         const char* src = (s & 1) ? ctgs + ctg_off[s >> 1] : reads + read_off[s >> 1];
         const int32_t len = c->h_len.ptr[s];
         const int64_t b0 = c->h_base.ptr[s];  // multiple of 32: whole dwords of both arrays
@@ -437,10 +484,52 @@ int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* 
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, c->h_base.ptr, (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, c->h_len.ptr, (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
   }
+  c->res = windowed ? in.res : nullptr;
+  if (windowed && npairs > 0) {
+    HIP_OR_FAIL(c, c->d_win_ctg.ensure((size_t)npairs));
+    HIP_OR_FAIL(c, c->d_win_off.ensure((size_t)npairs));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, in.ctg_id, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, in.win_off, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+  }
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   c->staged = true;
   return MHMSW_OK;
 }
+
+extern "C" {
+
+int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* read_off, const int32_t* read_len,
+                const char* ctgs, const int64_t* ctg_off, const int32_t* ctg_len) {
+  if (!c) return MHMSW_E_ARG;
+  StageInput in;
+  in.npairs = npairs;
+  in.reads = reads;
+  in.read_off = read_off;
+  in.read_len = read_len;
+  in.ctgs = ctgs;
+  in.ctg_off = ctg_off;
+  in.ctg_len = ctg_len;
+  return stage_impl(c, in);
+}
+
+int mhmsw_stage_windows(mhmsw_ctx* c, const mhmsw_contigs* contigs, int64_t npairs, const char* reads,
+                        const int64_t* read_off, const int32_t* read_len, const int32_t* ctg_id, const int32_t* win_off,
+                        const int32_t* win_len) {
+  if (!c || !contigs) return MHMSW_E_ARG;
+  StageInput in;
+  in.npairs = npairs;
+  in.reads = reads;
+  in.read_off = read_off;
+  in.read_len = read_len;
+  in.res = contigs;
+  in.ctg_id = ctg_id;
+  in.win_off = win_off;
+  in.win_len = win_len;
+  return stage_impl(c, in);
+}
+
+}  // extern "C"
+
 
 static BatchView make_view(mhmsw_ctx* c) {
   BatchView b;
@@ -463,6 +552,13 @@ static BatchView make_view(mhmsw_ctx* c) {
   b.counters = c->d_counters.ptr;
   b.sc = c->sp;
   b.mode = c->mode;
+  if (c->res) {
+    b.res2 = c->res->d2;
+    b.resN = c->res->dN;
+    b.res_base = c->res->dbase;
+    b.win_ctg = c->d_win_ctg.ptr;
+    b.win_off = c->d_win_off.ptr;
+  }
   return b;
 }
 
@@ -689,6 +785,343 @@ int mhmsw_get_stats(mhmsw_ctx* c, mhmsw_stats* out) {
   if (!c || !out) return MHMSW_E_ARG;
   if (!c->have_results) return fail(c, MHMSW_E_STATE, "no completed batch");
   *out = c->stats;
+  return MHMSW_OK;
+}
+
+
+// ----------------------------------------------------------------------------
+// resident contigs
+// ----------------------------------------------------------------------------
+extern "C" int mhmsw_contigs_load(int device_id, int64_t nctgs, const char* seqs, const int64_t* off, const int32_t* len,
+                                  mhmsw_contigs** out) {
+  if (!out || nctgs < 0 || (nctgs > 0 && (!seqs || !off || !len))) return MHMSW_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MHMSW_E_NODEVICE;
+  if (device_id < 0 || device_id >= n) return MHMSW_E_ARG;
+  if (hipSetDevice(device_id) != hipSuccess) return MHMSW_E_HIP;
+  mhmsw_contigs* r = new (std::nothrow) mhmsw_contigs();
+  if (!r) return MHMSW_E_NOMEM;
+  r->device = device_id;
+  r->n = nctgs;
+  r->len.assign(len, len + nctgs);
+  std::vector<int64_t> base((size_t)std::max<int64_t>(nctgs, 1));
+  int64_t acc = 0;
+  for (int64_t i = 0; i < nctgs; i++) {
+    if (len[i] < 0) {
+      delete r;
+      return MHMSW_E_ARG;
+    }
+    base[(size_t)i] = acc;
+    acc += ((int64_t)len[i] + 31) / 32 * 32;
+  }
+  const size_t n2 = (size_t)(acc / 16 + 4), nm = (size_t)(acc / 32 + 4);
+  std::vector<uint32_t> h2(n2, 0), hN(nm, 0);
+  {
+    int nth = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    auto work = [&](int tid) {
+      for (int64_t i = tid; i < nctgs; i += nth) {
+        const char* src = seqs + off[i];
+        uint32_t* two = h2.data() + base[(size_t)i] / 16;
+        uint32_t* nb = hN.data() + base[(size_t)i] / 32;
+        for (int32_t x = 0; x < len[i]; x++) {
+          const uint32_t code = base_code((unsigned char)src[x]);
+          if (code == 4u) {
+            nb[x >> 5] |= 1u << (x & 31);
+          } else {
+            two[x >> 4] |= code << (2 * (x & 15));
+          }
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++) th.emplace_back(work, t);
+    for (auto& t : th) t.join();
+  }
+  bool ok = hipMalloc((void**)&r->d2, n2 * 4) == hipSuccess && hipMalloc((void**)&r->dN, nm * 4) == hipSuccess &&
+            hipMalloc((void**)&r->dbase, base.size() * 8) == hipSuccess &&
+            hipMemcpy(r->d2, h2.data(), n2 * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(r->dN, hN.data(), nm * 4, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(r->dbase, base.data(), base.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    mhmsw_contigs_free(r);
+    return MHMSW_E_HIP;
+  }
+  r->bytes = (int64_t)(n2 * 4 + nm * 4 + base.size() * 8);
+  *out = r;
+  return MHMSW_OK;
+}
+
+extern "C" int mhmsw_contigs_free(mhmsw_contigs* r) {
+  if (!r) return MHMSW_E_ARG;
+  (void)hipSetDevice(r->device);
+  if (r->d2) (void)hipFree(r->d2);
+  if (r->dN) (void)hipFree(r->dN);
+  if (r->dbase) (void)hipFree(r->dbase);
+  delete r;
+  return MHMSW_OK;
+}
+
+extern "C" int64_t mhmsw_contigs_bytes(const mhmsw_contigs* r) { return r ? r->bytes : 0; }
+
+// ----------------------------------------------------------------------------
+// serving loop: continuous batching over several contexts
+// ----------------------------------------------------------------------------
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+namespace {
+
+double steady_now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  uint64_t id = 0;
+  int64_t npairs = 0;
+  std::vector<char> reads;
+  std::vector<int64_t> roff;
+  std::vector<int32_t> rlen, cid, woff, wlen;
+  int state = 0;  // 0 queued, 1 running, 2 done, < 0 failed
+  int slot = -1;
+  int64_t est = 0;
+  std::vector<mhmsw_result> results;
+  mhmsw_req_times times{};
+  std::string err;
+};
+
+// device bytes a request's job needs (codes 1 B/base for both copies, 2-bit +
+// bitmap 0.375 B/base, ~140 B per pair of tables)
+int64_t estimate_bytes(const Request& r) {
+  int64_t bases = 0;
+  for (int64_t p = 0; p < r.npairs; p++) bases += r.rlen[(size_t)p] + r.wlen[(size_t)p] + 64;
+  return bases * 3 / 2 + r.npairs * 140 + (1 << 20);
+}
+
+}  // namespace
+
+struct mhmsw_server {
+  int device = 0;
+  const mhmsw_contigs* contigs = nullptr;
+  std::vector<mhmsw_ctx*> slots;
+  std::vector<std::shared_ptr<Request>> running;  // per slot
+  int64_t budget = 0, used = 0;
+  std::mutex mu;
+  std::condition_variable cv_done, cv_work;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread th;
+  std::string err;
+};
+
+static void server_loop(mhmsw_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (!s->stop) {
+    bool progressed = false;
+    // completions
+    for (size_t k = 0; k < s->slots.size(); k++) {
+      std::shared_ptr<Request> r = s->running[k];
+      if (!r) continue;
+      lk.unlock();
+      int q = mhmsw_query(s->slots[k]);
+      int rc = MHMSW_OK;
+      mhmsw_stats st{};
+      if (q == 1) {
+        r->results.resize((size_t)std::max<int64_t>(r->npairs, 1));
+        rc = mhmsw_results(s->slots[k], r->results.data(), r->npairs);
+        if (rc == MHMSW_OK) mhmsw_get_stats(s->slots[k], &st);
+      }
+      lk.lock();
+      if (q == 0) continue;
+      progressed = true;
+      r->times.done_s = steady_now();
+      r->times.device_ms = st.job_ms;
+      if (q < 0 || rc != MHMSW_OK) {
+        r->state = q < 0 ? q : rc;
+        r->err = mhmsw_last_error(s->slots[k]);
+      } else {
+        r->state = 2;
+      }
+      s->running[k].reset();
+      s->used -= r->est;
+      r->reads.clear();
+      r->reads.shrink_to_fit();
+      s->cv_done.notify_all();
+    }
+    // admissions, in arrival order
+    while (!s->queue.empty()) {
+      int freek = -1, busy = 0;
+      for (size_t k = 0; k < s->slots.size(); k++) {
+        if (!s->running[k]) {
+          if (freek < 0) freek = (int)k;
+        } else {
+          busy++;
+        }
+      }
+      if (freek < 0) break;
+      std::shared_ptr<Request> r = s->queue.front();
+      if (busy > 0 && s->used + r->est > s->budget) break;  // wait for memory; alone it always runs
+      s->queue.pop_front();
+      r->state = 1;
+      r->slot = freek;
+      r->times.admit_s = steady_now();
+      s->running[(size_t)freek] = r;
+      s->used += r->est;
+      lk.unlock();
+      mhmsw_ctx* c = s->slots[(size_t)freek];
+      int rc = mhmsw_stage_windows(c, s->contigs, r->npairs, r->reads.data(), r->roff.data(), r->rlen.data(),
+                                   r->cid.data(), r->woff.data(), r->wlen.data());
+      if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
+      lk.lock();
+      progressed = true;
+      if (rc != MHMSW_OK) {
+        r->state = rc;
+        r->err = mhmsw_last_error(c);
+        r->times.done_s = steady_now();
+        s->running[(size_t)freek].reset();
+        s->used -= r->est;
+        s->cv_done.notify_all();
+      }
+    }
+    if (!progressed) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+  }
+}
+
+extern "C" {
+
+double mhmsw_server_clock(void) { return steady_now(); }
+
+const char* mhmsw_server_last_error(const mhmsw_server* s) { return s ? s->err.c_str() : "null server"; }
+
+int mhmsw_server_create(int device_id, const mhmsw_scoring* scoring, const mhmsw_contigs* contigs, int slots,
+                        int64_t hbm_budget, mhmsw_server** out) {
+  if (!out || !contigs || slots < 1 || slots > 16) return MHMSW_E_ARG;
+  *out = nullptr;
+  if (contigs->device != device_id) return MHMSW_E_ARG;
+  mhmsw_server* s = new (std::nothrow) mhmsw_server();
+  if (!s) return MHMSW_E_NOMEM;
+  s->device = device_id;
+  s->contigs = contigs;
+  for (int k = 0; k < slots; k++) {
+    mhmsw_ctx* c = nullptr;
+    int rc = mhmsw_create(device_id, &c);
+    if (rc == MHMSW_OK && scoring) rc = mhmsw_set_scoring(c, scoring);
+    if (rc == MHMSW_OK) rc = mhmsw_set_graphs(c, 0);  // every request has its own geometry
+    if (rc != MHMSW_OK) {
+      if (c) mhmsw_destroy(c);
+      for (auto* x : s->slots) mhmsw_destroy(x);
+      delete s;
+      return rc;
+    }
+    s->slots.push_back(c);
+  }
+  s->running.resize((size_t)slots);
+  if (hbm_budget <= 0) {
+    size_t tot = 0;
+    mhmsw_device_total_mem(device_id, &tot);
+    hbm_budget = (int64_t)(tot * 0.8) - contigs->bytes;
+  }
+  s->budget = hbm_budget;
+  s->th = std::thread(server_loop, s);
+  *out = s;
+  return MHMSW_OK;
+}
+
+int mhmsw_server_submit(mhmsw_server* s, int64_t npairs, const char* reads, const int64_t* read_off,
+                        const int32_t* read_len, const int32_t* ctg_id, const int32_t* win_off, const int32_t* win_len,
+                        uint64_t* req_id) {
+  if (!s || !req_id || npairs < 0 || (npairs > 0 && (!reads || !read_off || !read_len || !ctg_id || !win_off || !win_len)))
+    return MHMSW_E_ARG;
+  auto r = std::make_shared<Request>();
+  r->times.submit_s = steady_now();
+  r->npairs = npairs;
+  // private copy of the request: reads re-based to offset 0
+  int64_t tot = 0;
+  for (int64_t p = 0; p < npairs; p++) tot += read_len[p];
+  r->reads.resize((size_t)tot + 1);
+  r->roff.resize((size_t)npairs);
+  r->rlen.assign(read_len, read_len + npairs);
+  r->cid.assign(ctg_id, ctg_id + npairs);
+  r->woff.assign(win_off, win_off + npairs);
+  r->wlen.assign(win_len, win_len + npairs);
+  int64_t o = 0;
+  for (int64_t p = 0; p < npairs; p++) {
+    r->roff[(size_t)p] = o;
+    std::memcpy(r->reads.data() + o, reads + read_off[p], (size_t)read_len[p]);
+    o += read_len[p];
+  }
+  r->est = estimate_bytes(*r);
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    r->id = s->next_id++;
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    *req_id = r->id;
+  }
+  s->cv_work.notify_one();
+  return MHMSW_OK;
+}
+
+int mhmsw_server_status(mhmsw_server* s, uint64_t req_id) {
+  if (!s) return MHMSW_E_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(req_id);
+  if (it == s->reqs.end()) return MHMSW_E_ARG;
+  return it->second->state;
+}
+
+int mhmsw_server_wait(mhmsw_server* s, uint64_t req_id) {
+  if (!s) return MHMSW_E_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(req_id);
+  if (it == s->reqs.end()) return MHMSW_E_ARG;
+  std::shared_ptr<Request> r = it->second;
+  s->cv_done.wait(lk, [&] { return r->state == 2 || r->state < 0; });
+  if (r->state < 0) {
+    s->err = r->err;
+    return r->state;
+  }
+  return MHMSW_OK;
+}
+
+int mhmsw_server_results(mhmsw_server* s, uint64_t req_id, mhmsw_result* out, int64_t npairs, mhmsw_req_times* times) {
+  if (!s) return MHMSW_E_ARG;
+  int rc = mhmsw_server_wait(s, req_id);
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(req_id);
+  if (it == s->reqs.end()) return MHMSW_E_ARG;
+  std::shared_ptr<Request> r = it->second;
+  if (times) *times = r->times;
+  if (rc != MHMSW_OK) {
+    s->reqs.erase(it);
+    return rc;
+  }
+  if (npairs != r->npairs || (npairs > 0 && !out)) {
+    s->err = "results requested for a different number of pairs";
+    return MHMSW_E_ARG;
+  }
+  if (npairs > 0) std::memcpy(out, r->results.data(), (size_t)npairs * sizeof(mhmsw_result));
+  s->reqs.erase(it);
+  return MHMSW_OK;
+}
+
+int mhmsw_server_destroy(mhmsw_server* s) {
+  if (!s) return MHMSW_E_ARG;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (auto* c : s->slots) mhmsw_destroy(c);
+  delete s;
   return MHMSW_OK;
 }
 

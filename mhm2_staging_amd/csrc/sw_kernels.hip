@@ -151,6 +151,35 @@ __global__ void unpack_kernel(BatchView b) {
   }
 }
 
+// Contig windows of a windowed batch: output dword w of pair p's window holds
+// bases win_off+8w .. +7 of resident contig win_ctg[p], taken from the
+// resident 2-bit codes and N bitmap with one funnel shift each.
+__global__ void extract_windows_kernel(BatchView b) {
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int lane = threadIdx.x & 63;
+  for (int64_t p = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; p < b.npairs; p += nwaves) {
+    const int64_t s = 2 * p + 1;
+    const int64_t base = b.seq_base[s];
+    const int32_t len = b.seq_len[s];
+    const int64_t src0 = b.res_base[b.win_ctg[p]] + b.win_off[p];
+    const int32_t ndw = (len + 7) >> 3;
+    for (int32_t w = lane; w < ndw; w += 64) {
+      const int64_t sb = src0 + 8 * (int64_t)w;
+      const uint32_t two = __builtin_amdgcn_alignbit(b.res2[(sb >> 4) + 1], b.res2[sb >> 4], (uint32_t)(sb & 15) * 2u);
+      const uint32_t nm = __builtin_amdgcn_alignbit(b.resN[(sb >> 5) + 1], b.resN[sb >> 5], (uint32_t)(sb & 31));
+      uint32_t out = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (8 * w + k < len) {
+          const uint32_t c = ((nm >> k) & 1u) ? (uint32_t)kCodeN : (two >> (2 * k)) & 3u;
+          out |= c << (4 * k);
+        }
+      }
+      b.codes[(base >> 3) + w] = out;
+    }
+  }
+}
+
 // one block per sequence: reversed copy at [total + base, total + base + len).
 // Output dword w holds reversed positions 8w..8w+7 = source positions
 // len-1-8w down to len-8-8w: one funnel shift of two source dwords, then a
@@ -767,6 +796,8 @@ hipError_t launch_clear_counters(const BatchView& b, hipStream_t st) {
 
 hipError_t launch_unpack(const BatchView& b, hipStream_t st) {
   hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 3, 256)), dim3(256), 0, st, b);
+  if (b.win_ctg && b.npairs > 0)
+    hipLaunchKernelGGL(extract_windows_kernel, dim3(grid_for(b.npairs, 4)), dim3(256), 0, st, b);
   if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3(grid_for(2 * b.npairs, 4)), dim3(256), 0, st, b);
   return hipGetLastError();
 }

@@ -30,10 +30,16 @@ struct BatchView {
   unsigned long long* counters;  // [0] cells of the reverse pass
   ScoreParams sc;
   int32_t mode;     // 0 fast, 1 plain
+  // windowed batches: contig window of pair p = resident contig win_ctg[p] at win_off[p]
+  const uint32_t* res2;      // resident contigs, 2-bit codes (null for string batches)
+  const uint32_t* resN;      // resident contigs, N bitmap
+  const int64_t* res_base;   // first base of each resident contig
+  const int32_t* win_ctg;
+  const int32_t* win_off;
 };
 
 hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
-hipError_t launch_unpack(const BatchView& b, hipStream_t st);
+hipError_t launch_unpack(const BatchView& b, hipStream_t st);  // also extracts resident windows
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);

@@ -275,3 +275,94 @@ extern "C" void mhmsw_synth_free(mhmsw_synth_batch *b) {
   std::free(b->ctg_len);
   std::memset(b, 0, sizeof(*b));
 }
+
+extern "C" int mhmsw_synth_contigs(uint64_t seed, int64_t nctgs, int32_t min_len, int32_t max_len, int threads,
+                                   mhmsw_synth_contig_set *out) {
+  if (!out || nctgs <= 0 || min_len < 1 || max_len < min_len) return -1;
+  std::memset(out, 0, sizeof(*out));
+  std::vector<int32_t> len((size_t)nctgs);
+  std::vector<int64_t> off((size_t)nctgs);
+  Rng r(stream_seed(seed, 4, 0));
+  int64_t acc = 0;
+  for (int64_t c = 0; c < nctgs; c++) {
+    double l = std::exp(std::log((double)min_len) + r.uniform() * (std::log((double)max_len) - std::log((double)min_len)));
+    len[(size_t)c] = std::max<int32_t>(min_len, std::min<int32_t>(max_len, (int32_t)l));
+    off[(size_t)c] = acc;
+    acc += len[(size_t)c];
+  }
+  out->seqs = (char *)std::malloc((size_t)acc + 1);
+  out->off = (int64_t *)std::malloc(sizeof(int64_t) * (size_t)nctgs);
+  out->len = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)nctgs);
+  if (!out->seqs || !out->off || !out->len) {
+    mhmsw_synth_contigs_free(out);
+    return -3;
+  }
+  std::memcpy(out->off, off.data(), sizeof(int64_t) * (size_t)nctgs);
+  std::memcpy(out->len, len.data(), sizeof(int32_t) * (size_t)nctgs);
+  char *S = out->seqs;
+  parallel_pairs(nctgs, threads, [&](int64_t c) {
+    Rng rc(stream_seed(seed, 5, (uint64_t)c));
+    for (int32_t i = 0; i < len[(size_t)c]; i++) S[off[(size_t)c] + i] = random_base(rc);
+  });
+  S[acc] = 0;
+  out->nctgs = nctgs;
+  out->bytes = acc;
+  return 0;
+}
+
+extern "C" void mhmsw_synth_contigs_free(mhmsw_synth_contig_set *set) {
+  if (!set) return;
+  std::free(set->seqs);
+  std::free(set->off);
+  std::free(set->len);
+  std::memset(set, 0, sizeof(*set));
+}
+
+extern "C" int mhmsw_synth_window_pairs(const mhmsw_synth_contig_set *ctgs, uint64_t seed, int64_t npairs, int threads,
+                                        mhmsw_synth_window_batch *out) {
+  if (!ctgs || !out || npairs < 0 || ctgs->nctgs <= 0) return -1;
+  std::memset(out, 0, sizeof(*out));
+  const int32_t rl = 150;
+  out->reads = (char *)std::malloc((size_t)npairs * rl + 1);
+  out->read_off = (int64_t *)std::malloc(sizeof(int64_t) * (size_t)std::max<int64_t>(npairs, 1));
+  out->read_len = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(npairs, 1));
+  out->ctg_id = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(npairs, 1));
+  out->win_off = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(npairs, 1));
+  out->win_len = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(npairs, 1));
+  if (!out->reads || !out->read_off || !out->read_len || !out->ctg_id || !out->win_off || !out->win_len) {
+    mhmsw_synth_window_pairs_free(out);
+    return -3;
+  }
+  parallel_pairs(npairs, threads, [&](int64_t p) {
+    Rng r(stream_seed(seed, 6, (uint64_t)p));
+    int64_t c = r.range(0, ctgs->nctgs - 1);
+    const int32_t cl = ctgs->len[c];
+    const char *C = ctgs->seqs + ctgs->off[c];
+    int32_t wl = (int32_t)std::min<int64_t>(cl, r.range(300, 1024));
+    int32_t wo = (int32_t)r.range(0, cl - wl);
+    int64_t src = wo + r.range(0, std::max<int32_t>(0, wl - rl));
+    std::string read;
+    mutate_copy(r, C, cl, src, rl, 0.01, 0.001, read);
+    std::memcpy(out->reads + p * rl, read.data(), (size_t)rl);
+    out->read_off[p] = p * rl;
+    out->read_len[p] = rl;
+    out->ctg_id[p] = (int32_t)c;
+    out->win_off[p] = wo;
+    out->win_len[p] = wl;
+  });
+  out->reads[npairs * rl] = 0;
+  out->npairs = npairs;
+  out->reads_bytes = npairs * rl;
+  return 0;
+}
+
+extern "C" void mhmsw_synth_window_pairs_free(mhmsw_synth_window_batch *b) {
+  if (!b) return;
+  std::free(b->reads);
+  std::free(b->read_off);
+  std::free(b->read_len);
+  std::free(b->ctg_id);
+  std::free(b->win_off);
+  std::free(b->win_len);
+  std::memset(b, 0, sizeof(*b));
+}

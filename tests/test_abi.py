@@ -99,3 +99,18 @@ int main() {
                     f"-Wl,-rpath,{os.path.dirname(LIB)}"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     assert out.strip() in ("nogpu", "4 0 3 1 4")
+
+
+def test_serving_entry_points_fail_cleanly_on_cpu():
+    import ctypes
+    import mhm2_staging_amd as m
+    if m.get_num_node_gpus() > 0:
+        pytest.skip("a GPU is visible here")
+    L = m.lib()
+    h = ctypes.c_void_p()
+    buf = (ctypes.c_char * 8)(*b"ACGTACGT")
+    off = (ctypes.c_int64 * 1)(0)
+    ln = (ctypes.c_int32 * 1)(8)
+    assert L.mhmsw_contigs_load(0, 1, buf, off, ln, ctypes.byref(h)) == -4
+    srv = ctypes.c_void_p()
+    assert L.mhmsw_server_create(0, None, None, 1, 0, ctypes.byref(srv)) == -1

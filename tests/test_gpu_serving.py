@@ -1,0 +1,84 @@
+"""Device-resident contigs and the continuous-batching serving loop (GPU)."""
+import numpy as np
+import pytest
+
+import mhm2_staging_amd as m
+from golden.kat_cases import BASELINE_SCORING
+from test_gpu_parity import assert_same, gpu_table, oracle_table
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def serving_set():
+    cs = m.contigs(nctgs=4000, seed=51)
+    res = m.ResidentContigs(cs, device=0)
+    yield cs, res
+    res.close()
+
+
+def test_resident_windows_equal_string_windows_and_oracle(gpu_aligner, serving_set):
+    cs, res = serving_set
+    wb = m.window_pairs(cs, 3000, seed=52)
+    sb = wb.as_strings(cs)
+    want = oracle_table(sb, BASELINE_SCORING)
+    al = gpu_aligner
+    al.set_scoring(BASELINE_SCORING)
+    al.set_orientation(True)
+    for mode in (m.MODE_FAST, m.MODE_PLAIN):
+        al.set_mode(mode)
+        print(f"[gpu] resident windows mode={mode}", flush=True)
+        al.stage_windows(res, wb)
+        al.align_staged()
+        al.wait()
+        assert_same(gpu_table(al.results()), want, sb)
+        al.stage(sb)
+        al.align_staged()
+        al.wait()
+        assert_same(gpu_table(al.results()), want, sb)
+    al.set_mode(m.MODE_FAST)
+
+
+def test_server_results_same_alone_and_in_the_mix(serving_set):
+    cs, res = serving_set
+    sizes = [1000, 7000, 2500, 15000, 600, 4000]
+    reqs = [m.window_pairs(cs, n, seed=60 + i) for i, n in enumerate(sizes)]
+    # alone: a fresh server, one request at a time
+    alone = []
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    for r in reqs:
+        rid = srv.submit(r)
+        alone.append(srv.results(rid)[0])
+    srv.close()
+    # in the mix: all submitted back to back on 4 slots, overlapping on the device
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=4)
+    ids = [srv.submit(r) for r in reqs]
+    mixed = [srv.results(i) for i in ids]
+    srv.close()
+    overlapped = sum(1 for a, b in zip(mixed, mixed[1:]) if b[1]["admit_s"] < a[1]["done_s"])
+    assert overlapped > 0, "no two requests were on the device together"
+    for r, a, (b, t) in zip(reqs, alone, mixed):
+        assert (a == b).all()
+        assert t["submit_s"] <= t["admit_s"] <= t["done_s"]
+    # and they are the SSW answers
+    for r, a in zip(reqs[:3], alone[:3]):
+        sb = r.as_strings(cs)
+        assert_same(gpu_table(a), oracle_table(sb, BASELINE_SCORING), sb)
+
+
+def test_server_hbm_budget_admission_in_order(serving_set):
+    """A budget that fits one request at a time: requests queue, run one by one, in arrival order."""
+    cs, res = serving_set
+    reqs = [m.window_pairs(cs, 3000, seed=80 + i) for i in range(4)]
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=4, hbm_budget=8 << 20)
+    ids = [srv.submit(r) for r in reqs]
+    outs = [srv.results(i) for i in ids]
+    srv.close()
+    admits = [t["admit_s"] for _, t in outs]
+    dones = [t["done_s"] for _, t in outs]
+    assert admits == sorted(admits)
+    for k in range(1, len(outs)):
+        assert admits[k] >= dones[k - 1] - 1e-6  # serialized by the budget
+    for r, (o, _) in zip(reqs[:2], outs[:2]):
+        sb = r.as_strings(cs)
+        assert_same(gpu_table(o), oracle_table(sb, BASELINE_SCORING), sb)
