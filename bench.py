@@ -78,17 +78,18 @@ def serving_bench(qps_list, duration, slots=4, seed=5):
         arrivals = np.cumsum(gaps)
         arrivals = arrivals[arrivals < duration]
         t0 = time.perf_counter()
+        c0 = srv.clock()  # server clock at t0: latencies run from the scheduled arrival
         ids = []
         for i, a in enumerate(arrivals):
             d = t0 + a - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             b = pool[i % len(pool)]
-            ids.append((srv.submit(b), b.cells(), b.npairs))
+            ids.append((srv.submit(b), b.cells(), b.npairs, c0 + a))
         lat, cells, first, last, pairs = [], 0, None, None, 0
-        for rid, c, n in ids:
+        for rid, c, n, arrival in ids:
             _, t = srv.results(rid)
-            lat.append(t["done_s"] - t["submit_s"])
+            lat.append(t["done_s"] - arrival)
             cells += c
             pairs += n
             first = t["submit_s"] if first is None else min(first, t["submit_s"])

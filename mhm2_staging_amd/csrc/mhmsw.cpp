@@ -47,27 +47,6 @@ struct DevBuf {
   }
 };
 
-template <class T>
-struct HostBuf {
-  T* ptr = nullptr;
-  size_t cap = 0;
-  hipError_t ensure(size_t n) {
-    if (n <= cap) return hipSuccess;
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    size_t want = std::max<size_t>(n + n / 4, 64);
-    hipError_t e = hipHostMalloc((void**)&ptr, want * sizeof(T), hipHostMallocDefault);
-    if (e == hipSuccess) cap = want;
-    return e;
-  }
-  void release() {
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-  }
-};
-
 inline uint32_t base_code(unsigned char c) {
   switch (c) {
     case 'A': case 'a': case 'U': case 'u': return 0;
@@ -112,9 +91,6 @@ struct mhmsw_ctx {
   bool in_flight = false;
   bool have_results = false;
 
-  HostBuf<uint32_t> h_packed2, h_nmask;
-  HostBuf<int64_t> h_base;
-  HostBuf<int32_t> h_len;
   DevBuf<uint32_t> d_packed2, d_nmask, d_codes;
   DevBuf<int64_t> d_base;
   DevBuf<int32_t> d_len, d_bucket, d_rank, d_errors;
@@ -182,17 +158,6 @@ static void compute_score_params(mhmsw_ctx* c) {
   } else {
     p.perm_lo = p.perm_hi = 0;
   }
-}
-
-// Host copy of the device's routing (make_task in sw_kernels.hip) for the
-// forward pass; used to launch only the classes that have work.
-static int fwd_class(const mhmsw_ctx* c, int32_t lref, int32_t lqry) {
-  if (lref <= 0 || lqry <= 0) return -1;
-  const bool wide = c->mode == MHMSW_MODE_PLAIN || !c->sp.fast_ok;
-  const int32_t shorter = std::min(lref, lqry), longer = std::max(lref, lqry);
-  if (!wide && shorter <= kFast16MaxRows && longer <= kFastMaxCols) return kClassFast16 + (shorter + 31) / 32 - 1;
-  if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) return kClassFast64 + (shorter + 127) / 128 - 1;
-  return kClassPlain;
 }
 
 extern "C" {
@@ -265,10 +230,6 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   if (!c) return MHMSW_E_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  c->h_packed2.release();
-  c->h_nmask.release();
-  c->h_base.release();
-  c->h_len.release();
   c->d_packed2.release();
   c->d_nmask.release();
   c->d_codes.release();
@@ -354,21 +315,184 @@ struct StageInput {
   const int32_t* win_len = nullptr;
 };
 
-static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
+// What routing needs from a context: the forward work class of a pair.
+struct PackParams {
+  int mode = MHMSW_MODE_FAST;
+  int fast_ok = 1;
+  int read_is_ref = 1;
+};
+
+static int route_class(const PackParams& pp, int32_t lref, int32_t lqry) {
+  if (lref <= 0 || lqry <= 0) return -1;
+  const bool wide = pp.mode == MHMSW_MODE_PLAIN || !pp.fast_ok;
+  const int32_t shorter = std::min(lref, lqry), longer = std::max(lref, lqry);
+  if (!wide && shorter <= kFast16MaxRows && longer <= kFastMaxCols) return kClassFast16 + (shorter + 31) / 32 - 1;
+  if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) return kClassFast64 + (shorter + 127) / 128 - 1;
+  return kClassPlain;
+}
+
+// A batch laid out and packed on the host (2-bit codes + N bitmap), ready to
+// be copied to a context.  Packing is pure CPU work, so the serving loop does
+// it on the submitting thread.
+struct PackedBatch {
+  int64_t npairs = 0, total = 0, cells = 0;
+  int max_plain_cols = 0;
+  bool used[kNumClasses] = {};
+  int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
+  int64_t nfast = 0, nplain = 0;
+  std::vector<int64_t> base;
+  std::vector<int32_t> len;
+  std::vector<uint32_t> two, nm;
+  std::vector<int32_t> win_ctg, win_off;
+  const mhmsw_contigs* res = nullptr;
+  std::string err;
+};
+
+namespace {
+struct CodeTables {
+  uint8_t two[256];
+  uint8_t isn[256];
+  CodeTables() {
+    for (int c = 0; c < 256; c++) {
+      const uint32_t k = base_code((unsigned char)c);
+      two[c] = (uint8_t)(k == 4 ? 0 : k);
+      isn[c] = (uint8_t)(k == 4 ? 1 : 0);
+    }
+  }
+};
+const CodeTables kCodes;
+
+// one sequence into whole dwords (its base is a multiple of 32)
+void pack_seq(const char* src, int32_t len, uint32_t* two, uint32_t* nb) {
+  const unsigned char* u = (const unsigned char*)src;
+  for (int32_t x0 = 0; x0 < len; x0 += 32) {
+    uint32_t w0 = 0, w1 = 0, n = 0;
+    const int32_t m = std::min<int32_t>(32, len - x0);
+    for (int32_t k = 0; k < m; k++) {
+      const unsigned char ch = u[x0 + k];
+      const uint32_t code = kCodes.two[ch];
+      if (k < 16) {
+        w0 |= code << (2 * k);
+      } else {
+        w1 |= code << (2 * (k - 16));
+      }
+      n |= (uint32_t)kCodes.isn[ch] << k;
+    }
+    two[x0 >> 4] = w0;
+    if (m > 16) two[(x0 >> 4) + 1] = w1;
+    nb[x0 >> 5] = n;
+  }
+}
+}  // namespace
+
+static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& pb, int max_threads) {
   const int64_t npairs = in.npairs;
-  const char* reads = in.reads;
-  const int64_t* read_off = in.read_off;
-  const int32_t* read_len = in.read_len;
-  const char* ctgs = in.ctgs;
-  const int64_t* ctg_off = in.ctg_off;
   const bool windowed = in.res != nullptr;
-  if (npairs < 0 || (npairs > 0 && (!reads || !read_off || !read_len)) ||
-      (npairs > 0 && !windowed && (!ctgs || !ctg_off || !in.ctg_len)) ||
-      (npairs > 0 && windowed && (!in.ctg_id || !in.win_off || !in.win_len)))
-    return fail(c, MHMSW_E_ARG, "bad batch arguments");
-  if (windowed && in.res->device != c->device)
-    return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", in.res->device, c->device);
-  if (npairs > (int64_t)1 << 30) return fail(c, MHMSW_E_ARG, "batch of %lld pairs is too large", (long long)npairs);
+  char msg[256];
+  if (npairs < 0 || (npairs > 0 && (!in.reads || !in.read_off || !in.read_len)) ||
+      (npairs > 0 && !windowed && (!in.ctgs || !in.ctg_off || !in.ctg_len)) ||
+      (npairs > 0 && windowed && (!in.ctg_id || !in.win_off || !in.win_len))) {
+    pb.err = "bad batch arguments";
+    return MHMSW_E_ARG;
+  }
+  if (npairs > (int64_t)1 << 30) {
+    pb.err = "batch too large (more than 2^30 pairs)";
+    return MHMSW_E_ARG;
+  }
+  const int64_t nseq = 2 * npairs;
+  pb = PackedBatch();
+  pb.npairs = npairs;
+  pb.res = in.res;
+  pb.base.resize((size_t)std::max<int64_t>(nseq, 1));
+  pb.len.resize((size_t)std::max<int64_t>(nseq, 1));
+  int64_t acc = kPadBases;
+  for (int64_t p = 0; p < npairs; p++) {
+    const int32_t rl = in.read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
+    if (rl < 0 || cl < 0) {
+      snprintf(msg, sizeof(msg), "negative length at pair %lld", (long long)p);
+      pb.err = msg;
+      return MHMSW_E_ARG;
+    }
+    if (windowed) {
+      const int32_t id = in.ctg_id[p];
+      if (id < 0 || id >= in.res->n || in.win_off[p] < 0 || (int64_t)in.win_off[p] + cl > in.res->len[(size_t)id]) {
+        snprintf(msg, sizeof(msg), "pair %lld: window outside its resident contig", (long long)p);
+        pb.err = msg;
+        return MHMSW_E_ARG;
+      }
+    }
+    if (std::min(rl, cl) > kPlainMaxCols) {
+      snprintf(msg, sizeof(msg), "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl, cl);
+      pb.err = msg;
+      return MHMSW_E_ARG;
+    }
+    if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20)) {
+      snprintf(msg, sizeof(msg), "pair %lld: sequence longer than 2^20 bases", (long long)p);
+      pb.err = msg;
+      return MHMSW_E_ARG;
+    }
+    pb.len[(size_t)(2 * p)] = rl;
+    pb.len[(size_t)(2 * p + 1)] = cl;
+    pb.base[(size_t)(2 * p)] = acc;
+    acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+    pb.base[(size_t)(2 * p + 1)] = acc;
+    acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+    pb.cells += (int64_t)rl * cl;
+    pb.max_plain_cols = std::max(pb.max_plain_cols, (int)std::min(rl, cl));
+    const int32_t lref = pp.read_is_ref ? rl : cl, lqry = pp.read_is_ref ? cl : rl;
+    const int cls = route_class(pp, lref, lqry);
+    if (cls >= 0) {
+      pb.used[cls] = true;
+      pb.cpairs[cls]++;
+      pb.ccells[cls] += (int64_t)rl * cl;
+      if (cls == kClassPlain) {
+        pb.nplain++;
+      } else {
+        pb.nfast++;
+      }
+    }
+  }
+  acc += kPadBases;
+  pb.total = (acc + 31) / 32 * 32;
+  pb.two.assign((size_t)(pb.total / 16), 0u);
+  pb.nm.assign((size_t)(pb.total / 32), 0u);
+  int nth = std::max(1, std::min(max_threads, (int)std::max(1u, std::thread::hardware_concurrency())));
+  if (nseq < 8192) nth = 1;
+  auto work = [&](int tid) {
+    for (int64_t s = tid; s < nseq; s += nth) {
+      if ((s & 1) && windowed) continue;  // extracted on the device
+      const char* src = (s & 1) ? in.ctgs + in.ctg_off[s >> 1] : in.reads + in.read_off[s >> 1];
+      const int64_t b0 = pb.base[(size_t)s];
+      pack_seq(src, pb.len[(size_t)s], pb.two.data() + b0 / 16, pb.nm.data() + b0 / 32);
+    }
+  };
+  if (nth == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++) th.emplace_back(work, t);
+    for (auto& t : th) t.join();
+  }
+  if (windowed) {
+    pb.win_ctg.assign(in.ctg_id, in.ctg_id + npairs);
+    pb.win_off.assign(in.win_off, in.win_off + npairs);
+  }
+  return MHMSW_OK;
+}
+
+static PackParams pack_params(const mhmsw_ctx* c) {
+  PackParams pp;
+  pp.mode = c->mode;
+  pp.fast_ok = c->sp.fast_ok;
+  pp.read_is_ref = c->read_is_ref;
+  return pp;
+}
+
+// Copy a packed batch to the context's device buffers; returns when the host
+// buffers may be released.
+static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
+  if (pb.res && pb.res->device != c->device)
+    return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", pb.res->device, c->device);
   if (c->in_flight) {
     int rc = mhmsw_wait(c);
     if (rc != MHMSW_OK) return rc;
@@ -376,96 +500,17 @@ static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
   HIP_OR_FAIL(c, hipSetDevice(c->device));
   c->staged = false;
   c->have_results = false;
-  const int64_t nseq = 2 * npairs;
-  HIP_OR_FAIL(c, c->h_base.ensure((size_t)std::max<int64_t>(nseq, 1)));
-  HIP_OR_FAIL(c, c->h_len.ensure((size_t)std::max<int64_t>(nseq, 1)));
-  int64_t acc = kPadBases;
-  int64_t cells = 0;
-  int maxplain = 0;
-  bool used[kNumClasses] = {};
-  int64_t nfast = 0, nplain = 0;
-  int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
-  for (int64_t p = 0; p < npairs; p++) {
-    const int32_t rl = read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
-    if (rl < 0 || cl < 0) return fail(c, MHMSW_E_ARG, "negative length at pair %lld", (long long)p);
-    if (windowed) {
-      const int32_t id = in.ctg_id[p];
-      if (id < 0 || id >= in.res->n || in.win_off[p] < 0 || (int64_t)in.win_off[p] + cl > in.res->len[(size_t)id])
-        return fail(c, MHMSW_E_ARG, "pair %lld: window outside its resident contig", (long long)p);
-    }
-    if (std::min(rl, cl) > kPlainMaxCols)
-      return fail(c, MHMSW_E_ARG, "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl,
-                  cl);
-    if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20))
-      return fail(c, MHMSW_E_ARG, "pair %lld: sequence longer than 2^20 bases", (long long)p);
-    c->h_len.ptr[2 * p] = rl;
-    c->h_len.ptr[2 * p + 1] = cl;
-    c->h_base.ptr[2 * p] = acc;
-    acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-    c->h_base.ptr[2 * p + 1] = acc;
-    acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-    cells += (int64_t)rl * cl;
-    maxplain = std::max(maxplain, (int)std::min(rl, cl));
-    const int32_t lref = c->read_is_ref ? rl : cl, lqry = c->read_is_ref ? cl : rl;
-    const int cls = fwd_class(c, lref, lqry);
-    if (cls >= 0) {
-      used[cls] = true;
-      cpairs[cls]++;
-      ccells[cls] += (int64_t)rl * cl;
-      if (cls == kClassPlain) {
-        nplain++;
-      } else {
-        nfast++;
-      }
-    }
-  }
-  c->n_fast = nfast;
-  c->n_plain = nplain;
-  std::memcpy(c->class_pairs, cpairs, sizeof(cpairs));
-  std::memcpy(c->class_cells, ccells, sizeof(ccells));
-  acc += kPadBases;
-  const int64_t total = (acc + 31) / 32 * 32;
+  const int64_t npairs = pb.npairs, nseq = 2 * npairs, total = pb.total;
   c->npairs = npairs;
   c->total = total;
-  c->cells = cells;
-  c->max_plain_cols = maxplain;
-  std::memcpy(c->fast_class_used, used, sizeof(used));
-
-  // pack: 2-bit codes (16 per dword) and the N bitmap (32 per dword)
-  const size_t n2 = (size_t)(total / 16), nm = (size_t)(total / 32);
-  HIP_OR_FAIL(c, c->h_packed2.ensure(n2));
-  HIP_OR_FAIL(c, c->h_nmask.ensure(nm));
-  std::memset(c->h_packed2.ptr, 0, n2 * sizeof(uint32_t));
-  std::memset(c->h_nmask.ptr, 0, nm * sizeof(uint32_t));
-  {
-    int nth = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (nseq < 4096) nth = 1;
-    auto work = [&](int tid) {
-      for (int64_t s = tid; s < nseq; s += nth) {
-        if ((s & 1) && windowed) continue;  // extracted on the device
-        const char* src = (s & 1) ? ctgs + ctg_off[s >> 1] : reads + read_off[s >> 1];
-        const int32_t len = c->h_len.ptr[s];
-        const int64_t b0 = c->h_base.ptr[s];  // multiple of 32: whole dwords of both arrays
-        uint32_t* two = c->h_packed2.ptr + b0 / 16;
-        uint32_t* nb = c->h_nmask.ptr + b0 / 32;
-        for (int32_t x = 0; x < len; x++) {
-          const uint32_t code = base_code((unsigned char)src[x]);
-          if (code == 4u) {
-            nb[x >> 5] |= 1u << (x & 31);
-          } else {
-            two[x >> 4] |= code << (2 * (x & 15));
-          }
-        }
-      }
-    };
-    if (nth == 1) {
-      work(0);
-    } else {
-      std::vector<std::thread> th;
-      for (int t = 0; t < nth; t++) th.emplace_back(work, t);
-      for (auto& t : th) t.join();
-    }
-  }
+  c->cells = pb.cells;
+  c->max_plain_cols = pb.max_plain_cols;
+  c->n_fast = pb.nfast;
+  c->n_plain = pb.nplain;
+  std::memcpy(c->fast_class_used, pb.used, sizeof(pb.used));
+  std::memcpy(c->class_pairs, pb.cpairs, sizeof(pb.cpairs));
+  std::memcpy(c->class_cells, pb.ccells, sizeof(pb.ccells));
+  const size_t n2 = pb.two.size(), nm = pb.nm.size();
   const size_t np1 = (size_t)std::max<int64_t>(npairs, 1);
   HIP_OR_FAIL(c, c->d_packed2.ensure(n2));
   HIP_OR_FAIL(c, c->d_nmask.ensure(nm));
@@ -478,22 +523,33 @@ static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
   HIP_OR_FAIL(c, c->d_rev.ensure(np1));
   HIP_OR_FAIL(c, c->d_results.ensure(np1));
   HIP_OR_FAIL(c, c->d_tasks.ensure(np1 + 4 * kNumClasses + 64));
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr, c->h_packed2.ptr, n2 * 4, hipMemcpyHostToDevice, c->stream));
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr, c->h_nmask.ptr, nm * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr, pb.two.data(), n2 * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr, pb.nm.data(), nm * 4, hipMemcpyHostToDevice, c->stream));
   if (nseq > 0) {
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, c->h_base.ptr, (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, c->h_len.ptr, (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, pb.base.data(), (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, pb.len.data(), (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
   }
-  c->res = windowed ? in.res : nullptr;
-  if (windowed && npairs > 0) {
+  c->res = pb.res;
+  if (pb.res && npairs > 0) {
     HIP_OR_FAIL(c, c->d_win_ctg.ensure((size_t)npairs));
     HIP_OR_FAIL(c, c->d_win_off.ensure((size_t)npairs));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, in.ctg_id, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, in.win_off, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, pb.win_ctg.data(), (size_t)npairs * 4, hipMemcpyHostToDevice,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, pb.win_off.data(), (size_t)npairs * 4, hipMemcpyHostToDevice,
+                                  c->stream));
   }
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   c->staged = true;
   return MHMSW_OK;
+}
+
+static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
+  if (in.res && in.res->device != c->device)
+    return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", in.res->device, c->device);
+  PackedBatch pb;
+  int rc = pack_batch(pack_params(c), in, pb, 16);
+  if (rc != MHMSW_OK) return fail(c, rc, "%s", pb.err.c_str());
+  return upload_batch(c, pb);
 }
 
 extern "C" {
@@ -565,8 +621,14 @@ static BatchView make_view(mhmsw_ctx* c) {
 // ev: record phase events (eager launches only; events recorded inside a
 // captured graph cannot be timed)
 static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev, int* maxp16_out, int* maxp64_out) {
-  const int fast_grid = std::max(1, c->cus * 8);
-  const int plain_grid = std::max(1, c->cus * 8);
+  // persistent grids: enough wavefronts to fill the device, but no more
+  // blocks than a class can use (4 wavefronts per 256-thread block, 4 pairs
+  // per wavefront in 16-lane groups, 1 in 64-lane groups)
+  const int full = std::max(1, c->cus * 8);
+  auto grid_cap = [&](int64_t pairs, int pairs_per_block) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(full, (pairs + pairs_per_block - 1) / pairs_per_block));
+  };
+  const int plain_grid = grid_cap(c->npairs, 1);
   const int plain_cols = std::max(1, c->max_plain_cols);
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
   HIP_OR_FAIL(c, launch_clear_counters(b, st));
@@ -581,7 +643,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int k = kClassFast64 + P - 1;
       if (c->fast_class_used[k]) {
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
-        HIP_OR_FAIL(c, launch_fast(b, 64, P, fast_grid, st));
+        HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->class_pairs[k], 4), st));
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp64 = std::max(maxp64, P);
       }
@@ -590,7 +652,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int k = kClassFast16 + P - 1;
       if (c->fast_class_used[k]) {
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
-        HIP_OR_FAIL(c, launch_fast(b, 16, P, fast_grid, st));
+        HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->class_pairs[k], 16), st));
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp16 = std::max(maxp16, P);
       }
@@ -613,8 +675,11 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   if (c->npairs > 0) {
     HIP_OR_FAIL(c, launch_schedule(b, kPassRev, st));
     if (any_fast) {
-      for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, fast_grid, st));
-      for (int P = kMaxP; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, fast_grid, st));
+      // a reverse rectangle is never taller than its forward one, so the
+      // reverse classes are bounded by the forward ones
+      const int rev16 = maxp64 > 0 ? kMaxP : maxp16;
+      for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->n_fast, 4), st));
+      for (int P = rev16; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->n_fast, 16), st));
     }
     HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
   }
@@ -883,9 +948,7 @@ double steady_now() {
 struct Request {
   uint64_t id = 0;
   int64_t npairs = 0;
-  std::vector<char> reads;
-  std::vector<int64_t> roff;
-  std::vector<int32_t> rlen, cid, woff, wlen;
+  PackedBatch packed;  // packed on the submitting thread
   int state = 0;  // 0 queued, 1 running, 2 done, < 0 failed
   int slot = -1;
   int64_t est = 0;
@@ -897,15 +960,14 @@ struct Request {
 // device bytes a request's job needs (codes 1 B/base for both copies, 2-bit +
 // bitmap 0.375 B/base, ~140 B per pair of tables)
 int64_t estimate_bytes(const Request& r) {
-  int64_t bases = 0;
-  for (int64_t p = 0; p < r.npairs; p++) bases += r.rlen[(size_t)p] + r.wlen[(size_t)p] + 64;
-  return bases * 3 / 2 + r.npairs * 140 + (1 << 20);
+  return r.packed.total * 3 / 2 + r.npairs * 140 + (1 << 20);
 }
 
 }  // namespace
 
 struct mhmsw_server {
   int device = 0;
+  PackParams pp;
   const mhmsw_contigs* contigs = nullptr;
   std::vector<mhmsw_ctx*> slots;
   std::vector<std::shared_ptr<Request>> running;  // per slot
@@ -951,8 +1013,7 @@ static void server_loop(mhmsw_server* s) {
       }
       s->running[k].reset();
       s->used -= r->est;
-      r->reads.clear();
-      r->reads.shrink_to_fit();
+      r->packed = PackedBatch();  // release the host copy
       s->cv_done.notify_all();
     }
     // admissions, in arrival order
@@ -976,8 +1037,7 @@ static void server_loop(mhmsw_server* s) {
       s->used += r->est;
       lk.unlock();
       mhmsw_ctx* c = s->slots[(size_t)freek];
-      int rc = mhmsw_stage_windows(c, s->contigs, r->npairs, r->reads.data(), r->roff.data(), r->rlen.data(),
-                                   r->cid.data(), r->woff.data(), r->wlen.data());
+      int rc = upload_batch(c, r->packed);
       if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
       lk.lock();
       progressed = true;
@@ -1023,6 +1083,7 @@ int mhmsw_server_create(int device_id, const mhmsw_scoring* scoring, const mhmsw
     s->slots.push_back(c);
   }
   s->running.resize((size_t)slots);
+  s->pp = pack_params(s->slots[0]);
   if (hbm_budget <= 0) {
     size_t tot = 0;
     mhmsw_device_total_mem(device_id, &tot);
@@ -1042,20 +1103,21 @@ int mhmsw_server_submit(mhmsw_server* s, int64_t npairs, const char* reads, cons
   auto r = std::make_shared<Request>();
   r->times.submit_s = steady_now();
   r->npairs = npairs;
-  // private copy of the request: reads re-based to offset 0
-  int64_t tot = 0;
-  for (int64_t p = 0; p < npairs; p++) tot += read_len[p];
-  r->reads.resize((size_t)tot + 1);
-  r->roff.resize((size_t)npairs);
-  r->rlen.assign(read_len, read_len + npairs);
-  r->cid.assign(ctg_id, ctg_id + npairs);
-  r->woff.assign(win_off, win_off + npairs);
-  r->wlen.assign(win_len, win_len + npairs);
-  int64_t o = 0;
-  for (int64_t p = 0; p < npairs; p++) {
-    r->roff[(size_t)p] = o;
-    std::memcpy(r->reads.data() + o, reads + read_off[p], (size_t)read_len[p]);
-    o += read_len[p];
+  // pack here, on the submitting thread, off the dispatcher's path
+  StageInput in;
+  in.npairs = npairs;
+  in.reads = reads;
+  in.read_off = read_off;
+  in.read_len = read_len;
+  in.res = s->contigs;
+  in.ctg_id = ctg_id;
+  in.win_off = win_off;
+  in.win_len = win_len;
+  int rc = pack_batch(s->pp, in, r->packed, 4);
+  if (rc != MHMSW_OK) {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->err = r->packed.err;
+    return rc;
   }
   r->est = estimate_bytes(*r);
   {
