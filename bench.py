@@ -117,6 +117,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
     ap.add_argument("--serving-qps", type=str, default="50,400", help="comma-separated request rates")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -154,6 +155,11 @@ def main():
     t0 = time.perf_counter()
     al.stage(batch)
     log(f"[bench] staged (pack + H2D) in {time.perf_counter() - t0:.2f}s")
+    tuning = al.get_tuning()
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = al.autotune(repeats=2)
+        log(f"[bench] autotuned {tuning} in {time.perf_counter() - t0:.1f}s")
     for _ in range(args.warmup):
         al.align_staged()
         al.wait()
@@ -215,7 +221,8 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "i16 (packed, exact i32 rerun)",
            "data": "synthetic (seeded generator)",
            "config": {"workload": WORKLOADS[args.config], "npairs": batch.npairs, "cells": cells,
-                      "scoring": "match 1, mismatch 3, gap open 5, gap extend 2, N 2", "form": "fast"},
+                      "scoring": "match 1, mismatch 3, gap open 5, gap extend 2, N 2", "form": "fast",
+                      "tuning": tuning},
            "pairs_per_s": round(batch.npairs * args.steps / elapsed, 1),
            "device_job_ms": round(float(np.mean(job_ms)), 3),
            "stats": {kk: v for kk, v in st.items() if not kk.startswith("class_")},

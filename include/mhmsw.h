@@ -81,6 +81,20 @@ typedef struct {
   double class_ms[MHMSW_NUM_CLASSES];
 } mhmsw_stats;
 
+/* Scheduling parameters of the packed form (no counterpart in the reference,
+ * whose launch shape is fixed: one block per pair, driver.cpp:347-368).
+ *   blocks_per_cu  persistent 256-thread blocks launched per CU (wavefronts
+ *                  per CU = 4 x blocks_per_cu), 1..16
+ *   g64_min_rows   pairs whose shorter side has at least this many bases (and
+ *                  at most 1024) use 64-lane groups, shorter ones 16-lane
+ *                  groups; 33..257 (257 = 16-lane groups up to 256 rows)
+ *   bin_shift      column-length bucket width is 1 << bin_shift bases, 3..7 */
+typedef struct {
+  int32_t blocks_per_cu;
+  int32_t g64_min_rows;
+  int32_t bin_shift;
+} mhmsw_tuning;
+
 /* --- devices (replace adept_sw free functions, src/adept-sw/driver.cpp:28-77) --- */
 int mhmsw_device_count(int *count);
 int mhmsw_device_total_mem(int device_id, size_t *bytes);
@@ -101,6 +115,12 @@ int mhmsw_set_mode(mhmsw_ctx *ctx, int mode);
  * finalize: ~40 launches) is captured once as a hipGraph and replayed while
  * the staged geometry stays the same.  0: launches are issued one by one. */
 int mhmsw_set_graphs(mhmsw_ctx *ctx, int enable);
+int mhmsw_set_tuning(mhmsw_ctx *ctx, const mhmsw_tuning *tuning);
+int mhmsw_get_tuning(const mhmsw_ctx *ctx, mhmsw_tuning *tuning);
+/* Times candidate tunings on the staged batch (coordinate descent over the
+ * three parameters, best of `repeats` jobs each), keeps the fastest and
+ * returns it.  Results do not depend on the tuning. */
+int mhmsw_autotune(mhmsw_ctx *ctx, int repeats, mhmsw_tuning *best);
 
 /* --- batches (replace run_kernel_forwards / run_kernel_backwards,
  *     kernel_is_done / kernel_block and get_aln_results, driver.cpp:270-417) ---

@@ -50,6 +50,13 @@ class Stats(ctypes.Structure):
         return d
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_int32), ("g64_min_rows", ctypes.c_int32), ("bin_shift", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"blocks_per_cu": self.blocks_per_cu, "g64_min_rows": self.g64_min_rows, "bin_shift": self.bin_shift}
+
+
 class ReqTimes(ctypes.Structure):
     _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
                 ("device_ms", ctypes.c_double)]
@@ -76,7 +83,7 @@ class SynthBatch(ctypes.Structure):
 EXPORTS = [
     "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
     "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_graphs",
-    "mhmsw_stage",
+    "mhmsw_set_tuning", "mhmsw_get_tuning", "mhmsw_autotune", "mhmsw_stage",
     "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
     "mhmsw_contigs_load", "mhmsw_contigs_free", "mhmsw_contigs_bytes", "mhmsw_stage_windows",
@@ -108,6 +115,9 @@ def lib():
         "mhmsw_set_orientation": [vp, ctypes.c_int],
         "mhmsw_set_mode": [vp, ctypes.c_int],
         "mhmsw_set_graphs": [vp, ctypes.c_int],
+        "mhmsw_set_tuning": [vp, ctypes.POINTER(Tuning)],
+        "mhmsw_get_tuning": [vp, ctypes.POINTER(Tuning)],
+        "mhmsw_autotune": [vp, ctypes.c_int, ctypes.POINTER(Tuning)],
         "mhmsw_stage": [vp, i64, vp, vp, vp, vp, vp, vp],
         "mhmsw_align_staged": [vp],
         "mhmsw_submit": [vp, i64, vp, vp, vp, vp, vp, vp],

@@ -75,6 +75,21 @@ class Aligner:
     def set_graphs(self, enable):
         self._check(self._L.mhmsw_set_graphs(self._ctx, 1 if enable else 0), "set_graphs")
 
+    def set_tuning(self, blocks_per_cu=8, g64_min_rows=257, bin_shift=5):
+        t = _native.Tuning(int(blocks_per_cu), int(g64_min_rows), int(bin_shift))
+        self._check(self._L.mhmsw_set_tuning(self._ctx, ctypes.byref(t)), "set_tuning")
+
+    def get_tuning(self):
+        t = _native.Tuning()
+        self._check(self._L.mhmsw_get_tuning(self._ctx, ctypes.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def autotune(self, repeats=2):
+        """Pick the fastest scheduling parameters for the staged batch (kept on the context)."""
+        t = _native.Tuning()
+        self._check(self._L.mhmsw_autotune(self._ctx, int(repeats), ctypes.byref(t)), "autotune")
+        return t.as_dict()
+
     def stage(self, batch: PairBatch):
         self._batch = batch  # keep the host buffers alive while referenced
         self._check(self._L.mhmsw_stage(self._ctx, batch.npairs, _ptr(batch.reads), _ptr(batch.read_off),

@@ -80,6 +80,8 @@ struct mhmsw_ctx {
   ScoreParams sp{};
   int mode = MHMSW_MODE_FAST;
   int read_is_ref = 1;
+  mhmsw_tuning tuning{8, 257, 5};
+  std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
   // staged batch
   int64_t npairs = 0;
@@ -117,6 +119,8 @@ struct mhmsw_ctx {
   mhmsw_stats stats{};
   int32_t h_errors[4] = {};
 };
+
+static void reroute(mhmsw_ctx* c);
 
 static int fail(mhmsw_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -278,6 +282,7 @@ int mhmsw_set_scoring(mhmsw_ctx* c, const mhmsw_scoring* s) {
                 worst, worst);
   c->scoring = *s;
   compute_score_params(c);
+  if (c->staged) reroute(c);
   return MHMSW_OK;
 }
 
@@ -286,6 +291,7 @@ int mhmsw_set_orientation(mhmsw_ctx* c, int read_is_ssw_ref) {
   if (c->in_flight) return fail(c, MHMSW_E_STATE, "orientation changed while a batch is in flight");
   c->read_is_ref = read_is_ssw_ref ? 1 : 0;
   compute_score_params(c);
+  if (c->staged) reroute(c);
   return MHMSW_OK;
 }
 
@@ -294,6 +300,7 @@ int mhmsw_set_mode(mhmsw_ctx* c, int mode) {
   if (mode != MHMSW_MODE_FAST && mode != MHMSW_MODE_PLAIN) return fail(c, MHMSW_E_ARG, "unknown mode %d", mode);
   if (c->in_flight) return fail(c, MHMSW_E_STATE, "mode changed while a batch is in flight");
   c->mode = mode;
+  if (c->staged) reroute(c);
   return MHMSW_OK;
 }
 
@@ -320,13 +327,16 @@ struct PackParams {
   int mode = MHMSW_MODE_FAST;
   int fast_ok = 1;
   int read_is_ref = 1;
+  int g64_min_rows = 257;
 };
 
+// host copy of make_task's forward routing (sw_kernels.hip)
 static int route_class(const PackParams& pp, int32_t lref, int32_t lqry) {
   if (lref <= 0 || lqry <= 0) return -1;
   const bool wide = pp.mode == MHMSW_MODE_PLAIN || !pp.fast_ok;
   const int32_t shorter = std::min(lref, lqry), longer = std::max(lref, lqry);
-  if (!wide && shorter <= kFast16MaxRows && longer <= kFastMaxCols) return kClassFast16 + (shorter + 31) / 32 - 1;
+  if (!wide && shorter <= kFast16MaxRows && shorter < pp.g64_min_rows && longer <= kFastMaxCols)
+    return kClassFast16 + (shorter + 31) / 32 - 1;
   if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) return kClassFast64 + (shorter + 127) / 128 - 1;
   return kClassPlain;
 }
@@ -485,7 +495,34 @@ static PackParams pack_params(const mhmsw_ctx* c) {
   pp.mode = c->mode;
   pp.fast_ok = c->sp.fast_ok;
   pp.read_is_ref = c->read_is_ref;
+  pp.g64_min_rows = c->tuning.g64_min_rows;
   return pp;
+}
+
+// Recompute the forward class table of the staged batch (after a change of
+// mode or tuning) from the stored lengths.
+static void reroute(mhmsw_ctx* c) {
+  const PackParams pp = pack_params(c);
+  for (int k = 0; k < kNumClasses; k++) {
+    c->fast_class_used[k] = false;
+    c->class_pairs[k] = 0;
+    c->class_cells[k] = 0;
+  }
+  c->n_fast = c->n_plain = 0;
+  for (int64_t p = 0; p < c->npairs; p++) {
+    const int32_t rl = c->lens[(size_t)(2 * p)], cl = c->lens[(size_t)(2 * p + 1)];
+    const int32_t lref = pp.read_is_ref ? rl : cl, lqry = pp.read_is_ref ? cl : rl;
+    const int cls = route_class(pp, lref, lqry);
+    if (cls < 0) continue;
+    c->fast_class_used[cls] = true;
+    c->class_pairs[cls]++;
+    c->class_cells[cls] += (int64_t)lref * lqry;
+    if (cls == kClassPlain) {
+      c->n_plain++;
+    } else {
+      c->n_fast++;
+    }
+  }
 }
 
 // Copy a packed batch to the context's device buffers; returns when the host
@@ -510,6 +547,11 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
   std::memcpy(c->fast_class_used, pb.used, sizeof(pb.used));
   std::memcpy(c->class_pairs, pb.cpairs, sizeof(pb.cpairs));
   std::memcpy(c->class_cells, pb.ccells, sizeof(pb.ccells));
+  c->lens.resize((size_t)(2 * npairs));
+  for (int64_t p = 0; p < npairs; p++) {
+    c->lens[(size_t)(2 * p)] = pb.len[(size_t)(2 * p)];          // read
+    c->lens[(size_t)(2 * p + 1)] = pb.len[(size_t)(2 * p + 1)];  // contig window
+  }
   const size_t n2 = pb.two.size(), nm = pb.nm.size();
   const size_t np1 = (size_t)std::max<int64_t>(npairs, 1);
   HIP_OR_FAIL(c, c->d_packed2.ensure(n2));
@@ -608,6 +650,8 @@ static BatchView make_view(mhmsw_ctx* c) {
   b.counters = c->d_counters.ptr;
   b.sc = c->sp;
   b.mode = c->mode;
+  b.g64_min_rows = c->tuning.g64_min_rows;
+  b.bin_shift = c->tuning.bin_shift;
   if (c->res) {
     b.res2 = c->res->d2;
     b.resN = c->res->dN;
@@ -624,7 +668,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // persistent grids: enough wavefronts to fill the device, but no more
   // blocks than a class can use (4 wavefronts per 256-thread block, 4 pairs
   // per wavefront in 16-lane groups, 1 in 64-lane groups)
-  const int full = std::max(1, c->cus * 8);
+  const int full = std::max(1, c->cus * c->tuning.blocks_per_cu);
   auto grid_cap = [&](int64_t pairs, int pairs_per_block) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(full, (pairs + pairs_per_block - 1) / pairs_per_block));
   };
@@ -722,7 +766,7 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     unsigned char* kp = key.data();
     std::memcpy(kp, &b, sizeof(BatchView));
     std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
-    const int extra[2] = {c->max_plain_cols, c->mode};
+    const int extra[2] = {c->max_plain_cols, c->mode * 64 + c->tuning.blocks_per_cu};
     std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
     if (!c->graph_exec || key != c->graph_key) {
       drop_graph(c);
@@ -853,6 +897,76 @@ int mhmsw_get_stats(mhmsw_ctx* c, mhmsw_stats* out) {
   return MHMSW_OK;
 }
 
+
+// ----------------------------------------------------------------------------
+// tuning
+// ----------------------------------------------------------------------------
+extern "C" int mhmsw_set_tuning(mhmsw_ctx* c, const mhmsw_tuning* t) {
+  if (!c || !t) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "tuning changed while a batch is in flight");
+  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->g64_min_rows < 33 || t->g64_min_rows > 257 ||
+      t->bin_shift < 3 || t->bin_shift > 7)
+    return fail(c, MHMSW_E_ARG, "tuning out of range (blocks_per_cu 1..16, g64_min_rows 33..257, bin_shift 3..7)");
+  c->tuning = *t;
+  if (c->staged) reroute(c);
+  return MHMSW_OK;
+}
+
+extern "C" int mhmsw_get_tuning(const mhmsw_ctx* c, mhmsw_tuning* t) {
+  if (!c || !t) return MHMSW_E_ARG;
+  *t = c->tuning;
+  return MHMSW_OK;
+}
+
+static int time_job(mhmsw_ctx* c, int repeats, double* best_ms) {
+  double best = 1e30;
+  for (int r = 0; r < repeats; r++) {
+    int rc = mhmsw_align_staged(c);
+    if (rc == MHMSW_OK) rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+    best = std::min(best, c->stats.job_ms);
+  }
+  *best_ms = best;
+  return MHMSW_OK;
+}
+
+extern "C" int mhmsw_autotune(mhmsw_ctx* c, int repeats, mhmsw_tuning* best_out) {
+  if (!c) return MHMSW_E_ARG;
+  if (!c->staged) return fail(c, MHMSW_E_STATE, "autotune needs a staged batch");
+  if (repeats < 1) repeats = 1;
+  const bool graphs = c->use_graphs;
+  c->use_graphs = false;  // per-phase events of eager launches
+  mhmsw_tuning best = c->tuning;
+  double best_ms = 0;
+  int rc = time_job(c, repeats, &best_ms);
+  static const int kBlocks[] = {2, 4, 6, 8, 12};
+  static const int kG64[] = {257, 225, 193, 161, 129};
+  static const int kBins[] = {4, 5, 6};
+  for (int round = 0; rc == MHMSW_OK && round < 3; round++) {
+    const int* vals = round == 0 ? kBlocks : round == 1 ? kG64 : kBins;
+    const int nvals = round == 0 ? 5 : round == 1 ? 5 : 3;
+    for (int i = 0; i < nvals && rc == MHMSW_OK; i++) {
+      mhmsw_tuning t = best;
+      if (round == 0) t.blocks_per_cu = vals[i];
+      if (round == 1) t.g64_min_rows = vals[i];
+      if (round == 2) t.bin_shift = vals[i];
+      if (std::memcmp(&t, &best, sizeof(t)) == 0) continue;
+      rc = mhmsw_set_tuning(c, &t);
+      double ms = 0;
+      if (rc == MHMSW_OK) rc = time_job(c, repeats, &ms);
+      if (rc == MHMSW_OK && ms < best_ms * 0.995) {
+        best = t;
+        best_ms = ms;
+      }
+    }
+  }
+  c->use_graphs = graphs;
+  if (rc != MHMSW_OK) return rc;
+  rc = mhmsw_set_tuning(c, &best);
+  if (rc != MHMSW_OK) return rc;
+  if (best_out) *best_out = best;
+  return MHMSW_OK;
+}
 
 // ----------------------------------------------------------------------------
 // resident contigs

@@ -270,7 +270,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   const int32_t shorter = ref_len <= qry_len ? ref_len : qry_len;
   const int32_t longer = ref_len <= qry_len ? qry_len : ref_len;
   const bool ref_rows_if_short = ref_len <= qry_len;
-  if (!wide && shorter <= kFast16MaxRows && longer <= kFastMaxCols) {
+  if (!wide && shorter <= kFast16MaxRows && shorter < b.g64_min_rows && longer <= kFastMaxCols) {
     const int P = (shorter + 31) / 32;
     cls = kClassFast16 + P - 1;
   } else if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) {
@@ -298,7 +298,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     t.col_start = ref_start;
     t.col_len = ref_len;
   }
-  int bn = t.col_len / 32;
+  int bn = t.col_len >> b.bin_shift;
   bin = bn < kNumBins ? bn : kNumBins - 1;
   return true;
 }

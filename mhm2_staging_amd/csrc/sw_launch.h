@@ -36,6 +36,8 @@ struct BatchView {
   const int64_t* res_base;   // first base of each resident contig
   const int32_t* win_ctg;
   const int32_t* win_off;
+  int32_t g64_min_rows;  // shorter side at or above this: 64-lane groups
+  int32_t bin_shift;     // column bucket width 1 << bin_shift
 };
 
 hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);

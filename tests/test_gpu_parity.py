@@ -220,3 +220,28 @@ def test_graph_replays_match_eager_launches(gpu_aligner):
         assert_same(other, oracle_table(bb, BASELINE_SCORING), bb)
     finally:
         al.set_graphs(True)
+
+
+def test_tunings_give_identical_results_and_autotune(gpu_aligner):
+    """Scheduling parameters change speed only: every tuning matches the oracle."""
+    b = m.generate(2, npairs=4000, seed=41)
+    want = oracle_table(b, BASELINE_SCORING)
+    al = gpu_aligner
+    al.set_scoring(BASELINE_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.stage(b)
+    try:
+        for t in [(2, 257, 4), (8, 129, 6), (12, 193, 5), (1, 33, 3)]:
+            print(f"[gpu] tuning {t}", flush=True)
+            al.set_tuning(*t)
+            al.align_staged()
+            al.wait()
+            assert_same(gpu_table(al.results()), want, b)
+        best = al.autotune(repeats=1)
+        assert 1 <= best["blocks_per_cu"] <= 16
+        al.align_staged()
+        al.wait()
+        assert_same(gpu_table(al.results()), want, b)
+    finally:
+        al.set_tuning()
