@@ -672,7 +672,9 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   auto grid_cap = [&](int64_t pairs, int pairs_per_block) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(full, (pairs + pairs_per_block - 1) / pairs_per_block));
   };
-  const int plain_grid = grid_cap(c->npairs, 1);
+  // the 32-bit form runs one wavefront per 64-thread block; its grid does not
+  // follow the packed form's tuning
+  const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * 16, c->npairs));
   const int plain_cols = std::max(1, c->max_plain_cols);
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
   HIP_OR_FAIL(c, launch_clear_counters(b, st));
