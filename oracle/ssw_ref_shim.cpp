@@ -33,6 +33,29 @@ typedef struct {
 int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
                         const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
                         int nthreads);
+
+/* One call exactly as klign's ssw_align_read makes it (src/klign.cpp:251-273):
+ * query = NUL-terminated contig window, ref = read of ref_len bases.
+ * out[7] = sw_score, sw_score_next_best, ref_begin, ref_end, query_begin,
+ * query_end, mismatches.  Returns 0, or -1 when Align refuses the input. */
+int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc, int32_t *out);
+}
+
+int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc, int32_t *out) {
+  StripedSmithWaterman::Aligner aligner((uint8_t)sc->match, (uint8_t)sc->mismatch, (uint8_t)sc->gap_open,
+                                        (uint8_t)sc->gap_ext, (uint8_t)sc->ambiguity);
+  StripedSmithWaterman::Filter filter(true, false, 0, 32767);
+  StripedSmithWaterman::Alignment aln;
+  aln.Clear();
+  if (!aligner.Align(query, ref, ref_len, filter, &aln, std::max((int)(ref_len / 2), 15))) return -1;
+  out[0] = aln.sw_score;
+  out[1] = aln.sw_score_next_best;
+  out[2] = aln.ref_begin;
+  out[3] = aln.ref_end;
+  out[4] = aln.query_begin;
+  out[5] = aln.query_end;
+  out[6] = aln.mismatches;
+  return 0;
 }
 
 int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
