@@ -165,8 +165,10 @@ int mhmsw_stage_windows(mhmsw_ctx *ctx, const mhmsw_contigs *contigs, int64_t np
  * A server owns `slots` contexts (one HIP stream each) on one device and a
  * dispatcher thread.  Requests are admitted in arrival order as soon as a
  * slot is free and their device-memory estimate fits hbm_budget (bytes,
- * 0 = 80% of the device); admitted requests run concurrently on the device.
- * Results are identical to aligning the request alone. */
+ * 0 = 80% of the device); requests queued together when a slot frees are
+ * merged into one job (up to 2^20 pairs), and jobs on different slots run
+ * concurrently on the device.  Results are identical to aligning the request
+ * alone. */
 typedef struct mhmsw_server mhmsw_server;
 typedef struct {
   double submit_s;  /* host steady clock, seconds */

@@ -564,7 +564,7 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
   HIP_OR_FAIL(c, c->d_fwd.ensure(np1));
   HIP_OR_FAIL(c, c->d_rev.ensure(np1));
   HIP_OR_FAIL(c, c->d_results.ensure(np1));
-  HIP_OR_FAIL(c, c->d_tasks.ensure(np1 + 4 * kNumClasses + 64));
+  HIP_OR_FAIL(c, c->d_tasks.ensure(np1 + 64 * kNumClasses + 64));
   HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr, pb.two.data(), n2 * 4, hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr, pb.nm.data(), nm * 4, hipMemcpyHostToDevice, c->stream));
   if (nseq > 0) {
@@ -727,6 +727,9 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int rev16 = maxp64 > 0 ? kMaxP : maxp16;
       for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->n_fast, 4), st));
       for (int P = rev16; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->n_fast, 16), st));
+      // pairs whose begin lies within a narrow band of diagonals: one lane each
+      HIP_OR_FAIL(c, launch_band(b, 8, grid_cap(c->n_fast, 256), st));
+      HIP_OR_FAIL(c, launch_band(b, 4, grid_cap(c->n_fast, 256), st));
     }
     HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
   }
@@ -837,7 +840,8 @@ static int complete(mhmsw_ctx* c) {
   s.fast_pairs = c->n_fast;
   s.plain_pairs = c->n_plain;
   s.rerun_pairs = c->h_errors[1];
-  for (int k = 0; k < kNumClasses; k++) {
+  static_assert(kNumFwdClasses == MHMSW_NUM_CLASSES, "forward classes are the public ones");
+  for (int k = 0; k < kNumFwdClasses; k++) {
     s.class_pairs[k] = c->class_pairs[k];
     s.class_cells[k] = c->class_cells[k];
     float cm = 0.f;
@@ -1080,6 +1084,67 @@ int64_t estimate_bytes(const Request& r) {
   return r.packed.total * 3 / 2 + r.npairs * 140 + (1 << 20);
 }
 
+// Requests queued together run as one job: their packed batches are
+// concatenated (every part's total is a multiple of 32 bases, so each
+// sequence keeps its 32-base alignment) and the results split afterwards.
+// Pairs are independent, so a request's results do not depend on its company.
+struct Job {
+  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<int64_t> first;  // first pair of each request in the job
+  int64_t npairs = 0;
+  int64_t est = 0;
+  PackedBatch merged;  // used when the job holds more than one request
+  std::vector<mhmsw_result> results;
+};
+
+constexpr int64_t kMergeMaxPairs = (int64_t)1 << 20;
+
+void merge_packed(const std::vector<std::shared_ptr<Request>>& parts, PackedBatch& out) {
+  out = PackedBatch();
+  int64_t np = 0, tot = 0;
+  for (const auto& r : parts) {
+    np += r->packed.npairs;
+    tot += r->packed.total;
+  }
+  out.npairs = np;
+  out.total = tot;
+  out.res = parts[0]->packed.res;
+  out.base.resize((size_t)std::max<int64_t>(2 * np, 1));
+  out.len.resize((size_t)std::max<int64_t>(2 * np, 1));
+  out.two.assign((size_t)(tot / 16), 0u);
+  out.nm.assign((size_t)(tot / 32), 0u);
+  const bool windowed = out.res != nullptr;
+  if (windowed) {
+    out.win_ctg.reserve((size_t)np);
+    out.win_off.reserve((size_t)np);
+  }
+  int64_t sh = 0, ps = 0;
+  for (const auto& r : parts) {
+    const PackedBatch& b = r->packed;
+    for (int64_t k = 0; k < 2 * b.npairs; k++) {
+      out.base[(size_t)(2 * ps + k)] = b.base[(size_t)k] + sh;
+      out.len[(size_t)(2 * ps + k)] = b.len[(size_t)k];
+    }
+    std::copy(b.two.begin(), b.two.end(), out.two.begin() + sh / 16);
+    std::copy(b.nm.begin(), b.nm.end(), out.nm.begin() + sh / 32);
+    if (windowed) {
+      out.win_ctg.insert(out.win_ctg.end(), b.win_ctg.begin(), b.win_ctg.end());
+      out.win_off.insert(out.win_off.end(), b.win_off.begin(), b.win_off.end());
+    }
+    out.cells += b.cells;
+    out.max_plain_cols = std::max(out.max_plain_cols, b.max_plain_cols);
+    for (int k = 0; k < kNumClasses; k++) {
+      out.used[k] = out.used[k] || b.used[k];
+      out.cpairs[k] += b.cpairs[k];
+      out.ccells[k] += b.ccells[k];
+    }
+    out.nfast += b.nfast;
+    out.nplain += b.nplain;
+    sh += b.total;
+    ps += b.npairs;
+  }
+}
+
 }  // namespace
 
 struct mhmsw_server {
@@ -1087,7 +1152,7 @@ struct mhmsw_server {
   PackParams pp;
   const mhmsw_contigs* contigs = nullptr;
   std::vector<mhmsw_ctx*> slots;
-  std::vector<std::shared_ptr<Request>> running;  // per slot
+  std::vector<std::shared_ptr<Job>> running;  // per slot
   int64_t budget = 0, used = 0;
   std::mutex mu;
   std::condition_variable cv_done, cv_work;
@@ -1106,34 +1171,44 @@ static void server_loop(mhmsw_server* s) {
     bool progressed = false;
     // completions
     for (size_t k = 0; k < s->slots.size(); k++) {
-      std::shared_ptr<Request> r = s->running[k];
-      if (!r) continue;
+      std::shared_ptr<Job> j = s->running[k];
+      if (!j) continue;
       lk.unlock();
       int q = mhmsw_query(s->slots[k]);
       int rc = MHMSW_OK;
       mhmsw_stats st{};
       if (q == 1) {
-        r->results.resize((size_t)std::max<int64_t>(r->npairs, 1));
-        rc = mhmsw_results(s->slots[k], r->results.data(), r->npairs);
-        if (rc == MHMSW_OK) mhmsw_get_stats(s->slots[k], &st);
+        j->results.resize((size_t)std::max<int64_t>(j->npairs, 1));
+        rc = mhmsw_results(s->slots[k], j->results.data(), j->npairs);
+        if (rc == MHMSW_OK) {
+          mhmsw_get_stats(s->slots[k], &st);
+          for (size_t i = 0; i < j->reqs.size(); i++) {
+            Request& r = *j->reqs[i];
+            r.results.assign(j->results.begin() + j->first[i], j->results.begin() + j->first[i] + r.npairs);
+          }
+        }
       }
       lk.lock();
       if (q == 0) continue;
       progressed = true;
-      r->times.done_s = steady_now();
-      r->times.device_ms = st.job_ms;
-      if (q < 0 || rc != MHMSW_OK) {
-        r->state = q < 0 ? q : rc;
-        r->err = mhmsw_last_error(s->slots[k]);
-      } else {
-        r->state = 2;
+      const double now = steady_now();
+      for (auto& rp : j->reqs) {
+        Request& r = *rp;
+        r.times.done_s = now;
+        r.times.device_ms = st.job_ms;
+        if (q < 0 || rc != MHMSW_OK) {
+          r.state = q < 0 ? q : rc;
+          r.err = mhmsw_last_error(s->slots[k]);
+        } else {
+          r.state = 2;
+        }
+        r.packed = PackedBatch();  // release the host copy
       }
       s->running[k].reset();
-      s->used -= r->est;
-      r->packed = PackedBatch();  // release the host copy
+      s->used -= j->est;
       s->cv_done.notify_all();
     }
-    // admissions, in arrival order
+    // admissions, in arrival order; requests waiting together share a job
     while (!s->queue.empty()) {
       int freek = -1, busy = 0;
       for (size_t k = 0; k < s->slots.size(); k++) {
@@ -1144,26 +1219,49 @@ static void server_loop(mhmsw_server* s) {
         }
       }
       if (freek < 0) break;
-      std::shared_ptr<Request> r = s->queue.front();
-      if (busy > 0 && s->used + r->est > s->budget) break;  // wait for memory; alone it always runs
-      s->queue.pop_front();
-      r->state = 1;
-      r->slot = freek;
-      r->times.admit_s = steady_now();
-      s->running[(size_t)freek] = r;
-      s->used += r->est;
+      auto j = std::make_shared<Job>();
+      while (!s->queue.empty()) {
+        const std::shared_ptr<Request>& r = s->queue.front();
+        const bool first = j->reqs.empty();
+        if (!first && j->npairs + r->npairs > kMergeMaxPairs) break;
+        // wait for memory; a request alone on an idle device always runs
+        if (s->used + j->est + r->est > s->budget && !(first && busy == 0)) break;
+        j->first.push_back(j->npairs);
+        j->npairs += r->npairs;
+        j->est += r->est;
+        j->reqs.push_back(r);
+        s->queue.pop_front();
+      }
+      if (j->reqs.empty()) break;
+      const double now = steady_now();
+      for (auto& r : j->reqs) {
+        r->state = 1;
+        r->slot = freek;
+        r->times.admit_s = now;
+      }
+      s->running[(size_t)freek] = j;
+      s->used += j->est;
       lk.unlock();
       mhmsw_ctx* c = s->slots[(size_t)freek];
-      int rc = upload_batch(c, r->packed);
+      const PackedBatch* pb = &j->reqs[0]->packed;
+      if (j->reqs.size() > 1) {
+        merge_packed(j->reqs, j->merged);
+        pb = &j->merged;
+      }
+      int rc = upload_batch(c, *pb);
+      j->merged = PackedBatch();
       if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
       lk.lock();
       progressed = true;
       if (rc != MHMSW_OK) {
-        r->state = rc;
-        r->err = mhmsw_last_error(c);
-        r->times.done_s = steady_now();
+        const double t = steady_now();
+        for (auto& r : j->reqs) {
+          r->state = rc;
+          r->err = mhmsw_last_error(c);
+          r->times.done_s = t;
+        }
         s->running[(size_t)freek].reset();
-        s->used -= r->est;
+        s->used -= j->est;
         s->cv_done.notify_all();
       }
     }

@@ -20,7 +20,12 @@ constexpr int kMaxP = 8;                 // packed registers per lane (2 rows ea
 constexpr int kClassFast16 = 0;          // + (P-1): 16-lane groups, rows <= 32*P
 constexpr int kClassFast64 = kMaxP;      // + (P-1): 64-lane groups, rows <= 128*P
 constexpr int kClassPlain = 2 * kMaxP;   // 32-bit, one wavefront per pair
-constexpr int kNumClasses = 2 * kMaxP + 1;
+constexpr int kNumFwdClasses = 2 * kMaxP + 1;  // classes a forward pass uses
+// Reverse pass only: banded form, one lane per pair, diagonals |l - k| <= 7
+// (kClassBand, 4 packed registers) or <= 15 (kClassBand + 1, 8 registers).
+constexpr int kClassBand = kNumFwdClasses;
+constexpr int kBandMaxD = 15;
+constexpr int kNumClasses = kNumFwdClasses + 2;
 constexpr int kNumBins = 128;            // column-length bins of 32 columns
 constexpr int kNumBuckets = kNumClasses * kNumBins;
 
@@ -32,6 +37,7 @@ constexpr int kPlainMaxCols = 8192;          // LDS boundary row of the 32-bit k
 // Task flags.
 constexpr int32_t kTaskRowsAreRef = 1;  // rows hold SSW-ref (outer) positions
 constexpr int32_t kTaskReverse = 2;     // result goes to the reverse (begin) table
+constexpr int kTaskBandShift = 8;       // band tasks: flags >> kTaskBandShift = D
 
 struct SwTask {
   int32_t pair;       // -1: empty slot

@@ -253,10 +253,15 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     // side plus D cannot reach the score and are dropped (exact: DP values
     // inside the kept rectangle do not depend on the dropped cells).  The
     // plain form keeps the reference's full prefix rectangle (ssw_core.cpp:1167-1173).
+    // When D is small the band |k-l| <= D itself is all that is computed
+    // (sw_band_kernel): cells outside it never carry the score, and values
+    // inside it computed without them are lower bounds that are exact on
+    // every path reaching the score.
+    int64_t D = -1;
     if (b.mode != 1) {
       const int32_t L = ref_len < qry_len ? ref_len : qry_len;
       const int64_t slack = (int64_t)b.sc.match * L - fe.score;
-      int64_t D = 0;
+      D = 0;
       if (slack >= b.sc.gap_open) D = 1 + (slack - b.sc.gap_open) / b.sc.gap_ext;
       const int64_t rcap = (int64_t)qry_len + D, qcap = (int64_t)ref_len + D;
       if (ref_len > rcap) ref_len = (int32_t)rcap;
@@ -264,6 +269,19 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     }
     ref_start = b.total + b.seq_base[sref] + (lref - 1 - fe.i);
     qry_start = b.total + b.seq_base[sqry] + (lqry - 1 - fe.j);
+    if (!wide && D >= 0 && D <= kBandMaxD && ref_len + D < 0xFFF0) {
+      // rows = reversed SSW-ref, columns = reversed SSW-query, one lane per pair
+      t.pair = (int32_t)p;
+      t.flags = kTaskReverse | kTaskRowsAreRef | ((int32_t)D << kTaskBandShift);
+      t.row_start = ref_start;
+      t.row_len = ref_len;
+      t.col_start = qry_start;
+      t.col_len = qry_len;
+      cls = D <= 7 ? kClassBand : kClassBand + 1;
+      const int bn = (ref_len + (int32_t)D) >> b.bin_shift;  // the lane's loop runs K + D steps
+      bin = bn < kNumBins ? bn : kNumBins - 1;
+      return true;
+    }
   }
   t.pair = (int32_t)p;
   t.flags = pass == kPassRev ? kTaskReverse : 0;
@@ -343,7 +361,15 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
     bk[k] = cls * kNumBins + bin;
     lr[k] = atomicAdd(&hist[bk[k]], 1);
     if (pass == kPassRerun) atomicAdd(&b.errors[1], 1);
-    if (pass == kPassRev) atomicAdd(&b.counters[0], (unsigned long long)t.row_len * (unsigned long long)t.col_len);
+    if (pass == kPassRev) {
+      // cells computed: the whole rectangle, or for a band task its 2D+1 diagonals
+      int64_t w = t.col_len;
+      if (cls >= kClassBand) {
+        const int64_t bw = 2 * (int64_t)(t.flags >> kTaskBandShift) + 1;
+        w = w < bw ? w : bw;
+      }
+      atomicAdd(&b.counters[0], (unsigned long long)t.row_len * (unsigned long long)w);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) {
@@ -371,7 +397,8 @@ __global__ void schedule_kernel(BatchView b) {
     for (int bn = kNumBins - 1; bn >= 0; bn--) tot += s->bucket_count[c * kNumBins + bn];
     s->class_count[c] = tot;
     int32_t padded = tot;
-    if (c < kClassFast64) padded = (tot + 3) & ~3;  // 4 groups per wavefront
+    if (c < kClassFast64) padded = (tot + 3) & ~3;        // 4 groups per wavefront
+    if (c >= kClassBand) padded = (tot + 63) & ~63;       // one pair per lane
     slots[c] = padded;
     s->class_slots[c] = padded;
     s->class_next[c] = 0;
@@ -653,6 +680,220 @@ __global__ __launch_bounds__(256) void sw_fast_kernel(BatchView b) {
 }
 
 // ----------------------------------------------------------------------------
+// banded reverse kernel: one lane per pair
+// ----------------------------------------------------------------------------
+// The reverse rectangle's rows k index the reversed SSW-ref prefix, its
+// columns l the reversed SSW-query prefix, and only diagonals c = l - k + D in
+// [0, 2D] can hold the score.  A lane keeps 2P slots (P packed registers; low
+// halves are slots 0..P-1, high halves P..2P-1).  At u-step u, slot j works on
+// row k = u - j, first (even step) on diagonal c = 2j, then (odd step) on
+// c = 2j + 1, so 4P diagonals are covered.  Every input of a cell was produced
+// one step earlier:
+//   even step, c = 2j:   H(k-1,l-1) own slot, last even step
+//                        E from (k-1,l) = c+1: own slot, last odd step
+//                        F from (k,l-1) = c-1: slot j-1, last odd step
+//   odd step, c = 2j+1:  H(k-1,l-1) own slot, last odd step
+//                        E from (k-1,l) = c+1: slot j+1, this u's even step
+//                        F from (k,l-1) = c-1: own slot, this u's even step
+// so a u-step has no chain across slots and no cross-lane traffic.  Row codes
+// move up one slot per u-step (slot j needs R'[u-j]) and column codes down one
+// (slot j needs Q'[u+j-D] and Q'[u+j+1-D]); both are register rings indexed
+// with the step number, so a move costs one insertion per u-step.
+// Cells off the sequences are killed like the packed kernel's: left of column
+// 0 and above row 0 they hold the boundary (H = E = F = 0); right of the last
+// column and below the last row they only feed each other and stay below the
+// score.  The answer is the least (k, l) with H = score, which SSW's reverse
+// pass selects (ssw_core.cpp:1162-1178).
+template <int P>
+struct BandState {
+  uint32_t rr[P];          // row codes (selector form), slot p at rr[(p - s) mod P]
+  uint32_t qr[P];          // even-step column codes, slot p at qr[(p + s) mod P]
+  uint32_t hoe[P], hoo[P]; // 16*(H - gO) of the slot's last even / odd cell
+  uint32_t eo[P], fo[P];   // 16*E, 16*F leaving the last odd step (clamped >= 0)
+  uint32_t be[P], bo[P];   // max of 16*H + tag over this chunk's even / odd steps
+  uint32_t rcur, rnext, qcur, qnext;
+};
+
+__device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& ho_slot, uint32_t ein, uint32_t fin,
+                                          uint32_t& eout, uint32_t& fout, uint32_t& best, uint32_t tag, uint32_t tlo,
+                                          uint32_t thi, uint32_t go, uint32_t ge) {
+  const uint32_t sel = subst_sel(r, q);
+  const uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
+  const uint32_t d = pk_adds(ho_slot, sp);
+  uint32_t h = pk_max(d, ein);
+  h = pk_max(h, fin);
+  const uint32_t ho = pk_sub(h, go);
+  ho_slot = ho;
+  eout = pk_max(pk_subu_sat(ein, ge), ho);
+  fout = pk_max(pk_subu_sat(fin, ge), ho);
+  best = pk_maxu(best, h | tag);
+}
+
+// One chunk of 8 u-steps (16 cell steps).  Row codes R'[u0+1 .. u0+8] and
+// column codes Q'[u0+2P-D .. +7] come from one dword of each stream.
+template <int P, bool EDGE>
+__device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int L, int D, const uint32_t* __restrict__ codes,
+                                           int64_t& rdw, int64_t rlast, uint32_t rsh, int64_t& qdw, int64_t qlast,
+                                           uint32_t qsh, uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
+  const uint32_t rw = __builtin_amdgcn_alignbit(st.rnext, st.rcur, rsh);
+  const uint32_t qw = __builtin_amdgcn_alignbit(st.qnext, st.qcur, qsh);
+  st.rcur = st.rnext;
+  st.qcur = st.qnext;
+  rdw++;
+  qdw++;
+  st.rnext = codes[rdw < rlast ? rdw : rlast];
+  st.qnext = codes[qdw < qlast ? qdw : qlast];
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    st.be[p] = 0;
+    st.bo[p] = 0;
+  }
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const uint32_t tag = (uint32_t)(7 - s) * 0x00010001u;
+    uint32_t ee[P], fe[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      const uint32_t fin = p ? st.fo[p - 1] : (st.fo[P - 1] << 16);
+      band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s) % P], st.hoe[p], st.eo[p], fin, ee[p], fe[p], st.be[p], tag,
+                tlo, thi, go, ge);
+    }
+    // column code entering the top slot for the odd step
+    uint32_t qn = (qw >> (4 * s)) & 0xFu;
+    if constexpr (EDGE) qn = (u0 + s + 2 * P - D < L) ? qn : kKillCode;
+    st.qr[s % P] = __builtin_amdgcn_alignbit(qn, st.qr[s % P], 16);
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      const uint32_t ein = p < P - 1 ? ee[p + 1] : (ee[0] >> 16);
+      band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s + 1) % P], st.hoo[p], ein, fe[p], st.eo[p], st.fo[p],
+                st.bo[p], tag, tlo, thi, go, ge);
+    }
+    // row code entering slot 0 for the next u-step
+    uint32_t rn = ((rw >> (4 * s)) & 0xFu) | kRowHi;
+    if constexpr (EDGE) rn = (u0 + s + 1 < K) ? rn : kKillRow;
+    const int ri = (P - 1 - s + 8 * P) % P;
+    st.rr[ri] = (st.rr[ri] << 16) | rn;
+  }
+}
+
+// Least (k, l) among this chunk's cells holding the score (s16 = 16 * score).
+template <int P>
+__device__ __forceinline__ void band_hits(const BandState<P>& st, int u0, int K, int L, int D, uint32_t s16,
+                                          uint32_t& keymin) {
+#pragma unroll
+  for (int par = 0; par < 2; par++) {
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        const uint32_t v = ((par ? st.bo[p] : st.be[p]) >> (16 * half)) & 0xFFFFu;
+        if ((v & 0xFFF0u) == s16) {
+          const int j = half * P + p;
+          const int u = u0 + 7 - (int)(v & 0xFu);
+          const int k = u - j;
+          const int l = k + 2 * j + par - D;
+          if (k >= 0 && k < K && l >= 0 && l < L) {
+            const uint32_t key = ((uint32_t)k << 16) | (uint32_t)l;
+            keymin = key < keymin ? key : keymin;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int P>
+__device__ void band_pair(const BatchView& b, const SwTask& task) {
+  const bool active = task.pair >= 0;
+  const int K = active ? task.row_len : 0;
+  const int L = active ? task.col_len : 0;
+  const int D = active ? (task.flags >> kTaskBandShift) : 0;
+  const int score = active ? b.fwd[task.pair].score : 0;
+  // wave-uniform loop length and the chunks that need no end-of-sequence checks
+  const int nU = uni(wave_max_i32(active ? K + D : 0));
+  if (nU <= 0) return;
+  const int kmin = uni(wave_min_i32(active ? K : 0x7FFFFFFF));
+  const int qmin = uni(wave_min_i32(active ? L - 2 * P + D : 0x7FFFFFFF));
+
+  const uint32_t* __restrict__ codes = b.codes;
+  const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
+  const uint32_t go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
+  const uint32_t ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
+  const uint32_t neg = (uint32_t)(uint16_t)(-16 * b.sc.gap_open) * 0x00010001u;
+  const uint32_t s16 = (uint32_t)(16 * score);
+
+  BandState<P> st;
+  // u = 0: slot j holds R'[-j] (only slot 0 is on the sequence) and
+  // Q'[j - D] for the even step
+  const uint32_t r0 = K > 0 ? (code_at(codes, task.row_start) | kRowHi) : kKillRow;
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    st.rr[p] = p == 0 ? (r0 | (kKillRow << 16)) : (kKillRow | (kKillRow << 16));
+    const int ilo = p - D, ihi = P + p - D;
+    const uint32_t qlo = (ilo >= 0 && ilo < L) ? code_at(codes, task.col_start + ilo) : kKillCode;
+    const uint32_t qhi = (ihi >= 0 && ihi < L) ? code_at(codes, task.col_start + ihi) : kKillCode;
+    st.qr[p] = qlo | (qhi << 16);
+    st.hoe[p] = neg;
+    st.hoo[p] = neg;
+    st.eo[p] = 0;
+    st.fo[p] = 0;
+  }
+  // code streams: rows from R'[1], columns from Q'[2P - D] (2P - D >= 1)
+  const int64_t ra = task.row_start + 1, qa = task.col_start + 2 * P - D;
+  int64_t rdw = ra >> 3, qdw = qa >> 3;
+  const uint32_t rsh = (uint32_t)(ra & 7) * 4u, qsh = (uint32_t)(qa & 7) * 4u;
+  // stream reads stay inside the lane's own sequence (plus one dword)
+  const int64_t rlast = ((task.row_start + K) >> 3) + 1, qlast = ((task.col_start + L) >> 3) + 1;
+  st.rcur = codes[rdw < rlast ? rdw : rlast];
+  st.rnext = codes[rdw + 1 < rlast ? rdw + 1 : rlast];
+  st.qcur = codes[qdw < qlast ? qdw : qlast];
+  st.qnext = codes[qdw + 1 < qlast ? qdw + 1 : qlast];
+  rdw++;
+  qdw++;
+
+  uint32_t keymin = 0xFFFFFFFFu;
+  for (int u0 = 0; u0 < nU; u0 += 8) {
+    const bool middle = u0 + 8 < kmin && u0 + 8 <= qmin;
+    if (middle) {
+      band_chunk<P, false>(st, u0, K, L, D, codes, rdw, rlast, rsh, qdw, qlast, qsh, tlo, thi, go, ge);
+    } else {
+      band_chunk<P, true>(st, u0, K, L, D, codes, rdw, rlast, rsh, qdw, qlast, qsh, tlo, thi, go, ge);
+    }
+    // scores never exceed the forward score: a hit is a chunk maximum equal to it
+    uint32_t m = 0;
+#pragma unroll
+    for (int p = 0; p < P; p++) m = pk_maxu(m, pk_maxu(st.be[p], st.bo[p]));
+    const uint32_t mh = (m & 0xFFFFu) > (m >> 16) ? (m & 0xFFFFu) : (m >> 16);
+    if (active && score > 0 && (mh & 0xFFF0u) == s16) band_hits<P>(st, u0, K, L, D, s16, keymin);
+  }
+  if (active) {
+    PairEnd e{0, 0, 0, 0};
+    if (keymin != 0xFFFFFFFFu) e = PairEnd{score, (int32_t)(keymin >> 16), (int32_t)(keymin & 0xFFFFu), 0};
+    b.rev[task.pair] = e;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
+  const int cls = kClassBand + (P == 4 ? 0 : 1);
+  const int lane = threadIdx.x & 63;
+  PassSched* s = b.sched;
+  const int32_t nwt = s->class_slots[cls] / 64;
+  const int32_t base = s->class_base[cls];
+  int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+  SwTask task;
+  if (w < nwt) task = b.tasks[base + w * 64 + lane];
+  while (w < nwt) {
+    const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+    SwTask next;
+    if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
+    band_pair<P>(b, task);
+    task = next;
+    w = wn;
+  }
+}
+
+// ----------------------------------------------------------------------------
 // plain 32-bit kernel: one wavefront per pair, rows in tiles of 64
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void sw_plain_kernel(BatchView b) {
@@ -845,6 +1086,17 @@ hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream
     }
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st) {
+  if (P == 4) {
+    hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b);
+  } else if (P == 8) {
+    hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st) {

@@ -44,6 +44,8 @@ hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
 hipError_t launch_unpack(const BatchView& b, hipStream_t st);  // also extracts resident windows
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
+// reverse pass, banded form: P = 4 (|l - k| <= 7) or 8 (<= 15)
+hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
 hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu);

@@ -82,3 +82,34 @@ def test_server_hbm_budget_admission_in_order(serving_set):
     for r, (o, _) in zip(reqs[:2], outs[:2]):
         sb = r.as_strings(cs)
         assert_same(gpu_table(o), oracle_table(sb, BASELINE_SCORING), sb)
+
+
+def test_server_merges_queued_requests(serving_set):
+    """Requests that queue while a job runs share the next job; each one's
+    results equal the request aligned alone, and are SSW's."""
+    cs, res = serving_set
+    sizes = [20000] + [300 + 97 * i for i in range(15)]
+    reqs = [m.window_pairs(cs, n, seed=100 + i) for i, n in enumerate(sizes)]
+    al = m.Aligner(device=0)
+    al.set_scoring(BASELINE_SCORING)
+    alone = []
+    for r in reqs:
+        al.stage_windows(res, r)
+        al.align_staged()
+        al.wait()
+        alone.append(al.results())
+    al.close()
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    ids = [srv.submit(r) for r in reqs]
+    outs = [srv.results(i) for i in ids]
+    srv.close()
+    admits = [t["admit_s"] for _, t in outs]
+    assert admits == sorted(admits)
+    shared = sum(1 for a, b in zip(admits, admits[1:]) if a == b)
+    print(f"[gpu] {shared} requests admitted together with their predecessor", flush=True)
+    assert shared > 0, "no queued requests were merged"
+    for a, (b, _) in zip(alone, outs):
+        assert (gpu_table(a) == gpu_table(b)).all()
+    for r, (o, _) in list(zip(reqs, outs))[1:4]:
+        sb = r.as_strings(cs)
+        assert_same(gpu_table(o), oracle_table(sb, BASELINE_SCORING), sb)
