@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
     ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
+    ap.add_argument("--tuning", type=str, default="", metavar="B,G,S",
+                    help="fixed scheduling parameters (blocks_per_cu, g64_min_rows, bin_shift) instead of autotuning")
     ap.add_argument("--serving-qps", type=str, default="50,400", help="comma-separated request rates")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -156,7 +158,11 @@ def main():
     al.stage(batch)
     log(f"[bench] staged (pack + H2D) in {time.perf_counter() - t0:.2f}s")
     tuning = al.get_tuning()
-    if not args.no_autotune:
+    if args.tuning:
+        b_, g_, s_ = (int(x) for x in args.tuning.split(","))
+        al.set_tuning(blocks_per_cu=b_, g64_min_rows=g_, bin_shift=s_)
+        tuning = al.get_tuning()
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tuning = al.autotune(repeats=2)
         log(f"[bench] autotuned {tuning} in {time.perf_counter() - t0:.1f}s")
