@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -21,6 +22,7 @@
 #include "../../include/mhmsw.h"
 #include "sw_common.h"
 #include "sw_launch.h"
+#include "pack2bit.h"
 
 using namespace mhmsw;
 
@@ -47,15 +49,6 @@ struct DevBuf {
   }
 };
 
-inline uint32_t base_code(unsigned char c) {
-  switch (c) {
-    case 'A': case 'a': case 'U': case 'u': return 0;
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return 4;
-  }
-}
 
 constexpr int kEvents = 6;
 
@@ -341,6 +334,28 @@ static int route_class(const PackParams& pp, int32_t lref, int32_t lqry) {
   return kClassPlain;
 }
 
+// Allocator whose resize() leaves elements uninitialized: the packed code
+// buffers are filled in parallel by the packing threads (each writes, and so
+// first touches, its own sequences' dwords) instead of zeroed serially.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = UninitAlloc<U>;
+  };
+  UninitAlloc() = default;
+  template <class U>
+  UninitAlloc(const UninitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+
 // A batch laid out and packed on the host (2-bit codes + N bitmap), ready to
 // be copied to a context.  Packing is pure CPU work, so the serving loop does
 // it on the submitting thread.
@@ -352,53 +367,17 @@ struct PackedBatch {
   int64_t nfast = 0, nplain = 0;
   std::vector<int64_t> base;
   std::vector<int32_t> len;
-  std::vector<uint32_t> two, nm;
+  std::vector<uint32_t, UninitAlloc<uint32_t>> two, nm;  // every dword is written by pack_batch
   std::vector<int32_t> win_ctg, win_off;
   const mhmsw_contigs* res = nullptr;
   std::string err;
 };
 
-namespace {
-struct CodeTables {
-  uint8_t two[256];
-  uint8_t isn[256];
-  CodeTables() {
-    for (int c = 0; c < 256; c++) {
-      const uint32_t k = base_code((unsigned char)c);
-      two[c] = (uint8_t)(k == 4 ? 0 : k);
-      isn[c] = (uint8_t)(k == 4 ? 1 : 0);
-    }
-  }
-};
-const CodeTables kCodes;
-
-// one sequence into whole dwords (its base is a multiple of 32)
-void pack_seq(const char* src, int32_t len, uint32_t* two, uint32_t* nb) {
-  const unsigned char* u = (const unsigned char*)src;
-  for (int32_t x0 = 0; x0 < len; x0 += 32) {
-    uint32_t w0 = 0, w1 = 0, n = 0;
-    const int32_t m = std::min<int32_t>(32, len - x0);
-    for (int32_t k = 0; k < m; k++) {
-      const unsigned char ch = u[x0 + k];
-      const uint32_t code = kCodes.two[ch];
-      if (k < 16) {
-        w0 |= code << (2 * k);
-      } else {
-        w1 |= code << (2 * (k - 16));
-      }
-      n |= (uint32_t)kCodes.isn[ch] << k;
-    }
-    two[x0 >> 4] = w0;
-    if (m > 16) two[(x0 >> 4) + 1] = w1;
-    nb[x0 >> 5] = n;
-  }
-}
-}  // namespace
+using pack2::pack_seq;
 
 static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& pb, int max_threads) {
   const int64_t npairs = in.npairs;
   const bool windowed = in.res != nullptr;
-  char msg[256];
   if (npairs < 0 || (npairs > 0 && (!in.reads || !in.read_off || !in.read_len)) ||
       (npairs > 0 && !windowed && (!in.ctgs || !in.ctg_off || !in.ctg_len)) ||
       (npairs > 0 && windowed && (!in.ctg_id || !in.win_off || !in.win_len))) {
@@ -415,74 +394,135 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   pb.res = in.res;
   pb.base.resize((size_t)std::max<int64_t>(nseq, 1));
   pb.len.resize((size_t)std::max<int64_t>(nseq, 1));
-  int64_t acc = kPadBases;
-  for (int64_t p = 0; p < npairs; p++) {
-    const int32_t rl = in.read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
-    if (rl < 0 || cl < 0) {
-      snprintf(msg, sizeof(msg), "negative length at pair %lld", (long long)p);
-      pb.err = msg;
-      return MHMSW_E_ARG;
+  int nth = std::max(1, std::min(max_threads, (int)std::max(1u, std::thread::hardware_concurrency())));
+  if (npairs < 4096) nth = 1;
+  auto run = [&](auto&& fn) {
+    if (nth == 1) {
+      fn(0);
+      return;
     }
-    if (windowed) {
-      const int32_t id = in.ctg_id[p];
-      if (id < 0 || id >= in.res->n || in.win_off[p] < 0 || (int64_t)in.win_off[p] + cl > in.res->len[(size_t)id]) {
-        snprintf(msg, sizeof(msg), "pair %lld: window outside its resident contig", (long long)p);
-        pb.err = msg;
-        return MHMSW_E_ARG;
-      }
-    }
-    if (std::min(rl, cl) > kPlainMaxCols) {
-      snprintf(msg, sizeof(msg), "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl, cl);
-      pb.err = msg;
-      return MHMSW_E_ARG;
-    }
-    if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20)) {
-      snprintf(msg, sizeof(msg), "pair %lld: sequence longer than 2^20 bases", (long long)p);
-      pb.err = msg;
-      return MHMSW_E_ARG;
-    }
-    pb.len[(size_t)(2 * p)] = rl;
-    pb.len[(size_t)(2 * p + 1)] = cl;
-    pb.base[(size_t)(2 * p)] = acc;
-    acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-    pb.base[(size_t)(2 * p + 1)] = acc;
-    acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-    pb.cells += (int64_t)rl * cl;
-    pb.max_plain_cols = std::max(pb.max_plain_cols, (int)std::min(rl, cl));
-    const int32_t lref = pp.read_is_ref ? rl : cl, lqry = pp.read_is_ref ? cl : rl;
-    const int cls = route_class(pp, lref, lqry);
-    if (cls >= 0) {
-      pb.used[cls] = true;
-      pb.cpairs[cls]++;
-      pb.ccells[cls] += (int64_t)rl * cl;
-      if (cls == kClassPlain) {
-        pb.nplain++;
-      } else {
-        pb.nfast++;
-      }
-    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++) th.emplace_back(fn, t);
+    for (auto& t : th) t.join();
+  };
+  // pass 1, over contiguous chunks of pairs: validate, lay out each chunk from
+  // offset 0, route, and sum
+  struct Chunk {
+    int64_t p0 = 0, p1 = 0, bases = 0, cells = 0, nfast = 0, nplain = 0;
+    int max_plain_cols = 0;
+    bool used[kNumClasses] = {};
+    int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
+    int64_t bad = -1;  // first invalid pair of the chunk
+    std::string err;
+  };
+  const int nchunks = nth == 1 ? 1 : 4 * nth;
+  std::vector<Chunk> ch((size_t)nchunks);
+  for (int k = 0; k < nchunks; k++) {
+    ch[(size_t)k].p0 = npairs * k / nchunks;
+    ch[(size_t)k].p1 = npairs * (k + 1) / nchunks;
   }
+  std::atomic<int> next_chunk(0);
+  run([&](int) {
+    char m[256];
+    for (int k; (k = next_chunk.fetch_add(1)) < nchunks;) {
+      Chunk& c = ch[(size_t)k];
+      int64_t acc = 0;
+      for (int64_t p = c.p0; p < c.p1; p++) {
+        const int32_t rl = in.read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
+        m[0] = 0;
+        if (rl < 0 || cl < 0) {
+          snprintf(m, sizeof(m), "negative length at pair %lld", (long long)p);
+        } else if (windowed && (in.ctg_id[p] < 0 || in.ctg_id[p] >= in.res->n || in.win_off[p] < 0 ||
+                                (int64_t)in.win_off[p] + cl > in.res->len[(size_t)in.ctg_id[p]])) {
+          snprintf(m, sizeof(m), "pair %lld: window outside its resident contig", (long long)p);
+        } else if (std::min(rl, cl) > kPlainMaxCols) {
+          snprintf(m, sizeof(m), "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl,
+                   cl);
+        } else if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20)) {
+          snprintf(m, sizeof(m), "pair %lld: sequence longer than 2^20 bases", (long long)p);
+        }
+        if (m[0]) {
+          c.bad = p;
+          c.err = m;
+          break;
+        }
+        pb.len[(size_t)(2 * p)] = rl;
+        pb.len[(size_t)(2 * p + 1)] = cl;
+        pb.base[(size_t)(2 * p)] = acc;
+        acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        pb.base[(size_t)(2 * p + 1)] = acc;
+        acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        c.cells += (int64_t)rl * cl;
+        c.max_plain_cols = std::max(c.max_plain_cols, (int)std::min(rl, cl));
+        const int32_t lref = pp.read_is_ref ? rl : cl, lqry = pp.read_is_ref ? cl : rl;
+        const int cls = route_class(pp, lref, lqry);
+        if (cls >= 0) {
+          c.used[cls] = true;
+          c.cpairs[cls]++;
+          c.ccells[cls] += (int64_t)rl * cl;
+          if (cls == kClassPlain) {
+            c.nplain++;
+          } else {
+            c.nfast++;
+          }
+        }
+      }
+      c.bases = acc;
+    }
+  });
+  // the first invalid pair fails the batch; otherwise chunk offsets and totals
+  int64_t acc = kPadBases;
+  std::vector<int64_t> chunk_base((size_t)nchunks);
+  for (int k = 0; k < nchunks; k++) {
+    const Chunk& c = ch[(size_t)k];
+    if (c.bad >= 0) {
+      pb.err = c.err;
+      return MHMSW_E_ARG;
+    }
+    chunk_base[(size_t)k] = acc;
+    acc += c.bases;
+    pb.cells += c.cells;
+    pb.max_plain_cols = std::max(pb.max_plain_cols, c.max_plain_cols);
+    for (int q = 0; q < kNumClasses; q++) {
+      pb.used[q] = pb.used[q] || c.used[q];
+      pb.cpairs[q] += c.cpairs[q];
+      pb.ccells[q] += c.ccells[q];
+    }
+    pb.nfast += c.nfast;
+    pb.nplain += c.nplain;
+  }
+  const int64_t body_end = acc;
   acc += kPadBases;
   pb.total = (acc + 31) / 32 * 32;
-  pb.two.assign((size_t)(pb.total / 16), 0u);
-  pb.nm.assign((size_t)(pb.total / 32), 0u);
-  int nth = std::max(1, std::min(max_threads, (int)std::max(1u, std::thread::hardware_concurrency())));
-  if (nseq < 8192) nth = 1;
-  auto work = [&](int tid) {
-    for (int64_t s = tid; s < nseq; s += nth) {
-      if ((s & 1) && windowed) continue;  // extracted on the device
-      const char* src = (s & 1) ? in.ctgs + in.ctg_off[s >> 1] : in.reads + in.read_off[s >> 1];
-      const int64_t b0 = pb.base[(size_t)s];
-      pack_seq(src, pb.len[(size_t)s], pb.two.data() + b0 / 16, pb.nm.data() + b0 / 32);
+  pb.two.resize((size_t)(pb.total / 16));
+  pb.nm.resize((size_t)(pb.total / 32));
+  // pads before the first and after the last sequence
+  std::fill(pb.two.begin(), pb.two.begin() + kPadBases / 16, 0u);
+  std::fill(pb.nm.begin(), pb.nm.begin() + kPadBases / 32, 0u);
+  std::fill(pb.two.begin() + body_end / 16, pb.two.end(), 0u);
+  std::fill(pb.nm.begin() + body_end / 32, pb.nm.end(), 0u);
+  // pass 2, per chunk: final offsets, then zero each sequence's 32-base-rounded
+  // span and pack it (window spans of a windowed batch are only zeroed: the
+  // device extracts them from the resident contigs)
+  next_chunk = 0;
+  run([&](int) {
+    for (int k; (k = next_chunk.fetch_add(1)) < nchunks;) {
+      const Chunk& c = ch[(size_t)k];
+      const int64_t off = chunk_base[(size_t)k];
+      for (int64_t s = 2 * c.p0; s < 2 * c.p1; s++) {
+        const int64_t b0 = (pb.base[(size_t)s] += off);
+        const int32_t len = pb.len[(size_t)s];
+        const int64_t span = ((int64_t)len + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        uint32_t* two = pb.two.data() + b0 / 16;
+        uint32_t* nb = pb.nm.data() + b0 / 32;
+        std::fill(two, two + span / 16, 0u);
+        std::fill(nb, nb + span / 32, 0u);
+        if ((s & 1) && windowed) continue;
+        const char* src = (s & 1) ? in.ctgs + in.ctg_off[s >> 1] : in.reads + in.read_off[s >> 1];
+        pack_seq(src, len, two, nb);
+      }
     }
-  };
-  if (nth == 1) {
-    work(0);
-  } else {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; t++) th.emplace_back(work, t);
-    for (auto& t : th) t.join();
-  }
+  });
   if (windowed) {
     pb.win_ctg.assign(in.ctg_id, in.ctg_id + npairs);
     pb.win_off.assign(in.win_off, in.win_off + npairs);
@@ -1007,17 +1047,7 @@ extern "C" int mhmsw_contigs_load(int device_id, int64_t nctgs, const char* seqs
     int nth = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
     auto work = [&](int tid) {
       for (int64_t i = tid; i < nctgs; i += nth) {
-        const char* src = seqs + off[i];
-        uint32_t* two = h2.data() + base[(size_t)i] / 16;
-        uint32_t* nb = hN.data() + base[(size_t)i] / 32;
-        for (int32_t x = 0; x < len[i]; x++) {
-          const uint32_t code = base_code((unsigned char)src[x]);
-          if (code == 4u) {
-            nb[x >> 5] |= 1u << (x & 31);
-          } else {
-            two[x >> 4] |= code << (2 * (x & 15));
-          }
-        }
+        pack_seq(seqs + off[i], len[i], h2.data() + base[(size_t)i] / 16, hN.data() + base[(size_t)i] / 32);
       }
     };
     std::vector<std::thread> th;

@@ -8,7 +8,8 @@
 //   Align(query = SSW-query, ref = SSW-ref, |ref|,
 //         Filter(report_begin_position=true, report_cigar=false, 0, 32767),
 //         &aln, maskLen = max(|ref| / 2, 15))
-// and returns score, ref_begin/end and query_begin/end per pair.  Used to
+// and returns score, ref_begin/end and query_begin/end per pair (and, from
+// the *2 / _one entries, sw_score_next_best).  Used to
 // record golden vectors and, on the host cores of the GPU machine, as the
 // "reference" CPU baseline.
 #include <pthread.h>
@@ -33,6 +34,11 @@ typedef struct {
 int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
                         const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
                         int nthreads);
+
+/* As ssw_ref_align_batch, plus sw_score_next_best of each pair in score2[p]. */
+int ssw_ref_align_batch2(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                         const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
+                         int32_t *score2, int nthreads);
 
 /* One call exactly as klign's ssw_align_read makes it (src/klign.cpp:251-273):
  * query = NUL-terminated contig window, ref = read of ref_len bases.
@@ -61,6 +67,12 @@ int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const
 int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
                         const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
                         int nthreads) {
+  return ssw_ref_align_batch2(npairs, a, a_off, a_len, b, b_off, b_len, sc, out, nullptr, nthreads);
+}
+
+int ssw_ref_align_batch2(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                         const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
+                         int32_t *score2, int nthreads) {
   if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
   std::atomic<int64_t> next(0);
   auto work = [&]() {
@@ -83,8 +95,10 @@ int ssw_ref_align_batch(int64_t npairs, const char *a, const int64_t *a_off, con
         if (!ok) {
           o.score = -1;
           o.ref_begin = o.ref_end = o.query_begin = o.query_end = -2;
+          if (score2) score2[p] = -1;
           continue;
         }
+        if (score2) score2[p] = aln.sw_score_next_best;
         o.score = aln.sw_score;
         o.ref_begin = aln.ref_begin;
         o.ref_end = aln.ref_end;

@@ -57,6 +57,19 @@ int oracle_align_batch(int64_t npairs,
                        const char *b, const int64_t *b_off, const int32_t *b_len,
                        const oracle_scoring *sc, oracle_result *out, int nthreads);
 
+/* As oracle_align_batch, plus SSW's second-best score (sw_score_next_best for
+ * klign's maskLen = max(|SSW-ref|/2, 15)) of each pair into score2[p]. */
+int oracle_align_batch2(int64_t npairs,
+                        const char *a, const int64_t *a_off, const int32_t *a_len,
+                        const char *b, const int64_t *b_off, const int32_t *b_len,
+                        const oracle_scoring *sc, oracle_result *out, int32_t *score2, int nthreads);
+
+/* The masking rule alone: rowmax[i] = max_j H(i, j) over the m SSW-ref
+ * positions (inner sequence padded by oracle_score2_pad positions scoring 0),
+ * score and end_i from the forward pass. */
+int oracle_score2_pad(int n, int32_t score, const oracle_scoring *sc);
+int32_t oracle_score2_rule(const int32_t *rowmax, int m, int32_t score, int32_t end_i, const oracle_scoring *sc);
+
 #ifdef __cplusplus
 }
 #endif

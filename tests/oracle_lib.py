@@ -26,17 +26,21 @@ class Scoring(ctypes.Structure):
 _libs = {}
 
 
-def _load(path, fn):
-    if path not in _libs:
+def _load(path, fn, with_score2=False):
+    key = (path, fn)
+    if key not in _libs:
         if not os.path.exists(path):
             return None
         lib = ctypes.CDLL(path)
         f = getattr(lib, fn)
         f.restype = ctypes.c_int
-        f.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Scoring), ctypes.c_void_p, ctypes.c_int]
-        _libs[path] = f
-    return _libs[path]
+        args = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Scoring), ctypes.c_void_p]
+        if with_score2:
+            args.append(ctypes.c_void_p)
+        f.argtypes = args + [ctypes.c_int]
+        _libs[key] = f
+    return _libs[key]
 
 
 def have_ref():
@@ -54,7 +58,7 @@ def pack_strings(seqs):
     return buf, offs, lens
 
 
-def _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads):
+def _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads, score2=None):
     n = len(a_len)
     out = np.zeros(n, dtype=RESULT_DTYPE)
     sc = Scoring(*scoring)
@@ -64,9 +68,28 @@ def _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads):
     b_off = np.ascontiguousarray(b_off, dtype=np.int64)
     a_len = np.ascontiguousarray(a_len, dtype=np.int32)
     b_len = np.ascontiguousarray(b_len, dtype=np.int32)
+    extra = () if score2 is None else (score2.ctypes.data,)
     fn(n, a_buf.ctypes.data, a_off.ctypes.data, a_len.ctypes.data, b_buf.ctypes.data, b_off.ctypes.data,
-       b_len.ctypes.data, ctypes.byref(sc), out.ctypes.data, nthreads)
+       b_len.ctypes.data, ctypes.byref(sc), out.ctypes.data, *extra, nthreads)
     return out
+
+
+def oracle_batch2(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads=0):
+    """(results, score2): the oracle with SSW's second-best score."""
+    fn = _load(ORACLE_SO, "oracle_align_batch2", True)
+    if fn is None:
+        raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+    s2 = np.zeros(len(a_len), dtype=np.int32)
+    return _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads, s2), s2
+
+
+def ref_batch2(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads=0):
+    """(results, score2) from the reference SSW (sw_score_next_best)."""
+    fn = _load(REF_SO, "ssw_ref_align_batch2", True)
+    if fn is None:
+        raise RuntimeError("oracle/_ref/libssw_ref.so is not built")
+    s2 = np.zeros(len(a_len), dtype=np.int32)
+    return _run(fn, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads, s2), s2
 
 
 def oracle_batch(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, nthreads=0):

@@ -114,3 +114,13 @@ def test_serving_entry_points_fail_cleanly_on_cpu():
     assert L.mhmsw_contigs_load(0, 1, buf, off, ln, ctypes.byref(h)) == -4
     srv = ctypes.c_void_p()
     assert L.mhmsw_server_create(0, None, None, 1, 0, ctypes.byref(srv)) == -1
+
+
+def test_host_packer_simd_matches_scalar():
+    """The SSSE3 2-bit packer used for staging and resident contigs equals the
+    per-base table packer on every byte value and every length up to 300."""
+    exe = os.path.join(ROOT, "tests", "cpp", "pack_test")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/pack_test is not built (run __graft_entry__.build())")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0 and p.stdout.startswith("ok"), p.stdout + p.stderr

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from golden.kat_cases import BASELINE_SCORING, KAT_CASES, KAT_SCORINGS
-from oracle_lib import have_ref, oracle_batch, oracle_pairs, pack_strings, ref_batch
+from oracle_lib import have_ref, oracle_batch, oracle_batch2, oracle_pairs, pack_strings, ref_batch, ref_batch2
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 FIELDS = ("score", "ref_begin", "ref_end", "query_begin", "query_end")
@@ -101,3 +101,72 @@ def test_oracle_matches_reference_fresh_corpus(seed):
     a, b = pack_strings(refs), pack_strings(qrys)
     for sc in [BASELINE_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1), (3, 2, 6, 1, 2)]:
         np.testing.assert_array_equal(table(oracle_batch(*a, *b, sc)), table(ref_batch(*a, *b, sc)))
+
+
+# ---- second-best score (SSW's sw_score_next_best) ----
+
+def load_score2():
+    z = np.load(os.path.join(GOLD, "golden_score2.npz"))
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", ["ssw_default", "mhm2_tree", "cigar", "baseline"])
+def test_oracle_score2_matches_reference_golden_small(name):
+    z, s = load_small(), load_score2()
+    res, s2 = oracle_batch2(z["ref_buf"], z["ref_off"], z["ref_len"], z["qry_buf"], z["qry_off"], z["qry_len"],
+                            tuple(int(v) for v in z["scoring_" + name]))
+    np.testing.assert_array_equal(table(res), z["res_" + name])
+    np.testing.assert_array_equal(s2, s["small_score2_" + name])
+
+
+@pytest.mark.parametrize("name", ["baseline", "ssw_default", "cigar"])
+def test_oracle_score2_window_edge_of_16bit_pass(name):
+    """SSW's 16-bit pass includes position end + maskLen in score2, its 8-bit
+    pass does not; these SSW-refs score past 255 - bias and put the second-best
+    row right at that edge."""
+    s = load_score2()
+    p = f"edge_{name}_"
+    res, s2 = oracle_batch2(s[p + "ref_buf"], s[p + "ref_off"], s[p + "ref_len"], s[p + "qry_buf"],
+                            s[p + "qry_off"], s[p + "qry_len"], tuple(int(v) for v in s[p + "scoring"]))
+    np.testing.assert_array_equal(table(res), s[p + "res"])
+    np.testing.assert_array_equal(s2, s[p + "score2"])
+
+
+@pytest.mark.parametrize("name", ["baseline", "mhm2_tree"])
+def test_oracle_score2_short_refs_mask_at_least_15(name):
+    s = load_score2()
+    res, s2 = oracle_batch2(s["short_ref_buf"], s["short_ref_off"], s["short_ref_len"], s["short_qry_buf"],
+                            s["short_qry_off"], s["short_qry_len"], tuple(int(v) for v in s[f"short_{name}_scoring"]))
+    np.testing.assert_array_equal(table(res), s[f"short_{name}_res"])
+    np.testing.assert_array_equal(s2, s[f"short_{name}_score2"])
+
+
+def test_oracle_score2_first_principles():
+    """Two copies of a motif in the SSW-ref: the second copy's score is the
+    second best when it lies outside the mask window, and is masked (score2
+    from elsewhere, here 0-score rows) when it lies inside."""
+    sc = BASELINE_SCORING
+    motif = "ACGTTGCAAGTCCGATGCAT"                     # 20 bases, scores 20 against itself
+    far = motif + "T" * 60 + motif[:12]                # second copy (12 bases) 60 bases later
+    near = motif + "T" * 5 + motif[:12]                # inside max(m/2, 15) of the end
+    res, s2 = oracle_batch2(*pack_strings([far, near]), *pack_strings([motif, motif]), sc)
+    assert list(res["score"]) == [20, 20]
+    assert s2[0] == 12          # 92-base ref, mask 46: the copy ends at 91 > 19 + 46
+    assert s2[1] < 12           # the copy ends at 36 <= 19 + 18: masked
+
+
+@pytest.mark.skipif(not have_ref(), reason="oracle/_ref (reference SSW build) not present")
+@pytest.mark.parametrize("seed", [4, 5])
+def test_oracle_score2_matches_reference_fresh_corpus(seed):
+    rng = np.random.default_rng(seed)
+    refs, qrys = [], []
+    for i in range(1200):
+        alph = ["ACGT", "ACGTN", "AC", "GGGGT"][i % 4]
+        refs.append("".join(rng.choice(list(alph), int(rng.integers(1, 300)))))
+        qrys.append("".join(rng.choice(list(alph), int(rng.integers(1, 300)))))
+    a, b = pack_strings(refs), pack_strings(qrys)
+    for sc in [BASELINE_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1)]:
+        r1, s1 = oracle_batch2(*a, *b, sc)
+        r2, s2 = ref_batch2(*a, *b, sc)
+        np.testing.assert_array_equal(table(r1), table(r2))
+        np.testing.assert_array_equal(s1, s2)
