@@ -65,6 +65,12 @@ class GPUDriver {
   void set_ambiguity(short ambiguity_cost);
   // Orientation of SSW's roles; default true = klign's CPU call (read is SSW-ref).
   void set_read_is_ssw_ref(bool read_is_ref);
+  // SSW's second-best score (Alignment::sw_score_next_best) per pair, which
+  // klign's GPU path sets to 0 (src/klign.cpp:378-379).  Enable before
+  // run_kernel_forwards; get_score2() is valid after the second kernel_block
+  // of a batch run with it (nullptr otherwise).
+  void set_score2(bool enable);
+  const int *get_score2();
 };
 
 }  // namespace adept_sw

@@ -111,6 +111,11 @@ int mhmsw_set_scoring(mhmsw_ctx *ctx, const mhmsw_scoring *scoring);
  * 0: the opposite roles.  Tie-breaks depend on this choice. */
 int mhmsw_set_orientation(mhmsw_ctx *ctx, int read_is_ssw_ref);
 int mhmsw_set_mode(mhmsw_ctx *ctx, int mode);
+/* 1: jobs also compute SSW's second-best score (Alignment::sw_score_next_best,
+ * ssw_core.cpp:512-536 / :822-842 with klign's maskLen = max(|SSW-ref|/2, 15),
+ * klign.cpp:257), which klign's GPU path leaves at 0 (klign.cpp:378-379).
+ * Needs every SSW-query sequence to be at most 8177 bases.  0 (default): off. */
+int mhmsw_set_score2(mhmsw_ctx *ctx, int enable);
 /* 1 (default): the job of a staged batch (unpack, forward, reruns, reverse,
  * finalize: ~40 launches) is captured once as a hipGraph and replayed while
  * the staged geometry stays the same.  0: launches are issued one by one. */
@@ -141,6 +146,8 @@ int mhmsw_query(mhmsw_ctx *ctx);
 int mhmsw_wait(mhmsw_ctx *ctx);
 /* Copies npairs results (npairs must equal the batch size) to host memory. */
 int mhmsw_results(mhmsw_ctx *ctx, mhmsw_result *out, int64_t npairs);
+/* score2 of each pair of the last batch (which must have run with score2 on). */
+int mhmsw_results_score2(mhmsw_ctx *ctx, int32_t *out, int64_t npairs);
 int mhmsw_get_stats(mhmsw_ctx *ctx, mhmsw_stats *out);
 
 /* --- device-resident contigs (serving; no counterpart in the reference,

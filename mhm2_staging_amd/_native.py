@@ -82,9 +82,9 @@ class SynthBatch(ctypes.Structure):
 # every symbol include/mhmsw.h and include/mhmsw_synth.h declare
 EXPORTS = [
     "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
-    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_graphs",
+    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_score2", "mhmsw_set_graphs",
     "mhmsw_set_tuning", "mhmsw_get_tuning", "mhmsw_autotune", "mhmsw_stage",
-    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_get_stats",
+    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_results_score2", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
     "mhmsw_contigs_load", "mhmsw_contigs_free", "mhmsw_contigs_bytes", "mhmsw_stage_windows",
     "mhmsw_server_create", "mhmsw_server_submit", "mhmsw_server_status", "mhmsw_server_wait",
@@ -114,6 +114,7 @@ def lib():
         "mhmsw_set_scoring": [vp, ctypes.POINTER(Scoring)],
         "mhmsw_set_orientation": [vp, ctypes.c_int],
         "mhmsw_set_mode": [vp, ctypes.c_int],
+        "mhmsw_set_score2": [vp, ctypes.c_int],
         "mhmsw_set_graphs": [vp, ctypes.c_int],
         "mhmsw_set_tuning": [vp, ctypes.POINTER(Tuning)],
         "mhmsw_get_tuning": [vp, ctypes.POINTER(Tuning)],
@@ -124,6 +125,7 @@ def lib():
         "mhmsw_query": [vp],
         "mhmsw_wait": [vp],
         "mhmsw_results": [vp, vp, i64],
+        "mhmsw_results_score2": [vp, vp, i64],
         "mhmsw_get_stats": [vp, ctypes.POINTER(Stats)],
         "mhmsw_synth_generate": [ctypes.c_int, i64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(SynthBatch)],
         "mhmsw_contigs_load": [ctypes.c_int, i64, vp, vp, vp, ctypes.POINTER(vp)],

@@ -75,6 +75,10 @@ class Aligner:
     def set_graphs(self, enable):
         self._check(self._L.mhmsw_set_graphs(self._ctx, 1 if enable else 0), "set_graphs")
 
+    def set_score2(self, enable):
+        """Also compute SSW's second-best score (sw_score_next_best) per pair."""
+        self._check(self._L.mhmsw_set_score2(self._ctx, 1 if enable else 0), "set_score2")
+
     def set_tuning(self, blocks_per_cu=8, g64_min_rows=257, bin_shift=5):
         t = _native.Tuning(int(blocks_per_cu), int(g64_min_rows), int(bin_shift))
         self._check(self._L.mhmsw_set_tuning(self._ctx, ctypes.byref(t)), "set_tuning")
@@ -120,6 +124,13 @@ class Aligner:
         n = self._batch.npairs if self._batch is not None else 0
         out = np.zeros(n, dtype=_native.RESULT_DTYPE)
         self._check(self._L.mhmsw_results(self._ctx, _ptr(out), n), "results")
+        return out
+
+    def score2(self):
+        """Second-best scores of the last batch (int32 per pair); needs set_score2(True)."""
+        n = self._batch.npairs if self._batch is not None else 0
+        out = np.zeros(n, dtype=np.int32)
+        self._check(self._L.mhmsw_results_score2(self._ctx, _ptr(out), n), "results_score2")
         return out
 
     def stats(self):

@@ -38,6 +38,9 @@ struct DriverState {
   bool submitted = false;
   bool fetched = false;
   bool backwards_called = false;
+  bool score2 = false;        // batches compute score2 (set_score2)
+  bool batch_score2 = false;  // the submitted batch did
+  std::vector<int32_t> s2;
 };
 
 static void throw_ctx(DriverState *s, int rc, const char *what, const char *file, int line) {
@@ -142,6 +145,18 @@ void GPUDriver::set_ambiguity(short ambiguity_cost) {
   MHMSW_CHECK(driver_state, mhmsw_set_scoring(driver_state->ctx, &driver_state->scoring));
 }
 
+void GPUDriver::set_score2(bool enable) {
+  if (!driver_state) throw std::runtime_error("GPU assert set_score2 before init\n");
+  MHMSW_CHECK(driver_state, mhmsw_set_score2(driver_state->ctx, enable ? 1 : 0));
+  driver_state->score2 = enable;
+}
+
+const int *GPUDriver::get_score2() {
+  DriverState *s = driver_state;
+  if (!s || !s->fetched || !s->batch_score2) return nullptr;
+  return s->s2.data();
+}
+
 void GPUDriver::set_read_is_ssw_ref(bool read_is_ref) {
   if (!driver_state) throw std::runtime_error("GPU assert set_read_is_ssw_ref before init\n");
   MHMSW_CHECK(driver_state, mhmsw_set_orientation(driver_state->ctx, read_is_ref ? 1 : 0));
@@ -179,6 +194,7 @@ void GPUDriver::run_kernel_forwards(std::vector<std::string> &reads, std::vector
   s->submitted = true;
   s->fetched = false;
   s->backwards_called = false;
+  s->batch_score2 = s->score2;
 }
 
 void GPUDriver::run_kernel_backwards(std::vector<std::string> &reads, std::vector<std::string> &contigs,
@@ -197,6 +213,10 @@ static void fetch(DriverState *s, AlignmentResults &a) {
   const size_t n = (size_t)s->npairs;
   s->res.resize(std::max<size_t>(n, 1));
   MHMSW_CHECK(s, mhmsw_results(s->ctx, s->res.data(), s->npairs));
+  if (s->batch_score2) {
+    s->s2.resize(std::max<size_t>(n, 1));
+    MHMSW_CHECK(s, mhmsw_results_score2(s->ctx, s->s2.data(), s->npairs));
+  }
   if (n > s->cap) {
     s->cap = std::max<size_t>(n, KLIGN_GPU_BLOCK_SIZE);
     s->store.assign(5 * s->cap, 0);

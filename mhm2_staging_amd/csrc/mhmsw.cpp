@@ -73,6 +73,8 @@ struct mhmsw_ctx {
   ScoreParams sp{};
   int mode = MHMSW_MODE_FAST;
   int read_is_ref = 1;
+  int score2 = 0;               // compute SSW's second-best score
+  bool last_job_score2 = false;
   mhmsw_tuning tuning{8, 257, 5};
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
@@ -81,6 +83,7 @@ struct mhmsw_ctx {
   int64_t total = 0;
   int64_t cells = 0;
   int max_plain_cols = 0;
+  int max_read_len = 0, max_ctg_len = 0;
   bool fast_class_used[kNumClasses] = {};
   bool staged = false;
   bool in_flight = false;
@@ -94,6 +97,7 @@ struct mhmsw_ctx {
   const mhmsw_contigs* res = nullptr;  // resident contigs of the staged batch (windowed batches)
   int64_t n_fast = 0, n_plain = 0;
   DevBuf<PairEnd> d_fwd, d_rev;
+  DevBuf<int32_t> d_s2;
   DevBuf<SwTask> d_tasks;
   DevBuf<PassSched> d_sched;
   DevBuf<AlnResult> d_results;
@@ -141,6 +145,7 @@ static void compute_score_params(mhmsw_ctx* c) {
   p.gap_ext = s.gap_ext;
   p.ambiguity = s.ambiguity;
   p.read_is_ref = c->read_is_ref;
+  p.bias = std::max(0, std::max(s.mismatch, s.ambiguity));  // ssw_init, ssw_core.cpp:1076-1083
   // packed kernel: column and row codes select one byte of an 8-byte table
   // holding 16*(s + gap_open): bytes 0..3 = match, mismatch x3 (selector a^b),
   // bytes 4..7 = ambiguity (selector has the N bit).  Needs every entry in
@@ -288,6 +293,14 @@ int mhmsw_set_orientation(mhmsw_ctx* c, int read_is_ssw_ref) {
   return MHMSW_OK;
 }
 
+int mhmsw_set_score2(mhmsw_ctx* c, int enable) {
+  if (!c) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "score2 changed while a batch is in flight");
+  c->score2 = enable ? 1 : 0;
+  if (c->staged) reroute(c);
+  return MHMSW_OK;
+}
+
 int mhmsw_set_mode(mhmsw_ctx* c, int mode) {
   if (!c) return MHMSW_E_ARG;
   if (mode != MHMSW_MODE_FAST && mode != MHMSW_MODE_PLAIN) return fail(c, MHMSW_E_ARG, "unknown mode %d", mode);
@@ -321,12 +334,20 @@ struct PackParams {
   int fast_ok = 1;
   int read_is_ref = 1;
   int g64_min_rows = 257;
+  int score2 = 0;
 };
 
 // host copy of make_task's forward routing (sw_kernels.hip)
 static int route_class(const PackParams& pp, int32_t lref, int32_t lqry) {
   if (lref <= 0 || lqry <= 0) return -1;
   const bool wide = pp.mode == MHMSW_MODE_PLAIN || !pp.fast_ok;
+  if (pp.score2) {  // rows are SSW-ref
+    if (!wide && lref <= kFast16MaxRows && lref < pp.g64_min_rows && lqry + kMaxQueryPad <= kFastMaxCols)
+      return kClassFast16 + (lref + 31) / 32 - 1;
+    if (!wide && lref <= kFast64MaxRows && lqry + kMaxQueryPad <= kFastMaxCols)
+      return kClassFast64 + (lref + 127) / 128 - 1;
+    return kClassPlain;
+  }
   const int32_t shorter = std::min(lref, lqry), longer = std::max(lref, lqry);
   if (!wide && shorter <= kFast16MaxRows && shorter < pp.g64_min_rows && longer <= kFastMaxCols)
     return kClassFast16 + (shorter + 31) / 32 - 1;
@@ -362,6 +383,7 @@ struct UninitAlloc : std::allocator<T> {
 struct PackedBatch {
   int64_t npairs = 0, total = 0, cells = 0;
   int max_plain_cols = 0;
+  int max_read_len = 0, max_ctg_len = 0;
   bool used[kNumClasses] = {};
   int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
   int64_t nfast = 0, nplain = 0;
@@ -409,7 +431,7 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   // offset 0, route, and sum
   struct Chunk {
     int64_t p0 = 0, p1 = 0, bases = 0, cells = 0, nfast = 0, nplain = 0;
-    int max_plain_cols = 0;
+    int max_plain_cols = 0, max_rl = 0, max_cl = 0;
     bool used[kNumClasses] = {};
     int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
     int64_t bad = -1;  // first invalid pair of the chunk
@@ -440,6 +462,9 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
                    cl);
         } else if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20)) {
           snprintf(m, sizeof(m), "pair %lld: sequence longer than 2^20 bases", (long long)p);
+        } else if (pp.score2 && (pp.read_is_ref ? cl : rl) + kMaxQueryPad > kPlainMaxCols) {
+          snprintf(m, sizeof(m), "pair %lld: score2 needs SSW-query sequences of at most %d bases", (long long)p,
+                   kPlainMaxCols - kMaxQueryPad);
         }
         if (m[0]) {
           c.bad = p;
@@ -454,6 +479,8 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
         acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
         c.cells += (int64_t)rl * cl;
         c.max_plain_cols = std::max(c.max_plain_cols, (int)std::min(rl, cl));
+        c.max_rl = std::max(c.max_rl, (int)rl);
+        c.max_cl = std::max(c.max_cl, (int)cl);
         const int32_t lref = pp.read_is_ref ? rl : cl, lqry = pp.read_is_ref ? cl : rl;
         const int cls = route_class(pp, lref, lqry);
         if (cls >= 0) {
@@ -483,6 +510,8 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
     acc += c.bases;
     pb.cells += c.cells;
     pb.max_plain_cols = std::max(pb.max_plain_cols, c.max_plain_cols);
+    pb.max_read_len = std::max(pb.max_read_len, c.max_rl);
+    pb.max_ctg_len = std::max(pb.max_ctg_len, c.max_cl);
     for (int q = 0; q < kNumClasses; q++) {
       pb.used[q] = pb.used[q] || c.used[q];
       pb.cpairs[q] += c.cpairs[q];
@@ -536,6 +565,7 @@ static PackParams pack_params(const mhmsw_ctx* c) {
   pp.fast_ok = c->sp.fast_ok;
   pp.read_is_ref = c->read_is_ref;
   pp.g64_min_rows = c->tuning.g64_min_rows;
+  pp.score2 = c->score2;
   return pp;
 }
 
@@ -582,6 +612,8 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
   c->total = total;
   c->cells = pb.cells;
   c->max_plain_cols = pb.max_plain_cols;
+  c->max_read_len = pb.max_read_len;
+  c->max_ctg_len = pb.max_ctg_len;
   c->n_fast = pb.nfast;
   c->n_plain = pb.nplain;
   std::memcpy(c->fast_class_used, pb.used, sizeof(pb.used));
@@ -597,6 +629,7 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
   HIP_OR_FAIL(c, c->d_packed2.ensure(n2));
   HIP_OR_FAIL(c, c->d_nmask.ensure(nm));
   HIP_OR_FAIL(c, c->d_codes.ensure((size_t)(2 * total / 8 + 64)));
+  HIP_OR_FAIL(c, c->d_s2.ensure(np1));
   HIP_OR_FAIL(c, c->d_base.ensure((size_t)std::max<int64_t>(nseq, 1)));
   HIP_OR_FAIL(c, c->d_len.ensure((size_t)std::max<int64_t>(nseq, 1)));
   HIP_OR_FAIL(c, c->d_bucket.ensure(np1));
@@ -669,6 +702,13 @@ int mhmsw_stage_windows(mhmsw_ctx* c, const mhmsw_contigs* contigs, int64_t npai
 }  // extern "C"
 
 
+// Columns the 32-bit kernel keeps in LDS: the shorter sequence, or with
+// score2 the SSW-query plus its padding (any pair may be rerun there).
+static int plain_lds_cols(const mhmsw_ctx* c) {
+  if (c->score2) return (c->read_is_ref ? c->max_ctg_len : c->max_read_len) + kMaxQueryPad;
+  return c->max_plain_cols;
+}
+
 static BatchView make_view(mhmsw_ctx* c) {
   BatchView b;
   std::memset(&b, 0, sizeof(b));  // the graph cache compares views bytewise
@@ -692,6 +732,8 @@ static BatchView make_view(mhmsw_ctx* c) {
   b.mode = c->mode;
   b.g64_min_rows = c->tuning.g64_min_rows;
   b.bin_shift = c->tuning.bin_shift;
+  b.score2 = c->score2;
+  b.s2 = c->d_s2.ptr;
   if (c->res) {
     b.res2 = c->res->d2;
     b.resN = c->res->dN;
@@ -716,7 +758,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // follow the packed form's tuning (16 wavefronts per CU: its fastest
   // measured shape, 1.7x the rate at 8 on the length-skewed 4M-pair batch)
   const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * 16, c->npairs));
-  const int plain_cols = std::max(1, c->max_plain_cols);
+  const int plain_cols = std::max(1, plain_lds_cols(c));
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
   HIP_OR_FAIL(c, launch_clear_counters(b, st));
   if (c->npairs > 0) {
@@ -796,9 +838,13 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     int rc = mhmsw_wait(c);
     if (rc != MHMSW_OK) return rc;
   }
+  if (c->score2 && plain_lds_cols(c) > kPlainMaxCols)
+    return fail(c, MHMSW_E_ARG, "score2 needs SSW-query sequences of at most %d bases (longest: %d)",
+                kPlainMaxCols - kMaxQueryPad, plain_lds_cols(c) - kMaxQueryPad);
   HIP_OR_FAIL(c, hipSetDevice(c->device));
   hipStream_t st = c->stream;
   BatchView b = make_view(c);
+  c->last_job_score2 = c->score2 != 0;
   for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = c->npairs > 0 && c->fast_class_used[k];
   int maxp16 = 0, maxp64 = 0;
   c->last_job_graph = c->use_graphs;
@@ -812,7 +858,7 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     unsigned char* kp = key.data();
     std::memcpy(kp, &b, sizeof(BatchView));
     std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
-    const int extra[2] = {c->max_plain_cols, c->mode * 64 + c->tuning.blocks_per_cu};
+    const int extra[2] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu};
     std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
     if (!c->graph_exec || key != c->graph_key) {
       drop_graph(c);
@@ -934,6 +980,20 @@ int mhmsw_results(mhmsw_ctx* c, mhmsw_result* out, int64_t npairs) {
   if (c->h_errors[0] != 0 || c->h_errors[2] != 0)
     return fail(c, MHMSW_E_INTERNAL, "%d pairs: reverse pass did not reproduce the forward score; %d end points "
                 "outside their pair", c->h_errors[0], c->h_errors[2]);
+  return MHMSW_OK;
+}
+
+int mhmsw_results_score2(mhmsw_ctx* c, int32_t* out, int64_t npairs) {
+  if (!c || (!out && npairs > 0)) return MHMSW_E_ARG;
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  if (!c->have_results) return fail(c, MHMSW_E_STATE, "no results: submit a batch first");
+  if (!c->last_job_score2) return fail(c, MHMSW_E_STATE, "the last batch ran without score2 (mhmsw_set_score2)");
+  if (npairs != c->npairs) return fail(c, MHMSW_E_ARG, "score2 requested for %lld pairs, batch has %lld",
+                                       (long long)npairs, (long long)c->npairs);
+  if (npairs > 0) HIP_OR_FAIL(c, hipMemcpy(out, c->d_s2.ptr, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
   return MHMSW_OK;
 }
 
@@ -1163,6 +1223,8 @@ void merge_packed(const std::vector<std::shared_ptr<Request>>& parts, PackedBatc
     }
     out.cells += b.cells;
     out.max_plain_cols = std::max(out.max_plain_cols, b.max_plain_cols);
+    out.max_read_len = std::max(out.max_read_len, b.max_read_len);
+    out.max_ctg_len = std::max(out.max_ctg_len, b.max_ctg_len);
     for (int k = 0; k < kNumClasses; k++) {
       out.used[k] = out.used[k] || b.used[k];
       out.cpairs[k] += b.cpairs[k];

@@ -77,6 +77,7 @@ struct AlnResult {
 };
 
 constexpr int32_t kEndWide = 2;  // the pair's forward pass ran in 32-bit (reverse must too)
+constexpr int32_t kEndScore2 = 4;  // score2 must be recomputed in 32-bit (16-bit-pass padding differs)
 
 // Per-batch scoring, pre-scaled for the packed kernel.
 struct ScoreParams {
@@ -84,6 +85,16 @@ struct ScoreParams {
   uint32_t perm_lo, perm_hi;  // v_perm table of (s + gap_open) * 16 per class
   int32_t fast_ok;            // scoring fits the packed kernel
   int32_t read_is_ref;        // orientation
+  int32_t bias;               // SSW's profile bias, max(0, mismatch, ambiguity)
 };
+
+// SSW pads its inner (query) sequence to whole vectors of 16 (8-bit pass) or
+// 8 (16-bit pass, used once score + bias >= 255) positions that score 0
+// against everything; its per-position maxima, and so score2, include them.
+__host__ __device__ inline int ssw_query_pad(int n, int32_t score, int32_t bias) {
+  const int v = score + bias >= 255 ? 8 : 16;
+  return (v - n % v) % v;
+}
+constexpr int kMaxQueryPad = 15;
 
 }  // namespace mhmsw

@@ -229,7 +229,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     ref_start = b.seq_base[sref];
     qry_start = b.seq_base[sqry];
   } else if (pass == kPassRerun) {
-    if (!(fe.flags & kEndOverflow)) return false;
+    if (!(fe.flags & (kEndOverflow | kEndScore2))) return false;
     wide = true;
     ref_len = lref;
     qry_len = lqry;
@@ -285,6 +285,25 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   }
   t.pair = (int32_t)p;
   t.flags = pass == kPassRev ? kTaskReverse : 0;
+  if (b.score2 && pass != kPassRev) {
+    // score2 reads per-SSW-ref-position maxima: rows are SSW-ref, columns
+    // SSW-query plus up to kMaxQueryPad zero-scoring columns
+    if (!wide && ref_len <= kFast16MaxRows && ref_len < b.g64_min_rows && qry_len + kMaxQueryPad <= kFastMaxCols) {
+      cls = kClassFast16 + (ref_len + 31) / 32 - 1;
+    } else if (!wide && ref_len <= kFast64MaxRows && qry_len + kMaxQueryPad <= kFastMaxCols) {
+      cls = kClassFast64 + (ref_len + 127) / 128 - 1;
+    } else {
+      cls = kClassPlain;
+    }
+    t.flags |= kTaskRowsAreRef;
+    t.row_start = ref_start;
+    t.row_len = ref_len;
+    t.col_start = qry_start;
+    t.col_len = qry_len;
+    const int bn = t.col_len >> b.bin_shift;
+    bin = bn < kNumBins ? bn : kNumBins - 1;
+    return true;
+  }
   const int32_t shorter = ref_len <= qry_len ? ref_len : qry_len;
   const int32_t longer = ref_len <= qry_len ? qry_len : ref_len;
   const bool ref_rows_if_short = ref_len <= qry_len;
@@ -354,7 +373,10 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
     int cls = 0, bin = 0;
     if (!make_task(b, pass, p, t, cls, bin)) {
       b.bucket[p] = -1;
-      if (pass == kPassFwd) b.fwd[p] = PairEnd{0, -1, 0, 0};
+      if (pass == kPassFwd) {
+        b.fwd[p] = PairEnd{0, -1, 0, 0};
+        if (b.score2) b.s2[p] = 0;
+      }
       if (pass == kPassRev) b.rev[p] = PairEnd{0, 0, 0, 0};
       continue;
     }
@@ -456,14 +478,22 @@ struct FastState {
   uint32_t dcur, dnext;  // code dwords feeding the next refill
 };
 
-template <int G, int P, bool EDGE, int S>
-__device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, uint32_t tlo, uint32_t thi,
-                                          uint32_t go, uint32_t ge, uint32_t neg) {
+// PAD (edge chunks of score2 tasks): the npad columns after the last one
+// score 0 against every row, as SSW's query padding does; their cells get
+// s' = 16*(0 + gO) in place of the kill value.
+template <int G, int P, bool EDGE, bool PAD, int S>
+__device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, int npad, uint32_t tlo,
+                                          uint32_t thi, uint32_t go, uint32_t ge, uint32_t neg) {
   uint32_t code = st.w & 0xFu;
   st.w >>= 4;
   if constexpr (EDGE) {
     const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
     code = kill ? kKillCode : code;
+  }
+  uint32_t pm = 0;  // halves whose column is a padding column
+  if constexpr (PAD) {
+    const uint32_t clo = (uint32_t)(t - 2 * l - col_len);
+    pm = (clo < (uint32_t)npad ? 0x0000FFFFu : 0u) | (clo - 1u < (uint32_t)npad ? 0xFFFF0000u : 0u);
   }
   st.q = (st.q << 16) | code;
   constexpr uint32_t tag = (uint32_t)(15 - S) * 0x00010001u;
@@ -472,7 +502,8 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
 #pragma unroll
   for (int p = 0; p < P; p++) {
     const uint32_t sel = subst_sel(st.rb[p], st.q);
-    const uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
+    uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
+    if constexpr (PAD) sp = (sp & ~pm) | (go & pm);
     const uint32_t d = pk_adds(diag, sp);
     // E, F >= 0, so H = max(d, E, F) needs no separate max with 0
     uint32_t h = pk_max(d, st.hg[p]);
@@ -494,10 +525,10 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   st.vg = vnew;
 }
 
-template <int G, int P, bool EDGE>
-__device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, const uint32_t* __restrict__ codes,
-                                           int64_t& dw, uint32_t sh, uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge,
-                                           uint32_t neg) {
+template <int G, int P, bool EDGE, bool PAD = false>
+__device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
+                                           const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh, uint32_t tlo,
+                                           uint32_t thi, uint32_t go, uint32_t ge, uint32_t neg) {
   // w takes the 8 codes from dcur:dnext (loaded one refill earlier); the
   // dword after them is requested now and used at the next refill.
 #define MHMSW_REFILL()                                            \
@@ -508,23 +539,23 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
     st.dnext = codes[dw];                                         \
   }
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, 0>(st, t0 + 0, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 1>(st, t0 + 1, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 2>(st, t0 + 2, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 3>(st, t0 + 3, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 4>(st, t0 + 4, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 5>(st, t0 + 5, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 6>(st, t0 + 6, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 7>(st, t0 + 7, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 0>(st, t0 + 0, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 1>(st, t0 + 1, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 2>(st, t0 + 2, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 3>(st, t0 + 3, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 4>(st, t0 + 4, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 5>(st, t0 + 5, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 6>(st, t0 + 6, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 7>(st, t0 + 7, l, col_len, npad, tlo, thi, go, ge, neg);
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, 8>(st, t0 + 8, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 9>(st, t0 + 9, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 10>(st, t0 + 10, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 11>(st, t0 + 11, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 12>(st, t0 + 12, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 13>(st, t0 + 13, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 14>(st, t0 + 14, l, col_len, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, 15>(st, t0 + 15, l, col_len, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 8>(st, t0 + 8, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 9>(st, t0 + 9, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 10>(st, t0 + 10, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 11>(st, t0 + 11, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 12>(st, t0 + 12, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 13>(st, t0 + 13, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 14>(st, t0 + 14, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 15>(st, t0 + 15, l, col_len, npad, tlo, thi, go, ge, neg);
 #undef MHMSW_REFILL
   // fold the chunk's per-row maxima into the (score, first step) records
   const uint32_t cbase = 0xFFF0u - (uint32_t)t0;  // 0xFFFF - (t0 + 15 - tag) = cbase + tag
@@ -544,10 +575,13 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const bool active = task.pair >= 0;
   const int32_t row_len = active ? task.row_len : 0;
   const int32_t col_len = active ? task.col_len : 0;
+  // score2 (forward tasks): SSW's 8-bit-pass query padding after the last column
+  const bool s2task = b.score2 && !(task.flags & kTaskReverse);
+  const int npad = (active && s2task) ? ssw_query_pad(col_len, 0, 0) : 0;
   // per-group values are uniform within a group: combine the groups' lane 0
   int nsteps_need = 0, mincols = 0x7FFFFFFF;
   {
-    const int need = active ? col_len + 2 * G - 1 : 0;
+    const int need = active ? col_len + npad + 2 * G - 1 : 0;
     const int cmin = active ? col_len : 0x7FFFFFFF;
 #pragma unroll
     for (int g = 0; g < 64 / G; g++) {
@@ -600,12 +634,16 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   st.dnext = codes[dw];
   st.w = 0;
 
+  // uniform: a launch holds the tasks of one pass (padding slots carry no flags)
+  const bool pads = b.score2 != 0 && __ballot(task.flags & kTaskReverse) == 0;
   for (int t0 = 0; t0 < nsteps; t0 += 16) {
     const bool middle = t0 >= 2 * G && t0 + 16 <= mincols;
     if (middle) {
-      fast_chunk<G, P, false>(st, t0, l, col_len, codes, dw, sh, tlo, thi, go, ge, neg);
+      fast_chunk<G, P, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge, neg);
+    } else if (pads) {
+      fast_chunk<G, P, true, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge, neg);
     } else {
-      fast_chunk<G, P, true>(st, t0, l, col_len, codes, dw, sh, tlo, thi, go, ge, neg);
+      fast_chunk<G, P, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge, neg);
     }
   }
 
@@ -636,6 +674,36 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
     const int32_t ho2 = __shfl_xor(hmax, o, G);
     hmax = ho2 > hmax ? ho2 : hmax;
   }
+  // score2: the largest per-row maximum (padding columns included, as SSW's
+  // maxColumn) over the SSW-ref rows outside the mask window around the end
+  // (ssw_core.cpp:512-536, :822-842); rows are SSW-ref for these tasks.
+  int32_t s2v = 0;
+  bool s2_rerun = false;
+  if (pads) {
+    int32_t es = 0, ei = -1, ej = 0;
+    if (bestk != 0) key_decode(bestk, es, ei, ej);
+    const bool word = es + b.sc.bias >= 255;
+    const int32_t mask = row_len / 2 > 15 ? row_len / 2 : 15;
+    const int32_t lo = ei - mask, hi = ei + mask - (word ? 1 : 0);
+    int32_t m = 0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        const int32_t r = 2 * P * l + half * P + p;
+        const int32_t h = (int32_t)((half ? st.rec_hi[p] : st.rec_lo[p]) >> 16);
+        if (r < row_len && (r < lo || r > hi)) m = h > m ? h : m;
+      }
+    }
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) {
+      const int32_t mo = __shfl_xor(m, o, G);
+      m = mo > m ? mo : m;
+    }
+    s2v = m;
+    // the 16-bit pass pads to 8 positions: recompute when that differs
+    s2_rerun = ssw_query_pad(col_len, es, b.sc.bias) != npad;
+  }
   if (active && l == 0) {
     PairEnd e;
     if (bestk == 0) {
@@ -645,6 +713,13 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
       e.flags = 0;
     }
     if (hmax >= 2047) e.flags |= kEndOverflow;  // 16*H may have saturated: rerun in 32-bit
+    if (s2task) {
+      if (s2_rerun) {
+        e.flags |= kEndScore2;
+      } else {
+        b.s2[task.pair] = s2v;
+      }
+    }
     if (task.flags & kTaskReverse) {
       b.rev[task.pair] = e;
     } else {
@@ -896,15 +971,88 @@ __global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
 // ----------------------------------------------------------------------------
 // plain 32-bit kernel: one wavefront per pair, rows in tiles of 64
 // ----------------------------------------------------------------------------
+// One sweep of the 32-bit DP over a pair's rows in tiles of 64 (wave-uniform
+// task).  Columns [cols, cols + npad) score 0 against every row: SSW's query
+// padding, used only for score2.  s2 = false: returns the lexicographic end
+// key over the real columns.  s2 = true: returns the largest row maximum
+// (padding columns included) over the rows outside [lo, hi].
+__device__ uint64_t plain_sweep(const BatchView& b, const SwTask& task, int k, int npad, bool s2, int32_t lo,
+                                int32_t hi, int2* bnd) {
+  const int32_t M = b.sc.match, X = b.sc.mismatch, A = b.sc.ambiguity, gO = b.sc.gap_open, gE = b.sc.gap_ext;
+  const int32_t NEG = -(1 << 28);
+  const uint32_t* __restrict__ codes = b.codes;
+  const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
+  const int32_t rows = task.row_len, cols = task.col_len, ncols = cols + npad;
+  uint64_t pair_best = 0;
+  int32_t s2best = 0;
+  const int ntiles = (rows + 63) / 64;
+  for (int tile = 0; tile < ntiles; tile++) {
+    const int32_t r = tile * 64 + k;
+    const bool row_ok = r < rows;
+    const uint32_t rc = row_ok ? code_at(codes, task.row_start + r) : 0u;
+    int32_t hprev = 0, hg = NEG, best = 0, bestc = 0, rowall = 0;
+    int32_t up_h_prev = 0;  // H(r-1, c-1)
+    int32_t out_h = 0, out_e = NEG;
+    const int64_t cb = task.col_start - k;  // column c = t - k
+    for (int t = 0; t < ncols + 63; t++) {
+      const int32_t c = t - k;
+      const bool col_ok = c >= 0 && c < ncols;
+      const bool real = c < cols;
+      // inputs from the row above at column c
+      int32_t top_h = 0, top_e = NEG;
+      if (k == 0 && tile > 0 && col_ok) {
+        const int2 v = bnd[c];
+        top_h = v.x;
+        top_e = v.y;
+      }
+      const int32_t up_h = __builtin_amdgcn_update_dpp(top_h, out_h, 0x138, 0xF, 0xF, false);
+      const int32_t up_e = __builtin_amdgcn_update_dpp(top_e, out_e, 0x138, 0xF, 0xF, false);
+      const uint32_t q = (col_ok && real) ? code_at(codes, cb + t) : 0u;
+      const int32_t sub = real ? (((rc | q) & 4u) ? -A : (rc == q ? M : -X)) : 0;
+      const int32_t e = max(up_e - gE, up_h - gO);     // gap along the rows
+      const int32_t f = max(hg - gE, hprev - gO);      // gap along the columns
+      int32_t h = max(max(up_h_prev + sub, e), max(f, 0));
+      const bool ok = row_ok && col_ok;
+      h = ok ? h : 0;
+      const int32_t f_keep = ok ? f : NEG;
+      const int32_t e_out = ok ? e : NEG;
+      if (ok && real && h > best) {
+        best = h;
+        bestc = c;
+      }
+      rowall = h > rowall ? h : rowall;
+      if (k == 63 && col_ok) bnd[c] = make_int2(h, e_out);
+      up_h_prev = up_h;
+      hprev = h;
+      hg = f_keep;
+      out_h = h;
+      out_e = e_out;
+    }
+    if (s2) {
+      const int32_t v = (row_ok && (r < lo || r > hi)) ? rowall : 0;
+      const int32_t m = wave_max_i32(v);
+      s2best = m > s2best ? m : s2best;
+    } else {
+      uint64_t key = 0;
+      if (row_ok && best > 0) key = rows_ref ? end_key(best, r, bestc) : end_key(best, bestc, r);
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const uint64_t other = shfl_xor_u64(key, o, 64);
+        key = other > key ? other : key;
+      }
+      pair_best = key > pair_best ? key : pair_best;
+    }
+    __syncthreads();
+  }
+  return s2 ? (uint64_t)(uint32_t)s2best : pair_best;
+}
+
 __global__ __launch_bounds__(64) void sw_plain_kernel(BatchView b) {
   extern __shared__ int2 bnd[];  // last row of the previous tile: (H, E) per column
   const int k = threadIdx.x;
   PassSched* s = b.sched;
   const int32_t n = s->class_count[kClassPlain];
   const int32_t base = s->class_base[kClassPlain];
-  const int32_t M = b.sc.match, X = b.sc.mismatch, A = b.sc.ambiguity, gO = b.sc.gap_open, gE = b.sc.gap_ext;
-  const int32_t NEG = -(1 << 28);
-  const uint32_t* __restrict__ codes = b.codes;
   for (;;) {
     const int w = from_lane0(atomicAdd(&s->class_next[kClassPlain], k == 0 ? 1 : 0));
     if (w >= n) break;
@@ -916,68 +1064,25 @@ __global__ __launch_bounds__(64) void sw_plain_kernel(BatchView b) {
     task.col_start = uni64(tp->col_start);
     task.row_len = uni(tp->row_len);
     task.col_len = uni(tp->col_len);
-    const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
-    const int32_t rows = task.row_len, cols = task.col_len;
-    uint64_t pair_best = 0;
-    const int ntiles = (rows + 63) / 64;
-    for (int tile = 0; tile < ntiles; tile++) {
-      const int32_t r = tile * 64 + k;
-      const bool row_ok = r < rows;
-      const uint32_t rc = row_ok ? code_at(codes, task.row_start + r) : 0u;
-      int32_t hprev = 0, hg = NEG, best = 0, bestc = 0;
-      int32_t up_h_prev = 0;  // H(r-1, c-1)
-      int32_t out_h = 0, out_e = NEG;
-      const int64_t cb = task.col_start - k;  // column c = t - k
-      for (int t = 0; t < cols + 63; t++) {
-        const int32_t c = t - k;
-        const bool col_ok = c >= 0 && c < cols;
-        // inputs from the row above at column c
-        int32_t top_h = 0, top_e = NEG;
-        if (k == 0 && tile > 0 && col_ok) {
-          const int2 v = bnd[c];
-          top_h = v.x;
-          top_e = v.y;
-        }
-        const int32_t up_h = __builtin_amdgcn_update_dpp(top_h, out_h, 0x138, 0xF, 0xF, false);
-        const int32_t up_e = __builtin_amdgcn_update_dpp(top_e, out_e, 0x138, 0xF, 0xF, false);
-        const uint32_t q = col_ok ? code_at(codes, cb + t) : 0u;
-        const int32_t sub = ((rc | q) & 4u) ? -A : (rc == q ? M : -X);
-        const int32_t e = max(up_e - gE, up_h - gO);     // gap along the rows
-        const int32_t f = max(hg - gE, hprev - gO);      // gap along the columns
-        int32_t h = max(max(up_h_prev + sub, e), max(f, 0));
-        const bool ok = row_ok && col_ok;
-        h = ok ? h : 0;
-        const int32_t f_keep = ok ? f : NEG;
-        const int32_t e_out = ok ? e : NEG;
-        if (ok && h > best) {
-          best = h;
-          bestc = c;
-        }
-        if (k == 63 && col_ok) bnd[c] = make_int2(h, e_out);
-        up_h_prev = up_h;
-        hprev = h;
-        hg = f_keep;
-        out_h = h;
-        out_e = e_out;
-      }
-      uint64_t key = 0;
-      if (row_ok && best > 0) key = rows_ref ? end_key(best, r, bestc) : end_key(best, bestc, r);
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) {
-        const uint64_t other = shfl_xor_u64(key, o, 64);
-        key = other > key ? other : key;
-      }
-      pair_best = key > pair_best ? key : pair_best;
-      __syncthreads();
+    const uint64_t pair_best = plain_sweep(b, task, k, 0, false, 0, 0, bnd);
+    PairEnd e;
+    if (pair_best == 0) {
+      e = PairEnd{0, -1, 0, kEndWide};
+    } else {
+      key_decode(pair_best, e.score, e.i, e.j);
+      e.flags = kEndWide;
+    }
+    if (b.score2 && !(task.flags & kTaskReverse)) {
+      // score2 (rows are SSW-ref): a second sweep with the query padding of
+      // the pass SSW would use for this score, masked around the end row
+      const bool word = e.score + b.sc.bias >= 255;
+      const int32_t mask = task.row_len / 2 > 15 ? task.row_len / 2 : 15;
+      const int32_t lo = e.i - mask, hi = e.i + mask - (word ? 1 : 0);
+      const int npad = ssw_query_pad(task.col_len, e.score, b.sc.bias);
+      const int32_t v = (int32_t)plain_sweep(b, task, k, npad, true, lo, hi, bnd);
+      if (k == 0) b.s2[task.pair] = v;
     }
     if (k == 0) {
-      PairEnd e;
-      if (pair_best == 0) {
-        e = PairEnd{0, -1, 0, kEndWide};
-      } else {
-        key_decode(pair_best, e.score, e.i, e.j);
-        e.flags = kEndWide;
-      }
       if (task.flags & kTaskReverse) {
         b.rev[task.pair] = e;
       } else {

@@ -38,6 +38,10 @@ struct BatchView {
   const int32_t* win_off;
   int32_t g64_min_rows;  // shorter side at or above this: 64-lane groups
   int32_t bin_shift;     // column bucket width 1 << bin_shift
+  // second-best score (SSW's sw_score_next_best with klign's maskLen): forward
+  // rows are then always SSW-ref positions
+  int32_t score2;
+  int32_t* s2;           // [npairs]
 };
 
 hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);

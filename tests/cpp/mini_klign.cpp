@@ -10,6 +10,8 @@
 //   flush_remaining ...................................... :608-621
 //   _gpu_align_block_kernel (driver protocol, Aln mapping,
 //     score2 = mismatches = 0) ........................... :356-388
+// With --gpu-score2 the GPU side fills score2 from the driver's second-best
+// scores (GPUDriver::set_score2 / get_score2) instead of klign's 0.
 //   switch_orient ........................................ src/utils.cpp:163-167
 //   Aln fields and to_string ............................. include/alignments.hpp:58-72,
 //                                                           src/alignments.cpp:96-103
@@ -23,7 +25,7 @@
 //
 // usage: mini_klign --mode gpu|cpu --out FILE [--seed S] [--reads N]
 //          [--block B] [--mem BYTES] [--no-steal] [--gpu-delay-ms D]
-//          [--scoring m,x,o,e,a] [--ssw LIB] [--oracle LIB]
+//          [--scoring m,x,o,e,a] [--ssw LIB] [--oracle LIB] [--gpu-score2]
 // Writes one line per alignment (Aln::to_string, then identity, mismatches
 // and the side that aligned it: P perfect shortcut, G GPU, C CPU) and prints
 // a JSON summary on stdout.
@@ -134,6 +136,7 @@ struct KlignSim {
   int64_t block_size = KLIGN_GPU_BLOCK_SIZE;
   bool steal = true;
   int gpu_delay_ms = 0;
+  bool gpu_score2 = false;
   adept_sw::GPUDriver gpu_driver;
 
   std::vector<Aln> kernel_alns;
@@ -217,6 +220,8 @@ struct KlignSim {
     gpu_driver.kernel_block();
     gpu_secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     auto aln_results = gpu_driver.get_aln_results();
+    const int* s2 = gpu_score2 ? gpu_driver.get_score2() : nullptr;
+    if (gpu_score2 && !s2) throw std::runtime_error("the driver returned no score2");
     std::vector<Aln> out;
     out.reserve(b.kernel_alns.size());
     for (size_t i = 0; i < b.kernel_alns.size(); i++) {
@@ -227,7 +232,7 @@ struct KlignSim {
       aln.cstart += aln_results.ref_begin[i];
       if (aln.orient == '-') switch_orient(aln.rstart, aln.rstop, aln.rlen);
       aln.score1 = aln_results.top_scores[i];
-      aln.score2 = 0;
+      aln.score2 = s2 ? s2[i] : 0;
       aln.mismatches = 0;
       aln.identity = (int8_t)(100 * aln.score1 / aln_scoring.match / aln.rlen);
       aln.read_group_id = (int16_t)b.read_group_id;
@@ -403,7 +408,8 @@ Workload make_workload(uint64_t seed, int64_t nreads, int kmer_len) {
 void usage() {
   std::fprintf(stderr,
                "usage: mini_klign --mode gpu|cpu --out FILE [--seed S] [--reads N] [--block B] [--mem BYTES]\n"
-               "                  [--no-steal] [--gpu-delay-ms D] [--scoring m,x,o,e,a] [--ssw LIB] [--oracle LIB]\n");
+               "                  [--no-steal] [--gpu-delay-ms D] [--scoring m,x,o,e,a] [--ssw LIB] [--oracle LIB]\n"
+               "                  [--gpu-score2]\n");
 }
 
 }  // namespace
@@ -412,7 +418,7 @@ int main(int argc, char **argv) {
   std::string mode = "gpu", out_path, ssw_lib, oracle_lib;
   uint64_t seed = 11;
   int64_t nreads = 40000, block = KLIGN_GPU_BLOCK_SIZE, mem = 0;
-  bool steal = true;
+  bool steal = true, k_gpu_score2 = false;
   int delay = 0;
   AlnScoring sc{1, 3, 5, 2, 2};
   const int kmer_len = 21;
@@ -432,6 +438,7 @@ int main(int argc, char **argv) {
     else if (a == "--block") block = std::strtoll(val().c_str(), nullptr, 10);
     else if (a == "--mem") mem = std::strtoll(val().c_str(), nullptr, 10);
     else if (a == "--no-steal") steal = false;
+    else if (a == "--gpu-score2") k_gpu_score2 = true;
     else if (a == "--gpu-delay-ms") delay = std::atoi(val().c_str());
     else if (a == "--ssw") ssw_lib = val();
     else if (a == "--oracle") oracle_lib = val();
@@ -482,6 +489,8 @@ int main(int argc, char **argv) {
                       300);
     // klign's build passes N's cost through ALN_AMBIGUITY_COST; this harness sets it explicitly
     k.gpu_driver.set_ambiguity((short)sc.ambiguity);
+    if (k_gpu_score2) k.gpu_driver.set_score2(true);
+    k.gpu_score2 = k_gpu_score2;
   } else {
     k.gpu_devices = 0;
     k.gpu_mem_avail = mem > 0 ? (size_t)mem : 32 * 1024 * 1024;

@@ -52,14 +52,17 @@ def run_harness(tmp_path, name, *args):
     return summary, rows
 
 
-def compare(expected, got):
+def compare(expected, got, gpu_score2=False):
     assert len(expected) == len(got)
     bad = []
     for e, g in zip(expected, got):
         if [e[c] for c in COORD_COLS] != [g[c] for c in COORD_COLS]:
             bad.append((e, g))
             continue
-        if g[13] == "G":
+        if g[13] == "G" and gpu_score2:
+            if (g[10], g[12]) != (e[10], e[12]):   # every field as the CPU path writes it
+                bad.append((e, g))
+        elif g[13] == "G":
             if g[10] != "0" or g[12] != "0":
                 bad.append((e, g))
         elif e[10] != "-1" and g[10] != "-1" and (e[10], e[12]) != (g[10], g[12]):
@@ -143,3 +146,19 @@ def test_klign_gpu_memory_dispatch_rule(tmp_path):
     print(json.dumps(got_s), flush=True)
     assert got_s["stolen"] == 0 and got_s["blocks"] > 20
     compare(exp, got)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scoring", ["1,3,5,2,2", "1,1,1,1,1"])
+def test_klign_gpu_score2_records_identical(tmp_path, scoring):
+    """With the driver's score2 on, a GPU-aligned Aln equals the CPU-aligned
+    one in every field (SSW reports 0 mismatches without a CIGAR)."""
+    _need_harness()
+    if not have_ref():
+        pytest.skip("needs the reference SSW for the CPU side (its score2)")
+    exp_s, exp = run_harness(tmp_path, "cpu", "--mode", "cpu", "--reads", 30000, "--seed", 13, "--scoring", scoring)
+    got_s, got = run_harness(tmp_path, "gpu", "--mode", "gpu", "--reads", 30000, "--seed", 13, "--scoring", scoring,
+                             "--gpu-score2", "--block", 4000, "--gpu-delay-ms", 5)
+    print(json.dumps(got_s), flush=True)
+    assert got_s["gpu"] > 0 and got_s["stolen"] > 0
+    compare(exp, got, gpu_score2=True)
