@@ -98,6 +98,8 @@ struct mhmsw_ctx {
   int64_t n_fast = 0, n_plain = 0;
   DevBuf<PairEnd> d_fwd, d_rev;
   DevBuf<int32_t> d_s2;
+  std::vector<int64_t> h_base;  // joined per-sequence tables of a multi-part job
+  std::vector<int32_t> h_len, h_win_ctg, h_win_off;
   DevBuf<SwTask> d_tasks;
   DevBuf<PassSched> d_sched;
   DevBuf<AlnResult> d_results;
@@ -429,8 +431,11 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   };
   // pass 1, over contiguous chunks of pairs: validate, lay out each chunk from
   // offset 0, route, and sum
+  // String batches interleave each pair's read and window.  Windowed batches
+  // put every read first and every window after them: only the reads region
+  // is packed on the host and copied, the windows are extracted on the device.
   struct Chunk {
-    int64_t p0 = 0, p1 = 0, bases = 0, cells = 0, nfast = 0, nplain = 0;
+    int64_t p0 = 0, p1 = 0, bases = 0, wbases = 0, cells = 0, nfast = 0, nplain = 0;
     int max_plain_cols = 0, max_rl = 0, max_cl = 0;
     bool used[kNumClasses] = {};
     int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
@@ -448,7 +453,7 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
     char m[256];
     for (int k; (k = next_chunk.fetch_add(1)) < nchunks;) {
       Chunk& c = ch[(size_t)k];
-      int64_t acc = 0;
+      int64_t acc = 0, accw = 0;
       for (int64_t p = c.p0; p < c.p1; p++) {
         const int32_t rl = in.read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
         m[0] = 0;
@@ -475,8 +480,13 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
         pb.len[(size_t)(2 * p + 1)] = cl;
         pb.base[(size_t)(2 * p)] = acc;
         acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-        pb.base[(size_t)(2 * p + 1)] = acc;
-        acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        if (windowed) {
+          pb.base[(size_t)(2 * p + 1)] = accw;
+          accw += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        } else {
+          pb.base[(size_t)(2 * p + 1)] = acc;
+          acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        }
         c.cells += (int64_t)rl * cl;
         c.max_plain_cols = std::max(c.max_plain_cols, (int)std::min(rl, cl));
         c.max_rl = std::max(c.max_rl, (int)rl);
@@ -495,11 +505,12 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
         }
       }
       c.bases = acc;
+      c.wbases = accw;
     }
   });
   // the first invalid pair fails the batch; otherwise chunk offsets and totals
   int64_t acc = kPadBases;
-  std::vector<int64_t> chunk_base((size_t)nchunks);
+  std::vector<int64_t> chunk_base((size_t)nchunks), chunk_wbase((size_t)nchunks);
   for (int k = 0; k < nchunks; k++) {
     const Chunk& c = ch[(size_t)k];
     if (c.bad >= 0) {
@@ -520,33 +531,41 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
     pb.nfast += c.nfast;
     pb.nplain += c.nplain;
   }
+  const int64_t reads_end = acc;  // windowed: the packed region ends here
+  for (int k = 0; k < nchunks; k++) {
+    chunk_wbase[(size_t)k] = acc;
+    acc += ch[(size_t)k].wbases;
+  }
   const int64_t body_end = acc;
   acc += kPadBases;
   pb.total = (acc + 31) / 32 * 32;
-  pb.two.resize((size_t)(pb.total / 16));
-  pb.nm.resize((size_t)(pb.total / 32));
-  // pads before the first and after the last sequence
+  const int64_t packed_end = windowed ? reads_end : pb.total;
+  pb.two.resize((size_t)(packed_end / 16));
+  pb.nm.resize((size_t)(packed_end / 32));
+  // pads before the first and after the last packed sequence
   std::fill(pb.two.begin(), pb.two.begin() + kPadBases / 16, 0u);
   std::fill(pb.nm.begin(), pb.nm.begin() + kPadBases / 32, 0u);
-  std::fill(pb.two.begin() + body_end / 16, pb.two.end(), 0u);
-  std::fill(pb.nm.begin() + body_end / 32, pb.nm.end(), 0u);
-  // pass 2, per chunk: final offsets, then zero each sequence's 32-base-rounded
-  // span and pack it (window spans of a windowed batch are only zeroed: the
-  // device extracts them from the resident contigs)
+  if (!windowed) {
+    std::fill(pb.two.begin() + body_end / 16, pb.two.end(), 0u);
+    std::fill(pb.nm.begin() + body_end / 32, pb.nm.end(), 0u);
+  }
+  // pass 2, per chunk: final offsets, then zero each packed sequence's
+  // 32-base-rounded span and pack it (the windows of a windowed batch are not
+  // packed: the device extracts them from the resident contigs)
   next_chunk = 0;
   run([&](int) {
     for (int k; (k = next_chunk.fetch_add(1)) < nchunks;) {
       const Chunk& c = ch[(size_t)k];
-      const int64_t off = chunk_base[(size_t)k];
+      const int64_t off = chunk_base[(size_t)k], woff = windowed ? chunk_wbase[(size_t)k] : off;
       for (int64_t s = 2 * c.p0; s < 2 * c.p1; s++) {
-        const int64_t b0 = (pb.base[(size_t)s] += off);
+        const int64_t b0 = (pb.base[(size_t)s] += (s & 1) ? woff : off);
+        if ((s & 1) && windowed) continue;  // extracted on the device
         const int32_t len = pb.len[(size_t)s];
         const int64_t span = ((int64_t)len + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
         uint32_t* two = pb.two.data() + b0 / 16;
         uint32_t* nb = pb.nm.data() + b0 / 32;
         std::fill(two, two + span / 16, 0u);
         std::fill(nb, nb + span / 32, 0u);
-        if ((s & 1) && windowed) continue;
         const char* src = (s & 1) ? in.ctgs + in.ctg_off[s >> 1] : in.reads + in.read_off[s >> 1];
         pack_seq(src, len, two, nb);
       }
@@ -597,9 +616,19 @@ static void reroute(mhmsw_ctx* c) {
 
 // Copy a packed batch to the context's device buffers; returns when the host
 // buffers may be released.
-static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
-  if (pb.res && pb.res->device != c->device)
-    return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", pb.res->device, c->device);
+// Copy packed batches to the context as one job: the parts follow each other
+// in the device layout (every part's total is a multiple of 32 bases, so
+// each sequence keeps its 32-base alignment) and only each part's packed
+// region crosses PCIe (windowed batches pack their reads only).  Returns when
+// the host buffers may be released.
+static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t nparts) {
+  if (nparts == 0) return fail(c, MHMSW_E_ARG, "no batch");
+  const mhmsw_contigs* res = parts[0]->res;
+  for (size_t k = 0; k < nparts; k++) {
+    if (parts[k]->res != res) return fail(c, MHMSW_E_ARG, "parts of one job use different resident contigs");
+  }
+  if (res && res->device != c->device)
+    return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", res->device, c->device);
   if (c->in_flight) {
     int rc = mhmsw_wait(c);
     if (rc != MHMSW_OK) return rc;
@@ -607,27 +636,35 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
   HIP_OR_FAIL(c, hipSetDevice(c->device));
   c->staged = false;
   c->have_results = false;
-  const int64_t npairs = pb.npairs, nseq = 2 * npairs, total = pb.total;
+  int64_t npairs = 0, total = 0;
+  c->cells = 0;
+  c->max_plain_cols = c->max_read_len = c->max_ctg_len = 0;
+  c->n_fast = c->n_plain = 0;
+  std::memset(c->fast_class_used, 0, sizeof(c->fast_class_used));
+  std::memset(c->class_pairs, 0, sizeof(c->class_pairs));
+  std::memset(c->class_cells, 0, sizeof(c->class_cells));
+  for (size_t k = 0; k < nparts; k++) {
+    const PackedBatch& pb = *parts[k];
+    npairs += pb.npairs;
+    total += pb.total;
+    c->cells += pb.cells;
+    c->max_plain_cols = std::max(c->max_plain_cols, pb.max_plain_cols);
+    c->max_read_len = std::max(c->max_read_len, pb.max_read_len);
+    c->max_ctg_len = std::max(c->max_ctg_len, pb.max_ctg_len);
+    c->n_fast += pb.nfast;
+    c->n_plain += pb.nplain;
+    for (int q = 0; q < kNumClasses; q++) {
+      c->fast_class_used[q] = c->fast_class_used[q] || pb.used[q];
+      c->class_pairs[q] += pb.cpairs[q];
+      c->class_cells[q] += pb.ccells[q];
+    }
+  }
+  const int64_t nseq = 2 * npairs;
   c->npairs = npairs;
   c->total = total;
-  c->cells = pb.cells;
-  c->max_plain_cols = pb.max_plain_cols;
-  c->max_read_len = pb.max_read_len;
-  c->max_ctg_len = pb.max_ctg_len;
-  c->n_fast = pb.nfast;
-  c->n_plain = pb.nplain;
-  std::memcpy(c->fast_class_used, pb.used, sizeof(pb.used));
-  std::memcpy(c->class_pairs, pb.cpairs, sizeof(pb.cpairs));
-  std::memcpy(c->class_cells, pb.ccells, sizeof(pb.ccells));
-  c->lens.resize((size_t)(2 * npairs));
-  for (int64_t p = 0; p < npairs; p++) {
-    c->lens[(size_t)(2 * p)] = pb.len[(size_t)(2 * p)];          // read
-    c->lens[(size_t)(2 * p + 1)] = pb.len[(size_t)(2 * p + 1)];  // contig window
-  }
-  const size_t n2 = pb.two.size(), nm = pb.nm.size();
   const size_t np1 = (size_t)std::max<int64_t>(npairs, 1);
-  HIP_OR_FAIL(c, c->d_packed2.ensure(n2));
-  HIP_OR_FAIL(c, c->d_nmask.ensure(nm));
+  HIP_OR_FAIL(c, c->d_packed2.ensure((size_t)(total / 16 + 4)));
+  HIP_OR_FAIL(c, c->d_nmask.ensure((size_t)(total / 32 + 4)));
   HIP_OR_FAIL(c, c->d_codes.ensure((size_t)(2 * total / 8 + 64)));
   HIP_OR_FAIL(c, c->d_s2.ensure(np1));
   HIP_OR_FAIL(c, c->d_base.ensure((size_t)std::max<int64_t>(nseq, 1)));
@@ -638,24 +675,70 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
   HIP_OR_FAIL(c, c->d_rev.ensure(np1));
   HIP_OR_FAIL(c, c->d_results.ensure(np1));
   HIP_OR_FAIL(c, c->d_tasks.ensure(np1 + 64 * kNumClasses + 64));
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr, pb.two.data(), n2 * 4, hipMemcpyHostToDevice, c->stream));
-  HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr, pb.nm.data(), nm * 4, hipMemcpyHostToDevice, c->stream));
-  if (nseq > 0) {
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, pb.base.data(), (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, pb.len.data(), (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
+  // per-sequence tables: one part is copied as it is, several are joined with
+  // their bases shifted by the parts before them
+  c->lens.resize((size_t)nseq);
+  const int64_t* hbase = parts[0]->base.data();
+  const int32_t* hlen = parts[0]->len.data();
+  const int32_t* hwc = parts[0]->win_ctg.data();
+  const int32_t* hwo = parts[0]->win_off.data();
+  if (nparts > 1) {
+    c->h_base.resize((size_t)std::max<int64_t>(nseq, 1));
+    c->h_len.resize((size_t)std::max<int64_t>(nseq, 1));
+    if (res) {
+      c->h_win_ctg.resize(np1);
+      c->h_win_off.resize(np1);
+    }
+    int64_t sh = 0, ps = 0;
+    for (size_t k = 0; k < nparts; k++) {
+      const PackedBatch& pb = *parts[k];
+      for (int64_t q = 0; q < 2 * pb.npairs; q++) {
+        c->h_base[(size_t)(2 * ps + q)] = pb.base[(size_t)q] + sh;
+        c->h_len[(size_t)(2 * ps + q)] = pb.len[(size_t)q];
+      }
+      if (res && pb.npairs > 0) {
+        std::memcpy(c->h_win_ctg.data() + ps, pb.win_ctg.data(), (size_t)pb.npairs * 4);
+        std::memcpy(c->h_win_off.data() + ps, pb.win_off.data(), (size_t)pb.npairs * 4);
+      }
+      sh += pb.total;
+      ps += pb.npairs;
+    }
+    hbase = c->h_base.data();
+    hlen = c->h_len.data();
+    hwc = c->h_win_ctg.data();
+    hwo = c->h_win_off.data();
   }
-  c->res = pb.res;
-  if (pb.res && npairs > 0) {
+  std::memcpy(c->lens.data(), hlen, (size_t)nseq * sizeof(int32_t));
+  int64_t sh = 0;
+  for (size_t k = 0; k < nparts; k++) {
+    const PackedBatch& pb = *parts[k];
+    if (!pb.two.empty())
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr + sh / 16, pb.two.data(), pb.two.size() * 4,
+                                    hipMemcpyHostToDevice, c->stream));
+    if (!pb.nm.empty())
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr + sh / 32, pb.nm.data(), pb.nm.size() * 4, hipMemcpyHostToDevice,
+                                    c->stream));
+    sh += pb.total;
+  }
+  if (nseq > 0) {
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, hbase, (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, hlen, (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  c->res = res;
+  if (res && npairs > 0) {
     HIP_OR_FAIL(c, c->d_win_ctg.ensure((size_t)npairs));
     HIP_OR_FAIL(c, c->d_win_off.ensure((size_t)npairs));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, pb.win_ctg.data(), (size_t)npairs * 4, hipMemcpyHostToDevice,
-                                  c->stream));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, pb.win_off.data(), (size_t)npairs * 4, hipMemcpyHostToDevice,
-                                  c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, hwc, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, hwo, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
   }
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   c->staged = true;
   return MHMSW_OK;
+}
+
+static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
+  const PackedBatch* one = &pb;
+  return upload_parts(c, &one, 1);
 }
 
 static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
@@ -1175,67 +1258,17 @@ int64_t estimate_bytes(const Request& r) {
 }
 
 // Requests queued together run as one job: their packed batches are
-// concatenated (every part's total is a multiple of 32 bases, so each
-// sequence keeps its 32-base alignment) and the results split afterwards.
+// uploaded one after another (upload_parts) and the results split afterwards.
 // Pairs are independent, so a request's results do not depend on its company.
 struct Job {
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<int64_t> first;  // first pair of each request in the job
   int64_t npairs = 0;
   int64_t est = 0;
-  PackedBatch merged;  // used when the job holds more than one request
   std::vector<mhmsw_result> results;
 };
 
 constexpr int64_t kMergeMaxPairs = (int64_t)1 << 20;
-
-void merge_packed(const std::vector<std::shared_ptr<Request>>& parts, PackedBatch& out) {
-  out = PackedBatch();
-  int64_t np = 0, tot = 0;
-  for (const auto& r : parts) {
-    np += r->packed.npairs;
-    tot += r->packed.total;
-  }
-  out.npairs = np;
-  out.total = tot;
-  out.res = parts[0]->packed.res;
-  out.base.resize((size_t)std::max<int64_t>(2 * np, 1));
-  out.len.resize((size_t)std::max<int64_t>(2 * np, 1));
-  out.two.assign((size_t)(tot / 16), 0u);
-  out.nm.assign((size_t)(tot / 32), 0u);
-  const bool windowed = out.res != nullptr;
-  if (windowed) {
-    out.win_ctg.reserve((size_t)np);
-    out.win_off.reserve((size_t)np);
-  }
-  int64_t sh = 0, ps = 0;
-  for (const auto& r : parts) {
-    const PackedBatch& b = r->packed;
-    for (int64_t k = 0; k < 2 * b.npairs; k++) {
-      out.base[(size_t)(2 * ps + k)] = b.base[(size_t)k] + sh;
-      out.len[(size_t)(2 * ps + k)] = b.len[(size_t)k];
-    }
-    std::copy(b.two.begin(), b.two.end(), out.two.begin() + sh / 16);
-    std::copy(b.nm.begin(), b.nm.end(), out.nm.begin() + sh / 32);
-    if (windowed) {
-      out.win_ctg.insert(out.win_ctg.end(), b.win_ctg.begin(), b.win_ctg.end());
-      out.win_off.insert(out.win_off.end(), b.win_off.begin(), b.win_off.end());
-    }
-    out.cells += b.cells;
-    out.max_plain_cols = std::max(out.max_plain_cols, b.max_plain_cols);
-    out.max_read_len = std::max(out.max_read_len, b.max_read_len);
-    out.max_ctg_len = std::max(out.max_ctg_len, b.max_ctg_len);
-    for (int k = 0; k < kNumClasses; k++) {
-      out.used[k] = out.used[k] || b.used[k];
-      out.cpairs[k] += b.cpairs[k];
-      out.ccells[k] += b.ccells[k];
-    }
-    out.nfast += b.nfast;
-    out.nplain += b.nplain;
-    sh += b.total;
-    ps += b.npairs;
-  }
-}
 
 }  // namespace
 
@@ -1335,13 +1368,10 @@ static void server_loop(mhmsw_server* s) {
       s->used += j->est;
       lk.unlock();
       mhmsw_ctx* c = s->slots[(size_t)freek];
-      const PackedBatch* pb = &j->reqs[0]->packed;
-      if (j->reqs.size() > 1) {
-        merge_packed(j->reqs, j->merged);
-        pb = &j->merged;
-      }
-      int rc = upload_batch(c, *pb);
-      j->merged = PackedBatch();
+      std::vector<const PackedBatch*> parts;
+      parts.reserve(j->reqs.size());
+      for (auto& r : j->reqs) parts.push_back(&r->packed);
+      int rc = upload_parts(c, parts.data(), parts.size());
       if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
       lk.lock();
       progressed = true;
