@@ -9,7 +9,8 @@
 //         Filter(report_begin_position=true, report_cigar=false, 0, 32767),
 //         &aln, maskLen = max(|ref| / 2, 15))
 // and returns score, ref_begin/end and query_begin/end per pair (and, from
-// the *2 / _one entries, sw_score_next_best).  Used to
+// the *2 / _one entries, sw_score_next_best); ssw_ref_cigar_batch makes the
+// same call with report_cigar = true, as klign does when it reports CIGARs.  Used to
 // record golden vectors and, on the host cores of the GPU machine, as the
 // "reference" CPU baseline.
 #include <pthread.h>
@@ -45,6 +46,65 @@ int ssw_ref_align_batch2(int64_t npairs, const char *a, const int64_t *a_off, co
  * out[7] = sw_score, sw_score_next_best, ref_begin, ref_end, query_begin,
  * query_end, mismatches.  Returns 0, or -1 when Align refuses the input. */
 int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc, int32_t *out);
+
+/* klign's call when it reports CIGARs (report_cigar = true, klign.cpp:435):
+ * results, Alignment::mismatches, and Alignment::cigar (BAM-encoded ops) of
+ * pair p at ops[p * maxops ..] with their count in nops[p] (only the first
+ * maxops are written when there are more).  A pair that Align refuses gets
+ * score -1 and nops -1. */
+int ssw_ref_cigar_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                        const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
+                        int32_t *mismatches, int32_t *nops, uint32_t *ops, int maxops, int nthreads);
+}
+
+int ssw_ref_cigar_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                        const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
+                        int32_t *mismatches, int32_t *nops, uint32_t *ops, int maxops, int nthreads) {
+  if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+  std::atomic<int64_t> next(0);
+  auto work = [&]() {
+    StripedSmithWaterman::Aligner aligner((uint8_t)sc->match, (uint8_t)sc->mismatch, (uint8_t)sc->gap_open,
+                                          (uint8_t)sc->gap_ext, (uint8_t)sc->ambiguity);
+    StripedSmithWaterman::Filter filter(true, true, 0, 32767);
+    std::string q, r;
+    for (;;) {
+      int64_t p0 = next.fetch_add(16);
+      if (p0 >= npairs) break;
+      int64_t p1 = std::min<int64_t>(p0 + 16, npairs);
+      for (int64_t p = p0; p < p1; p++) {
+        r.assign(a + a_off[p], (size_t)a_len[p]);
+        q.assign(b + b_off[p], (size_t)b_len[p]);
+        StripedSmithWaterman::Alignment aln;
+        aln.Clear();
+        int mask_len = std::max((int)(r.size() / 2), 15);
+        bool ok = aligner.Align(q.c_str(), r.c_str(), (int)r.size(), filter, &aln, mask_len);
+        ssw_ref_result &o = out[p];
+        if (!ok) {
+          o.score = -1;
+          o.ref_begin = o.ref_end = o.query_begin = o.query_end = -2;
+          mismatches[p] = -1;
+          nops[p] = -1;
+          continue;
+        }
+        o.score = aln.sw_score;
+        o.ref_begin = aln.ref_begin;
+        o.ref_end = aln.ref_end;
+        o.query_begin = aln.query_begin;
+        o.query_end = aln.query_end;
+        mismatches[p] = aln.mismatches;
+        nops[p] = (int32_t)aln.cigar.size();
+        for (size_t k = 0; k < aln.cigar.size() && (int)k < maxops; k++) ops[(size_t)p * maxops + k] = aln.cigar[k];
+      }
+    }
+  };
+  if (nthreads == 1) {
+    work();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+  }
+  return 0;
 }
 
 int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc, int32_t *out) {

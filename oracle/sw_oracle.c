@@ -250,3 +250,235 @@ int oracle_align_batch2(int64_t npairs, const char *a, const int64_t *a_off, con
   free(th);
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * CIGAR and mismatch count, as klign gets them when it reports CIGARs
+ * (Filter report_begin_position = report_cigar = true, score_filter 0,
+ * distance_filter 32767: klign.cpp:435 with ssw.hpp:59-64).
+ *
+ *  - ssw_align's CIGAR step (ssw_core.cpp:1180-1192): the rectangle from the
+ *    begin to the end point, rows over the SSW-query segment and columns over
+ *    the SSW-ref segment, goes through banded_sw with band half-width
+ *    |columns - rows| + 1; no CIGAR when either span exceeds the distance
+ *    filter (:1180);
+ *  - banded_sw (ssw_core.cpp:854-1053): local affine DP restricted to
+ *    |j - i| <= w, every cell outside the band (and above row 0 or left of
+ *    column 0) counting as H = E = F = 0, E and F not clamped, H =
+ *    max(max(E,0), max(F,0), H(i-1,j-1) + s).  Choices: E opened from H when
+ *    H - gO > E - gE strictly (else extended), F likewise; H takes the diagonal
+ *    when max(E,F clamped) <= diagonal, else E when E > F strictly, else F.
+ *    The band doubles until its maximum reaches the score.  The traceback
+ *    starts at the last cell in state H and follows the choices while the
+ *    row index is above 0 (diagonal = M, E = I, F = D); the cell where it
+ *    stops counts as one more M;
+ *  - ConvertAlignment (ssw.cpp:57-93): soft clips of the SSW-query before the
+ *    begin and after the end;
+ *  - CalculateNumberMismatch (ssw.cpp:129-254): each M position compares the
+ *    translated codes under the CIGAR's own walk from (ref_begin,
+ *    query_begin) -- '=' when equal (N against N too), 'X' otherwise --
+ *    and mismatches = X + I + D positions.
+ *
+ * A score-0 pair has ref_begin = -1, so SSW compares SSW-query[0] with the
+ * byte before its translated SSW-ref buffer: the top byte of glibc's chunk
+ * size word, 0 (code A) for every allocation below 2^56 bytes.  That byte is
+ * taken as 0 here.
+ * ---------------------------------------------------------------------- */
+enum { CG_M = 0, CG_I = 1, CG_D = 2, CG_S = 4, CG_EQ = 7, CG_X = 8 };
+#define CG_WRITTEN 0x80u
+#define CG_E_OPEN 0x08u
+#define CG_F_OPEN 0x10u
+
+/* One band pass; dir[i * rl + j] gets the choices of every band cell: bits
+ * 0-2 the value SSW's traceback reads in state H (1 diagonal, 2/3 E
+ * extended/opened, 4/5 F extended/opened), CG_E_OPEN, CG_F_OPEN.  Returns the
+ * band's maximum H. */
+static int32_t cigar_band(const int8_t *R, int rl, const int8_t *Q, int ql, int bw, const oracle_scoring *sc,
+                          uint8_t *dir, int32_t *hp, int32_t *ep, int32_t *hc, int32_t *ec) {
+  const int32_t gO = sc->gap_open, gE = sc->gap_ext;
+  int32_t best = 0;
+  memset(dir, 0, (size_t)ql * (size_t)rl);
+  for (int j = 0; j < rl; j++) { hp[j] = 0; ep[j] = 0; }
+  for (int i = 0; i < ql; i++) {
+    const int beg = i - bw > 0 ? i - bw : 0;
+    const int end = i + bw < rl - 1 ? i + bw : rl - 1;
+    for (int j = 0; j < rl; j++) { hc[j] = 0; ec[j] = 0; }
+    int32_t f = 0, hleft = 0;
+    for (int j = beg; j <= end; j++) {
+      /* hp/ep hold row i-1 with zeros outside its band (all zeros for i = 0) */
+      const int32_t hup = hp[j], eup = ep[j];
+      const int32_t hdiag = j > 0 ? hp[j - 1] : 0;
+      const int32_t eo = hup - gO, ee = eup - gE;
+      const int32_t e = eo > ee ? eo : ee;
+      const int32_t fo = hleft - gO, fe = f - gE;
+      f = fo > fe ? fo : fe;
+      const int32_t e1 = e > 0 ? e : 0, f1 = f > 0 ? f : 0;
+      const int32_t t1 = e1 > f1 ? e1 : f1;
+      const int32_t t2 = hdiag + subst(R[j], Q[i], sc);
+      const int32_t h = t1 > t2 ? t1 : t2;
+      uint8_t v;
+      if (t1 <= t2) v = 1;
+      else if (e1 > f1) v = eo > ee ? 3 : 2;
+      else v = fo > fe ? 5 : 4;
+      dir[(size_t)i * rl + j] = (uint8_t)(v | CG_WRITTEN | (eo > ee ? CG_E_OPEN : 0) | (fo > fe ? CG_F_OPEN : 0));
+      hc[j] = h;
+      ec[j] = e;
+      hleft = h;
+      if (h > best) best = h;
+    }
+    int32_t *t = hp; hp = hc; hc = t;
+    t = ep; ep = ec; ec = t;
+  }
+  return best;
+}
+
+typedef struct {
+  uint32_t *ops;
+  int cap, n;
+  int cur_op;
+  uint32_t cur_len;
+  int32_t mism;
+} cigar_out;
+
+static void cg_flush(cigar_out *o) {
+  if (o->cur_len == 0) return;
+  if (o->n < o->cap) o->ops[o->n] = (o->cur_len << 4) | (uint32_t)o->cur_op;
+  o->n++;
+  if (o->cur_op == CG_X || o->cur_op == CG_I || o->cur_op == CG_D) o->mism += (int32_t)o->cur_len;
+  o->cur_len = 0;
+}
+
+static void cg_push(cigar_out *o, int op, uint32_t len) {
+  if (len == 0) return;
+  if (o->cur_len && o->cur_op != op) cg_flush(o);
+  o->cur_op = op;
+  o->cur_len += len;
+}
+
+int oracle_cigar_codes(const int8_t *ref, int m, const int8_t *query, int n, const oracle_scoring *sc,
+                       const oracle_result *r, uint32_t *ops, int cap, int32_t *mismatches) {
+  cigar_out o = {ops, cap, 0, 0, 0, 0};
+  *mismatches = 0;
+  if (n <= 0) return 0; /* SSW refuses an empty query (ssw.cpp:450) */
+  const int rb = r->ref_begin, re = r->ref_end, qb = r->query_begin, qe = r->query_end;
+  if (re - rb > 32767 || qe - qb > 32767) return 0; /* distance filter: no CIGAR, 0 mismatches */
+  const int rl = re - rb + 1, ql = qe - qb + 1;
+  /* trace steps in trace order: op and cell */
+  int nst = 0;
+  int32_t *st_op = NULL, *st_i = NULL, *st_j = NULL;
+  int jfin = 0;
+  if (r->score > 0) {
+    uint8_t *dir = (uint8_t *)malloc((size_t)ql * (size_t)rl);
+    int32_t *rows = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)rl);
+    int bw = (rl > ql ? rl - ql : ql - rl) + 1;
+    int32_t mx = 0;
+    for (;;) {
+      int32_t b = cigar_band(ref + rb, rl, query + qb, ql, bw, sc, dir, rows, rows + rl, rows + 2 * rl, rows + 3 * rl);
+      if (b > mx) mx = b;
+      if (mx >= r->score) break;
+      if (bw > 2 * (rl + ql) + 2) { free(dir); free(rows); return -1; } /* score unreachable: inconsistent input */
+      bw *= 2;
+    }
+    st_op = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(rl + ql + 2));
+    st_i = st_op + (rl + ql + 2);
+    st_j = st_i + (rl + ql + 2);
+    int i = ql - 1, j = rl - 1, state = 2; /* 0 E, 1 F, 2 H as in ssw_core.cpp:963-1009 */
+    while (i > 0) {
+      if (j < 0 || j - i > bw || i - j > bw) { free(dir); free(rows); free(st_op); return -1; }
+      const uint8_t d = dir[(size_t)i * rl + j];
+      if (!(d & CG_WRITTEN)) { free(dir); free(rows); free(st_op); return -1; }
+      int v = state == 2 ? (d & 7) : state == 0 ? ((d & CG_E_OPEN) ? 3 : 2) : ((d & CG_F_OPEN) ? 5 : 4);
+      st_i[nst] = i;
+      st_j[nst] = j;
+      switch (v) {
+        case 1: st_op[nst] = CG_M; i--; j--; state = 2; break;
+        case 2: st_op[nst] = CG_I; i--; state = 0; break;
+        case 3: st_op[nst] = CG_I; i--; state = 2; break;
+        case 4: st_op[nst] = CG_D; j--; state = 1; break;
+        case 5: st_op[nst] = CG_D; j--; state = 2; break;
+        default: free(dir); free(rows); free(st_op); return -1;
+      }
+      nst++;
+    }
+    jfin = j;
+    free(dir);
+    free(rows);
+  }
+  /* forward order: head clip, the cell the trace stopped at (one M), then the
+   * trace steps backwards; M positions compared along the CIGAR's own walk */
+  if (qb > 0) cg_push(&o, CG_S, (uint32_t)qb);
+  for (int k = nst; k >= 0; k--) {
+    const int op = k == nst ? CG_M : st_op[k];
+    const int i = k == nst ? 0 : st_i[k], j = k == nst ? jfin : st_j[k];
+    if (op == CG_M) {
+      const int rj = rb + j - jfin;
+      const int8_t rc = rj >= 0 ? ref[rj] : 0;
+      cg_push(&o, rc == query[qb + i] ? CG_EQ : CG_X, 1);
+    } else {
+      cg_push(&o, op, 1);
+    }
+  }
+  cg_flush(&o);
+  if (n - qe - 1 > 0) cg_push(&o, CG_S, (uint32_t)(n - qe - 1));
+  cg_flush(&o);
+  free(st_op);
+  *mismatches = o.mism;
+  return o.n;
+}
+
+typedef struct {
+  int64_t npairs;
+  const char *a, *b;
+  const int64_t *a_off, *b_off;
+  const int32_t *a_len, *b_len;
+  const oracle_scoring *sc;
+  oracle_result *out;
+  int32_t *mism, *nops;
+  uint32_t *ops;
+  int maxops;
+  atomic_llong next;
+} cigar_job;
+
+static void *cigar_worker(void *arg) {
+  cigar_job *job = (cigar_job *)arg;
+  for (;;) {
+    int64_t p = atomic_fetch_add(&job->next, 16);
+    if (p >= job->npairs) break;
+    int64_t pe = p + 16 < job->npairs ? p + 16 : job->npairs;
+    for (; p < pe; p++) {
+      int m = job->a_len[p], n = job->b_len[p];
+      int8_t *ca = (int8_t *)malloc((size_t)m + 1), *cb = (int8_t *)malloc((size_t)n + 1);
+      oracle_translate(job->a + job->a_off[p], m, ca);
+      oracle_translate(job->b + job->b_off[p], n, cb);
+      oracle_align_codes(ca, m, cb, n, job->sc, &job->out[p]);
+      job->nops[p] = oracle_cigar_codes(ca, m, cb, n, job->sc, &job->out[p], job->ops + (size_t)p * job->maxops,
+                                        job->maxops, &job->mism[p]);
+      free(ca);
+      free(cb);
+    }
+  }
+  return NULL;
+}
+
+int oracle_cigar_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                       const int64_t *b_off, const int32_t *b_len, const oracle_scoring *sc, oracle_result *out,
+                       int32_t *mismatches, int32_t *nops, uint32_t *ops, int maxops, int nthreads) {
+  cigar_job job;
+  job.npairs = npairs;
+  job.a = a; job.b = b;
+  job.a_off = a_off; job.b_off = b_off;
+  job.a_len = a_len; job.b_len = b_len;
+  job.sc = sc;
+  job.out = out;
+  job.mism = mismatches;
+  job.nops = nops;
+  job.ops = ops;
+  job.maxops = maxops;
+  atomic_init(&job.next, 0);
+  if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (nthreads < 1) nthreads = 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, cigar_worker, &job);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  free(th);
+  return 0;
+}

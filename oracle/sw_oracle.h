@@ -14,7 +14,12 @@
  *     under tests/golden/ by tests/golden/make_golden.py;
  *   - the 18 known-answer cases of test/ssw-test.cpp:151-238;
  *   - brute-force path enumeration on tiny inputs (tests/test_oracle_bruteforce.py),
- *     which defines the optimum and both tie-breaks from first principles.
+ *     which defines the optimum and both tie-breaks from first principles;
+ *   - for the CIGAR: CIGARs and mismatch counts recorded from the reference
+ *     (tests/golden/make_golden_cigar.py), and invariants no restatement
+ *     detail can satisfy by accident: the CIGAR spans exactly the query and
+ *     the aligned rectangle, rescoring it gives the score, and = / X agree
+ *     with the bases (tests/test_oracle_cigar.py).
  */
 #ifndef MHMSW_ORACLE_H
 #define MHMSW_ORACLE_H
@@ -69,6 +74,20 @@ int oracle_align_batch2(int64_t npairs,
  * score and end_i from the forward pass. */
 int oracle_score2_pad(int n, int32_t score, const oracle_scoring *sc);
 int32_t oracle_score2_rule(const int32_t *rowmax, int m, int32_t score, int32_t end_i, const oracle_scoring *sc);
+
+/* SSW's CIGAR and mismatch count for one pair whose result r came from
+ * oracle_align_codes (klign's call with report_cigar set).  Ops use BAM's
+ * encoding, length << 4 | op with op 0 M, 1 I, 2 D, 4 S, 7 '=', 8 X; at most
+ * cap are written.  Returns the number of ops (0 when SSW gives no CIGAR), or
+ * -1 when SSW's traceback would leave the band or fail. */
+int oracle_cigar_codes(const int8_t *ref, int m, const int8_t *query, int n, const oracle_scoring *sc,
+                       const oracle_result *r, uint32_t *ops, int cap, int32_t *mismatches);
+
+/* Batch on ASCII (as oracle_align_batch): results, mismatches[p], nops[p] and
+ * the ops of pair p at ops[p * maxops ..] (only the first maxops if more). */
+int oracle_cigar_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
+                       const int64_t *b_off, const int32_t *b_len, const oracle_scoring *sc, oracle_result *out,
+                       int32_t *mismatches, int32_t *nops, uint32_t *ops, int maxops, int nthreads);
 
 #ifdef __cplusplus
 }

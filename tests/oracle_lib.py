@@ -116,3 +116,49 @@ def ref_pairs(refs, queries, scoring, nthreads=0):
     a = pack_strings(refs)
     b = pack_strings(queries)
     return ref_batch(*a, *b, scoring, nthreads)
+
+
+CIGAR_OPS = "MIDNSHP=X"
+
+
+def cigar_string(ops):
+    """BAM-encoded ops -> CIGAR text, as ssw.cpp's ConvertAlignment prints them."""
+    return "".join(f"{int(o) >> 4}{CIGAR_OPS[int(o) & 0xF]}" for o in ops)
+
+
+def _cigar_run(path, name, a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, maxops, nthreads):
+    key = (path, name)
+    if key not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built")
+        f = getattr(ctypes.CDLL(path), name)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 6 + [ctypes.POINTER(Scoring)] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_int, ctypes.c_int]
+        _libs[key] = f
+    fn = _libs[key]
+    n = len(a_len)
+    if maxops is None:
+        maxops = int(2 * (max(np.max(a_len, initial=0), np.max(b_len, initial=0))) + 8)
+    out = np.zeros(n, dtype=RESULT_DTYPE)
+    mism = np.zeros(n, dtype=np.int32)
+    nops = np.zeros(n, dtype=np.int32)
+    ops = np.zeros((n, maxops), dtype=np.uint32)
+    sc = Scoring(*scoring)
+    arrs = [np.ascontiguousarray(x, dtype=d) for x, d in ((a_buf, np.uint8), (a_off, np.int64), (a_len, np.int32),
+                                                          (b_buf, np.uint8), (b_off, np.int64), (b_len, np.int32))]
+    fn(n, *[x.ctypes.data for x in arrs], ctypes.byref(sc), out.ctypes.data, mism.ctypes.data, nops.ctypes.data,
+       ops.ctypes.data, maxops, nthreads)
+    return out, mism, nops, ops
+
+
+def oracle_cigar_batch(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, maxops=None, nthreads=0):
+    """(results, mismatches, nops, ops[npairs, maxops]) from the oracle's CIGAR restatement."""
+    return _cigar_run(ORACLE_SO, "oracle_cigar_batch", a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, maxops,
+                      nthreads)
+
+
+def ref_cigar_batch(a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, maxops=None, nthreads=0):
+    """The same from the reference SSW called with report_cigar = true."""
+    return _cigar_run(REF_SO, "ssw_ref_cigar_batch", a_buf, a_off, a_len, b_buf, b_off, b_len, scoring, maxops,
+                      nthreads)
