@@ -79,6 +79,8 @@ typedef struct {
   int64_t class_pairs[MHMSW_NUM_CLASSES];
   int64_t class_cells[MHMSW_NUM_CLASSES];
   double class_ms[MHMSW_NUM_CLASSES];
+  double cigar_ms;          /* device time of the CIGAR pass (eager launches) */
+  int64_t cigar_reruns;     /* CIGAR passes rerun with larger pools */
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,
@@ -116,6 +118,13 @@ int mhmsw_set_mode(mhmsw_ctx *ctx, int mode);
  * klign.cpp:257), which klign's GPU path leaves at 0 (klign.cpp:378-379).
  * Needs every SSW-query sequence to be at most 8177 bases.  0 (default): off. */
 int mhmsw_set_score2(mhmsw_ctx *ctx, int enable);
+/* 1: jobs also compute SSW's CIGAR and mismatch count of every pair
+ * (Alignment::cigar and ::mismatches of klign's call with report_cigar set,
+ * klign.cpp:251-273 and :435: ssw_core.cpp:1180-1192 banded_sw :854-1053,
+ * ssw.cpp:57-93 ConvertAlignment, :129-254 CalculateNumberMismatch), which
+ * ADEPT's GPU path cannot give (klign.cpp:589-599 runs such batches on the
+ * CPU; its GPU mapping leaves them at 0 / "*", :378-385).  0 (default): off. */
+int mhmsw_set_cigar(mhmsw_ctx *ctx, int enable);
 /* 1 (default): the job of a staged batch (unpack, forward, reruns, reverse,
  * finalize: ~40 launches) is captured once as a hipGraph and replayed while
  * the staged geometry stays the same.  0: launches are issued one by one. */
@@ -148,6 +157,15 @@ int mhmsw_wait(mhmsw_ctx *ctx);
 int mhmsw_results(mhmsw_ctx *ctx, mhmsw_result *out, int64_t npairs);
 /* score2 of each pair of the last batch (which must have run with score2 on). */
 int mhmsw_results_score2(mhmsw_ctx *ctx, int32_t *out, int64_t npairs);
+/* CIGARs of the last batch (which must have run with the CIGAR pass on):
+ * mismatches[p] (may be NULL); op_off[0..npairs] prefix offsets of each
+ * pair's ops in ops (may be NULL); ops (may be NULL: only *total_ops and the
+ * other outputs are filled) receives every pair's ops in pair order, BAM
+ * encoding: length << 4 | op, op 0 M, 1 I, 2 D, 4 S, 7 '=', 8 X (SSW's
+ * cigar after its =/X split: soft clips of the SSW-query, '=', 'X', I, D);
+ * ops_cap = its capacity in ops. */
+int mhmsw_results_cigar(mhmsw_ctx *ctx, int64_t npairs, int32_t *mismatches, int64_t *op_off, uint32_t *ops,
+                        int64_t ops_cap, int64_t *total_ops);
 int mhmsw_get_stats(mhmsw_ctx *ctx, mhmsw_stats *out);
 
 /* --- device-resident contigs (serving; no counterpart in the reference,

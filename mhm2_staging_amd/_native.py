@@ -40,7 +40,7 @@ class Stats(ctypes.Structure):
                 ("job_ms", ctypes.c_double), ("fwd_fast_ms", ctypes.c_double), ("fwd_plain_ms", ctypes.c_double),
                 ("rev_ms", ctypes.c_double), ("prep_ms", ctypes.c_double),
                 ("class_pairs", ctypes.c_int64 * 17), ("class_cells", ctypes.c_int64 * 17),
-                ("class_ms", ctypes.c_double * 17)]
+                ("class_ms", ctypes.c_double * 17), ("cigar_ms", ctypes.c_double), ("cigar_reruns", ctypes.c_int64)]
 
     def as_dict(self):
         d = {}
@@ -82,9 +82,9 @@ class SynthBatch(ctypes.Structure):
 # every symbol include/mhmsw.h and include/mhmsw_synth.h declare
 EXPORTS = [
     "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
-    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_score2", "mhmsw_set_graphs",
+    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_score2", "mhmsw_set_cigar", "mhmsw_set_graphs",
     "mhmsw_set_tuning", "mhmsw_get_tuning", "mhmsw_autotune", "mhmsw_stage",
-    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_results_score2", "mhmsw_get_stats",
+    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_results_score2", "mhmsw_results_cigar", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
     "mhmsw_contigs_load", "mhmsw_contigs_free", "mhmsw_contigs_bytes", "mhmsw_stage_windows",
     "mhmsw_server_create", "mhmsw_server_submit", "mhmsw_server_status", "mhmsw_server_wait",
@@ -115,6 +115,7 @@ def lib():
         "mhmsw_set_orientation": [vp, ctypes.c_int],
         "mhmsw_set_mode": [vp, ctypes.c_int],
         "mhmsw_set_score2": [vp, ctypes.c_int],
+        "mhmsw_set_cigar": [vp, ctypes.c_int],
         "mhmsw_set_graphs": [vp, ctypes.c_int],
         "mhmsw_set_tuning": [vp, ctypes.POINTER(Tuning)],
         "mhmsw_get_tuning": [vp, ctypes.POINTER(Tuning)],
@@ -126,6 +127,7 @@ def lib():
         "mhmsw_wait": [vp],
         "mhmsw_results": [vp, vp, i64],
         "mhmsw_results_score2": [vp, vp, i64],
+        "mhmsw_results_cigar": [vp, i64, vp, vp, vp, i64, ctypes.POINTER(i64)],
         "mhmsw_get_stats": [vp, ctypes.POINTER(Stats)],
         "mhmsw_synth_generate": [ctypes.c_int, i64, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(SynthBatch)],
         "mhmsw_contigs_load": [ctypes.c_int, i64, vp, vp, vp, ctypes.POINTER(vp)],

@@ -79,6 +79,10 @@ class Aligner:
         """Also compute SSW's second-best score (sw_score_next_best) per pair."""
         self._check(self._L.mhmsw_set_score2(self._ctx, 1 if enable else 0), "set_score2")
 
+    def set_cigar(self, enable):
+        """Also compute SSW's CIGAR and mismatch count per pair (klign with report_cigar)."""
+        self._check(self._L.mhmsw_set_cigar(self._ctx, 1 if enable else 0), "set_cigar")
+
     def set_tuning(self, blocks_per_cu=8, g64_min_rows=257, bin_shift=5):
         t = _native.Tuning(int(blocks_per_cu), int(g64_min_rows), int(bin_shift))
         self._check(self._L.mhmsw_set_tuning(self._ctx, ctypes.byref(t)), "set_tuning")
@@ -133,6 +137,20 @@ class Aligner:
         self._check(self._L.mhmsw_results_score2(self._ctx, _ptr(out), n), "results_score2")
         return out
 
+    def cigars(self):
+        """(mismatches int32[n], op_off int64[n+1], ops uint32[total]) of the last batch, ops BAM-encoded
+        (length << 4 | op, op 0 M, 1 I, 2 D, 4 S, 7 =, 8 X) in pair order; needs set_cigar(True)."""
+        n = self._batch.npairs if self._batch is not None else 0
+        mism = np.zeros(n, dtype=np.int32)
+        off = np.zeros(n + 1, dtype=np.int64)
+        total = ctypes.c_int64(0)
+        self._check(self._L.mhmsw_results_cigar(self._ctx, n, _ptr(mism), _ptr(off), None, 0, ctypes.byref(total)),
+                    "results_cigar")
+        ops = np.zeros(max(int(total.value), 1), dtype=np.uint32)
+        self._check(self._L.mhmsw_results_cigar(self._ctx, n, None, None, _ptr(ops), int(total.value),
+                                                ctypes.byref(total)), "results_cigar")
+        return mism, off, ops[:int(total.value)]
+
     def stats(self):
         st = _native.Stats()
         self._check(self._L.mhmsw_get_stats(self._ctx, ctypes.byref(st)), "get_stats")
@@ -142,6 +160,14 @@ class Aligner:
         self.submit(batch)
         self.wait()
         return self.results()
+
+
+CIGAR_OPS = "MIDNSHP=X"
+
+
+def cigar_string(ops):
+    """BAM-encoded ops -> CIGAR text as SSW's Alignment::cigar_string spells it (src/ssw/ssw.cpp:57-93)."""
+    return "".join(f"{int(o) >> 4}{CIGAR_OPS[int(o) & 0xF]}" for o in ops)
 
 
 class AlignmentResults:

@@ -1,0 +1,341 @@
+// Per-pair CIGAR work shared by the HIP kernels (cigar_kernels.hip) and a
+// CPU check of the same code (tests/cpp/cigar_core_check.cpp).
+//
+// What it computes is SSW's CIGAR for klign's call with report_cigar set
+// (src/ssw/ssw_core.cpp:1180-1192 and banded_sw :854-1053, then
+// ConvertAlignment src/ssw/ssw.cpp:57-93 and CalculateNumberMismatch
+// :129-254):
+//   * rectangle: rows i over the SSW-query segment [qb, qe], columns j over
+//     the SSW-ref segment [rb, re]; band |j - i| <= w, w starting at
+//     |columns - rows| + 1 and doubling until the band holds the score;
+//   * band DP: every cell outside the band, above row 0 or left of column 0
+//     counts as H = E = F = 0; E(i,j) = max(H(i-1,j) - gO, E(i-1,j) - gE),
+//     F(i,j) = max(H(i,j-1) - gO, F(i,j-1) - gE), H = max(E, F, 0,
+//     H(i-1,j-1) + s).  E and F are kept clamped at 0: SSW keeps them
+//     unclamped but only uses max(E, 0) and max(F, 0) in H, and its
+//     open/extend choice for a cell differs from the clamped one only where
+//     the gap value is <= 0, which no traceback reads (a traceback enters a
+//     gap state only where that gap value is positive);
+//   * per cell 4 direction bits: bit 0 "H did not take the diagonal"
+//     (max(E, F) > H(i-1,j-1) + s), bit 1 "H took E" (E > F), bit 2 "E was
+//     opened" (H(i-1,j) - gO > E(i-1,j) - gE), bit 3 "F was opened";
+//   * the band holds the score exactly when the rectangle's last cell does:
+//     every cell of the rectangle is at most the forward score and only the
+//     end point reaches it (the end is the least such cell);
+//   * traceback from the last cell in state H while the row is above 0
+//     (diagonal = M, E = I, F = D); the cell where it stops is one more M.
+//     The CIGAR then walks from (rb, qb): M positions compare the codes
+//     ('=' when equal, N against N included, else 'X'), so when the
+//     traceback stops at column jf > 0 every compared SSW-ref position is
+//     shifted by jf, as in SSW;
+//   * soft clips: qb bases before, n - 1 - qe after; mismatches = X + I + D.
+//
+// Band cells are addressed by diagonal d = j - i + BW in [0, 2*BW]; a row
+// keeps H and E of its 2*BW+1 diagonals, so cell (i, d) reads (i-1, d) for
+// the diagonal, (i-1, d+1) for E and (i, d-1) for F.
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define MHMSW_HD __host__ __device__
+#else
+#define MHMSW_HD
+#endif
+
+namespace mhmsw {
+
+// BAM CIGAR op codes (MAPSTR "MIDNSHP=X", src/ssw/ssw_core.hpp:23)
+constexpr uint32_t kOpM = 0, kOpI = 1, kOpD = 2, kOpS = 4, kOpEq = 7, kOpX = 8;
+// SSW's distance filter for klign's Filter (src/ssw/ssw.hpp:59-64)
+constexpr int32_t kCigarDistanceFilter = 32767;
+// band classes: half-width 1..7 exact (class bw - 1), 8..15 in a 31-wide
+// band with the outer diagonals masked (class 7), wider in the generic class
+constexpr int kCigExactMaxBw = 7;
+constexpr int kCigMaskedBw = 15;
+constexpr int kCigClassMasked = 7;
+constexpr int kCigClassGeneric = 8;
+constexpr int kCigClasses = 9;
+// rectangles taller than this always take the generic class
+constexpr int32_t kCigMaxRows = 65536;
+
+MHMSW_HD inline int cigar_class(int32_t bw, int32_t rows) {
+  if (rows > kCigMaxRows) return kCigClassGeneric;
+  if (bw <= kCigExactMaxBw) return bw - 1;
+  if (bw <= kCigMaskedBw) return kCigClassMasked;
+  return kCigClassGeneric;
+}
+
+MHMSW_HD inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+MHMSW_HD inline int32_t imax3(int32_t a, int32_t b, int32_t c) { return imax(imax(a, b), c); }
+
+// One pair's rectangle in SSW naming.
+struct CigarGeo {
+  int64_t rbase, qbase;  // nibble index of SSW-ref / SSW-query position 0
+  int32_t rb, re, qb, qe;
+  int32_t n;             // SSW-query length
+  int32_t score;
+  MHMSW_HD int32_t rl() const { return re - rb + 1; }
+  MHMSW_HD int32_t ql() const { return qe - qb + 1; }
+};
+
+MHMSW_HD inline uint32_t nib_at(const uint32_t* codes, int64_t idx) {
+  return (codes[idx >> 3] >> ((uint32_t)(idx & 7) * 4u)) & 0xFu;
+}
+
+// Sequential nibble reader that loads each dword once.
+struct NibCursor {
+  int64_t dw = -1;
+  uint32_t val = 0;
+  MHMSW_HD uint32_t get(const uint32_t* codes, int64_t idx) {
+    if ((idx >> 3) != dw) {
+      dw = idx >> 3;
+      val = codes[dw];
+    }
+    return (val >> ((uint32_t)(idx & 7) * 4u)) & 0xFu;
+  }
+};
+
+// Substitution scores of one row against SSW-ref codes 0..7, biased by +128,
+// as the 8 bytes (lo, hi) of a byte-select table: codes 0..3 score match or
+// -mismatch, 4 (N) -ambiguity, 5 (left of column 0) a value <= 0.  A row
+// whose code is N scores -ambiguity everywhere.
+MHMSW_HD inline void subst_table(uint32_t qc, int32_t match, int32_t mismatch, int32_t amb, uint32_t& lo,
+                                 uint32_t& hi) {
+  const uint32_t bx = (uint32_t)(128 - (qc < 4 ? mismatch : amb));
+  const uint32_t ba = (uint32_t)(128 - amb);
+  lo = bx * 0x01010101u;
+  if (qc < 4) lo = (lo & ~(0xFFu << (8 * qc))) | ((uint32_t)(128 + match) << (8 * qc));
+  hi = ba | (bx << 8);
+}
+
+MHMSW_HD inline int32_t subst_lookup(uint32_t lo, uint32_t hi, uint32_t rc) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return (int32_t)(__builtin_amdgcn_perm(hi, lo, rc) & 0xFFu) - 128;
+#else
+  const uint64_t t = ((uint64_t)hi << 32) | lo;
+  return (int32_t)((t >> (8 * rc)) & 0xFFu) - 128;
+#endif
+}
+
+constexpr uint32_t kCigKill = 5;  // SSW-ref code left of column 0
+
+// Band DP for half-width BW (template) and the pair's half-width bw (== BW
+// unless MASKED).  Writes 4 direction bits per cell: row i, diagonal d at
+// bit 4*(d % 8) of dword dirs[(i * DW + d / 8) * stride].  Returns H of the
+// rectangle's last cell.
+template <int BW, bool MASKED>
+MHMSW_HD int32_t cigar_band_dp(const uint32_t* codes, const CigarGeo& g, int32_t bw, int32_t match, int32_t mismatch,
+                               int32_t amb, int32_t gO, int32_t gE, uint32_t* dirs, int64_t stride) {
+  constexpr int W = 2 * BW + 1;
+  constexpr int DW = (4 * W + 31) / 32;
+  const int32_t rl = g.rl(), ql = g.ql();
+  int32_t H[W], E[W];
+  uint32_t rc[W];
+  NibCursor rcur, qcur;
+#pragma unroll
+  for (int d = 0; d < W; d++) {
+    H[d] = 0;
+    E[d] = 0;
+    const int32_t j = d - BW;
+    rc[d] = j < 0 ? kCigKill : rcur.get(codes, g.rbase + g.rb + j);
+  }
+  for (int32_t i = 0; i < ql; i++) {
+    uint32_t tlo, thi;
+    subst_table(qcur.get(codes, g.qbase + g.qb + i), match, mismatch, amb, tlo, thi);
+    int32_t hl = 0, fl = 0;
+    uint32_t dw[DW];
+#pragma unroll
+    for (int k = 0; k < DW; k++) dw[k] = 0;
+#pragma unroll
+    for (int d = 0; d < W; d++) {
+      const int32_t hu = d + 1 < W ? H[d + 1] : 0;
+      const int32_t eu = d + 1 < W ? E[d + 1] : 0;
+      const int32_t eo = hu - gO, ee = eu - gE;
+      int32_t e = imax3(eo, ee, 0);
+      const int32_t fo = hl - gO, fe = fl - gE;
+      int32_t f = imax3(fo, fe, 0);
+      const int32_t t2 = H[d] + subst_lookup(tlo, thi, rc[d]);
+      const int32_t t1 = imax(e, f);
+      int32_t h = imax(t1, t2);
+      const uint32_t bits = (t1 > t2 ? 1u : 0u) | (e > f ? 2u : 0u) | (eo > ee ? 4u : 0u) | (fo > fe ? 8u : 0u);
+      if (MASKED) {
+        const int32_t off = d > BW ? d - BW : BW - d;
+        if (off > bw) {
+          h = 0;
+          e = 0;
+          f = 0;
+        }
+      }
+      dw[(4 * d) / 32] |= bits << ((4 * d) % 32);
+      H[d] = h;
+      E[d] = e;
+      hl = h;
+      fl = f;
+    }
+#pragma unroll
+    for (int k = 0; k < DW; k++) dirs[((int64_t)i * DW + k) * stride] = dw[k];
+#pragma unroll
+    for (int d = 0; d < W - 1; d++) rc[d] = rc[d + 1];
+    rc[W - 1] = rcur.get(codes, g.rbase + g.rb + i + 1 + BW);
+  }
+  // last cell: row ql - 1, column rl - 1
+  const int32_t dc = (rl - 1) - (ql - 1) + BW;
+  int32_t corner = -1;
+#pragma unroll
+  for (int d = 0; d < W; d++) corner = d == dc ? H[d] : corner;
+  return corner;
+}
+
+// Run-length emitter of the CIGAR, fed in reverse order (last op first).
+struct CigarEmit {
+  uint32_t* out;   // null: count only
+  int32_t nops;    // total ops (for placing them from the end), write mode
+  int32_t count = 0;
+  int32_t mism = 0;
+  uint32_t op = 0xFFu;
+  uint32_t len = 0;
+  MHMSW_HD void flush() {
+    if (len == 0) return;
+    if (out && count < nops) out[nops - 1 - count] = (len << 4) | op;
+    count++;
+    if (op == kOpX || op == kOpI || op == kOpD) mism += (int32_t)len;
+    len = 0;
+  }
+  MHMSW_HD void push(uint32_t o, uint32_t l) {
+    if (l == 0) return;
+    if (len && o != op) flush();
+    op = o;
+    len += l;
+  }
+};
+
+// Traceback over the direction bits of a band laid out with half-width bw
+// (diagonal index d = j - i + bw), emitting the final CIGAR backwards.
+// shift = the column where the traceback stops (SSW-ref positions of M are
+// compared at rb + j - shift).  Returns that column, or -1 if the traceback
+// would leave the band or the rectangle (no correct input does; the reads
+// stay inside the band's storage either way).
+MHMSW_HD inline int32_t cigar_walk(const uint32_t* codes, const CigarGeo& g, int32_t bw, const uint32_t* dirs, int DW,
+                                   int64_t stride, int32_t shift, CigarEmit& em) {
+  NibCursor rcur, qcur;
+  em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
+  int32_t i = g.ql() - 1, j = g.rl() - 1;
+  int32_t d = j - i + bw;
+  int state = 2;  // 0 E, 1 F, 2 H
+  int64_t cached_row = -1;
+  uint32_t cached = 0;
+  while (i > 0) {
+    if (d < 0 || d > 2 * bw || j < 0) return -1;
+    const int64_t at = ((int64_t)i * DW + (d >> 3)) * stride;
+    if (at != cached_row) {
+      cached_row = at;
+      cached = dirs[at];
+    }
+    const uint32_t nib = (cached >> ((uint32_t)(d & 7) * 4u)) & 0xFu;
+    uint32_t op;
+    if (state == 2) {
+      if (!(nib & 1u)) {
+        op = kOpM;
+      } else if (nib & 2u) {
+        op = kOpI;
+        state = (nib & 4u) ? 2 : 0;
+      } else {
+        op = kOpD;
+        state = (nib & 8u) ? 2 : 1;
+      }
+    } else if (state == 0) {
+      op = kOpI;
+      state = (nib & 4u) ? 2 : 0;
+    } else {
+      op = kOpD;
+      state = (nib & 8u) ? 2 : 1;
+    }
+    if (op == kOpM) {
+      const uint32_t a = qcur.get(codes, g.qbase + g.qb + i);
+      const uint32_t b = rcur.get(codes, g.rbase + g.rb + j - shift);
+      em.push(a == b ? kOpEq : kOpX, 1);
+      i--;
+      j--;
+    } else if (op == kOpI) {
+      em.push(kOpI, 1);
+      i--;
+      d++;
+    } else {
+      em.push(kOpD, 1);
+      j--;
+      d--;
+    }
+  }
+  {
+    const uint32_t a = qcur.get(codes, g.qbase + g.qb);
+    const uint32_t b = rcur.get(codes, g.rbase + g.rb + j - shift);
+    em.push(a == b ? kOpEq : kOpX, 1);
+  }
+  em.push(kOpS, (uint32_t)g.qb);
+  em.flush();
+  return j;
+}
+
+// Pairs whose CIGAR needs no band: an empty SSW-query (SSW refuses it: no
+// CIGAR), spans beyond the distance filter (no CIGAR, 0 mismatches) and
+// score 0.  A score-0 pair has rb = -1, and SSW's CIGAR is one M comparing
+// SSW-query[0] with the byte before its SSW-ref buffer -- the top byte of
+// glibc's chunk size word, 0 (code A) -- then the clip of the rest.
+// Returns true when the pair is one of these; em then holds its CIGAR.
+MHMSW_HD inline bool cigar_trivial(const uint32_t* codes, const CigarGeo& g, CigarEmit& em) {
+  if (g.n <= 0) return true;
+  if (g.re - g.rb > kCigarDistanceFilter || g.qe - g.qb > kCigarDistanceFilter) return true;
+  if (g.score > 0) return false;
+  em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
+  em.push(nib_at(codes, g.qbase + g.qb) == 0 ? kOpEq : kOpX, 1);
+  em.push(kOpS, (uint32_t)g.qb);
+  em.flush();
+  return true;
+}
+
+// Generic band DP for any half-width (the slow path): rows of H and E in
+// hrow/erow (2*bw+1 entries each), direction dwords row-major with
+// DW = ceil((2*bw+1)/8) dwords per row.  Returns H of the last cell.
+MHMSW_HD inline int32_t cigar_band_dp_any(const uint32_t* codes, const CigarGeo& g, int32_t bw, int32_t match,
+                                          int32_t mismatch, int32_t amb, int32_t gO, int32_t gE, int32_t* hrow,
+                                          int32_t* erow, uint32_t* dirs) {
+  const int32_t W = 2 * bw + 1, DW = (4 * W + 31) / 32;
+  const int32_t rl = g.rl(), ql = g.ql();
+  for (int32_t d = 0; d < W; d++) {
+    hrow[d] = 0;
+    erow[d] = 0;
+  }
+  for (int32_t i = 0; i < ql; i++) {
+    uint32_t tlo, thi;
+    subst_table(nib_at(codes, g.qbase + g.qb + i), match, mismatch, amb, tlo, thi);
+    int32_t hl = 0, fl = 0;
+    uint32_t cur = 0;
+    for (int32_t d = 0; d < W; d++) {
+      // columns right of the rectangle never feed it: not read
+      const int32_t j = i - bw + d;
+      const uint32_t r = (j < 0 || j >= rl) ? kCigKill : nib_at(codes, g.rbase + g.rb + j);
+      const int32_t hu = d + 1 < W ? hrow[d + 1] : 0;
+      const int32_t eu = d + 1 < W ? erow[d + 1] : 0;
+      const int32_t eo = hu - gO, ee = eu - gE;
+      const int32_t e = imax3(eo, ee, 0);
+      const int32_t fo = hl - gO, fe = fl - gE;
+      const int32_t f = imax3(fo, fe, 0);
+      const int32_t t2 = hrow[d] + subst_lookup(tlo, thi, r);
+      const int32_t t1 = imax(e, f);
+      const int32_t h = imax(t1, t2);
+      const uint32_t bits = (t1 > t2 ? 1u : 0u) | (e > f ? 2u : 0u) | (eo > ee ? 4u : 0u) | (fo > fe ? 8u : 0u);
+      cur |= bits << ((4 * (d & 7)));
+      if ((d & 7) == 7 || d == W - 1) {
+        dirs[(int64_t)i * DW + (d >> 3)] = cur;
+        cur = 0;
+      }
+      hrow[d] = h;
+      erow[d] = e;
+      hl = h;
+      fl = f;
+    }
+  }
+  return hrow[(rl - 1) - (ql - 1) + bw];
+}
+
+}  // namespace mhmsw

@@ -22,6 +22,7 @@
 #include "../../include/mhmsw.h"
 #include "sw_common.h"
 #include "sw_launch.h"
+#include "cigar_core.h"
 #include "pack2bit.h"
 
 using namespace mhmsw;
@@ -50,7 +51,7 @@ struct DevBuf {
 };
 
 
-constexpr int kEvents = 6;
+constexpr int kEvents = 7;  // job start, 5 phase ends, CIGAR pass end
 
 }  // namespace
 
@@ -75,6 +76,8 @@ struct mhmsw_ctx {
   int read_is_ref = 1;
   int score2 = 0;               // compute SSW's second-best score
   bool last_job_score2 = false;
+  int cigar = 0;                // compute SSW's CIGAR and mismatch count
+  bool last_job_cigar = false;
   mhmsw_tuning tuning{8, 257, 5};
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
@@ -103,6 +106,13 @@ struct mhmsw_ctx {
   DevBuf<SwTask> d_tasks;
   DevBuf<PassSched> d_sched;
   DevBuf<AlnResult> d_results;
+  // CIGAR pass (cigar_kernels.hip)
+  DevBuf<int32_t> d_cig_lists, d_cig_count, d_cig_band, d_cig_rows, d_cig_nops, d_cig_mism;
+  DevBuf<int64_t> d_cig_off;
+  DevBuf<uint32_t> d_cig_dirs, d_cig_gen, d_cig_ops;
+  DevBuf<unsigned long long> d_cig_used;
+  int64_t cig_ops_cap = 0, cig_gen_dwords = 0, cig_dirs_dwords = 0;
+  unsigned long long h_cig_used[8] = {};
   // hipGraph of the whole job, replayed while the staged geometry is unchanged
   bool use_graphs = true;
   bool last_job_graph = false;
@@ -250,6 +260,17 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   c->d_tasks.release();
   c->d_sched.release();
   c->d_results.release();
+  c->d_cig_lists.release();
+  c->d_cig_count.release();
+  c->d_cig_band.release();
+  c->d_cig_rows.release();
+  c->d_cig_nops.release();
+  c->d_cig_mism.release();
+  c->d_cig_off.release();
+  c->d_cig_dirs.release();
+  c->d_cig_gen.release();
+  c->d_cig_ops.release();
+  c->d_cig_used.release();
   for (int i = 0; i < kEvents; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   for (int i = 0; i < kNumClasses; i++)
@@ -827,6 +848,62 @@ static BatchView make_view(mhmsw_ctx* c) {
   return b;
 }
 
+// Device buffers of the CIGAR pass for the staged batch: op pool and generic
+// pool grow when a pass overflows them (complete() reruns the pass);
+// direction scratch is a fixed 256 MiB (one slot per wavefront, rows of the
+// tallest rectangle; at least one slot of kCigMaxRows rows always fits).
+constexpr int64_t kCigDirsDwords = 64ll << 20;
+static int prepare_cigar(mhmsw_ctx* c) {
+  const size_t np = (size_t)std::max<int64_t>(c->npairs, 1);
+  c->cig_ops_cap = std::max<int64_t>(c->cig_ops_cap, 12 * (int64_t)np + 1024);
+  c->cig_gen_dwords = std::max<int64_t>(c->cig_gen_dwords, 4ll << 20);
+  c->cig_dirs_dwords = kCigDirsDwords;
+  HIP_OR_FAIL(c, c->d_cig_lists.ensure((size_t)kCigClasses * np));
+  HIP_OR_FAIL(c, c->d_cig_count.ensure(2 * kCigClasses));
+  HIP_OR_FAIL(c, c->d_cig_band.ensure(np));
+  HIP_OR_FAIL(c, c->d_cig_rows.ensure(1));
+  HIP_OR_FAIL(c, c->d_cig_nops.ensure(np));
+  HIP_OR_FAIL(c, c->d_cig_mism.ensure(np));
+  HIP_OR_FAIL(c, c->d_cig_off.ensure(np));
+  HIP_OR_FAIL(c, c->d_cig_used.ensure(8));
+  HIP_OR_FAIL(c, c->d_cig_ops.ensure((size_t)c->cig_ops_cap));
+  HIP_OR_FAIL(c, c->d_cig_gen.ensure((size_t)c->cig_gen_dwords));
+  HIP_OR_FAIL(c, c->d_cig_dirs.ensure((size_t)c->cig_dirs_dwords));
+  return MHMSW_OK;
+}
+
+static CigarView make_cigar_view(mhmsw_ctx* c) {
+  CigarView v;
+  std::memset(&v, 0, sizeof(v));  // part of the graph key, compared bytewise
+  v.npairs = c->npairs;
+  v.codes = c->d_codes.ptr;
+  v.seq_base = c->d_base.ptr;
+  v.seq_len = c->d_len.ptr;
+  v.results = c->d_results.ptr;
+  v.sc = c->sp;
+  v.lists = c->d_cig_lists.ptr;
+  v.list_count = c->d_cig_count.ptr;
+  v.band = c->d_cig_band.ptr;
+  v.max_rows = c->d_cig_rows.ptr;
+  v.dirs = c->d_cig_dirs.ptr;
+  v.dirs_dwords = (int64_t)c->d_cig_dirs.cap;
+  v.gen = c->d_cig_gen.ptr;
+  v.gen_dwords = (int64_t)c->d_cig_gen.cap;
+  v.n_ops = c->d_cig_nops.ptr;
+  v.mism = c->d_cig_mism.ptr;
+  v.off = c->d_cig_off.ptr;
+  v.ops = c->d_cig_ops.ptr;
+  v.ops_cap = (int64_t)c->d_cig_ops.cap;
+  v.used = c->d_cig_used.ptr;
+  return v;
+}
+
+// wavefronts of the CIGAR launches: 16 per CU for the register classes, 1
+// per CU for the generic class
+static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
+  return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus, st);
+}
+
 // ev: record phase events (eager launches only; events recorded inside a
 // captured graph cannot be timed)
 static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev, int* maxp16_out, int* maxp64_out) {
@@ -901,6 +978,8 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
   if (c->npairs > 0) HIP_OR_FAIL(c, launch_finalize(b, st));
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[5], st));
+  if (c->cigar) HIP_OR_FAIL(c, enqueue_cigar(c, st));
+  if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[6], st));
   *maxp16_out = maxp16;
   *maxp64_out = maxp64;
   return MHMSW_OK;
@@ -928,6 +1007,11 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   hipStream_t st = c->stream;
   BatchView b = make_view(c);
   c->last_job_score2 = c->score2 != 0;
+  c->last_job_cigar = c->cigar != 0;
+  if (c->cigar) {
+    int rc = prepare_cigar(c);
+    if (rc != MHMSW_OK) return rc;
+  }
   for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = c->npairs > 0 && c->fast_class_used[k];
   int maxp16 = 0, maxp64 = 0;
   c->last_job_graph = c->use_graphs;
@@ -937,12 +1021,17 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   } else {
     // the job depends only on the view, the classes used, the mode and the
     // plain LDS size: replay the captured graph while those are unchanged
-    std::vector<unsigned char> key(sizeof(BatchView) + sizeof(c->fast_class_used) + 2 * sizeof(int));
+    std::vector<unsigned char> key(sizeof(BatchView) + sizeof(c->fast_class_used) + 3 * sizeof(int) +
+                                   sizeof(CigarView));
     unsigned char* kp = key.data();
     std::memcpy(kp, &b, sizeof(BatchView));
     std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
-    const int extra[2] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu};
+    const int extra[3] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar};
     std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
+    if (c->cigar) {
+      const CigarView cv = make_cigar_view(c);
+      std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used) + sizeof(extra), &cv, sizeof(cv));
+    }
     if (!c->graph_exec || key != c->graph_key) {
       drop_graph(c);
       HIP_OR_FAIL(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -989,6 +1078,29 @@ int mhmsw_submit(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t*
   return mhmsw_align_staged(c);
 }
 
+// After a job with the CIGAR pass: a pass whose op pool or generic pool was
+// too small is rerun with pools of the size it asked for (the device counted
+// what it needed), synchronously on the context stream.
+static int settle_cigar(mhmsw_ctx* c) {
+  for (int attempt = 0;; attempt++) {
+    HIP_OR_FAIL(c, hipMemcpy(c->h_cig_used, c->d_cig_used.ptr, sizeof(c->h_cig_used), hipMemcpyDeviceToHost));
+    const unsigned long long* u = c->h_cig_used;
+    if (u[3] != 0)
+      return fail(c, MHMSW_E_INTERNAL, "CIGAR pass: %llu tracebacks left their band", (unsigned long long)u[3]);
+    const bool grow_ops = (int64_t)u[0] > (int64_t)c->d_cig_ops.cap;
+    const bool grow_gen = u[2] != 0;
+    if (!grow_ops && !grow_gen) return MHMSW_OK;
+    if (attempt >= 8) return fail(c, MHMSW_E_INTERNAL, "CIGAR pass: pools did not settle");
+    if (grow_ops) c->cig_ops_cap = (int64_t)u[0] + (int64_t)(u[0] / 4) + 1024;
+    if (grow_gen) c->cig_gen_dwords = std::max<int64_t>(2 * (int64_t)u[1], 2 * c->cig_gen_dwords);
+    int rc = prepare_cigar(c);
+    if (rc != MHMSW_OK) return rc;
+    HIP_OR_FAIL(c, enqueue_cigar(c, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    c->stats.cigar_reruns++;
+  }
+}
+
 static int complete(mhmsw_ctx* c) {
   c->in_flight = false;
   HIP_OR_FAIL(c, hipMemcpy(c->h_errors, c->d_errors.ptr, sizeof(c->h_errors), hipMemcpyDeviceToHost));
@@ -1022,7 +1134,13 @@ static int complete(mhmsw_ctx* c) {
   s.fwd_fast_ms = ms[2];
   s.fwd_plain_ms = ms[3];
   s.rev_ms = ms[4] + ms[5];
+  s.cigar_ms = ms[6];
   s.job_ms = job;
+  s.cigar_reruns = 0;
+  if (c->last_job_cigar) {
+    int rc = settle_cigar(c);
+    if (rc != MHMSW_OK) return rc;
+  }
   c->have_results = true;
   if (c->h_errors[0] != 0 || c->h_errors[2] != 0)
     return fail(c, MHMSW_E_INTERNAL, "%d pairs: reverse pass did not reproduce the forward score; %d end points "
@@ -1077,6 +1195,67 @@ int mhmsw_results_score2(mhmsw_ctx* c, int32_t* out, int64_t npairs) {
   if (npairs != c->npairs) return fail(c, MHMSW_E_ARG, "score2 requested for %lld pairs, batch has %lld",
                                        (long long)npairs, (long long)c->npairs);
   if (npairs > 0) HIP_OR_FAIL(c, hipMemcpy(out, c->d_s2.ptr, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return MHMSW_OK;
+}
+
+int mhmsw_set_cigar(mhmsw_ctx* c, int enable) {
+  if (!c) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "cigar changed while a batch is in flight");
+  c->cigar = enable ? 1 : 0;
+  return MHMSW_OK;
+}
+
+int mhmsw_results_cigar(mhmsw_ctx* c, int64_t npairs, int32_t* mismatches, int64_t* op_off, uint32_t* ops,
+                        int64_t ops_cap, int64_t* total_ops) {
+  if (!c) return MHMSW_E_ARG;
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  if (!c->have_results) return fail(c, MHMSW_E_STATE, "no results: submit a batch first");
+  if (!c->last_job_cigar) return fail(c, MHMSW_E_STATE, "the last batch ran without the CIGAR pass (mhmsw_set_cigar)");
+  if (npairs != c->npairs) return fail(c, MHMSW_E_ARG, "CIGARs requested for %lld pairs, batch has %lld",
+                                       (long long)npairs, (long long)c->npairs);
+  std::vector<int32_t> n((size_t)npairs), mm((size_t)npairs);
+  std::vector<int64_t> off((size_t)npairs);
+  if (npairs > 0) {
+    HIP_OR_FAIL(c, hipMemcpy(n.data(), c->d_cig_nops.ptr, (size_t)npairs * 4, hipMemcpyDeviceToHost));
+    HIP_OR_FAIL(c, hipMemcpy(mm.data(), c->d_cig_mism.ptr, (size_t)npairs * 4, hipMemcpyDeviceToHost));
+    HIP_OR_FAIL(c, hipMemcpy(off.data(), c->d_cig_off.ptr, (size_t)npairs * 8, hipMemcpyDeviceToHost));
+  }
+  int64_t total = 0;
+  for (int64_t p = 0; p < npairs; p++) {
+    if (n[(size_t)p] < 0)
+      return fail(c, MHMSW_E_INTERNAL, "pair %lld: CIGAR not computed (%d)", (long long)p, n[(size_t)p]);
+    total += n[(size_t)p];
+  }
+  if (total_ops) *total_ops = total;
+  if (mismatches && npairs > 0) std::memcpy(mismatches, mm.data(), (size_t)npairs * 4);
+  if (op_off) {
+    int64_t acc = 0;
+    for (int64_t p = 0; p < npairs; p++) {
+      op_off[p] = acc;
+      acc += n[(size_t)p];
+    }
+    op_off[npairs] = acc;
+  }
+  if (!ops) return MHMSW_OK;
+  if (ops_cap < total) return fail(c, MHMSW_E_ARG, "ops buffer holds %lld ops, batch has %lld", (long long)ops_cap,
+                                   (long long)total);
+  const int64_t used = (int64_t)c->h_cig_used[0];
+  std::vector<uint32_t> pool((size_t)std::max<int64_t>(used, 1));
+  if (used > 0)
+    HIP_OR_FAIL(c, hipMemcpy(pool.data(), c->d_cig_ops.ptr, (size_t)used * 4, hipMemcpyDeviceToHost));
+  int64_t acc = 0;
+  for (int64_t p = 0; p < npairs; p++) {
+    const int32_t k = n[(size_t)p];
+    if (k > 0) {
+      if (off[(size_t)p] < 0 || off[(size_t)p] + k > used)
+        return fail(c, MHMSW_E_INTERNAL, "pair %lld: CIGAR outside the op pool", (long long)p);
+      std::memcpy(ops + acc, pool.data() + off[(size_t)p], (size_t)k * 4);
+    }
+    acc += k;
+  }
   return MHMSW_OK;
 }
 
