@@ -58,6 +58,63 @@ def cpu_baseline(batch, scoring, seconds_target=12.0):
             "sample": f"first {k2} of {n} pairs of the same workload ({cells} cells), {dt:.1f} s"}, res, k2
 
 
+def cigar_leg(al, batch, res, scoring, steps, check):
+    """Jobs with the CIGAR pass on (SSW's CIGAR + mismatches, klign with report_cigar): its device time,
+    the job's rate, and -- when check -- the reference SSW with report_cigar on host cores on a bounded
+    sample, every GPU CIGAR of that sample compared with the reference's."""
+    import mhm2_staging_amd as m
+    al.set_cigar(True)
+    al.align_staged()
+    al.wait()
+    cms, jms = [], []
+    t = time.perf_counter()
+    for _ in range(steps):
+        al.align_staged()
+        al.wait()
+        st = al.stats()
+        cms.append(st["cigar_ms"])
+        jms.append(st["job_ms"])
+    el = time.perf_counter() - t
+    cres = al.results()
+    mism, off, ops = al.cigars()
+    al.set_cigar(False)
+    cells = batch.cells()
+    out = {"cigar_ms": round(float(np.mean(cms)), 3), "job_ms": round(float(np.mean(jms)), 3),
+           "value": round(cells * steps / el / 1e9, 2), "unit": "GCUPS (jobs with the CIGAR pass)",
+           "pairs_per_s": round(batch.npairs * steps / el, 1),
+           "cigar_pairs_per_s_device": round(batch.npairs / (float(np.mean(cms)) * 1e-3), 1),
+           "ops": int(len(ops)), "reruns": int(st["cigar_reruns"]),
+           "identical_results": bool((cres == res).all())}
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        if oracle_lib.have_ref():
+            cores = min(os.cpu_count() or 1, 16)
+            k = min(batch.npairs, 2000)
+            args = lambda n: (batch.reads, batch.read_off[:n], batch.read_len[:n], batch.ctgs, batch.ctg_off[:n],
+                              batch.ctg_len[:n], scoring)
+            t = time.perf_counter()
+            oracle_lib.ref_cigar_batch(*args(k), maxops=64, nthreads=cores)
+            dt = max(time.perf_counter() - t, 1e-6)
+            k = int(min(batch.npairs, max(k, k * 10.0 / dt)))
+            maxops = int(np.diff(off[:k + 1]).max(initial=0)) + 8
+            t = time.perf_counter()
+            rres, rmism, rnops, rops = oracle_lib.ref_cigar_batch(*args(k), maxops=maxops, nthreads=cores)
+            dt = time.perf_counter() - t
+            kc = int(np.dot(batch.read_len[:k].astype(np.int64), batch.ctg_len[:k].astype(np.int64)))
+            bad = (np.diff(off[:k + 1]) != rnops) | (mism[:k] != rmism)
+            for p in np.nonzero(~bad)[0]:
+                if not np.array_equal(ops[off[p]:off[p + 1]], rops[p, :rnops[p]]):
+                    bad[p] = True
+            out["cpu_reference"] = {"value": round(kc / dt / 1e9, 3), "unit": "GCUPS", "cores": cores,
+                                    "pairs_per_s": round(k / dt, 1),
+                                    "sample": f"first {k} of {batch.npairs} pairs, reference SSW with "
+                                              f"report_cigar, {dt:.1f} s"}
+            out["bit_exact_check"] = {"pairs": k, "mismatches": int(bad.sum()), "against": "reference",
+                                      "fields": "CIGAR ops, mismatches"}
+    return out
+
+
 def serving_bench(qps_list, duration, slots=4, seed=5):
     """Config 5: 64k resident contigs, Poisson requests of log-uniform 1e3..1e5 pairs."""
     import mhm2_staging_amd as m
@@ -117,6 +174,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
+    ap.add_argument("--no-cigar", action="store_true", help="skip the CIGAR-pass measurement")
     ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
     ap.add_argument("--tuning", type=str, default="", metavar="B,G,S",
                     help="fixed scheduling parameters (blocks_per_cu, g64_min_rows, bin_shift) instead of autotuning")
@@ -280,6 +338,10 @@ def main():
                              "identical_results": bool((pres == res).all())}
         al.set_mode(m.MODE_FAST)
         log(f"[bench] plain form: {plain_gcups:.1f} GCUPS")
+
+    if not args.no_cigar:
+        out["cigar_pass"] = cigar_leg(al, batch, res, scoring, args.steps, not args.no_cpu)
+        log(f"[bench] cigar pass: {out['cigar_pass']}")
 
     if not args.no_cpu:
         cb, cres, kk = cpu_baseline(batch, scoring)

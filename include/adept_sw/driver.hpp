@@ -71,6 +71,15 @@ class GPUDriver {
   // of a batch run with it (nullptr otherwise).
   void set_score2(bool enable);
   const int *get_score2();
+  // SSW's CIGAR string and mismatch count per pair (Alignment::cigar_string
+  // and ::mismatches of klign's call with report_cigar set,
+  // src/klign.cpp:251-273), which klign cannot take from ADEPT (it keeps such
+  // batches on the CPU, src/klign.cpp:589-599, and its GPU mapping writes
+  // "*", :385).  Enable before run_kernel_forwards; the getters are valid
+  // after the second kernel_block of a batch run with it (nullptr otherwise).
+  void set_cigar(bool enable);
+  const std::string *get_cigar_strings();
+  const int *get_mismatches();
 };
 
 }  // namespace adept_sw

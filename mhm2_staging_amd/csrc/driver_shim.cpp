@@ -41,6 +41,12 @@ struct DriverState {
   bool score2 = false;        // batches compute score2 (set_score2)
   bool batch_score2 = false;  // the submitted batch did
   std::vector<int32_t> s2;
+  bool cigar = false;         // batches compute CIGARs (set_cigar)
+  bool batch_cigar = false;
+  std::vector<int32_t> mism;
+  std::vector<int64_t> op_off;
+  std::vector<uint32_t> ops;
+  std::vector<std::string> cigars;
 };
 
 static void throw_ctx(DriverState *s, int rc, const char *what, const char *file, int line) {
@@ -157,6 +163,24 @@ const int *GPUDriver::get_score2() {
   return s->s2.data();
 }
 
+void GPUDriver::set_cigar(bool enable) {
+  if (!driver_state) throw std::runtime_error("GPU assert set_cigar before init\n");
+  MHMSW_CHECK(driver_state, mhmsw_set_cigar(driver_state->ctx, enable ? 1 : 0));
+  driver_state->cigar = enable;
+}
+
+const std::string *GPUDriver::get_cigar_strings() {
+  DriverState *s = driver_state;
+  if (!s || !s->fetched || !s->batch_cigar) return nullptr;
+  return s->cigars.data();
+}
+
+const int *GPUDriver::get_mismatches() {
+  DriverState *s = driver_state;
+  if (!s || !s->fetched || !s->batch_cigar) return nullptr;
+  return s->mism.data();
+}
+
 void GPUDriver::set_read_is_ssw_ref(bool read_is_ref) {
   if (!driver_state) throw std::runtime_error("GPU assert set_read_is_ssw_ref before init\n");
   MHMSW_CHECK(driver_state, mhmsw_set_orientation(driver_state->ctx, read_is_ref ? 1 : 0));
@@ -195,6 +219,7 @@ void GPUDriver::run_kernel_forwards(std::vector<std::string> &reads, std::vector
   s->fetched = false;
   s->backwards_called = false;
   s->batch_score2 = s->score2;
+  s->batch_cigar = s->cigar;
 }
 
 void GPUDriver::run_kernel_backwards(std::vector<std::string> &reads, std::vector<std::string> &contigs,
@@ -216,6 +241,25 @@ static void fetch(DriverState *s, AlignmentResults &a) {
   if (s->batch_score2) {
     s->s2.resize(std::max<size_t>(n, 1));
     MHMSW_CHECK(s, mhmsw_results_score2(s->ctx, s->s2.data(), s->npairs));
+  }
+  if (s->batch_cigar) {
+    // SSW's cigar_string spelling: <length><op> per op (src/ssw/ssw.cpp:72-90)
+    static const char kOps[] = "MIDNSHP=X";
+    int64_t total = 0;
+    s->mism.resize(std::max<size_t>(n, 1));
+    s->op_off.resize(n + 1);
+    MHMSW_CHECK(s, mhmsw_results_cigar(s->ctx, s->npairs, s->mism.data(), s->op_off.data(), nullptr, 0, &total));
+    s->ops.resize((size_t)std::max<int64_t>(total, 1));
+    MHMSW_CHECK(s, mhmsw_results_cigar(s->ctx, s->npairs, nullptr, nullptr, s->ops.data(), total, &total));
+    s->cigars.resize(std::max<size_t>(n, 1));
+    for (size_t i = 0; i < n; i++) {
+      std::string &c = s->cigars[i];
+      c.clear();
+      for (int64_t k = s->op_off[i]; k < s->op_off[i + 1]; k++) {
+        c += std::to_string(s->ops[(size_t)k] >> 4);
+        c += kOps[s->ops[(size_t)k] & 0xFu];
+      }
+    }
   }
   if (n > s->cap) {
     s->cap = std::max<size_t>(n, KLIGN_GPU_BLOCK_SIZE);

@@ -47,6 +47,11 @@ int ssw_ref_align_batch2(int64_t npairs, const char *a, const int64_t *a_off, co
  * query_end, mismatches.  Returns 0, or -1 when Align refuses the input. */
 int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc, int32_t *out);
 
+/* As ssw_ref_align_one with report_cigar = true: out[6] = mismatches and the
+ * NUL-terminated cigar_string in cigar (truncated to cigar_cap bytes). */
+int ssw_ref_align_one_cigar(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc,
+                            int32_t *out, char *cigar, int32_t cigar_cap);
+
 /* klign's call when it reports CIGARs (report_cigar = true, klign.cpp:435):
  * results, Alignment::mismatches, and Alignment::cigar (BAM-encoded ops) of
  * pair p at ops[p * maxops ..] with their count in nops[p] (only the first
@@ -55,6 +60,29 @@ int ssw_ref_align_one(const char *query, const char *ref, int32_t ref_len, const
 int ssw_ref_cigar_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,
                         const int64_t *b_off, const int32_t *b_len, const ssw_ref_scoring *sc, ssw_ref_result *out,
                         int32_t *mismatches, int32_t *nops, uint32_t *ops, int maxops, int nthreads);
+}
+
+int ssw_ref_align_one_cigar(const char *query, const char *ref, int32_t ref_len, const ssw_ref_scoring *sc,
+                            int32_t *out, char *cigar, int32_t cigar_cap) {
+  StripedSmithWaterman::Aligner aligner((uint8_t)sc->match, (uint8_t)sc->mismatch, (uint8_t)sc->gap_open,
+                                        (uint8_t)sc->gap_ext, (uint8_t)sc->ambiguity);
+  StripedSmithWaterman::Filter filter(true, true, 0, 32767);
+  StripedSmithWaterman::Alignment aln;
+  aln.Clear();
+  if (!aligner.Align(query, ref, ref_len, filter, &aln, std::max((int)(ref_len / 2), 15))) return -1;
+  out[0] = aln.sw_score;
+  out[1] = aln.sw_score_next_best;
+  out[2] = aln.ref_begin;
+  out[3] = aln.ref_end;
+  out[4] = aln.query_begin;
+  out[5] = aln.query_end;
+  out[6] = aln.mismatches;
+  if (cigar_cap > 0) {
+    const size_t k = std::min(aln.cigar_string.size(), (size_t)cigar_cap - 1);
+    aln.cigar_string.copy(cigar, k);
+    cigar[k] = 0;
+  }
+  return 0;
 }
 
 int ssw_ref_cigar_batch(int64_t npairs, const char *a, const int64_t *a_off, const int32_t *a_len, const char *b,

@@ -12,6 +12,13 @@
 //     score2 = mismatches = 0) ........................... :356-388
 // With --gpu-score2 the GPU side fills score2 from the driver's second-best
 // scores (GPUDriver::set_score2 / get_score2) instead of klign's 0.
+// With --report-cigar the run is klign's CIGAR-reporting one (its Filter's
+// report_cigar, klign.cpp:435): every record gets set_sam_string's SAM line
+// (:178-209), the perfect-match shortcut writes "<n>=" (:229) and the CPU
+// side takes SSW's cigar_string and mismatches (:257-271).  klign then keeps
+// blocks off the GPU (:589-599, the block goes to ssw_align_block); with
+// --gpu-cigar as well, blocks go to the drop-in driver, which returns the
+// same CIGARs, mismatches and score2 (set_cigar / set_score2).
 //   switch_orient ........................................ src/utils.cpp:163-167
 //   Aln fields and to_string ............................. include/alignments.hpp:58-72,
 //                                                           src/alignments.cpp:96-103
@@ -26,13 +33,15 @@
 // usage: mini_klign --mode gpu|cpu --out FILE [--seed S] [--reads N]
 //          [--block B] [--mem BYTES] [--no-steal] [--gpu-delay-ms D]
 //          [--scoring m,x,o,e,a] [--ssw LIB] [--oracle LIB] [--gpu-score2]
-// Writes one line per alignment (Aln::to_string, then identity, mismatches
-// and the side that aligned it: P perfect shortcut, G GPU, C CPU) and prints
-// a JSON summary on stdout.
+//          [--report-cigar] [--gpu-cigar]
+// Writes one line per alignment (Aln::to_string, then identity, mismatches,
+// the side that aligned it: P perfect shortcut, G GPU, C CPU, and the SAM
+// line when CIGARs are reported) and prints a JSON summary on stdout.
 #include <dlfcn.h>
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -68,6 +77,7 @@ struct Aln {
   int8_t identity = 0;
   char side = '?';
   int64_t read_idx = 0;
+  std::string sam_string;
 
   Aln() = default;
   Aln(const std::string &rid, int64_t cid_, int rstart_, int rstop_, int rlen_, int cstart_, int cstop_, int clen_,
@@ -85,6 +95,8 @@ struct Aln {
   }
 };
 
+std::string revcomp(const std::string &s);
+
 void switch_orient(int &start, int &stop, int &len) {
   int tmp = start;
   start = len - stop;
@@ -96,13 +108,49 @@ struct CScoring {
   int32_t match, mismatch, gap_open, gap_ext, ambiguity;
 };
 typedef int (*ssw_one_fn)(const char *, const char *, int32_t, const CScoring *, int32_t *);
+typedef int (*ssw_one_cigar_fn)(const char *, const char *, int32_t, const CScoring *, int32_t *, char *, int32_t);
+typedef int (*oracle_cigar_fn)(int64_t, const char *, const int64_t *, const int32_t *, const char *,
+                               const int64_t *, const int32_t *, const CScoring *, int32_t *, int32_t *, int32_t *,
+                               uint32_t *, int, int);
 typedef int (*oracle_batch_fn)(int64_t, const char *, const int64_t *, const int32_t *, const char *,
                                const int64_t *, const int32_t *, const CScoring *, int32_t *, int);
 
 struct CpuAligner {
   ssw_one_fn ssw_one = nullptr;
+  ssw_one_cigar_fn ssw_one_cigar = nullptr;
   oracle_batch_fn oracle = nullptr;
+  oracle_cigar_fn oracle_cigar = nullptr;
   CScoring sc{};
+
+  // The same call with report_cigar set: out as align(), plus cigar_string.
+  void align_cigar(const std::string &query, const std::string &ref, int32_t out[7], std::string &cigar) const {
+    if (ssw_one_cigar) {
+      std::vector<char> buf(2 * (query.size() + ref.size()) * 8 + 64);
+      if (ssw_one_cigar(query.c_str(), ref.c_str(), (int32_t)ref.size(), &sc, out, buf.data(), (int32_t)buf.size()))
+        throw std::runtime_error("SSW refused a pair");
+      cigar = buf.data();
+      return;
+    }
+    int64_t aoff = 0, boff = 0;
+    int32_t alen = (int32_t)ref.size(), blen = (int32_t)query.size();
+    int32_t r[5], mism = 0, nops = 0;
+    const int maxops = (int)(2 * (query.size() + ref.size()) + 8);
+    std::vector<uint32_t> ops((size_t)maxops);
+    oracle_cigar(1, ref.data(), &aoff, &alen, query.data(), &boff, &blen, &sc, r, &mism, &nops, ops.data(), maxops, 1);
+    out[0] = r[0];
+    out[1] = -1;
+    out[2] = r[1];
+    out[3] = r[2];
+    out[4] = r[3];
+    out[5] = r[4];
+    out[6] = mism;
+    static const char kOps[] = "MIDNSHP=X";
+    cigar.clear();
+    for (int k = 0; k < nops && k < maxops; k++) {
+      cigar += std::to_string(ops[(size_t)k] >> 4);
+      cigar += kOps[ops[(size_t)k] & 0xFu];
+    }
+  }
 
   // SSW's Alignment fields klign reads: sw_score, sw_score_next_best,
   // ref_begin, ref_end, query_begin, query_end, mismatches.
@@ -137,6 +185,8 @@ struct KlignSim {
   bool steal = true;
   int gpu_delay_ms = 0;
   bool gpu_score2 = false;
+  bool report_cigar = false;  // klign's ssw_filter.report_cigar
+  bool gpu_cigar = false;     // CIGAR blocks go to the GPU driver too
   adept_sw::GPUDriver gpu_driver;
 
   std::vector<Aln> kernel_alns;
@@ -158,6 +208,30 @@ struct KlignSim {
     alns.push_back(std::move(a));
   }
 
+  // klign.cpp:178-209
+  static void set_sam_string(Aln &aln, std::string read_seq, const std::string &cigar) {
+    aln.sam_string = aln.read_id + "\t";
+    if (aln.orient == '-') {
+      aln.sam_string += "16\t";
+      if (read_seq != "*") read_seq = revcomp(read_seq);
+    } else {
+      aln.sam_string += "0\t";
+    }
+    aln.sam_string += "Contig" + std::to_string(aln.cid) + "\t" + std::to_string(aln.cstart + 1) + "\t";
+    uint32_t mapq;
+    if (aln.score2 == 0) {
+      mapq = 60;
+    } else {
+      mapq = -4.343 * std::log(1 - (double)std::abs(aln.score1 - aln.score2) / (double)aln.score1);
+      mapq = (uint32_t)(mapq + 4.99);
+      mapq = mapq < 254 ? mapq : 254;
+    }
+    aln.sam_string += std::to_string(mapq) + "\t";
+    aln.sam_string += cigar + "\t*\t0\t0\t*\t*\t";
+    aln.sam_string += "AS:i:" + std::to_string(aln.score1) + "\tNM:i:" + std::to_string(aln.mismatches) +
+                      "\tRG:Z:" + std::to_string(aln.read_group_id);
+  }
+
   // klign.cpp:218-249
   void align_read(const std::string &rname, int64_t read_idx, int64_t cid, const std::string &rseq,
                   const std::string &cseq, int rstart, int rlen, int cstart, int clen, char orient, int overlap_len,
@@ -170,6 +244,7 @@ struct KlignSim {
       int score1 = overlap_len * aln_scoring.match;
       int identity = 100 * score1 / aln_scoring.match / rlen;
       Aln aln(rname, cid, rstart, rstop, rlen, cstart, cstop, clen, orient, score1, 0, identity, 0, read_group_id);
+      if (report_cigar) set_sam_string(aln, rseq, std::to_string(overlap_len) + "=");
       aln.side = 'P';
       aln.read_idx = read_idx;
       add_aln(std::move(aln));
@@ -190,7 +265,12 @@ struct KlignSim {
   // klign.cpp:251-273
   void ssw_align_read(Aln &aln, const std::string &cseq, const std::string &rseq, int read_group_id) {
     int32_t r[7];
-    cpu.align(cseq, rseq, r);
+    std::string cigar;
+    if (report_cigar) {
+      cpu.align_cigar(cseq, rseq, r, cigar);
+    } else {
+      cpu.align(cseq, rseq, r);
+    }
     aln.rstop = aln.rstart + r[3] + 1;
     aln.rstart += r[2];
     aln.cstop = aln.cstart + r[5] + 1;
@@ -201,6 +281,7 @@ struct KlignSim {
     aln.mismatches = r[6];
     aln.identity = (int8_t)(100 * r[0] / aln_scoring.match / aln.rlen);
     aln.read_group_id = (int16_t)read_group_id;
+    if (report_cigar) set_sam_string(aln, rseq, cigar);
     aln.side = 'C';
     add_aln(std::move(aln));
   }
@@ -222,6 +303,9 @@ struct KlignSim {
     auto aln_results = gpu_driver.get_aln_results();
     const int* s2 = gpu_score2 ? gpu_driver.get_score2() : nullptr;
     if (gpu_score2 && !s2) throw std::runtime_error("the driver returned no score2");
+    const std::string *cig = report_cigar ? gpu_driver.get_cigar_strings() : nullptr;
+    const int *mism = report_cigar ? gpu_driver.get_mismatches() : nullptr;
+    if (report_cigar && (!cig || !mism)) throw std::runtime_error("the driver returned no CIGARs");
     std::vector<Aln> out;
     out.reserve(b.kernel_alns.size());
     for (size_t i = 0; i < b.kernel_alns.size(); i++) {
@@ -233,9 +317,10 @@ struct KlignSim {
       if (aln.orient == '-') switch_orient(aln.rstart, aln.rstop, aln.rlen);
       aln.score1 = aln_results.top_scores[i];
       aln.score2 = s2 ? s2[i] : 0;
-      aln.mismatches = 0;
+      aln.mismatches = mism ? mism[i] : 0;
       aln.identity = (int8_t)(100 * aln.score1 / aln_scoring.match / aln.rlen);
       aln.read_group_id = (int16_t)b.read_group_id;
+      if (report_cigar) set_sam_string(aln, b.read_seqs[i], cig[i]);
       aln.side = 'G';
       out.push_back(std::move(aln));
     }
@@ -276,7 +361,15 @@ struct KlignSim {
       blk->max_rlen = max_rlen;
       blk->read_group_id = read_group_id;
       num_blocks++;
-      active_kernel_fut = std::async(std::launch::async, [this, blk]() { gpu_align_block_kernel(*blk); });
+      if (!report_cigar || gpu_cigar) {
+        active_kernel_fut = std::async(std::launch::async, [this, blk]() { gpu_align_block_kernel(*blk); });
+      } else {
+        // ssw_align_block: the block on the CPU, on its own thread
+        active_kernel_fut = std::async(std::launch::async, [this, blk]() {
+          for (size_t i = 0; i < blk->kernel_alns.size(); i++)
+            ssw_align_read(blk->kernel_alns[i], blk->ctg_seqs[i], blk->read_seqs[i], blk->read_group_id);
+        });
+      }
     }
     clear_aln_bufs();
   }
@@ -409,7 +502,7 @@ void usage() {
   std::fprintf(stderr,
                "usage: mini_klign --mode gpu|cpu --out FILE [--seed S] [--reads N] [--block B] [--mem BYTES]\n"
                "                  [--no-steal] [--gpu-delay-ms D] [--scoring m,x,o,e,a] [--ssw LIB] [--oracle LIB]\n"
-               "                  [--gpu-score2]\n");
+               "                  [--gpu-score2] [--report-cigar] [--gpu-cigar]\n");
 }
 
 }  // namespace
@@ -418,7 +511,7 @@ int main(int argc, char **argv) {
   std::string mode = "gpu", out_path, ssw_lib, oracle_lib;
   uint64_t seed = 11;
   int64_t nreads = 40000, block = KLIGN_GPU_BLOCK_SIZE, mem = 0;
-  bool steal = true, k_gpu_score2 = false;
+  bool steal = true, k_gpu_score2 = false, k_report_cigar = false, k_gpu_cigar = false;
   int delay = 0;
   AlnScoring sc{1, 3, 5, 2, 2};
   const int kmer_len = 21;
@@ -439,6 +532,8 @@ int main(int argc, char **argv) {
     else if (a == "--mem") mem = std::strtoll(val().c_str(), nullptr, 10);
     else if (a == "--no-steal") steal = false;
     else if (a == "--gpu-score2") k_gpu_score2 = true;
+    else if (a == "--report-cigar") k_report_cigar = true;
+    else if (a == "--gpu-cigar") k_gpu_cigar = true;
     else if (a == "--gpu-delay-ms") delay = std::atoi(val().c_str());
     else if (a == "--ssw") ssw_lib = val();
     else if (a == "--oracle") oracle_lib = val();
@@ -459,6 +554,12 @@ int main(int argc, char **argv) {
   }
 
   KlignSim k;
+  k.report_cigar = k_report_cigar;
+  k.gpu_cigar = k_gpu_cigar;
+  if (k_gpu_cigar && !k_report_cigar) {
+    usage();
+    return 2;
+  }
   k.aln_scoring = sc;
   k.block_size = block;
   k.steal = steal;
@@ -467,10 +568,12 @@ int main(int argc, char **argv) {
   if (!ssw_lib.empty()) {
     void *h = dlopen(ssw_lib.c_str(), RTLD_NOW | RTLD_LOCAL);
     if (h) k.cpu.ssw_one = (ssw_one_fn)dlsym(h, "ssw_ref_align_one");
+    if (h) k.cpu.ssw_one_cigar = (ssw_one_cigar_fn)dlsym(h, "ssw_ref_align_one_cigar");
   }
   if (!k.cpu.ssw_one && !oracle_lib.empty()) {
     void *h = dlopen(oracle_lib.c_str(), RTLD_NOW | RTLD_LOCAL);
     if (h) k.cpu.oracle = (oracle_batch_fn)dlsym(h, "oracle_align_batch");
+    if (h) k.cpu.oracle_cigar = (oracle_cigar_fn)dlsym(h, "oracle_cigar_batch");
   }
   if (!k.cpu.ssw_one && !k.cpu.oracle && (steal || mode == "cpu")) {
     std::fprintf(stderr, "no CPU aligner: give --ssw or --oracle\n");
@@ -489,7 +592,10 @@ int main(int argc, char **argv) {
                       300);
     // klign's build passes N's cost through ALN_AMBIGUITY_COST; this harness sets it explicitly
     k.gpu_driver.set_ambiguity((short)sc.ambiguity);
+    // CIGAR blocks on the GPU need score2 too: set_sam_string's mapq reads it
+    if (k_gpu_cigar) k_gpu_score2 = true;
     if (k_gpu_score2) k.gpu_driver.set_score2(true);
+    if (k_gpu_cigar) k.gpu_driver.set_cigar(true);
     k.gpu_score2 = k_gpu_score2;
   } else {
     k.gpu_devices = 0;
@@ -544,7 +650,8 @@ int main(int argc, char **argv) {
     return 2;
   }
   for (const Aln &a : k.alns)
-    std::fprintf(f, "%s\t%d\t%d\t%c\n", a.to_string().c_str(), (int)a.identity, a.mismatches, a.side);
+    std::fprintf(f, "%s\t%d\t%d\t%c\t%s\n", a.to_string().c_str(), (int)a.identity, a.mismatches, a.side,
+                 a.sam_string.c_str());
   std::fclose(f);
   std::printf(
       "{\"mode\": \"%s\", \"alns\": %lld, \"records\": %lld, \"perfect\": %lld, \"gpu\": %lld, \"stolen\": %lld, "

@@ -162,3 +162,60 @@ def test_klign_gpu_score2_records_identical(tmp_path, scoring):
     print(json.dumps(got_s), flush=True)
     assert got_s["gpu"] > 0 and got_s["stolen"] > 0
     compare(exp, got, gpu_score2=True)
+
+
+def test_cpu_harness_report_cigar_sam_lines(tmp_path):
+    """klign's CIGAR-reporting run on the CPU: every record carries
+    set_sam_string's line; perfect matches report "<n>=" and NM 0; SSW's
+    CIGARs span the whole contig window."""
+    _need_harness()
+    if not have_ref():
+        pytest.skip("reference SSW library not built")
+    s, rows = run_harness(tmp_path, "cpu", "--mode", "cpu", "--reads", 4000, "--seed", 3, "--report-cigar")
+    assert s["perfect"] > 0 and s["stolen"] > 0
+    import re
+    for r in rows:
+        sam = r[14:]  # read, flag, contig, pos, mapq, cigar, * 0 0 * *, AS, NM, RG
+        assert len(sam) == 14, r
+        assert sam[0] == r[0] and sam[1] == ("0" if r[8] == "Plus" else "16") and sam[2] == r[4] and sam[3] == r[5]
+        assert sam[6:11] == ["*", "0", "0", "*", "*"]
+        assert sam[11] == f"AS:i:{r[9]}" and sam[12] == f"NM:i:{r[12]}" and sam[13] == "RG:Z:0"
+        cig = re.findall(r"(\d+)([MIDNSHP=X])", sam[5])
+        assert "".join(n + o for n, o in cig) == sam[5]
+        if r[13] == "P":
+            assert sam[5] == f"{int(r[6]) - int(r[5]) + 1}=" and r[12] == "0"
+        else:
+            # SSW's CIGAR walks the whole SSW-query (the contig window) with soft clips
+            qlen = sum(int(n) for n, o in cig if o in "SM=XI")
+            assert qlen > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scoring", ["1,3,5,2,2", "1,1,1,1,1"])
+def test_klign_gpu_cigar_records_identical(tmp_path, scoring):
+    """klign's CIGAR-reporting run with blocks on the GPU (the driver's
+    CIGARs, mismatches and score2) writes exactly the records and SAM lines
+    of the all-CPU run, with work-steal mixing both sides."""
+    _need_harness()
+    if not have_ref():
+        pytest.skip("needs the reference SSW for the CPU side")
+    exp_s, exp = run_harness(tmp_path, "cpu", "--mode", "cpu", "--reads", 30000, "--seed", 17, "--scoring", scoring,
+                             "--report-cigar")
+    got_s, got = run_harness(tmp_path, "gpu", "--mode", "gpu", "--reads", 30000, "--seed", 17, "--scoring", scoring,
+                             "--report-cigar", "--gpu-cigar", "--block", 4000, "--gpu-delay-ms", 5)
+    print(json.dumps(got_s), flush=True)
+    assert got_s["gpu"] > 0 and got_s["stolen"] > 0
+    assert len(exp) == len(got)
+    bad = [(e, g) for e, g in zip(exp, got) if e[:13] + e[14:] != g[:13] + g[14:]]
+    assert not bad, "%d records differ, first: %r" % (len(bad), bad[:2])
+
+
+@pytest.mark.gpu
+def test_klign_report_cigar_keeps_blocks_on_cpu(tmp_path):
+    """Without the driver's CIGARs, klign sends CIGAR-reporting blocks to
+    ssw_align_block (src/klign.cpp:589-599): no pair is aligned on the GPU."""
+    _need_harness()
+    exp_s, exp = run_harness(tmp_path, "cpu", "--mode", "cpu", "--reads", 5000, "--seed", 19, "--report-cigar")
+    got_s, got = run_harness(tmp_path, "gpu", "--mode", "gpu", "--reads", 5000, "--seed", 19, "--report-cigar")
+    assert got_s["gpu"] == 0
+    assert [r[:13] + r[14:] for r in exp] == [r[:13] + r[14:] for r in got]
