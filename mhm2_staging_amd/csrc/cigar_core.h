@@ -16,9 +16,10 @@
 //     open/extend choice for a cell differs from the clamped one only where
 //     the gap value is <= 0, which no traceback reads (a traceback enters a
 //     gap state only where that gap value is positive);
-//   * per cell 4 direction bits: bit 0 "H did not take the diagonal"
-//     (max(E, F) > H(i-1,j-1) + s), bit 1 "H took E" (E > F), bit 2 "E was
-//     opened" (H(i-1,j) - gO > E(i-1,j) - gE), bit 3 "F was opened";
+//   * per cell 4 direction bits: bits 0-1 what H took -- 0 the diagonal
+//     (max(E, F) <= H(i-1,j-1) + s) with equal codes, 1 the diagonal with
+//     different codes, 2 E (E > F), 3 F -- bit 2 "E was opened" (H(i-1,j) -
+//     gO > E(i-1,j) - gE), bit 3 "F was opened";
 //   * the band holds the score exactly when the rectangle's last cell does:
 //     every cell of the rectangle is at most the forward score and only the
 //     end point reaches it (the end is the least such cell);
@@ -27,7 +28,10 @@
 //     The CIGAR then walks from (rb, qb): M positions compare the codes
 //     ('=' when equal, N against N included, else 'X'), so when the
 //     traceback stops at column jf > 0 every compared SSW-ref position is
-//     shifted by jf, as in SSW;
+//     shifted by jf, as in SSW.  A traceback that stops at column 0 (always,
+//     when match <= gap_open: a path that starts with a gap never beats the
+//     one that skips it) takes '=' / 'X' from the cells' equality bits; one
+//     that stops elsewhere is walked again comparing shifted codes;
 //   * soft clips: qb bases before, n - 1 - qe after; mismatches = X + I + D.
 //
 // Band cells are addressed by diagonal d = j - i + BW in [0, 2*BW]; a row
@@ -119,6 +123,34 @@ MHMSW_HD inline int32_t subst_lookup(uint32_t lo, uint32_t hi, uint32_t rc) {
 
 constexpr uint32_t kCigKill = 5;  // SSW-ref code left of column 0
 
+// The 4 direction bits of a cell (t1 = max(E, F), t2 = H(i-1,j-1) + s).
+MHMSW_HD inline uint32_t cell_bits(int32_t t1, int32_t t2, int32_t e, int32_t f, bool eq, bool eopen, bool fopen) {
+  const uint32_t h = t1 > t2 ? (e > f ? 2u : 3u) : (eq ? 0u : 1u);
+  return h | (eopen ? 4u : 0u) | (fopen ? 8u : 0u);
+}
+
+// One traceback step at a cell with bits nib from state (0 E, 1 F, 2 H):
+// the op (kOpEq / kOpX for the diagonal as the equality bit says, kOpI,
+// kOpD); state becomes the state at the next cell.
+MHMSW_HD inline uint32_t step_op(uint32_t nib, int& state) {
+  if (state == 2) {
+    const uint32_t h = nib & 3u;
+    if (h < 2u) return h == 0u ? kOpEq : kOpX;
+    if (h == 2u) {
+      state = (nib & 4u) ? 2 : 0;
+      return kOpI;
+    }
+    state = (nib & 8u) ? 2 : 1;
+    return kOpD;
+  }
+  if (state == 0) {
+    state = (nib & 4u) ? 2 : 0;
+    return kOpI;
+  }
+  state = (nib & 8u) ? 2 : 1;
+  return kOpD;
+}
+
 // Band DP for half-width BW (template) and the pair's half-width bw (== BW
 // unless MASKED).  Writes 4 direction bits per cell: row i, diagonal d at
 // bit 4*(d % 8) of dword dirs[(i * DW + d / 8) * stride].  Returns H of the
@@ -141,7 +173,8 @@ MHMSW_HD int32_t cigar_band_dp(const uint32_t* codes, const CigarGeo& g, int32_t
   }
   for (int32_t i = 0; i < ql; i++) {
     uint32_t tlo, thi;
-    subst_table(qcur.get(codes, g.qbase + g.qb + i), match, mismatch, amb, tlo, thi);
+    const uint32_t qc = qcur.get(codes, g.qbase + g.qb + i);
+    subst_table(qc, match, mismatch, amb, tlo, thi);
     int32_t hl = 0, fl = 0;
     uint32_t dw[DW];
 #pragma unroll
@@ -157,7 +190,7 @@ MHMSW_HD int32_t cigar_band_dp(const uint32_t* codes, const CigarGeo& g, int32_t
       const int32_t t2 = H[d] + subst_lookup(tlo, thi, rc[d]);
       const int32_t t1 = imax(e, f);
       int32_t h = imax(t1, t2);
-      const uint32_t bits = (t1 > t2 ? 1u : 0u) | (e > f ? 2u : 0u) | (eo > ee ? 4u : 0u) | (fo > fe ? 8u : 0u);
+      const uint32_t bits = cell_bits(t1, t2, e, f, rc[d] == qc, eo > ee, fo > fe);
       if (MASKED) {
         const int32_t off = d > BW ? d - BW : BW - d;
         if (off > bw) {
@@ -210,67 +243,105 @@ struct CigarEmit {
 };
 
 // Traceback over the direction bits of a band laid out with half-width bw
-// (diagonal index d = j - i + bw), emitting the final CIGAR backwards.
-// shift = the column where the traceback stops (SSW-ref positions of M are
-// compared at rb + j - shift).  Returns that column, or -1 if the traceback
-// would leave the band or the rectangle (no correct input does; the reads
-// stay inside the band's storage either way).
+// (diagonal index d = j - i + bw; row i, diagonal d at bit 4*(d % 8) of
+// dirs[(i * DW + d / 8) * stride]), emitting the final CIGAR backwards.
+// shift = the column where the traceback stops: 0 takes '=' / 'X' from the
+// equality bits (eq00 = SSW-query[qb] == SSW-ref[rb] for the last M), any
+// other value compares the codes at rb + j - shift.  Returns the column where
+// the traceback stopped, or -1 if it would leave the band or the rectangle
+// (no correct input does; reads stay inside the band's storage either way).
 MHMSW_HD inline int32_t cigar_walk(const uint32_t* codes, const CigarGeo& g, int32_t bw, const uint32_t* dirs, int DW,
-                                   int64_t stride, int32_t shift, CigarEmit& em) {
+                                   int64_t stride, int32_t shift, bool eq00, CigarEmit& em) {
   NibCursor rcur, qcur;
   em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
   int32_t i = g.ql() - 1, j = g.rl() - 1;
   int32_t d = j - i + bw;
-  int state = 2;  // 0 E, 1 F, 2 H
-  int64_t cached_row = -1;
+  int state = 2;
+  int64_t cached_at = -1;
   uint32_t cached = 0;
   while (i > 0) {
     if (d < 0 || d > 2 * bw || j < 0) return -1;
     const int64_t at = ((int64_t)i * DW + (d >> 3)) * stride;
-    if (at != cached_row) {
-      cached_row = at;
+    if (at != cached_at) {
+      cached_at = at;
       cached = dirs[at];
     }
-    const uint32_t nib = (cached >> ((uint32_t)(d & 7) * 4u)) & 0xFu;
-    uint32_t op;
-    if (state == 2) {
-      if (!(nib & 1u)) {
-        op = kOpM;
-      } else if (nib & 2u) {
-        op = kOpI;
-        state = (nib & 4u) ? 2 : 0;
-      } else {
-        op = kOpD;
-        state = (nib & 8u) ? 2 : 1;
+    uint32_t op = step_op((cached >> ((uint32_t)(d & 7) * 4u)) & 0xFu, state);
+    if (op == kOpEq || op == kOpX) {
+      if (shift != 0) {
+        const uint32_t a = qcur.get(codes, g.qbase + g.qb + i);
+        const uint32_t b = rcur.get(codes, g.rbase + g.rb + j - shift);
+        op = a == b ? kOpEq : kOpX;
       }
-    } else if (state == 0) {
-      op = kOpI;
-      state = (nib & 4u) ? 2 : 0;
-    } else {
-      op = kOpD;
-      state = (nib & 8u) ? 2 : 1;
-    }
-    if (op == kOpM) {
-      const uint32_t a = qcur.get(codes, g.qbase + g.qb + i);
-      const uint32_t b = rcur.get(codes, g.rbase + g.rb + j - shift);
-      em.push(a == b ? kOpEq : kOpX, 1);
       i--;
       j--;
     } else if (op == kOpI) {
-      em.push(kOpI, 1);
       i--;
       d++;
     } else {
-      em.push(kOpD, 1);
       j--;
       d--;
     }
+    em.push(op, 1);
   }
-  {
-    const uint32_t a = qcur.get(codes, g.qbase + g.qb);
-    const uint32_t b = rcur.get(codes, g.rbase + g.rb + j - shift);
-    em.push(a == b ? kOpEq : kOpX, 1);
+  bool last_eq = eq00;
+  if (shift != 0) last_eq = nib_at(codes, g.qbase + g.qb) == nib_at(codes, g.rbase + g.rb + j - shift);
+  em.push(last_eq ? kOpEq : kOpX, 1);
+  em.push(kOpS, (uint32_t)g.qb);
+  em.flush();
+  return j;
+}
+
+// The same traceback for a band of the register classes (stride 64, DW
+// dwords per row) with shift 0: direction rows are loaded GR at a time, all
+// loads of a group in flight together, so a traceback waits for memory once
+// per GR rows instead of once per row.
+template <int DW>
+MHMSW_HD int32_t cigar_walk_rows(const CigarGeo& g, int32_t bw, const uint32_t* dirs, bool eq00, CigarEmit& em) {
+  constexpr int GR = DW > 2 ? 4 : 8;
+  em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
+  int32_t i = g.ql() - 1, j = g.rl() - 1;
+  int32_t d = j - i + bw;
+  int state = 2;
+  uint32_t rows[GR][DW];
+  uint32_t cur[DW];
+  int32_t lo = i + 1, cur_row = -1;
+  while (i > 0) {
+    if (d < 0 || d > 2 * bw || j < 0) return -1;
+    if (i != cur_row) {
+      if (i < lo) {
+        lo = i - (GR - 1);
+#pragma unroll
+        for (int k = 0; k < GR; k++) {
+          const int64_t r = lo + k > 0 ? lo + k : 0;
+#pragma unroll
+          for (int w = 0; w < DW; w++) rows[k][w] = dirs[(r * DW + w) * 64];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < GR; k++) {
+#pragma unroll
+        for (int w = 0; w < DW; w++) cur[w] = (lo + k == i) ? rows[k][w] : cur[w];
+      }
+      cur_row = i;
+    }
+    uint32_t word = cur[0];
+#pragma unroll
+    for (int w = 1; w < DW; w++) word = (d >> 3) == w ? cur[w] : word;
+    const uint32_t op = step_op((word >> ((uint32_t)(d & 7) * 4u)) & 0xFu, state);
+    if (op == kOpEq || op == kOpX) {
+      i--;
+      j--;
+    } else if (op == kOpI) {
+      i--;
+      d++;
+    } else {
+      j--;
+      d--;
+    }
+    em.push(op, 1);
   }
+  em.push(eq00 ? kOpEq : kOpX, 1);
   em.push(kOpS, (uint32_t)g.qb);
   em.flush();
   return j;
@@ -307,7 +378,8 @@ MHMSW_HD inline int32_t cigar_band_dp_any(const uint32_t* codes, const CigarGeo&
   }
   for (int32_t i = 0; i < ql; i++) {
     uint32_t tlo, thi;
-    subst_table(nib_at(codes, g.qbase + g.qb + i), match, mismatch, amb, tlo, thi);
+    const uint32_t qc = nib_at(codes, g.qbase + g.qb + i);
+    subst_table(qc, match, mismatch, amb, tlo, thi);
     int32_t hl = 0, fl = 0;
     uint32_t cur = 0;
     for (int32_t d = 0; d < W; d++) {
@@ -323,7 +395,7 @@ MHMSW_HD inline int32_t cigar_band_dp_any(const uint32_t* codes, const CigarGeo&
       const int32_t t2 = hrow[d] + subst_lookup(tlo, thi, r);
       const int32_t t1 = imax(e, f);
       const int32_t h = imax(t1, t2);
-      const uint32_t bits = (t1 > t2 ? 1u : 0u) | (e > f ? 2u : 0u) | (eo > ee ? 4u : 0u) | (fo > fe ? 8u : 0u);
+      const uint32_t bits = cell_bits(t1, t2, e, f, r == qc, eo > ee, fo > fe);
       cur |= bits << ((4 * (d & 7)));
       if ((d & 7) == 7 || d == W - 1) {
         dirs[(int64_t)i * DW + (d >> 3)] = cur;

@@ -84,27 +84,44 @@ __device__ __forceinline__ void enqueue(const CigarView& v, int cls, int64_t p, 
   v.lists[(int64_t)cls * v.npairs + at] = (int32_t)p;
 }
 
-// Counts, recounts when the traceback stops off column 0, and returns the
-// shift for the writing walk (-1: the traceback left the band).
-__device__ __forceinline__ int32_t count_walks(const CigarView& v, const CigarGeo& g, int32_t layout_bw,
-                                               const uint32_t* dirs, int DW, int64_t stride, CigarEmit& em) {
+// Work index of the wavefront: one atomic from lane 0, broadcast.
+__device__ __forceinline__ int32_t next_work(int32_t* ctr) {
+  int32_t w = 0;
+  if ((threadIdx.x & 63) == 0) w = atomicAdd(ctr, 1);
+  return lane0(w);
+}
+
+// Whether SSW-query[qb] equals SSW-ref[rb] (the traceback's last M).
+__device__ __forceinline__ bool first_eq(const CigarView& v, const CigarGeo& g) {
+  return nib_at(v.codes, g.qbase + g.qb) == nib_at(v.codes, g.rbase + g.rb);
+}
+
+// Counting walks over a register-class band: rows fetched in groups and
+// '=' / 'X' from the equality bits; when the traceback stops off column 0
+// the count is redone comparing shifted codes.  Returns the shift of the
+// writing walk (-1: the traceback left the band).
+template <int DW>
+__device__ __forceinline__ int32_t count_band(const CigarView& v, const CigarGeo& g, int32_t bw, const uint32_t* dirs,
+                                              bool eq00, CigarEmit& em) {
   em = CigarEmit();
   em.out = nullptr;
   em.nops = 0;
-  int32_t jf = cigar_walk(v.codes, g, layout_bw, dirs, DW, stride, 0, em);
+  int32_t jf = cigar_walk_rows<DW>(g, bw, dirs, eq00, em);
   if (jf > 0) {
     em = CigarEmit();
     em.out = nullptr;
     em.nops = 0;
-    jf = cigar_walk(v.codes, g, layout_bw, dirs, DW, stride, jf, em);
+    jf = cigar_walk(v.codes, g, bw, dirs, DW, 64, jf, eq00, em);
   }
   return jf;
 }
 
-// Reserves the ops of every lane's finished pair and writes them.
+// Reserves the ops of every lane's finished pair and writes them (all lanes
+// of the wavefront call it).  BAND: dirs is a register-class slot.
+template <int DW, bool BAND>
 __device__ __forceinline__ void finish_pairs(const CigarView& v, int64_t p, bool done, int32_t jf, const CigarGeo& g,
-                                             int32_t layout_bw, const uint32_t* dirs, int DW, int64_t stride,
-                                             const CigarEmit& em) {
+                                             int32_t layout_bw, const uint32_t* dirs, int dwn, int64_t stride,
+                                             bool eq00, const CigarEmit& em) {
   const bool ok = done && jf >= 0;
   const int64_t base = wave_reserve(&v.used[0], ok ? em.count : 0);
   if (!done) return;
@@ -120,7 +137,11 @@ __device__ __forceinline__ void finish_pairs(const CigarView& v, int64_t p, bool
     CigarEmit w;
     w.out = v.ops + base;
     w.nops = em.count;
-    cigar_walk(v.codes, g, layout_bw, dirs, DW, stride, jf, w);
+    if (BAND && jf == 0) {
+      cigar_walk_rows<DW>(g, layout_bw, dirs, eq00, w);
+    } else {
+      cigar_walk(v.codes, g, layout_bw, dirs, dwn, stride, jf, eq00, w);
+    }
   }
   v.n_ops[p] = fits ? em.count : -1;
   v.mism[p] = em.mism;
@@ -136,64 +157,72 @@ __global__ void cigar_clear_kernel(CigarView v) {
   if (t == 0) v.max_rows[0] = 0;
 }
 
+// Each 256-thread block classifies chunks of kCigClassifyPerThread*256
+// pairs: ranks inside a class from LDS atomics, then one global atomic per
+// class used by the chunk reserves its range of the class list.
+constexpr int kCigClassifyPerThread = 8;
+constexpr int kCigChunk = 256 * kCigClassifyPerThread;
+
 __global__ __launch_bounds__(256) void cigar_classify_kernel(CigarView v) {
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // wave-uniform trip count so that every lane reaches the aggregated appends
-  const int64_t first = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
-  for (int64_t p0 = first; p0 < v.npairs; p0 += stride) {
-    const int64_t p = p0 + lane;
-    const bool live = p < v.npairs;
-    int cls = -1;
-    int32_t trivial_n = 0, band_rows = 0;
-    CigarGeo g;
-    CigarEmit em;
-    em.out = nullptr;
-    em.nops = 0;
-    if (live) {
-      g = pair_geo(v, p);
-      if (cigar_trivial(v.codes, g, em)) {
-        trivial_n = em.count;
-      } else {
-        const int32_t bw = (g.rl() > g.ql() ? g.rl() - g.ql() : g.ql() - g.rl()) + 1;
-        cls = cigar_class(bw, g.ql());
-        v.band[p] = bw;
-        if (cls != kCigClassGeneric) band_rows = g.ql();
-      }
-    }
-    // tallest rectangle of the register classes (sizes the scratch slots)
+  __shared__ int32_t cnt[kCigClasses], base[kCigClasses], rows_max;
+  const int t = threadIdx.x;
+  for (int64_t chunk = blockIdx.x; chunk * kCigChunk < v.npairs; chunk += gridDim.x) {
+    if (t < kCigClasses) cnt[t] = 0;
+    if (t == 0) rows_max = 0;
+    __syncthreads();
+    int cls[kCigClassifyPerThread];
+    int32_t rank[kCigClassifyPerThread];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) band_rows = std::max(band_rows, __shfl_xor(band_rows, o, 64));
-    if (lane == 0 && band_rows > 0) atomicMax(&v.max_rows[0], band_rows);
-    // trivial pairs: reserve and write now
-    const bool triv = live && cls < 0;
-    const int64_t base = wave_reserve(&v.used[0], triv ? trivial_n : 0);
-    if (triv) {
-      const bool fits = base + trivial_n <= v.ops_cap;
-      if (fits && trivial_n > 0) {
-        CigarEmit w;
-        w.out = v.ops + base;
-        w.nops = trivial_n;
-        cigar_trivial(v.codes, g, w);
+    for (int k = 0; k < kCigClassifyPerThread; k++) {
+      const int64_t p = chunk * kCigChunk + k * 256 + t;
+      const bool live = p < v.npairs;
+      cls[k] = -1;
+      rank[k] = 0;
+      int32_t trivial_n = 0;
+      CigarGeo g;
+      CigarEmit em;
+      em.out = nullptr;
+      em.nops = 0;
+      if (live) {
+        g = pair_geo(v, p);
+        if (cigar_trivial(v.codes, g, em)) {
+          trivial_n = em.count;
+        } else {
+          const int32_t bw = (g.rl() > g.ql() ? g.rl() - g.ql() : g.ql() - g.rl()) + 1;
+          cls[k] = cigar_class(bw, g.ql());
+          v.band[p] = bw;
+          rank[k] = atomicAdd(&cnt[cls[k]], 1);
+          if (cls[k] != kCigClassGeneric) atomicMax(&rows_max, g.ql());
+        }
       }
-      v.n_ops[p] = fits ? trivial_n : -1;
-      v.mism[p] = em.mism;
-      v.off[p] = base;
+      // trivial pairs (empty query, distance filter, score 0): reserve and write now
+      const bool triv = live && cls[k] < 0;
+      const int64_t at = wave_reserve(&v.used[0], triv ? trivial_n : 0);
+      if (triv) {
+        const bool fits = at + trivial_n <= v.ops_cap;
+        if (fits && trivial_n > 0) {
+          CigarEmit w;
+          w.out = v.ops + at;
+          w.nops = trivial_n;
+          cigar_trivial(v.codes, g, w);
+        }
+        v.n_ops[p] = fits ? trivial_n : -1;
+        v.mism[p] = em.mism;
+        v.off[p] = at;
+      }
     }
-    // band pairs: one atomic per class present in the wavefront
+    __syncthreads();
+    if (t < kCigClasses && cnt[t] > 0) base[t] = atomicAdd(&v.list_count[t], cnt[t]);
+    if (t == 0 && rows_max > 0) atomicMax(&v.max_rows[0], rows_max);
+    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kCigClasses; c++) {
-      const uint64_t m = __ballot(cls == c);
-      if (m == 0) continue;
-      const int leader = __builtin_ctzll(m);
-      int32_t at = 0;
-      if (lane == leader) at = atomicAdd(&v.list_count[c], __popcll(m));
-      at = __builtin_amdgcn_readlane(at, leader);
-      if (cls == c) {
-        const int rank = __popcll(m & ((1ull << lane) - 1ull));
-        v.lists[(int64_t)c * v.npairs + at + rank] = (int32_t)p;
+    for (int k = 0; k < kCigClassifyPerThread; k++) {
+      if (cls[k] >= 0) {
+        const int64_t p = chunk * kCigChunk + k * 256 + t;
+        v.lists[(int64_t)cls[k] * v.npairs + base[cls[k]] + rank[k]] = (int32_t)p;
       }
     }
+    __syncthreads();
   }
 }
 
@@ -211,11 +240,11 @@ __global__ __launch_bounds__(256) void cigar_band_kernel(CigarView v) {
   const int32_t count = lane0(v.list_count[cls]);
   int32_t* next = &v.list_count[kCigClasses + cls];
   for (;;) {
-    const int32_t w = lane0(atomicAdd(next, lane == 0 ? 1 : 0));
+    const int32_t w = next_work(next);
     if ((int64_t)w * 64 >= count) break;
     const int32_t idx = w * 64 + lane;
     const int64_t p = idx < count ? v.lists[(int64_t)cls * v.npairs + idx] : -1;
-    bool done = false;
+    bool done = false, eq00 = false;
     int32_t jf = -1;
     CigarGeo g;
     CigarEmit em;
@@ -226,12 +255,13 @@ __global__ __launch_bounds__(256) void cigar_band_kernel(CigarView v) {
                                                        v.sc.gap_open, v.sc.gap_ext, dirs, 64);
       if (corner == g.score) {
         done = true;
-        jf = count_walks(v, g, BW, dirs, DW, 64, em);
+        eq00 = first_eq(v, g);
+        jf = count_band<DW>(v, g, BW, dirs, eq00, em);
       } else {
         enqueue(v, cigar_class(2 * bw, g.ql()), p, 2 * bw);
       }
     }
-    finish_pairs(v, p, done, jf, g, BW, dirs, DW, 64, em);
+    finish_pairs<DW, true>(v, p, done, jf, g, BW, dirs, DW, 64, eq00, em);
   }
 }
 
@@ -240,11 +270,11 @@ __global__ __launch_bounds__(64) void cigar_generic_kernel(CigarView v) {
   const int32_t count = lane0(v.list_count[kCigClassGeneric]);
   int32_t* next = &v.list_count[kCigClasses + kCigClassGeneric];
   for (;;) {
-    const int32_t w = lane0(atomicAdd(next, lane == 0 ? 1 : 0));
+    const int32_t w = next_work(next);
     if ((int64_t)w * 64 >= count) break;
     const int32_t idx = w * 64 + lane;
     const int64_t p = idx < count ? v.lists[(int64_t)kCigClassGeneric * v.npairs + idx] : -1;
-    bool done = false;
+    bool done = false, eq00 = false;
     int32_t jf = -1, bw = 0, DW = 0;
     const uint32_t* dirs = nullptr;
     CigarGeo g;
@@ -269,7 +299,17 @@ __global__ __launch_bounds__(64) void cigar_generic_kernel(CigarView v) {
         if (corner == g.score) {
           dirs = mem + 2 * W;
           done = true;
-          jf = count_walks(v, g, bw, dirs, DW, 1, em);
+          eq00 = first_eq(v, g);
+          em = CigarEmit();
+          em.out = nullptr;
+          em.nops = 0;
+          jf = cigar_walk(v.codes, g, bw, dirs, DW, 1, 0, eq00, em);
+          if (jf > 0) {
+            em = CigarEmit();
+            em.out = nullptr;
+            em.nops = 0;
+            jf = cigar_walk(v.codes, g, bw, dirs, DW, 1, jf, eq00, em);
+          }
           break;
         }
         if (bw > g.rl() + g.ql() + 2) {  // a band this wide holds the whole rectangle
@@ -280,7 +320,7 @@ __global__ __launch_bounds__(64) void cigar_generic_kernel(CigarView v) {
         bw *= 2;
       }
     }
-    finish_pairs(v, p, done, jf, g, bw, dirs, DW, 1, em);
+    finish_pairs<1, false>(v, p, done, jf, g, bw, dirs, DW, 1, eq00, em);
   }
 }
 
@@ -294,7 +334,7 @@ static int cig_grid(int64_t n, int block) {
 hipError_t launch_cigar(const CigarView& v, int band_waves, int generic_waves, hipStream_t st) {
   hipLaunchKernelGGL(cigar_clear_kernel, dim3(1), dim3(64), 0, st, v);
   if (v.npairs <= 0) return hipGetLastError();
-  hipLaunchKernelGGL(cigar_classify_kernel, dim3(cig_grid(v.npairs, 256 * 4)), dim3(256), 0, st, v);
+  hipLaunchKernelGGL(cigar_classify_kernel, dim3(cig_grid(v.npairs, kCigChunk)), dim3(256), 0, st, v);
   const int blocks = std::max(1, band_waves / 4);
   hipLaunchKernelGGL((cigar_band_kernel<1, false>), dim3(blocks), dim3(256), 0, st, v);
   hipLaunchKernelGGL((cigar_band_kernel<2, false>), dim3(blocks), dim3(256), 0, st, v);

@@ -132,16 +132,29 @@ int main() {
         bw *= 2;
       }
       cls_count[first_cls]++;
+      // as the kernels: band classes walk rows in groups first, the generic
+      // class walks row by row; a traceback that stops off column 0 is
+      // walked again comparing shifted codes
+      const bool eq00 = nib_at(codes.data(), g.qbase + g.qb) == nib_at(codes.data(), g.rbase + g.rb);
+      const bool band = stride == 64;
+      auto fast_walk = [&](CigarEmit &e) -> int32_t {
+        switch (dwn) {
+          case 1: return cigar_walk_rows<1>(g, layout, dirs, eq00, e);
+          case 2: return cigar_walk_rows<2>(g, layout, dirs, eq00, e);
+          case 4: return cigar_walk_rows<4>(g, layout, dirs, eq00, e);
+          default: return -9;
+        }
+      };
       CigarEmit c1;
       c1.out = nullptr;
       c1.nops = 0;
-      int32_t jf = cigar_walk(codes.data(), g, layout, dirs, dwn, stride, 0, c1);
+      int32_t jf = band ? fast_walk(c1) : cigar_walk(codes.data(), g, layout, dirs, dwn, stride, 0, eq00, c1);
       CigarEmit c2 = c1;
       if (jf > 0) {
         c2 = CigarEmit();
         c2.out = nullptr;
         c2.nops = 0;
-        jf = cigar_walk(codes.data(), g, layout, dirs, dwn, stride, jf, c2);
+        jf = cigar_walk(codes.data(), g, layout, dirs, dwn, stride, jf, eq00, c2);
       }
       if (jf < 0) {
         got_n = -1;
@@ -150,7 +163,11 @@ int main() {
         CigarEmit w;
         w.out = got.data();
         w.nops = c2.count;
-        cigar_walk(codes.data(), g, layout, dirs, dwn, stride, jf, w);
+        if (band && jf == 0) {
+          fast_walk(w);
+        } else {
+          cigar_walk(codes.data(), g, layout, dirs, dwn, stride, jf, eq00, w);
+        }
         got_n = c2.count;
         got_mism = c2.mism;
         if (w.count != c2.count || w.mism != c2.mism) got_n = -2;
