@@ -52,21 +52,17 @@ namespace mhmsw {
 constexpr uint32_t kOpM = 0, kOpI = 1, kOpD = 2, kOpS = 4, kOpEq = 7, kOpX = 8;
 // SSW's distance filter for klign's Filter (src/ssw/ssw.hpp:59-64)
 constexpr int32_t kCigarDistanceFilter = 32767;
-// band classes: half-width 1..7 exact (class bw - 1), 8..15 in a 31-wide
-// band with the outer diagonals masked (class 7), wider in the generic class
+// band classes: half-width 1..7 one lane per pair with the band row in
+// registers (class bw - 1); wider bands, and rectangles taller than
+// kCigNarrowMaxRows, one wavefront per pair (the wide class)
 constexpr int kCigExactMaxBw = 7;
-constexpr int kCigMaskedBw = 15;
-constexpr int kCigClassMasked = 7;
-constexpr int kCigClassGeneric = 8;
-constexpr int kCigClasses = 9;
-// rectangles taller than this always take the generic class
-constexpr int32_t kCigMaxRows = 65536;
+constexpr int kCigClassWide = 7;
+constexpr int kCigClasses = 8;
+constexpr int32_t kCigNarrowMaxRows = 2048;
 
 MHMSW_HD inline int cigar_class(int32_t bw, int32_t rows) {
-  if (rows > kCigMaxRows) return kCigClassGeneric;
-  if (bw <= kCigExactMaxBw) return bw - 1;
-  if (bw <= kCigMaskedBw) return kCigClassMasked;
-  return kCigClassGeneric;
+  if (rows > kCigNarrowMaxRows || bw > kCigExactMaxBw) return kCigClassWide;
+  return bw - 1;
 }
 
 MHMSW_HD inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
@@ -151,65 +147,96 @@ MHMSW_HD inline uint32_t step_op(uint32_t nib, int& state) {
   return kOpD;
 }
 
-// Band DP for half-width BW (template) and the pair's half-width bw (== BW
-// unless MASKED).  Writes 4 direction bits per cell: row i, diagonal d at
-// bit 4*(d % 8) of dword dirs[(i * DW + d / 8) * stride].  Returns H of the
-// rectangle's last cell.
-template <int BW, bool MASKED>
-MHMSW_HD int32_t cigar_band_dp(const uint32_t* codes, const CigarGeo& g, int32_t bw, int32_t match, int32_t mismatch,
-                               int32_t amb, int32_t gO, int32_t gE, uint32_t* dirs, int64_t stride) {
+// 8 nibble codes starting at nibble index a (any alignment): two dwords and
+// a funnel shift.
+MHMSW_HD inline uint32_t load8(const uint32_t* codes, int64_t a) {
+  const uint32_t lo = codes[a >> 3], hi = codes[(a >> 3) + 1];
+  const uint32_t sh = (uint32_t)(a & 7) * 4u;
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_alignbit(hi, lo, sh);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> sh);
+#endif
+}
+
+// Direction layouts: 4 bits per cell, diagonal d at bit 4*(d % 8) of word
+// d / 8 of its row; a row takes DS words (at least the (4W+31)/32 its W
+// cells need).
+//  * blocked (narrow bands, one lane per pair): rows 8b..8b+7 of a lane are
+//    8*DS consecutive words, the blocks of the 64 lanes of a wavefront follow
+//    each other, so a lane reads or writes 8 rows with 2*DS 16-byte
+//    accesses; dirs points at the lane's first block;
+//  * row-major (the serial path of the widest bands): row i at i * DS.
+MHMSW_HD inline int64_t dir_at(bool blocked, int DS, int32_t i, int32_t k) {
+  return blocked ? (int64_t)(i >> 3) * 512 * DS + (i & 7) * DS + k : (int64_t)i * DS + k;
+}
+
+struct Words4 {
+  uint32_t w[4];
+};
+
+// Band DP for half-width BW, directions in the blocked layout with DS words
+// per row.
+// Returns H of the rectangle's last cell.  Rows go in blocks of 8 whose codes
+// (one SSW-query code per row, the SSW-ref code entering the band) are loaded
+// one block ahead, so no row waits for memory.
+template <int BW, int DS>
+MHMSW_HD int32_t cigar_band_dp(const uint32_t* codes, const CigarGeo& g, int32_t match, int32_t mismatch, int32_t amb,
+                               int32_t gO, int32_t gE, uint32_t* dirs) {
   constexpr int W = 2 * BW + 1;
-  constexpr int DW = (4 * W + 31) / 32;
+  static_assert((4 * W + 31) / 32 <= DS, "a row's words");
   const int32_t rl = g.rl(), ql = g.ql();
   int32_t H[W], E[W];
   uint32_t rc[W];
-  NibCursor rcur, qcur;
 #pragma unroll
   for (int d = 0; d < W; d++) {
     H[d] = 0;
     E[d] = 0;
     const int32_t j = d - BW;
-    rc[d] = j < 0 ? kCigKill : rcur.get(codes, g.rbase + g.rb + j);
+    rc[d] = j < 0 ? kCigKill : nib_at(codes, g.rbase + g.rb + j);
   }
-  for (int32_t i = 0; i < ql; i++) {
-    uint32_t tlo, thi;
-    const uint32_t qc = qcur.get(codes, g.qbase + g.qb + i);
-    subst_table(qc, match, mismatch, amb, tlo, thi);
-    int32_t hl = 0, fl = 0;
-    uint32_t dw[DW];
+  // row i's SSW-query code at qa + i; the SSW-ref code entering the band
+  // after row i (column i + 1 + BW) at ra + i
+  const int64_t qa = g.qbase + g.qb, ra = g.rbase + g.rb + BW + 1;
+  uint32_t qn = load8(codes, qa), rn = load8(codes, ra);
+  for (int32_t i0 = 0; i0 < ql; i0 += 8) {
+    const uint32_t qw = qn, rw = rn;
+    qn = load8(codes, qa + i0 + 8);
+    rn = load8(codes, ra + i0 + 8);
+    uint32_t* out = dirs + dir_at(true, DS, i0, 0);
 #pragma unroll
-    for (int k = 0; k < DW; k++) dw[k] = 0;
+    for (int r = 0; r < 8; r++) {
+      if (i0 + r >= ql) break;
+      const uint32_t qc = (qw >> (4 * r)) & 0xFu;
+      uint32_t tlo, thi;
+      subst_table(qc, match, mismatch, amb, tlo, thi);
+      int32_t hl = 0, fl = 0;
+      uint32_t dw[DS];
 #pragma unroll
-    for (int d = 0; d < W; d++) {
-      const int32_t hu = d + 1 < W ? H[d + 1] : 0;
-      const int32_t eu = d + 1 < W ? E[d + 1] : 0;
-      const int32_t eo = hu - gO, ee = eu - gE;
-      int32_t e = imax3(eo, ee, 0);
-      const int32_t fo = hl - gO, fe = fl - gE;
-      int32_t f = imax3(fo, fe, 0);
-      const int32_t t2 = H[d] + subst_lookup(tlo, thi, rc[d]);
-      const int32_t t1 = imax(e, f);
-      int32_t h = imax(t1, t2);
-      const uint32_t bits = cell_bits(t1, t2, e, f, rc[d] == qc, eo > ee, fo > fe);
-      if (MASKED) {
-        const int32_t off = d > BW ? d - BW : BW - d;
-        if (off > bw) {
-          h = 0;
-          e = 0;
-          f = 0;
-        }
+      for (int k = 0; k < DS; k++) dw[k] = 0;
+#pragma unroll
+      for (int d = 0; d < W; d++) {
+        const int32_t hu = d + 1 < W ? H[d + 1] : 0;
+        const int32_t eu = d + 1 < W ? E[d + 1] : 0;
+        const int32_t eo = hu - gO, ee = eu - gE;
+        const int32_t e = imax3(eo, ee, 0);
+        const int32_t fo = hl - gO, fe = fl - gE;
+        const int32_t f = imax3(fo, fe, 0);
+        const int32_t t2 = H[d] + subst_lookup(tlo, thi, rc[d]);
+        const int32_t t1 = imax(e, f);
+        const int32_t h = imax(t1, t2);
+        dw[(4 * d) / 32] |= cell_bits(t1, t2, e, f, rc[d] == qc, eo > ee, fo > fe) << ((4 * d) % 32);
+        H[d] = h;
+        E[d] = e;
+        hl = h;
+        fl = f;
       }
-      dw[(4 * d) / 32] |= bits << ((4 * d) % 32);
-      H[d] = h;
-      E[d] = e;
-      hl = h;
-      fl = f;
+#pragma unroll
+      for (int k = 0; k < DS; k++) out[r * DS + k] = dw[k];
+#pragma unroll
+      for (int d = 0; d < W - 1; d++) rc[d] = rc[d + 1];
+      rc[W - 1] = (rw >> (4 * r)) & 0xFu;
     }
-#pragma unroll
-    for (int k = 0; k < DW; k++) dirs[((int64_t)i * DW + k) * stride] = dw[k];
-#pragma unroll
-    for (int d = 0; d < W - 1; d++) rc[d] = rc[d + 1];
-    rc[W - 1] = rcur.get(codes, g.rbase + g.rb + i + 1 + BW);
   }
   // last cell: row ql - 1, column rl - 1
   const int32_t dc = (rl - 1) - (ql - 1) + BW;
@@ -242,16 +269,76 @@ struct CigarEmit {
   }
 };
 
+// Run-length emitter for narrow bands that keeps the CIGAR in registers:
+// up to 16 ops between the soft clips as 16-bit halves (length << 4 | op;
+// lengths below 4096, which rectangles of at most kCigNarrowMaxRows rows
+// guarantee) in q[0..7], newest first, and the two clips apart.  The walk
+// emits backwards, so the newest op is the first in CIGAR order: op m of
+// the part between the clips is half m of q.  fits() tells whether the
+// whole CIGAR was kept.
+struct CigarEmitQ {
+  int32_t count = 0;
+  int32_t mism = 0;
+  int32_t body = 0;
+  uint32_t op = 0xFFu;
+  uint32_t len = 0;
+  uint32_t clip_tail = 0, clip_head = 0;
+  bool long_run = false;
+  uint32_t q[8];
+  MHMSW_HD CigarEmitQ() {
+#pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = 0;
+  }
+  MHMSW_HD void flush() {
+    if (len == 0) return;
+    count++;
+    if (op == kOpS) {
+      if (body == 0 && clip_tail == 0 && count == 1) {
+        clip_tail = len;
+      } else {
+        clip_head = len;
+      }
+      len = 0;
+      return;
+    }
+    if (op == kOpX || op == kOpI || op == kOpD) mism += (int32_t)len;
+    long_run = long_run || len >= 4096u;
+    const uint32_t v = ((len << 4) | op) & 0xFFFFu;
+#pragma unroll
+    for (int k = 7; k > 0; k--) q[k] = (q[k] << 16) | (q[k - 1] >> 16);
+    q[0] = (q[0] << 16) | v;
+    body++;
+    len = 0;
+  }
+  MHMSW_HD void push(uint32_t o, uint32_t l) {
+    if (l == 0) return;
+    if (len && o != op) flush();
+    op = o;
+    len += l;
+  }
+  MHMSW_HD bool fits() const { return body <= 16 && !long_run; }
+  // the CIGAR in order into out[0 .. count)
+  MHMSW_HD void write(uint32_t* out) const {
+    int32_t at = 0;
+    if (clip_head) out[at++] = (clip_head << 4) | kOpS;
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (m < body) out[at + m] = (q[m >> 1] >> (16 * (m & 1))) & 0xFFFFu;
+    at += body;
+    if (clip_tail) out[at] = (clip_tail << 4) | kOpS;
+  }
+};
+
 // Traceback over the direction bits of a band laid out with half-width bw
-// (diagonal index d = j - i + bw; row i, diagonal d at bit 4*(d % 8) of
-// dirs[(i * DW + d / 8) * stride]), emitting the final CIGAR backwards.
+// (diagonal index d = j - i + bw; blocked or row-major layout with DW words
+// per row), emitting the final CIGAR backwards.
 // shift = the column where the traceback stops: 0 takes '=' / 'X' from the
 // equality bits (eq00 = SSW-query[qb] == SSW-ref[rb] for the last M), any
 // other value compares the codes at rb + j - shift.  Returns the column where
 // the traceback stopped, or -1 if it would leave the band or the rectangle
 // (no correct input does; reads stay inside the band's storage either way).
 MHMSW_HD inline int32_t cigar_walk(const uint32_t* codes, const CigarGeo& g, int32_t bw, const uint32_t* dirs, int DW,
-                                   int64_t stride, int32_t shift, bool eq00, CigarEmit& em) {
+                                   bool blocked, int32_t shift, bool eq00, CigarEmit& em) {
   NibCursor rcur, qcur;
   em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
   int32_t i = g.ql() - 1, j = g.rl() - 1;
@@ -261,7 +348,7 @@ MHMSW_HD inline int32_t cigar_walk(const uint32_t* codes, const CigarGeo& g, int
   uint32_t cached = 0;
   while (i > 0) {
     if (d < 0 || d > 2 * bw || j < 0) return -1;
-    const int64_t at = ((int64_t)i * DW + (d >> 3)) * stride;
+    const int64_t at = dir_at(blocked, DW, i, d >> 3);
     if (at != cached_at) {
       cached_at = at;
       cached = dirs[at];
@@ -292,36 +379,40 @@ MHMSW_HD inline int32_t cigar_walk(const uint32_t* codes, const CigarGeo& g, int
   return j;
 }
 
-// The same traceback for a band of the register classes (stride 64, DW
-// dwords per row) with shift 0: direction rows are loaded GR at a time, all
-// loads of a group in flight together, so a traceback waits for memory once
-// per GR rows instead of once per row.
-template <int DW>
-MHMSW_HD int32_t cigar_walk_rows(const CigarGeo& g, int32_t bw, const uint32_t* dirs, bool eq00, CigarEmit& em) {
-  constexpr int GR = DW > 2 ? 4 : 8;
+// The same traceback over a narrow band (blocked layout, DS words per row)
+// with shift 0: each block of 8 rows comes in with 2*DS 16-byte loads, so a
+// traceback waits for memory once per 8 rows.
+template <int DS, class Emit>
+MHMSW_HD int32_t cigar_walk_rows(const CigarGeo& g, int32_t bw, const uint32_t* dirs, bool eq00, Emit& em) {
+  constexpr int DW = DS;
   em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
   int32_t i = g.ql() - 1, j = g.rl() - 1;
   int32_t d = j - i + bw;
   int state = 2;
-  uint32_t rows[GR][DW];
+  uint32_t rows[8 * DW];
   uint32_t cur[DW];
-  int32_t lo = i + 1, cur_row = -1;
+#pragma unroll
+  for (int w = 0; w < DW; w++) cur[w] = 0;
+  int32_t blk_lo = i + 1, cur_row = -1;
   while (i > 0) {
     if (d < 0 || d > 2 * bw || j < 0) return -1;
     if (i != cur_row) {
-      if (i < lo) {
-        lo = i - (GR - 1);
+      if (i < blk_lo) {
+        blk_lo = i & ~7;
+        const Words4* src = reinterpret_cast<const Words4*>(dirs + dir_at(true, DW, blk_lo, 0));
 #pragma unroll
-        for (int k = 0; k < GR; k++) {
-          const int64_t r = lo + k > 0 ? lo + k : 0;
-#pragma unroll
-          for (int w = 0; w < DW; w++) rows[k][w] = dirs[(r * DW + w) * 64];
+        for (int k = 0; k < 2 * DW; k++) {
+          const Words4 x = src[k];
+          rows[4 * k] = x.w[0];
+          rows[4 * k + 1] = x.w[1];
+          rows[4 * k + 2] = x.w[2];
+          rows[4 * k + 3] = x.w[3];
         }
       }
 #pragma unroll
-      for (int k = 0; k < GR; k++) {
+      for (int r = 0; r < 8; r++) {
 #pragma unroll
-        for (int w = 0; w < DW; w++) cur[w] = (lo + k == i) ? rows[k][w] : cur[w];
+        for (int w = 0; w < DW; w++) cur[w] = (i & 7) == r ? rows[r * DW + w] : cur[w];
       }
       cur_row = i;
     }
@@ -364,9 +455,10 @@ MHMSW_HD inline bool cigar_trivial(const uint32_t* codes, const CigarGeo& g, Cig
   return true;
 }
 
-// Generic band DP for any half-width (the slow path): rows of H and E in
-// hrow/erow (2*bw+1 entries each), direction dwords row-major with
-// DW = ceil((2*bw+1)/8) dwords per row.  Returns H of the last cell.
+// Band DP for any half-width, one cell at a time (the serial path for bands
+// wider than the wavefront kernel takes): rows of H and E in hrow/erow
+// (2*bw+1 entries each), direction dwords row-major with DW =
+// ceil((2*bw+1)/8) dwords per row.  Returns H of the last cell.
 MHMSW_HD inline int32_t cigar_band_dp_any(const uint32_t* codes, const CigarGeo& g, int32_t bw, int32_t match,
                                           int32_t mismatch, int32_t amb, int32_t gO, int32_t gE, int32_t* hrow,
                                           int32_t* erow, uint32_t* dirs) {

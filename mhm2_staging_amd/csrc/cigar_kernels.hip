@@ -2,24 +2,34 @@
 // pair of a finished job, as klign gets them when it reports CIGARs
 // (src/ssw/ssw_core.cpp:1180-1192 banded_sw, src/ssw/ssw.cpp:57-93 and
 // :129-254; klign.cpp:589-599 keeps such batches off ADEPT's GPU path).
-// The per-pair arithmetic lives in cigar_core.h.
+// The per-pair arithmetic of the narrow bands lives in cigar_core.h.
 //
-//  * cigar_classify_kernel: the rectangle of each pair from its result,
-//    trivial pairs (empty query, distance filter, score 0) written at once,
-//    the rest appended to the list of their band class (band half-width
-//    |columns - rows| + 1: 1..7 exact, 8..15 masked in a 31-wide band,
-//    wider or taller than kCigMaxRows generic), wavefront-aggregated.
-//  * cigar_band_kernel<BW, MASKED>: persistent, one lane per pair, the band
-//    row (2*BW+1 diagonals of H and E) in registers, 4 direction bits per
-//    cell stored lane-interleaved in the wavefront's scratch slot (coalesced
-//    row stores).  A pair whose band misses the score doubles its half-width
-//    and moves to that class's list (launched later).  Then two walks over
-//    the directions count the ops (a third when the traceback stops off
-//    column 0), the wavefront reserves its ops with one atomic, and a last
-//    walk writes them.
-//  * cigar_generic_kernel: the same for any half-width, rows of H/E and the
-//    directions in a bump-allocated pool; pairs that do not fit are counted
-//    and the host reruns the pass with a larger pool.
+//  * cigar_classify_kernel: the rectangle of each pair from its result;
+//    trivial pairs (empty query, distance filter, score 0) written at once;
+//    the rest appended to the list of their band class (half-width
+//    |columns - rows| + 1: 1..7 narrow, wider -- or taller than
+//    kCigNarrowMaxRows -- wide), block-aggregated.
+//  * Narrow bands, one lane per pair, the band row (2*BW+1 diagonals of H
+//    and E) in registers, 4 direction bits per cell stored in the
+//    wavefront's scratch slot in blocks of 8 rows per lane (16-byte stores
+//    and loads): cigar_bw1_kernel for half-width 1 (most pairs),
+//    cigar_narrow_kernel for half-widths 2..7 in two phases.  A pair whose
+//    band misses the score doubles its half-width: to half-width 2 for the
+//    next launch of phase 0, to 4 or 6 for phase 1, to 8 or more for the wide
+//    kernel.  A counting walk over the directions ('=' / 'X' from the
+//    equality bits) that keeps up to 16 ops besides the clips in registers,
+//    one atomic per wavefront to reserve the ops, and a writing walk only
+//    for longer CIGARs.
+//  * cigar_wide_kernel: one wavefront per pair.  Lane l holds diagonals
+//    l + 64c; E and the diagonal come from the previous row (lane shifts),
+//    F from a prefix maximum over the row: with gap_open >= gap_ext,
+//    F(d) = max(0, max_{k<d} (H0(k) - gO - (d-1-k)*gE)) where H0 = max(diag + s,
+//    E, 0), because a gap extended from an F cell costs more than opening it
+//    from that cell's H0.  Directions as one byte per cell; the traceback is
+//    walked by all 64 lanes in step over blocks of rows copied to LDS.  Bands
+//    wider than 1024 diagonals take a serial path on lane 0.
+//  Scratch that a pass finds too small is counted; the host reruns the pass
+//  with what it asked for.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,9 +39,25 @@
 
 namespace mhmsw {
 
+// layout of CigarView::list_count
+constexpr int kCigPhases = 2;
+constexpr int kCigWork = kCigPhases * kCigClasses;  // work counters: bw1, narrow phase 0, phase 1, wide
+constexpr int kWideBlkBytes = 16384;                 // LDS rows of a wide traceback
+constexpr int kWideMaxChunks = 16;                   // wide kernel: up to 1024 diagonals in lanes
+
 namespace {
 
 __device__ __forceinline__ int32_t lane0(int32_t v) { return __builtin_amdgcn_readlane(v, 0); }
+
+__device__ __forceinline__ int64_t lane0_64(int64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)v, 0);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)v >> 32), 0);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int32_t* list_of(const CigarView& v, int ph, int cls) {
+  return v.lists + (int64_t)(ph * kCigClasses + cls) * v.npairs;
+}
 
 // Rectangle of pair p in SSW naming (rows: SSW-query, columns: SSW-ref).
 __device__ __forceinline__ CigarGeo pair_geo(const CigarView& v, int64_t p) {
@@ -72,16 +98,7 @@ __device__ __forceinline__ int64_t wave_reserve(unsigned long long* ctr, int32_t
   const int32_t total = __builtin_amdgcn_readlane(x, 63);
   unsigned long long base = 0;
   if (lane == 0 && total > 0) base = atomicAdd(ctr, (unsigned long long)total);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)base, 0);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(base >> 32), 0);
-  return (int64_t)(((uint64_t)hi << 32) | lo) + (x - n);
-}
-
-// Appends p to the list of class cls (rare path: per-lane atomic).
-__device__ __forceinline__ void enqueue(const CigarView& v, int cls, int64_t p, int32_t bw) {
-  v.band[p] = bw;
-  const int32_t at = atomicAdd(&v.list_count[cls], 1);
-  v.lists[(int64_t)cls * v.npairs + at] = (int32_t)p;
+  return lane0_64((int64_t)base) + (x - n);
 }
 
 // Work index of the wavefront: one atomic from lane 0, broadcast.
@@ -91,39 +108,67 @@ __device__ __forceinline__ int32_t next_work(int32_t* ctr) {
   return lane0(w);
 }
 
+// Atomic add from lane 0 only, result broadcast.
+__device__ __forceinline__ int64_t lane0_add(unsigned long long* ctr, int64_t n) {
+  unsigned long long r = 0;
+  if ((threadIdx.x & 63) == 0) r = atomicAdd(ctr, (unsigned long long)n);
+  return lane0_64((int64_t)r);
+}
+
 // Whether SSW-query[qb] equals SSW-ref[rb] (the traceback's last M).
 __device__ __forceinline__ bool first_eq(const CigarView& v, const CigarGeo& g) {
   return nib_at(v.codes, g.qbase + g.qb) == nib_at(v.codes, g.rbase + g.rb);
 }
 
-// Counting walks over a register-class band: rows fetched in groups and
-// '=' / 'X' from the equality bits; when the traceback stops off column 0
-// the count is redone comparing shifted codes.  Returns the shift of the
-// writing walk (-1: the traceback left the band).
-template <int DW>
-__device__ __forceinline__ int32_t count_band(const CigarView& v, const CigarGeo& g, int32_t bw, const uint32_t* dirs,
-                                              bool eq00, CigarEmit& em) {
-  em = CigarEmit();
-  em.out = nullptr;
-  em.nops = 0;
-  int32_t jf = cigar_walk_rows<DW>(g, bw, dirs, eq00, em);
-  if (jf > 0) {
-    em = CigarEmit();
-    em.out = nullptr;
-    em.nops = 0;
-    jf = cigar_walk(v.codes, g, bw, dirs, DW, 64, jf, eq00, em);
-  }
-  return jf;
+// A pair whose band missed the score retries with half-width nb: in the
+// narrow class of phase ph, or in the wide class.  Rare: per-lane atomic.
+__device__ __forceinline__ void retry(const CigarView& v, int ph, int64_t p, int32_t nb, int32_t rows) {
+  int cls = cigar_class(nb, rows);
+  if (ph >= kCigPhases) cls = kCigClassWide;
+  if (cls == kCigClassWide) ph = 0;
+  v.band[p] = nb;
+  const int32_t at = atomicAdd(&v.list_count[ph * kCigClasses + cls], 1);
+  list_of(v, ph, cls)[at] = (int32_t)p;
 }
 
-// Reserves the ops of every lane's finished pair and writes them (all lanes
-// of the wavefront call it).  BAND: dirs is a register-class slot.
-template <int DW, bool BAND>
-__device__ __forceinline__ void finish_pairs(const CigarView& v, int64_t p, bool done, int32_t jf, const CigarGeo& g,
-                                             int32_t layout_bw, const uint32_t* dirs, int dwn, int64_t stride,
-                                             bool eq00, const CigarEmit& em) {
+// One narrow pair per lane (p < 0: idle lane); all lanes call it.  DS is
+// the words per row of the wavefront's slot.  The counting walk keeps the
+// CIGAR in registers when it has at most 16 ops between the clips; the
+// wavefront then reserves its ops with one atomic, and only pairs with more
+// ops (or whose traceback stops off column 0) walk again to write them.
+template <int BW, int DS>
+__device__ __forceinline__ void narrow_pair(const CigarView& v, int64_t p, uint32_t* dirs, int retry_phase) {
+  constexpr int DW = DS;
+  bool done = false, eq00 = false, queued = false;
+  int32_t jf = -1, count = 0, mism = 0;
+  CigarGeo g;
+  CigarEmitQ q;
+  if (p >= 0) {
+    g = pair_geo(v, p);
+    const int32_t corner = cigar_band_dp<BW, DS>(v.codes, g, v.sc.match, v.sc.mismatch, v.sc.ambiguity,
+                                                 v.sc.gap_open, v.sc.gap_ext, dirs);
+    if (corner == g.score) {
+      done = true;
+      eq00 = first_eq(v, g);
+      jf = cigar_walk_rows<DW>(g, BW, dirs, eq00, q);
+      if (jf > 0) {  // traceback stopped off column 0: count again comparing shifted codes
+        CigarEmit em;
+        em.out = nullptr;
+        em.nops = 0;
+        jf = cigar_walk(v.codes, g, BW, dirs, DW, true, jf, eq00, em);
+        count = em.count;
+        mism = em.mism;
+      } else {
+        count = q.count;
+        mism = q.mism;
+        queued = q.fits();
+      }
+    } else {
+      retry(v, retry_phase, p, 2 * BW, g.ql());
+    }
+  }
   const bool ok = done && jf >= 0;
-  const int64_t base = wave_reserve(&v.used[0], ok ? em.count : 0);
+  const int64_t base = wave_reserve(&v.used[0], ok ? count : 0);
   if (!done) return;
   if (!ok) {
     atomicAdd(&v.used[3], 1ull);  // traceback left the band: reported as an internal error
@@ -132,27 +177,43 @@ __device__ __forceinline__ void finish_pairs(const CigarView& v, int64_t p, bool
     v.off[p] = 0;
     return;
   }
-  const bool fits = base + em.count <= v.ops_cap;
-  if (fits && em.count > 0) {
-    CigarEmit w;
-    w.out = v.ops + base;
-    w.nops = em.count;
-    if (BAND && jf == 0) {
-      cigar_walk_rows<DW>(g, layout_bw, dirs, eq00, w);
+  const bool fits = base + count <= v.ops_cap;
+  if (fits && count > 0) {
+    uint32_t* out = v.ops + base;
+    if (queued) {
+      q.write(out);
     } else {
-      cigar_walk(v.codes, g, layout_bw, dirs, dwn, stride, jf, eq00, w);
+      CigarEmit w;
+      w.out = out;
+      w.nops = count;
+      if (jf == 0) {
+        cigar_walk_rows<DW>(g, BW, dirs, eq00, w);
+      } else {
+        cigar_walk(v.codes, g, BW, dirs, DW, true, jf, eq00, w);
+      }
     }
   }
-  v.n_ops[p] = fits ? em.count : -1;
-  v.mism[p] = em.mism;
+  v.n_ops[p] = fits ? count : -1;
+  v.mism[p] = mism;
   v.off[p] = base;
+}
+
+// The lane's first block in its wavefront's scratch slot (rows of the
+// tallest narrow rectangle, rounded up to 8, x DW words x 64 lanes, blocked
+// layout), or null when the scratch has no slot for the wavefront.
+__device__ __forceinline__ uint32_t* narrow_slot(const CigarView& v, int DW) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t slot_dwords = (int64_t)((lane0(v.max_rows[0]) + 7) & ~7) * DW * 64;
+  if (slot_dwords <= 0 || (wave + 1) * slot_dwords > v.dirs_dwords) return nullptr;
+  return v.dirs + wave * slot_dwords + (int64_t)lane * 8 * DW;
 }
 
 }  // namespace
 
 __global__ void cigar_clear_kernel(CigarView v) {
   const int t = threadIdx.x;
-  if (t < 2 * kCigClasses) v.list_count[t] = 0;
+  if (t < kCigWork + 4) v.list_count[t] = 0;
   if (t < 8) v.used[t] = 0ull;
   if (t == 0) v.max_rows[0] = 0;
 }
@@ -192,7 +253,7 @@ __global__ __launch_bounds__(256) void cigar_classify_kernel(CigarView v) {
           cls[k] = cigar_class(bw, g.ql());
           v.band[p] = bw;
           rank[k] = atomicAdd(&cnt[cls[k]], 1);
-          if (cls[k] != kCigClassGeneric) atomicMax(&rows_max, g.ql());
+          if (cls[k] != kCigClassWide) atomicMax(&rows_max, g.ql());
         }
       }
       // trivial pairs (empty query, distance filter, score 0): reserve and write now
@@ -219,108 +280,329 @@ __global__ __launch_bounds__(256) void cigar_classify_kernel(CigarView v) {
     for (int k = 0; k < kCigClassifyPerThread; k++) {
       if (cls[k] >= 0) {
         const int64_t p = chunk * kCigChunk + k * 256 + t;
-        v.lists[(int64_t)cls[k] * v.npairs + base[cls[k]] + rank[k]] = (int32_t)p;
+        list_of(v, 0, cls[k])[base[cls[k]] + rank[k]] = (int32_t)p;
       }
     }
     __syncthreads();
   }
 }
 
-template <int BW, bool MASKED>
-__global__ __launch_bounds__(256) void cigar_band_kernel(CigarView v) {
-  constexpr int W = 2 * BW + 1;
-  constexpr int DW = (4 * W + 31) / 32;
-  constexpr int cls = MASKED ? kCigClassMasked : BW - 1;
+// Half-width 1, phase 0.  Misses retry at half-width 2 in phase 0 (the
+// narrow kernel's first launch runs after this one).
+__global__ __launch_bounds__(256) void cigar_bw1_kernel(CigarView v) {
+  uint32_t* dirs = narrow_slot(v, 1);
+  if (!dirs) return;
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  // one scratch slot per wavefront: rows of the tallest rectangle x DW dwords x 64 lanes
-  const int64_t slot_dwords = (int64_t)lane0(v.max_rows[0]) * DW * 64;
-  if (slot_dwords <= 0 || (wave + 1) * slot_dwords > v.dirs_dwords) return;
-  uint32_t* dirs = v.dirs + wave * slot_dwords + lane;
-  const int32_t count = lane0(v.list_count[cls]);
-  int32_t* next = &v.list_count[kCigClasses + cls];
+  const int32_t count = lane0(v.list_count[0]);
+  const int32_t* list = list_of(v, 0, 0);
   for (;;) {
-    const int32_t w = next_work(next);
+    const int32_t w = next_work(&v.list_count[kCigWork + 0]);
     if ((int64_t)w * 64 >= count) break;
     const int32_t idx = w * 64 + lane;
-    const int64_t p = idx < count ? v.lists[(int64_t)cls * v.npairs + idx] : -1;
-    bool done = false, eq00 = false;
-    int32_t jf = -1;
-    CigarGeo g;
-    CigarEmit em;
-    if (p >= 0) {
-      g = pair_geo(v, p);
-      const int32_t bw = v.band[p];
-      const int32_t corner = cigar_band_dp<BW, MASKED>(v.codes, g, bw, v.sc.match, v.sc.mismatch, v.sc.ambiguity,
-                                                       v.sc.gap_open, v.sc.gap_ext, dirs, 64);
-      if (corner == g.score) {
-        done = true;
-        eq00 = first_eq(v, g);
-        jf = count_band<DW>(v, g, BW, dirs, eq00, em);
-      } else {
-        enqueue(v, cigar_class(2 * bw, g.ql()), p, 2 * bw);
-      }
-    }
-    finish_pairs<DW, true>(v, p, done, jf, g, BW, dirs, DW, 64, eq00, em);
+    narrow_pair<1, 1>(v, idx < count ? list[idx] : -1, dirs, 0);
   }
 }
 
-__global__ __launch_bounds__(64) void cigar_generic_kernel(CigarView v) {
+// Half-widths 2..7 of phase ph, one class per wavefront iteration.  Misses
+// retry in phase ph + 1 (half-width 4 or 6) or in the wide class.
+__global__ __launch_bounds__(256) void cigar_narrow_kernel(CigarView v, int ph) {
+  uint32_t* dirs = narrow_slot(v, 2);
+  if (!dirs) return;
   const int lane = threadIdx.x & 63;
-  const int32_t count = lane0(v.list_count[kCigClassGeneric]);
-  int32_t* next = &v.list_count[kCigClasses + kCigClassGeneric];
+  int32_t items[kCigExactMaxBw], cnt[kCigExactMaxBw];
+  int32_t total = 0;
+  items[0] = 0;
+  cnt[0] = 0;
+#pragma unroll
+  for (int c = 1; c < kCigExactMaxBw; c++) {
+    cnt[c] = lane0(v.list_count[ph * kCigClasses + c]);
+    total += (cnt[c] + 63) / 64;
+    items[c] = total;  // items of classes 1..c
+  }
   for (;;) {
-    const int32_t w = next_work(next);
-    if ((int64_t)w * 64 >= count) break;
-    const int32_t idx = w * 64 + lane;
-    const int64_t p = idx < count ? v.lists[(int64_t)kCigClassGeneric * v.npairs + idx] : -1;
-    bool done = false, eq00 = false;
-    int32_t jf = -1, bw = 0, DW = 0;
-    const uint32_t* dirs = nullptr;
-    CigarGeo g;
-    CigarEmit em;
-    if (p >= 0) {
-      g = pair_geo(v, p);
-      bw = v.band[p];
-      for (;;) {
-        const int32_t W = 2 * bw + 1;
-        DW = (4 * W + 31) / 32;
-        const int64_t need = 2 * (int64_t)W + (int64_t)g.ql() * DW;
-        const int64_t at = (int64_t)atomicAdd(&v.used[1], (unsigned long long)need);
-        if (at + need > v.gen_dwords) {
-          atomicAdd(&v.used[2], 1ull);  // deferred: the host grows the pool and reruns
-          v.n_ops[p] = -2;
-          break;
-        }
-        uint32_t* mem = v.gen + at;
-        const int32_t corner = cigar_band_dp_any(v.codes, g, bw, v.sc.match, v.sc.mismatch, v.sc.ambiguity,
-                                                 v.sc.gap_open, v.sc.gap_ext, (int32_t*)mem, (int32_t*)mem + W,
-                                                 mem + 2 * W);
-        if (corner == g.score) {
-          dirs = mem + 2 * W;
-          done = true;
-          eq00 = first_eq(v, g);
-          em = CigarEmit();
-          em.out = nullptr;
-          em.nops = 0;
-          jf = cigar_walk(v.codes, g, bw, dirs, DW, 1, 0, eq00, em);
-          if (jf > 0) {
-            em = CigarEmit();
-            em.out = nullptr;
-            em.nops = 0;
-            jf = cigar_walk(v.codes, g, bw, dirs, DW, 1, jf, eq00, em);
-          }
-          break;
-        }
-        if (bw > g.rl() + g.ql() + 2) {  // a band this wide holds the whole rectangle
-          done = true;
-          jf = -1;
-          break;
-        }
-        bw *= 2;
+    const int32_t w = next_work(&v.list_count[kCigWork + 1 + ph]);
+    if (w >= total) break;
+    int c = 1, first = 0;
+#pragma unroll
+    for (int k = 1; k < kCigExactMaxBw - 1; k++) {
+      if (w >= items[k]) {
+        c = k + 1;
+        first = items[k];
       }
     }
-    finish_pairs<1, false>(v, p, done, jf, g, bw, dirs, DW, 1, eq00, em);
+    const int32_t idx = (w - first) * 64 + lane;
+    int32_t n = cnt[1];
+#pragma unroll
+    for (int k = 2; k < kCigExactMaxBw; k++) n = c == k ? cnt[k] : n;
+    const int64_t p = idx < n ? list_of(v, ph, c)[idx] : -1;
+    switch (c) {
+      case 1: narrow_pair<2, 2>(v, p, dirs, ph + 1); break;
+      case 2: narrow_pair<3, 2>(v, p, dirs, ph + 1); break;
+      case 3: narrow_pair<4, 2>(v, p, dirs, ph + 1); break;
+      case 4: narrow_pair<5, 2>(v, p, dirs, ph + 1); break;
+      case 5: narrow_pair<6, 2>(v, p, dirs, ph + 1); break;
+      default: narrow_pair<7, 2>(v, p, dirs, ph + 1); break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// wide bands: one wavefront per pair
+// ---------------------------------------------------------------------------
+namespace {
+
+// Band DP of half-width bw with C chunks of 64 diagonals per lane; writes one
+// direction byte per cell at dirs[i * 64C + d].  Returns H of the last cell
+// (the same in every lane).
+template <int C>
+__device__ int32_t wide_dp(const CigarView& v, const CigarGeo& g, int32_t bw, uint8_t* dirs) {
+  constexpr int32_t NEG = -(1 << 29);
+  const int lane = threadIdx.x & 63;
+  const int32_t W = 2 * bw + 1, rl = g.rl(), ql = g.ql();
+  const int32_t gO = v.sc.gap_open, gE = v.sc.gap_ext;
+  int32_t H[C], E[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    H[c] = 0;
+    E[c] = 0;
+  }
+  for (int32_t i = 0; i < ql; i++) {
+    const uint32_t qc = nib_at(v.codes, g.qbase + g.qb + i);
+    uint32_t tlo, thi;
+    subst_table(qc, v.sc.match, v.sc.mismatch, v.sc.ambiguity, tlo, thi);
+    int32_t carry = NEG;           // max X over earlier chunks
+    int32_t hl63 = 0, fl63 = 0;    // H and F of the diagonal left of this chunk's lane 0
+    uint8_t* row = dirs + (int64_t)i * (64 * C);
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const int32_t d = lane + 64 * c;
+      const int32_t j = i - bw + d;
+      const bool valid = d < W;
+      const uint32_t rc = (valid && j >= 0 && j < rl) ? nib_at(v.codes, g.rbase + g.rb + j) : kCigKill;
+      // E from (i-1, d+1): the next lane, or lane 0 of the next chunk
+      int32_t hu = __shfl_down(H[c], 1, 64), eu = __shfl_down(E[c], 1, 64);
+      if (lane == 63) {
+        hu = c + 1 < C ? __builtin_amdgcn_readlane(H[(c + 1) % C], 0) : 0;
+        eu = c + 1 < C ? __builtin_amdgcn_readlane(E[(c + 1) % C], 0) : 0;
+      }
+      if (d + 1 >= W) {
+        hu = 0;
+        eu = 0;
+      }
+      const int32_t eo = hu - gO, ee = eu - gE;
+      const int32_t e = imax3(eo, ee, 0);
+      const int32_t t2 = H[c] + subst_lookup(tlo, thi, rc);
+      const int32_t h0 = imax3(e, t2, 0);
+      // F(d) = max(0, max_{k<d} X(k) - (d-1)*gE), X(k) = H0(k) - gO + k*gE
+      int32_t incl = valid ? h0 - gO + d * gE : NEG;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl = imax(incl, y);
+      }
+      int32_t excl = __shfl_up(incl, 1, 64);
+      if (lane == 0) excl = NEG;
+      const int32_t f = imax(0, imax(excl, carry) - (d - 1) * gE);
+      carry = imax(carry, __builtin_amdgcn_readlane(incl, 63));
+      const int32_t h = imax(h0, f);
+      // H and F of (i, d-1) for F's open/extend bit
+      int32_t hl = __shfl_up(h, 1, 64), fl = __shfl_up(f, 1, 64);
+      if (lane == 0) {
+        hl = hl63;
+        fl = fl63;
+      }
+      hl63 = __builtin_amdgcn_readlane(h, 63);
+      fl63 = __builtin_amdgcn_readlane(f, 63);
+      const int32_t t1 = imax(e, f);
+      if (valid) row[d] = (uint8_t)cell_bits(t1, t2, e, f, rc == qc, eo > ee, hl - gO > fl - gE);
+      H[c] = valid ? h : 0;
+      E[c] = valid ? e : 0;
+    }
+  }
+  const int32_t dc = (rl - 1) - (ql - 1) + bw;
+  int32_t corner = -1;
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const int32_t x = __builtin_amdgcn_readlane(H[c], dc & 63);
+    corner = (dc >> 6) == c ? x : corner;
+  }
+  return corner;
+}
+
+// Traceback of a wide band (bytes, Wp per row), walked by all lanes in step
+// over blocks of rows copied to LDS; ops written from lane 0.  As cigar_walk.
+__device__ int32_t wide_walk(const CigarView& v, const CigarGeo& g, int32_t bw, int32_t Wp, const uint8_t* dirs,
+                             uint32_t* blk, int32_t shift, bool eq00, CigarEmit& em) {
+  const int lane = threadIdx.x & 63;
+  const int32_t R = kWideBlkBytes / Wp;
+  const uint8_t* b8 = reinterpret_cast<const uint8_t*>(blk);
+  NibCursor rcur, qcur;
+  em.push(kOpS, (uint32_t)(g.n - 1 - g.qe));
+  int32_t i = g.ql() - 1, j = g.rl() - 1;
+  int32_t d = j - i + bw;
+  int state = 2;
+  int32_t lo = i + 1;
+  while (i > 0) {
+    if (d < 0 || d > 2 * bw || j < 0) return -1;
+    if (i < lo) {
+      lo = i - R + 1 > 0 ? i - R + 1 : 0;
+      const int32_t ndw = (i - lo + 1) * (Wp >> 2);
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(dirs + (int64_t)lo * Wp);
+      __syncthreads();
+      for (int32_t k = lane; k < ndw; k += 64) blk[k] = src[k];
+      __syncthreads();
+    }
+    uint32_t op = step_op(b8[(i - lo) * Wp + d] & 0xFu, state);
+    if (op == kOpEq || op == kOpX) {
+      if (shift != 0) {
+        const uint32_t a = qcur.get(v.codes, g.qbase + g.qb + i);
+        const uint32_t b = rcur.get(v.codes, g.rbase + g.rb + j - shift);
+        op = a == b ? kOpEq : kOpX;
+      }
+      i--;
+      j--;
+    } else if (op == kOpI) {
+      i--;
+      d++;
+    } else {
+      j--;
+      d--;
+    }
+    em.push(op, 1);
+  }
+  bool last_eq = eq00;
+  if (shift != 0) last_eq = nib_at(v.codes, g.qbase + g.qb) == nib_at(v.codes, g.rbase + g.rb + j - shift);
+  em.push(last_eq ? kOpEq : kOpX, 1);
+  em.push(kOpS, (uint32_t)g.qb);
+  em.flush();
+  return j;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void cigar_wide_kernel(CigarView v) {
+  __shared__ uint32_t blk[kWideBlkBytes / 4];
+  const int lane = threadIdx.x & 63;
+  const int32_t count = lane0(v.list_count[kCigClassWide]);
+  const int32_t* list = list_of(v, 0, kCigClassWide);
+  for (;;) {
+    const int32_t w = next_work(&v.list_count[kCigWork + 3]);
+    if (w >= count) break;
+    const int64_t p = lane0(list[w]);
+    const CigarGeo g = pair_geo(v, p);
+    const bool eq00 = first_eq(v, g);
+    int32_t bw = v.band[p];
+    int status = 0;  // 1 done (lanes), 2 done (serial, lane 0), 3 deferred, 4 error
+    int32_t Wp = 0, sDW = 0;
+    uint32_t* mem = nullptr;
+    int32_t corner = -1;
+    for (;;) {
+      const int32_t W = 2 * bw + 1;
+      const int32_t C = (W + 63) / 64;
+      const bool lanes = C <= kWideMaxChunks;
+      const int32_t Cp = C <= 1 ? 1 : C <= 2 ? 2 : C <= 4 ? 4 : C <= 8 ? 8 : 16;
+      Wp = 64 * Cp;
+      sDW = (4 * W + 31) / 32;
+      const int64_t need = lanes ? ((int64_t)g.ql() * Wp + 3) / 4 : 2 * (int64_t)W + (int64_t)g.ql() * sDW;
+      const int64_t at = lane0_add(&v.used[1], need);
+      if (at + need > v.gen_dwords) {
+        status = 3;
+        break;
+      }
+      mem = v.gen + at;
+      if (lanes) {
+        uint8_t* dirs = reinterpret_cast<uint8_t*>(mem);
+        switch (Cp) {
+          case 1: corner = wide_dp<1>(v, g, bw, dirs); break;
+          case 2: corner = wide_dp<2>(v, g, bw, dirs); break;
+          case 4: corner = wide_dp<4>(v, g, bw, dirs); break;
+          case 8: corner = wide_dp<8>(v, g, bw, dirs); break;
+          default: corner = wide_dp<16>(v, g, bw, dirs); break;
+        }
+      } else {
+        // bands beyond the lanes: one cell at a time on lane 0
+        int32_t cr = -1;
+        if (lane == 0)
+          cr = cigar_band_dp_any(v.codes, g, bw, v.sc.match, v.sc.mismatch, v.sc.ambiguity, v.sc.gap_open,
+                                 v.sc.gap_ext, (int32_t*)mem, (int32_t*)mem + W, mem + 2 * W);
+        corner = lane0(cr);
+      }
+      if (corner == g.score) {
+        status = lanes ? 1 : 2;
+        break;
+      }
+      if (bw > g.rl() + g.ql() + 2) {  // a band this wide holds the whole rectangle
+        status = 4;
+        break;
+      }
+      bw *= 2;
+    }
+    if (status == 1) {
+      const uint8_t* dirs = reinterpret_cast<const uint8_t*>(mem);
+      CigarEmit em;
+      em.out = nullptr;
+      em.nops = 0;
+      int32_t jf = wide_walk(v, g, bw, Wp, dirs, blk, 0, eq00, em);
+      if (jf > 0) {
+        em = CigarEmit();
+        em.out = nullptr;
+        em.nops = 0;
+        jf = wide_walk(v, g, bw, Wp, dirs, blk, jf, eq00, em);
+      }
+      if (jf < 0) {
+        status = 4;
+      } else {
+        const int64_t base = lane0_add(&v.used[0], em.count);
+        const bool fits = base + em.count <= v.ops_cap;
+        if (fits && em.count > 0) {
+          CigarEmit wr;
+          wr.out = lane == 0 ? v.ops + base : nullptr;
+          wr.nops = em.count;
+          wide_walk(v, g, bw, Wp, dirs, blk, jf, eq00, wr);
+        }
+        if (lane == 0) {
+          v.n_ops[p] = fits ? em.count : -1;
+          v.mism[p] = em.mism;
+          v.off[p] = base;
+        }
+      }
+    } else if (status == 2 && lane == 0) {
+      const uint32_t* dirs = mem + 2 * (2 * bw + 1);
+      CigarEmit em;
+      em.out = nullptr;
+      em.nops = 0;
+      int32_t jf = cigar_walk(v.codes, g, bw, dirs, sDW, false, 0, eq00, em);
+      if (jf > 0) {
+        em = CigarEmit();
+        em.out = nullptr;
+        em.nops = 0;
+        jf = cigar_walk(v.codes, g, bw, dirs, sDW, false, jf, eq00, em);
+      }
+      if (jf < 0) {
+        atomicAdd(&v.used[3], 1ull);
+        v.n_ops[p] = -3;
+      } else {
+        const int64_t base = (int64_t)atomicAdd(&v.used[0], (unsigned long long)em.count);
+        const bool fits = base + em.count <= v.ops_cap;
+        if (fits && em.count > 0) {
+          CigarEmit wr;
+          wr.out = v.ops + base;
+          wr.nops = em.count;
+          cigar_walk(v.codes, g, bw, dirs, sDW, false, jf, eq00, wr);
+        }
+        v.n_ops[p] = fits ? em.count : -1;
+        v.mism[p] = em.mism;
+        v.off[p] = base;
+      }
+    }
+    if (status == 3 && lane == 0) {
+      atomicAdd(&v.used[2], 1ull);  // deferred: the host grows the pool and reruns the pass
+      v.n_ops[p] = -2;
+    }
+    if (status == 4 && lane == 0) {
+      atomicAdd(&v.used[3], 1ull);
+      v.n_ops[p] = -3;
+    }
   }
 }
 
@@ -331,20 +613,15 @@ static int cig_grid(int64_t n, int block) {
   return (int)g;
 }
 
-hipError_t launch_cigar(const CigarView& v, int band_waves, int generic_waves, hipStream_t st) {
+hipError_t launch_cigar(const CigarView& v, int narrow_waves, int wide_waves, hipStream_t st) {
   hipLaunchKernelGGL(cigar_clear_kernel, dim3(1), dim3(64), 0, st, v);
   if (v.npairs <= 0) return hipGetLastError();
   hipLaunchKernelGGL(cigar_classify_kernel, dim3(cig_grid(v.npairs, kCigChunk)), dim3(256), 0, st, v);
-  const int blocks = std::max(1, band_waves / 4);
-  hipLaunchKernelGGL((cigar_band_kernel<1, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<2, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<3, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<4, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<5, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<6, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<7, false>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL((cigar_band_kernel<kCigMaskedBw, true>), dim3(blocks), dim3(256), 0, st, v);
-  hipLaunchKernelGGL(cigar_generic_kernel, dim3(std::max(1, generic_waves)), dim3(64), 0, st, v);
+  const int blocks = std::max(1, narrow_waves / 4);
+  hipLaunchKernelGGL(cigar_bw1_kernel, dim3(blocks), dim3(256), 0, st, v);
+  for (int ph = 0; ph < kCigPhases; ph++)
+    hipLaunchKernelGGL(cigar_narrow_kernel, dim3(blocks), dim3(256), 0, st, v, ph);
+  hipLaunchKernelGGL(cigar_wide_kernel, dim3(std::max(1, wide_waves)), dim3(64), 0, st, v);
   return hipGetLastError();
 }
 

@@ -851,15 +851,15 @@ static BatchView make_view(mhmsw_ctx* c) {
 // Device buffers of the CIGAR pass for the staged batch: op pool and generic
 // pool grow when a pass overflows them (complete() reruns the pass);
 // direction scratch is a fixed 256 MiB (one slot per wavefront, rows of the
-// tallest rectangle; at least one slot of kCigMaxRows rows always fits).
+// tallest narrow rectangle; a slot of kCigNarrowMaxRows rows always fits).
 constexpr int64_t kCigDirsDwords = 64ll << 20;
 static int prepare_cigar(mhmsw_ctx* c) {
   const size_t np = (size_t)std::max<int64_t>(c->npairs, 1);
   c->cig_ops_cap = std::max<int64_t>(c->cig_ops_cap, 12 * (int64_t)np + 1024);
   c->cig_gen_dwords = std::max<int64_t>(c->cig_gen_dwords, 4ll << 20);
   c->cig_dirs_dwords = kCigDirsDwords;
-  HIP_OR_FAIL(c, c->d_cig_lists.ensure((size_t)kCigClasses * np));
-  HIP_OR_FAIL(c, c->d_cig_count.ensure(2 * kCigClasses));
+  HIP_OR_FAIL(c, c->d_cig_lists.ensure((size_t)2 * kCigClasses * np));  // two phases of class lists
+  HIP_OR_FAIL(c, c->d_cig_count.ensure(4 * kCigClasses));
   HIP_OR_FAIL(c, c->d_cig_band.ensure(np));
   HIP_OR_FAIL(c, c->d_cig_rows.ensure(1));
   HIP_OR_FAIL(c, c->d_cig_nops.ensure(np));
@@ -898,8 +898,8 @@ static CigarView make_cigar_view(mhmsw_ctx* c) {
   return v;
 }
 
-// wavefronts of the CIGAR launches: 16 per CU for the register classes, 1
-// per CU for the generic class
+// wavefronts of the CIGAR launches: 16 per CU for the narrow bands, 1 per CU
+// for the wide ones (one pair per wavefront)
 static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
   return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus, st);
 }

@@ -52,13 +52,13 @@ struct CigarView {
   const int32_t* seq_len;
   const AlnResult* results;
   ScoreParams sc;
-  int32_t* lists;       // [kCigClasses * npairs] pairs of each band class
-  int32_t* list_count;  // [2 * kCigClasses] entries, then work counters
+  int32_t* lists;       // [2 phases][kCigClasses][npairs] pairs of each band class
+  int32_t* list_count;  // [2 phases][kCigClasses] entries, then work counters
   int32_t* band;        // [npairs] band half-width of the pair's next attempt
   int32_t* max_rows;    // [1] tallest rectangle of the register classes
   uint32_t* dirs;       // direction scratch, one slot per wavefront
   int64_t dirs_dwords;
-  uint32_t* gen;        // pool of the generic class
+  uint32_t* gen;        // pool of the wide class
   int64_t gen_dwords;
   int32_t* n_ops;       // [npairs] op count (-1 op pool full, -2 deferred, -3 traceback error)
   int32_t* mism;        // [npairs]
@@ -67,8 +67,8 @@ struct CigarView {
   int64_t ops_cap;
   unsigned long long* used;  // [0] ops reserved, [1] pool dwords reserved, [2] deferred pairs, [3] errors
 };
-// band_waves: wavefronts of each register-class launch; generic_waves: of the generic one
-hipError_t launch_cigar(const CigarView& v, int band_waves, int generic_waves, hipStream_t st);
+// narrow_waves: wavefronts of each narrow-band launch; wide_waves: of the wide one
+hipError_t launch_cigar(const CigarView& v, int narrow_waves, int wide_waves, hipStream_t st);
 
 hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
 hipError_t launch_unpack(const BatchView& b, hipStream_t st);  // also extracts resident windows

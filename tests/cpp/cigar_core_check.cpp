@@ -1,8 +1,11 @@
 // CPU check of the per-pair CIGAR code the HIP kernels run
-// (mhm2_staging_amd/csrc/cigar_core.h), compiled for the host: the same band
-// classes, band doubling, count / recount / write walks and trivial cases as
-// cigar_kernels.hip, with the direction bits laid out lane-interleaved
-// (stride 64) as in a wavefront's scratch slot.  Every pair is compared with
+// (mhm2_staging_amd/csrc/cigar_core.h), compiled for the host: the same
+// narrow band classes, band doubling, count / recount / write walks (CIGARs
+// kept in registers up to 16 ops between the clips) and trivial cases as
+// cigar_kernels.hip, with
+// the direction bits in the blocked layout of a wavefront's scratch slot.  Bands the
+// kernels give to their wavefront-per-pair kernel run here through the
+// one-cell-at-a-time band DP of the same header (same cells, same bits).  Every pair is compared with
 // the oracle (oracle/sw_oracle.c), which is what this test program links it
 // against.
 //
@@ -29,13 +32,20 @@ static void pack(const std::string& s, std::vector<uint32_t>& codes, int64_t bas
   }
 }
 
-template <int BW, bool MASKED>
-static int32_t run_dp(const uint32_t* codes, const CigarGeo& g, int32_t bw, const oracle_scoring& sc,
+// words per row of the slot of the kernel that runs half-width BW: 1 for
+// half-width 1, 2 for the merged half-widths 2..7
+constexpr int slot_words(int bw) { return bw == 1 ? 1 : 2; }
+
+template <int BW>
+static int32_t run_dp(const uint32_t* codes, const CigarGeo& g, const oracle_scoring& sc,
                       std::vector<uint32_t>& slot) {
-  constexpr int W = 2 * BW + 1, DW = (4 * W + 31) / 32;
-  slot.assign((size_t)g.ql() * DW * 64 + 64, 0xDEADBEEFu);
-  return cigar_band_dp<BW, MASKED>(codes, g, bw, sc.match, sc.mismatch, sc.ambiguity, sc.gap_open, sc.gap_ext,
-                                   slot.data() + 17, 64);
+  constexpr int DS = slot_words(BW);
+  slot.assign((size_t)((g.ql() + 7) & ~7) * DS * 64, 0xDEADBEEFu);
+  // lane 17 and lane 49 of a wavefront hold the same pair: both must give it intact
+  cigar_band_dp<BW, DS>(codes, g, sc.match, sc.mismatch, sc.ambiguity, sc.gap_open, sc.gap_ext,
+                        slot.data() + 49 * 8 * DS);
+  return cigar_band_dp<BW, DS>(codes, g, sc.match, sc.mismatch, sc.ambiguity, sc.gap_open, sc.gap_ext,
+                               slot.data() + 17 * 8 * DS);
 }
 
 int main() {
@@ -89,21 +99,20 @@ int main() {
     } else {
       int32_t bw = std::abs(g.rl() - g.ql()) + 1, layout = 0, dwn = 0;
       const uint32_t* dirs = nullptr;
-      int64_t stride = 64;
+      bool blocked = true;
       int first_cls = -1;
       for (;;) {
         const int cls = cigar_class(bw, g.ql());
         if (first_cls < 0) first_cls = cls;
         int32_t corner;
         switch (cls) {
-          case 0: corner = run_dp<1, false>(codes.data(), g, bw, sc, slot); break;
-          case 1: corner = run_dp<2, false>(codes.data(), g, bw, sc, slot); break;
-          case 2: corner = run_dp<3, false>(codes.data(), g, bw, sc, slot); break;
-          case 3: corner = run_dp<4, false>(codes.data(), g, bw, sc, slot); break;
-          case 4: corner = run_dp<5, false>(codes.data(), g, bw, sc, slot); break;
-          case 5: corner = run_dp<6, false>(codes.data(), g, bw, sc, slot); break;
-          case 6: corner = run_dp<7, false>(codes.data(), g, bw, sc, slot); break;
-          case kCigClassMasked: corner = run_dp<kCigMaskedBw, true>(codes.data(), g, bw, sc, slot); break;
+          case 0: corner = run_dp<1>(codes.data(), g, sc, slot); break;
+          case 1: corner = run_dp<2>(codes.data(), g, sc, slot); break;
+          case 2: corner = run_dp<3>(codes.data(), g, sc, slot); break;
+          case 3: corner = run_dp<4>(codes.data(), g, sc, slot); break;
+          case 4: corner = run_dp<5>(codes.data(), g, sc, slot); break;
+          case 5: corner = run_dp<6>(codes.data(), g, sc, slot); break;
+          case 6: corner = run_dp<7>(codes.data(), g, sc, slot); break;
           default: {
             const int32_t W = 2 * bw + 1, DW = (4 * W + 31) / 32;
             gen.assign((size_t)2 * W + (size_t)g.ql() * DW, 0xDEADBEEFu);
@@ -112,16 +121,16 @@ int main() {
           }
         }
         if (corner == g.score) {
-          if (cls == kCigClassGeneric) {
+          if (cls == kCigClassWide) {
             layout = bw;
             dwn = (4 * (2 * bw + 1) + 31) / 32;
             dirs = gen.data() + 2 * (2 * bw + 1);
-            stride = 1;
+            blocked = false;
           } else {
-            layout = cls == kCigClassMasked ? kCigMaskedBw : bw;
-            dwn = (4 * (2 * layout + 1) + 31) / 32;
-            dirs = slot.data() + 17;
-            stride = 64;
+            layout = bw;
+            dwn = slot_words(bw);
+            dirs = slot.data() + 17 * 8 * dwn;
+            blocked = true;
           }
           break;
         }
@@ -136,25 +145,34 @@ int main() {
       // class walks row by row; a traceback that stops off column 0 is
       // walked again comparing shifted codes
       const bool eq00 = nib_at(codes.data(), g.qbase + g.qb) == nib_at(codes.data(), g.rbase + g.rb);
-      const bool band = stride == 64;
-      auto fast_walk = [&](CigarEmit &e) -> int32_t {
+      const bool band = blocked;
+      auto fast_walk = [&](auto &e) -> int32_t {
         switch (dwn) {
           case 1: return cigar_walk_rows<1>(g, layout, dirs, eq00, e);
           case 2: return cigar_walk_rows<2>(g, layout, dirs, eq00, e);
-          case 4: return cigar_walk_rows<4>(g, layout, dirs, eq00, e);
           default: return -9;
         }
       };
+      CigarEmitQ q1;
       CigarEmit c1;
       c1.out = nullptr;
       c1.nops = 0;
-      int32_t jf = band ? fast_walk(c1) : cigar_walk(codes.data(), g, layout, dirs, dwn, stride, 0, eq00, c1);
+      int32_t jf;
+      bool queued = false;
+      if (band) {
+        jf = fast_walk(q1);
+        c1.count = q1.count;
+        c1.mism = q1.mism;
+        queued = jf == 0 && q1.fits();
+      } else {
+        jf = cigar_walk(codes.data(), g, layout, dirs, dwn, blocked, 0, eq00, c1);
+      }
       CigarEmit c2 = c1;
       if (jf > 0) {
         c2 = CigarEmit();
         c2.out = nullptr;
         c2.nops = 0;
-        jf = cigar_walk(codes.data(), g, layout, dirs, dwn, stride, jf, eq00, c2);
+        jf = cigar_walk(codes.data(), g, layout, dirs, dwn, blocked, jf, eq00, c2);
       }
       if (jf < 0) {
         got_n = -1;
@@ -163,10 +181,14 @@ int main() {
         CigarEmit w;
         w.out = got.data();
         w.nops = c2.count;
-        if (band && jf == 0) {
+        if (queued) {
+          q1.write(got.data());
+          w.count = c2.count;
+          w.mism = c2.mism;
+        } else if (band && jf == 0) {
           fast_walk(w);
         } else {
-          cigar_walk(codes.data(), g, layout, dirs, dwn, stride, jf, eq00, w);
+          cigar_walk(codes.data(), g, layout, dirs, dwn, blocked, jf, eq00, w);
         }
         got_n = c2.count;
         got_mism = c2.mism;

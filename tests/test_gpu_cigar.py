@@ -129,11 +129,15 @@ def test_cigar_edge_cases(gpu_aligner):
     ctgs.append(w[200:2200])
     reads.append(rs(300, "AC"))     # many equal-score paths
     ctgs.append(rs(900, "AC"))
-    # a 40-base deletion and a 40-base insertion: bands of 41 and wider (generic class)
-    reads.append(w[:300] + w[340:600])
-    ctgs.append(w[:600])
-    reads.append(w[:300] + rs(40) + w[300:600])
-    ctgs.append(w[:600])
+    # deletions and insertions of 40..600 bases: wide bands of 1, 2, 16 chunks of
+    # 64 diagonals and, for 600, wider than the lanes (serial path)
+    for gap in (40, 100, 300):
+        reads.append(w[:300] + w[300 + gap:600 + gap])
+        ctgs.append(w[:600 + gap])
+        reads.append(w[:300] + rs(gap) + w[300:600])
+        ctgs.append(w[:600])
+    reads.append(w[:1500] + w[2100:3600])
+    ctgs.append(w[:3600])
     b = m.from_strings(reads, ctgs)
     for sc in (BASELINE_SCORING, (1, 1, 1, 1, 1), (10, 2, 3, 1, 2)):
         for mode in (m.MODE_FAST, m.MODE_PLAIN):
