@@ -13,10 +13,9 @@
 //    and E) in registers, 4 direction bits per cell stored in the
 //    wavefront's scratch slot in blocks of 8 rows per lane (16-byte stores
 //    and loads): cigar_bw1_kernel for half-width 1 (most pairs),
-//    cigar_narrow_kernel for half-widths 2..7 in two phases.  A pair whose
-//    band misses the score doubles its half-width: to half-width 2 for the
-//    next launch of phase 0, to 4 or 6 for phase 1, to 8 or more for the wide
-//    kernel.  A counting walk over the directions ('=' / 'X' from the
+//    cigar_narrow_kernel for half-widths 2..7.  A pair whose band misses the
+//    score doubles its half-width in the wide kernel.  A counting walk over
+//    the directions ('=' / 'X' from the
 //    equality bits) that keeps up to 16 ops besides the clips in registers,
 //    one atomic per wavefront to reserve the ops, and a writing walk only
 //    for longer CIGARs.
@@ -39,10 +38,10 @@
 
 namespace mhmsw {
 
-// layout of CigarView::list_count
-constexpr int kCigPhases = 2;
-constexpr int kCigWork = kCigPhases * kCigClasses;  // work counters: bw1, narrow phase 0, phase 1, wide
-constexpr int kWideBlkBytes = 16384;                 // LDS rows of a wide traceback
+// layout of CigarView::list_count: entries of each class, then the work
+// counters of the launches (half-width 1, half-widths 2..7, wide)
+constexpr int kCigWork = kCigClasses;
+constexpr int kWideBlkBytes = 8192;                  // LDS rows of a wide traceback
 constexpr int kWideMaxChunks = 16;                   // wide kernel: up to 1024 diagonals in lanes
 
 namespace {
@@ -55,8 +54,8 @@ __device__ __forceinline__ int64_t lane0_64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ int32_t* list_of(const CigarView& v, int ph, int cls) {
-  return v.lists + (int64_t)(ph * kCigClasses + cls) * v.npairs;
+__device__ __forceinline__ int32_t* list_of(const CigarView& v, int cls) {
+  return v.lists + (int64_t)cls * v.npairs;
 }
 
 // Rectangle of pair p in SSW naming (rows: SSW-query, columns: SSW-ref).
@@ -120,15 +119,13 @@ __device__ __forceinline__ bool first_eq(const CigarView& v, const CigarGeo& g) 
   return nib_at(v.codes, g.qbase + g.qb) == nib_at(v.codes, g.rbase + g.rb);
 }
 
-// A pair whose band missed the score retries with half-width nb: in the
-// narrow class of phase ph, or in the wide class.  Rare: per-lane atomic.
-__device__ __forceinline__ void retry(const CigarView& v, int ph, int64_t p, int32_t nb, int32_t rows) {
-  int cls = cigar_class(nb, rows);
-  if (ph >= kCigPhases) cls = kCigClassWide;
-  if (cls == kCigClassWide) ph = 0;
+// A pair whose band missed the score retries with half-width nb in the
+// wide class (one wavefront per pair: the few pairs that retry would each
+// take a whole narrow launch's latency).  Rare: per-lane atomic.
+__device__ __forceinline__ void retry(const CigarView& v, int64_t p, int32_t nb) {
   v.band[p] = nb;
-  const int32_t at = atomicAdd(&v.list_count[ph * kCigClasses + cls], 1);
-  list_of(v, ph, cls)[at] = (int32_t)p;
+  const int32_t at = atomicAdd(&v.list_count[kCigClassWide], 1);
+  list_of(v, kCigClassWide)[at] = (int32_t)p;
 }
 
 // One narrow pair per lane (p < 0: idle lane); all lanes call it.  DS is
@@ -137,7 +134,7 @@ __device__ __forceinline__ void retry(const CigarView& v, int ph, int64_t p, int
 // wavefront then reserves its ops with one atomic, and only pairs with more
 // ops (or whose traceback stops off column 0) walk again to write them.
 template <int BW, int DS>
-__device__ __forceinline__ void narrow_pair(const CigarView& v, int64_t p, uint32_t* dirs, int retry_phase) {
+__device__ __forceinline__ void narrow_pair(const CigarView& v, int64_t p, uint32_t* dirs) {
   constexpr int DW = DS;
   bool done = false, eq00 = false, queued = false;
   int32_t jf = -1, count = 0, mism = 0;
@@ -164,7 +161,7 @@ __device__ __forceinline__ void narrow_pair(const CigarView& v, int64_t p, uint3
         queued = q.fits();
       }
     } else {
-      retry(v, retry_phase, p, 2 * BW, g.ql());
+      retry(v, p, 2 * BW);
     }
   }
   const bool ok = done && jf >= 0;
@@ -213,7 +210,7 @@ __device__ __forceinline__ uint32_t* narrow_slot(const CigarView& v, int DW) {
 
 __global__ void cigar_clear_kernel(CigarView v) {
   const int t = threadIdx.x;
-  if (t < kCigWork + 4) v.list_count[t] = 0;
+  if (t < kCigWork + 3) v.list_count[t] = 0;
   if (t < 8) v.used[t] = 0ull;
   if (t == 0) v.max_rows[0] = 0;
 }
@@ -280,32 +277,30 @@ __global__ __launch_bounds__(256) void cigar_classify_kernel(CigarView v) {
     for (int k = 0; k < kCigClassifyPerThread; k++) {
       if (cls[k] >= 0) {
         const int64_t p = chunk * kCigChunk + k * 256 + t;
-        list_of(v, 0, cls[k])[base[cls[k]] + rank[k]] = (int32_t)p;
+        list_of(v, cls[k])[base[cls[k]] + rank[k]] = (int32_t)p;
       }
     }
     __syncthreads();
   }
 }
 
-// Half-width 1, phase 0.  Misses retry at half-width 2 in phase 0 (the
-// narrow kernel's first launch runs after this one).
+// Half-width 1.
 __global__ __launch_bounds__(256) void cigar_bw1_kernel(CigarView v) {
   uint32_t* dirs = narrow_slot(v, 1);
   if (!dirs) return;
   const int lane = threadIdx.x & 63;
   const int32_t count = lane0(v.list_count[0]);
-  const int32_t* list = list_of(v, 0, 0);
+  const int32_t* list = list_of(v, 0);
   for (;;) {
     const int32_t w = next_work(&v.list_count[kCigWork + 0]);
     if ((int64_t)w * 64 >= count) break;
     const int32_t idx = w * 64 + lane;
-    narrow_pair<1, 1>(v, idx < count ? list[idx] : -1, dirs, 0);
+    narrow_pair<1, 1>(v, idx < count ? list[idx] : -1, dirs);
   }
 }
 
-// Half-widths 2..7 of phase ph, one class per wavefront iteration.  Misses
-// retry in phase ph + 1 (half-width 4 or 6) or in the wide class.
-__global__ __launch_bounds__(256) void cigar_narrow_kernel(CigarView v, int ph) {
+// Half-widths 2..7, one class per wavefront iteration.
+__global__ __launch_bounds__(256) void cigar_narrow_kernel(CigarView v) {
   uint32_t* dirs = narrow_slot(v, 2);
   if (!dirs) return;
   const int lane = threadIdx.x & 63;
@@ -315,12 +310,12 @@ __global__ __launch_bounds__(256) void cigar_narrow_kernel(CigarView v, int ph) 
   cnt[0] = 0;
 #pragma unroll
   for (int c = 1; c < kCigExactMaxBw; c++) {
-    cnt[c] = lane0(v.list_count[ph * kCigClasses + c]);
+    cnt[c] = lane0(v.list_count[c]);
     total += (cnt[c] + 63) / 64;
     items[c] = total;  // items of classes 1..c
   }
   for (;;) {
-    const int32_t w = next_work(&v.list_count[kCigWork + 1 + ph]);
+    const int32_t w = next_work(&v.list_count[kCigWork + 1]);
     if (w >= total) break;
     int c = 1, first = 0;
 #pragma unroll
@@ -334,14 +329,14 @@ __global__ __launch_bounds__(256) void cigar_narrow_kernel(CigarView v, int ph) 
     int32_t n = cnt[1];
 #pragma unroll
     for (int k = 2; k < kCigExactMaxBw; k++) n = c == k ? cnt[k] : n;
-    const int64_t p = idx < n ? list_of(v, ph, c)[idx] : -1;
+    const int64_t p = idx < n ? list_of(v, c)[idx] : -1;
     switch (c) {
-      case 1: narrow_pair<2, 2>(v, p, dirs, ph + 1); break;
-      case 2: narrow_pair<3, 2>(v, p, dirs, ph + 1); break;
-      case 3: narrow_pair<4, 2>(v, p, dirs, ph + 1); break;
-      case 4: narrow_pair<5, 2>(v, p, dirs, ph + 1); break;
-      case 5: narrow_pair<6, 2>(v, p, dirs, ph + 1); break;
-      default: narrow_pair<7, 2>(v, p, dirs, ph + 1); break;
+      case 1: narrow_pair<2, 2>(v, p, dirs); break;
+      case 2: narrow_pair<3, 2>(v, p, dirs); break;
+      case 3: narrow_pair<4, 2>(v, p, dirs); break;
+      case 4: narrow_pair<5, 2>(v, p, dirs); break;
+      case 5: narrow_pair<6, 2>(v, p, dirs); break;
+      default: narrow_pair<7, 2>(v, p, dirs); break;
     }
   }
 }
@@ -351,9 +346,33 @@ __global__ __launch_bounds__(256) void cigar_narrow_kernel(CigarView v, int ph) 
 // ---------------------------------------------------------------------------
 namespace {
 
+// Lane shifts and scans over the wavefront with DPP (no LDS round trip).
+// from_next: lane l gets lane l+1's v, lane 63 gets last.  from_prev: lane l
+// gets lane l-1's v, lane 0 gets first.
+__device__ __forceinline__ int32_t from_next(int32_t v, int32_t last) {
+  return __builtin_amdgcn_update_dpp(last, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+}
+__device__ __forceinline__ int32_t from_prev(int32_t v, int32_t first) {
+  return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+// inclusive maximum over lanes 0..l: row_shr 1, 2, 4, 8 inside rows of 16
+// lanes, then row_bcast 15 and 31 across rows
+__device__ __forceinline__ int32_t wave_prefix_max(int32_t x, int32_t neg) {
+  x = imax(x, __builtin_amdgcn_update_dpp(neg, x, 0x111, 0xF, 0xF, false));
+  x = imax(x, __builtin_amdgcn_update_dpp(neg, x, 0x112, 0xF, 0xF, false));
+  x = imax(x, __builtin_amdgcn_update_dpp(neg, x, 0x114, 0xF, 0xF, false));
+  x = imax(x, __builtin_amdgcn_update_dpp(neg, x, 0x118, 0xF, 0xF, false));
+  x = imax(x, __builtin_amdgcn_update_dpp(neg, x, 0x142, 0xA, 0xF, false));
+  x = imax(x, __builtin_amdgcn_update_dpp(neg, x, 0x143, 0xC, 0xF, false));
+  return x;
+}
+
 // Band DP of half-width bw with C chunks of 64 diagonals per lane; writes one
 // direction byte per cell at dirs[i * 64C + d].  Returns H of the last cell
-// (the same in every lane).
+// (the same in every lane).  The SSW-ref codes read up to bw + 8 positions
+// before and 64C + 8 after the rectangle's columns, inside the slack around
+// every sequence of the code buffer for bands of up to 1024 diagonals;
+// codes off the rectangle are not used.
 template <int C>
 __device__ int32_t wide_dp(const CigarView& v, const CigarGeo& g, int32_t bw, uint8_t* dirs) {
   constexpr int32_t NEG = -(1 << 29);
@@ -366,57 +385,63 @@ __device__ int32_t wide_dp(const CigarView& v, const CigarGeo& g, int32_t bw, ui
     H[c] = 0;
     E[c] = 0;
   }
-  for (int32_t i = 0; i < ql; i++) {
-    const uint32_t qc = nib_at(v.codes, g.qbase + g.qb + i);
-    uint32_t tlo, thi;
-    subst_table(qc, v.sc.match, v.sc.mismatch, v.sc.ambiguity, tlo, thi);
-    int32_t carry = NEG;           // max X over earlier chunks
-    int32_t hl63 = 0, fl63 = 0;    // H and F of the diagonal left of this chunk's lane 0
-    uint8_t* row = dirs + (int64_t)i * (64 * C);
+  // codes in blocks of 8 rows, loaded one block ahead: the SSW-query code of
+  // each row, and for each chunk the SSW-ref codes of the lane's diagonal
+  // (column i - bw + d moves one to the right per row)
+  const int64_t qa = g.qbase + g.qb, ra = g.rbase + g.rb - bw + lane;
+  uint32_t qn = load8(v.codes, qa);
+  uint32_t rn[C];
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-      const int32_t d = lane + 64 * c;
-      const int32_t j = i - bw + d;
-      const bool valid = d < W;
-      const uint32_t rc = (valid && j >= 0 && j < rl) ? nib_at(v.codes, g.rbase + g.rb + j) : kCigKill;
-      // E from (i-1, d+1): the next lane, or lane 0 of the next chunk
-      int32_t hu = __shfl_down(H[c], 1, 64), eu = __shfl_down(E[c], 1, 64);
-      if (lane == 63) {
-        hu = c + 1 < C ? __builtin_amdgcn_readlane(H[(c + 1) % C], 0) : 0;
-        eu = c + 1 < C ? __builtin_amdgcn_readlane(E[(c + 1) % C], 0) : 0;
-      }
-      if (d + 1 >= W) {
-        hu = 0;
-        eu = 0;
-      }
-      const int32_t eo = hu - gO, ee = eu - gE;
-      const int32_t e = imax3(eo, ee, 0);
-      const int32_t t2 = H[c] + subst_lookup(tlo, thi, rc);
-      const int32_t h0 = imax3(e, t2, 0);
-      // F(d) = max(0, max_{k<d} X(k) - (d-1)*gE), X(k) = H0(k) - gO + k*gE
-      int32_t incl = valid ? h0 - gO + d * gE : NEG;
+  for (int c = 0; c < C; c++) rn[c] = load8(v.codes, ra + 64 * c);
+  for (int32_t i0 = 0; i0 < ql; i0 += 8) {
+    const uint32_t qw = qn;
+    uint32_t rw[C];
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl = imax(incl, y);
+    for (int c = 0; c < C; c++) rw[c] = rn[c];
+    qn = load8(v.codes, qa + i0 + 8);
+#pragma unroll
+    for (int c = 0; c < C; c++) rn[c] = load8(v.codes, ra + 64 * c + i0 + 8);
+    for (int r = 0; r < 8 && i0 + r < ql; r++) {
+      const int32_t i = i0 + r;
+      const uint32_t qc = (qw >> (4 * r)) & 0xFu;
+      uint32_t tlo, thi;
+      subst_table(qc, v.sc.match, v.sc.mismatch, v.sc.ambiguity, tlo, thi);
+      int32_t carry = NEG;         // max X over earlier chunks
+      int32_t hl63 = 0, fl63 = 0;  // H and F of the diagonal left of this chunk's lane 0
+      uint8_t* row = dirs + (int64_t)i * (64 * C);
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        const int32_t d = lane + 64 * c;
+        const int32_t j = i - bw + d;
+        const bool valid = d < W;
+        const uint32_t rc = (valid && j >= 0 && j < rl) ? (rw[c] >> (4 * r)) & 0xFu : kCigKill;
+        // E from (i-1, d+1): the next lane, or lane 0 of the next chunk
+        const int32_t hnext = c + 1 < C ? __builtin_amdgcn_readlane(H[(c + 1) % C], 0) : 0;
+        const int32_t enext = c + 1 < C ? __builtin_amdgcn_readlane(E[(c + 1) % C], 0) : 0;
+        int32_t hu = from_next(H[c], hnext), eu = from_next(E[c], enext);
+        if (d + 1 >= W) {
+          hu = 0;
+          eu = 0;
+        }
+        const int32_t eo = hu - gO, ee = eu - gE;
+        const int32_t e = imax3(eo, ee, 0);
+        const int32_t t2 = H[c] + subst_lookup(tlo, thi, rc);
+        const int32_t h0 = imax3(e, t2, 0);
+        // F(d) = max(0, max_{k<d} X(k) - (d-1)*gE), X(k) = H0(k) - gO + k*gE
+        const int32_t incl = wave_prefix_max(valid ? h0 - gO + d * gE : NEG, NEG);
+        const int32_t excl = from_prev(incl, NEG);
+        const int32_t f = imax(0, imax(excl, carry) - (d - 1) * gE);
+        carry = imax(carry, __builtin_amdgcn_readlane(incl, 63));
+        const int32_t h = imax(h0, f);
+        // H and F of (i, d-1) for F's open/extend bit
+        const int32_t hl = from_prev(h, hl63), fl = from_prev(f, fl63);
+        hl63 = __builtin_amdgcn_readlane(h, 63);
+        fl63 = __builtin_amdgcn_readlane(f, 63);
+        const int32_t t1 = imax(e, f);
+        if (valid) row[d] = (uint8_t)cell_bits(t1, t2, e, f, rc == qc, eo > ee, hl - gO > fl - gE);
+        H[c] = valid ? h : 0;
+        E[c] = valid ? e : 0;
       }
-      int32_t excl = __shfl_up(incl, 1, 64);
-      if (lane == 0) excl = NEG;
-      const int32_t f = imax(0, imax(excl, carry) - (d - 1) * gE);
-      carry = imax(carry, __builtin_amdgcn_readlane(incl, 63));
-      const int32_t h = imax(h0, f);
-      // H and F of (i, d-1) for F's open/extend bit
-      int32_t hl = __shfl_up(h, 1, 64), fl = __shfl_up(f, 1, 64);
-      if (lane == 0) {
-        hl = hl63;
-        fl = fl63;
-      }
-      hl63 = __builtin_amdgcn_readlane(h, 63);
-      fl63 = __builtin_amdgcn_readlane(f, 63);
-      const int32_t t1 = imax(e, f);
-      if (valid) row[d] = (uint8_t)cell_bits(t1, t2, e, f, rc == qc, eo > ee, hl - gO > fl - gE);
-      H[c] = valid ? h : 0;
-      E[c] = valid ? e : 0;
     }
   }
   const int32_t dc = (rl - 1) - (ql - 1) + bw;
@@ -431,8 +456,9 @@ __device__ int32_t wide_dp(const CigarView& v, const CigarGeo& g, int32_t bw, ui
 
 // Traceback of a wide band (bytes, Wp per row), walked by all lanes in step
 // over blocks of rows copied to LDS; ops written from lane 0.  As cigar_walk.
+template <class Emit>
 __device__ int32_t wide_walk(const CigarView& v, const CigarGeo& g, int32_t bw, int32_t Wp, const uint8_t* dirs,
-                             uint32_t* blk, int32_t shift, bool eq00, CigarEmit& em) {
+                             uint32_t* blk, int32_t shift, bool eq00, Emit& em) {
   const int lane = threadIdx.x & 63;
   const int32_t R = kWideBlkBytes / Wp;
   const uint8_t* b8 = reinterpret_cast<const uint8_t*>(blk);
@@ -484,9 +510,9 @@ __global__ __launch_bounds__(64) void cigar_wide_kernel(CigarView v) {
   __shared__ uint32_t blk[kWideBlkBytes / 4];
   const int lane = threadIdx.x & 63;
   const int32_t count = lane0(v.list_count[kCigClassWide]);
-  const int32_t* list = list_of(v, 0, kCigClassWide);
+  const int32_t* list = list_of(v, kCigClassWide);
   for (;;) {
-    const int32_t w = next_work(&v.list_count[kCigWork + 3]);
+    const int32_t w = next_work(&v.list_count[kCigWork + 2]);
     if (w >= count) break;
     const int64_t p = lane0(list[w]);
     const CigarGeo g = pair_geo(v, p);
@@ -538,31 +564,39 @@ __global__ __launch_bounds__(64) void cigar_wide_kernel(CigarView v) {
       bw *= 2;
     }
     if (status == 1) {
+      // every lane walks the same traceback; the CIGAR stays in registers
+      // when it is short, else a second walk writes it from lane 0
       const uint8_t* dirs = reinterpret_cast<const uint8_t*>(mem);
-      CigarEmit em;
-      em.out = nullptr;
-      em.nops = 0;
-      int32_t jf = wide_walk(v, g, bw, Wp, dirs, blk, 0, eq00, em);
+      CigarEmitQ q;
+      int32_t jf = wide_walk(v, g, bw, Wp, dirs, blk, 0, eq00, q);
+      int32_t count = q.count, mism = q.mism;
+      bool queued = jf == 0 && q.fits();
       if (jf > 0) {
-        em = CigarEmit();
+        CigarEmit em;
         em.out = nullptr;
         em.nops = 0;
         jf = wide_walk(v, g, bw, Wp, dirs, blk, jf, eq00, em);
+        count = em.count;
+        mism = em.mism;
       }
       if (jf < 0) {
         status = 4;
       } else {
-        const int64_t base = lane0_add(&v.used[0], em.count);
-        const bool fits = base + em.count <= v.ops_cap;
-        if (fits && em.count > 0) {
-          CigarEmit wr;
-          wr.out = lane == 0 ? v.ops + base : nullptr;
-          wr.nops = em.count;
-          wide_walk(v, g, bw, Wp, dirs, blk, jf, eq00, wr);
+        const int64_t base = lane0_add(&v.used[0], count);
+        const bool fits = base + count <= v.ops_cap;
+        if (fits && count > 0) {
+          if (queued) {
+            if (lane == 0) q.write(v.ops + base);
+          } else {
+            CigarEmit wr;
+            wr.out = lane == 0 ? v.ops + base : nullptr;
+            wr.nops = count;
+            wide_walk(v, g, bw, Wp, dirs, blk, jf, eq00, wr);
+          }
         }
         if (lane == 0) {
-          v.n_ops[p] = fits ? em.count : -1;
-          v.mism[p] = em.mism;
+          v.n_ops[p] = fits ? count : -1;
+          v.mism[p] = mism;
           v.off[p] = base;
         }
       }
@@ -619,8 +653,7 @@ hipError_t launch_cigar(const CigarView& v, int narrow_waves, int wide_waves, hi
   hipLaunchKernelGGL(cigar_classify_kernel, dim3(cig_grid(v.npairs, kCigChunk)), dim3(256), 0, st, v);
   const int blocks = std::max(1, narrow_waves / 4);
   hipLaunchKernelGGL(cigar_bw1_kernel, dim3(blocks), dim3(256), 0, st, v);
-  for (int ph = 0; ph < kCigPhases; ph++)
-    hipLaunchKernelGGL(cigar_narrow_kernel, dim3(blocks), dim3(256), 0, st, v, ph);
+  hipLaunchKernelGGL(cigar_narrow_kernel, dim3(blocks), dim3(256), 0, st, v);
   hipLaunchKernelGGL(cigar_wide_kernel, dim3(std::max(1, wide_waves)), dim3(64), 0, st, v);
   return hipGetLastError();
 }

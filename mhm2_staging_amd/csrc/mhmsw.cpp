@@ -856,10 +856,11 @@ constexpr int64_t kCigDirsDwords = 64ll << 20;
 static int prepare_cigar(mhmsw_ctx* c) {
   const size_t np = (size_t)std::max<int64_t>(c->npairs, 1);
   c->cig_ops_cap = std::max<int64_t>(c->cig_ops_cap, 12 * (int64_t)np + 1024);
-  c->cig_gen_dwords = std::max<int64_t>(c->cig_gen_dwords, 4ll << 20);
+  // wide-band pool: about 1% of pairs take a wide band, ~4 KB each for 250-row rectangles
+  c->cig_gen_dwords = std::max<int64_t>(c->cig_gen_dwords, std::max<int64_t>(4ll << 20, 32 * (int64_t)np));
   c->cig_dirs_dwords = kCigDirsDwords;
-  HIP_OR_FAIL(c, c->d_cig_lists.ensure((size_t)2 * kCigClasses * np));  // two phases of class lists
-  HIP_OR_FAIL(c, c->d_cig_count.ensure(4 * kCigClasses));
+  HIP_OR_FAIL(c, c->d_cig_lists.ensure((size_t)kCigClasses * np));
+  HIP_OR_FAIL(c, c->d_cig_count.ensure(2 * kCigClasses));
   HIP_OR_FAIL(c, c->d_cig_band.ensure(np));
   HIP_OR_FAIL(c, c->d_cig_rows.ensure(1));
   HIP_OR_FAIL(c, c->d_cig_nops.ensure(np));
@@ -898,10 +899,10 @@ static CigarView make_cigar_view(mhmsw_ctx* c) {
   return v;
 }
 
-// wavefronts of the CIGAR launches: 16 per CU for the narrow bands, 1 per CU
-// for the wide ones (one pair per wavefront)
+// wavefronts of the CIGAR launches: 16 per CU for the narrow bands, 12 per CU
+// for the wide ones (one pair per wavefront, 8 KB of LDS each)
 static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
-  return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus, st);
+  return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus * 12, st);
 }
 
 // ev: record phase events (eager launches only; events recorded inside a

@@ -52,8 +52,8 @@ struct CigarView {
   const int32_t* seq_len;
   const AlnResult* results;
   ScoreParams sc;
-  int32_t* lists;       // [2 phases][kCigClasses][npairs] pairs of each band class
-  int32_t* list_count;  // [2 phases][kCigClasses] entries, then work counters
+  int32_t* lists;       // [kCigClasses][npairs] pairs of each band class
+  int32_t* list_count;  // [kCigClasses] entries, then work counters
   int32_t* band;        // [npairs] band half-width of the pair's next attempt
   int32_t* max_rows;    // [1] tallest rectangle of the register classes
   uint32_t* dirs;       // direction scratch, one slot per wavefront

@@ -52,7 +52,7 @@ int main() {
   int64_t np;
   oracle_scoring sc;
   if (!(std::cin >> np >> sc.match >> sc.mismatch >> sc.gap_open >> sc.gap_ext >> sc.ambiguity)) return 2;
-  int64_t bad = 0, cls_count[kCigClasses + 1] = {0};
+  int64_t bad = 0, cls_count[kCigClasses + 1] = {0}, retries[kCigClasses] = {0}, final_wide = 0;
   std::vector<uint32_t> slot, gen;
   for (int64_t p = 0; p < np; p++) {
     std::string R, Q;
@@ -134,6 +134,7 @@ int main() {
           }
           break;
         }
+        retries[cls]++;
         if (bw > 4 * (g.rl() + g.ql()) + 4) {
           std::printf("pair %lld: band never reaches the score\n", (long long)p);
           return 1;
@@ -141,6 +142,7 @@ int main() {
         bw *= 2;
       }
       cls_count[first_cls]++;
+      if (cigar_class(bw, g.ql()) == kCigClassWide) final_wide++;
       // as the kernels: band classes walk rows in groups first, the generic
       // class walks row by row; a traceback that stops off column 0 is
       // walked again comparing shifted codes
@@ -212,6 +214,9 @@ int main() {
   }
   std::printf("ok %lld", (long long)np);
   for (int c = 0; c <= kCigClasses; c++) std::printf(" %lld", (long long)cls_count[c]);
+  std::fprintf(stderr, "retries by class:");
+  for (int c = 0; c < kCigClasses; c++) std::fprintf(stderr, " %lld", (long long)retries[c]);
+  std::fprintf(stderr, "; pairs ending in the wide class: %lld\n", (long long)final_wide);
   std::printf("\n");
   return 0;
 }
