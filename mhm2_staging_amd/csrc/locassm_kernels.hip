@@ -1,0 +1,329 @@
+// Local assembly walks on gfx950: one wavefront per contig side.
+//
+// The reference (src/gpu_loc_assem/kernel.cpp:391-545) gives each contig a
+// warp whose lanes count mers into a string-keyed hash table in global
+// memory (MurmurHash of the mer bytes at every probe, kernel.cpp:36-136,
+// 197-241), after which one lane walks the table, re-hashing the growing
+// walk at every step and checking a second table for repeats
+// (kernel.cpp:243-307).  Here, per mer length:
+//
+//   1. count: the side's mer occurrences are split evenly over the 64 lanes;
+//      each lane rolls a polynomial hash along its run of occurrences and
+//      inserts into an open-addressing table whose 64-bit slot word holds a
+//      32-bit fingerprint and the position of one occurrence (claimed with
+//      one 64-bit CAS, so the position is never read before it is written);
+//      a fingerprint match is confirmed by comparing the bytes;
+//   2. decide: every occupied slot gets its extension (the reference's
+//      set_ext pass, kernel.cpp:376-381) and, when the extension is a base,
+//      the slot of the mer that follows it (the mer the walk would look up
+//      next), found with the rolled hash of the slot's own occurrence;
+//   3. walk: one lane follows successor links from the slot of the contig's
+//      last mer.  A step is one atomic OR on the slot's info word, which
+//      reads the extension and successor and marks the slot visited: a
+//      visited slot is a repeat (the reference's repeat table holds exactly
+//      the mers of the walk so far, each of which is a slot).
+//
+// Slot words, counts and info words change through atomics (performed at
+// L2), so every read of them goes through L2 (relaxed agent-scope atomic
+// loads).
+#include "locassm_launch.h"
+
+namespace mhmla {
+namespace {
+
+constexpr int kLanes = 64;
+
+__device__ __forceinline__ uint64_t ld64(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld32(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool same_bytes(const uint8_t* a, const uint8_t* b, int n) {
+  for (int i = 0; i < n; i++)
+    if (a[i] != b[i]) return false;
+  return true;
+}
+
+__device__ __forceinline__ uint64_t hash_bytes(const uint8_t* s, int k) {
+  uint64_t h = 0;
+  for (int i = 0; i < k; i++) h = h * kHashBase + s[i];
+  return h;
+}
+
+struct Side {
+  const uint8_t* seq;   // all read bases
+  const uint8_t* qual;  // all read quality bytes
+  uint64_t* key;
+  uint64_t* hsh;
+  uint32_t* cnt;
+  uint32_t* info;
+  int64_t* roff;        // offsets of the side's reads
+  int32_t* rlen;
+  uint32_t* cum;        // occurrences before read i, at the current mer length
+  uint8_t* walk[2];
+  int cap;
+  int nr;
+
+  __device__ const uint8_t* occ(uint64_t w) const {
+    return seq + roff[(w >> 16) & 0xFFFF] + (w & 0xFFFF);
+  }
+};
+
+// Find the slot whose mer equals the bytes 'same' accepts; insert 'mine'
+// (fingerprint | read | start) there when absent and insert is set.
+// Returns the slot, -1 when absent and not inserted, -2 if the table is full.
+template <class Same>
+__device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool insert, bool* won, Same same) {
+  const uint64_t m = mix64(h);
+  const uint32_t fp = (uint32_t)(m >> 32);
+  const uint64_t mine = ((uint64_t)fp << 32) | mine_low;
+  int s = (int)(m & (uint64_t)(S.cap - 1));
+  for (int probes = 0; probes < S.cap; probes++) {
+    uint64_t w = ld64(&S.key[s]);
+    if (w == kEmpty) {
+      if (!insert) return -1;
+      const uint64_t prev = atomicCAS((unsigned long long*)&S.key[s], (unsigned long long)kEmpty,
+                                      (unsigned long long)mine);
+      if (prev == kEmpty) {
+        *won = true;
+        return s;
+      }
+      w = prev;
+    }
+    if ((uint32_t)(w >> 32) == fp && same(S.occ(w))) return s;
+    s = (s + 1) & (S.cap - 1);
+  }
+  return -2;
+}
+
+__device__ __forceinline__ uint8_t side_byte(const uint8_t* ctg, int L, bool right, int i) {
+  return right ? ctg[i] : complement(ctg[L - 1 - i]);
+}
+
+__global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
+  __shared__ int sh_t, sh_res;
+  const int lane = threadIdx.x;
+  const int slot = blockIdx.x;
+  const int job = v.jobs[slot];
+  const int c = job >> 1;
+  const bool right = (job & 1) != 0;
+  const int L = v.ctg_len[c];
+  const uint8_t* ctg = v.ctg_seq + v.ctg_off[c];
+  const int64_t* first = right ? v.right_first : v.left_first;
+  const int64_t f0 = first[c];
+  const int nr = (int)min(first[c + 1] - f0, (int64_t)v.max_reads);
+  const int W = v.walk_len_limit;
+
+  Side S;
+  S.seq = v.read_seq;
+  S.qual = v.read_qual;
+  S.cap = v.cap[slot];
+  S.nr = nr;
+  {
+    uint8_t* base = v.scratch + v.scratch_off[slot];
+    const JobLayout lay = job_layout(S.cap, nr, W);
+    S.key = (uint64_t*)(base + lay.key);
+    S.hsh = (uint64_t*)(base + lay.hsh);
+    S.cnt = (uint32_t*)(base + lay.cnt);
+    S.info = (uint32_t*)(base + lay.info);
+    S.roff = (int64_t*)(base + lay.roff);
+    S.rlen = (int32_t*)(base + lay.rlen);
+    S.cum = (uint32_t*)(base + lay.cum);
+    S.walk[0] = base + lay.walk0;
+    S.walk[1] = base + lay.walk1;
+  }
+  for (int i = lane; i < nr; i += kLanes) {
+    const int64_t r = v.read_idx[f0 + i];
+    S.roff[i] = v.read_off[r];
+    S.rlen[i] = v.read_len[r];
+  }
+  __threadfence();
+  __syncthreads();
+
+  const int depth = v.depth[c];
+  const int max_len = min(v.max_kmer_len, L);
+  int longest = 0, best = 0, cur = 0, shift = 0;
+  int term_x = 0, term_f = 0, term_r = 0;
+  unsigned long long mers = 0, steps = 0, walks = 0;
+
+  for (int k = v.kmer_len; k >= v.min_kmer_len && k <= max_len; k += shift) {
+    // ---- clear the table and count occurrences per read ----
+    for (int s = lane; s < S.cap; s += kLanes) {
+      S.key[s] = kEmpty;
+      uint4* cp = (uint4*)(S.cnt + 8 * (int64_t)s);
+      cp[0] = make_uint4(0, 0, 0, 0);
+      cp[1] = make_uint4(0, 0, 0, 0);
+    }
+    uint32_t carry = 0;
+    if (lane == 0) S.cum[0] = 0;
+    for (int i0 = 0; i0 < nr; i0 += kLanes) {
+      const int i = i0 + lane;
+      uint32_t x = i < nr ? (uint32_t)max(0, S.rlen[i] - k) : 0u;
+      for (int d = 1; d < kLanes; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, kLanes);
+        if (lane >= d) x += y;
+      }
+      if (i < nr) S.cum[i + 1] = carry + x;
+      carry += __shfl(x, kLanes - 1, kLanes);
+    }
+    const uint32_t N = carry;
+    __threadfence();
+    __syncthreads();
+
+    // ---- count: lane takes occurrences [q0, q1) of the side's flattened list ----
+    const uint64_t Bk1 = pow_base(k - 1);
+    {
+      const uint32_t per = (N + kLanes - 1) / kLanes;
+      const uint32_t q0 = lane * per, q1 = min(N, q0 + per);
+      if (q0 < q1) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (S.cum[mid] <= q0) lo = mid; else hi = mid - 1;
+        }
+        int i = lo, pos = (int)(q0 - S.cum[i]);
+        const uint8_t* rd = S.seq + S.roff[i];
+        const uint8_t* rq = S.qual + S.roff[i];
+        uint64_t h = hash_bytes(rd + pos, k);
+        for (uint32_t q = q0;;) {
+          bool won = false;
+          const uint8_t* mine = rd + pos;
+          const int s = find_slot(S, h, ((uint64_t)i << 16) | (uint64_t)pos, true, &won,
+                                  [&](const uint8_t* o) { return same_bytes(o, mine, k); });
+          if (s == -2) atomicAdd(&v.counters[3], 1ull);
+          if (s >= 0) {
+            if (won) S.hsh[s] = h;
+            const uint8_t e = rd[pos + k];
+            const int b = vote_base(e);
+            if (b >= 0) {
+              const int qd = (int)(int8_t)rq[pos + k] - v.qual_offset;
+              if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)s + b], 1u);
+              if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)s + 4 + b], 1u);
+            }
+          }
+          if (++q >= q1) break;
+          if (++pos >= S.rlen[i] - k) {
+            do { i++; } while (S.rlen[i] - k <= 0);
+            pos = 0;
+            rd = S.seq + S.roff[i];
+            rq = S.qual + S.roff[i];
+            h = hash_bytes(rd, k);
+          } else {
+            h = (h - (uint64_t)rd[pos - 1] * Bk1) * kHashBase + rd[pos - 1 + k];
+          }
+        }
+      }
+    }
+    mers += N;
+    __threadfence();
+    __syncthreads();
+
+    // ---- decide: extension and successor of every occupied slot ----
+    for (int s = lane; s < S.cap; s += kLanes) {
+      const uint64_t w = ld64(&S.key[s]);
+      if (w == kEmpty) continue;
+      uint32_t low[4], hiq[4];
+      for (int b = 0; b < 4; b++) {
+        low[b] = ld32(&S.cnt[8 * (int64_t)s + b]);
+        hiq[b] = ld32(&S.cnt[8 * (int64_t)s + 4 + b]);
+      }
+      const int ext = choose_ext(low, hiq, depth, v.rules);
+      uint32_t succ = kNoSlot;
+      if (ext < 4) {
+        const uint8_t* rep = S.occ(w);
+        const uint8_t ec = (uint8_t)ext_char(ext);
+        const uint64_t hs = (ld64(&S.hsh[s]) - (uint64_t)rep[0] * Bk1) * kHashBase + ec;
+        bool won = false;
+        const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
+          return same_bytes(o, rep + 1, k - 1) && o[k - 1] == ec;
+        });
+        if (t >= 0) succ = (uint32_t)t;
+      }
+      S.info[s] = (uint32_t)ext | (succ << 3);
+    }
+    __threadfence();
+    __syncthreads();
+
+    // ---- walk from the contig's last mer ----
+    if (lane == 0) {
+      uint64_t h0 = 0;
+      for (int j = 0; j < k; j++) h0 = h0 * kHashBase + side_byte(ctg, L, right, L - k + j);
+      bool won = false;
+      int e = find_slot(S, h0, 0, false, &won, [&](const uint8_t* o) {
+        for (int j = 0; j < k; j++)
+          if (o[j] != side_byte(ctg, L, right, L - k + j)) return false;
+        return true;
+      });
+      uint8_t* wb = S.walk[cur];
+      int t = 0, res = 0;  // 0 X, 1 F, 2 R
+      while (t < W) {
+        if (e < 0) {
+          res = 0;
+          break;
+        }
+        const uint32_t old = atomicOr(&S.info[e], kVisited);
+        if (old & kVisited) {
+          res = 2;
+          break;
+        }
+        const int ext = (int)(old & 7u);
+        if (ext >= 4) {
+          res = ext == kExtF ? 1 : 0;
+          break;
+        }
+        wb[t++] = (uint8_t)ext_char(ext);
+        const uint32_t nx = (old >> 3) & kNoSlot;
+        e = nx == kNoSlot ? -1 : (int)nx;
+      }
+      sh_t = t;
+      sh_res = res;
+    }
+    __threadfence();
+    __syncthreads();
+    const int t = sh_t, res = sh_res;
+    steps += (unsigned long long)t;
+    walks++;
+    term_x += res == 0;
+    term_f += res == 1;
+    term_r += res == 2;
+    if (t > longest) {
+      longest = t;
+      best = cur;
+      cur ^= 1;
+    }
+    __syncthreads();  // sh_t / sh_res are rewritten by the next walk
+    if (res == 0) {
+      if (shift == v.shift) break;
+      shift = -v.shift;
+    } else {
+      if (shift == -v.shift) break;
+      if (k > L) break;
+      shift = v.shift;
+    }
+  }
+
+  // ---- the longest walk, reverse-complemented on the left side ----
+  uint8_t* out = v.out_ext + (int64_t)job * W;
+  const uint8_t* src = S.walk[best];
+  for (int j = lane; j < longest; j += kLanes) out[j] = right ? src[j] : complement(src[longest - 1 - j]);
+  if (lane == 0) {
+    v.out_len[job] = longest;
+    v.out_terms[3 * job] = term_x;
+    v.out_terms[3 * job + 1] = term_f;
+    v.out_terms[3 * job + 2] = term_r;
+    atomicAdd(&v.counters[0], mers);
+    atomicAdd(&v.counters[1], steps);
+    atomicAdd(&v.counters[2], walks);
+  }
+}
+
+}  // namespace
+
+void launch_walks(const WalkView& v, int64_t njobs, hipStream_t st) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(walk_kernel, dim3((unsigned)njobs), dim3(kLanes), 0, st, v);
+}
+
+}  // namespace mhmla

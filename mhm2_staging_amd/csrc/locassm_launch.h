@@ -1,0 +1,73 @@
+// Device view of a staged local assembly batch and one launch of the walk
+// kernel (locassm_kernels.hip), filled by the host runtime (locassm.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "locassm_core.h"
+
+namespace mhmla {
+
+// a table slot's key word: fingerprint (32 bits) | read number within the
+// side (16 bits) | start of the mer in that read (16 bits); all ones = empty
+constexpr uint64_t kEmpty = ~0ull;
+// a slot's info word: extension code (3 bits) | successor slot << 3; the
+// walk marks a visited slot with the top bit
+constexpr uint32_t kNoSlot = 0x0FFFFFFFu;
+constexpr uint32_t kVisited = 0x80000000u;
+
+struct WalkView {
+  // staged batch
+  const uint8_t* ctg_seq;
+  const int64_t* ctg_off;
+  const int32_t* ctg_len;
+  const int32_t* depth;        // (int) of the double depth, as set_ext receives it
+  const int64_t* left_first;
+  const int64_t* right_first;
+  const int64_t* read_idx;
+  const uint8_t* read_seq;
+  const uint8_t* read_qual;
+  const int64_t* read_off;
+  const int32_t* read_len;
+  // this launch: jobs (contig * 2 + side, side 1 = right) and their scratch
+  const int32_t* jobs;
+  const int64_t* scratch_off;  // bytes into scratch, per launch slot
+  const int32_t* cap;          // table slots (power of two), per launch slot
+  uint8_t* scratch;
+  // outputs, indexed by job
+  int32_t* out_len;
+  uint8_t* out_ext;            // walk_len_limit bytes per job
+  int32_t* out_terms;          // 3 per job: walks ending in X, F, R
+  unsigned long long* counters;  // [0] mers counted, [1] walk steps, [2] walks, [3] full tables
+  // parameters
+  int32_t kmer_len, min_kmer_len, max_kmer_len, shift, walk_len_limit;
+  int32_t qual_offset, min_qual, min_hi_qual, max_reads;
+  Rules rules;
+};
+
+// scratch layout of one job (byte offsets from its base), shared by the host
+// planner and the kernel
+struct JobLayout {
+  int64_t key, hsh, cnt, info, roff, rlen, cum, walk0, walk1, bytes;
+};
+
+inline __host__ __device__ int64_t la_align(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+inline __host__ __device__ JobLayout job_layout(int64_t cap, int64_t nreads, int64_t walk_len_limit) {
+  JobLayout l;
+  l.key = 0;
+  l.hsh = la_align(l.key + 8 * cap);
+  l.cnt = la_align(l.hsh + 8 * cap);
+  l.info = la_align(l.cnt + 32 * cap);
+  l.roff = la_align(l.info + 4 * cap);
+  l.rlen = la_align(l.roff + 8 * nreads);
+  l.cum = la_align(l.rlen + 4 * nreads);
+  l.walk0 = la_align(l.cum + 4 * (nreads + 1));
+  l.walk1 = la_align(l.walk0 + walk_len_limit);
+  l.bytes = la_align(l.walk1 + walk_len_limit);
+  return l;
+}
+
+void launch_walks(const WalkView& v, int64_t njobs, hipStream_t st);
+
+}  // namespace mhmla

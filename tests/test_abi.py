@@ -9,12 +9,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "mhm2_staging_amd", "libmhmsw.so")
 
 
-def declared_c_functions():
+def declared_c_functions(headers=("mhmsw.h", "mhmsw_synth.h"), prefix="mhmsw_"):
     names = []
-    for h in ("mhmsw.h", "mhmsw_synth.h"):
+    for h in headers:
         with open(os.path.join(ROOT, "include", h)) as f:
             txt = f.read()
-        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(mhmsw_\w+)\s*\(", txt, re.M)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(" + prefix + r"\w+)\s*\(", txt, re.M)
     return sorted(set(names))
 
 
@@ -32,6 +32,20 @@ def test_exports_every_declared_c_symbol():
     assert len(decl) >= 18
     missing = [d for d in decl if d not in exported()]
     assert not missing, missing
+
+
+def test_exports_every_local_assembly_symbol():
+    decl = declared_c_functions(("mhmla.h",), "mhmla_")
+    assert len(decl) == 11
+    missing = [d for d in decl if d not in exported()]
+    assert not missing, missing
+    from mhm2_staging_amd import locassm
+    assert sorted(locassm.MHMLA_EXPORTS) == decl
+    L = locassm.lib()
+    p = locassm.params()
+    assert (p.kmer_len, p.min_kmer_len, p.max_kmer_len, p.shift, p.min_qual, p.min_hi_qual, p.max_reads) == \
+        (33, 21, 121, 8, 10, 20, 3000)
+    assert L.mhmla_device_count() >= 0
 
 
 def test_exports_adept_sw_driver_symbols():
@@ -69,6 +83,10 @@ def test_no_device_paths_fail_cleanly_on_cpu():
     assert L.mhmsw_create(0, ctypes.byref(ctx)) == -4  # MHMSW_E_NODEVICE
     with pytest.raises(m.MhmswError):
         m.Aligner()
+    from mhm2_staging_amd import locassm
+    assert L.mhmla_create(0, ctypes.byref(ctx)) == -4  # MHMLA_E_NODEVICE
+    with pytest.raises(locassm.LocAssmError):
+        locassm.LocalAssembler()
     drv = m.GPUDriver()
     with pytest.raises(RuntimeError):
         drv.init(0, 1, 1, -3, -5, -2, 300)

@@ -1,0 +1,155 @@
+"""Local assembly walks on the GPU (include/mhmla.h) against the oracle
+(oracle/locassm_oracle.c, the reference kernel's semantics restated): every
+extension byte, every length and every walk-termination count, on the
+closed-form cases of tests/test_oracle_locassm.py, generated batches at
+several mer lengths and walk limits, and edge cases."""
+import numpy as np
+import pytest
+
+from mhm2_staging_amd import locassm
+from mhm2_staging_amd import locassm_data as ld
+from oracle_lib import la_extend_batch, la_params
+
+pytestmark = pytest.mark.gpu
+
+COMP = str.maketrans("ACGTN", "TGCAN")
+
+
+def rc(s):
+    return s.translate(COMP)[::-1]
+
+
+def rand_seq(rng, n):
+    return "".join(rng.choice(list("ACGT"), n))
+
+
+@pytest.fixture(scope="module")
+def assembler():
+    assert locassm.lib().mhmla_device_count() > 0, "no GPU visible: GPU tests must run on the GPU machine"
+    a = locassm.LocalAssembler(0)
+    yield a
+    a.close()
+
+
+ORACLE_FIELDS = ("left_len", "left", "right_len", "right", "terms")
+
+
+def check(assembler, b, **kw):
+    print(f"[gpu] locassm nctgs={b.nctgs} nreads={b.nreads} params={kw}", flush=True)
+    got = assembler.extend(b, locassm.params(**kw))
+    want = la_extend_batch(la_params(**kw), b, with_terms=True)
+    ll, le, rl, re, tm = want
+    for name, g, w in (("left_len", got.left_len, ll), ("right_len", got.right_len, rl)):
+        bad = np.nonzero(g != w)[0]
+        assert len(bad) == 0, f"{name} differs at contig {bad[0]}: got {g[bad[0]]}, want {w[bad[0]]}"
+    for c in range(b.nctgs):
+        assert got.left_str(c) == bytes(le[c, :ll[c]]).decode(), f"left extension of contig {c}"
+        assert got.right_str(c) == bytes(re[c, :rl[c]]).decode(), f"right extension of contig {c}"
+    bad = np.nonzero((got.terms != tm).any(axis=(1, 2)))[0]
+    assert len(bad) == 0, f"walk terminations differ at contig {bad[0]}: got {got.terms[bad[0]]}, want {tm[bad[0]]}"
+    return got
+
+
+def closed_form_batch():
+    """The cases of tests/test_oracle_locassm.py as one batch (right sides),
+    plus the left-side case."""
+    ctgs = []
+    rng = np.random.default_rng(12)
+    g = rand_seq(rng, 3000)
+    c = 1000
+    alt = g[:c + 50] + ("A" if g[c + 50] != "A" else "C") + g[c + 51:]
+    reads = []
+    for s in (c - 140, c - 130, c - 125):
+        reads.append((g[s:s + 300], "I" * 300))
+        reads.append((alt[s:s + 300], "I" * 300))
+    ctgs.append((g[:c], 10.0, [], reads))                       # fork at every mer length
+    rng = np.random.default_rng(13)
+    unit = rand_seq(rng, 25)
+    per = unit * 40
+    ctgs.append((rand_seq(rng, 200) + per[:150], 10.0, [], [(per[:300], "I" * 300)]))  # repeat after 25
+    rng = np.random.default_rng(11)
+    g = rand_seq(rng, 2000)
+    ctgs.append((g[500:1500], 10.0, [(rc(g[440:600]), "I" * 160), (rc(g[400:560]), "I" * 160)], []))
+    for seed in range(6):
+        rng = np.random.default_rng(seed)
+        g = rand_seq(rng, 3000)
+        reads = []
+        for _ in range(int(rng.integers(1, 12))):
+            s = int(rng.integers(1200 - 150, 1200 + 400))
+            L = int(rng.integers(60, 151))
+            reads.append((g[s:s + L], "I" * L))
+        ctgs.append((g[:1200], 10.0, [], reads))
+    return ld.from_contigs(ctgs)
+
+
+def test_closed_form_cases(assembler):
+    b = closed_form_batch()
+    got = check(assembler, b, kmer_len=33, walk_len_limit=300)
+    assert got.right_len[0] == 50 and tuple(got.terms[0, 1]) == (0, 12, 0)
+    assert got.right_len[1] == 25 and tuple(got.terms[1, 1]) == (0, 0, 12)
+    assert got.left_len[2] == 100
+
+
+@pytest.mark.parametrize("kmer_len,walk,seed", [(21, 300, 1), (33, 360, 2), (55, 360, 3), (77, 120, 4),
+                                                (99, 500, 5), (121, 360, 6)])
+def test_generated_batches(assembler, kmer_len, walk, seed):
+    b = ld.generate(1500, seed=seed)
+    got = check(assembler, b, kmer_len=kmer_len, walk_len_limit=walk)
+    assert (got.right_len > 0).mean() > 0.2
+
+
+def test_other_rules_and_qualities(assembler):
+    b = ld.generate(800, seed=9, qual_offset=64)
+    check(assembler, b, kmer_len=33, walk_len_limit=300, qual_offset=64)
+    check(assembler, b, kmer_len=41, walk_len_limit=200, qual_offset=64, min_qual=15, min_hi_qual=30,
+          min_viable_depth=0.1, min_expected_depth=0.5, rating_thres=2, shift=4, min_kmer_len=17, max_kmer_len=65)
+
+
+def test_edge_cases(assembler):
+    rng = np.random.default_rng(21)
+    g = rand_seq(rng, 2000)
+    c = 600
+    ctgs = [
+        ("", 5.0, [], []),                                            # empty contig
+        (g[:20], 5.0, [(g[:100], "I" * 100)], [(g[:100], "I" * 100)]),  # shorter than the smallest mer
+        (g[:c], 5.0, [], []),                                         # no reads
+        (g[:c], 5.0, [], [("ACGT", "IIII"), ("", "")]),               # reads shorter than any mer
+        (g[:c - 10] + "N" + g[c - 9:c], 7.9, [], [(g[c - 60:c - 10] + "N" + g[c - 9:c + 90], "I" * 150)]),
+        (g[:c - 5] + "R" + g[c - 4:c], 7.0, [(rc(g[:60]), "I" * 60)], []),  # IUPAC: N on the left side
+        (g[:c], 30.0, [], [(g[c - 50:c + 100].lower(), "I" * 150)]),  # lowercase bases: no votes
+        (g[:c], 2.0, [], [(g[c - 40:c + 10], "I" * 50)] * 3000 + [(g[c - 40:c + 110], "I" * 150)]),  # 3001 reads
+        (g[:c], 20.0, [], [(g[c - 200:c + 300], "I" * 500)]),         # 500-base read
+    ]
+    # quality bytes below the offset (negative quality) and mixed qualities
+    q = "".join(chr(33 + int(x)) for x in rng.integers(0, 41, 150))
+    ctgs.append((g[:c], 6.0, [], [(g[c - 50:c + 100], q), (g[c - 50:c + 100], "!" * 150)]))
+    b = ld.from_contigs(ctgs)
+    for kw in (dict(kmer_len=21, walk_len_limit=300), dict(kmer_len=33, walk_len_limit=0),
+               dict(kmer_len=33, walk_len_limit=1), dict(kmer_len=25, walk_len_limit=50, max_reads=2)):
+        check(assembler, b, **kw)
+
+
+def test_restaged_and_rerun(assembler):
+    """A staged batch runs again with other parameters; a new batch replaces it."""
+    b1 = ld.generate(600, seed=31)
+    b2 = ld.generate(300, seed=32, ctg_len=(60, 400))
+    assembler.stage(b1)
+    for k in (33, 21):
+        assembler.run(locassm.params(kmer_len=k, walk_len_limit=300))
+        assembler.wait()
+        got = assembler.results()
+        ll, le, rl, re = la_extend_batch(la_params(kmer_len=k, walk_len_limit=300), b1)
+        assert np.array_equal(got.left_len, ll) and np.array_equal(got.right_len, rl)
+    check(assembler, b2, kmer_len=33, walk_len_limit=300)
+    st = assembler.stats()
+    assert st["contigs"] == 300 and st["walks"] > 0 and st["mers"] > 0
+
+
+def test_errors(assembler):
+    b = ld.from_contigs([("ACGTXACGT", 1.0, [], [])])
+    with pytest.raises(locassm.LocAssmError):
+        assembler.stage(b)
+    good = ld.from_contigs([("ACGT" * 10, 1.0, [], [])])
+    assembler.stage(good)
+    with pytest.raises(locassm.LocAssmError):
+        assembler.run(locassm.params(shift=0))
