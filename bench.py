@@ -115,6 +115,67 @@ def cigar_leg(al, batch, res, scoring, steps, check):
     return out
 
 
+def locassm_leg(nctgs, steps, warmup, check):
+    """Local assembly (contig end extension) on a generated batch resident in
+    HBM: one step = both sides of every contig walked at every mer length
+    the schedule visits (what local_assem_driver does for a slice)."""
+    from mhm2_staging_amd import locassm
+    from mhm2_staging_amd import locassm_data as ld
+
+    t = time.perf_counter()
+    b = ld.generate(nctgs, seed=7)
+    log(f"[bench] local assembly batch: {b.nctgs} contigs, {b.nreads} reads ({time.perf_counter() - t:.1f} s)")
+    kw = dict(kmer_len=33, walk_len_limit=360)
+    la = locassm.LocalAssembler(0)
+    la.stage(b)
+    p = locassm.params(**kw)
+    for _ in range(warmup):
+        la.run(p)
+        la.wait()
+    ms = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        la.run(p)
+        la.wait()
+        ms.append(la.stats()["job_ms"])
+    wall = (time.perf_counter() - t) / steps
+    st = la.stats()
+    got = la.results()
+    la.close()
+    dev_ms = float(np.mean(ms))
+    out = {"workload": f"{b.nctgs} contigs U[200,2000] bp, both ends; per side 0 (10%), 1-9 (55%) or 10-80 (35%) "
+                       "150 bp reads, 1% substitutions, 0.1% N; 10% of sides with a two-haplotype fork, 5% with a "
+                       "repeat; kmer_len 33, walk_len_limit 360, depth U[2,50)",
+           "contigs": b.nctgs, "reads": b.nreads, "sides_on_device": st["sides"], "launches": st["launches"],
+           "mers_per_step": st["mers"], "walks_per_step": st["walks"], "walk_steps_per_step": st["walk_steps"],
+           "device_ms": round(dev_ms, 3), "ms_per_step": round(wall * 1e3, 3),
+           "value": round(b.nctgs / (dev_ms / 1e3), 1), "unit": "contigs/s (both ends)",
+           "mers_per_s": round(st["mers"] / (dev_ms / 1e3), 1),
+           "scratch_bytes": st["scratch_bytes"],
+           "phase_share": dict(zip(("clear", "count", "decide", "walk"),
+                                   (round(x / max(1, sum(st["phase_cycles"])), 3) for x in st["phase_cycles"]))),
+           "extended_sides": int((got.left_len > 0).sum() + (got.right_len > 0).sum()),
+           "bases_added": int(got.left_len.sum() + got.right_len.sum())}
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        cores = min(os.cpu_count() or 1, 16)
+        t = time.perf_counter()
+        ll, le, rl, re, tm = oracle_lib.la_extend_batch(oracle_lib.la_params(**kw), b, nthreads=cores, with_terms=True)
+        dt = time.perf_counter() - t
+        bad = int(((ll != got.left_len) | (rl != got.right_len) | (tm != got.terms).any(axis=(1, 2))).sum())
+        for c in np.nonzero((ll == got.left_len) & (rl == got.right_len))[0]:
+            if (not np.array_equal(le[c, :ll[c]], got.left[c, :ll[c]])
+                    or not np.array_equal(re[c, :rl[c]], got.right[c, :rl[c]])):
+                bad += 1
+        out["cpu_baseline"] = {"value": round(b.nctgs / dt, 1), "unit": "contigs/s (both ends)", "cores": cores,
+                               "kind": "port", "sample": f"all {b.nctgs} contigs of the same batch, {dt:.1f} s "
+                               "(oracle/locassm_oracle.c, the reference kernel's semantics, one contig per task)"}
+        out["bit_exact_check"] = {"contigs": b.nctgs, "mismatches": bad, "against": "oracle",
+                                  "fields": "both extensions (bytes and lengths), walk terminations"}
+    return out
+
+
 def serving_bench(qps_list, duration, slots=4, seed=5):
     """Config 5: 64k resident contigs, Poisson requests of log-uniform 1e3..1e5 pairs."""
     import mhm2_staging_amd as m
@@ -175,6 +236,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
     ap.add_argument("--no-cigar", action="store_true", help="skip the CIGAR-pass measurement")
+    ap.add_argument("--no-locassm", action="store_true", help="skip the local assembly measurement")
+    ap.add_argument("--locassm-only", action="store_true", help="only the local assembly measurement")
+    ap.add_argument("--locassm-contigs", type=int, default=50000)
     ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
     ap.add_argument("--tuning", type=str, default="", metavar="B,G,S",
                     help="fixed scheduling parameters (blocks_per_cu, g64_min_rows, bin_shift) instead of autotuning")
@@ -198,6 +262,12 @@ def main():
               "with no exchange, so multi-GPU throughput is N times the one-GPU value by construction.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.locassm_only:
+        la = locassm_leg(args.locassm_contigs, args.steps, args.warmup, not args.no_cpu)
+        log(f"[bench] local assembly: {la}")
+        print(json.dumps({"local_assembly": la}), flush=True)
+        return
 
     import mhm2_staging_amd as m
     from mhm2_staging_amd import _native
@@ -352,6 +422,9 @@ def main():
                                   "against": cb["kind"]}
         log(f"[bench] cpu baseline {cb}")
     al.close()
+    if not args.no_locassm:
+        out["local_assembly"] = locassm_leg(args.locassm_contigs, args.steps, args.warmup, not args.no_cpu)
+        log(f"[bench] local assembly: {out['local_assembly']}")
     if not args.no_serving:
         out["serving"] = serving_bench([float(x) for x in args.serving_qps.split(",")], args.serving_seconds)
     print(json.dumps(out), flush=True)

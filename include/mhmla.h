@@ -58,6 +58,8 @@ typedef struct {
   int64_t walk_steps;   /* walk steps taken (all sides, all mer lengths) */
   int64_t walks;        /* walks (one per side and mer length tried) */
   int64_t scratch_bytes;/* device scratch used by the largest launch */
+  int64_t phase_cycles[4]; /* shader clock cycles per phase, summed over sides: table clear,
+                              mer counting, extension and successor choice, walk */
 } mhmla_stats;
 
 /* Defaults: the reference's constants, kmer_len 33, walk_len_limit 300,

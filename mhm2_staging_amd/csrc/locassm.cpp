@@ -46,7 +46,8 @@ struct Buf {
   }
 };
 
-constexpr int64_t kDefaultBudget = int64_t(2) << 30;  // scratch bytes per launch
+constexpr int64_t kDefaultBudget = int64_t(16) << 30;  // scratch bytes per launch (at most half the free memory)
+constexpr int64_t kPad = 64;                           // bytes readable past every staged buffer
 constexpr int64_t kMaxCap = int64_t(1) << 27;         // slots of one table
 
 }  // namespace
@@ -256,7 +257,7 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
     rend = std::max(rend, read_off[r] + read_len[r]);
   }
   LA_HIP(hipSetDevice(c->device), "hipSetDevice");
-  LA_HIP(c->d_ctg.ensure((size_t)total + 1), "contig buffer");
+  LA_HIP(c->d_ctg.ensure((size_t)(total + kPad)), "contig buffer");
   LA_HIP(c->d_ctg_off.ensure((size_t)nctgs + 1), "contig offsets");
   LA_HIP(c->d_ctg_len.ensure((size_t)nctgs + 1), "contig lengths");
   LA_HIP(c->d_depth.ensure((size_t)nctgs + 1), "depths");
@@ -265,8 +266,10 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
   LA_HIP(c->d_read_idx.ensure((size_t)nidx + 1), "read numbers");
   LA_HIP(c->d_read_off.ensure((size_t)nreads + 1), "read offsets");
   LA_HIP(c->d_read_len.ensure((size_t)nreads + 1), "read lengths");
-  LA_HIP(c->d_seq.ensure((size_t)rend + 1), "read bases");
-  LA_HIP(c->d_qual.ensure((size_t)rend + 1), "read qualities");
+  if (rend > int64_t(0xFFFFFFFFu) - kPad)
+    return fail(c, MHMLA_E_ARG, "read buffers of %lld bytes exceed 4 GiB", (long long)rend);
+  LA_HIP(c->d_seq.ensure((size_t)(rend + kPad)), "read bases");
+  LA_HIP(c->d_qual.ensure((size_t)(rend + kPad)), "read qualities");
   hipStream_t st = c->stream;
   LA_HIP(hipMemcpyAsync(c->d_ctg.ptr, cbuf.data(), (size_t)total + 1, hipMemcpyHostToDevice, st), "copy contigs");
   if (nctgs) {
@@ -328,12 +331,14 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
         c->host_terms[(size_t)job * 3] = iters;  // every walk a dead end
         continue;
       }
-      int64_t cap = 64;
-      while (cap < occ + occ / 2 + 1) cap <<= 1;
+      // at most 80% full; 16-bit vote counters while no count can reach 65536
+      const int64_t cap = (occ + occ / 4 + 64 + 63) & ~int64_t(63);
+      const bool wide = occ >= 65535;
       if (cap > kMaxCap)
         return fail(c, MHMLA_E_ARG, "contig %lld side %d: %lld mers exceed one table", (long long)ci, side,
                     (long long)occ);
-      plan.push_back({(int32_t)job, (int32_t)cap, job_layout(cap, nr, W).bytes});
+      plan.push_back({(int32_t)job, (int32_t)cap | (wide ? kWideCounts : 0),
+                      job_layout(cap, wide, nr, W, p->max_kmer_len).bytes});
       c->on_device[(size_t)job] = 1;
     }
   }
@@ -342,7 +347,11 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
   std::vector<int32_t> jobs((size_t)nd), caps((size_t)nd);
   std::vector<int64_t> soff((size_t)nd);
   std::vector<std::pair<int64_t, int64_t>> launches;  // [first, end) of plan
-  const int64_t budget = std::max<int64_t>(c->budget, nd ? plan[0].bytes : 0);
+  size_t free_b = 0, total_b = 0;
+  LA_HIP(hipSetDevice(c->device), "hipSetDevice");
+  LA_HIP(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  const int64_t budget = std::max<int64_t>(std::min<int64_t>(c->budget, (int64_t)(free_b / 2) + c->d_scratch.cap),
+                                           nd ? plan[0].bytes : 0);
   int64_t used = 0, start = 0, biggest = 0;
   for (int64_t i = 0; i < nd; i++) {
     if (used + plan[(size_t)i].bytes > budget) {
@@ -369,14 +378,14 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
   LA_HIP(c->d_out_len.ensure((size_t)njobs + 1), "walk lengths");
   LA_HIP(c->d_out_terms.ensure((size_t)njobs * 3 + 1), "walk terminations");
   LA_HIP(c->d_out_ext.ensure((size_t)njobs * (size_t)W + 1), "walks");
-  LA_HIP(c->d_counters.ensure(4), "counters");
+  LA_HIP(c->d_counters.ensure(8), "counters");
   hipStream_t st = c->stream;
   if (nd) {
     LA_HIP(hipMemcpyAsync(c->d_jobs.ptr, jobs.data(), 4 * (size_t)nd, hipMemcpyHostToDevice, st), "copy jobs");
     LA_HIP(hipMemcpyAsync(c->d_cap.ptr, caps.data(), 4 * (size_t)nd, hipMemcpyHostToDevice, st), "copy jobs");
     LA_HIP(hipMemcpyAsync(c->d_soff.ptr, soff.data(), 8 * (size_t)nd, hipMemcpyHostToDevice, st), "copy jobs");
   }
-  LA_HIP(hipMemsetAsync(c->d_counters.ptr, 0, 4 * sizeof(unsigned long long), st), "clear counters");
+  LA_HIP(hipMemsetAsync(c->d_counters.ptr, 0, 8 * sizeof(unsigned long long), st), "clear counters");
   LA_HIP(hipEventRecord(c->ev0, st), "event");
   WalkView v{};
   v.ctg_seq = c->d_ctg.ptr;
@@ -427,13 +436,14 @@ int mhmla_wait(mhmla_ctx* c) {
   LA_HIP(hipStreamSynchronize(c->stream), "local assembly walks");
   float ms = 0.f;
   LA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event timing");
-  unsigned long long cnt[4];
+  unsigned long long cnt[8];
   LA_HIP(hipMemcpy(cnt, c->d_counters.ptr, sizeof cnt, hipMemcpyDeviceToHost), "copy counters");
   if (cnt[3]) return fail(c, MHMLA_E_HIP, "%llu mer insertions found their table full", cnt[3]);
   c->stats.job_ms = ms;
   c->stats.mers = (int64_t)cnt[0];
   c->stats.walk_steps = (int64_t)cnt[1];
   c->stats.walks = (int64_t)cnt[2];
+  for (int i = 0; i < 4; i++) c->stats.phase_cycles[i] = (int64_t)cnt[4 + i];
   for (size_t j = 0; j < c->on_device.size(); j++)
     if (!c->on_device[j]) c->stats.walks += c->host_terms[3 * j];
   return MHMLA_OK;

@@ -44,13 +44,28 @@ MHMLA_HD int rating(uint32_t n, uint32_t hi, double min_viable, double min_expec
   return 7;
 }
 
-// true when base a ranks strictly before base b: (rating, hi, low) larger,
-// then the lower base index (an arbitrary strict order among full ties)
-MHMLA_HD bool ranks_before(int ra, uint32_t ha, uint32_t la, int a, int rb, uint32_t hb, uint32_t lb, int b) {
-  if (ra != rb) return ra > rb;
-  if (ha != hb) return ha > hb;
-  if (la != lb) return la > lb;
-  return a < b;
+// one base's ranking key
+struct BaseKey {
+  int rating;
+  uint32_t hi, low;
+  int base;
+};
+
+// true when a ranks strictly before b: (rating, hi, low) larger, then the
+// lower base index (an arbitrary strict order among full ties)
+MHMLA_HD bool ranks_before(const BaseKey& a, const BaseKey& b) {
+  if (a.rating != b.rating) return a.rating > b.rating;
+  if (a.hi != b.hi) return a.hi > b.hi;
+  if (a.low != b.low) return a.low > b.low;
+  return a.base < b.base;
+}
+
+MHMLA_HD void order(BaseKey& x, BaseKey& y) {
+  if (ranks_before(y, x)) {
+    const BaseKey t = x;
+    x = y;
+    y = t;
+  }
 }
 
 // set_ext (kernel.hpp:112-161): the extension code for votes low[b] (quality
@@ -60,25 +75,24 @@ MHMLA_HD int choose_ext(const uint32_t low[4], const uint32_t hi[4], int depth, 
   if (min_viable < 2.0) min_viable = 2.0;
   double min_expected = r.min_expected_depth * depth;
   if (min_expected < 2.0) min_expected = 2.0;
-  int rt[4];
-  for (int b = 0; b < 4; b++) rt[b] = rating(low[b], hi[b], min_viable, min_expected);
-  // the first three of the ranking
-  int o[4] = {0, 1, 2, 3};
-  for (int i = 0; i < 3; i++)
-    for (int j = 3; j > i; j--)
-      if (ranks_before(rt[o[j]], hi[o[j]], low[o[j]], o[j], rt[o[j - 1]], hi[o[j - 1]], low[o[j - 1]], o[j - 1])) {
-        const int t = o[j];
-        o[j] = o[j - 1];
-        o[j - 1] = t;
-      }
-  const int top = rt[o[0]], runner_up = rt[o[1]];
+  BaseKey e0{rating(low[0], hi[0], min_viable, min_expected), hi[0], low[0], 0};
+  BaseKey e1{rating(low[1], hi[1], min_viable, min_expected), hi[1], low[1], 1};
+  BaseKey e2{rating(low[2], hi[2], min_viable, min_expected), hi[2], low[2], 2};
+  BaseKey e3{rating(low[3], hi[3], min_viable, min_expected), hi[3], low[3], 3};
+  // sorting network for four: e0 ranks first, e3 last
+  order(e0, e1);
+  order(e2, e3);
+  order(e0, e2);
+  order(e1, e3);
+  order(e1, e2);
+  const int top = e0.rating, runner_up = e1.rating;
   if (top <= r.rating_thres) return kExtX;
-  if (top <= 3) return runner_up == 0 ? o[0] : kExtX;
-  if (top < 6) return runner_up < 3 ? o[0] : kExtX;
-  if (top == 6) return runner_up < 4 ? o[0] : kExtX;
-  if (runner_up < 7) return o[0];
-  if (rt[o[2]] == 7 || low[o[0]] == low[o[1]]) return kExtF;
-  return low[o[0]] > low[o[1]] ? o[0] : o[1];
+  if (top <= 3) return runner_up == 0 ? e0.base : kExtX;
+  if (top < 6) return runner_up < 3 ? e0.base : kExtX;
+  if (top == 6) return runner_up < 4 ? e0.base : kExtX;
+  if (runner_up < 7) return e0.base;
+  if (e2.rating == 7 || e0.low == e1.low) return kExtF;
+  return e0.low > e1.low ? e0.base : e1.base;
 }
 
 MHMLA_HD char ext_char(int code) { return "ACGTFX"[code]; }

@@ -40,15 +40,51 @@ __device__ __forceinline__ uint32_t ld32(uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ bool same_bytes(const uint8_t* a, const uint8_t* b, int n) {
-  for (int i = 0; i < n; i++)
-    if (a[i] != b[i]) return false;
+// Bytes a[0..n) and b[0..n) equal?  Aligned dword loads funnel-shifted with
+// v_alignbyte, C dwords per round trip, no branch inside a round; reads up to
+// 7 bytes past either window (every buffer read here is padded).
+template <int C>
+__device__ __forceinline__ bool same_window(const uint8_t* a, const uint8_t* b, int n) {
+  const uint32_t* pa = (const uint32_t*)((uintptr_t)a & ~(uintptr_t)3);
+  const uint32_t* pb = (const uint32_t*)((uintptr_t)b & ~(uintptr_t)3);
+  const uint32_t sa = (uint32_t)((uintptr_t)a & 3), sb = (uint32_t)((uintptr_t)b & 3);
+  const int nw = (n + 3) >> 2;
+  for (int g = 0; g < nw; g += C) {
+    uint32_t A[C + 1], B[C + 1];
+#pragma unroll
+    for (int j = 0; j <= C; j++) {
+      const bool in = g + j <= nw;
+      A[j] = in ? pa[g + j] : 0u;
+      B[j] = in ? pb[g + j] : 0u;
+    }
+    uint32_t diff = 0;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      const int rem = n - 4 * (g + j);
+      if (rem > 0) {
+        const uint32_t m = rem >= 4 ? 0xFFFFFFFFu : (1u << (8 * rem)) - 1u;
+        diff |= (__builtin_amdgcn_alignbyte(A[j + 1], A[j], sa) ^ __builtin_amdgcn_alignbyte(B[j + 1], B[j], sb)) & m;
+      }
+    }
+    if (diff) return false;
+  }
   return true;
 }
 
-__device__ __forceinline__ uint64_t hash_bytes(const uint8_t* s, int k) {
+// polynomial hash of s[0..k) (bytes in order), from aligned dword loads
+__device__ __forceinline__ uint64_t hash_window(const uint8_t* s, int k) {
+  const uint32_t* p = (const uint32_t*)((uintptr_t)s & ~(uintptr_t)3);
+  const uint32_t sh = (uint32_t)((uintptr_t)s & 3);
+  const int nw = (k + 3) >> 2;
   uint64_t h = 0;
-  for (int i = 0; i < k; i++) h = h * kHashBase + s[i];
+  uint32_t prev = p[0];
+  for (int j = 0; j < nw; j++) {
+    const uint32_t nxt = p[j + 1];
+    const uint32_t w = __builtin_amdgcn_alignbyte(nxt, prev, sh);
+    prev = nxt;
+    const int m = min(4, k - 4 * j);
+    for (int b = 0; b < m; b++) h = h * kHashBase + ((w >> (8 * b)) & 0xFFu);
+  }
   return h;
 }
 
@@ -56,30 +92,30 @@ struct Side {
   const uint8_t* seq;   // all read bases
   const uint8_t* qual;  // all read quality bytes
   uint64_t* key;
-  uint64_t* hsh;
-  uint32_t* cnt;
+  uint32_t* cnt;        // per slot: 4 words of 16-bit vote pairs, or 8 words (wide)
   uint32_t* info;
   int64_t* roff;        // offsets of the side's reads
   int32_t* rlen;
   uint32_t* cum;        // occurrences before read i, at the current mer length
   uint8_t* walk[2];
+  uint8_t* smer;        // the side's last mer (reverse-complemented on the left)
   int cap;
   int nr;
+  bool wide;            // 32-bit vote counters (sides with 65535 mers or more)
 
-  __device__ const uint8_t* occ(uint64_t w) const {
-    return seq + roff[(w >> 16) & 0xFFFF] + (w & 0xFFFF);
-  }
+  __device__ const uint8_t* occ(uint64_t w) const { return seq + (uint32_t)w; }
 };
 
 // Find the slot whose mer equals the bytes 'same' accepts; insert 'mine'
-// (fingerprint | read | start) there when absent and insert is set.
+// (fingerprint | byte offset of the occurrence) there when absent and insert
+// is set.
 // Returns the slot, -1 when absent and not inserted, -2 if the table is full.
 template <class Same>
 __device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool insert, bool* won, Same same) {
   const uint64_t m = mix64(h);
   const uint32_t fp = (uint32_t)(m >> 32);
   const uint64_t mine = ((uint64_t)fp << 32) | mine_low;
-  int s = (int)(m & (uint64_t)(S.cap - 1));
+  int s = (int)(((uint64_t)(uint32_t)m * (uint64_t)S.cap) >> 32);
   for (int probes = 0; probes < S.cap; probes++) {
     uint64_t w = ld64(&S.key[s]);
     if (w == kEmpty) {
@@ -93,7 +129,7 @@ __device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool inse
       w = prev;
     }
     if ((uint32_t)(w >> 32) == fp && same(S.occ(w))) return s;
-    s = (s + 1) & (S.cap - 1);
+    s = s + 1 == S.cap ? 0 : s + 1;
   }
   return -2;
 }
@@ -119,13 +155,13 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
   Side S;
   S.seq = v.read_seq;
   S.qual = v.read_qual;
-  S.cap = v.cap[slot];
+  S.cap = v.cap[slot] & kCapMask;
+  S.wide = (v.cap[slot] & kWideCounts) != 0;
   S.nr = nr;
   {
     uint8_t* base = v.scratch + v.scratch_off[slot];
-    const JobLayout lay = job_layout(S.cap, nr, W);
+    const JobLayout lay = job_layout(S.cap, S.wide, nr, W, v.max_kmer_len);
     S.key = (uint64_t*)(base + lay.key);
-    S.hsh = (uint64_t*)(base + lay.hsh);
     S.cnt = (uint32_t*)(base + lay.cnt);
     S.info = (uint32_t*)(base + lay.info);
     S.roff = (int64_t*)(base + lay.roff);
@@ -133,6 +169,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     S.cum = (uint32_t*)(base + lay.cum);
     S.walk[0] = base + lay.walk0;
     S.walk[1] = base + lay.walk1;
+    S.smer = base + lay.smer;
   }
   for (int i = lane; i < nr; i += kLanes) {
     const int64_t r = v.read_idx[f0 + i];
@@ -147,14 +184,16 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
   int longest = 0, best = 0, cur = 0, shift = 0;
   int term_x = 0, term_f = 0, term_r = 0;
   unsigned long long mers = 0, steps = 0, walks = 0;
+  unsigned long long cyc[4] = {0, 0, 0, 0};  // clear, count, decide, walk (shader clock, lane 0)
+  long long t0 = clock64();
 
   for (int k = v.kmer_len; k >= v.min_kmer_len && k <= max_len; k += shift) {
     // ---- clear the table and count occurrences per read ----
-    for (int s = lane; s < S.cap; s += kLanes) {
-      S.key[s] = kEmpty;
-      uint4* cp = (uint4*)(S.cnt + 8 * (int64_t)s);
-      cp[0] = make_uint4(0, 0, 0, 0);
-      cp[1] = make_uint4(0, 0, 0, 0);
+    for (int s = lane; s < S.cap; s += kLanes) S.key[s] = kEmpty;
+    {
+      uint4* cp = (uint4*)S.cnt;
+      const int nq = S.wide ? 2 * S.cap : S.cap;
+      for (int s = lane; s < nq; s += kLanes) cp[s] = make_uint4(0, 0, 0, 0);
     }
     uint32_t carry = 0;
     if (lane == 0) S.cum[0] = 0;
@@ -171,6 +210,9 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     const uint32_t N = carry;
     __threadfence();
     __syncthreads();
+    long long t1 = clock64();
+    cyc[0] += (unsigned long long)(t1 - t0);
+    t0 = t1;
 
     // ---- count: lane takes occurrences [q0, q1) of the side's flattened list ----
     const uint64_t Bk1 = pow_base(k - 1);
@@ -186,21 +228,26 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
         int i = lo, pos = (int)(q0 - S.cum[i]);
         const uint8_t* rd = S.seq + S.roff[i];
         const uint8_t* rq = S.qual + S.roff[i];
-        uint64_t h = hash_bytes(rd + pos, k);
+        uint64_t h = hash_window(rd + pos, k);
         for (uint32_t q = q0;;) {
           bool won = false;
           const uint8_t* mine = rd + pos;
-          const int s = find_slot(S, h, ((uint64_t)i << 16) | (uint64_t)pos, true, &won,
-                                  [&](const uint8_t* o) { return same_bytes(o, mine, k); });
+          const int s = find_slot(S, h, (uint64_t)(uint32_t)(mine - S.seq), true, &won,
+                                  [&](const uint8_t* o) { return same_window<8>(o, mine, k); });
           if (s == -2) atomicAdd(&v.counters[3], 1ull);
           if (s >= 0) {
-            if (won) S.hsh[s] = h;
             const uint8_t e = rd[pos + k];
             const int b = vote_base(e);
             if (b >= 0) {
               const int qd = (int)(int8_t)rq[pos + k] - v.qual_offset;
-              if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)s + b], 1u);
-              if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)s + 4 + b], 1u);
+              if (S.wide) {
+                if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)s + b], 1u);
+                if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)s + 4 + b], 1u);
+              } else {
+                const uint32_t one = 1u << (16 * (b & 1));
+                if (qd >= v.min_qual) atomicAdd(&S.cnt[4 * (int64_t)s + (b >> 1)], one);
+                if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[4 * (int64_t)s + 2 + (b >> 1)], one);
+              }
             }
           }
           if (++q >= q1) break;
@@ -209,7 +256,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
             pos = 0;
             rd = S.seq + S.roff[i];
             rq = S.qual + S.roff[i];
-            h = hash_bytes(rd, k);
+            h = hash_window(rd, k);
           } else {
             h = (h - (uint64_t)rd[pos - 1] * Bk1) * kHashBase + rd[pos - 1 + k];
           }
@@ -219,25 +266,41 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     mers += N;
     __threadfence();
     __syncthreads();
+    t1 = clock64();
+    cyc[1] += (unsigned long long)(t1 - t0);
+    t0 = t1;
 
     // ---- decide: extension and successor of every occupied slot ----
     for (int s = lane; s < S.cap; s += kLanes) {
       const uint64_t w = ld64(&S.key[s]);
       if (w == kEmpty) continue;
       uint32_t low[4], hiq[4];
-      for (int b = 0; b < 4; b++) {
-        low[b] = ld32(&S.cnt[8 * (int64_t)s + b]);
-        hiq[b] = ld32(&S.cnt[8 * (int64_t)s + 4 + b]);
+      if (S.wide) {
+        for (int b = 0; b < 4; b++) {
+          low[b] = ld32(&S.cnt[8 * (int64_t)s + b]);
+          hiq[b] = ld32(&S.cnt[8 * (int64_t)s + 4 + b]);
+        }
+      } else {
+        const uint32_t w0 = ld32(&S.cnt[4 * (int64_t)s]), w1 = ld32(&S.cnt[4 * (int64_t)s + 1]);
+        const uint32_t w2 = ld32(&S.cnt[4 * (int64_t)s + 2]), w3 = ld32(&S.cnt[4 * (int64_t)s + 3]);
+        low[0] = w0 & 0xFFFFu;
+        low[1] = w0 >> 16;
+        low[2] = w1 & 0xFFFFu;
+        low[3] = w1 >> 16;
+        hiq[0] = w2 & 0xFFFFu;
+        hiq[1] = w2 >> 16;
+        hiq[2] = w3 & 0xFFFFu;
+        hiq[3] = w3 >> 16;
       }
       const int ext = choose_ext(low, hiq, depth, v.rules);
       uint32_t succ = kNoSlot;
       if (ext < 4) {
         const uint8_t* rep = S.occ(w);
         const uint8_t ec = (uint8_t)ext_char(ext);
-        const uint64_t hs = (ld64(&S.hsh[s]) - (uint64_t)rep[0] * Bk1) * kHashBase + ec;
+        const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
         bool won = false;
         const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
-          return same_bytes(o, rep + 1, k - 1) && o[k - 1] == ec;
+          return same_window<8>(o, rep + 1, k - 1) && o[k - 1] == ec;
         });
         if (t >= 0) succ = (uint32_t)t;
       }
@@ -245,18 +308,19 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     }
     __threadfence();
     __syncthreads();
+    t1 = clock64();
+    cyc[2] += (unsigned long long)(t1 - t0);
+    t0 = t1;
 
     // ---- walk from the contig's last mer ----
+    for (int j = lane; j < k; j += kLanes) S.smer[j] = side_byte(ctg, L, right, L - k + j);
+    __threadfence();
+    __syncthreads();
     if (lane == 0) {
-      uint64_t h0 = 0;
-      for (int j = 0; j < k; j++) h0 = h0 * kHashBase + side_byte(ctg, L, right, L - k + j);
+      const uint64_t h0 = hash_window(S.smer, k);
       bool won = false;
-      int e = find_slot(S, h0, 0, false, &won, [&](const uint8_t* o) {
-        for (int j = 0; j < k; j++)
-          if (o[j] != side_byte(ctg, L, right, L - k + j)) return false;
-        return true;
-      });
-      uint8_t* wb = S.walk[cur];
+      int e = find_slot(S, h0, 0, false, &won, [&](const uint8_t* o) { return same_window<8>(o, S.smer, k); });
+      uint8_t* wb = cur ? S.walk[1] : S.walk[0];
       int t = 0, res = 0;  // 0 X, 1 F, 2 R
       while (t < W) {
         if (e < 0) {
@@ -283,6 +347,9 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     __threadfence();
     __syncthreads();
     const int t = sh_t, res = sh_res;
+    t1 = clock64();
+    cyc[3] += (unsigned long long)(t1 - t0);
+    t0 = t1;
     steps += (unsigned long long)t;
     walks++;
     term_x += res == 0;
@@ -306,7 +373,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
 
   // ---- the longest walk, reverse-complemented on the left side ----
   uint8_t* out = v.out_ext + (int64_t)job * W;
-  const uint8_t* src = S.walk[best];
+  const uint8_t* src = best ? S.walk[1] : S.walk[0];
   for (int j = lane; j < longest; j += kLanes) out[j] = right ? src[j] : complement(src[longest - 1 - j]);
   if (lane == 0) {
     v.out_len[job] = longest;
@@ -316,6 +383,10 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     atomicAdd(&v.counters[0], mers);
     atomicAdd(&v.counters[1], steps);
     atomicAdd(&v.counters[2], walks);
+    atomicAdd(&v.counters[4], cyc[0]);
+    atomicAdd(&v.counters[5], cyc[1]);
+    atomicAdd(&v.counters[6], cyc[2]);
+    atomicAdd(&v.counters[7], cyc[3]);
   }
 }
 

@@ -8,13 +8,16 @@
 
 namespace mhmla {
 
-// a table slot's key word: fingerprint (32 bits) | read number within the
-// side (16 bits) | start of the mer in that read (16 bits); all ones = empty
+// a table slot's key word: fingerprint (32 bits) | byte offset of one
+// occurrence of the mer in the read buffer (32 bits); all ones = empty
 constexpr uint64_t kEmpty = ~0ull;
 // a slot's info word: extension code (3 bits) | successor slot << 3; the
 // walk marks a visited slot with the top bit
 constexpr uint32_t kNoSlot = 0x0FFFFFFFu;
 constexpr uint32_t kVisited = 0x80000000u;
+// a job's table word: slots (any count up to 2^27) | wide-counter flag
+constexpr int32_t kCapMask = 0x0FFFFFFF;
+constexpr int32_t kWideCounts = 0x40000000;
 
 struct WalkView {
   // staged batch
@@ -32,13 +35,14 @@ struct WalkView {
   // this launch: jobs (contig * 2 + side, side 1 = right) and their scratch
   const int32_t* jobs;
   const int64_t* scratch_off;  // bytes into scratch, per launch slot
-  const int32_t* cap;          // table slots (power of two), per launch slot
+  const int32_t* cap;          // table slots | kWideCounts, per launch slot
   uint8_t* scratch;
   // outputs, indexed by job
   int32_t* out_len;
   uint8_t* out_ext;            // walk_len_limit bytes per job
   int32_t* out_terms;          // 3 per job: walks ending in X, F, R
-  unsigned long long* counters;  // [0] mers counted, [1] walk steps, [2] walks, [3] full tables
+  unsigned long long* counters;  // [0] mers counted, [1] walk steps, [2] walks, [3] full tables,
+                                 // [4..7] shader cycles in clear, count, decide, walk (summed over sides)
   // parameters
   int32_t kmer_len, min_kmer_len, max_kmer_len, shift, walk_len_limit;
   int32_t qual_offset, min_qual, min_hi_qual, max_reads;
@@ -48,23 +52,24 @@ struct WalkView {
 // scratch layout of one job (byte offsets from its base), shared by the host
 // planner and the kernel
 struct JobLayout {
-  int64_t key, hsh, cnt, info, roff, rlen, cum, walk0, walk1, bytes;
+  int64_t key, cnt, info, roff, rlen, cum, walk0, walk1, smer, bytes;
 };
 
 inline __host__ __device__ int64_t la_align(int64_t x) { return (x + 255) & ~int64_t(255); }
 
-inline __host__ __device__ JobLayout job_layout(int64_t cap, int64_t nreads, int64_t walk_len_limit) {
+inline __host__ __device__ JobLayout job_layout(int64_t cap, bool wide, int64_t nreads, int64_t walk_len_limit,
+                                                int64_t max_kmer_len) {
   JobLayout l;
   l.key = 0;
-  l.hsh = la_align(l.key + 8 * cap);
-  l.cnt = la_align(l.hsh + 8 * cap);
-  l.info = la_align(l.cnt + 32 * cap);
+  l.cnt = la_align(l.key + 8 * cap);
+  l.info = la_align(l.cnt + (wide ? 32 : 16) * cap);
   l.roff = la_align(l.info + 4 * cap);
   l.rlen = la_align(l.roff + 8 * nreads);
   l.cum = la_align(l.rlen + 4 * nreads);
   l.walk0 = la_align(l.cum + 4 * (nreads + 1));
   l.walk1 = la_align(l.walk0 + walk_len_limit);
-  l.bytes = la_align(l.walk1 + walk_len_limit);
+  l.smer = la_align(l.walk1 + walk_len_limit);
+  l.bytes = la_align(l.smer + max_kmer_len + 64);
   return l;
 }
 

@@ -115,8 +115,8 @@ def generate(nctgs, seed=0, ctg_len=(200, 2000), read_len=150, flank=520, sub_ra
     bases on each side.  Each side gets 0 reads (10% of sides), 1..9 reads
     (55%) or 10..80 reads (35%), of ``read_len`` bases starting uniformly
     between 140 bases inside the contig end and 350 bases past it, with
-    substitutions at ``sub_rate`` (quality 2..15), other bases at quality
-    25..40 except 3% at 2..19, and N at ``n_rate`` (quality 2).  Left reads
+    substitutions at ``sub_rate`` (quality 2..15), N at ``n_rate`` (quality
+    2), other bases at quality 25..40 except 3% at 2..19.  Left reads
     are drawn from the reverse complement of the source, the orientation in
     which the left side is walked.  ``fork_rate`` of the sides draw every
     second read from a second haplotype with one substitution in the flank,
@@ -171,7 +171,9 @@ def generate(nctgs, seed=0, ctg_len=(200, 2000), read_len=150, flank=520, sub_ra
 
     read_seq = np.empty(nr * read_len, dtype=np.uint8)
     read_qual = np.empty(nr * read_len, dtype=np.uint8)
-    chunk = 200000
+    code = np.zeros(256, dtype=np.int64)
+    code[alpha] = np.arange(4)
+    chunk = 50000
     ar = np.arange(read_len)
     for a in range(0, nr, chunk):
         b = min(nr, a + chunk)
@@ -182,19 +184,20 @@ def generate(nctgs, seed=0, ctg_len=(200, 2000), read_len=150, flank=520, sub_ra
         bases = src[src_pos]
         bases = np.where(right, bases, _COMP[bases])
         hit = alt[a:b, None] & (pos == snp_at[a:b, None])
-        bases = np.where(hit, alpha[(np.searchsorted(alpha, bases) + 1) % 4], bases)
-        q = rng.integers(25, 41, (b - a, read_len))
-        low = rng.random((b - a, read_len)) < 0.03
-        q = np.where(low, rng.integers(2, 20, (b - a, read_len)), q)
-        sub = rng.random((b - a, read_len)) < sub_rate
-        shift = rng.integers(1, 4, (b - a, read_len))
-        bases = np.where(sub, alpha[(np.searchsorted(alpha, bases) + shift) % 4], bases)
-        q = np.where(sub, rng.integers(2, 16, (b - a, read_len)), q)
-        nn = rng.random((b - a, read_len)) < n_rate
-        bases = np.where(nn, ord("N"), bases)
-        q = np.where(nn, 2, q)
+        bases[hit] = alpha[(code[bases[hit]] + 1) % 4]
+        q = rng.integers(25, 41, (b - a, read_len), dtype=np.uint8)
+        u = rng.random((b - a, read_len), dtype=np.float32)
+        sub = u < sub_rate
+        nn = (u >= sub_rate) & (u < sub_rate + n_rate)
+        low = (u >= sub_rate + n_rate) & (u < sub_rate + n_rate + 0.03)
+        ns = int(sub.sum())
+        bases[sub] = alpha[(code[bases[sub]] + rng.integers(1, 4, ns)) % 4]
+        q[sub] = rng.integers(2, 16, ns, dtype=np.uint8)
+        q[low] = rng.integers(2, 20, int(low.sum()), dtype=np.uint8)
+        bases[nn] = ord("N")
+        q[nn] = 2
         read_seq[a * read_len:b * read_len] = bases.reshape(-1)
-        read_qual[a * read_len:b * read_len] = (q + qual_offset).astype(np.uint8).reshape(-1)
+        read_qual[a * read_len:b * read_len] = (q + qual_offset).reshape(-1)
 
     ctg_seq = np.empty(int(clen.sum()) + 1, dtype=np.uint8)
     ctg_off = np.zeros(nctgs, dtype=np.int64)

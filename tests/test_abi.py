@@ -65,6 +65,18 @@ def test_exports_adept_sw_driver_symbols():
         assert sym in out, sym
 
 
+def test_exports_locassm_driver_symbols():
+    """The local assembly drop-in (include/gpu_loc_assem/driver.hpp) is in the library."""
+    out = subprocess.run(["nm", "-DC", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    for sym in ["locassm_driver::local_assem_driver(std::vector<CtgWithReads, std::allocator<CtgWithReads> >&, "
+                "unsigned int, unsigned int, unsigned int, unsigned int, int, int, locassm_driver::accum_data&, "
+                "int, int, int, int, int)",
+                "locassm_driver::get_gpu_per_node()",
+                "locassm_driver::get_device_mem(int, int)",
+                "locassm_driver::ctg_bucket::clear()"]:
+        assert sym in out, sym
+
+
 def test_python_binding_lists_all_exports():
     from mhm2_staging_amd import _native
     assert sorted(_native.EXPORTS) == declared_c_functions()

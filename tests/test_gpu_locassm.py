@@ -31,8 +31,6 @@ def assembler():
     a.close()
 
 
-ORACLE_FIELDS = ("left_len", "left", "right_len", "right", "terms")
-
 
 def check(assembler, b, **kw):
     print(f"[gpu] locassm nctgs={b.nctgs} nreads={b.nreads} params={kw}", flush=True)
@@ -153,3 +151,32 @@ def test_errors(assembler):
     assembler.stage(good)
     with pytest.raises(locassm.LocAssmError):
         assembler.run(locassm.params(shift=0))
+
+
+def driver_input(b, mer_len, walk, qual_offset=33):
+    lines = [f"{b.nctgs} {mer_len} {walk} {qual_offset}"]
+    for c in range(b.nctgs):
+        left, right = b.reads_of(c, "left"), b.reads_of(c, "right")
+        lines.append(f"{int(b.cid[c])} {float(b.depth[c])!r} {b.ctg(c) or '-'} {len(left)} {len(right)}")
+        lines += [f"{r or '-'} {q or '-'}" for r, q in left + right]
+    return "\n".join(lines) + "\n"
+
+
+def test_reference_driver_interface(assembler):
+    """locassm_driver::local_assem_driver, called through the reference's
+    header by a C++ program (tests/cpp/locassm_driver_test.cpp), extends every
+    contig exactly as the oracle's walks say (left + contig + right)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "locassm_driver_test")
+    assert os.path.exists(exe), "tests/cpp/locassm_driver_test not built (run __graft_entry__.build())"
+    b = ld.generate(400, seed=41)
+    for mer_len, walk in ((33, 300), (21, 150)):
+        out = subprocess.run([exe], input=driver_input(b, mer_len, walk), capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        ll, le, rl, re = la_extend_batch(la_params(kmer_len=mer_len, walk_len_limit=walk), b)
+        got = out.stdout.split("\n")
+        for c in range(b.nctgs):
+            want = bytes(le[c, :ll[c]]).decode() + b.ctg(c) + bytes(re[c, :rl[c]]).decode()
+            cid, seq = got[c].split(" ")
+            assert int(cid) == int(b.cid[c]) and seq == (want or "-"), f"contig {c}"
