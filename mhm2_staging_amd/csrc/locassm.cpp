@@ -61,10 +61,10 @@ struct mhmla_ctx {
   bool staged = false;
   int64_t nctgs = 0;
   std::vector<int32_t> ctg_len;
-  std::vector<int64_t> left_first, right_first, read_idx;
-  std::vector<int32_t> read_len;
+  std::vector<int64_t> rfirst;  // reads of side j: [rfirst[j], rfirst[j+1]) in side order
+  std::vector<int32_t> clen;    // their lengths
   Buf<uint8_t> d_ctg, d_seq, d_qual;
-  Buf<int64_t> d_ctg_off, d_left_first, d_right_first, d_read_idx, d_read_off;
+  Buf<int64_t> d_ctg_off, d_rfirst, d_read_off;
   Buf<int32_t> d_ctg_len, d_depth, d_read_len;
   // last run
   bool ran = false;
@@ -178,9 +178,7 @@ void mhmla_destroy(mhmla_ctx* c) {
   c->d_seq.release();
   c->d_qual.release();
   c->d_ctg_off.release();
-  c->d_left_first.release();
-  c->d_right_first.release();
-  c->d_read_idx.release();
+  c->d_rfirst.release();
   c->d_read_off.release();
   c->d_ctg_len.release();
   c->d_depth.release();
@@ -235,41 +233,62 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
     o += ctg_len[i];
     idepth[(size_t)i] = (int32_t)depth[i];  // set_ext(int) receives the double depth (kernel.cpp:377)
   }
-  // read lists
-  c->left_first.assign(left_first, left_first + nctgs + 1);
-  c->right_first.assign(right_first, right_first + nctgs + 1);
-  int64_t nidx = 0;
+  // read lists, validated, then the reads copied in side order (side j =
+  // contig * 2 + side, left 0, right 1) so each side's bytes are contiguous
   for (int64_t i = 0; i < nctgs; i++) {
     if (left_first[i + 1] < left_first[i] || right_first[i + 1] < right_first[i] || left_first[i] < 0 ||
         right_first[i] < 0)
       return fail(c, MHMLA_E_ARG, "contig %lld: read ranges decrease", (long long)i);
   }
-  if (nctgs > 0) nidx = std::max(left_first[nctgs], right_first[nctgs]);
-  c->read_idx.assign(read_idx, read_idx + nidx);
+  const int64_t nidx = nctgs > 0 ? std::max(left_first[nctgs], right_first[nctgs]) : 0;
   for (int64_t j = 0; j < nidx; j++)
     if (read_idx[j] < 0 || read_idx[j] >= nreads)
       return fail(c, MHMLA_E_ARG, "read_idx[%lld] = %lld is not a read number", (long long)j, (long long)read_idx[j]);
-  c->read_len.assign(read_len, read_len + nreads);
-  int64_t rend = 0;
-  for (int64_t r = 0; r < nreads; r++) {
+  for (int64_t r = 0; r < nreads; r++)
     if (read_len[r] < 0 || read_len[r] > 65535 || read_off[r] < 0)
       return fail(c, MHMLA_E_ARG, "read %lld: length %d outside 0..65535 or negative offset", (long long)r, read_len[r]);
-    rend = std::max(rend, read_off[r] + read_len[r]);
+  c->rfirst.assign((size_t)(2 * nctgs + 1), 0);
+  c->clen.clear();
+  std::vector<int64_t> croff;
+  int64_t cbytes = 0;
+  for (int64_t i = 0; i < nctgs; i++)
+    for (int side = 0; side < 2; side++) {
+      const int64_t* first = side ? right_first : left_first;
+      c->rfirst[(size_t)(2 * i + side)] = (int64_t)c->clen.size();
+      for (int64_t j = first[i]; j < first[i + 1]; j++) {
+        croff.push_back(cbytes);
+        c->clen.push_back(read_len[read_idx[j]]);
+        cbytes += read_len[read_idx[j]];
+      }
+    }
+  c->rfirst[(size_t)(2 * nctgs)] = (int64_t)c->clen.size();
+  if (cbytes > int64_t(kOffMask) - kPad)
+    return fail(c, MHMLA_E_ARG, "the sides' reads hold %lld bytes, above 2 GiB", (long long)cbytes);
+  std::vector<uint8_t> cseq((size_t)cbytes), cqual((size_t)cbytes);
+  {
+    size_t k = 0;
+    for (int64_t i = 0; i < nctgs; i++)
+      for (int side = 0; side < 2; side++) {
+        const int64_t* first = side ? right_first : left_first;
+        for (int64_t j = first[i]; j < first[i + 1]; j++) {
+          const int64_t r = read_idx[j];
+          memcpy(cseq.data() + croff[k], read_seq + read_off[r], (size_t)read_len[r]);
+          memcpy(cqual.data() + croff[k], read_qual + read_off[r], (size_t)read_len[r]);
+          k++;
+        }
+      }
   }
+  const int64_t ncr = (int64_t)c->clen.size();
   LA_HIP(hipSetDevice(c->device), "hipSetDevice");
   LA_HIP(c->d_ctg.ensure((size_t)(total + kPad)), "contig buffer");
   LA_HIP(c->d_ctg_off.ensure((size_t)nctgs + 1), "contig offsets");
   LA_HIP(c->d_ctg_len.ensure((size_t)nctgs + 1), "contig lengths");
   LA_HIP(c->d_depth.ensure((size_t)nctgs + 1), "depths");
-  LA_HIP(c->d_left_first.ensure((size_t)nctgs + 1), "left read ranges");
-  LA_HIP(c->d_right_first.ensure((size_t)nctgs + 1), "right read ranges");
-  LA_HIP(c->d_read_idx.ensure((size_t)nidx + 1), "read numbers");
-  LA_HIP(c->d_read_off.ensure((size_t)nreads + 1), "read offsets");
-  LA_HIP(c->d_read_len.ensure((size_t)nreads + 1), "read lengths");
-  if (rend > int64_t(0xFFFFFFFFu) - kPad)
-    return fail(c, MHMLA_E_ARG, "read buffers of %lld bytes exceed 4 GiB", (long long)rend);
-  LA_HIP(c->d_seq.ensure((size_t)(rend + kPad)), "read bases");
-  LA_HIP(c->d_qual.ensure((size_t)(rend + kPad)), "read qualities");
+  LA_HIP(c->d_rfirst.ensure((size_t)(2 * nctgs + 1)), "side read ranges");
+  LA_HIP(c->d_read_off.ensure((size_t)ncr + 1), "read offsets");
+  LA_HIP(c->d_read_len.ensure((size_t)ncr + 1), "read lengths");
+  LA_HIP(c->d_seq.ensure((size_t)(cbytes + kPad)), "read bases");
+  LA_HIP(c->d_qual.ensure((size_t)(cbytes + kPad)), "read qualities");
   hipStream_t st = c->stream;
   LA_HIP(hipMemcpyAsync(c->d_ctg.ptr, cbuf.data(), (size_t)total + 1, hipMemcpyHostToDevice, st), "copy contigs");
   if (nctgs) {
@@ -277,17 +296,15 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
     LA_HIP(hipMemcpyAsync(c->d_ctg_len.ptr, ctg_len, 4 * (size_t)nctgs, hipMemcpyHostToDevice, st), "copy");
     LA_HIP(hipMemcpyAsync(c->d_depth.ptr, idepth.data(), 4 * (size_t)nctgs, hipMemcpyHostToDevice, st), "copy");
   }
-  LA_HIP(hipMemcpyAsync(c->d_left_first.ptr, left_first, 8 * (size_t)(nctgs + 1), hipMemcpyHostToDevice, st), "copy");
-  LA_HIP(hipMemcpyAsync(c->d_right_first.ptr, right_first, 8 * (size_t)(nctgs + 1), hipMemcpyHostToDevice, st),
-         "copy");
-  if (nidx) LA_HIP(hipMemcpyAsync(c->d_read_idx.ptr, read_idx, 8 * (size_t)nidx, hipMemcpyHostToDevice, st), "copy");
-  if (nreads) {
-    LA_HIP(hipMemcpyAsync(c->d_read_off.ptr, read_off, 8 * (size_t)nreads, hipMemcpyHostToDevice, st), "copy");
-    LA_HIP(hipMemcpyAsync(c->d_read_len.ptr, read_len, 4 * (size_t)nreads, hipMemcpyHostToDevice, st), "copy");
+  LA_HIP(hipMemcpyAsync(c->d_rfirst.ptr, c->rfirst.data(), 8 * c->rfirst.size(), hipMemcpyHostToDevice, st), "copy");
+  if (ncr) {
+    LA_HIP(hipMemcpyAsync(c->d_read_off.ptr, croff.data(), 8 * (size_t)ncr, hipMemcpyHostToDevice, st), "copy");
+    LA_HIP(hipMemcpyAsync(c->d_read_len.ptr, c->clen.data(), 4 * (size_t)ncr, hipMemcpyHostToDevice, st), "copy");
   }
-  if (rend) {
-    LA_HIP(hipMemcpyAsync(c->d_seq.ptr, read_seq, (size_t)rend, hipMemcpyHostToDevice, st), "copy read bases");
-    LA_HIP(hipMemcpyAsync(c->d_qual.ptr, read_qual, (size_t)rend, hipMemcpyHostToDevice, st), "copy read qualities");
+  if (cbytes) {
+    LA_HIP(hipMemcpyAsync(c->d_seq.ptr, cseq.data(), (size_t)cbytes, hipMemcpyHostToDevice, st), "copy read bases");
+    LA_HIP(hipMemcpyAsync(c->d_qual.ptr, cqual.data(), (size_t)cbytes, hipMemcpyHostToDevice, st),
+           "copy read qualities");
   }
   // the host vectors above go out of scope: wait for the copies
   LA_HIP(hipStreamSynchronize(st), "staging copies");
@@ -322,11 +339,10 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
     const int32_t iters = empty_side_walks(*p, L);
     for (int side = 0; side < 2; side++) {
       const int64_t job = 2 * ci + side;
-      const std::vector<int64_t>& first = side ? c->right_first : c->left_first;
-      const int64_t nr = std::min<int64_t>(first[(size_t)ci + 1] - first[(size_t)ci], p->max_reads);
+      const int64_t f0 = c->rfirst[(size_t)job];
+      const int64_t nr = std::min<int64_t>(c->rfirst[(size_t)job + 1] - f0, p->max_reads);
       int64_t occ = 0;
-      for (int64_t i = 0; i < nr; i++)
-        occ += std::max(0, c->read_len[(size_t)c->read_idx[(size_t)(first[(size_t)ci] + i)]] - p->min_kmer_len);
+      for (int64_t i = 0; i < nr; i++) occ += std::max(0, c->clen[(size_t)(f0 + i)] - p->min_kmer_len);
       if (iters == 0 || occ == 0) {
         c->host_terms[(size_t)job * 3] = iters;  // every walk a dead end
         continue;
@@ -392,9 +408,7 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
   v.ctg_off = c->d_ctg_off.ptr;
   v.ctg_len = c->d_ctg_len.ptr;
   v.depth = c->d_depth.ptr;
-  v.left_first = c->d_left_first.ptr;
-  v.right_first = c->d_right_first.ptr;
-  v.read_idx = c->d_read_idx.ptr;
+  v.rfirst = c->d_rfirst.ptr;
   v.read_seq = c->d_seq.ptr;
   v.read_qual = c->d_qual.ptr;
   v.read_off = c->d_read_off.ptr;

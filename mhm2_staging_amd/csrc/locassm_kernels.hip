@@ -103,7 +103,8 @@ struct Side {
   int nr;
   bool wide;            // 32-bit vote counters (sides with 65535 mers or more)
 
-  __device__ const uint8_t* occ(uint64_t w) const { return seq + (uint32_t)w; }
+
+  __device__ const uint8_t* occ(uint64_t w) const { return seq + ((uint32_t)w & kOffMask); }
 };
 
 // Find the slot whose mer equals the bytes 'same' accepts; insert 'mine'
@@ -117,16 +118,16 @@ __device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool inse
   const uint64_t mine = ((uint64_t)fp << 32) | mine_low;
   int s = (int)(((uint64_t)(uint32_t)m * (uint64_t)S.cap) >> 32);
   for (int probes = 0; probes < S.cap; probes++) {
+    // read first: most probes find an occupied slot (a mer seen before), and
+    // loads of one hot slot by many lanes do not serialize as atomics do
     uint64_t w = ld64(&S.key[s]);
     if (w == kEmpty) {
       if (!insert) return -1;
-      const uint64_t prev = atomicCAS((unsigned long long*)&S.key[s], (unsigned long long)kEmpty,
-                                      (unsigned long long)mine);
-      if (prev == kEmpty) {
+      w = atomicCAS((unsigned long long*)&S.key[s], (unsigned long long)kEmpty, (unsigned long long)mine);
+      if (w == kEmpty) {
         *won = true;
         return s;
       }
-      w = prev;
     }
     if ((uint32_t)(w >> 32) == fp && same(S.occ(w))) return s;
     s = s + 1 == S.cap ? 0 : s + 1;
@@ -147,9 +148,8 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
   const bool right = (job & 1) != 0;
   const int L = v.ctg_len[c];
   const uint8_t* ctg = v.ctg_seq + v.ctg_off[c];
-  const int64_t* first = right ? v.right_first : v.left_first;
-  const int64_t f0 = first[c];
-  const int nr = (int)min(first[c + 1] - f0, (int64_t)v.max_reads);
+  const int64_t f0 = v.rfirst[job];
+  const int nr = (int)min(v.rfirst[job + 1] - f0, (int64_t)v.max_reads);
   const int W = v.walk_len_limit;
 
   Side S;
@@ -172,9 +172,8 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     S.smer = base + lay.smer;
   }
   for (int i = lane; i < nr; i += kLanes) {
-    const int64_t r = v.read_idx[f0 + i];
-    S.roff[i] = v.read_off[r];
-    S.rlen[i] = v.read_len[r];
+    S.roff[i] = v.read_off[f0 + i];
+    S.rlen[i] = v.read_len[f0 + i];
   }
   __threadfence();
   __syncthreads();
@@ -229,12 +228,18 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
         const uint8_t* rd = S.seq + S.roff[i];
         const uint8_t* rq = S.qual + S.roff[i];
         uint64_t h = hash_window(rd + pos, k);
+        int won_prev = -1;  // slot this lane claimed for the previous occurrence, if its next is this one
         for (uint32_t q = q0;;) {
           bool won = false;
           const uint8_t* mine = rd + pos;
-          const int s = find_slot(S, h, (uint64_t)(uint32_t)(mine - S.seq), true, &won,
-                                  [&](const uint8_t* o) { return same_window<8>(o, mine, k); });
+          // the successor hint below is written by this lane, so the flag is set
+          // only when this lane also takes the next occurrence of the read
+          const bool next_here = pos + 1 < S.rlen[i] - k && q + 1 < q1;
+          const uint64_t low = (uint64_t)(uint32_t)(mine - S.seq) | (next_here ? kHasNext : 0ull);
+          const int s = find_slot(S, h, low, true, &won, [&](const uint8_t* o) { return same_window<8>(o, mine, k); });
           if (s == -2) atomicAdd(&v.counters[3], 1ull);
+          if (won_prev >= 0 && s >= 0) S.info[won_prev] = (uint32_t)s;
+          won_prev = (won && next_here) ? s : -1;
           if (s >= 0) {
             const uint8_t e = rd[pos + k];
             const int b = vote_base(e);
@@ -243,10 +248,11 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
               if (S.wide) {
                 if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)s + b], 1u);
                 if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)s + 4 + b], 1u);
-              } else {
-                const uint32_t one = 1u << (16 * (b & 1));
-                if (qd >= v.min_qual) atomicAdd(&S.cnt[4 * (int64_t)s + (b >> 1)], one);
-                if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[4 * (int64_t)s + 2 + (b >> 1)], one);
+              } else if (qd >= v.min_qual) {
+                // one word per base: votes in the low half, high-quality votes in the high half
+                atomicAdd(&S.cnt[4 * (int64_t)s + b], qd >= v.min_hi_qual ? 0x10001u : 1u);
+              } else if (qd >= v.min_hi_qual) {
+                atomicAdd(&S.cnt[4 * (int64_t)s + b], 0x10000u);
               }
             }
           }
@@ -281,28 +287,30 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
           hiq[b] = ld32(&S.cnt[8 * (int64_t)s + 4 + b]);
         }
       } else {
-        const uint32_t w0 = ld32(&S.cnt[4 * (int64_t)s]), w1 = ld32(&S.cnt[4 * (int64_t)s + 1]);
-        const uint32_t w2 = ld32(&S.cnt[4 * (int64_t)s + 2]), w3 = ld32(&S.cnt[4 * (int64_t)s + 3]);
-        low[0] = w0 & 0xFFFFu;
-        low[1] = w0 >> 16;
-        low[2] = w1 & 0xFFFFu;
-        low[3] = w1 >> 16;
-        hiq[0] = w2 & 0xFFFFu;
-        hiq[1] = w2 >> 16;
-        hiq[2] = w3 & 0xFFFFu;
-        hiq[3] = w3 >> 16;
+        for (int b = 0; b < 4; b++) {
+          const uint32_t w2 = ld32(&S.cnt[4 * (int64_t)s + b]);
+          low[b] = w2 & 0xFFFFu;
+          hiq[b] = w2 >> 16;
+        }
       }
       const int ext = choose_ext(low, hiq, depth, v.rules);
       uint32_t succ = kNoSlot;
       if (ext < 4) {
         const uint8_t* rep = S.occ(w);
         const uint8_t ec = (uint8_t)ext_char(ext);
-        const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
-        bool won = false;
-        const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
-          return same_window<8>(o, rep + 1, k - 1) && o[k - 1] == ec;
-        });
-        if (t >= 0) succ = (uint32_t)t;
+        if ((w & kHasNext) && rep[k] == ec) {
+          // the stored occurrence continues with the chosen base: the next
+          // occurrence in its read is the successor mer, and the lane that
+          // stored it left that occurrence's slot in the info word
+          succ = ld32(&S.info[s]);
+        } else {
+          const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
+          bool won = false;
+          const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
+            return same_window<8>(o, rep + 1, k - 1) && o[k - 1] == ec;
+          });
+          if (t >= 0) succ = (uint32_t)t;
+        }
       }
       S.info[s] = (uint32_t)ext | (succ << 3);
     }

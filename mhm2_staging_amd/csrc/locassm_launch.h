@@ -8,11 +8,16 @@
 
 namespace mhmla {
 
-// a table slot's key word: fingerprint (32 bits) | byte offset of one
-// occurrence of the mer in the read buffer (32 bits); all ones = empty
+// a table slot's key word: fingerprint (32 bits) | whether the slot's info
+// word holds the slot of the next occurrence in the stored occurrence's read
+// (1 bit) | byte offset of the stored occurrence in the read buffer (31
+// bits); all ones = empty
 constexpr uint64_t kEmpty = ~0ull;
-// a slot's info word: extension code (3 bits) | successor slot << 3; the
-// walk marks a visited slot with the top bit
+constexpr uint64_t kHasNext = 0x80000000ull;
+constexpr uint32_t kOffMask = 0x7FFFFFFFu;
+// a slot's info word: during counting, the slot of the occurrence after the
+// stored one (see kHasNext); then extension code (3 bits) | successor slot
+// << 3, the walk marking a visited slot with the top bit
 constexpr uint32_t kNoSlot = 0x0FFFFFFFu;
 constexpr uint32_t kVisited = 0x80000000u;
 // a job's table word: slots (any count up to 2^27) | wide-counter flag
@@ -25,9 +30,9 @@ struct WalkView {
   const int64_t* ctg_off;
   const int32_t* ctg_len;
   const int32_t* depth;        // (int) of the double depth, as set_ext receives it
-  const int64_t* left_first;
-  const int64_t* right_first;
-  const int64_t* read_idx;
+  // reads in side order: side j (contig * 2 + side) owns reads [rfirst[j], rfirst[j+1]),
+  // whose bytes are contiguous in read_seq / read_qual
+  const int64_t* rfirst;
   const uint8_t* read_seq;
   const uint8_t* read_qual;
   const int64_t* read_off;
