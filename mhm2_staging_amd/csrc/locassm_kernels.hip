@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
           // only when this lane also takes the next occurrence of the read
           const bool next_here = pos + 1 < S.rlen[i] - k && q + 1 < q1;
           const uint64_t low = (uint64_t)(uint32_t)(mine - S.seq) | (next_here ? kHasNext : 0ull);
-          const int s = find_slot(S, h, low, true, &won, [&](const uint8_t* o) { return same_window<8>(o, mine, k); });
+          const int s = find_slot(S, h, low, true, &won, [&](const uint8_t* o) { return same_window<12>(o, mine, k); });
           if (s == -2) atomicAdd(&v.counters[3], 1ull);
           if (won_prev >= 0 && s >= 0) S.info[won_prev] = (uint32_t)s;
           won_prev = (won && next_here) ? s : -1;
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
           const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
           bool won = false;
           const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
-            return same_window<8>(o, rep + 1, k - 1) && o[k - 1] == ec;
+            return same_window<12>(o, rep + 1, k - 1) && o[k - 1] == ec;
           });
           if (t >= 0) succ = (uint32_t)t;
         }
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     if (lane == 0) {
       const uint64_t h0 = hash_window(S.smer, k);
       bool won = false;
-      int e = find_slot(S, h0, 0, false, &won, [&](const uint8_t* o) { return same_window<8>(o, S.smer, k); });
+      int e = find_slot(S, h0, 0, false, &won, [&](const uint8_t* o) { return same_window<12>(o, S.smer, k); });
       uint8_t* wb = cur ? S.walk[1] : S.walk[0];
       int t = 0, res = 0;  // 0 X, 1 F, 2 R
       while (t < W) {
