@@ -180,3 +180,19 @@ def test_reference_driver_interface(assembler):
             want = bytes(le[c, :ll[c]]).decode() + b.ctg(c) + bytes(re[c, :rl[c]]).decode()
             cid, seq = got[c].split(" ")
             assert int(cid) == int(b.cid[c]) and seq == (want or "-"), f"contig {c}"
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_randomized_shapes(assembler, seed):
+    """Generated batches with random read lengths, contig lengths, error and
+    fork/repeat rates, and random parameters, against the oracle."""
+    rng = np.random.default_rng(100 + seed)
+    b = ld.generate(int(rng.integers(200, 900)), seed=200 + seed,
+                    ctg_len=(int(rng.integers(20, 200)), int(rng.integers(300, 3000))),
+                    read_len=int(rng.choice([60, 100, 150, 250])), sub_rate=float(rng.uniform(0, 0.05)),
+                    n_rate=float(rng.uniform(0, 0.02)), fork_rate=float(rng.uniform(0, 0.5)),
+                    repeat_rate=float(rng.uniform(0, 0.3)))
+    kw = dict(kmer_len=int(rng.integers(15, 80)), walk_len_limit=int(rng.integers(1, 600)),
+              shift=int(rng.integers(1, 12)), min_kmer_len=int(rng.integers(9, 25)),
+              max_kmer_len=int(rng.integers(60, 140)), max_reads=int(rng.choice([5, 40, 3000])))
+    check(assembler, b, **kw)
