@@ -127,7 +127,9 @@ def locassm_leg(nctgs, steps, warmup, check):
     log(f"[bench] local assembly batch: {b.nctgs} contigs, {b.nreads} reads ({time.perf_counter() - t:.1f} s)")
     kw = dict(kmer_len=33, walk_len_limit=360)
     la = locassm.LocalAssembler(0)
+    t = time.perf_counter()
     la.stage(b)
+    stage_s = time.perf_counter() - t
     p = locassm.params(**kw)
     for _ in range(warmup):
         la.run(p)
@@ -149,6 +151,7 @@ def locassm_leg(nctgs, steps, warmup, check):
            "contigs": b.nctgs, "reads": b.nreads, "sides_on_device": st["sides"], "launches": st["launches"],
            "mers_per_step": st["mers"], "walks_per_step": st["walks"], "walk_steps_per_step": st["walk_steps"],
            "device_ms": round(dev_ms, 3), "ms_per_step": round(wall * 1e3, 3),
+           "stage_ms": round(stage_s * 1e3, 1),
            "value": round(b.nctgs / (dev_ms / 1e3), 1), "unit": "contigs/s (both ends)",
            "mers_per_s": round(st["mers"] / (dev_ms / 1e3), 1),
            "scratch_bytes": st["scratch_bytes"],
