@@ -212,6 +212,17 @@ int mhmsw_server_status(mhmsw_server *srv, uint64_t req_id);
 int mhmsw_server_wait(mhmsw_server *srv, uint64_t req_id);
 /* Waits, copies npairs results and the request's times, and forgets the request. */
 int mhmsw_server_results(mhmsw_server *srv, uint64_t req_id, mhmsw_result *out, int64_t npairs, mhmsw_req_times *times);
+/* Second-best scores and CIGARs for the jobs admitted after the call (both
+ * off by default), computed as mhmsw_set_score2 / mhmsw_set_cigar do. */
+int mhmsw_server_set_outputs(mhmsw_server *srv, int score2, int cigar);
+/* After mhmsw_server_results of a request that ran with them: its second-best
+ * scores (score2, may be NULL) and CIGARs in the layout of
+ * mhmsw_results_cigar (mismatches, op_off[0..npairs], ops; call with ops ==
+ * NULL first to learn *total_ops).  The server keeps them until a call passes
+ * ops (or, for a request without CIGARs, until the first call). */
+int mhmsw_server_results_extra(mhmsw_server *srv, uint64_t req_id, int64_t npairs, int32_t *score2,
+                               int32_t *mismatches, int64_t *op_off, uint32_t *ops, int64_t ops_cap,
+                               int64_t *total_ops);
 double mhmsw_server_clock(void); /* the clock of mhmsw_req_times */
 const char *mhmsw_server_last_error(const mhmsw_server *srv);
 int mhmsw_server_destroy(mhmsw_server *srv);

@@ -89,6 +89,7 @@ EXPORTS = [
     "mhmsw_contigs_load", "mhmsw_contigs_free", "mhmsw_contigs_bytes", "mhmsw_stage_windows",
     "mhmsw_server_create", "mhmsw_server_submit", "mhmsw_server_status", "mhmsw_server_wait",
     "mhmsw_server_results", "mhmsw_server_clock", "mhmsw_server_last_error", "mhmsw_server_destroy",
+    "mhmsw_server_set_outputs", "mhmsw_server_results_extra",
     "mhmsw_synth_contigs", "mhmsw_synth_contigs_free", "mhmsw_synth_window_pairs", "mhmsw_synth_window_pairs_free",
 ]
 
@@ -139,6 +140,8 @@ def lib():
         "mhmsw_server_wait": [vp, ctypes.c_uint64],
         "mhmsw_server_results": [vp, ctypes.c_uint64, vp, i64, ctypes.POINTER(ReqTimes)],
         "mhmsw_server_destroy": [vp],
+        "mhmsw_server_set_outputs": [vp, ctypes.c_int, ctypes.c_int],
+        "mhmsw_server_results_extra": [vp, ctypes.c_uint64, i64, vp, vp, vp, vp, i64, ctypes.POINTER(i64)],
         "mhmsw_synth_contigs": [ctypes.c_uint64, i64, i32, i32, ctypes.c_int, ctypes.POINTER(SynthContigs)],
         "mhmsw_synth_window_pairs": [ctypes.POINTER(SynthContigs), ctypes.c_uint64, i64, ctypes.c_int,
                                      ctypes.POINTER(SynthWindows)],

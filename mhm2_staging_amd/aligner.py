@@ -21,7 +21,7 @@ MHM2_TREE_SCORING = (1, 1, 1, 1, 1)
 
 
 def _ptr(a):
-    return a.ctypes.data_as(ctypes.c_void_p)
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
 class Aligner:
@@ -298,6 +298,15 @@ class Server:
         if rc != 0:
             raise _native.MhmswError(rc, "mhmsw_server_create failed")
         self._pending = {}
+        self._extra = {}
+        self._outputs = (False, False)
+
+    def set_outputs(self, score2=False, cigar=False):
+        """Second-best scores and/or CIGARs for requests admitted from now on (mhmsw_server_set_outputs)."""
+        rc = self._L.mhmsw_server_set_outputs(self._h, int(bool(score2)), int(bool(cigar)))
+        if rc != 0:
+            raise _native.MhmswError(rc, "mhmsw_server_set_outputs failed")
+        self._outputs = (bool(score2), bool(cigar))
 
     def submit(self, batch: WindowBatch):
         rid = ctypes.c_uint64(0)
@@ -307,6 +316,7 @@ class Server:
         if rc != 0:
             raise _native.MhmswError(rc, "submit")
         self._pending[rid.value] = batch.npairs
+        self._extra[rid.value] = self._outputs
         return rid.value
 
     def status(self, rid):
@@ -322,6 +332,32 @@ class Server:
             msg = self._L.mhmsw_server_last_error(self._h)
             raise _native.MhmswError(rc, f"request {rid}: {msg.decode() if msg else ''}")
         return out, {"submit_s": t.submit_s, "admit_s": t.admit_s, "done_s": t.done_s, "device_ms": t.device_ms}
+
+    def extras(self, rid, npairs):
+        """After results(rid): (score2 or None, mismatches, op_off, ops) or CIGAR fields None when off."""
+        s2_on, cig_on = self._extra.pop(rid, (False, False))
+        s2 = np.zeros(npairs, np.int32) if s2_on else None
+        mism = off = ops = None
+        if cig_on:
+            mism = np.zeros(npairs, np.int32)
+            off = np.zeros(npairs + 1, np.int64)
+            tot = ctypes.c_int64(0)
+            rc = self._L.mhmsw_server_results_extra(self._h, ctypes.c_uint64(rid), npairs, _ptr(s2), _ptr(mism),
+                                                    _ptr(off), None, 0, ctypes.byref(tot))
+            if rc == 0:
+                ops = np.zeros(max(tot.value, 1), np.uint32)
+                rc = self._L.mhmsw_server_results_extra(self._h, ctypes.c_uint64(rid), npairs, None, None, None,
+                                                        _ptr(ops), tot.value, ctypes.byref(tot))
+                ops = ops[:tot.value]
+        elif s2_on:
+            rc = self._L.mhmsw_server_results_extra(self._h, ctypes.c_uint64(rid), npairs, _ptr(s2), None, None,
+                                                    None, 0, None)
+        else:
+            return None, None, None, None
+        if rc != 0:
+            msg = self._L.mhmsw_server_last_error(self._h)
+            raise _native.MhmswError(rc, f"request {rid}: {msg.decode() if msg else ''}")
+        return s2, mism, off, ops
 
     def clock(self):
         return self._L.mhmsw_server_clock()

@@ -1427,6 +1427,11 @@ struct Request {
   int slot = -1;
   int64_t est = 0;
   std::vector<mhmsw_result> results;
+  // second-best scores and CIGARs, when the server computes them
+  bool has_s2 = false, has_cigar = false;
+  std::vector<int32_t> s2, mism;
+  std::vector<int64_t> op_off;
+  std::vector<uint32_t> ops;
   mhmsw_req_times times{};
   std::string err;
 };
@@ -1445,8 +1450,46 @@ struct Job {
   std::vector<int64_t> first;  // first pair of each request in the job
   int64_t npairs = 0;
   int64_t est = 0;
+  bool s2 = false, cigar = false;  // the outputs this job computes
   std::vector<mhmsw_result> results;
 };
+
+// the second-best scores and CIGARs of a finished job, split by request
+int split_extras(mhmsw_ctx* c, Job& j) {
+  std::vector<int32_t> s2, mism;
+  std::vector<int64_t> off;
+  std::vector<uint32_t> ops;
+  const size_t n = (size_t)std::max<int64_t>(j.npairs, 1);
+  if (j.s2) {
+    s2.resize(n);
+    const int rc = mhmsw_results_score2(c, s2.data(), j.npairs);
+    if (rc != MHMSW_OK) return rc;
+  }
+  if (j.cigar) {
+    mism.resize(n);
+    off.resize(n + 1);
+    int64_t total = 0;
+    int rc = mhmsw_results_cigar(c, j.npairs, mism.data(), off.data(), nullptr, 0, &total);
+    if (rc != MHMSW_OK) return rc;
+    ops.resize((size_t)std::max<int64_t>(total, 1));
+    rc = mhmsw_results_cigar(c, j.npairs, nullptr, nullptr, ops.data(), total, &total);
+    if (rc != MHMSW_OK) return rc;
+  }
+  for (size_t i = 0; i < j.reqs.size(); i++) {
+    Request& r = *j.reqs[i];
+    const int64_t a = j.first[i], b = a + r.npairs;
+    r.has_s2 = j.s2;
+    r.has_cigar = j.cigar;
+    if (j.s2) r.s2.assign(s2.begin() + a, s2.begin() + b);
+    if (j.cigar) {
+      r.mism.assign(mism.begin() + a, mism.begin() + b);
+      r.op_off.resize((size_t)r.npairs + 1);
+      for (int64_t p = 0; p <= r.npairs; p++) r.op_off[(size_t)p] = off[(size_t)(a + p)] - off[(size_t)a];
+      r.ops.assign(ops.begin() + off[(size_t)a], ops.begin() + off[(size_t)b]);
+    }
+  }
+  return MHMSW_OK;
+}
 
 constexpr int64_t kMergeMaxPairs = (int64_t)1 << 20;
 
@@ -1463,6 +1506,8 @@ struct mhmsw_server {
   std::condition_variable cv_done, cv_work;
   std::deque<std::shared_ptr<Request>> queue;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> extras;  // results taken, extras not yet
+  int score2 = 0, cigar = 0;  // outputs of jobs admitted from now on
   uint64_t next_id = 1;
   bool stop = false;
   std::thread th;
@@ -1485,6 +1530,7 @@ static void server_loop(mhmsw_server* s) {
       if (q == 1) {
         j->results.resize((size_t)std::max<int64_t>(j->npairs, 1));
         rc = mhmsw_results(s->slots[k], j->results.data(), j->npairs);
+        if (rc == MHMSW_OK && (j->s2 || j->cigar)) rc = split_extras(s->slots[k], *j);
         if (rc == MHMSW_OK) {
           mhmsw_get_stats(s->slots[k], &st);
           for (size_t i = 0; i < j->reqs.size(); i++) {
@@ -1538,6 +1584,8 @@ static void server_loop(mhmsw_server* s) {
         s->queue.pop_front();
       }
       if (j->reqs.empty()) break;
+      j->s2 = s->score2 != 0;
+      j->cigar = s->cigar != 0;
       const double now = steady_now();
       for (auto& r : j->reqs) {
         r->state = 1;
@@ -1548,6 +1596,9 @@ static void server_loop(mhmsw_server* s) {
       s->used += j->est;
       lk.unlock();
       mhmsw_ctx* c = s->slots[(size_t)freek];
+      // the slot's context belongs to this thread while no job runs on it
+      mhmsw_set_score2(c, j->s2 ? 1 : 0);
+      mhmsw_set_cigar(c, j->cigar ? 1 : 0);
       std::vector<const PackedBatch*> parts;
       parts.reserve(j->reqs.size());
       for (auto& r : j->reqs) parts.push_back(&r->packed);
@@ -1687,7 +1738,61 @@ int mhmsw_server_results(mhmsw_server* s, uint64_t req_id, mhmsw_result* out, in
     return MHMSW_E_ARG;
   }
   if (npairs > 0) std::memcpy(out, r->results.data(), (size_t)npairs * sizeof(mhmsw_result));
+  r->results.clear();
+  if (r->has_s2 || r->has_cigar) s->extras[req_id] = r;
   s->reqs.erase(it);
+  return MHMSW_OK;
+}
+
+int mhmsw_server_set_outputs(mhmsw_server* s, int score2, int cigar) {
+  if (!s) return MHMSW_E_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  s->score2 = score2 != 0;
+  s->cigar = cigar != 0;
+  return MHMSW_OK;
+}
+
+int mhmsw_server_results_extra(mhmsw_server* s, uint64_t req_id, int64_t npairs, int32_t* score2,
+                               int32_t* mismatches, int64_t* op_off, uint32_t* ops, int64_t ops_cap,
+                               int64_t* total_ops) {
+  if (!s) return MHMSW_E_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->extras.find(req_id);
+  if (it == s->extras.end()) {
+    s->err = "no second-best scores or CIGARs held for this request (results not taken, or outputs off)";
+    return MHMSW_E_STATE;
+  }
+  Request& r = *it->second;
+  if (npairs != r.npairs) {
+    s->err = "extras requested for a different number of pairs";
+    return MHMSW_E_ARG;
+  }
+  if (score2) {
+    if (!r.has_s2) {
+      s->err = "the request ran without second-best scores";
+      return MHMSW_E_STATE;
+    }
+    if (npairs > 0) std::memcpy(score2, r.s2.data(), (size_t)npairs * sizeof(int32_t));
+  }
+  if (mismatches || op_off || ops || total_ops) {
+    if (!r.has_cigar) {
+      s->err = "the request ran without CIGARs";
+      return MHMSW_E_STATE;
+    }
+    const int64_t total = (int64_t)r.ops.size();
+    if (total_ops) *total_ops = total;
+    if (mismatches && npairs > 0) std::memcpy(mismatches, r.mism.data(), (size_t)npairs * sizeof(int32_t));
+    if (op_off) std::memcpy(op_off, r.op_off.data(), (size_t)(npairs + 1) * sizeof(int64_t));
+    if (ops) {
+      if (ops_cap < total) {
+        s->err = "ops buffer smaller than the request's total ops";
+        return MHMSW_E_ARG;
+      }
+      if (total > 0) std::memcpy(ops, r.ops.data(), (size_t)total * sizeof(uint32_t));
+    }
+  }
+  // released once the ops (or, without CIGARs, the scores) have been taken
+  if (ops || !r.has_cigar) s->extras.erase(it);
   return MHMSW_OK;
 }
 

@@ -113,3 +113,43 @@ def test_server_merges_queued_requests(serving_set):
     for r, (o, _) in list(zip(reqs, outs))[1:4]:
         sb = r.as_strings(cs)
         assert_same(gpu_table(o), oracle_table(sb, BASELINE_SCORING), sb)
+
+
+def test_server_score2_and_cigars_match_the_batch_api(gpu_aligner, serving_set):
+    """With second-best scores and CIGARs on, each request's extras equal what
+    the batch API computes for the same windows, also when queued requests
+    share one job (one slot, submitted back to back)."""
+    cs, res = serving_set
+    reqs = [m.window_pairs(cs, n, seed=90 + i) for i, n in enumerate([800, 3000, 150, 2200])]
+    al = gpu_aligner
+    al.set_scoring(BASELINE_SCORING)
+    al.set_mode(m.MODE_FAST)
+    al.set_orientation(True)
+    al.set_score2(True)
+    al.set_cigar(True)
+    want = []
+    try:
+        for r in reqs:
+            al.stage_windows(res, r)
+            al.align_staged()
+            al.wait()
+            want.append((al.results(), al.score2(), al.cigars()))
+    finally:
+        al.set_score2(False)
+        al.set_cigar(False)
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv.set_outputs(score2=True, cigar=True)
+    ids = [srv.submit(r) for r in reqs]
+    for r, rid, (wres, ws2, (wm, woff, wops)) in zip(reqs, ids, want):
+        got, _ = srv.results(rid)
+        s2, mism, off, ops = srv.extras(rid, r.npairs)
+        assert (got == wres).all()
+        assert np.array_equal(s2, ws2)
+        assert np.array_equal(mism, wm) and np.array_equal(off, woff) and np.array_equal(ops, wops)
+    # outputs off again: plain results, no extras held
+    srv.set_outputs(False, False)
+    rid = srv.submit(reqs[0])
+    got, _ = srv.results(rid)
+    assert (got == want[0][0]).all()
+    assert srv.extras(rid, reqs[0].npairs) == (None, None, None, None)
+    srv.close()
