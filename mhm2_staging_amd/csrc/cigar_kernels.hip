@@ -119,13 +119,18 @@ __device__ __forceinline__ bool first_eq(const CigarView& v, const CigarGeo& g) 
   return nib_at(v.codes, g.qbase + g.qb) == nib_at(v.codes, g.rbase + g.rb);
 }
 
-// A pair whose band missed the score retries with half-width nb in the
-// wide class (one wavefront per pair: the few pairs that retry would each
-// take a whole narrow launch's latency).  Rare: per-lane atomic.
-__device__ __forceinline__ void retry(const CigarView& v, int64_t p, int32_t nb) {
-  v.band[p] = nb;
-  const int32_t at = atomicAdd(&v.list_count[kCigClassWide], 1);
-  list_of(v, kCigClassWide)[at] = (int32_t)p;
+// A pair whose band of half-width BW missed the score retries with twice
+// the half-width.  Half-width 1 retries go to the half-width-2 list, which
+// the narrow kernel (launched after this one) reads; later retries go to the
+// wide class (one wavefront per pair: the few pairs that retry from the
+// narrow kernel would otherwise need another narrow launch).  Rare: per-lane
+// atomic.
+template <int BW>
+__device__ __forceinline__ void retry(const CigarView& v, int64_t p) {
+  v.band[p] = 2 * BW;
+  const int cls = BW == 1 ? 1 : kCigClassWide;
+  const int32_t at = atomicAdd(&v.list_count[cls], 1);
+  list_of(v, cls)[at] = (int32_t)p;
 }
 
 // One narrow pair per lane (p < 0: idle lane); all lanes call it.  DS is
@@ -161,7 +166,7 @@ __device__ __forceinline__ void narrow_pair(const CigarView& v, int64_t p, uint3
         queued = q.fits();
       }
     } else {
-      retry(v, p, 2 * BW);
+      retry<BW>(v, p);
     }
   }
   const bool ok = done && jf >= 0;
