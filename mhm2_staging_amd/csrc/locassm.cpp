@@ -49,6 +49,7 @@ struct Buf {
 constexpr int64_t kDefaultBudget = int64_t(16) << 30;  // scratch bytes per launch (at most half the free memory)
 constexpr int64_t kPad = 64;                           // bytes readable past every staged buffer
 constexpr int64_t kMaxCap = int64_t(1) << 27;         // slots of one table
+constexpr int32_t kLdsReadBytes = 9728;                // LDS per side for its reads (16 sides per CU)
 
 }  // namespace
 
@@ -427,6 +428,7 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
   v.min_qual = p->min_qual;
   v.min_hi_qual = p->min_hi_qual;
   v.max_reads = p->max_reads;
+  v.lds_bytes = kLdsReadBytes;
   v.rules = Rules{p->min_viable_depth, p->min_expected_depth, p->rating_thres};
   for (const auto& lr : launches) {
     v.jobs = c->d_jobs.ptr + lr.first;

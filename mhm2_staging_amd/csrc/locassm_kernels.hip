@@ -153,8 +153,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
   const int W = v.walk_len_limit;
 
   Side S;
-  S.seq = v.read_seq;
-  S.qual = v.read_qual;
+  extern __shared__ __align__(16) uint8_t lds_reads[];
   S.cap = v.cap[slot] & kCapMask;
   S.wide = (v.cap[slot] & kWideCounts) != 0;
   S.nr = nr;
@@ -171,9 +170,25 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     S.walk[1] = base + lay.walk1;
     S.smer = base + lay.smer;
   }
+  // the side's reads are contiguous from base; offsets below are relative to it
+  const int64_t base = nr > 0 ? v.read_off[f0] : 0;
+  const int64_t side_bytes = nr > 0 ? v.read_off[f0 + nr - 1] + v.read_len[f0 + nr - 1] - base : 0;
   for (int i = lane; i < nr; i += kLanes) {
-    S.roff[i] = v.read_off[f0 + i];
+    S.roff[i] = v.read_off[f0 + i] - base;
     S.rlen[i] = v.read_len[f0 + i];
+  }
+  S.qual = v.read_qual + base;
+  if (side_bytes + 64 <= v.lds_bytes) {
+    // small side: its bases in LDS (every compare and hash below reads them)
+    const uint32_t* src = (const uint32_t*)((uintptr_t)(v.read_seq + base) & ~(uintptr_t)3);
+    const int sh = (int)((uintptr_t)(v.read_seq + base) & 3);
+    const int nw = (int)((side_bytes + 3) >> 2);
+    uint32_t* dst = (uint32_t*)lds_reads;
+    for (int j = lane; j < nw; j += kLanes) dst[j] = __builtin_amdgcn_alignbyte(src[j + 1], src[j], sh);
+    for (int j = nw + lane; j < nw + 16; j += kLanes) dst[j] = 0;
+    S.seq = lds_reads;
+  } else {
+    S.seq = v.read_seq + base;
   }
   __threadfence();
   __syncthreads();
@@ -402,7 +417,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
 
 void launch_walks(const WalkView& v, int64_t njobs, hipStream_t st) {
   if (njobs <= 0) return;
-  hipLaunchKernelGGL(walk_kernel, dim3((unsigned)njobs), dim3(kLanes), 0, st, v);
+  hipLaunchKernelGGL(walk_kernel, dim3((unsigned)njobs), dim3(kLanes), (size_t)v.lds_bytes, st, v);
 }
 
 }  // namespace mhmla

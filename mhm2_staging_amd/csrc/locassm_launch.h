@@ -51,6 +51,7 @@ struct WalkView {
   // parameters
   int32_t kmer_len, min_kmer_len, max_kmer_len, shift, walk_len_limit;
   int32_t qual_offset, min_qual, min_hi_qual, max_reads;
+  int32_t lds_bytes;  // dynamic LDS per side: sides whose reads (+64) fit it read them from LDS
   Rules rules;
 };
 
