@@ -187,6 +187,9 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     for (int j = lane; j < nw; j += kLanes) dst[j] = __builtin_amdgcn_alignbyte(src[j + 1], src[j], sh);
     for (int j = nw + lane; j < nw + 16; j += kLanes) dst[j] = 0;
     S.seq = lds_reads;
+    // a small table's info words in LDS too: the walk chases them
+    const int64_t info_at = (side_bytes + 64 + 15) & ~(int64_t)15;
+    if (info_at + 4 * (int64_t)S.cap <= v.lds_bytes) S.info = (uint32_t*)(lds_reads + info_at);
   } else {
     S.seq = v.read_seq + base;
   }
