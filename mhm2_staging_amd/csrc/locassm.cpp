@@ -49,7 +49,7 @@ struct Buf {
 constexpr int64_t kDefaultBudget = int64_t(16) << 30;  // scratch bytes per launch (at most half the free memory)
 constexpr int64_t kPad = 64;                           // bytes readable past every staged buffer
 constexpr int64_t kMaxCap = int64_t(1) << 27;         // slots of one table
-constexpr int32_t kLdsReadBytes = 9728;                // LDS per side for its reads (16 sides per CU)
+constexpr int32_t kLdsReadBytes = 12288;               // LDS per side for reads and a small table's info words
 
 }  // namespace
 
