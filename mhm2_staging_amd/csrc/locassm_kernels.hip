@@ -295,20 +295,28 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     t0 = t1;
 
     // ---- decide: extension and successor of every occupied slot ----
+    // Keys and votes no longer change, and the fence above invalidated this
+    // CU's L1: plain (vector) loads read what the atomics left in L2.
     for (int s = lane; s < S.cap; s += kLanes) {
-      const uint64_t w = ld64(&S.key[s]);
+      const uint64_t w = S.key[s];
       if (w == kEmpty) continue;
       uint32_t low[4], hiq[4];
       if (S.wide) {
-        for (int b = 0; b < 4; b++) {
-          low[b] = ld32(&S.cnt[8 * (int64_t)s + b]);
-          hiq[b] = ld32(&S.cnt[8 * (int64_t)s + 4 + b]);
-        }
+        const uint4 a = *(const uint4*)(S.cnt + 8 * (int64_t)s), b4 = *(const uint4*)(S.cnt + 8 * (int64_t)s + 4);
+        low[0] = a.x;
+        low[1] = a.y;
+        low[2] = a.z;
+        low[3] = a.w;
+        hiq[0] = b4.x;
+        hiq[1] = b4.y;
+        hiq[2] = b4.z;
+        hiq[3] = b4.w;
       } else {
+        const uint4 a = *(const uint4*)(S.cnt + 4 * (int64_t)s);
+        const uint32_t wv[4] = {a.x, a.y, a.z, a.w};
         for (int b = 0; b < 4; b++) {
-          const uint32_t w2 = ld32(&S.cnt[4 * (int64_t)s + b]);
-          low[b] = w2 & 0xFFFFu;
-          hiq[b] = w2 >> 16;
+          low[b] = wv[b] & 0xFFFFu;
+          hiq[b] = wv[b] >> 16;
         }
       }
       const int ext = choose_ext(low, hiq, depth, v.rules);
@@ -320,7 +328,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
           // the stored occurrence continues with the chosen base: the next
           // occurrence in its read is the successor mer, and the lane that
           // stored it left that occurrence's slot in the info word
-          succ = ld32(&S.info[s]);
+          succ = S.info[s];
         } else {
           const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
           bool won = false;
