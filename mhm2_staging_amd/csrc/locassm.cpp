@@ -70,7 +70,7 @@ struct mhmla_ctx {
   // last run
   bool ran = false;
   mhmla_params params{};
-  std::vector<int32_t> host_len, host_terms;  // per job: results of sides planned on the host
+  std::vector<int32_t> host_terms;            // per job: walk terminations of sides answered on the host
   std::vector<uint8_t> on_device;             // per job
   Buf<int32_t> d_jobs, d_cap, d_out_len, d_out_terms;
   Buf<int64_t> d_soff;
@@ -325,7 +325,6 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
   c->params = *p;
   const int64_t n = c->nctgs, njobs = 2 * n;
   const int W = p->walk_len_limit;
-  c->host_len.assign((size_t)njobs, 0);
   c->host_terms.assign((size_t)njobs * 3, 0);
   c->on_device.assign((size_t)njobs, 0);
   // plan: one job per contig side with reads and at least one mer length to try
