@@ -154,7 +154,7 @@ def locassm_leg(nctgs, steps, warmup, check):
            "stage_ms": round(stage_s * 1e3, 1),
            "value": round(b.nctgs / (dev_ms / 1e3), 1), "unit": "contigs/s (both ends)",
            "mers_per_s": round(st["mers"] / (dev_ms / 1e3), 1),
-           "scratch_bytes": st["scratch_bytes"],
+           "scratch_bytes": st["scratch_bytes"], "table_resizes": st["table_resizes"],
            "phase_share": dict(zip(("clear", "count", "decide", "walk"),
                                    (round(x / max(1, sum(st["phase_cycles"])), 3) for x in st["phase_cycles"]))),
            "extended_sides": int((got.left_len > 0).sum() + (got.right_len > 0).sum()),

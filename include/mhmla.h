@@ -60,6 +60,7 @@ typedef struct {
   int64_t scratch_bytes;/* device scratch used by the largest launch */
   int64_t phase_cycles[4]; /* shader clock cycles per phase, summed over sides: table clear,
                               mer counting, extension and successor choice, walk */
+  int64_t table_resizes; /* mer lengths whose count was redone in a larger table */
 } mhmla_stats;
 
 /* Defaults: the reference's constants, kmer_len 33, walk_len_limit 300,

@@ -49,6 +49,7 @@ struct Buf {
 constexpr int64_t kDefaultBudget = int64_t(16) << 30;  // scratch bytes per launch (at most half the free memory)
 constexpr int64_t kPad = 64;                           // bytes readable past every staged buffer
 constexpr int64_t kMaxCap = int64_t(1) << 27;         // slots of one table
+constexpr int kCounters = 9;                          // WalkView::counters
 constexpr int32_t kLdsReadBytes = 12288;               // LDS per side for reads and a small table's info words
 
 }  // namespace
@@ -394,14 +395,14 @@ int mhmla_run(mhmla_ctx* c, const mhmla_params* p) {
   LA_HIP(c->d_out_len.ensure((size_t)njobs + 1), "walk lengths");
   LA_HIP(c->d_out_terms.ensure((size_t)njobs * 3 + 1), "walk terminations");
   LA_HIP(c->d_out_ext.ensure((size_t)njobs * (size_t)W + 1), "walks");
-  LA_HIP(c->d_counters.ensure(8), "counters");
+  LA_HIP(c->d_counters.ensure(kCounters), "counters");
   hipStream_t st = c->stream;
   if (nd) {
     LA_HIP(hipMemcpyAsync(c->d_jobs.ptr, jobs.data(), 4 * (size_t)nd, hipMemcpyHostToDevice, st), "copy jobs");
     LA_HIP(hipMemcpyAsync(c->d_cap.ptr, caps.data(), 4 * (size_t)nd, hipMemcpyHostToDevice, st), "copy jobs");
     LA_HIP(hipMemcpyAsync(c->d_soff.ptr, soff.data(), 8 * (size_t)nd, hipMemcpyHostToDevice, st), "copy jobs");
   }
-  LA_HIP(hipMemsetAsync(c->d_counters.ptr, 0, 8 * sizeof(unsigned long long), st), "clear counters");
+  LA_HIP(hipMemsetAsync(c->d_counters.ptr, 0, kCounters * sizeof(unsigned long long), st), "clear counters");
   LA_HIP(hipEventRecord(c->ev0, st), "event");
   WalkView v{};
   v.ctg_seq = c->d_ctg.ptr;
@@ -451,7 +452,7 @@ int mhmla_wait(mhmla_ctx* c) {
   LA_HIP(hipStreamSynchronize(c->stream), "local assembly walks");
   float ms = 0.f;
   LA_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1), "event timing");
-  unsigned long long cnt[8];
+  unsigned long long cnt[kCounters];
   LA_HIP(hipMemcpy(cnt, c->d_counters.ptr, sizeof cnt, hipMemcpyDeviceToHost), "copy counters");
   if (cnt[3]) return fail(c, MHMLA_E_HIP, "%llu mer insertions found their table full", cnt[3]);
   c->stats.job_ms = ms;
@@ -459,6 +460,7 @@ int mhmla_wait(mhmla_ctx* c) {
   c->stats.walk_steps = (int64_t)cnt[1];
   c->stats.walks = (int64_t)cnt[2];
   for (int i = 0; i < 4; i++) c->stats.phase_cycles[i] = (int64_t)cnt[4 + i];
+  c->stats.table_resizes = (int64_t)cnt[8];
   for (size_t j = 0; j < c->on_device.size(); j++)
     if (!c->on_device[j]) c->stats.walks += c->host_terms[3 * j];
   return MHMLA_OK;

@@ -32,11 +32,12 @@ namespace mhmla {
 namespace {
 
 constexpr int kLanes = 64;
+constexpr int kU = 4;  // mer occurrences per lane per count batch
+constexpr int kD = 4;  // slots per lane per decide batch
+constexpr int kLoadNum = 16;    // table slots per 16 mer occurrences, before a resize
+constexpr int kMaxProbes = 96;  // longest probe run in a table below its allocated size
 
 __device__ __forceinline__ uint64_t ld64(uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld32(uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -99,7 +100,8 @@ struct Side {
   uint32_t* cum;        // occurrences before read i, at the current mer length
   uint8_t* walk[2];
   uint8_t* smer;        // the side's last mer (reverse-complemented on the left)
-  int cap;
+  int cap;              // slots in use at the current mer length
+  int cap_alloc;        // slots the scratch holds
   int nr;
   bool wide;            // 32-bit vote counters (sides with 65535 mers or more)
 
@@ -107,17 +109,21 @@ struct Side {
   __device__ const uint8_t* occ(uint64_t w) const { return seq + ((uint32_t)w & kOffMask); }
 };
 
-// Find the slot whose mer equals the bytes 'same' accepts; insert 'mine'
-// (fingerprint | byte offset of the occurrence) there when absent and insert
-// is set.
+// First probe slot and fingerprint of a mer with rolled hash h.
+__device__ __forceinline__ int home_slot(const Side& S, uint64_t h, uint32_t* fp) {
+  const uint64_t m = mix64(h);
+  *fp = (uint32_t)(m >> 32);
+  return (int)(((uint64_t)(uint32_t)m * (uint64_t)S.cap) >> 32);
+}
+
+// Probe from slot s for the slot whose mer equals the bytes 'same' accepts,
+// at most 'probes' slots; insert 'mine' (fingerprint | byte offset of the
+// occurrence) into the first empty slot when insert is set.
 // Returns the slot, -1 when absent and not inserted, -2 if the table is full.
 template <class Same>
-__device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool insert, bool* won, Same same) {
-  const uint64_t m = mix64(h);
-  const uint32_t fp = (uint32_t)(m >> 32);
-  const uint64_t mine = ((uint64_t)fp << 32) | mine_low;
-  int s = (int)(((uint64_t)(uint32_t)m * (uint64_t)S.cap) >> 32);
-  for (int probes = 0; probes < S.cap; probes++) {
+__device__ int probe_from(const Side& S, int s, int probes, uint32_t fp, uint64_t mine, bool insert, bool* won,
+                          Same same) {
+  for (; probes > 0; probes--) {
     // read first: most probes find an occupied slot (a mer seen before), and
     // loads of one hot slot by many lanes do not serialize as atomics do
     uint64_t w = ld64(&S.key[s]);
@@ -135,12 +141,20 @@ __device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool inse
   return -2;
 }
 
+// Find (and with insert, claim) the slot of the mer with rolled hash h.
+template <class Same>
+__device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool insert, bool* won, Same same) {
+  uint32_t fp;
+  const int s = home_slot(S, h, &fp);
+  return probe_from(S, s, S.cap, fp, ((uint64_t)fp << 32) | mine_low, insert, won, same);
+}
+
 __device__ __forceinline__ uint8_t side_byte(const uint8_t* ctg, int L, bool right, int i) {
   return right ? ctg[i] : complement(ctg[L - 1 - i]);
 }
 
 __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
-  __shared__ int sh_t, sh_res;
+  __shared__ int sh_t, sh_res, sh_over;
   const int lane = threadIdx.x;
   const int slot = blockIdx.x;
   const int job = v.jobs[slot];
@@ -154,12 +168,14 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
 
   Side S;
   extern __shared__ __align__(16) uint8_t lds_reads[];
-  S.cap = v.cap[slot] & kCapMask;
+  S.cap_alloc = v.cap[slot] & kCapMask;
+  S.cap = S.cap_alloc;
   S.wide = (v.cap[slot] & kWideCounts) != 0;
   S.nr = nr;
+  uint32_t* info_glb;
   {
     uint8_t* base = v.scratch + v.scratch_off[slot];
-    const JobLayout lay = job_layout(S.cap, S.wide, nr, W, v.max_kmer_len);
+    const JobLayout lay = job_layout(S.cap_alloc, S.wide, nr, W, v.max_kmer_len);
     S.key = (uint64_t*)(base + lay.key);
     S.cnt = (uint32_t*)(base + lay.cnt);
     S.info = (uint32_t*)(base + lay.info);
@@ -177,7 +193,9 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     S.roff[i] = v.read_off[f0 + i] - base;
     S.rlen[i] = v.read_len[f0 + i];
   }
+  info_glb = S.info;
   S.qual = v.read_qual + base;
+  int64_t info_at = 0;  // LDS bytes taken by the reads
   if (side_bytes + 64 <= v.lds_bytes) {
     // small side: its bases in LDS (every compare and hash below reads them)
     const uint32_t* src = (const uint32_t*)((uintptr_t)(v.read_seq + base) & ~(uintptr_t)3);
@@ -187,12 +205,11 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     for (int j = lane; j < nw; j += kLanes) dst[j] = __builtin_amdgcn_alignbyte(src[j + 1], src[j], sh);
     for (int j = nw + lane; j < nw + 16; j += kLanes) dst[j] = 0;
     S.seq = lds_reads;
-    // a small table's info words in LDS too: the walk chases them
-    const int64_t info_at = (side_bytes + 64 + 15) & ~(int64_t)15;
-    if (info_at + 4 * (int64_t)S.cap <= v.lds_bytes) S.info = (uint32_t*)(lds_reads + info_at);
+    info_at = (side_bytes + 64 + 15) & ~(int64_t)15;
   } else {
     S.seq = v.read_seq + base;
   }
+  if (lane == 0) sh_over = 0;
   __threadfence();
   __syncthreads();
 
@@ -200,18 +217,12 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
   const int max_len = min(v.max_kmer_len, L);
   int longest = 0, best = 0, cur = 0, shift = 0;
   int term_x = 0, term_f = 0, term_r = 0;
-  unsigned long long mers = 0, steps = 0, walks = 0;
+  unsigned long long mers = 0, steps = 0, walks = 0, resizes = 0;
   unsigned long long cyc[4] = {0, 0, 0, 0};  // clear, count, decide, walk (shader clock, lane 0)
   long long t0 = clock64();
 
   for (int k = v.kmer_len; k >= v.min_kmer_len && k <= max_len; k += shift) {
-    // ---- clear the table and count occurrences per read ----
-    for (int s = lane; s < S.cap; s += kLanes) S.key[s] = kEmpty;
-    {
-      uint4* cp = (uint4*)S.cnt;
-      const int nq = S.wide ? 2 * S.cap : S.cap;
-      for (int s = lane; s < nq; s += kLanes) cp[s] = make_uint4(0, 0, 0, 0);
-    }
+    // ---- occurrences per read at this mer length ----
     uint32_t carry = 0;
     if (lane == 0) S.cum[0] = 0;
     for (int i0 = 0; i0 < nr; i0 += kLanes) {
@@ -225,120 +236,222 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
       carry += __shfl(x, kLanes - 1, kLanes);
     }
     const uint32_t N = carry;
-    __threadfence();
-    __syncthreads();
-    long long t1 = clock64();
-    cyc[0] += (unsigned long long)(t1 - t0);
-    t0 = t1;
-
-    // ---- count: lane takes occurrences [q0, q1) of the side's flattened list ----
     const uint64_t Bk1 = pow_base(k - 1);
-    {
-      const uint32_t per = (N + kLanes - 1) / kLanes;
-      const uint32_t q0 = lane * per, q1 = min(N, q0 + per);
-      if (q0 < q1) {
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (S.cum[mid] <= q0) lo = mid; else hi = mid - 1;
-        }
-        int i = lo, pos = (int)(q0 - S.cum[i]);
-        const uint8_t* rd = S.seq + S.roff[i];
-        const uint8_t* rq = S.qual + S.roff[i];
-        uint64_t h = hash_window(rd + pos, k);
-        int won_prev = -1;  // slot this lane claimed for the previous occurrence, if its next is this one
-        for (uint32_t q = q0;;) {
-          bool won = false;
-          const uint8_t* mine = rd + pos;
-          // the successor hint below is written by this lane, so the flag is set
-          // only when this lane also takes the next occurrence of the read
-          const bool next_here = pos + 1 < S.rlen[i] - k && q + 1 < q1;
-          const uint64_t low = (uint64_t)(uint32_t)(mine - S.seq) | (next_here ? kHasNext : 0ull);
-          const int s = find_slot(S, h, low, true, &won, [&](const uint8_t* o) { return same_window<12>(o, mine, k); });
-          if (s == -2) atomicAdd(&v.counters[3], 1ull);
-          if (won_prev >= 0 && s >= 0) S.info[won_prev] = (uint32_t)s;
-          won_prev = (won && next_here) ? s : -1;
-          if (s >= 0) {
-            const uint8_t e = rd[pos + k];
-            const int b = vote_base(e);
-            if (b >= 0) {
-              const int qd = (int)(int8_t)rq[pos + k] - v.qual_offset;
-              if (S.wide) {
-                if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)s + b], 1u);
-                if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)s + 4 + b], 1u);
-              } else if (qd >= v.min_qual) {
-                // one word per base: votes in the low half, high-quality votes in the high half
-                atomicAdd(&S.cnt[4 * (int64_t)s + b], qd >= v.min_hi_qual ? 0x10001u : 1u);
-              } else if (qd >= v.min_hi_qual) {
-                atomicAdd(&S.cnt[4 * (int64_t)s + b], 0x10000u);
+    // Table size for this mer length: kLoadNum/16 slots per occurrence (the
+    // distinct mers are fewer: sequencing errors aside, reads overlap).  A
+    // table that proves too small (a probe run longer than kMaxProbes, or
+    // full) is cleared and the count redone at the allocated size, which
+    // holds every occurrence at most 80% full.
+    S.cap = (int)min((int64_t)S.cap_alloc, ((int64_t)N * kLoadNum / 16 + 64 + 63) & ~(int64_t)63);
+    for (;;) {
+      const bool bounded = S.cap < S.cap_alloc;
+      const int probes = bounded ? min(kMaxProbes, S.cap - 1) : S.cap - 1;
+      // info words in LDS when they fit beside the reads: the walk chases them
+      S.info = info_at + 4 * (int64_t)S.cap <= v.lds_bytes ? (uint32_t*)(lds_reads + info_at) : info_glb;
+      for (int s = lane; s < S.cap; s += kLanes) S.key[s] = kEmpty;
+      {
+        uint4* cp = (uint4*)S.cnt;
+        const int nq = S.wide ? 2 * S.cap : S.cap;
+        for (int s = lane; s < nq; s += kLanes) cp[s] = make_uint4(0, 0, 0, 0);
+      }
+      __threadfence();
+      __syncthreads();
+      long long t1 = clock64();
+      cyc[0] += (unsigned long long)(t1 - t0);
+      t0 = t1;
+
+      // ---- count: lane takes occurrences [q0, q1) of the side's flattened list ----
+      {
+        const uint32_t per = (N + kLanes - 1) / kLanes;
+        const uint32_t q0 = lane * per, q1 = min(N, q0 + per);
+        if (q0 < q1) {
+          int lo = 0, hi = nr - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (S.cum[mid] <= q0) lo = mid; else hi = mid - 1;
+          }
+          int i = lo, pos = (int)(q0 - S.cum[i]);
+          int run = S.rlen[i] - k;            // occurrences in read i
+          uint32_t ro = (uint32_t)S.roff[i];  // side-local offset of read i
+          uint64_t h = hash_window(S.seq + ro + pos, k);
+          int won_prev = -1;  // slot this lane claimed for the previous occurrence, if its next is this one
+          // kU occurrences at a time: their first probes, following bases and
+          // qualities load together, then their empty first probes are claimed
+          // together, so a lane waits on one round trip per batch, not per mer
+          for (uint32_t q = q0; q < q1; q += kU) {
+            // the table is too small at this size: every lane stops, to start again
+            if (__hip_atomic_load(&sh_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            const int nb = (int)min((uint32_t)kU, q1 - q);
+            uint32_t off[kU], fp[kU];
+            uint64_t w[kU];
+            int s[kU];
+            bool nh[kU], won[kU];
+            uint8_t eb[kU], qb[kU];
+  #pragma unroll
+            for (int u = 0; u < kU; u++) {
+              if (u >= nb) break;
+              off[u] = ro + (uint32_t)pos;
+              // the successor hint is written by this lane, so the flag is set
+              // only when this lane also takes the next occurrence of the read
+              const bool more = q + u + 1 < q1;
+              nh[u] = pos + 1 < run && more;
+              s[u] = home_slot(S, h, &fp[u]);
+              if (!more) break;
+              if (++pos >= run) {
+                do { i++; } while (S.rlen[i] - k <= 0);
+                run = S.rlen[i] - k;
+                ro = (uint32_t)S.roff[i];
+                pos = 0;
+                h = hash_window(S.seq + ro, k);
+              } else {
+                const uint8_t* rd = S.seq + ro + pos;
+                h = (h - (uint64_t)rd[-1] * Bk1) * kHashBase + rd[k - 1];
+              }
+            }
+  #pragma unroll
+            for (int u = 0; u < kU; u++) {
+              if (u >= nb) break;
+              w[u] = ld64(&S.key[s[u]]);
+              eb[u] = S.seq[off[u] + k];
+              qb[u] = S.qual[off[u] + k];
+            }
+  #pragma unroll
+            for (int u = 0; u < kU; u++) {
+              if (u >= nb) break;
+              const uint64_t mine = ((uint64_t)fp[u] << 32) | off[u] | (nh[u] ? kHasNext : 0ull);
+              won[u] = false;
+              if (w[u] == kEmpty) {
+                const uint64_t old =
+                    atomicCAS((unsigned long long*)&S.key[s[u]], (unsigned long long)kEmpty, (unsigned long long)mine);
+                won[u] = old == kEmpty;
+                w[u] = won[u] ? mine : old;
+              }
+            }
+  #pragma unroll
+            for (int u = 0; u < kU; u++) {
+              if (u >= nb) break;
+              const uint8_t* mine_at = S.seq + off[u];
+              auto same = [&](const uint8_t* o) { return same_window<12>(o, mine_at, k); };
+              int sl = s[u];
+              if (!won[u] && !((uint32_t)(w[u] >> 32) == fp[u] && same(S.occ(w[u])))) {
+                const uint64_t mine = ((uint64_t)fp[u] << 32) | off[u] | (nh[u] ? kHasNext : 0ull);
+                sl = probe_from(S, s[u] + 1 == S.cap ? 0 : s[u] + 1, probes, fp[u], mine, true, &won[u], same);
+              }
+              if (sl == -2) {
+                if (!bounded) {
+                  atomicAdd(&v.counters[3], 1ull);
+                } else {
+                  __hip_atomic_store(&sh_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                  break;
+                }
+              }
+              if (won_prev >= 0 && sl >= 0) S.info[won_prev] = (uint32_t)sl;
+              won_prev = (won[u] && nh[u]) ? sl : -1;
+              const int b = vote_base(eb[u]);
+              if (sl >= 0 && b >= 0) {
+                const int qd = (int)(int8_t)qb[u] - v.qual_offset;
+                if (S.wide) {
+                  if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)sl + b], 1u);
+                  if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)sl + 4 + b], 1u);
+                } else if (qd >= v.min_qual) {
+                  // one word per base: votes in the low half, high-quality votes in the high half
+                  atomicAdd(&S.cnt[4 * (int64_t)sl + b], qd >= v.min_hi_qual ? 0x10001u : 1u);
+                } else if (qd >= v.min_hi_qual) {
+                  atomicAdd(&S.cnt[4 * (int64_t)sl + b], 0x10000u);
+                }
               }
             }
           }
-          if (++q >= q1) break;
-          if (++pos >= S.rlen[i] - k) {
-            do { i++; } while (S.rlen[i] - k <= 0);
-            pos = 0;
-            rd = S.seq + S.roff[i];
-            rq = S.qual + S.roff[i];
-            h = hash_window(rd, k);
-          } else {
-            h = (h - (uint64_t)rd[pos - 1] * Bk1) * kHashBase + rd[pos - 1 + k];
-          }
         }
       }
+      __threadfence();
+      __syncthreads();
+      t1 = clock64();
+      cyc[1] += (unsigned long long)(t1 - t0);
+      t0 = t1;
+      const bool over = sh_over != 0;
+      __syncthreads();  // every lane has read the flag before it is cleared
+      if (!over) break;
+      if (lane == 0) sh_over = 0;
+      S.cap = S.cap_alloc;
+      resizes++;
     }
     mers += N;
-    __threadfence();
-    __syncthreads();
-    t1 = clock64();
-    cyc[1] += (unsigned long long)(t1 - t0);
-    t0 = t1;
+    long long t1;
 
     // ---- decide: extension and successor of every occupied slot ----
     // Keys and votes no longer change, and the fence above invalidated this
     // CU's L1: plain (vector) loads read what the atomics left in L2.
-    for (int s = lane; s < S.cap; s += kLanes) {
-      const uint64_t w = S.key[s];
-      if (w == kEmpty) continue;
-      uint32_t low[4], hiq[4];
-      if (S.wide) {
-        const uint4 a = *(const uint4*)(S.cnt + 8 * (int64_t)s), b4 = *(const uint4*)(S.cnt + 8 * (int64_t)s + 4);
-        low[0] = a.x;
-        low[1] = a.y;
-        low[2] = a.z;
-        low[3] = a.w;
-        hiq[0] = b4.x;
-        hiq[1] = b4.y;
-        hiq[2] = b4.z;
-        hiq[3] = b4.w;
-      } else {
-        const uint4 a = *(const uint4*)(S.cnt + 4 * (int64_t)s);
-        const uint32_t wv[4] = {a.x, a.y, a.z, a.w};
-        for (int b = 0; b < 4; b++) {
-          low[b] = wv[b] & 0xFFFFu;
-          hiq[b] = wv[b] >> 16;
+    // kD slots per lane at a time: keys, votes and hints load together
+    for (int s0 = lane; s0 < S.cap; s0 += kLanes * kD) {
+      uint64_t w[kD];
+      uint4 va[kD], vb[kD];
+      uint32_t hint[kD];
+      int ext[kD];
+      uint8_t nextc[kD];
+#pragma unroll
+      for (int u = 0; u < kD; u++) {
+        const int s = s0 + u * kLanes;
+        w[u] = kEmpty;
+        if (s < S.cap) {
+          w[u] = S.key[s];
+          hint[u] = S.info[s];
+          if (S.wide) {
+            va[u] = *(const uint4*)(S.cnt + 8 * (int64_t)s);
+            vb[u] = *(const uint4*)(S.cnt + 8 * (int64_t)s + 4);
+          } else {
+            va[u] = *(const uint4*)(S.cnt + 4 * (int64_t)s);
+          }
         }
       }
-      const int ext = choose_ext(low, hiq, depth, v.rules);
-      uint32_t succ = kNoSlot;
-      if (ext < 4) {
-        const uint8_t* rep = S.occ(w);
-        const uint8_t ec = (uint8_t)ext_char(ext);
-        if ((w & kHasNext) && rep[k] == ec) {
-          // the stored occurrence continues with the chosen base: the next
-          // occurrence in its read is the successor mer, and the lane that
-          // stored it left that occurrence's slot in the info word
-          succ = S.info[s];
+#pragma unroll
+      for (int u = 0; u < kD; u++) {
+        ext[u] = kExtX;
+        if (w[u] == kEmpty) continue;
+        uint32_t low[4], hiq[4];
+        if (S.wide) {
+          low[0] = va[u].x;
+          low[1] = va[u].y;
+          low[2] = va[u].z;
+          low[3] = va[u].w;
+          hiq[0] = vb[u].x;
+          hiq[1] = vb[u].y;
+          hiq[2] = vb[u].z;
+          hiq[3] = vb[u].w;
         } else {
-          const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
-          bool won = false;
-          const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
-            return same_window<12>(o, rep + 1, k - 1) && o[k - 1] == ec;
-          });
-          if (t >= 0) succ = (uint32_t)t;
+          const uint32_t wv[4] = {va[u].x, va[u].y, va[u].z, va[u].w};
+          for (int b = 0; b < 4; b++) {
+            low[b] = wv[b] & 0xFFFFu;
+            hiq[b] = wv[b] >> 16;
+          }
         }
+        ext[u] = choose_ext(low, hiq, depth, v.rules);
+        // the base after the stored occurrence, where its read continues
+        if (ext[u] < 4 && (w[u] & kHasNext)) nextc[u] = S.occ(w[u])[k];
       }
-      S.info[s] = (uint32_t)ext | (succ << 3);
+#pragma unroll
+      for (int u = 0; u < kD; u++) {
+        if (w[u] == kEmpty) continue;
+        uint32_t succ = kNoSlot;
+        if (ext[u] < 4) {
+          const uint8_t* rep = S.occ(w[u]);
+          const uint8_t ec = (uint8_t)ext_char(ext[u]);
+          if ((w[u] & kHasNext) && nextc[u] == ec) {
+            // the stored occurrence continues with the chosen base: the next
+            // occurrence in its read is the successor mer, and the lane that
+            // stored it left that occurrence's slot in the info word
+            succ = hint[u];
+          } else {
+            const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
+            bool won = false;
+            const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
+              return same_window<12>(o, rep + 1, k - 1) && o[k - 1] == ec;
+            });
+            if (t >= 0) succ = (uint32_t)t;
+          }
+        }
+        S.info[s0 + u * kLanes] = (uint32_t)ext[u] | (succ << 3);
+      }
     }
     __threadfence();
     __syncthreads();
@@ -421,6 +534,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     atomicAdd(&v.counters[5], cyc[1]);
     atomicAdd(&v.counters[6], cyc[2]);
     atomicAdd(&v.counters[7], cyc[3]);
+    atomicAdd(&v.counters[8], resizes);
   }
 }
 

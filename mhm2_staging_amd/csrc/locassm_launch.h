@@ -47,7 +47,8 @@ struct WalkView {
   uint8_t* out_ext;            // walk_len_limit bytes per job
   int32_t* out_terms;          // 3 per job: walks ending in X, F, R
   unsigned long long* counters;  // [0] mers counted, [1] walk steps, [2] walks, [3] full tables,
-                                 // [4..7] shader cycles in clear, count, decide, walk (summed over sides)
+                                 // [4..7] shader cycles in clear, count, decide, walk (summed over sides),
+                                 // [8] counts redone in a larger table
   // parameters
   int32_t kmer_len, min_kmer_len, max_kmer_len, shift, walk_len_limit;
   int32_t qual_offset, min_qual, min_hi_qual, max_reads;

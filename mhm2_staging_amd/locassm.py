@@ -35,7 +35,8 @@ class Params(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("job_ms", ctypes.c_double), ("contigs", ctypes.c_int64), ("sides", ctypes.c_int64),
                 ("launches", ctypes.c_int64), ("mers", ctypes.c_int64), ("walk_steps", ctypes.c_int64),
-                ("walks", ctypes.c_int64), ("scratch_bytes", ctypes.c_int64), ("phase_cycles", ctypes.c_int64 * 4)]
+                ("walks", ctypes.c_int64), ("scratch_bytes", ctypes.c_int64), ("phase_cycles", ctypes.c_int64 * 4),
+                ("table_resizes", ctypes.c_int64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

@@ -127,6 +127,27 @@ def test_edge_cases(assembler):
         check(assembler, b, **kw)
 
 
+def test_tables_resized_when_mers_are_distinct(assembler):
+    """Sides whose reads share almost no mers overflow a table sized for
+    overlapping reads: the count is redone at the allocated size, and the
+    walks are the oracle's."""
+    rng = np.random.default_rng(55)
+    ctgs = []
+    for _ in range(80):
+        g = rand_seq(rng, 1400)
+        c = 600
+        true_reads = [(g[c - 100 + 13 * j:c + 50 + 13 * j], "I" * 150) for j in range(int(rng.integers(1, 8)))]
+        junk = [(rand_seq(rng, 150), "I" * 150) for _ in range(int(rng.integers(3, 70)))]
+        right = true_reads + junk
+        rng.shuffle(right)
+        left = [(rand_seq(rng, 120), "I" * 120) for _ in range(int(rng.integers(0, 30)))]
+        ctgs.append((g[:c], float(rng.uniform(2, 30)), left, right))
+    b = ld.from_contigs(ctgs)
+    for k in (33, 21):
+        check(assembler, b, kmer_len=k, walk_len_limit=300)
+        assert assembler.stats()["table_resizes"] > 0
+
+
 def test_restaged_and_rerun(assembler):
     """A staged batch runs again with other parameters; a new batch replaces it."""
     b1 = ld.generate(600, seed=31)
