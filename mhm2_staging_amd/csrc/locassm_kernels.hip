@@ -18,36 +18,56 @@
 //      the slot of the mer that follows it (the mer the walk would look up
 //      next), found with the rolled hash of the slot's own occurrence;
 //   3. walk: one lane follows successor links from the slot of the contig's
-//      last mer.  A step is one atomic OR on the slot's info word, which
-//      reads the extension and successor and marks the slot visited: a
-//      visited slot is a repeat (the reference's repeat table holds exactly
-//      the mers of the walk so far, each of which is a slot).
+//      last mer.  A step reads the slot's info word (extension and successor)
+//      and marks the slot visited: a visited slot is a repeat (the
+//      reference's repeat table holds exactly the mers of the walk so far,
+//      each of which is a slot).
 //
-// Slot words, counts and info words change through atomics (performed at
-// L2), so every read of them goes through L2 (relaxed agent-scope atomic
-// loads).
+// Slot words and counts change through atomics during a phase, so reads of
+// them in that phase are relaxed atomic loads.
+// A side whose reads fit the workgroup's LDS keeps them there, and a table
+// whose info words fit beside them keeps those there too.  Pointers carry
+// their address space in their type (side_walk<AS>), so LDS data is read
+// with ds_read and not with flat loads, which go through the texture
+// address path like global loads.
 #include "locassm_launch.h"
 
 namespace mhmla {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kU = 4;  // mer occurrences per lane per count batch
-constexpr int kD = 4;  // slots per lane per decide batch
+constexpr int kU = 4;           // mer occurrences per lane per count batch
+constexpr int kD = 4;           // slots per lane per decide batch
 constexpr int kLoadNum = 16;    // table slots per 16 mer occurrences, before a resize
 constexpr int kMaxProbes = 96;  // longest probe run in a table below its allocated size
 
+constexpr int kGlobalAS = 1;
+constexpr int kLdsAS = 3;
+template <int AS> using cu8 = const __attribute__((address_space(AS))) uint8_t;
+template <int AS> using cu32 = const __attribute__((address_space(AS))) uint32_t;
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+using glb_u32 = __attribute__((address_space(1))) uint32_t;
+
+extern __shared__ __align__(16) uint8_t lds_side[];  // dynamic LDS: reads, then info words
+
+// A side is one workgroup of one wavefront, and every access to its scratch
+// and LDS comes from that wavefront, on one CU: ordering at workgroup scope
+// is all the phases need.  (Agent scope would write back and invalidate the
+// L2 at every fence.)  Atomics are performed at L2 through the CU's L1, which
+// keeps the L1 coherent with them for the workgroup.
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
 __device__ __forceinline__ uint64_t ld64(uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Bytes a[0..n) and b[0..n) equal?  Aligned dword loads funnel-shifted with
 // v_alignbyte, C dwords per round trip, no branch inside a round; reads up to
 // 7 bytes past either window (every buffer read here is padded).
-template <int C>
-__device__ __forceinline__ bool same_window(const uint8_t* a, const uint8_t* b, int n) {
-  const uint32_t* pa = (const uint32_t*)((uintptr_t)a & ~(uintptr_t)3);
-  const uint32_t* pb = (const uint32_t*)((uintptr_t)b & ~(uintptr_t)3);
+template <int C, int AA, int AB>
+__device__ __forceinline__ bool same_window(cu8<AA>* a, cu8<AB>* b, int n) {
+  cu32<AA>* pa = (cu32<AA>*)((uintptr_t)a & ~(uintptr_t)3);
+  cu32<AB>* pb = (cu32<AB>*)((uintptr_t)b & ~(uintptr_t)3);
   const uint32_t sa = (uint32_t)((uintptr_t)a & 3), sb = (uint32_t)((uintptr_t)b & 3);
   const int nw = (n + 3) >> 2;
   for (int g = 0; g < nw; g += C) {
@@ -73,8 +93,9 @@ __device__ __forceinline__ bool same_window(const uint8_t* a, const uint8_t* b, 
 }
 
 // polynomial hash of s[0..k) (bytes in order), from aligned dword loads
-__device__ __forceinline__ uint64_t hash_window(const uint8_t* s, int k) {
-  const uint32_t* p = (const uint32_t*)((uintptr_t)s & ~(uintptr_t)3);
+template <int AS>
+__device__ __forceinline__ uint64_t hash_window(cu8<AS>* s, int k) {
+  cu32<AS>* p = (cu32<AS>*)((uintptr_t)s & ~(uintptr_t)3);
   const uint32_t sh = (uint32_t)((uintptr_t)s & 3);
   const int nw = (k + 3) >> 2;
   uint64_t h = 0;
@@ -89,12 +110,16 @@ __device__ __forceinline__ uint64_t hash_window(const uint8_t* s, int k) {
   return h;
 }
 
+// One side's data; RAS is the address space of its read bases.
+template <int RAS>
 struct Side {
-  const uint8_t* seq;   // all read bases
-  const uint8_t* qual;  // all read quality bytes
+  cu8<RAS>* seq;        // the side's read bases
+  const uint8_t* qual;  // the side's read quality bytes (global)
   uint64_t* key;
   uint32_t* cnt;        // per slot: 4 words of 16-bit vote pairs, or 8 words (wide)
-  uint32_t* info;
+  glb_u32* info_g;      // info words in the scratch
+  lds_u32* info_l;      // info words in LDS, when info_lds
+  bool info_lds;
   int64_t* roff;        // offsets of the side's reads
   int32_t* rlen;
   uint32_t* cum;        // occurrences before read i, at the current mer length
@@ -105,12 +130,17 @@ struct Side {
   int nr;
   bool wide;            // 32-bit vote counters (sides with 65535 mers or more)
 
-
-  __device__ const uint8_t* occ(uint64_t w) const { return seq + ((uint32_t)w & kOffMask); }
+  __device__ cu8<RAS>* occ(uint64_t w) const { return seq + ((uint32_t)w & kOffMask); }
+  __device__ uint32_t info_get(int s) const { return info_lds ? info_l[s] : info_g[s]; }
+  __device__ void info_put(int s, uint32_t x) const {
+    if (info_lds) info_l[s] = x;
+    else info_g[s] = x;
+  }
 };
 
 // First probe slot and fingerprint of a mer with rolled hash h.
-__device__ __forceinline__ int home_slot(const Side& S, uint64_t h, uint32_t* fp) {
+template <int RAS>
+__device__ __forceinline__ int home_slot(const Side<RAS>& S, uint64_t h, uint32_t* fp) {
   const uint64_t m = mix64(h);
   *fp = (uint32_t)(m >> 32);
   return (int)(((uint64_t)(uint32_t)m * (uint64_t)S.cap) >> 32);
@@ -120,8 +150,8 @@ __device__ __forceinline__ int home_slot(const Side& S, uint64_t h, uint32_t* fp
 // at most 'probes' slots; insert 'mine' (fingerprint | byte offset of the
 // occurrence) into the first empty slot when insert is set.
 // Returns the slot, -1 when absent and not inserted, -2 if the table is full.
-template <class Same>
-__device__ int probe_from(const Side& S, int s, int probes, uint32_t fp, uint64_t mine, bool insert, bool* won,
+template <int RAS, class Same>
+__device__ int probe_from(const Side<RAS>& S, int s, int probes, uint32_t fp, uint64_t mine, bool insert, bool* won,
                           Same same) {
   for (; probes > 0; probes--) {
     // read first: most probes find an occupied slot (a mer seen before), and
@@ -142,8 +172,8 @@ __device__ int probe_from(const Side& S, int s, int probes, uint32_t fp, uint64_
 }
 
 // Find (and with insert, claim) the slot of the mer with rolled hash h.
-template <class Same>
-__device__ int find_slot(const Side& S, uint64_t h, uint64_t mine_low, bool insert, bool* won, Same same) {
+template <int RAS, class Same>
+__device__ int find_slot(const Side<RAS>& S, uint64_t h, uint64_t mine_low, bool insert, bool* won, Same same) {
   uint32_t fp;
   const int s = home_slot(S, h, &fp);
   return probe_from(S, s, S.cap, fp, ((uint64_t)fp << 32) | mine_low, insert, won, same);
@@ -153,64 +183,60 @@ __device__ __forceinline__ uint8_t side_byte(const uint8_t* ctg, int L, bool rig
   return right ? ctg[i] : complement(ctg[L - 1 - i]);
 }
 
-__global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
+// Every mer length the schedule visits for one contig side, then the side's
+// output.  RAS = kLdsAS: the reads are staged into lds_side first.
+template <int RAS>
+__device__ __forceinline__ void side_walk(const WalkView& v, int slot, int job, int nr, int64_t f0, int64_t base,
+                                          int64_t side_bytes) {
   __shared__ int sh_t, sh_res, sh_over;
   const int lane = threadIdx.x;
-  const int slot = blockIdx.x;
-  const int job = v.jobs[slot];
   const int c = job >> 1;
   const bool right = (job & 1) != 0;
   const int L = v.ctg_len[c];
   const uint8_t* ctg = v.ctg_seq + v.ctg_off[c];
-  const int64_t f0 = v.rfirst[job];
-  const int nr = (int)min(v.rfirst[job + 1] - f0, (int64_t)v.max_reads);
   const int W = v.walk_len_limit;
 
-  Side S;
-  extern __shared__ __align__(16) uint8_t lds_reads[];
+  Side<RAS> S;
   S.cap_alloc = v.cap[slot] & kCapMask;
   S.cap = S.cap_alloc;
   S.wide = (v.cap[slot] & kWideCounts) != 0;
   S.nr = nr;
-  uint32_t* info_glb;
   {
-    uint8_t* base = v.scratch + v.scratch_off[slot];
+    uint8_t* sb = v.scratch + v.scratch_off[slot];
     const JobLayout lay = job_layout(S.cap_alloc, S.wide, nr, W, v.max_kmer_len);
-    S.key = (uint64_t*)(base + lay.key);
-    S.cnt = (uint32_t*)(base + lay.cnt);
-    S.info = (uint32_t*)(base + lay.info);
-    S.roff = (int64_t*)(base + lay.roff);
-    S.rlen = (int32_t*)(base + lay.rlen);
-    S.cum = (uint32_t*)(base + lay.cum);
-    S.walk[0] = base + lay.walk0;
-    S.walk[1] = base + lay.walk1;
-    S.smer = base + lay.smer;
+    S.key = (uint64_t*)(sb + lay.key);
+    S.cnt = (uint32_t*)(sb + lay.cnt);
+    S.info_g = (glb_u32*)(sb + lay.info);
+    S.roff = (int64_t*)(sb + lay.roff);
+    S.rlen = (int32_t*)(sb + lay.rlen);
+    S.cum = (uint32_t*)(sb + lay.cum);
+    S.walk[0] = sb + lay.walk0;
+    S.walk[1] = sb + lay.walk1;
+    S.smer = sb + lay.smer;
   }
   // the side's reads are contiguous from base; offsets below are relative to it
-  const int64_t base = nr > 0 ? v.read_off[f0] : 0;
-  const int64_t side_bytes = nr > 0 ? v.read_off[f0 + nr - 1] + v.read_len[f0 + nr - 1] - base : 0;
   for (int i = lane; i < nr; i += kLanes) {
     S.roff[i] = v.read_off[f0 + i] - base;
     S.rlen[i] = v.read_len[f0 + i];
   }
-  info_glb = S.info;
   S.qual = v.read_qual + base;
   int64_t info_at = 0;  // LDS bytes taken by the reads
-  if (side_bytes + 64 <= v.lds_bytes) {
+  if constexpr (RAS == kLdsAS) {
     // small side: its bases in LDS (every compare and hash below reads them)
     const uint32_t* src = (const uint32_t*)((uintptr_t)(v.read_seq + base) & ~(uintptr_t)3);
     const int sh = (int)((uintptr_t)(v.read_seq + base) & 3);
     const int nw = (int)((side_bytes + 3) >> 2);
-    uint32_t* dst = (uint32_t*)lds_reads;
+    lds_u32* dst = (lds_u32*)lds_side;
     for (int j = lane; j < nw; j += kLanes) dst[j] = __builtin_amdgcn_alignbyte(src[j + 1], src[j], sh);
     for (int j = nw + lane; j < nw + 16; j += kLanes) dst[j] = 0;
-    S.seq = lds_reads;
+    S.seq = (cu8<RAS>*)lds_side;
     info_at = (side_bytes + 64 + 15) & ~(int64_t)15;
   } else {
-    S.seq = v.read_seq + base;
+    S.seq = (cu8<RAS>*)(v.read_seq + base);
   }
+  S.info_l = (lds_u32*)(lds_side + info_at);
   if (lane == 0) sh_over = 0;
-  __threadfence();
+  wave_fence();
   __syncthreads();
 
   const int depth = v.depth[c];
@@ -247,124 +273,122 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
       const bool bounded = S.cap < S.cap_alloc;
       const int probes = bounded ? min(kMaxProbes, S.cap - 1) : S.cap - 1;
       // info words in LDS when they fit beside the reads: the walk chases them
-      S.info = info_at + 4 * (int64_t)S.cap <= v.lds_bytes ? (uint32_t*)(lds_reads + info_at) : info_glb;
+      S.info_lds = info_at + 4 * (int64_t)S.cap <= v.lds_bytes;
       for (int s = lane; s < S.cap; s += kLanes) S.key[s] = kEmpty;
       {
         uint4* cp = (uint4*)S.cnt;
         const int nq = S.wide ? 2 * S.cap : S.cap;
         for (int s = lane; s < nq; s += kLanes) cp[s] = make_uint4(0, 0, 0, 0);
       }
-      __threadfence();
+      wave_fence();
       __syncthreads();
       long long t1 = clock64();
       cyc[0] += (unsigned long long)(t1 - t0);
       t0 = t1;
 
       // ---- count: lane takes occurrences [q0, q1) of the side's flattened list ----
-      {
-        const uint32_t per = (N + kLanes - 1) / kLanes;
-        const uint32_t q0 = lane * per, q1 = min(N, q0 + per);
-        if (q0 < q1) {
-          int lo = 0, hi = nr - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (S.cum[mid] <= q0) lo = mid; else hi = mid - 1;
+      const uint32_t per = (N + kLanes - 1) / kLanes;
+      const uint32_t q0 = lane * per, q1 = min(N, q0 + per);
+      if (q0 < q1) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (S.cum[mid] <= q0) lo = mid; else hi = mid - 1;
+        }
+        int i = lo, pos = (int)(q0 - S.cum[i]);
+        int run = S.rlen[i] - k;            // occurrences in read i
+        uint32_t ro = (uint32_t)S.roff[i];  // side-local offset of read i
+        uint64_t h = hash_window<RAS>(S.seq + ro + pos, k);
+        int won_prev = -1;  // slot this lane claimed for the previous occurrence, if its next is this one
+        // kU occurrences at a time: their first probes, following bases and
+        // qualities load together, then their empty first probes are claimed
+        // together, so a lane waits on one round trip per batch, not per mer
+        for (uint32_t q = q0; q < q1; q += kU) {
+          // the table is too small at this size: every lane stops, to start again
+          if (__hip_atomic_load(&sh_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+          const int nb = (int)min((uint32_t)kU, q1 - q);
+          uint32_t off[kU], fp[kU];
+          uint64_t w[kU];
+          int s[kU];
+          bool nh[kU], won[kU];
+          uint8_t eb[kU], qb[kU];
+#pragma unroll
+          for (int u = 0; u < kU; u++) {
+            if (u >= nb) break;
+            off[u] = ro + (uint32_t)pos;
+            // the successor hint is written by this lane, so the flag is set
+            // only when this lane also takes the next occurrence of the read
+            const bool more = q + u + 1 < q1;
+            nh[u] = pos + 1 < run && more;
+            s[u] = home_slot(S, h, &fp[u]);
+            if (!more) break;
+            if (++pos >= run) {
+              do { i++; } while (S.rlen[i] - k <= 0);
+              run = S.rlen[i] - k;
+              ro = (uint32_t)S.roff[i];
+              pos = 0;
+              h = hash_window<RAS>(S.seq + ro, k);
+            } else {
+              cu8<RAS>* rd = S.seq + ro + pos;
+              h = (h - (uint64_t)rd[-1] * Bk1) * kHashBase + rd[k - 1];
+            }
           }
-          int i = lo, pos = (int)(q0 - S.cum[i]);
-          int run = S.rlen[i] - k;            // occurrences in read i
-          uint32_t ro = (uint32_t)S.roff[i];  // side-local offset of read i
-          uint64_t h = hash_window(S.seq + ro + pos, k);
-          int won_prev = -1;  // slot this lane claimed for the previous occurrence, if its next is this one
-          // kU occurrences at a time: their first probes, following bases and
-          // qualities load together, then their empty first probes are claimed
-          // together, so a lane waits on one round trip per batch, not per mer
-          for (uint32_t q = q0; q < q1; q += kU) {
-            // the table is too small at this size: every lane stops, to start again
-            if (__hip_atomic_load(&sh_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-            const int nb = (int)min((uint32_t)kU, q1 - q);
-            uint32_t off[kU], fp[kU];
-            uint64_t w[kU];
-            int s[kU];
-            bool nh[kU], won[kU];
-            uint8_t eb[kU], qb[kU];
-  #pragma unroll
-            for (int u = 0; u < kU; u++) {
-              if (u >= nb) break;
-              off[u] = ro + (uint32_t)pos;
-              // the successor hint is written by this lane, so the flag is set
-              // only when this lane also takes the next occurrence of the read
-              const bool more = q + u + 1 < q1;
-              nh[u] = pos + 1 < run && more;
-              s[u] = home_slot(S, h, &fp[u]);
-              if (!more) break;
-              if (++pos >= run) {
-                do { i++; } while (S.rlen[i] - k <= 0);
-                run = S.rlen[i] - k;
-                ro = (uint32_t)S.roff[i];
-                pos = 0;
-                h = hash_window(S.seq + ro, k);
-              } else {
-                const uint8_t* rd = S.seq + ro + pos;
-                h = (h - (uint64_t)rd[-1] * Bk1) * kHashBase + rd[k - 1];
-              }
+#pragma unroll
+          for (int u = 0; u < kU; u++) {
+            if (u >= nb) break;
+            w[u] = ld64(&S.key[s[u]]);
+            eb[u] = S.seq[off[u] + k];
+            qb[u] = S.qual[off[u] + k];
+          }
+#pragma unroll
+          for (int u = 0; u < kU; u++) {
+            if (u >= nb) break;
+            const uint64_t mine = ((uint64_t)fp[u] << 32) | off[u] | (nh[u] ? kHasNext : 0ull);
+            won[u] = false;
+            if (w[u] == kEmpty) {
+              const uint64_t old =
+                  atomicCAS((unsigned long long*)&S.key[s[u]], (unsigned long long)kEmpty, (unsigned long long)mine);
+              won[u] = old == kEmpty;
+              w[u] = won[u] ? mine : old;
             }
-  #pragma unroll
-            for (int u = 0; u < kU; u++) {
-              if (u >= nb) break;
-              w[u] = ld64(&S.key[s[u]]);
-              eb[u] = S.seq[off[u] + k];
-              qb[u] = S.qual[off[u] + k];
-            }
-  #pragma unroll
-            for (int u = 0; u < kU; u++) {
-              if (u >= nb) break;
+          }
+#pragma unroll
+          for (int u = 0; u < kU; u++) {
+            if (u >= nb) break;
+            cu8<RAS>* mine_at = S.seq + off[u];
+            auto same = [&](cu8<RAS>* o) { return same_window<12, RAS, RAS>(o, mine_at, k); };
+            int sl = s[u];
+            if (!won[u] && !((uint32_t)(w[u] >> 32) == fp[u] && same(S.occ(w[u])))) {
               const uint64_t mine = ((uint64_t)fp[u] << 32) | off[u] | (nh[u] ? kHasNext : 0ull);
-              won[u] = false;
-              if (w[u] == kEmpty) {
-                const uint64_t old =
-                    atomicCAS((unsigned long long*)&S.key[s[u]], (unsigned long long)kEmpty, (unsigned long long)mine);
-                won[u] = old == kEmpty;
-                w[u] = won[u] ? mine : old;
+              sl = probe_from(S, s[u] + 1 == S.cap ? 0 : s[u] + 1, probes, fp[u], mine, true, &won[u], same);
+            }
+            if (sl == -2) {
+              if (!bounded) {
+                atomicAdd(&v.counters[3], 1ull);
+              } else {
+                __hip_atomic_store(&sh_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                break;
               }
             }
-  #pragma unroll
-            for (int u = 0; u < kU; u++) {
-              if (u >= nb) break;
-              const uint8_t* mine_at = S.seq + off[u];
-              auto same = [&](const uint8_t* o) { return same_window<12>(o, mine_at, k); };
-              int sl = s[u];
-              if (!won[u] && !((uint32_t)(w[u] >> 32) == fp[u] && same(S.occ(w[u])))) {
-                const uint64_t mine = ((uint64_t)fp[u] << 32) | off[u] | (nh[u] ? kHasNext : 0ull);
-                sl = probe_from(S, s[u] + 1 == S.cap ? 0 : s[u] + 1, probes, fp[u], mine, true, &won[u], same);
-              }
-              if (sl == -2) {
-                if (!bounded) {
-                  atomicAdd(&v.counters[3], 1ull);
-                } else {
-                  __hip_atomic_store(&sh_over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                  break;
-                }
-              }
-              if (won_prev >= 0 && sl >= 0) S.info[won_prev] = (uint32_t)sl;
-              won_prev = (won[u] && nh[u]) ? sl : -1;
-              const int b = vote_base(eb[u]);
-              if (sl >= 0 && b >= 0) {
-                const int qd = (int)(int8_t)qb[u] - v.qual_offset;
-                if (S.wide) {
-                  if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)sl + b], 1u);
-                  if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)sl + 4 + b], 1u);
-                } else if (qd >= v.min_qual) {
-                  // one word per base: votes in the low half, high-quality votes in the high half
-                  atomicAdd(&S.cnt[4 * (int64_t)sl + b], qd >= v.min_hi_qual ? 0x10001u : 1u);
-                } else if (qd >= v.min_hi_qual) {
-                  atomicAdd(&S.cnt[4 * (int64_t)sl + b], 0x10000u);
-                }
+            if (won_prev >= 0 && sl >= 0) S.info_put(won_prev, (uint32_t)sl);
+            won_prev = (won[u] && nh[u]) ? sl : -1;
+            const int b = vote_base(eb[u]);
+            if (sl >= 0 && b >= 0) {
+              const int qd = (int)(int8_t)qb[u] - v.qual_offset;
+              if (S.wide) {
+                if (qd >= v.min_qual) atomicAdd(&S.cnt[8 * (int64_t)sl + b], 1u);
+                if (qd >= v.min_hi_qual) atomicAdd(&S.cnt[8 * (int64_t)sl + 4 + b], 1u);
+              } else if (qd >= v.min_qual) {
+                // one word per base: votes in the low half, high-quality votes in the high half
+                atomicAdd(&S.cnt[4 * (int64_t)sl + b], qd >= v.min_hi_qual ? 0x10001u : 1u);
+              } else if (qd >= v.min_hi_qual) {
+                atomicAdd(&S.cnt[4 * (int64_t)sl + b], 0x10000u);
               }
             }
           }
         }
       }
-      __threadfence();
+      wave_fence();
       __syncthreads();
       t1 = clock64();
       cyc[1] += (unsigned long long)(t1 - t0);
@@ -377,11 +401,9 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
       resizes++;
     }
     mers += N;
-    long long t1;
 
     // ---- decide: extension and successor of every occupied slot ----
-    // Keys and votes no longer change, and the fence above invalidated this
-    // CU's L1: plain (vector) loads read what the atomics left in L2.
+    // Keys and votes no longer change: plain (vector) loads.
     // kD slots per lane at a time: keys, votes and hints load together
     for (int s0 = lane; s0 < S.cap; s0 += kLanes * kD) {
       uint64_t w[kD];
@@ -395,7 +417,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
         w[u] = kEmpty;
         if (s < S.cap) {
           w[u] = S.key[s];
-          hint[u] = S.info[s];
+          hint[u] = S.info_get(s);
           if (S.wide) {
             va[u] = *(const uint4*)(S.cnt + 8 * (int64_t)s);
             vb[u] = *(const uint4*)(S.cnt + 8 * (int64_t)s + 4);
@@ -434,7 +456,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
         if (w[u] == kEmpty) continue;
         uint32_t succ = kNoSlot;
         if (ext[u] < 4) {
-          const uint8_t* rep = S.occ(w[u]);
+          cu8<RAS>* rep = S.occ(w[u]);
           const uint8_t ec = (uint8_t)ext_char(ext[u]);
           if ((w[u] & kHasNext) && nextc[u] == ec) {
             // the stored occurrence continues with the chosen base: the next
@@ -442,31 +464,33 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
             // stored it left that occurrence's slot in the info word
             succ = hint[u];
           } else {
-            const uint64_t hs = hash_window(rep + 1, k - 1) * kHashBase + ec;
+            const uint64_t hs = hash_window<RAS>(rep + 1, k - 1) * kHashBase + ec;
             bool won = false;
-            const int t = find_slot(S, hs, 0, false, &won, [&](const uint8_t* o) {
-              return same_window<12>(o, rep + 1, k - 1) && o[k - 1] == ec;
+            const int t = find_slot(S, hs, 0, false, &won, [&](cu8<RAS>* o) {
+              return same_window<12, RAS, RAS>(o, rep + 1, k - 1) && o[k - 1] == ec;
             });
             if (t >= 0) succ = (uint32_t)t;
           }
         }
-        S.info[s0 + u * kLanes] = (uint32_t)ext[u] | (succ << 3);
+        S.info_put(s0 + u * kLanes, (uint32_t)ext[u] | (succ << 3));
       }
     }
-    __threadfence();
+    wave_fence();
     __syncthreads();
-    t1 = clock64();
+    long long t1 = clock64();
     cyc[2] += (unsigned long long)(t1 - t0);
     t0 = t1;
 
     // ---- walk from the contig's last mer ----
     for (int j = lane; j < k; j += kLanes) S.smer[j] = side_byte(ctg, L, right, L - k + j);
-    __threadfence();
+    wave_fence();
     __syncthreads();
     if (lane == 0) {
-      const uint64_t h0 = hash_window(S.smer, k);
+      cu8<kGlobalAS>* smer = (cu8<kGlobalAS>*)S.smer;
+      const uint64_t h0 = hash_window<kGlobalAS>(smer, k);
       bool won = false;
-      int e = find_slot(S, h0, 0, false, &won, [&](const uint8_t* o) { return same_window<12>(o, S.smer, k); });
+      int e = find_slot(S, h0, 0, false, &won,
+                        [&](cu8<RAS>* o) { return same_window<12, RAS, kGlobalAS>(o, smer, k); });
       uint8_t* wb = cur ? S.walk[1] : S.walk[0];
       int t = 0, res = 0;  // 0 X, 1 F, 2 R
       while (t < W) {
@@ -474,7 +498,14 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
           res = 0;
           break;
         }
-        const uint32_t old = atomicOr(&S.info[e], kVisited);
+        // only this lane touches the info words now: read, then mark visited
+        uint32_t old;
+        if (S.info_lds) {
+          old = S.info_l[e];
+          S.info_l[e] = old | kVisited;
+        } else {
+          old = __hip_atomic_fetch_or(S.info_g + e, kVisited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
         if (old & kVisited) {
           res = 2;
           break;
@@ -491,7 +522,7 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
       sh_t = t;
       sh_res = res;
     }
-    __threadfence();
+    wave_fence();
     __syncthreads();
     const int t = sh_t, res = sh_res;
     t1 = clock64();
@@ -536,6 +567,17 @@ __global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
     atomicAdd(&v.counters[7], cyc[3]);
     atomicAdd(&v.counters[8], resizes);
   }
+}
+
+__global__ __launch_bounds__(kLanes) void walk_kernel(WalkView v) {
+  const int slot = blockIdx.x;
+  const int job = v.jobs[slot];
+  const int64_t f0 = v.rfirst[job];
+  const int nr = (int)min(v.rfirst[job + 1] - f0, (int64_t)v.max_reads);
+  const int64_t base = nr > 0 ? v.read_off[f0] : 0;
+  const int64_t side_bytes = nr > 0 ? v.read_off[f0 + nr - 1] + v.read_len[f0 + nr - 1] - base : 0;
+  if (side_bytes + 64 <= v.lds_bytes) side_walk<kLdsAS>(v, slot, job, nr, f0, base, side_bytes);
+  else side_walk<kGlobalAS>(v, slot, job, nr, f0, base, side_bytes);
 }
 
 }  // namespace
