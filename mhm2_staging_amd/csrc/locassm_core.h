@@ -95,7 +95,9 @@ MHMLA_HD int choose_ext(const uint32_t low[4], const uint32_t hi[4], int depth, 
   return e0.low > e1.low ? e0.base : e1.base;
 }
 
-MHMLA_HD char ext_char(int code) { return "ACGTFX"[code]; }
+// from a packed constant, not a string: on the GPU a string is a load from
+// global memory, on the walk's critical path
+MHMLA_HD char ext_char(int code) { return (char)((0x584654474341ull >> (8 * code)) & 0xFFu); }
 
 // base index of a vote byte: A C G T -> 0..3, everything else -> -1
 MHMLA_HD int vote_base(uint8_t c) {

@@ -46,6 +46,7 @@ constexpr int kLdsAS = 3;
 template <int AS> using cu8 = const __attribute__((address_space(AS))) uint8_t;
 template <int AS> using cu32 = const __attribute__((address_space(AS))) uint32_t;
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
+using lds_u16 = __attribute__((address_space(3))) uint16_t;
 using glb_u32 = __attribute__((address_space(1))) uint32_t;
 
 extern __shared__ __align__(16) uint8_t lds_side[];  // dynamic LDS: reads, then info words
@@ -179,6 +180,17 @@ __device__ int find_slot(const Side<RAS>& S, uint64_t h, uint64_t mine_low, bool
   return probe_from(S, s, S.cap, fp, ((uint64_t)fp << 32) | mine_low, insert, won, same);
 }
 
+// A decided info word in 16 bits, for a walk in LDS (tables of at most 8190
+// slots): successor slot << 2 | base, where successor 8190 means a base with no
+// successor and 8191 no base (low bits 1: a fork, 0: a dead end); bit 15 marks
+// a visited slot.
+constexpr uint32_t kW16NoSucc = 8190u, kW16End = 8191u, kW16Visited = 0x8000u;
+__device__ __forceinline__ uint16_t pack_info16(uint32_t x) {
+  const uint32_t ext = x & 7u, succ = (x >> 3) & kNoSlot;
+  if (ext >= 4) return (uint16_t)((kW16End << 2) | (ext == (uint32_t)kExtF ? 1u : 0u));
+  return (uint16_t)((((succ == kNoSlot ? kW16NoSucc : succ) & 8191u) << 2) | ext);
+}
+
 __device__ __forceinline__ uint8_t side_byte(const uint8_t* ctg, int L, bool right, int i) {
   return right ? ctg[i] : complement(ctg[L - 1 - i]);
 }
@@ -188,7 +200,7 @@ __device__ __forceinline__ uint8_t side_byte(const uint8_t* ctg, int L, bool rig
 template <int RAS>
 __device__ __forceinline__ void side_walk(const WalkView& v, int slot, int job, int nr, int64_t f0, int64_t base,
                                           int64_t side_bytes) {
-  __shared__ int sh_t, sh_res, sh_over;
+  __shared__ int sh_t, sh_res, sh_over, sh_e;
   const int lane = threadIdx.x;
   const int c = job >> 1;
   const bool right = (job & 1) != 0;
@@ -221,14 +233,17 @@ __device__ __forceinline__ void side_walk(const WalkView& v, int slot, int job, 
   }
   S.qual = v.read_qual + base;
   int64_t info_at = 0;  // LDS bytes taken by the reads
-  if constexpr (RAS == kLdsAS) {
-    // small side: its bases in LDS (every compare and hash below reads them)
+  // small side: its bases in LDS (every compare and hash below reads them)
+  auto stage_reads = [&]() {
     const uint32_t* src = (const uint32_t*)((uintptr_t)(v.read_seq + base) & ~(uintptr_t)3);
     const int sh = (int)((uintptr_t)(v.read_seq + base) & 3);
     const int nw = (int)((side_bytes + 3) >> 2);
     lds_u32* dst = (lds_u32*)lds_side;
     for (int j = lane; j < nw; j += kLanes) dst[j] = __builtin_amdgcn_alignbyte(src[j + 1], src[j], sh);
     for (int j = nw + lane; j < nw + 16; j += kLanes) dst[j] = 0;
+  };
+  if constexpr (RAS == kLdsAS) {
+    stage_reads();
     S.seq = (cu8<RAS>*)lds_side;
     info_at = (side_bytes + 64 + 15) & ~(int64_t)15;
   } else {
@@ -489,8 +504,21 @@ __device__ __forceinline__ void side_walk(const WalkView& v, int slot, int job, 
       cu8<kGlobalAS>* smer = (cu8<kGlobalAS>*)S.smer;
       const uint64_t h0 = hash_window<kGlobalAS>(smer, k);
       bool won = false;
-      int e = find_slot(S, h0, 0, false, &won,
-                        [&](cu8<RAS>* o) { return same_window<12, RAS, kGlobalAS>(o, smer, k); });
+      sh_e = find_slot(S, h0, 0, false, &won,
+                       [&](cu8<RAS>* o) { return same_window<12, RAS, kGlobalAS>(o, smer, k); });
+    }
+    // A table whose info words are in HBM but fit LDS as 16-bit entries is
+    // walked there: the walk needs no read bases once its first slot is found.
+    const bool walk16 = !S.info_lds && S.cap <= 8190 && 2 * (int64_t)S.cap <= v.lds_bytes;
+    lds_u16* w16 = (lds_u16*)lds_side;
+    if (walk16) {
+      __syncthreads();  // the first slot is found before the reads are overwritten
+      for (int s = lane; s < S.cap; s += kLanes) w16[s] = pack_info16(*(S.info_g + s));
+      wave_fence();
+      __syncthreads();
+    }
+    if (lane == 0) {
+      int e = sh_e;
       uint8_t* wb = cur ? S.walk[1] : S.walk[0];
       int t = 0, res = 0;  // 0 X, 1 F, 2 R
       while (t < W) {
@@ -499,6 +527,22 @@ __device__ __forceinline__ void side_walk(const WalkView& v, int slot, int job, 
           break;
         }
         // only this lane touches the info words now: read, then mark visited
+        if (walk16) {
+          const uint32_t x = w16[e];
+          w16[e] = (uint16_t)(x | kW16Visited);
+          if (x & kW16Visited) {
+            res = 2;
+            break;
+          }
+          const uint32_t f = (x >> 2) & 8191u, b = x & 3u;
+          if (f == kW16End) {
+            res = b == 1u ? 1 : 0;
+            break;
+          }
+          wb[t++] = (uint8_t)ext_char((int)b);
+          e = f == kW16NoSucc ? -1 : (int)f;
+          continue;
+        }
         uint32_t old;
         if (S.info_lds) {
           old = S.info_l[e];
@@ -524,6 +568,10 @@ __device__ __forceinline__ void side_walk(const WalkView& v, int slot, int job, 
     }
     wave_fence();
     __syncthreads();
+    // the next mer length counts from the reads again
+    if constexpr (RAS == kLdsAS) {
+      if (walk16) stage_reads();
+    }
     const int t = sh_t, res = sh_res;
     t1 = clock64();
     cyc[3] += (unsigned long long)(t1 - t0);
