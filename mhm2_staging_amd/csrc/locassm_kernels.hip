@@ -36,8 +36,8 @@ namespace mhmla {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kU = 4;           // mer occurrences per lane per count batch
-constexpr int kD = 4;           // slots per lane per decide batch
+constexpr int kU = 2;           // mer occurrences per lane per count batch
+constexpr int kD = 2;           // slots per lane per decide batch
 constexpr int kLoadNum = 16;    // table slots per 16 mer occurrences, before a resize
 constexpr int kMaxProbes = 96;  // longest probe run in a table below its allocated size
 
