@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mhmla.h"
@@ -46,6 +47,53 @@ struct Buf {
   }
 };
 
+// Pinned host memory, grown as needed and kept: staging copies go from it at
+// the link's full rate.
+struct HostPinned {
+  uint8_t* ptr = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(n + n / 8, 4096);
+    const hipError_t e = hipHostMalloc((void**)&ptr, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+// fn(t, begin, end) over [0, n) split in contiguous parts, on up to 16
+// threads (one below 'serial_below' items)
+template <class Fn>
+void parallel_parts(int64_t n, int64_t serial_below, Fn fn) {
+  int nth = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < serial_below) nth = 1;
+  if (nth == 1) {
+    fn(0, int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; t++) th.emplace_back(fn, t, n * t / nth, n * (t + 1) / nth);
+  for (auto& x : th) x.join();
+}
+
+// bytes the reverse complement maps (ACGTN and the IUPAC codes)
+const bool* valid_ctg_byte() {
+  static const auto table = [] {
+    static bool t[256];
+    for (int b = 0; b < 256; b++) t[b] = complement((uint8_t)b) != 0;
+    return t;
+  }();
+  return table;
+}
+
 constexpr int64_t kDefaultBudget = int64_t(16) << 30;  // scratch bytes per launch (at most half the free memory)
 constexpr int64_t kPad = 64;                           // bytes readable past every staged buffer
 constexpr int64_t kMaxCap = int64_t(1) << 27;         // slots of one table
@@ -68,6 +116,7 @@ struct mhmla_ctx {
   Buf<uint8_t> d_ctg, d_seq, d_qual;
   Buf<int64_t> d_ctg_off, d_rfirst, d_read_off;
   Buf<int32_t> d_ctg_len, d_depth, d_read_len;
+  HostPinned h_ctg, h_seq, h_qual;  // staging copies of the contigs and the side-ordered reads
   // last run
   bool ran = false;
   mhmla_params params{};
@@ -179,6 +228,9 @@ void mhmla_destroy(mhmla_ctx* c) {
   c->d_ctg.release();
   c->d_seq.release();
   c->d_qual.release();
+  c->h_ctg.release();
+  c->h_seq.release();
+  c->h_qual.release();
   c->d_ctg_off.release();
   c->d_rfirst.release();
   c->d_read_off.release();
@@ -215,25 +267,46 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
   if (nreads > 0 && (!read_seq || !read_qual || !read_off || !read_len)) return fail(c, MHMLA_E_ARG, "null read array");
   // contigs: packed, checked for bytes the reverse complement maps
   int64_t total = 0;
+  std::vector<int64_t> coff((size_t)nctgs);
   for (int64_t i = 0; i < nctgs; i++) {
     if (ctg_len[i] < 0 || ctg_off[i] < 0) return fail(c, MHMLA_E_ARG, "contig %lld: negative offset or length", (long long)i);
+    coff[(size_t)i] = total;
     total += ctg_len[i];
   }
-  std::vector<uint8_t> cbuf((size_t)total + 1, 0);
-  std::vector<int64_t> coff((size_t)nctgs);
+  LA_HIP(hipSetDevice(c->device), "hipSetDevice");
+  LA_HIP(c->h_ctg.ensure((size_t)total + 1), "pinned contig staging");
+  uint8_t* cbuf = c->h_ctg.ptr;
+  cbuf[total] = 0;
   c->ctg_len.assign(ctg_len, ctg_len + nctgs);
   std::vector<int32_t> idepth((size_t)nctgs);
-  int64_t o = 0;
-  for (int64_t i = 0; i < nctgs; i++) {
-    const uint8_t* s = (const uint8_t*)ctg_seq + ctg_off[i];
-    for (int32_t j = 0; j < ctg_len[i]; j++)
-      if (!complement(s[j]))
+  {
+    // per thread: the first contig holding a byte outside the alphabet
+    std::vector<int64_t> bad_ctg(16, -1);
+    std::vector<int32_t> bad_pos(16, 0);
+    const bool* ok = valid_ctg_byte();
+    parallel_parts(nctgs, 256, [&](int t, int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; i++) {
+        const uint8_t* sq = (const uint8_t*)ctg_seq + ctg_off[i];
+        bool good = true;
+        for (int32_t j = 0; j < ctg_len[i]; j++) good &= ok[sq[j]];
+        if (!good) {
+          int32_t j = 0;
+          while (ok[sq[j]]) j++;
+          bad_ctg[(size_t)t] = i;
+          bad_pos[(size_t)t] = j;
+          return;
+        }
+        memcpy(cbuf + coff[(size_t)i], sq, (size_t)ctg_len[i]);
+        idepth[(size_t)i] = (int32_t)depth[i];  // set_ext(int) receives the double depth (kernel.cpp:377)
+      }
+    });
+    for (size_t t = 0; t < bad_ctg.size(); t++)
+      if (bad_ctg[t] >= 0) {
+        const int64_t i = bad_ctg[t];
+        const int32_t j = bad_pos[t];
         return fail(c, MHMLA_E_ARG, "contig %lld holds byte 0x%02x at %d (not ACGTN or an IUPAC code)", (long long)i,
-                    (unsigned)s[j], j);
-    memcpy(cbuf.data() + o, s, (size_t)ctg_len[i]);
-    coff[(size_t)i] = o;
-    o += ctg_len[i];
-    idepth[(size_t)i] = (int32_t)depth[i];  // set_ext(int) receives the double depth (kernel.cpp:377)
+                    (unsigned)(uint8_t)ctg_seq[ctg_off[i] + j], j);
+      }
   }
   // read lists, validated, then the reads copied in side order (side j =
   // contig * 2 + side, left 0, right 1) so each side's bytes are contiguous
@@ -266,22 +339,23 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
   c->rfirst[(size_t)(2 * nctgs)] = (int64_t)c->clen.size();
   if (cbytes > int64_t(kOffMask) - kPad)
     return fail(c, MHMLA_E_ARG, "the sides' reads hold %lld bytes, above 2 GiB", (long long)cbytes);
-  std::vector<uint8_t> cseq((size_t)cbytes), cqual((size_t)cbytes);
-  {
-    size_t k = 0;
-    for (int64_t i = 0; i < nctgs; i++)
+  LA_HIP(c->h_seq.ensure((size_t)std::max<int64_t>(cbytes, 1)), "pinned read staging");
+  LA_HIP(c->h_qual.ensure((size_t)std::max<int64_t>(cbytes, 1)), "pinned read staging");
+  uint8_t* cseq = c->h_seq.ptr;
+  uint8_t* cqual = c->h_qual.ptr;
+  parallel_parts(nctgs, 256, [&](int, int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; i++)
       for (int side = 0; side < 2; side++) {
         const int64_t* first = side ? right_first : left_first;
-        for (int64_t j = first[i]; j < first[i + 1]; j++) {
+        int64_t k = c->rfirst[(size_t)(2 * i + side)];
+        for (int64_t j = first[i]; j < first[i + 1]; j++, k++) {
           const int64_t r = read_idx[j];
-          memcpy(cseq.data() + croff[k], read_seq + read_off[r], (size_t)read_len[r]);
-          memcpy(cqual.data() + croff[k], read_qual + read_off[r], (size_t)read_len[r]);
-          k++;
+          memcpy(cseq + croff[(size_t)k], read_seq + read_off[r], (size_t)read_len[r]);
+          memcpy(cqual + croff[(size_t)k], read_qual + read_off[r], (size_t)read_len[r]);
         }
       }
-  }
+  });
   const int64_t ncr = (int64_t)c->clen.size();
-  LA_HIP(hipSetDevice(c->device), "hipSetDevice");
   LA_HIP(c->d_ctg.ensure((size_t)(total + kPad)), "contig buffer");
   LA_HIP(c->d_ctg_off.ensure((size_t)nctgs + 1), "contig offsets");
   LA_HIP(c->d_ctg_len.ensure((size_t)nctgs + 1), "contig lengths");
@@ -292,7 +366,7 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
   LA_HIP(c->d_seq.ensure((size_t)(cbytes + kPad)), "read bases");
   LA_HIP(c->d_qual.ensure((size_t)(cbytes + kPad)), "read qualities");
   hipStream_t st = c->stream;
-  LA_HIP(hipMemcpyAsync(c->d_ctg.ptr, cbuf.data(), (size_t)total + 1, hipMemcpyHostToDevice, st), "copy contigs");
+  LA_HIP(hipMemcpyAsync(c->d_ctg.ptr, cbuf, (size_t)total + 1, hipMemcpyHostToDevice, st), "copy contigs");
   if (nctgs) {
     LA_HIP(hipMemcpyAsync(c->d_ctg_off.ptr, coff.data(), 8 * (size_t)nctgs, hipMemcpyHostToDevice, st), "copy");
     LA_HIP(hipMemcpyAsync(c->d_ctg_len.ptr, ctg_len, 4 * (size_t)nctgs, hipMemcpyHostToDevice, st), "copy");
@@ -304,11 +378,12 @@ int mhmla_stage(mhmla_ctx* c, int64_t nctgs, const char* ctg_seq, const int64_t*
     LA_HIP(hipMemcpyAsync(c->d_read_len.ptr, c->clen.data(), 4 * (size_t)ncr, hipMemcpyHostToDevice, st), "copy");
   }
   if (cbytes) {
-    LA_HIP(hipMemcpyAsync(c->d_seq.ptr, cseq.data(), (size_t)cbytes, hipMemcpyHostToDevice, st), "copy read bases");
-    LA_HIP(hipMemcpyAsync(c->d_qual.ptr, cqual.data(), (size_t)cbytes, hipMemcpyHostToDevice, st),
+    LA_HIP(hipMemcpyAsync(c->d_seq.ptr, cseq, (size_t)cbytes, hipMemcpyHostToDevice, st), "copy read bases");
+    LA_HIP(hipMemcpyAsync(c->d_qual.ptr, cqual, (size_t)cbytes, hipMemcpyHostToDevice, st),
            "copy read qualities");
   }
-  // the host vectors above go out of scope: wait for the copies
+  // the host vectors above go out of scope, and the pinned buffers are
+  // refilled by the next stage: wait for the copies
   LA_HIP(hipStreamSynchronize(st), "staging copies");
   c->nctgs = nctgs;
   c->staged = true;
