@@ -200,6 +200,8 @@ typedef struct {
   double admit_s;
   double done_s;
   double device_ms; /* device time of the request's job */
+  double launched_s; /* the job's copies and kernels were enqueued */
+  double finished_s; /* the dispatcher saw the job finish (results not yet copied out) */
 } mhmsw_req_times;
 int mhmsw_server_create(int device_id, const mhmsw_scoring *scoring, const mhmsw_contigs *contigs, int slots,
                         int64_t hbm_budget, mhmsw_server **out);

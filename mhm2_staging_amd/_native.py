@@ -59,7 +59,7 @@ class Tuning(ctypes.Structure):
 
 class ReqTimes(ctypes.Structure):
     _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
-                ("device_ms", ctypes.c_double)]
+                ("device_ms", ctypes.c_double), ("launched_s", ctypes.c_double), ("finished_s", ctypes.c_double)]
 
 
 class SynthContigs(ctypes.Structure):

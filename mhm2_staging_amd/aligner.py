@@ -331,7 +331,8 @@ class Server:
         if rc != 0:
             msg = self._L.mhmsw_server_last_error(self._h)
             raise _native.MhmswError(rc, f"request {rid}: {msg.decode() if msg else ''}")
-        return out, {"submit_s": t.submit_s, "admit_s": t.admit_s, "done_s": t.done_s, "device_ms": t.device_ms}
+        return out, {"submit_s": t.submit_s, "admit_s": t.admit_s, "done_s": t.done_s, "device_ms": t.device_ms,
+                     "launched_s": t.launched_s, "finished_s": t.finished_s}
 
     def extras(self, rid, npairs):
         """After results(rid): (score2 or None, mismatches, op_off, ops) or CIGAR fields None when off."""

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/mhmla.h"
+#include "host_pinned.h"
 #include "locassm_launch.h"
 
 using namespace mhmla;
@@ -42,28 +43,6 @@ struct Buf {
   }
   void release() {
     if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-  }
-};
-
-// Pinned host memory, grown as needed and kept: staging copies go from it at
-// the link's full rate.
-struct HostPinned {
-  uint8_t* ptr = nullptr;
-  size_t cap = 0;
-  hipError_t ensure(size_t n) {
-    if (n <= cap) return hipSuccess;
-    if (ptr) (void)hipHostFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-    const size_t want = std::max<size_t>(n + n / 8, 4096);
-    const hipError_t e = hipHostMalloc((void**)&ptr, want, hipHostMallocDefault);
-    if (e == hipSuccess) cap = want;
-    return e;
-  }
-  void release() {
-    if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
     cap = 0;
   }

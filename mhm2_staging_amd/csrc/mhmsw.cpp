@@ -23,6 +23,7 @@
 #include "sw_common.h"
 #include "sw_launch.h"
 #include "cigar_core.h"
+#include "host_pinned.h"
 #include "pack2bit.h"
 
 using namespace mhmsw;
@@ -101,8 +102,6 @@ struct mhmsw_ctx {
   int64_t n_fast = 0, n_plain = 0;
   DevBuf<PairEnd> d_fwd, d_rev;
   DevBuf<int32_t> d_s2;
-  std::vector<int64_t> h_base;  // joined per-sequence tables of a multi-part job
-  std::vector<int32_t> h_len, h_win_ctg, h_win_off;
   DevBuf<SwTask> d_tasks;
   DevBuf<PassSched> d_sched;
   DevBuf<AlnResult> d_results;
@@ -127,6 +126,26 @@ struct mhmsw_ctx {
   int64_t class_cells[kNumClasses] = {};
   mhmsw_stats stats{};
   int32_t h_errors[4] = {};
+  // An upload goes through pinned staging with asynchronous copies on the
+  // context stream; ev_upload follows them, and the staging (and the device
+  // buffers they fill) are reused only once it has passed.
+  HostPinned h_stage;
+  hipEvent_t ev_upload = nullptr;
+  bool upload_pending = false;
+  // Words the job copies back into pinned memory at its end, before ev_done:
+  // completion needs no further copy (a null-stream copy can wait behind
+  // another context's work on a shared hardware queue).
+  struct JobTail {
+    int32_t errors[4];
+    unsigned long long counters[2];
+    unsigned long long cig_used[8];
+  };
+  JobTail* tail = nullptr;
+  hipEvent_t ev_done = nullptr;
+  // tail_results: the job also copies its results into pinned memory before
+  // ev_done (serving slots), so taking them needs no device copy
+  bool tail_results = false, last_job_tail_results = false;
+  HostPinned h_results;
 };
 
 static void reroute(mhmsw_ctx* c);
@@ -139,6 +158,14 @@ static int fail(mhmsw_ctx* c, int code, const char* fmt, ...) {
   va_end(ap);
   if (c) c->err = buf;
   return code;
+}
+
+// Device-to-host copy on the context's own stream (never the null stream,
+// which can share a hardware queue with another context's running job).
+static hipError_t d2h_sync(mhmsw_ctx* c, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
 }
 
 #define HIP_OR_FAIL(c, expr)                                                                        \
@@ -230,11 +257,19 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
       return MHMSW_E_HIP;
     }
   }
+  if (hipEventCreateWithFlags(&c->ev_upload, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
+    mhmsw_destroy(c);
+    return MHMSW_E_HIP;
+  }
   if (c->d_sched.ensure(1) != hipSuccess || c->d_errors.ensure(4) != hipSuccess ||
-      c->d_counters.ensure(2) != hipSuccess) {
+      c->d_counters.ensure(2) != hipSuccess ||
+      hipHostMalloc((void**)&c->tail, sizeof(mhmsw_ctx::JobTail), hipHostMallocDefault) != hipSuccess) {
+    c->tail = nullptr;
     mhmsw_destroy(c);
     return MHMSW_E_NOMEM;
   }
+  std::memset(c->tail, 0, sizeof(mhmsw_ctx::JobTail));
   compute_score_params(c);
   *out = c;
   return MHMSW_OK;
@@ -273,6 +308,11 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   c->d_cig_used.release();
   for (int i = 0; i < kEvents; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
+  if (c->ev_done) (void)hipEventDestroy(c->ev_done);
+  if (c->tail) (void)hipHostFree(c->tail);
+  c->h_stage.release();
+  c->h_results.release();
   for (int i = 0; i < kNumClasses; i++)
     for (int k = 0; k < 2; k++)
       if (c->cev[i][k]) (void)hipEventDestroy(c->cev[i][k]);
@@ -642,6 +682,25 @@ static void reroute(mhmsw_ctx* c) {
 // each sequence keeps its 32-base alignment) and only each part's packed
 // region crosses PCIe (windowed batches pack their reads only).  Returns when
 // the host buffers may be released.
+// Device buffers of a job of npairs pairs over total padded bases (they only
+// grow: a later, smaller job reuses them).
+static hipError_t ensure_job_buffers(mhmsw_ctx* c, int64_t npairs, int64_t total) {
+  const size_t np1 = (size_t)std::max<int64_t>(npairs, 1), ns1 = (size_t)std::max<int64_t>(2 * npairs, 1);
+  hipError_t e = c->d_packed2.ensure((size_t)(total / 16 + 4));
+  if (e == hipSuccess) e = c->d_nmask.ensure((size_t)(total / 32 + 4));
+  if (e == hipSuccess) e = c->d_codes.ensure((size_t)(2 * total / 8 + 64));
+  if (e == hipSuccess) e = c->d_s2.ensure(np1);
+  if (e == hipSuccess) e = c->d_base.ensure(ns1);
+  if (e == hipSuccess) e = c->d_len.ensure(ns1);
+  if (e == hipSuccess) e = c->d_bucket.ensure(np1);
+  if (e == hipSuccess) e = c->d_rank.ensure(np1);
+  if (e == hipSuccess) e = c->d_fwd.ensure(np1);
+  if (e == hipSuccess) e = c->d_rev.ensure(np1);
+  if (e == hipSuccess) e = c->d_results.ensure(np1);
+  if (e == hipSuccess) e = c->d_tasks.ensure(np1 + 64 * kNumClasses + 64);
+  return e;
+}
+
 static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t nparts) {
   if (nparts == 0) return fail(c, MHMSW_E_ARG, "no batch");
   const mhmsw_contigs* res = parts[0]->res;
@@ -655,6 +714,10 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
     if (rc != MHMSW_OK) return rc;
   }
   HIP_OR_FAIL(c, hipSetDevice(c->device));
+  if (c->upload_pending) {
+    HIP_OR_FAIL(c, hipEventSynchronize(c->ev_upload));
+    c->upload_pending = false;
+  }
   c->staged = false;
   c->have_results = false;
   int64_t npairs = 0, total = 0;
@@ -683,76 +746,96 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
   const int64_t nseq = 2 * npairs;
   c->npairs = npairs;
   c->total = total;
-  const size_t np1 = (size_t)std::max<int64_t>(npairs, 1);
-  HIP_OR_FAIL(c, c->d_packed2.ensure((size_t)(total / 16 + 4)));
-  HIP_OR_FAIL(c, c->d_nmask.ensure((size_t)(total / 32 + 4)));
-  HIP_OR_FAIL(c, c->d_codes.ensure((size_t)(2 * total / 8 + 64)));
-  HIP_OR_FAIL(c, c->d_s2.ensure(np1));
-  HIP_OR_FAIL(c, c->d_base.ensure((size_t)std::max<int64_t>(nseq, 1)));
-  HIP_OR_FAIL(c, c->d_len.ensure((size_t)std::max<int64_t>(nseq, 1)));
-  HIP_OR_FAIL(c, c->d_bucket.ensure(np1));
-  HIP_OR_FAIL(c, c->d_rank.ensure(np1));
-  HIP_OR_FAIL(c, c->d_fwd.ensure(np1));
-  HIP_OR_FAIL(c, c->d_rev.ensure(np1));
-  HIP_OR_FAIL(c, c->d_results.ensure(np1));
-  HIP_OR_FAIL(c, c->d_tasks.ensure(np1 + 64 * kNumClasses + 64));
-  // per-sequence tables: one part is copied as it is, several are joined with
-  // their bases shifted by the parts before them
-  c->lens.resize((size_t)nseq);
-  const int64_t* hbase = parts[0]->base.data();
-  const int32_t* hlen = parts[0]->len.data();
-  const int32_t* hwc = parts[0]->win_ctg.data();
-  const int32_t* hwo = parts[0]->win_off.data();
-  if (nparts > 1) {
-    c->h_base.resize((size_t)std::max<int64_t>(nseq, 1));
-    c->h_len.resize((size_t)std::max<int64_t>(nseq, 1));
-    if (res) {
-      c->h_win_ctg.resize(np1);
-      c->h_win_off.resize(np1);
+  HIP_OR_FAIL(c, ensure_job_buffers(c, npairs, total));
+  // everything the job reads from the host, laid out in the pinned staging;
+  // the per-sequence tables of several parts are joined there, with their
+  // bases shifted by the parts before them
+  size_t stage_bytes = 0;
+  auto stage_at = [&](size_t bytes) {
+    const size_t at = stage_bytes;
+    stage_bytes += (bytes + 255) & ~(size_t)255;
+    return at;
+  };
+  std::vector<size_t> at_two(nparts), at_nm(nparts);
+  for (size_t k = 0; k < nparts; k++) {
+    at_two[k] = stage_at(parts[k]->two.size() * 4);
+    at_nm[k] = stage_at(parts[k]->nm.size() * 4);
+  }
+  const size_t at_base = stage_at((size_t)nseq * 8), at_len = stage_at((size_t)nseq * 4);
+  const bool windows = res && npairs > 0;
+  const size_t at_wc = stage_at(windows ? (size_t)npairs * 4 : 0), at_wo = stage_at(windows ? (size_t)npairs * 4 : 0);
+  if (windows) {
+    HIP_OR_FAIL(c, c->d_win_ctg.ensure((size_t)npairs));
+    HIP_OR_FAIL(c, c->d_win_off.ensure((size_t)npairs));
+  }
+  HIP_OR_FAIL(c, c->h_stage.ensure(stage_bytes));
+  uint8_t* hs = c->h_stage.ptr;
+  int64_t* sbase = reinterpret_cast<int64_t*>(hs + at_base);
+  int32_t* slen = reinterpret_cast<int32_t*>(hs + at_len);
+  int32_t* swc = reinterpret_cast<int32_t*>(hs + at_wc);
+  int32_t* swo = reinterpret_cast<int32_t*>(hs + at_wo);
+  // fill the staging (on up to 4 threads for large jobs), then copy it out
+  auto fill_part = [&](size_t k, int64_t sh, int64_t ps) {
+    const PackedBatch& pb = *parts[k];
+    if (!pb.two.empty()) std::memcpy(hs + at_two[k], pb.two.data(), pb.two.size() * 4);
+    if (!pb.nm.empty()) std::memcpy(hs + at_nm[k], pb.nm.data(), pb.nm.size() * 4);
+    const size_t nq = (size_t)(2 * pb.npairs);
+    if (sh == 0) {
+      if (nq) std::memcpy(sbase + 2 * ps, pb.base.data(), nq * 8);
+    } else {
+      for (size_t q = 0; q < nq; q++) sbase[2 * ps + (int64_t)q] = pb.base[q] + sh;
     }
+    if (nq) std::memcpy(slen + 2 * ps, pb.len.data(), nq * 4);
+    if (windows && pb.npairs > 0) {
+      std::memcpy(swc + ps, pb.win_ctg.data(), (size_t)pb.npairs * 4);
+      std::memcpy(swo + ps, pb.win_off.data(), (size_t)pb.npairs * 4);
+    }
+  };
+  std::vector<int64_t> part_sh(nparts), part_ps(nparts);
+  {
     int64_t sh = 0, ps = 0;
     for (size_t k = 0; k < nparts; k++) {
-      const PackedBatch& pb = *parts[k];
-      for (int64_t q = 0; q < 2 * pb.npairs; q++) {
-        c->h_base[(size_t)(2 * ps + q)] = pb.base[(size_t)q] + sh;
-        c->h_len[(size_t)(2 * ps + q)] = pb.len[(size_t)q];
-      }
-      if (res && pb.npairs > 0) {
-        std::memcpy(c->h_win_ctg.data() + ps, pb.win_ctg.data(), (size_t)pb.npairs * 4);
-        std::memcpy(c->h_win_off.data() + ps, pb.win_off.data(), (size_t)pb.npairs * 4);
-      }
-      sh += pb.total;
-      ps += pb.npairs;
+      part_sh[k] = sh;
+      part_ps[k] = ps;
+      sh += parts[k]->total;
+      ps += parts[k]->npairs;
     }
-    hbase = c->h_base.data();
-    hlen = c->h_len.data();
-    hwc = c->h_win_ctg.data();
-    hwo = c->h_win_off.data();
   }
-  std::memcpy(c->lens.data(), hlen, (size_t)nseq * sizeof(int32_t));
-  int64_t sh = 0;
+  const int nfill = nparts > 1 && npairs >= (1 << 16) ? (int)std::min<size_t>(4, nparts) : 1;
+  if (nfill == 1) {
+    for (size_t k = 0; k < nparts; k++) fill_part(k, part_sh[k], part_ps[k]);
+  } else {
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+      for (size_t k; (k = next.fetch_add(1)) < nparts;) fill_part(k, part_sh[k], part_ps[k]);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nfill; t++) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+  }
   for (size_t k = 0; k < nparts; k++) {
     const PackedBatch& pb = *parts[k];
     if (!pb.two.empty())
-      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr + sh / 16, pb.two.data(), pb.two.size() * 4,
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr + part_sh[k] / 16, hs + at_two[k], pb.two.size() * 4,
                                     hipMemcpyHostToDevice, c->stream));
     if (!pb.nm.empty())
-      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr + sh / 32, pb.nm.data(), pb.nm.size() * 4, hipMemcpyHostToDevice,
-                                    c->stream));
-    sh += pb.total;
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr + part_sh[k] / 32, hs + at_nm[k], pb.nm.size() * 4,
+                                    hipMemcpyHostToDevice, c->stream));
   }
+  c->lens.resize((size_t)nseq);
   if (nseq > 0) {
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, hbase, (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, hlen, (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
+    std::memcpy(c->lens.data(), slen, (size_t)nseq * sizeof(int32_t));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_base.ptr, sbase, (size_t)nseq * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_len.ptr, slen, (size_t)nseq * 4, hipMemcpyHostToDevice, c->stream));
   }
   c->res = res;
-  if (res && npairs > 0) {
-    HIP_OR_FAIL(c, c->d_win_ctg.ensure((size_t)npairs));
-    HIP_OR_FAIL(c, c->d_win_off.ensure((size_t)npairs));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, hwc, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, hwo, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+  if (windows) {
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, swc, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, swo, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
   }
-  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  HIP_OR_FAIL(c, hipEventRecord(c->ev_upload, c->stream));
+  c->upload_pending = true;
   c->staged = true;
   return MHMSW_OK;
 }
@@ -1016,9 +1099,26 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = c->npairs > 0 && c->fast_class_used[k];
   int maxp16 = 0, maxp64 = 0;
   c->last_job_graph = c->use_graphs;
+  c->last_job_tail_results = c->tail_results && c->npairs > 0;
+  if (c->last_job_tail_results) HIP_OR_FAIL(c, c->h_results.ensure((size_t)c->npairs * sizeof(AlnResult)));
+  // after the job: its error words, counters and CIGAR pool use into pinned
+  // memory, then ev_done (outside any captured graph)
+  auto enqueue_tail = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(c->tail->errors, c->d_errors.ptr, sizeof(c->tail->errors), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->tail->counters, c->d_counters.ptr, sizeof(c->tail->counters), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && c->cigar)
+      e = hipMemcpyAsync(c->tail->cig_used, c->d_cig_used.ptr, sizeof(c->tail->cig_used), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && c->last_job_tail_results)
+      e = hipMemcpyAsync(c->h_results.ptr, c->d_results.ptr, (size_t)c->npairs * sizeof(AlnResult),
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_done, st);
+    return e;
+  };
   if (!c->use_graphs) {
     int rc = enqueue_job(c, b, st, true, &maxp16, &maxp64);
     if (rc != MHMSW_OK) return rc;
+    HIP_OR_FAIL(c, enqueue_tail());
   } else {
     // the job depends only on the view, the classes used, the mode and the
     // plain LDS size: replay the captured graph while those are unchanged
@@ -1058,6 +1158,7 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
     HIP_OR_FAIL(c, hipGraphLaunch(c->graph_exec, st));
     HIP_OR_FAIL(c, hipEventRecord(c->ev[kEvents - 1], st));
+    HIP_OR_FAIL(c, enqueue_tail());
   }
   c->in_flight = true;
   c->have_results = false;
@@ -1084,7 +1185,11 @@ int mhmsw_submit(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t*
 // what it needed), synchronously on the context stream.
 static int settle_cigar(mhmsw_ctx* c) {
   for (int attempt = 0;; attempt++) {
-    HIP_OR_FAIL(c, hipMemcpy(c->h_cig_used, c->d_cig_used.ptr, sizeof(c->h_cig_used), hipMemcpyDeviceToHost));
+    if (attempt == 0) {
+      std::memcpy(c->h_cig_used, c->tail->cig_used, sizeof(c->h_cig_used));  // copied by the job
+    } else {
+      HIP_OR_FAIL(c, d2h_sync(c, c->h_cig_used, c->d_cig_used.ptr, sizeof(c->h_cig_used)));
+    }
     const unsigned long long* u = c->h_cig_used;
     if (u[3] != 0)
       return fail(c, MHMSW_E_INTERNAL, "CIGAR pass: %llu tracebacks left their band", (unsigned long long)u[3]);
@@ -1104,9 +1209,10 @@ static int settle_cigar(mhmsw_ctx* c) {
 
 static int complete(mhmsw_ctx* c) {
   c->in_flight = false;
-  HIP_OR_FAIL(c, hipMemcpy(c->h_errors, c->d_errors.ptr, sizeof(c->h_errors), hipMemcpyDeviceToHost));
+  c->upload_pending = false;  // ev_done follows the upload's copies
+  std::memcpy(c->h_errors, c->tail->errors, sizeof(c->h_errors));
   unsigned long long counters[2] = {0, 0};
-  HIP_OR_FAIL(c, hipMemcpy(counters, c->d_counters.ptr, sizeof(counters), hipMemcpyDeviceToHost));
+  std::memcpy(counters, c->tail->counters, sizeof(counters));
   float ms[kEvents] = {};
   float job = 0.f;
   if (c->last_job_graph) {
@@ -1152,7 +1258,7 @@ static int complete(mhmsw_ctx* c) {
 int mhmsw_query(mhmsw_ctx* c) {
   if (!c) return MHMSW_E_ARG;
   if (!c->in_flight) return c->have_results ? 1 : fail(c, MHMSW_E_STATE, "no batch submitted");
-  hipError_t e = hipEventQuery(c->ev[kEvents - 1]);
+  hipError_t e = hipEventQuery(c->ev_done);
   if (e == hipErrorNotReady) return 0;
   if (e != hipSuccess) return fail(c, MHMSW_E_HIP, "hipEventQuery: %s", hipGetErrorString(e));
   int rc = complete(c);
@@ -1163,7 +1269,7 @@ int mhmsw_wait(mhmsw_ctx* c) {
   if (!c) return MHMSW_E_ARG;
   if (!c->in_flight) return c->have_results ? MHMSW_OK : fail(c, MHMSW_E_STATE, "no batch submitted");
   HIP_OR_FAIL(c, hipSetDevice(c->device));
-  HIP_OR_FAIL(c, hipEventSynchronize(c->ev[kEvents - 1]));
+  HIP_OR_FAIL(c, hipEventSynchronize(c->ev_done));
   return complete(c);
 }
 
@@ -1177,8 +1283,11 @@ int mhmsw_results(mhmsw_ctx* c, mhmsw_result* out, int64_t npairs) {
   if (npairs != c->npairs) return fail(c, MHMSW_E_ARG, "results requested for %lld pairs, batch has %lld",
                                        (long long)npairs, (long long)c->npairs);
   static_assert(sizeof(mhmsw_result) == sizeof(AlnResult), "result layout");
-  if (npairs > 0)
-    HIP_OR_FAIL(c, hipMemcpy(out, c->d_results.ptr, (size_t)npairs * sizeof(AlnResult), hipMemcpyDeviceToHost));
+  if (npairs > 0 && c->last_job_tail_results) {
+    std::memcpy(out, c->h_results.ptr, (size_t)npairs * sizeof(AlnResult));  // copied by the job
+  } else if (npairs > 0) {
+    HIP_OR_FAIL(c, d2h_sync(c, out, c->d_results.ptr, (size_t)npairs * sizeof(AlnResult)));
+  }
   if (c->h_errors[0] != 0 || c->h_errors[2] != 0)
     return fail(c, MHMSW_E_INTERNAL, "%d pairs: reverse pass did not reproduce the forward score; %d end points "
                 "outside their pair", c->h_errors[0], c->h_errors[2]);
@@ -1195,7 +1304,7 @@ int mhmsw_results_score2(mhmsw_ctx* c, int32_t* out, int64_t npairs) {
   if (!c->last_job_score2) return fail(c, MHMSW_E_STATE, "the last batch ran without score2 (mhmsw_set_score2)");
   if (npairs != c->npairs) return fail(c, MHMSW_E_ARG, "score2 requested for %lld pairs, batch has %lld",
                                        (long long)npairs, (long long)c->npairs);
-  if (npairs > 0) HIP_OR_FAIL(c, hipMemcpy(out, c->d_s2.ptr, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (npairs > 0) HIP_OR_FAIL(c, d2h_sync(c, out, c->d_s2.ptr, (size_t)npairs * sizeof(int32_t)));
   return MHMSW_OK;
 }
 
@@ -1220,9 +1329,9 @@ int mhmsw_results_cigar(mhmsw_ctx* c, int64_t npairs, int32_t* mismatches, int64
   std::vector<int32_t> n((size_t)npairs), mm((size_t)npairs);
   std::vector<int64_t> off((size_t)npairs);
   if (npairs > 0) {
-    HIP_OR_FAIL(c, hipMemcpy(n.data(), c->d_cig_nops.ptr, (size_t)npairs * 4, hipMemcpyDeviceToHost));
-    HIP_OR_FAIL(c, hipMemcpy(mm.data(), c->d_cig_mism.ptr, (size_t)npairs * 4, hipMemcpyDeviceToHost));
-    HIP_OR_FAIL(c, hipMemcpy(off.data(), c->d_cig_off.ptr, (size_t)npairs * 8, hipMemcpyDeviceToHost));
+    HIP_OR_FAIL(c, d2h_sync(c, n.data(), c->d_cig_nops.ptr, (size_t)npairs * 4));
+    HIP_OR_FAIL(c, d2h_sync(c, mm.data(), c->d_cig_mism.ptr, (size_t)npairs * 4));
+    HIP_OR_FAIL(c, d2h_sync(c, off.data(), c->d_cig_off.ptr, (size_t)npairs * 8));
   }
   int64_t total = 0;
   for (int64_t p = 0; p < npairs; p++) {
@@ -1246,7 +1355,7 @@ int mhmsw_results_cigar(mhmsw_ctx* c, int64_t npairs, int32_t* mismatches, int64
   const int64_t used = (int64_t)c->h_cig_used[0];
   std::vector<uint32_t> pool((size_t)std::max<int64_t>(used, 1));
   if (used > 0)
-    HIP_OR_FAIL(c, hipMemcpy(pool.data(), c->d_cig_ops.ptr, (size_t)used * 4, hipMemcpyDeviceToHost));
+    HIP_OR_FAIL(c, d2h_sync(c, pool.data(), c->d_cig_ops.ptr, (size_t)used * 4));
   int64_t acc = 0;
   for (int64_t p = 0; p < npairs; p++) {
     const int32_t k = n[(size_t)p];
@@ -1492,6 +1601,10 @@ int split_extras(mhmsw_ctx* c, Job& j) {
 }
 
 constexpr int64_t kMergeMaxPairs = (int64_t)1 << 20;
+// Buffers every slot holds from the start, so that jobs up to this size never
+// reallocate while others run (hipFree waits for the whole device).
+constexpr int64_t kSlotReservePairs = (int64_t)1 << 18;
+constexpr int64_t kSlotReserveBases = kSlotReservePairs * 1280;
 
 }  // namespace
 
@@ -1500,10 +1613,12 @@ struct mhmsw_server {
   PackParams pp;
   const mhmsw_contigs* contigs = nullptr;
   std::vector<mhmsw_ctx*> slots;
-  std::vector<std::shared_ptr<Job>> running;  // per slot
+  std::vector<std::shared_ptr<Job>> running;  // per slot: the job handed to its worker
+  std::vector<char> taken;                    // per slot: the worker has started that job
+  std::vector<std::thread> workers;           // one per slot
   int64_t budget = 0, used = 0;
   std::mutex mu;
-  std::condition_variable cv_done, cv_work;
+  std::condition_variable cv_done, cv_work, cv_slot;
   std::deque<std::shared_ptr<Request>> queue;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> extras;  // results taken, extras not yet
@@ -1514,52 +1629,78 @@ struct mhmsw_server {
   std::string err;
 };
 
-static void server_loop(mhmsw_server* s) {
+// One worker per slot runs its jobs start to end: upload, launch, wait for the
+// device, copy the results out and split them by request.  A slow step on one
+// slot (a large upload, a buffer that grows) never holds up another slot's
+// completion.
+static void slot_worker(mhmsw_server* s, size_t k) {
   (void)hipSetDevice(s->device);
+  mhmsw_ctx* c = s->slots[k];
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_slot.wait(lk, [&] { return s->stop || (s->running[k] && !s->taken[k]); });
+    if (s->stop) break;
+    std::shared_ptr<Job> j = s->running[k];
+    s->taken[k] = 1;
+    lk.unlock();
+    // the slot's context belongs to this thread while it runs the job
+    mhmsw_set_score2(c, j->s2 ? 1 : 0);
+    mhmsw_set_cigar(c, j->cigar ? 1 : 0);
+    std::vector<const PackedBatch*> parts;
+    parts.reserve(j->reqs.size());
+    for (auto& r : j->reqs) parts.push_back(&r->packed);
+    int rc = upload_parts(c, parts.data(), parts.size());
+    if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
+    const double launched = steady_now();
+    int q = rc == MHMSW_OK ? 0 : rc;
+    while (q == 0) {
+      q = mhmsw_query(c);
+      if (q == 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    const double fin = steady_now();
+    mhmsw_stats st{};
+    if (q == 1) {
+      j->results.resize((size_t)std::max<int64_t>(j->npairs, 1));
+      rc = mhmsw_results(c, j->results.data(), j->npairs);
+      if (rc == MHMSW_OK && (j->s2 || j->cigar)) rc = split_extras(c, *j);
+      if (rc == MHMSW_OK) {
+        mhmsw_get_stats(c, &st);
+        for (size_t i = 0; i < j->reqs.size(); i++) {
+          Request& r = *j->reqs[i];
+          r.results.assign(j->results.begin() + j->first[i], j->results.begin() + j->first[i] + r.npairs);
+        }
+      }
+    }
+    lk.lock();
+    const double now = steady_now();
+    for (auto& rp : j->reqs) {
+      Request& r = *rp;
+      r.times.launched_s = launched;
+      r.times.finished_s = fin;
+      r.times.done_s = now;
+      r.times.device_ms = st.job_ms;
+      if (q < 0 || rc != MHMSW_OK) {
+        r.state = q < 0 ? q : rc;
+        r.err = mhmsw_last_error(c);
+      } else {
+        r.state = 2;
+      }
+      r.packed = PackedBatch();  // release the host copy
+    }
+    s->running[k].reset();
+    s->taken[k] = 0;
+    s->used -= j->est;
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+  }
+}
+
+// The dispatcher admits queued requests, in arrival order, to free slots;
+// requests waiting together share a job.
+static void server_loop(mhmsw_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop) {
-    bool progressed = false;
-    // completions
-    for (size_t k = 0; k < s->slots.size(); k++) {
-      std::shared_ptr<Job> j = s->running[k];
-      if (!j) continue;
-      lk.unlock();
-      int q = mhmsw_query(s->slots[k]);
-      int rc = MHMSW_OK;
-      mhmsw_stats st{};
-      if (q == 1) {
-        j->results.resize((size_t)std::max<int64_t>(j->npairs, 1));
-        rc = mhmsw_results(s->slots[k], j->results.data(), j->npairs);
-        if (rc == MHMSW_OK && (j->s2 || j->cigar)) rc = split_extras(s->slots[k], *j);
-        if (rc == MHMSW_OK) {
-          mhmsw_get_stats(s->slots[k], &st);
-          for (size_t i = 0; i < j->reqs.size(); i++) {
-            Request& r = *j->reqs[i];
-            r.results.assign(j->results.begin() + j->first[i], j->results.begin() + j->first[i] + r.npairs);
-          }
-        }
-      }
-      lk.lock();
-      if (q == 0) continue;
-      progressed = true;
-      const double now = steady_now();
-      for (auto& rp : j->reqs) {
-        Request& r = *rp;
-        r.times.done_s = now;
-        r.times.device_ms = st.job_ms;
-        if (q < 0 || rc != MHMSW_OK) {
-          r.state = q < 0 ? q : rc;
-          r.err = mhmsw_last_error(s->slots[k]);
-        } else {
-          r.state = 2;
-        }
-        r.packed = PackedBatch();  // release the host copy
-      }
-      s->running[k].reset();
-      s->used -= j->est;
-      s->cv_done.notify_all();
-    }
-    // admissions, in arrival order; requests waiting together share a job
+    bool admitted = false;
     while (!s->queue.empty()) {
       int freek = -1, busy = 0;
       for (size_t k = 0; k < s->slots.size(); k++) {
@@ -1593,32 +1734,13 @@ static void server_loop(mhmsw_server* s) {
         r->times.admit_s = now;
       }
       s->running[(size_t)freek] = j;
+      s->taken[(size_t)freek] = 0;
       s->used += j->est;
-      lk.unlock();
-      mhmsw_ctx* c = s->slots[(size_t)freek];
-      // the slot's context belongs to this thread while no job runs on it
-      mhmsw_set_score2(c, j->s2 ? 1 : 0);
-      mhmsw_set_cigar(c, j->cigar ? 1 : 0);
-      std::vector<const PackedBatch*> parts;
-      parts.reserve(j->reqs.size());
-      for (auto& r : j->reqs) parts.push_back(&r->packed);
-      int rc = upload_parts(c, parts.data(), parts.size());
-      if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
-      lk.lock();
-      progressed = true;
-      if (rc != MHMSW_OK) {
-        const double t = steady_now();
-        for (auto& r : j->reqs) {
-          r->state = rc;
-          r->err = mhmsw_last_error(c);
-          r->times.done_s = t;
-        }
-        s->running[(size_t)freek].reset();
-        s->used -= j->est;
-        s->cv_done.notify_all();
-      }
+      admitted = true;
     }
-    if (!progressed) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+    if (admitted) s->cv_slot.notify_all();
+    // woken by a submission, a finished job or stop
+    s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
   }
 }
 
@@ -1642,6 +1764,15 @@ int mhmsw_server_create(int device_id, const mhmsw_scoring* scoring, const mhmsw
     int rc = mhmsw_create(device_id, &c);
     if (rc == MHMSW_OK && scoring) rc = mhmsw_set_scoring(c, scoring);
     if (rc == MHMSW_OK) rc = mhmsw_set_graphs(c, 0);  // every request has its own geometry
+    if (rc == MHMSW_OK && (ensure_job_buffers(c, kSlotReservePairs, kSlotReserveBases) != hipSuccess ||
+                           c->h_stage.ensure((size_t)kSlotReservePairs * 128) != hipSuccess ||
+                           c->h_results.ensure((size_t)kSlotReservePairs * sizeof(AlnResult)) != hipSuccess))
+      rc = fail(c, MHMSW_E_NOMEM, "cannot reserve the slot's buffers");
+    if (c) c->tail_results = true;
+    // first use of the slot's stream now, not inside the first request's latency
+    if (rc == MHMSW_OK && (hipMemsetAsync(c->d_errors.ptr, 0, 4 * sizeof(int32_t), c->stream) != hipSuccess ||
+                           hipStreamSynchronize(c->stream) != hipSuccess))
+      rc = fail(c, MHMSW_E_HIP, "slot stream set-up failed");
     if (rc != MHMSW_OK) {
       if (c) mhmsw_destroy(c);
       for (auto* x : s->slots) mhmsw_destroy(x);
@@ -1651,6 +1782,7 @@ int mhmsw_server_create(int device_id, const mhmsw_scoring* scoring, const mhmsw
     s->slots.push_back(c);
   }
   s->running.resize((size_t)slots);
+  s->taken.assign((size_t)slots, 0);
   s->pp = pack_params(s->slots[0]);
   if (hbm_budget <= 0) {
     size_t tot = 0;
@@ -1659,6 +1791,7 @@ int mhmsw_server_create(int device_id, const mhmsw_scoring* scoring, const mhmsw
   }
   s->budget = hbm_budget;
   s->th = std::thread(server_loop, s);
+  for (size_t k = 0; k < s->slots.size(); k++) s->workers.emplace_back(slot_worker, s, k);
   *out = s;
   return MHMSW_OK;
 }
@@ -1803,7 +1936,10 @@ int mhmsw_server_destroy(mhmsw_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_slot.notify_all();
   if (s->th.joinable()) s->th.join();
+  for (auto& w : s->workers)
+    if (w.joinable()) w.join();
   for (auto* c : s->slots) mhmsw_destroy(c);
   delete s;
   return MHMSW_OK;
