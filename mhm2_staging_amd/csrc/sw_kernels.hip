@@ -737,6 +737,9 @@ __global__ __launch_bounds__(256) void sw_fast_kernel(BatchView b) {
   const int grp = lane / G;
   PassSched* s = b.sched;
   const int32_t nwt = s->class_slots[cls] / GPW;
+  // wavefronts numbered past the class's work leave before touching the shared
+  // counter (the rest still take every task: each loops until it runs out)
+  if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;
   const int32_t base = s->class_base[cls];
   // every lane takes part in the atomic (lane 0 adds 1, the rest 0): no
   // divergent branch in the loop, so the wavefront stays whole.  The next
@@ -954,6 +957,7 @@ __global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
   const int lane = threadIdx.x & 63;
   PassSched* s = b.sched;
   const int32_t nwt = s->class_slots[cls] / 64;
+  if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;  // as in sw_fast_kernel
   const int32_t base = s->class_base[cls];
   int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
   SwTask task;
@@ -1052,6 +1056,7 @@ __global__ __launch_bounds__(64) void sw_plain_kernel(BatchView b) {
   const int k = threadIdx.x;
   PassSched* s = b.sched;
   const int32_t n = s->class_count[kClassPlain];
+  if ((int32_t)blockIdx.x >= n) return;  // one wavefront per block; as in sw_fast_kernel
   const int32_t base = s->class_base[kClassPlain];
   for (;;) {
     const int w = from_lane0(atomicAdd(&s->class_next[kClassPlain], k == 0 ? 1 : 0));
