@@ -988,6 +988,10 @@ static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
   return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus * 12, st);
 }
 
+// jobs with at most this many packed-form pairs run both band classes of the
+// reverse pass in one launch
+constexpr int64_t kBandBothMaxPairs = 65536;
+
 // ev: record phase events (eager launches only; events recorded inside a
 // captured graph cannot be timed)
 static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev, int* maxp16_out, int* maxp64_out) {
@@ -1054,8 +1058,13 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->n_fast, 4), st));
       for (int P = rev16; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->n_fast, 16), st));
       // pairs whose begin lies within a narrow band of diagonals: one lane each
-      HIP_OR_FAIL(c, launch_band(b, 8, grid_cap(c->n_fast, 256), st));
-      HIP_OR_FAIL(c, launch_band(b, 4, grid_cap(c->n_fast, 256), st));
+      if (c->n_fast <= kBandBothMaxPairs) {
+        // one launch for both classes: enough wavefronts for every task of both
+        HIP_OR_FAIL(c, launch_band(b, 0, grid_cap(c->n_fast + 128, 256), st));
+      } else {
+        HIP_OR_FAIL(c, launch_band(b, 8, grid_cap(c->n_fast, 256), st));
+        HIP_OR_FAIL(c, launch_band(b, 4, grid_cap(c->n_fast, 256), st));
+      }
     }
     HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
   }

@@ -972,6 +972,40 @@ __global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
   }
 }
 
+// Both band classes in one launch, for small jobs whose two classes each fill
+// only a few wavefronts: the wide class (D <= 15) is taken by the wavefronts
+// at the start of the grid, the narrow one (D <= 7) by those at its end, so
+// the two run side by side instead of one launch after the other.  (Large
+// jobs keep one launch per class: this kernel holds the wide class's
+// registers, which would cost the narrow class a wave per SIMD.)
+template <int P>
+__device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t wave) {
+  const int cls = kClassBand + (P == 4 ? 0 : 1);
+  PassSched* s = b.sched;
+  const int32_t nwt = s->class_slots[cls] / 64;
+  if (wave >= nwt) return;
+  const int32_t base = s->class_base[cls];
+  int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+  SwTask task;
+  if (w < nwt) task = b.tasks[base + w * 64 + lane];
+  while (w < nwt) {
+    const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+    SwTask next;
+    if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
+    band_pair<P>(b, task);
+    task = next;
+    w = wn;
+  }
+}
+
+__global__ __launch_bounds__(256) void sw_band_both_kernel(BatchView b) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+  const int32_t nwaves = (int32_t)(gridDim.x * (blockDim.x / 64));
+  band_class<8>(b, lane, wave);
+  band_class<4>(b, lane, nwaves - 1 - wave);
+}
+
 // ----------------------------------------------------------------------------
 // plain 32-bit kernel: one wavefront per pair, rows in tiles of 64
 // ----------------------------------------------------------------------------
@@ -1199,7 +1233,9 @@ hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream
 }
 
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st) {
-  if (P == 4) {
+  if (P == 0) {
+    hipLaunchKernelGGL(sw_band_both_kernel, dim3(grid), dim3(256), 0, st, b);
+  } else if (P == 4) {
     hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b);
   } else if (P == 8) {
     hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b);

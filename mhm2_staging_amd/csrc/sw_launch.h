@@ -74,7 +74,7 @@ hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
 hipError_t launch_unpack(const BatchView& b, hipStream_t st);  // also extracts resident windows
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
-// reverse pass, banded form: P = 4 (|l - k| <= 7) or 8 (<= 15)
+// reverse pass, banded form: P = 4 (|l - k| <= 7), 8 (<= 15) or 0 (both classes in one launch)
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
