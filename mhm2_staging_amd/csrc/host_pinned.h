@@ -12,7 +12,9 @@
 struct HostPinned {
   uint8_t* ptr = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t n) {
+  // touch: write every page now (a latency-bound caller reserving ahead),
+  // not on a later copy's critical path
+  hipError_t ensure(size_t n, bool touch = false) {
     if (n <= cap) return hipSuccess;
     if (ptr) (void)hipHostFree(ptr);
     ptr = nullptr;
@@ -21,7 +23,7 @@ struct HostPinned {
     const hipError_t e = hipHostMalloc((void**)&ptr, want, hipHostMallocDefault);
     if (e == hipSuccess) {
       cap = want;
-      std::memset(ptr, 0, want);  // touch every page now, not on a later copy's critical path
+      if (touch) std::memset(ptr, 0, want);
     }
     return e;
   }

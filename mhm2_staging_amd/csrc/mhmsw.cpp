@@ -682,6 +682,26 @@ static void reroute(mhmsw_ctx* c) {
 // each sequence keeps its 32-base alignment) and only each part's packed
 // region crosses PCIe (windowed batches pack their reads only).  Returns when
 // the host buffers may be released.
+// memcpy split over up to 16 threads for large copies (host memory bandwidth
+// is several times one core's copy rate)
+static void par_memcpy(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kChunk = (size_t)8 << 20;
+  const int nth = (int)std::min<size_t>(std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())),
+                                        (bytes + kChunk - 1) / kChunk);
+  if (nth <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  auto part = [&](int t) {
+    const size_t a = bytes * (size_t)t / (size_t)nth, e = bytes * (size_t)(t + 1) / (size_t)nth;
+    std::memcpy(static_cast<uint8_t*>(dst) + a, static_cast<const uint8_t*>(src) + a, e - a);
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; t++) th.emplace_back(part, t);
+  part(0);
+  for (auto& t : th) t.join();
+}
+
 // Device buffers of a job of npairs pairs over total padded bases (they only
 // grow: a later, smaller job reuses them).
 static hipError_t ensure_job_buffers(mhmsw_ctx* c, int64_t npairs, int64_t total) {
@@ -777,15 +797,23 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
   // fill the staging (on up to 4 threads for large jobs), then copy it out
   auto fill_part = [&](size_t k, int64_t sh, int64_t ps) {
     const PackedBatch& pb = *parts[k];
-    if (!pb.two.empty()) std::memcpy(hs + at_two[k], pb.two.data(), pb.two.size() * 4);
-    if (!pb.nm.empty()) std::memcpy(hs + at_nm[k], pb.nm.data(), pb.nm.size() * 4);
+    // one part alone fills the staging on its own up to 16 threads per array
+    auto copy = [&](void* d, const void* src, size_t n) {
+      if (nparts == 1) {
+        par_memcpy(d, src, n);
+      } else {
+        std::memcpy(d, src, n);
+      }
+    };
+    if (!pb.two.empty()) copy(hs + at_two[k], pb.two.data(), pb.two.size() * 4);
+    if (!pb.nm.empty()) copy(hs + at_nm[k], pb.nm.data(), pb.nm.size() * 4);
     const size_t nq = (size_t)(2 * pb.npairs);
     if (sh == 0) {
-      if (nq) std::memcpy(sbase + 2 * ps, pb.base.data(), nq * 8);
+      if (nq) copy(sbase + 2 * ps, pb.base.data(), nq * 8);
     } else {
       for (size_t q = 0; q < nq; q++) sbase[2 * ps + (int64_t)q] = pb.base[q] + sh;
     }
-    if (nq) std::memcpy(slen + 2 * ps, pb.len.data(), nq * 4);
+    if (nq) copy(slen + 2 * ps, pb.len.data(), nq * 4);
     if (windows && pb.npairs > 0) {
       std::memcpy(swc + ps, pb.win_ctg.data(), (size_t)pb.npairs * 4);
       std::memcpy(swo + ps, pb.win_off.data(), (size_t)pb.npairs * 4);
@@ -1774,8 +1802,8 @@ int mhmsw_server_create(int device_id, const mhmsw_scoring* scoring, const mhmsw
     if (rc == MHMSW_OK && scoring) rc = mhmsw_set_scoring(c, scoring);
     if (rc == MHMSW_OK) rc = mhmsw_set_graphs(c, 0);  // every request has its own geometry
     if (rc == MHMSW_OK && (ensure_job_buffers(c, kSlotReservePairs, kSlotReserveBases) != hipSuccess ||
-                           c->h_stage.ensure((size_t)kSlotReservePairs * 128) != hipSuccess ||
-                           c->h_results.ensure((size_t)kSlotReservePairs * sizeof(AlnResult)) != hipSuccess))
+                           c->h_stage.ensure((size_t)kSlotReservePairs * 128, true) != hipSuccess ||
+                           c->h_results.ensure((size_t)kSlotReservePairs * sizeof(AlnResult), true) != hipSuccess))
       rc = fail(c, MHMSW_E_NOMEM, "cannot reserve the slot's buffers");
     if (c) c->tail_results = true;
     // first use of the slot's stream now, not inside the first request's latency

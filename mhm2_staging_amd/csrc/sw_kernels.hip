@@ -359,8 +359,11 @@ constexpr int kClassifyPerThread = 8;
 
 __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
   __shared__ int32_t hist[kNumBuckets];
+  __shared__ unsigned long long blk_cells;  // reverse cells of the block's pairs: one global atomic per block
   for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) hist[i] = 0;
+  if (threadIdx.x == 0) blk_cells = 0;
   __syncthreads();
+  unsigned long long my_cells = 0;
   const int64_t p0 = (int64_t)blockIdx.x * blockDim.x * kClassifyPerThread;
   int32_t bk[kClassifyPerThread], lr[kClassifyPerThread];
 #pragma unroll
@@ -390,10 +393,12 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
         const int64_t bw = 2 * (int64_t)(t.flags >> kTaskBandShift) + 1;
         w = w < bw ? w : bw;
       }
-      atomicAdd(&b.counters[0], (unsigned long long)t.row_len * (unsigned long long)w);
+      my_cells += (unsigned long long)t.row_len * (unsigned long long)w;
     }
   }
+  if (my_cells) atomicAdd(&blk_cells, my_cells);
   __syncthreads();
+  if (threadIdx.x == 0 && blk_cells) atomicAdd(&b.counters[0], blk_cells);
   for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) {
     const int32_t n = hist[i];
     hist[i] = n ? atomicAdd(&b.sched->bucket_count[i], n) : 0;
