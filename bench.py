@@ -368,8 +368,9 @@ def main():
            "roofline": roofline}
 
     al.set_graphs(True)
-    al.align_staged()
-    al.wait()
+    for _ in range(2):  # a geometry's first job runs eagerly, the second is captured
+        al.align_staged()
+        al.wait()
     t = time.perf_counter()
     for _ in range(args.steps):
         al.align_staged()

@@ -126,8 +126,10 @@ int mhmsw_set_score2(mhmsw_ctx *ctx, int enable);
  * CPU; its GPU mapping leaves them at 0 / "*", :378-385).  0 (default): off. */
 int mhmsw_set_cigar(mhmsw_ctx *ctx, int enable);
 /* 1 (default): the job of a staged batch (unpack, forward, reruns, reverse,
- * finalize: ~40 launches) is captured once as a hipGraph and replayed while
- * the staged geometry stays the same.  0: launches are issued one by one. */
+ * finalize: ~40 launches) runs eagerly the first time its geometry is seen,
+ * is captured as a hipGraph when the next job repeats that geometry, and is
+ * replayed while it stays the same (batches that change every call never pay
+ * for a capture).  0: launches are always issued one by one. */
 int mhmsw_set_graphs(mhmsw_ctx *ctx, int enable);
 int mhmsw_set_tuning(mhmsw_ctx *ctx, const mhmsw_tuning *tuning);
 int mhmsw_get_tuning(const mhmsw_ctx *ctx, mhmsw_tuning *tuning);

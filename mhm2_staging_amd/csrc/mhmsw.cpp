@@ -118,6 +118,7 @@ struct mhmsw_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_key;
+  std::vector<unsigned char> eager_key;  // geometry of the last eager job (captured if the next one repeats it)
   int64_t graph_captures = 0, graph_replays = 0;
   hipEvent_t ev[kEvents] = {};
   hipEvent_t cev[kNumClasses][2] = {};
@@ -1112,6 +1113,7 @@ static void drop_graph(mhmsw_ctx* c) {
   c->graph_exec = nullptr;
   c->graph = nullptr;
   c->graph_key.clear();
+  c->eager_key.clear();
 }
 
 int mhmsw_align_staged(mhmsw_ctx* c) {
@@ -1135,7 +1137,27 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   }
   for (int k = 0; k < kNumClasses; k++) c->class_launched[k] = c->npairs > 0 && c->fast_class_used[k];
   int maxp16 = 0, maxp64 = 0;
-  c->last_job_graph = c->use_graphs;
+  // The job depends only on the view, the classes used, the mode and the
+  // plain LDS size.  A geometry's first job runs eagerly; it is captured as a
+  // graph when the next job has the same geometry, and replayed while that
+  // lasts.  One-off geometries (a driver fed new batches, serving) never pay
+  // for a capture.
+  std::vector<unsigned char> key;
+  bool graph = false;
+  if (c->use_graphs) {
+    key.resize(sizeof(BatchView) + sizeof(c->fast_class_used) + 3 * sizeof(int) + sizeof(CigarView));
+    unsigned char* kp = key.data();
+    std::memcpy(kp, &b, sizeof(BatchView));
+    std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
+    const int extra[3] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar};
+    std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
+    if (c->cigar) {
+      const CigarView cv = make_cigar_view(c);
+      std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used) + sizeof(extra), &cv, sizeof(cv));
+    }
+    graph = (c->graph_exec && key == c->graph_key) || key == c->eager_key;
+  }
+  c->last_job_graph = graph;
   c->last_job_tail_results = c->tail_results && c->npairs > 0;
   if (c->last_job_tail_results) HIP_OR_FAIL(c, c->h_results.ensure((size_t)c->npairs * sizeof(AlnResult)));
   // after the job: its error words, counters and CIGAR pool use into pinned
@@ -1152,24 +1174,12 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     if (e == hipSuccess) e = hipEventRecord(c->ev_done, st);
     return e;
   };
-  if (!c->use_graphs) {
+  if (!graph) {
     int rc = enqueue_job(c, b, st, true, &maxp16, &maxp64);
     if (rc != MHMSW_OK) return rc;
     HIP_OR_FAIL(c, enqueue_tail());
+    if (c->use_graphs) c->eager_key = key;
   } else {
-    // the job depends only on the view, the classes used, the mode and the
-    // plain LDS size: replay the captured graph while those are unchanged
-    std::vector<unsigned char> key(sizeof(BatchView) + sizeof(c->fast_class_used) + 3 * sizeof(int) +
-                                   sizeof(CigarView));
-    unsigned char* kp = key.data();
-    std::memcpy(kp, &b, sizeof(BatchView));
-    std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
-    const int extra[3] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar};
-    std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
-    if (c->cigar) {
-      const CigarView cv = make_cigar_view(c);
-      std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used) + sizeof(extra), &cv, sizeof(cv));
-    }
     if (!c->graph_exec || key != c->graph_key) {
       drop_graph(c);
       HIP_OR_FAIL(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
