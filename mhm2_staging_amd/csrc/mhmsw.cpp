@@ -25,6 +25,7 @@
 #include "cigar_core.h"
 #include "host_pinned.h"
 #include "pack2bit.h"
+#include "thread_pool.h"
 
 using namespace mhmsw;
 
@@ -482,15 +483,7 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   pb.len.resize((size_t)std::max<int64_t>(nseq, 1));
   int nth = std::max(1, std::min(max_threads, (int)std::max(1u, std::thread::hardware_concurrency())));
   if (npairs < 4096) nth = 1;
-  auto run = [&](auto&& fn) {
-    if (nth == 1) {
-      fn(0);
-      return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; t++) th.emplace_back(fn, t);
-    for (auto& t : th) t.join();
-  };
+  auto run = [&](auto&& fn) { hostpool::pool().run(nth, std::function<void(int)>(fn)); };
   // pass 1, over contiguous chunks of pairs: validate, lay out each chunk from
   // offset 0, route, and sum
   // String batches interleave each pair's read and window.  Windowed batches
@@ -693,14 +686,10 @@ static void par_memcpy(void* dst, const void* src, size_t bytes) {
     std::memcpy(dst, src, bytes);
     return;
   }
-  auto part = [&](int t) {
+  hostpool::pool().run(nth, [&](int t) {
     const size_t a = bytes * (size_t)t / (size_t)nth, e = bytes * (size_t)(t + 1) / (size_t)nth;
     std::memcpy(static_cast<uint8_t*>(dst) + a, static_cast<const uint8_t*>(src) + a, e - a);
-  };
-  std::vector<std::thread> th;
-  for (int t = 1; t < nth; t++) th.emplace_back(part, t);
-  part(0);
-  for (auto& t : th) t.join();
+  });
 }
 
 // Device buffers of a job of npairs pairs over total padded bases (they only

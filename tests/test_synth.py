@@ -32,3 +32,16 @@ def test_config_shapes():
 def test_from_strings_roundtrip():
     b = m.from_strings(["ACGT", "", "NNA"], ["AACGT", "G", ""])
     assert b.npairs == 3 and b.read(0) == "ACGT" and b.ctg(0) == "AACGT" and b.read(2) == "NNA" and b.ctg(2) == ""
+
+
+def test_host_worker_pool(tmp_path):
+    """The host worker pool used for packing and staging copies runs every index
+    of every region exactly once, with regions issued from several threads."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "thread_pool_test"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", os.path.join(root, "tests", "cpp", "thread_pool_test.cpp"),
+                    "-o", str(exe)], check=True)
+    p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
