@@ -129,11 +129,9 @@ struct mhmsw_ctx {
   mhmsw_stats stats{};
   int32_t h_errors[4] = {};
   // An upload goes through pinned staging with asynchronous copies on the
-  // context stream; ev_upload follows them, and the staging (and the device
-  // buffers they fill) are reused only once it has passed.
+  // context stream; the next upload first waits for that stream, so the
+  // staging (and the device buffers the copies fill) are reused only after.
   HostPinned h_stage;
-  hipEvent_t ev_upload = nullptr;
-  bool upload_pending = false;
   // Words the job copies back into pinned memory at its end, before ev_done:
   // completion needs no further copy (a null-stream copy can wait behind
   // another context's work on a shared hardware queue).
@@ -259,8 +257,7 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
       return MHMSW_E_HIP;
     }
   }
-  if (hipEventCreateWithFlags(&c->ev_upload, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
     mhmsw_destroy(c);
     return MHMSW_E_HIP;
   }
@@ -310,7 +307,6 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   c->d_cig_used.release();
   for (int i = 0; i < kEvents; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
-  if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
   if (c->ev_done) (void)hipEventDestroy(c->ev_done);
   if (c->tail) (void)hipHostFree(c->tail);
   c->h_stage.release();
@@ -724,10 +720,7 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
     if (rc != MHMSW_OK) return rc;
   }
   HIP_OR_FAIL(c, hipSetDevice(c->device));
-  if (c->upload_pending) {
-    HIP_OR_FAIL(c, hipEventSynchronize(c->ev_upload));
-    c->upload_pending = false;
-  }
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));  // copies of an earlier upload (a stage without a job)
   c->staged = false;
   c->have_results = false;
   int64_t npairs = 0, total = 0;
@@ -852,8 +845,6 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_ctg.ptr, swc, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_win_off.ptr, swo, (size_t)npairs * 4, hipMemcpyHostToDevice, c->stream));
   }
-  HIP_OR_FAIL(c, hipEventRecord(c->ev_upload, c->stream));
-  c->upload_pending = true;
   c->staged = true;
   return MHMSW_OK;
 }
@@ -1245,7 +1236,6 @@ static int settle_cigar(mhmsw_ctx* c) {
 
 static int complete(mhmsw_ctx* c) {
   c->in_flight = false;
-  c->upload_pending = false;  // ev_done follows the upload's copies
   std::memcpy(c->h_errors, c->tail->errors, sizeof(c->h_errors));
   unsigned long long counters[2] = {0, 0};
   std::memcpy(counters, c->tail->counters, sizeof(counters));

@@ -59,7 +59,8 @@ def test_server_results_same_alone_and_in_the_mix(serving_set):
     assert overlapped > 0, "no two requests were on the device together"
     for r, a, (b, t) in zip(reqs, alone, mixed):
         assert (a == b).all()
-        assert t["submit_s"] <= t["admit_s"] <= t["done_s"]
+        assert t["submit_s"] <= t["admit_s"] <= t["launched_s"] <= t["finished_s"] <= t["done_s"]
+        assert t["device_ms"] > 0
     # and they are the SSW answers
     for r, a in zip(reqs[:3], alone[:3]):
         sb = r.as_strings(cs)
