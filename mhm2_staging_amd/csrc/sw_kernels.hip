@@ -180,33 +180,33 @@ __global__ void extract_windows_kernel(BatchView b) {
   }
 }
 
-// one block per sequence: reversed copy at [total + base, total + base + len).
-// Output dword w holds reversed positions 8w..8w+7 = source positions
-// len-1-8w down to len-8-8w: one funnel shift of two source dwords, then a
-// nibble reversal (byte permute + nibble swap).  Nibbles past len are 0.
-__device__ void reverse_one_impl(BatchView b, int64_t s, int lane);
-__device__ __forceinline__ void reverse_one(const BatchView& b, int64_t s, int lane) { reverse_one_impl(b, s, lane); }
-
+// Reversed copies: sequence s at [total + base, total + base + len).  Output
+// dword w holds reversed positions 8w..8w+7 = source positions len-1-8w down
+// to len-8-8w: one funnel shift of two source dwords, then a nibble reversal
+// (byte permute + nibble swap).  Nibbles past len are 0.  One wavefront per
+// pair: the output dwords of its read and its window form one flat range over
+// the lanes (a 150-base read alone would leave 45 of 64 lanes idle).
 __global__ void reverse_kernel(BatchView b) {
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
-  for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; s < 2 * b.npairs; s += nwaves)
-    reverse_one(b, s, threadIdx.x & 63);
-}
-
-__device__ void reverse_one_impl(BatchView b, int64_t s, int lane) {
-  const int64_t base = b.seq_base[s];
-  const int32_t len = b.seq_len[s];
-  const int32_t ndw = (len + 7) >> 3;
-  const int64_t out0 = (b.total + base) >> 3;
-  for (int32_t w = lane; w < ndw; w += 64) {
-    const int64_t s0 = base + len - 8 - 8 * (int64_t)w;  // >= base - 8 >= 0 (front pad)
-    const uint32_t d0 = b.codes[s0 >> 3], d1 = b.codes[(s0 >> 3) + 1];
-    const uint32_t v = __builtin_amdgcn_alignbit(d1, d0, (uint32_t)(s0 & 7) * 4u);
-    const uint32_t y = __builtin_amdgcn_perm(v, v, 0x00010203u);  // byte reverse
-    uint32_t out = ((y >> 4) & 0x0F0F0F0Fu) | ((y << 4) & 0xF0F0F0F0u);
-    const int32_t keep = len - 8 * w;  // valid nibbles in this dword
-    if (keep < 8) out &= (1u << (4 * keep)) - 1u;
-    b.codes[out0 + w] = out;
+  const int lane = threadIdx.x & 63;
+  for (int64_t p = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; p < b.npairs; p += nwaves) {
+    const int64_t base0 = b.seq_base[2 * p], base1 = b.seq_base[2 * p + 1];
+    const int32_t len0 = b.seq_len[2 * p], len1 = b.seq_len[2 * p + 1];
+    const int32_t n0 = (len0 + 7) >> 3, n = n0 + ((len1 + 7) >> 3);
+    for (int32_t k = lane; k < n; k += 64) {
+      const bool first = k < n0;
+      const int64_t base = first ? base0 : base1;
+      const int32_t len = first ? len0 : len1;
+      const int32_t w = first ? k : k - n0;
+      const int64_t s0 = base + len - 8 - 8 * (int64_t)w;  // >= base - 8 >= 0 (front pad)
+      const uint32_t d0 = b.codes[s0 >> 3], d1 = b.codes[(s0 >> 3) + 1];
+      const uint32_t v = __builtin_amdgcn_alignbit(d1, d0, (uint32_t)(s0 & 7) * 4u);
+      const uint32_t y = __builtin_amdgcn_perm(v, v, 0x00010203u);  // byte reverse
+      uint32_t out = ((y >> 4) & 0x0F0F0F0Fu) | ((y << 4) & 0xF0F0F0F0u);
+      const int32_t keep = len - 8 * w;  // valid nibbles in this dword
+      if (keep < 8) out &= (1u << (4 * keep)) - 1u;
+      b.codes[((b.total + base) >> 3) + w] = out;
+    }
   }
 }
 
@@ -1188,7 +1188,7 @@ hipError_t launch_unpack(const BatchView& b, hipStream_t st) {
   hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 3, 256)), dim3(256), 0, st, b);
   if (b.win_ctg && b.npairs > 0)
     hipLaunchKernelGGL(extract_windows_kernel, dim3(grid_for(b.npairs, 4)), dim3(256), 0, st, b);
-  if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3(grid_for(2 * b.npairs, 4)), dim3(256), 0, st, b);
+  if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3(grid_for(b.npairs, 4)), dim3(256), 0, st, b);
   return hipGetLastError();
 }
 
