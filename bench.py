@@ -2,9 +2,14 @@
 
 One step = one whole job over the staged batch (unpack, forward pass, 32-bit
 reruns, reverse pass, finalize) on inputs already resident in HBM.  Prints one
-JSON line.  Default workload: the uneven batched launch (1M pairs, reads
-U{100..250}, windows U{150..1024}, 0.5% N, 2% substitutions, 0.2% indels, 10%
-unrelated pairs) with mhm2's 1/3/5/2/2 scoring.
+JSON line.  Headline workload: the length-skewed 4M-pair batch (windows
+log-normal with median 400 bp clipped to [150, 1024], reads 150/250 bp split
+70/30, 25% of reads with a 20-60 bp random chimeric tail) with mhm2's
+1/3/5/2/2 scoring, autotuned, compared with the plain one-launch form on the
+same GPU and with the reference CPU aligner (SSW) on the host cores, every
+pair checked bit-exact.  The other batch workloads (the 10k plain pairs, the
+uneven 1M-pair batch and the 500k overflow pairs) run as secondary legs with
+the same checks; local assembly and the serving loop follow.
 """
 import argparse
 import json
@@ -18,51 +23,267 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "batched SW GCUPS + pairs/s, bit-exact vs SSW; p99 latency at fixed QPS"
+# the synthetic generator's workload numbers (mhmsw_synth_generate) and what each one is
 WORKLOADS = {
-    1: "plain SW: 10k pairs, 150 bp reads vs 350 bp windows, 1% subs, 0.1% indels",
-    2: "uneven batched launch: 1M pairs, reads U{100..250} vs windows U{150..1024}, 0.5% N, 2% subs, 0.2% indels, 10% unrelated",
-    3: "length-skewed batch: 4M pairs, windows lognormal(median 400) in [150,1024], reads 150/250 (70/30), 25% chimeric tails",
-    4: "overflow-exact: 500k pairs, 250 bp reads vs 1024 bp tandem-repeat windows, 5% 4000 bp reads",
+    1: ("plain_10k", "plain SW: 10k pairs, 150 bp reads vs 350 bp windows, 1% subs, 0.1% indels"),
+    2: ("uneven_1M", "uneven batched launch: 1M pairs, reads U{100..250} vs windows U{150..1024}, 0.5% N, "
+                     "2% subs, 0.2% indels, 10% unrelated"),
+    3: ("skewed_4M", "length-skewed batch: 4M pairs, windows lognormal(median 400) in [150,1024], reads 150/250 "
+                     "(70/30), 25% chimeric tails"),
+    4: ("overflow_500k", "overflow-exact: 500k pairs, 250 bp reads vs 1024 bp tandem-repeat windows, "
+                         "5% 4000 bp reads"),
 }
+HEADLINE = 3
 VALU_OPS_PER_CELL = 10  # t=H-gO, E, F (2 each), d=Hdiag+s, H=max of 4, best=max(best,H)
+NOMINAL_PEAK = 78.6     # 256 CUs x 64 lanes x 2 int16 per lane x 2.4 GHz, T lane-ops/s
+SCORING_TEXT = "match 1, mismatch 3, gap open 5, gap extend 2, N 2"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(batch, scoring, seconds_target=12.0):
-    """Reference SSW (oracle/_ref) or, when absent, the oracle port, timed on host cores."""
+def host_cores():
+    """Host cores this job may use: the affinity mask, bounded by a cgroup CPU quota when one is set and
+    by the host share of one GPU on the GPU machines (16 cores)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        quota = None
+    cores = min(aff, quota) if quota else aff
+    share = 16
+    info = {"os_cpu_count": os.cpu_count(), "affinity": aff, "cgroup_quota": quota, "gpu_host_share": share}
+    return min(cores, share), info
+
+
+def spread(n, k):
+    """k pair indices spread evenly over [0, n)."""
+    if k >= n:
+        return np.arange(n, dtype=np.int64)
+    return np.unique(np.linspace(0, n - 1, k).astype(np.int64))
+
+
+def ref_run(batch, scoring, idx, nthreads):
+    """Reference SSW (oracle/_ref) or, when absent, the oracle port on the pairs idx; (results, kind, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-
     kind = "reference" if oracle_lib.have_ref() else "port"
     fn = oracle_lib.ref_batch if kind == "reference" else oracle_lib.oracle_batch
-    cores = min(os.cpu_count() or 1, 16)
+    sel = slice(None) if idx is None else idx
+    t = time.perf_counter()
+    res = fn(batch.reads, batch.read_off[sel], batch.read_len[sel], batch.ctgs, batch.ctg_off[sel],
+             batch.ctg_len[sel], scoring, nthreads)
+    return res, kind, time.perf_counter() - t
+
+
+def cells_of(batch, idx=None):
+    sel = slice(None) if idx is None else idx
+    return int(np.dot(batch.read_len[sel].astype(np.int64), batch.ctg_len[sel].astype(np.int64)))
+
+
+def compare(res, ref, idx=None):
+    got = np.stack([res[c] for c in ("score", "read_begin", "read_end", "ctg_begin", "ctg_end")], 1)
+    if idx is not None:
+        got = got[idx]
+    want = np.stack([ref[c] for c in ("score", "ref_begin", "ref_end", "query_begin", "query_end")], 1)
+    return int((got != want).any(1).sum())
+
+
+def cpu_legs(batch, scoring, res, cores, one_core_seconds=8.0):
+    """cpu_baseline (every pair on all usable host cores, which is also the bit-exact check) and a one-core
+    figure on pairs spread over the batch, sized for about one_core_seconds."""
     n = batch.npairs
-    # calibrate on a small slice, then run a slice sized for ~seconds_target
-    k = min(n, 2000)
-    t = time.perf_counter()
-    fn(batch.reads, batch.read_off[:k], batch.read_len[:k], batch.ctgs, batch.ctg_off[:k], batch.ctg_len[:k],
-       scoring, cores)
-    dt = max(time.perf_counter() - t, 1e-6)
-    k2 = int(min(n, max(k, k * seconds_target / dt)))
-    sl = slice(0, k2)
-    cells = int(np.dot(batch.read_len[sl].astype(np.int64), batch.ctg_len[sl].astype(np.int64)))
-    t = time.perf_counter()
-    res = fn(batch.reads, batch.read_off[sl], batch.read_len[sl], batch.ctgs, batch.ctg_off[sl], batch.ctg_len[sl],
-             scoring, cores)
-    dt = time.perf_counter() - t
-    return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": kind,
-            "pairs_per_s": round(k2 / dt, 1),
-            "sample": f"first {k2} of {n} pairs of the same workload ({cells} cells), {dt:.1f} s"}, res, k2
+    ref, kind, dt = ref_run(batch, scoring, None, cores)
+    cells = cells_of(batch)
+    out = {"cpu_baseline": {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": kind,
+                            "pairs_per_s": round(n / dt, 1),
+                            "sample": f"all {n} pairs of the same workload ({cells} cells), {dt:.1f} s, one pair "
+                                      f"per task over {cores} threads"},
+           "bit_exact_check": {"pairs": n, "mismatches": compare(res, ref), "against": kind,
+                               "fields": "score, read_begin, read_end, ctg_begin, ctg_end"}}
+    # one core: calibrate on a small spread sample, then time one sized for one_core_seconds
+    k = min(n, 500)
+    idx = spread(n, k)
+    _, _, t1 = ref_run(batch, scoring, idx, 1)
+    k = int(min(n, max(k, k * one_core_seconds / max(t1, 1e-6))))
+    idx = spread(n, k)
+    r1, _, t1 = ref_run(batch, scoring, idx, 1)
+    c1 = cells_of(batch, idx)
+    out["cpu_one_core"] = {"value": round(c1 / t1 / 1e9, 3), "unit": "GCUPS", "cores": 1, "kind": kind,
+                           "pairs_per_s": round(len(idx) / t1, 1),
+                           "sample": f"{len(idx)} pairs spread over the batch ({c1} cells), {t1:.1f} s",
+                           "mismatches": compare(res, r1, idx)}
+    return out
 
 
-def cigar_leg(al, batch, res, scoring, steps, check):
+def class_roofline(st, class_ms, peak_measured, traffic):
+    """Roofline of the forward class with the most device time (HIP events around its launch on the context
+    stream): achieved = 10 int16 lane-ops per cell of the class / its launch's duration."""
+    from mhm2_staging_amd import _native
+    cm = np.mean(np.array(class_ms), 0)
+    k = int(np.argmax(cm))
+    kcells = st["class_cells"][k]
+    achieved = VALU_OPS_PER_CELL * kcells / (cm[k] * 1e-3) / 1e12
+    name = _native.class_name(k)
+    rl = {"bound": "valu", "kernel": name, "achieved": round(achieved, 2), "peak": NOMINAL_PEAK,
+          "unit": "TOP/s (int16 lane-ops)", "frac": round(achieved / NOMINAL_PEAK, 4), "traffic": None,
+          "kernel_ms": round(float(cm[k]), 3), "kernel_cells": int(kcells), "kernel_pairs": int(st["class_pairs"][k]),
+          "ops_per_cell": VALU_OPS_PER_CELL}
+    if peak_measured:
+        rl["peak_measured"] = round(peak_measured["lane_ops_per_s"] / 1e12, 2)
+        rl["frac_of_measured"] = round(achieved * 1e12 / peak_measured["lane_ops_per_s"], 4)
+    if traffic and name in traffic.get("per_launch_bytes", {}):
+        rl["traffic"] = traffic["per_launch_bytes"][name]
+        rl["traffic_source"] = traffic["source"]
+    classes = {_native.class_name(i): {"pairs": st["class_pairs"][i], "cells": st["class_cells"][i],
+                                       "ms": round(float(cm[i]), 3)}
+               for i in range(_native.NUM_CLASSES) if st["class_pairs"][i]}
+    return rl, classes
+
+
+def load_traffic(key):
+    """HBM bytes per launch of the forward classes of a workload, from rocprofv3 PMC runs under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tr = json.load(f)
+    return tr.get("workloads", {}).get(key)
+
+
+def timed_jobs(al, steps):
+    """Time `steps` jobs on the staged batch (eager launches, so every kernel's HIP events are live)."""
+    job_ms, class_ms = [], []
+    t = time.perf_counter()
+    for _ in range(steps):
+        al.align_staged()
+        al.wait()
+        st = al.stats()
+        job_ms.append(st["job_ms"])
+        class_ms.append(st["class_ms"])
+    return time.perf_counter() - t, st, job_ms, class_ms
+
+
+def plain_leg(al, batch, res, gcups):
+    """The plain form (one 32-bit launch per batch, one wavefront per pair) on the same staged batch."""
+    import mhm2_staging_amd as m
+    al.set_mode(m.MODE_PLAIN)
+    al.align_staged()
+    al.wait()
+    t = time.perf_counter()
+    al.align_staged()
+    al.wait()
+    pe = time.perf_counter() - t
+    pres = al.results()
+    al.set_mode(m.MODE_FAST)
+    plain = batch.cells() / pe / 1e9
+    return {"value": round(plain, 2), "unit": "GCUPS", "ms_per_step": round(pe * 1e3, 3),
+            "fast_over_plain": round(gcups / plain, 2), "identical_results": bool((pres == res).all())}
+
+
+def batch_leg(cfg, args, cores, peak, headline=False):
+    """One batch workload: autotuned fast form (timed), plain form, CPU legs and the bit-exact check."""
+    import mhm2_staging_amd as m
+    key, text = WORKLOADS[cfg]
+    scoring = m.MHM2_SCORING
+    steps = args.steps if headline else max(3, min(args.steps, 10))
+    t0 = time.perf_counter()
+    batch = m.generate(cfg, npairs=args.npairs if headline else 0)
+    log(f"[bench] {key}: generated {batch.npairs} pairs, {batch.cells():.3e} cells in {time.perf_counter() - t0:.1f}s")
+    al = m.Aligner(device=0, scoring=scoring, read_is_ssw_ref=True, mode=m.MODE_FAST)
+    # eager launches in the timed region so each kernel's HIP events are live
+    # (a replayed job graph reports only its total time); the graph-replay
+    # rate of the same job is reported separately below
+    al.set_graphs(False)
+    t0 = time.perf_counter()
+    al.stage(batch)
+    stage_s = time.perf_counter() - t0
+    out = {"workload": text, "npairs": batch.npairs, "cells": batch.cells(), "stage_s": round(stage_s, 3)}
+    default_tuning = al.get_tuning()
+    if headline and not args.tuning and not args.no_autotune:
+        # the default scheduling parameters, timed before autotuning changes them
+        for _ in range(args.warmup):
+            al.align_staged()
+            al.wait()
+        el, st, _, _ = timed_jobs(al, steps)
+        out["default_tuning"] = {"tuning": default_tuning, "value": round(batch.cells() * steps / el / 1e9, 2),
+                                 "unit": "GCUPS", "ms_per_step": round(el / steps * 1e3, 3)}
+    if headline and args.tuning:
+        b_, g_, s_ = (int(x) for x in args.tuning.split(","))
+        al.set_tuning(blocks_per_cu=b_, g64_min_rows=g_, bin_shift=s_)
+    elif not args.no_autotune:
+        t0 = time.perf_counter()
+        al.autotune(repeats=2)
+        log(f"[bench] {key}: autotuned {al.get_tuning()} in {time.perf_counter() - t0:.1f}s")
+    out["tuning"] = al.get_tuning()
+    for _ in range(args.warmup):
+        al.align_staged()
+        al.wait()
+    el, st, job_ms, class_ms = timed_jobs(al, steps)
+    res = al.results()
+    cells = batch.cells()
+    gcups = cells * steps / el / 1e9
+    log(f"[bench] {key}: fast form {gcups:.1f} GCUPS, {el / steps * 1e3:.2f} ms/step, job {st['job_ms']:.2f} ms")
+    if "default_tuning" in out:
+        out["default_tuning"]["autotuned_over_default"] = round(gcups / out["default_tuning"]["value"], 3)
+    rl, classes = class_roofline(st, class_ms, peak, load_traffic(key))
+    out.update({"value": round(gcups, 2), "unit": "GCUPS", "steps": steps,
+                "ms_per_step": round(el / steps * 1e3, 3), "pairs_per_s": round(batch.npairs * steps / el, 1),
+                "device_job_ms": round(float(np.mean(job_ms)), 3),
+                "stats": {kk: v for kk, v in st.items() if not kk.startswith("class_")},
+                "classes": classes, "roofline": rl})
+    if headline:
+        al.set_graphs(True)
+        for _ in range(2):  # a geometry's first job runs eagerly, the second is captured
+            al.align_staged()
+            al.wait()
+        t = time.perf_counter()
+        for _ in range(steps):
+            al.align_staged()
+            al.wait()
+        ge = time.perf_counter() - t
+        out["graph_replay"] = {"value": round(cells * steps / ge / 1e9, 2), "unit": "GCUPS",
+                               "ms_per_step": round(ge / steps * 1e3, 3),
+                               "identical_results": bool((al.results() == res).all())}
+        al.set_graphs(False)
+        # host-inclusive rate: host strings in, host results out (2-bit packing on
+        # the host, PCIe both ways, the job); never the headline value
+        t = time.perf_counter()
+        e2e_steps = 2
+        for _ in range(e2e_steps):
+            al.stage(batch)
+            al.align_staged()
+            al.wait()
+            eres = al.results()
+        ee = time.perf_counter() - t
+        out["end_to_end"] = {"value": round(cells * e2e_steps / ee / 1e9, 2), "unit": "GCUPS",
+                             "ms_per_step": round(ee / e2e_steps * 1e3, 3),
+                             "includes": "host 2-bit packing, H2D copy, job, D2H of results",
+                             "identical_results": bool((eres == res).all())}
+        log(f"[bench] {key}: end to end {out['end_to_end']}")
+    if not args.no_plain:
+        out["plain_form"] = plain_leg(al, batch, res, gcups)
+        log(f"[bench] {key}: plain form {out['plain_form']}")
+    if headline and not args.no_cigar:
+        out["cigar_pass"] = cigar_leg(al, batch, res, scoring, steps, not args.no_cpu, cores)
+        log(f"[bench] {key}: cigar pass {out['cigar_pass']}")
+    al.close()
+    if not args.no_cpu:
+        out.update(cpu_legs(batch, scoring, res, cores, 8.0 if headline else 3.0))
+        log(f"[bench] {key}: cpu {out['cpu_baseline']} one core {out['cpu_one_core']['value']} GCUPS, "
+            f"bit-exact {out['bit_exact_check']}")
+    return out
+
+
+def cigar_leg(al, batch, res, scoring, steps, check, cores):
     """Jobs with the CIGAR pass on (SSW's CIGAR + mismatches, klign with report_cigar): its device time,
     the job's rate, and -- when check -- the reference SSW with report_cigar on host cores on a bounded
-    sample, every GPU CIGAR of that sample compared with the reference's."""
-    import mhm2_staging_amd as m
+    sample spread over the batch, every GPU CIGAR of that sample compared with the reference's."""
     al.set_cigar(True)
     al.align_staged()
     al.wait()
@@ -89,33 +310,71 @@ def cigar_leg(al, batch, res, scoring, steps, check):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
         if oracle_lib.have_ref():
-            cores = min(os.cpu_count() or 1, 16)
-            k = min(batch.npairs, 2000)
-            args = lambda n: (batch.reads, batch.read_off[:n], batch.read_len[:n], batch.ctgs, batch.ctg_off[:n],
-                              batch.ctg_len[:n], scoring)
+            n = batch.npairs
+
+            def run(idx, maxops):
+                return oracle_lib.ref_cigar_batch(batch.reads, batch.read_off[idx], batch.read_len[idx], batch.ctgs,
+                                                  batch.ctg_off[idx], batch.ctg_len[idx], scoring, maxops=maxops,
+                                                  nthreads=cores)
+            idx = spread(n, min(n, 2000))
             t = time.perf_counter()
-            oracle_lib.ref_cigar_batch(*args(k), maxops=64, nthreads=cores)
+            run(idx, 64)
             dt = max(time.perf_counter() - t, 1e-6)
-            k = int(min(batch.npairs, max(k, k * 10.0 / dt)))
-            maxops = int(np.diff(off[:k + 1]).max(initial=0)) + 8
+            idx = spread(n, int(min(n, max(len(idx), len(idx) * 10.0 / dt))))
+            nops_gpu = np.diff(off)[idx]
+            maxops = int(nops_gpu.max(initial=0)) + 8
             t = time.perf_counter()
-            rres, rmism, rnops, rops = oracle_lib.ref_cigar_batch(*args(k), maxops=maxops, nthreads=cores)
+            rres, rmism, rnops, rops = run(idx, maxops)
             dt = time.perf_counter() - t
-            kc = int(np.dot(batch.read_len[:k].astype(np.int64), batch.ctg_len[:k].astype(np.int64)))
-            bad = (np.diff(off[:k + 1]) != rnops) | (mism[:k] != rmism)
-            for p in np.nonzero(~bad)[0]:
-                if not np.array_equal(ops[off[p]:off[p + 1]], rops[p, :rnops[p]]):
-                    bad[p] = True
+            kc = cells_of(batch, idx)
+            bad = (nops_gpu != rnops) | (mism[idx] != rmism)
+            for q in np.nonzero(~bad)[0]:
+                p = idx[q]
+                if not np.array_equal(ops[off[p]:off[p + 1]], rops[q, :rnops[q]]):
+                    bad[q] = True
             out["cpu_reference"] = {"value": round(kc / dt / 1e9, 3), "unit": "GCUPS", "cores": cores,
-                                    "pairs_per_s": round(k / dt, 1),
-                                    "sample": f"first {k} of {batch.npairs} pairs, reference SSW with "
-                                              f"report_cigar, {dt:.1f} s"}
-            out["bit_exact_check"] = {"pairs": k, "mismatches": int(bad.sum()), "against": "reference",
+                                    "pairs_per_s": round(len(idx) / dt, 1),
+                                    "sample": f"{len(idx)} of {n} pairs spread over the batch, reference SSW "
+                                              f"with report_cigar, {dt:.1f} s"}
+            out["bit_exact_check"] = {"pairs": int(len(idx)), "mismatches": int(bad.sum()), "against": "reference",
                                       "fields": "CIGAR ops, mismatches"}
     return out
 
 
-def locassm_leg(nctgs, steps, warmup, check):
+def klign_call_leg(calls=40, npairs=20000):
+    """klign-sized calls: new batches of 20k pairs of 150 bp reads vs 350 bp windows, host strings in and
+    results out per call (stage = pack + copy, the job, the result copy), as the drop-in driver is called
+    once per block (klign.cpp:356-388)."""
+    import mhm2_staging_amd as m
+    batches = [m.generate(1, npairs=npairs, seed=101 + k) for k in range(4)]
+    al = m.Aligner(device=0, scoring=m.MHM2_SCORING, read_is_ssw_ref=True)
+    for b in batches:  # every buffer at its size, the worker pool started
+        al.align(b)
+    stage, job, take, wall = [], [], [], []
+    for i in range(calls):
+        b = batches[i % len(batches)]
+        t0 = time.perf_counter()
+        al.stage(b)
+        t1 = time.perf_counter()
+        al.align_staged()
+        al.wait()
+        t2 = time.perf_counter()
+        al.results()
+        t3 = time.perf_counter()
+        stage.append(t1 - t0)
+        job.append(t2 - t1)
+        take.append(t3 - t2)
+        wall.append(t3 - t0)
+    al.close()
+    cells = float(np.mean([b.cells() for b in batches]))
+    w = float(np.median(wall))
+    return {"workload": f"{npairs} pairs per call, 150 bp reads vs 350 bp windows, a new batch every call",
+            "calls": calls, "ms_per_call": round(w * 1e3, 3), "value": round(cells / w / 1e9, 1), "unit": "GCUPS",
+            "stage_ms": round(float(np.median(stage)) * 1e3, 3), "job_ms": round(float(np.median(job)) * 1e3, 3),
+            "results_ms": round(float(np.median(take)) * 1e3, 3)}
+
+
+def locassm_leg(nctgs, steps, warmup, check, cores):
     """Local assembly (contig end extension) on a generated batch resident in
     HBM: one step = both sides of every contig walked at every mer length
     the schedule visits (what local_assem_driver does for a slice)."""
@@ -162,7 +421,6 @@ def locassm_leg(nctgs, steps, warmup, check):
     if check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
-        cores = min(os.cpu_count() or 1, 16)
         t = time.perf_counter()
         ll, le, rl, re, tm = oracle_lib.la_extend_batch(oracle_lib.la_params(**kw), b, nthreads=cores, with_terms=True)
         dt = time.perf_counter() - t
@@ -180,7 +438,8 @@ def locassm_leg(nctgs, steps, warmup, check):
 
 
 def serving_bench(qps_list, duration, slots=4, seed=5):
-    """Config 5: 64k resident contigs, Poisson requests of log-uniform 1e3..1e5 pairs."""
+    """Serving loop: 64k resident contigs, Poisson requests of log-uniform 1e3..1e5 pairs, at each request rate
+    of qps_list; plus the HBM rate of the window extraction from the resident contigs."""
     import mhm2_staging_amd as m
     rng = np.random.default_rng(seed)
     cs = m.contigs(nctgs=64000, seed=seed)
@@ -190,6 +449,27 @@ def serving_bench(qps_list, duration, slots=4, seed=5):
     for k in range(48):
         n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
         pool.append(m.window_pairs(cs, n, seed=1000 + k))
+    # window extraction (resident 2-bit codes + N bitmap -> nibble codes of every window) on one large
+    # windowed batch, eager launches so its HIP events are live
+    big = m.window_pairs(cs, 1 << 20, seed=999)
+    al = m.Aligner(device=0, scoring=m.MHM2_SCORING)
+    al.set_graphs(False)
+    al.stage_windows(res, big)
+    wms = []
+    for _ in range(4):
+        al.align_staged()
+        al.wait()
+        wms.append(al.stats()["window_ms"])
+    st = al.stats()
+    al.close()
+    wbases = st["window_bases"]
+    wbytes = wbases * (0.25 + 0.125 + 0.5)  # read 2-bit codes and N bits, write 4-bit codes
+    wmed = float(np.median(wms[1:]))
+    extraction = {"pairs": big.npairs, "window_bases": int(wbases), "ms": round(wmed, 4),
+                  "algorithmic_bytes": int(wbytes), "achieved_gbs": round(wbytes / (wmed * 1e-3) / 1e9, 1),
+                  "peak_gbs": 8000.0, "frac": round(wbytes / (wmed * 1e-3) / 8e12, 4),
+                  "bytes_per_base": 0.875}
+    log(f"[bench] serving: window extraction {extraction}")
     srv = m.Server(res, scoring=m.MHM2_SCORING, slots=slots)
     rid = srv.submit(pool[0])  # warm up every code path once
     srv.results(rid)
@@ -207,7 +487,7 @@ def serving_bench(qps_list, duration, slots=4, seed=5):
                 time.sleep(d)
             b = pool[i % len(pool)]
             ids.append((srv.submit(b), b.cells(), b.npairs, c0 + a))
-        lat, cells, first, last, pairs = [], 0, None, None, 0
+        lat, cells, first, last, pairs, dev = [], 0, None, None, 0, 0.0
         for rid, c, n, arrival in ids:
             _, t = srv.results(rid)
             lat.append(t["done_s"] - arrival)
@@ -217,35 +497,43 @@ def serving_bench(qps_list, duration, slots=4, seed=5):
             last = t["done_s"] if last is None else max(last, t["done_s"])
         lat = np.array(lat) * 1e3
         span = last - first
+        offered = sum(c for _, c, _, _ in ids) / duration
         points.append({"qps": qps, "requests": len(ids), "p50_ms": round(float(np.percentile(lat, 50)), 2),
                        "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                       "offered_gcups": round(offered / 1e9, 1),
                        "sustained_gcups": round(cells / span / 1e9, 1), "pairs_per_s": round(pairs / span, 1)})
         log(f"[bench] serving {points[-1]}")
     srv.close()
     res.close()
     return {"workload": "64k resident contigs (log-uniform 500..20000 bp), Poisson requests of log-uniform "
                         "1e3..1e5 pairs (150 bp reads, 1% error) vs U[300,1024] windows",
-            "slots": slots, "duration_s_per_point": duration, "points": points}
+            "slots": slots, "duration_s_per_point": duration, "points": points, "window_extraction": extraction}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
-    ap.add_argument("--npairs", type=int, default=0)
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison run")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop latency measurement")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=HEADLINE, choices=sorted(WORKLOADS),
+                    help="headline batch workload (generator number: 1 plain 10k, 2 uneven 1M, 3 skewed 4M, "
+                         "4 overflow 500k)")
+    ap.add_argument("--npairs", type=int, default=0, help="headline batch size (0: the workload's own)")
+    ap.add_argument("--secondary", type=str, default="", metavar="N,N",
+                    help="other batch workloads to run after the headline (default: every other one)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison runs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and bit-exact checks")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-cigar", action="store_true", help="skip the CIGAR-pass measurement")
     ap.add_argument("--no-locassm", action="store_true", help="skip the local assembly measurement")
+    ap.add_argument("--no-klign", action="store_true", help="skip the klign-sized call measurement")
+    ap.add_argument("--only-headline", action="store_true", help="the headline batch workload only")
     ap.add_argument("--locassm-only", action="store_true", help="only the local assembly measurement")
     ap.add_argument("--locassm-contigs", type=int, default=50000)
     ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
     ap.add_argument("--tuning", type=str, default="", metavar="B,G,S",
                     help="fixed scheduling parameters (blocks_per_cu, g64_min_rows, bin_shift) instead of autotuning")
-    ap.add_argument("--serving-qps", type=str, default="50,400", help="comma-separated request rates")
+    ap.add_argument("--serving-qps", type=str, default="50,400,1000,1600,2400", help="comma-separated request rates")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times) to calibrate HBM byte counters")
@@ -265,172 +553,53 @@ def main():
               "with no exchange, so multi-GPU throughput is N times the one-GPU value by construction.",
               file=sys.stderr)
         sys.exit(2)
+    cores, core_info = host_cores()
+    log(f"[bench] host cores used: {cores} ({core_info})")
 
     if args.locassm_only:
-        la = locassm_leg(args.locassm_contigs, args.steps, args.warmup, not args.no_cpu)
+        la = locassm_leg(args.locassm_contigs, args.steps, args.warmup, not args.no_cpu, cores)
         log(f"[bench] local assembly: {la}")
         print(json.dumps({"local_assembly": la}), flush=True)
         return
 
-    import mhm2_staging_amd as m
     from mhm2_staging_amd import _native
+    peak = _native.measure_valu_peak(0)
+    log(f"[bench] packed int16 VALU peak measured: {peak['lane_ops_per_s'] / 1e12:.2f} T lane-ops/s at "
+        f"{peak['clock_mhz']:.0f} MHz (nominal {NOMINAL_PEAK} T at 2400 MHz)")
 
-    scoring = m.MHM2_SCORING
-    t0 = time.perf_counter()
-    batch = m.generate(args.config, npairs=args.npairs)
-    log(f"[bench] generated config {args.config}: {batch.npairs} pairs, {batch.cells():.3e} cells "
-        f"in {time.perf_counter() - t0:.1f}s")
-    al = m.Aligner(device=0, scoring=scoring, read_is_ssw_ref=True, mode=m.MODE_FAST)
-    # eager launches in the timed region so each kernel's HIP events are live
-    # (a replayed job graph reports only its total time); the graph-replay
-    # rate of the same job is reported separately below
-    al.set_graphs(False)
-    t0 = time.perf_counter()
-    al.stage(batch)
-    log(f"[bench] staged (pack + H2D) in {time.perf_counter() - t0:.2f}s")
-    tuning = al.get_tuning()
-    if args.tuning:
-        b_, g_, s_ = (int(x) for x in args.tuning.split(","))
-        al.set_tuning(blocks_per_cu=b_, g64_min_rows=g_, bin_shift=s_)
-        tuning = al.get_tuning()
-    elif not args.no_autotune:
-        t0 = time.perf_counter()
-        tuning = al.autotune(repeats=2)
-        log(f"[bench] autotuned {tuning} in {time.perf_counter() - t0:.1f}s")
-    for _ in range(args.warmup):
-        al.align_staged()
-        al.wait()
-    log("[bench] warmup done")
-    job_ms, class_ms = [], []
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        al.align_staged()
-        al.wait()
-        st = al.stats()
-        job_ms.append(st["job_ms"])
-        class_ms.append(st["class_ms"])
-    elapsed = time.perf_counter() - t
-    res = al.results()
-    cells = batch.cells()
-    gcups = cells * args.steps / elapsed / 1e9
-    log(f"[bench] fast form: {gcups:.1f} GCUPS, {elapsed / args.steps * 1e3:.1f} ms/step, stats {st['job_ms']:.1f} ms")
-
-    # dominant kernel: the forward class with the most device time
-    cm = np.mean(np.array(class_ms), 0)
-    k = int(np.argmax(cm))
-    kcells = st["class_cells"][k]
-    ops = VALU_OPS_PER_CELL * kcells
-    achieved = ops / (cm[k] * 1e-3) / 1e12
-    dev_peak = None
-    try:
-        import ctypes
-        hip = ctypes.CDLL("libamdhip64.so")
-
-        class Prop(ctypes.Structure):
-            _fields_ = [("raw", ctypes.c_char * 4096)]
-        cus = ctypes.c_int(0)
-        clk = ctypes.c_int(0)
-        # hipDeviceAttributeMultiprocessorCount / hipDeviceAttributeClockRate
-        hip.hipDeviceGetAttribute(ctypes.byref(cus), 63, 0)
-        hip.hipDeviceGetAttribute(ctypes.byref(clk), 5, 0)
-        if cus.value > 0 and clk.value > 0:
-            dev_peak = cus.value * 64 * 2 * clk.value * 1e3 / 1e12
-    except Exception:
-        dev_peak = None
-    peak = 78.6  # 256 CUs x 64 lanes x 2 int16 per lane x 2.4 GHz, T lane-ops/s
-    roofline = {"bound": "valu", "kernel": _native.class_name(k), "achieved": round(achieved, 2), "peak": peak,
-                "unit": "TOP/s (int16 lane-ops)", "frac": round(achieved / peak, 4), "traffic": None,
-                "kernel_ms": round(float(cm[k]), 3), "kernel_cells": int(kcells),
-                "ops_per_cell": VALU_OPS_PER_CELL}
-    if dev_peak:
-        roofline["peak_from_device_clock"] = round(dev_peak, 1)
-    # HBM bytes per launch come from rocprofv3 PMC runs recorded under profiles/
-    tfile = os.path.join(ROOT, "profiles", "traffic_uneven_1M.json")
-    if args.config == 2 and not args.npairs and os.path.exists(tfile):
-        with open(tfile) as f:
-            tr = json.load(f)
-        if roofline["kernel"] in tr.get("per_launch_bytes", {}):
-            roofline["traffic"] = tr["per_launch_bytes"][roofline["kernel"]]
-            roofline["traffic_source"] = tr["source"]
-
-    out = {"metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+    h = batch_leg(args.config, args, cores, peak, headline=True)
+    key, text = WORKLOADS[args.config]
+    out = {"metric": METRIC, "value": h["value"], "unit": "GCUPS", "n_gpus": 1, "steps": h["steps"],
+           "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "i16 (packed, exact i32 rerun)",
            "data": "synthetic (seeded generator)",
-           "config": {"workload": WORKLOADS[args.config], "npairs": batch.npairs, "cells": cells,
-                      "scoring": "match 1, mismatch 3, gap open 5, gap extend 2, N 2", "form": "fast",
-                      "tuning": tuning},
-           "pairs_per_s": round(batch.npairs * args.steps / elapsed, 1),
-           "device_job_ms": round(float(np.mean(job_ms)), 3),
-           "stats": {kk: v for kk, v in st.items() if not kk.startswith("class_")},
-           "classes": {_native.class_name(i): {"pairs": st["class_pairs"][i], "ms": round(float(cm[i]), 3)}
-                       for i in range(_native.NUM_CLASSES) if st["class_pairs"][i]},
-           "roofline": roofline}
-
-    al.set_graphs(True)
-    for _ in range(2):  # a geometry's first job runs eagerly, the second is captured
-        al.align_staged()
-        al.wait()
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        al.align_staged()
-        al.wait()
-    ge = time.perf_counter() - t
-    out["graph_replay"] = {"value": round(cells * args.steps / ge / 1e9, 2), "unit": "GCUPS",
-                           "ms_per_step": round(ge / args.steps * 1e3, 3),
-                           "identical_results": bool((al.results() == res).all())}
-    al.set_graphs(False)
-
-    # host-inclusive rate: host strings in, host results out (2-bit packing on
-    # the host, PCIe both ways, the job); never the headline value
-    t = time.perf_counter()
-    e2e_steps = 2
-    for _ in range(e2e_steps):
-        al.stage(batch)
-        al.align_staged()
-        al.wait()
-        eres = al.results()
-    ee = time.perf_counter() - t
-    out["end_to_end"] = {"value": round(cells * e2e_steps / ee / 1e9, 2), "unit": "GCUPS",
-                         "ms_per_step": round(ee / e2e_steps * 1e3, 3),
-                         "includes": "host 2-bit packing, H2D copy, job, D2H of results",
-                         "identical_results": bool((eres == res).all())}
-    log(f"[bench] end to end: {out['end_to_end']}")
-
-    if not args.no_plain:
-        al.set_mode(m.MODE_PLAIN)
-        al.align_staged()
-        al.wait()
-        t = time.perf_counter()
-        al.align_staged()
-        al.wait()
-        pe = time.perf_counter() - t
-        pres = al.results()
-        plain_gcups = cells / pe / 1e9
-        out["plain_form"] = {"value": round(plain_gcups, 2), "unit": "GCUPS",
-                             "fast_over_plain": round(gcups / plain_gcups, 2),
-                             "identical_results": bool((pres == res).all())}
-        al.set_mode(m.MODE_FAST)
-        log(f"[bench] plain form: {plain_gcups:.1f} GCUPS")
-
-    if not args.no_cigar:
-        out["cigar_pass"] = cigar_leg(al, batch, res, scoring, args.steps, not args.no_cpu)
-        log(f"[bench] cigar pass: {out['cigar_pass']}")
-
-    if not args.no_cpu:
-        cb, cres, kk = cpu_baseline(batch, scoring)
-        out["cpu_baseline"] = cb
-        got = np.stack([res[c][:kk] for c in ("score", "read_begin", "read_end", "ctg_begin", "ctg_end")], 1)
-        want = np.stack([cres[c] for c in ("score", "ref_begin", "ref_end", "query_begin", "query_end")], 1)
-        out["bit_exact_check"] = {"pairs": int(kk), "mismatches": int((got != want).any(1).sum()),
-                                  "against": cb["kind"]}
-        log(f"[bench] cpu baseline {cb}")
-    al.close()
-    if not args.no_locassm:
-        out["local_assembly"] = locassm_leg(args.locassm_contigs, args.steps, args.warmup, not args.no_cpu)
-        log(f"[bench] local assembly: {out['local_assembly']}")
-    if not args.no_serving:
-        out["serving"] = serving_bench([float(x) for x in args.serving_qps.split(",")], args.serving_seconds)
+           "config": {"workload": text, "name": key, "npairs": h["npairs"], "cells": h["cells"],
+                      "scoring": SCORING_TEXT, "form": "fast", "tuning": h["tuning"]},
+           "pairs_per_s": h["pairs_per_s"], "device_job_ms": h["device_job_ms"], "stats": h["stats"],
+           "classes": h["classes"], "roofline": h["roofline"],
+           "valu_peak_measured": {"lane_ops_per_s": peak["lane_ops_per_s"], "clock_mhz": round(peak["clock_mhz"], 1),
+                                  "max_clock_mhz": peak["max_clock_mhz"], "cus": peak["cus"]},
+           "host_cores": dict(core_info, used=cores)}
+    for k in ("default_tuning", "graph_replay", "end_to_end", "plain_form", "cigar_pass", "cpu_baseline",
+              "cpu_one_core", "bit_exact_check"):
+        if k in h:
+            out[k] = h[k]
+    if not args.only_headline:
+        sec = [int(x) for x in args.secondary.split(",") if x] if args.secondary else \
+            [c for c in sorted(WORKLOADS) if c != args.config]
+        out["workloads"] = {}
+        for c in sec:
+            leg = batch_leg(c, args, cores, peak)
+            out["workloads"][WORKLOADS[c][0]] = leg
+        if not args.no_klign:
+            out["klign_call"] = klign_call_leg()
+            log(f"[bench] klign-sized calls: {out['klign_call']}")
+        if not args.no_locassm:
+            out["local_assembly"] = locassm_leg(args.locassm_contigs, min(args.steps, 10), min(args.warmup, 2),
+                                                not args.no_cpu, cores)
+            log(f"[bench] local assembly: {out['local_assembly']}")
+        if not args.no_serving:
+            out["serving"] = serving_bench([float(x) for x in args.serving_qps.split(",")], args.serving_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -81,6 +81,11 @@ typedef struct {
   double class_ms[MHMSW_NUM_CLASSES];
   double cigar_ms;          /* device time of the CIGAR pass (eager launches) */
   int64_t cigar_reruns;     /* CIGAR passes rerun with larger pools */
+  double window_ms;         /* device time of the window extraction of a windowed batch (eager launches) */
+  int64_t window_bases;     /* bases of the extracted windows */
+  int32_t packed_scoring;   /* 1: the scoring fits the packed 16-bit form; 0: every pair runs 32-bit,
+                               about 10x slower (16*(match+gap_open) > 255 or gap_open below mismatch
+                               or ambiguity) */
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,
@@ -101,6 +106,18 @@ typedef struct {
 int mhmsw_device_count(int *count);
 int mhmsw_device_total_mem(int device_id, size_t *bytes);
 int mhmsw_device_name(int device_id, char *buf, size_t buflen);
+
+/* Sustained packed 16-bit integer VALU rate of a device: a chain of
+ * v_pk_add_u16 / v_pk_max_i16 on 32 wavefronts per CU for about 100 ms (the
+ * roofline of the alignment kernels; no counterpart in the reference). */
+typedef struct {
+  double lane_ops_per_s; /* int16 lane-ops per second (2 per lane per packed instruction) */
+  double clock_mhz;      /* shader clock sustained during the run (cycle counter / wall clock) */
+  double max_clock_mhz;  /* the device's nominal maximum clock */
+  double kernel_ms;
+  int32_t cus;
+} mhmsw_valu_peak;
+int mhmsw_measure_valu_peak(int device_id, mhmsw_valu_peak *out);
 
 /* --- context (replaces GPUDriver::init / ~GPUDriver, driver.cpp:198-268) --- */
 int mhmsw_create(int device_id, mhmsw_ctx **out);

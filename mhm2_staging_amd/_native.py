@@ -40,7 +40,8 @@ class Stats(ctypes.Structure):
                 ("job_ms", ctypes.c_double), ("fwd_fast_ms", ctypes.c_double), ("fwd_plain_ms", ctypes.c_double),
                 ("rev_ms", ctypes.c_double), ("prep_ms", ctypes.c_double),
                 ("class_pairs", ctypes.c_int64 * 17), ("class_cells", ctypes.c_int64 * 17),
-                ("class_ms", ctypes.c_double * 17), ("cigar_ms", ctypes.c_double), ("cigar_reruns", ctypes.c_int64)]
+                ("class_ms", ctypes.c_double * 17), ("cigar_ms", ctypes.c_double), ("cigar_reruns", ctypes.c_int64),
+                ("window_ms", ctypes.c_double), ("window_bases", ctypes.c_int64), ("packed_scoring", ctypes.c_int32)]
 
     def as_dict(self):
         d = {}
@@ -60,6 +61,11 @@ class Tuning(ctypes.Structure):
 class ReqTimes(ctypes.Structure):
     _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
                 ("device_ms", ctypes.c_double), ("launched_s", ctypes.c_double), ("finished_s", ctypes.c_double)]
+
+
+class ValuPeak(ctypes.Structure):
+    _fields_ = [("lane_ops_per_s", ctypes.c_double), ("clock_mhz", ctypes.c_double),
+                ("max_clock_mhz", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("cus", ctypes.c_int32)]
 
 
 class SynthContigs(ctypes.Structure):
@@ -91,6 +97,7 @@ EXPORTS = [
     "mhmsw_server_results", "mhmsw_server_clock", "mhmsw_server_last_error", "mhmsw_server_destroy",
     "mhmsw_server_set_outputs", "mhmsw_server_results_extra",
     "mhmsw_synth_contigs", "mhmsw_synth_contigs_free", "mhmsw_synth_window_pairs", "mhmsw_synth_window_pairs_free",
+    "mhmsw_measure_valu_peak",
 ]
 
 _lib = None
@@ -145,6 +152,7 @@ def lib():
         "mhmsw_synth_contigs": [ctypes.c_uint64, i64, i32, i32, ctypes.c_int, ctypes.POINTER(SynthContigs)],
         "mhmsw_synth_window_pairs": [ctypes.POINTER(SynthContigs), ctypes.c_uint64, i64, ctypes.c_int,
                                      ctypes.POINTER(SynthWindows)],
+        "mhmsw_measure_valu_peak": [ctypes.c_int, ctypes.POINTER(ValuPeak)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -172,3 +180,13 @@ class MhmswError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mhmsw error {code}: {msg}")
         self.code = code
+
+
+def measure_valu_peak(device=0):
+    """Sustained packed-int16 VALU rate of the device (mhmsw_measure_valu_peak) as a dict."""
+    p = ValuPeak()
+    rc = lib().mhmsw_measure_valu_peak(int(device), ctypes.byref(p))
+    if rc != 0:
+        raise MhmswError(rc, "mhmsw_measure_valu_peak failed")
+    return {"lane_ops_per_s": p.lane_ops_per_s, "clock_mhz": p.clock_mhz, "max_clock_mhz": p.max_clock_mhz,
+            "kernel_ms": p.kernel_ms, "cus": p.cus}

@@ -23,6 +23,7 @@
 #include "sw_common.h"
 #include "sw_launch.h"
 #include "cigar_core.h"
+#include "ctx_info.h"
 #include "host_pinned.h"
 #include "pack2bit.h"
 #include "thread_pool.h"
@@ -123,6 +124,8 @@ struct mhmsw_ctx {
   int64_t graph_captures = 0, graph_replays = 0;
   hipEvent_t ev[kEvents] = {};
   hipEvent_t cev[kNumClasses][2] = {};
+  hipEvent_t wev[2] = {};  // around the window extraction of a windowed batch (eager jobs)
+  bool last_job_windows = false;
   bool class_launched[kNumClasses] = {};
   int64_t class_pairs[kNumClasses] = {};
   int64_t class_cells[kNumClasses] = {};
@@ -149,6 +152,22 @@ struct mhmsw_ctx {
 };
 
 static void reroute(mhmsw_ctx* c);
+
+bool mhmsw::ctx_info(const mhmsw_ctx* c, CtxInfo* out) {
+  if (!c || !out) return false;
+  out->device = c->device;
+  out->cus = c->cus;
+  out->stream = c->stream;
+  out->sc = c->sp;
+  out->mode = c->mode;
+  out->tuning = c->tuning;
+  return true;
+}
+
+int mhmsw::ctx_fail(mhmsw_ctx* c, int code, const char* msg) {
+  if (c) c->err = msg;
+  return code;
+}
 
 static int fail(mhmsw_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -257,7 +276,8 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
       return MHMSW_E_HIP;
     }
   }
-  if (hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreate(&c->wev[0]) != hipSuccess || hipEventCreate(&c->wev[1]) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
     mhmsw_destroy(c);
     return MHMSW_E_HIP;
   }
@@ -308,6 +328,8 @@ int mhmsw_destroy(mhmsw_ctx* c) {
   for (int i = 0; i < kEvents; i++)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->ev_done) (void)hipEventDestroy(c->ev_done);
+  for (int i = 0; i < 2; i++)
+    if (c->wev[i]) (void)hipEventDestroy(c->wev[i]);
   if (c->tail) (void)hipHostFree(c->tail);
   c->h_stage.release();
   c->h_results.release();
@@ -1020,6 +1042,12 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   HIP_OR_FAIL(c, launch_clear_counters(b, st));
   if (c->npairs > 0) {
     HIP_OR_FAIL(c, launch_unpack(b, st));
+    if (b.win_ctg) {
+      if (ev) HIP_OR_FAIL(c, hipEventRecord(c->wev[0], st));
+      HIP_OR_FAIL(c, launch_extract_windows(b, st));
+      if (ev) HIP_OR_FAIL(c, hipEventRecord(c->wev[1], st));
+    }
+    HIP_OR_FAIL(c, launch_reverse_copies(b, st));
     HIP_OR_FAIL(c, launch_schedule(b, kPassFwd, st));
   }
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[1], st));
@@ -1125,19 +1153,26 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   std::vector<unsigned char> key;
   bool graph = false;
   if (c->use_graphs) {
-    key.resize(sizeof(BatchView) + sizeof(c->fast_class_used) + 3 * sizeof(int) + sizeof(CigarView));
+    // everything the launches depend on: the view, the classes used and
+    // their pair counts (grid sizes), the reverse pass's launch choice (from
+    // the packed pair count), the plain kernel's LDS size, mode, grid tuning
+    // and the CIGAR pass
+    const size_t kv = sizeof(BatchView), kc = sizeof(c->fast_class_used), kq = sizeof(c->class_pairs);
+    const int64_t extra[4] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar, c->n_fast};
+    key.resize(kv + kc + kq + sizeof(extra) + sizeof(CigarView));
     unsigned char* kp = key.data();
-    std::memcpy(kp, &b, sizeof(BatchView));
-    std::memcpy(kp + sizeof(BatchView), c->fast_class_used, sizeof(c->fast_class_used));
-    const int extra[3] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar};
-    std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used), extra, sizeof(extra));
+    std::memcpy(kp, &b, kv);
+    std::memcpy(kp + kv, c->fast_class_used, kc);
+    std::memcpy(kp + kv + kc, c->class_pairs, kq);
+    std::memcpy(kp + kv + kc + kq, extra, sizeof(extra));
     if (c->cigar) {
       const CigarView cv = make_cigar_view(c);
-      std::memcpy(kp + sizeof(BatchView) + sizeof(c->fast_class_used) + sizeof(extra), &cv, sizeof(cv));
+      std::memcpy(kp + kv + kc + kq + sizeof(extra), &cv, sizeof(cv));
     }
     graph = (c->graph_exec && key == c->graph_key) || key == c->eager_key;
   }
   c->last_job_graph = graph;
+  c->last_job_windows = c->res != nullptr && c->npairs > 0;
   c->last_job_tail_results = c->tail_results && c->npairs > 0;
   if (c->last_job_tail_results) HIP_OR_FAIL(c, c->h_results.ensure((size_t)c->npairs * sizeof(AlnResult)));
   // after the job: its error words, counters and CIGAR pool use into pinned
@@ -1264,6 +1299,13 @@ static int complete(mhmsw_ctx* c) {
     s.class_ms[k] = cm;
   }
   s.prep_ms = ms[1];
+  float wm = 0.f;
+  if (c->last_job_windows && !c->last_job_graph) HIP_OR_FAIL(c, hipEventElapsedTime(&wm, c->wev[0], c->wev[1]));
+  s.window_ms = wm;
+  s.window_bases = 0;
+  if (c->last_job_windows)
+    for (int64_t p = 0; p < c->npairs; p++) s.window_bases += c->lens[(size_t)(2 * p + 1)];
+  s.packed_scoring = c->sp.fast_ok;
   s.fwd_fast_ms = ms[2];
   s.fwd_plain_ms = ms[3];
   s.rev_ms = ms[4] + ms[5];

@@ -1186,8 +1186,16 @@ hipError_t launch_clear_counters(const BatchView& b, hipStream_t st) {
 
 hipError_t launch_unpack(const BatchView& b, hipStream_t st) {
   hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 3, 256)), dim3(256), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract_windows(const BatchView& b, hipStream_t st) {
   if (b.win_ctg && b.npairs > 0)
     hipLaunchKernelGGL(extract_windows_kernel, dim3(grid_for(b.npairs, 4)), dim3(256), 0, st, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_reverse_copies(const BatchView& b, hipStream_t st) {
   if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3(grid_for(b.npairs, 4)), dim3(256), 0, st, b);
   return hipGetLastError();
 }

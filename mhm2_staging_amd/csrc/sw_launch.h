@@ -71,7 +71,9 @@ struct CigarView {
 hipError_t launch_cigar(const CigarView& v, int narrow_waves, int wide_waves, hipStream_t st);
 
 hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
-hipError_t launch_unpack(const BatchView& b, hipStream_t st);  // also extracts resident windows
+hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit + N mask -> nibble codes
+hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
+hipError_t launch_reverse_copies(const BatchView& b, hipStream_t st);   // reversed copy of every sequence
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
 // reverse pass, banded form: P = 4 (|l - k| <= 7), 8 (<= 15) or 0 (both classes in one launch)
