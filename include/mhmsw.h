@@ -187,6 +187,30 @@ int mhmsw_results_cigar(mhmsw_ctx *ctx, int64_t npairs, int32_t *mismatches, int
                         int64_t ops_cap, int64_t *total_ops);
 int mhmsw_get_stats(mhmsw_ctx *ctx, mhmsw_stats *out);
 
+/* No-allocation variant of mhmsw_submit + mhmsw_results, for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * Every pointer is DEVICE memory on the context's device: pair p aligns
+ * reads[read_off[p] .. +read_len[p]) with ctgs[ctg_off[p] .. +ctg_len[p])
+ * (ASCII, read as mhmsw_submit reads it), results[p] receives its result,
+ * status[0] the batch's status and status[1] the first invalid pair (-1: none).
+ * The work is enqueued on `stream` (a hipStream_t; NULL: the context's stream)
+ * and the call returns without waiting; the outputs are complete when the
+ * stream reaches that point.  It calls no allocator: the job's working memory
+ * is a fixed array in device memory, one per GPU, held until the process ends.
+ * All calls on one GPU share that array, so calls that overlap on one GPU race.
+ * Limits: npairs <= MHMSW_NOALLOC_MAX_PAIRS (else MHMSW_E_ARG at once); the
+ * sequences' lengths rounded up to 32 bases sum to at most
+ * MHMSW_NOALLOC_MAX_BASES, and each pair passes mhmsw_stage's length checks,
+ * else status[0] = MHMSW_E_ARG and every result is the score-0 sentinel.
+ * Scoring, orientation, mode and tuning come from the context; its score2 and
+ * CIGAR settings do not apply. */
+#define MHMSW_NOALLOC_MAX_PAIRS 131072
+#define MHMSW_NOALLOC_MAX_BASES 33554432
+int mhmsw_submit_noalloc(mhmsw_ctx *ctx, int64_t npairs, const char *reads, const int64_t *read_off,
+                         const int32_t *read_len, const char *ctgs, const int64_t *ctg_off, const int32_t *ctg_len,
+                         mhmsw_result *results, int32_t *status, void *stream);
+
 /* --- device-resident contigs (serving; no counterpart in the reference,
  *     whose windows are copied per batch as strings, klign.cpp:691-706) ---
  * Contigs are packed once to 2-bit codes plus an N bitmap and kept in HBM;

@@ -12,6 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmhmsw.so")
 
 MHMSW_OK = 0
+MHMSW_E_ARG = -1
+NOALLOC_MAX_PAIRS = 131072       # MHMSW_NOALLOC_MAX_PAIRS
+NOALLOC_MAX_BASES = 33554432     # MHMSW_NOALLOC_MAX_BASES
 MODE_FAST = 0
 MODE_PLAIN = 1
 NUM_CLASSES = 17
@@ -97,7 +100,7 @@ EXPORTS = [
     "mhmsw_server_results", "mhmsw_server_clock", "mhmsw_server_last_error", "mhmsw_server_destroy",
     "mhmsw_server_set_outputs", "mhmsw_server_results_extra",
     "mhmsw_synth_contigs", "mhmsw_synth_contigs_free", "mhmsw_synth_window_pairs", "mhmsw_synth_window_pairs_free",
-    "mhmsw_measure_valu_peak",
+    "mhmsw_measure_valu_peak", "mhmsw_submit_noalloc",
 ]
 
 _lib = None
@@ -153,6 +156,7 @@ def lib():
         "mhmsw_synth_window_pairs": [ctypes.POINTER(SynthContigs), ctypes.c_uint64, i64, ctypes.c_int,
                                      ctypes.POINTER(SynthWindows)],
         "mhmsw_measure_valu_peak": [ctypes.c_int, ctypes.POINTER(ValuPeak)],
+        "mhmsw_submit_noalloc": [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -23,4 +23,8 @@ bool ctx_info(const mhmsw_ctx* c, CtxInfo* out);
 // sets the context's last-error message and returns code
 int ctx_fail(mhmsw_ctx* c, int code, const char* msg);
 
+// Maps the no-allocation entry point's scratch array on `device` (once per
+// device; noalloc.hip).  Returns MHMSW_OK or MHMSW_E_HIP.
+int noalloc_setup(int device);
+
 }  // namespace mhmsw

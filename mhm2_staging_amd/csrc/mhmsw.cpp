@@ -290,6 +290,9 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
   }
   std::memset(c->tail, 0, sizeof(mhmsw_ctx::JobTail));
   compute_score_params(c);
+  // the no-allocation entry point's scratch on this device (a failure here
+  // is reported again, with its message, by that entry point's first call)
+  (void)noalloc_setup(device_id);
   *out = c;
   return MHMSW_OK;
 }
