@@ -26,7 +26,8 @@
 //    per-row maxima are folded into 32-bit (score, first step) records.  Gap
 //    values are clamped at 0 with unsigned saturating subtracts (as SSW's
 //    are), so H = max(d, E, F) needs no extra max with 0: 12 VALU ops per
-//    packed register per step, i.e. 6 per cell.
+//    packed register per step, i.e. 6 per cell (11 in the forward kernel,
+//    whose H = max(d, E, F) is one v_pk_maximum3_f16 over biased values).
 //  * plain (32-bit): one wavefront per pair, one row per lane, rows in tiles
 //    of 64, the last row of a tile passed to the next through LDS.
 #include <hip/hip_runtime.h>
@@ -58,6 +59,17 @@ __device__ __forceinline__ uint32_t pk_adds(uint32_t a, uint32_t b) {
   return u32(__builtin_elementwise_add_sat(vi(a), vi(b)));
 }
 __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return u32(vi(a) - vi(b)); }
+__device__ __forceinline__ uint32_t pk_addu_sat(uint32_t a, uint32_t b) {
+  return u32u(__builtin_elementwise_add_sat(vu(a), vu(b)));
+}
+// max of three packed u16 values, each a multiple of 16 below 0x7C00: as
+// f16 bit patterns these are finite and non-negative (denormals included,
+// which gfx950 kernels keep), and f16 maximum returns one of its inputs
+typedef _Float16 v2f16 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_max3_u16f(uint32_t a, uint32_t b, uint32_t c) {
+  const v2f16 x = __builtin_bit_cast(v2f16, a), y = __builtin_bit_cast(v2f16, b), z = __builtin_bit_cast(v2f16, c);
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
+}
 // unsigned saturating subtract: gap values are kept >= 0 (as SSW's are)
 __device__ __forceinline__ uint32_t pk_subu_sat(uint32_t a, uint32_t b) {
   return u32u(__builtin_elementwise_sub_sat(vu(a), vu(b)));
@@ -70,8 +82,20 @@ __device__ __forceinline__ uint32_t subst_sel(uint32_t a, uint32_t b) {
   return __builtin_amdgcn_bitop3_b32(a, b, 0x00030003u, 0x7C);
 }
 
-constexpr uint32_t kKillCode = 0x1010u;  // column code outside the pair: selector >= 13 -> s' = 0xFFFF
-constexpr uint32_t kKillRow = 0x1C10u;   // row code of a padding row (same effect)
+constexpr uint32_t kKillCode = 0x1010u;  // band kernel: column code outside the pair: selector >= 13 -> s' = 0xFFFF
+constexpr uint32_t kKillRow = 0x1C10u;   // band kernel: row code of a padding row (same effect)
+// Forward kernel: a column outside the pair, or a padding row, compares as N
+// (s' = 16*(gO - ambiguity), a finite non-negative value).  Its cells then
+// hold values no larger than the real cells they follow from, so they can
+// never beat a real cell, and in front of column 0 they stay at the boundary.
+constexpr uint32_t kKillColN = (uint32_t)kCodeN;       // selector 4..7: ambiguity
+constexpr uint32_t kKillRowN = (uint32_t)kCodeN | 0x0C00u;
+// The forward kernel's H, E and F are kept as unsigned 16-bit multiples of
+// 16 below 0x7C00, where the bit patterns of finite non-negative f16 values
+// order as the integers do, so H = max(d, E, F) is one v_pk_maximum3_f16.  A
+// pair whose score reaches this bound (with the largest substitution still
+// below 0x7C00) is rerun exactly in 32-bit.
+constexpr int32_t kFastOverflowScore = 1900;
 constexpr uint32_t kRowHi = 0x0C00u;     // high selector byte of a row code: 0x0C -> byte 0x00
 
 __device__ __forceinline__ uint32_t code_at(const uint32_t* __restrict__ codes, int64_t idx) {
@@ -470,13 +494,13 @@ __global__ void scatter_kernel(BatchView b, int pass) {
 template <int G, int P>
 struct FastState {
   uint32_t rb[P];      // row codes (selector form), packed
-  uint32_t ho[P];      // 16*(H(r, c_last) - gO), packed
-  uint32_t hg[P];      // 16*F(r, c_last + 1), packed (gap along the columns), clamped >= 0
+  uint32_t ho[P];      // 16*H(r, c_last), packed
+  uint32_t hg[P];      // 16*(F(r, c_last + 1) + gO), packed (gap along the columns), clamped >= 0
   uint32_t best[P];    // per-row running max of 16*H + tag, packed u16
   uint32_t rec_lo[P];  // (H << 16) | (0xFFFF - first step), low-half rows
   uint32_t rec_hi[P];  // same, high-half rows
-  uint32_t xa, xb;     // exchanged 16*(H-gO) of the row above, 2 steps / 1 step ago
-  uint32_t vg;         // exchanged 16*E of the first row, clamped >= 0
+  uint32_t xa, xb;     // exchanged 16*H of the row above, 2 steps / 1 step ago
+  uint32_t vg;         // exchanged 16*(E + gO) of the first row, clamped >= 0
   uint32_t xs, vs;     // DPP shift results; lane 0 of a group never receives and keeps its boundary
   uint32_t q;          // column codes (low: this step, high: last step)
   uint32_t w;          // next 8 column codes
@@ -493,12 +517,15 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   st.w >>= 4;
   if constexpr (EDGE) {
     const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
-    code = kill ? kKillCode : code;
+    code = kill ? kKillColN : code;
   }
-  uint32_t pm = 0;  // halves whose column is a padding column
+  uint32_t pm = 0;    // halves whose column is a padding column
+  uint32_t keep = 0;  // halves whose column is a real column
   if constexpr (PAD) {
     const uint32_t clo = (uint32_t)(t - 2 * l - col_len);
     pm = (clo < (uint32_t)npad ? 0x0000FFFFu : 0u) | (clo - 1u < (uint32_t)npad ? 0xFFFF0000u : 0u);
+    const uint32_t rlo = (uint32_t)(t - 2 * l);
+    keep = (rlo < (uint32_t)col_len ? 0x0000FFFFu : 0u) | (rlo - 1u < (uint32_t)col_len ? 0xFFFF0000u : 0u);
   }
   st.q = (st.q << 16) | code;
   constexpr uint32_t tag = (uint32_t)(15 - S) * 0x00010001u;
@@ -508,16 +535,21 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   for (int p = 0; p < P; p++) {
     const uint32_t sel = subst_sel(st.rb[p], st.q);
     uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
-    if constexpr (PAD) sp = (sp & ~pm) | (go & pm);
-    const uint32_t d = pk_adds(diag, sp);
-    // E, F >= 0, so H = max(d, E, F) needs no separate max with 0
-    uint32_t h = pk_max(d, st.hg[p]);
-    h = pk_max(h, vg);
+    // score2 reads every row's maximum over the real and padding columns, so
+    // columns past the padding get s' = 0 (d = Hdiag - gO): below the row's
+    // own cells, where N-like values (d = Hdiag - ambiguity) might not be
+    if constexpr (PAD) sp = (sp & keep) | (go & pm);
+    // d, E and F carry a bias of 16*gO: d = 16*(Hdiag + s + gO) (sp holds
+    // 16*(s + gO) >= 0), E' = 16*(E + gO) >= 0, F' likewise.  Removing the bias
+    // with an unsigned saturating subtract is H = max(0, d, E, F).
+    const uint32_t d = pk_addu_sat(diag, sp);
+    const uint32_t hb = pk_max3_u16f(d, st.hg[p], vg);
+    const uint32_t h = pk_subu_sat(hb, go);
     diag = st.ho[p];
-    const uint32_t ho = pk_sub(h, go);
-    st.ho[p] = ho;
-    st.hg[p] = pk_max(pk_subu_sat(st.hg[p], ge), ho);
-    vg = pk_max(pk_subu_sat(vg, ge), ho);
+    st.ho[p] = h;
+    // E' and F' update: max(E' - gE, H) is 16*(max(E - gE, H - gO) + gO)
+    st.hg[p] = pk_max(pk_subu_sat(st.hg[p], ge), h);
+    vg = pk_max(pk_subu_sat(vg, ge), h);
     st.best[p] = pk_maxu(st.best[p], h | tag);
   }
   const uint32_t last = st.ho[P - 1];
@@ -616,21 +648,21 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
     const int rlo = 2 * P * l + p, rhi = 2 * P * l + P + p;
     const uint32_t nlo = ((p < 8 ? rw0 : rw1) >> (4 * (p & 7))) & 0xFu;
     const uint32_t nhi = ((P + p < 8 ? rw0 : rw1) >> (4 * ((P + p) & 7))) & 0xFu;
-    const uint32_t clo = rlo < row_len ? (nlo | kRowHi) : kKillRow;
-    const uint32_t chi = rhi < row_len ? (nhi | kRowHi) : kKillRow;
+    const uint32_t clo = rlo < row_len ? (nlo | kRowHi) : kKillRowN;
+    const uint32_t chi = rhi < row_len ? (nhi | kRowHi) : kKillRowN;
     st.rb[p] = clo | (chi << 16);
-    st.ho[p] = neg;
+    st.ho[p] = 0;
     st.hg[p] = 0;
     st.best[p] = 0;
     st.rec_lo[p] = 0;
     st.rec_hi[p] = 0;
   }
-  st.xa = neg;  // boundary row: H = 0, stored as 16*(0 - gO)
-  st.xb = neg;
-  st.vg = 0;    // boundary E, clamped
-  st.xs = neg;
+  st.xa = 0;  // boundary row: H = 0
+  st.xb = 0;
+  st.vg = 0;  // boundary E' (E = -gO: never above 0, so never part of an H)
+  st.xs = 0;
   st.vs = 0;
-  st.q = (kKillCode << 16) | kKillCode;
+  st.q = (kKillColN << 16) | kKillColN;
   // column stream: at step t the low half reads column t - 2l
   const int64_t cb = task.col_start - 2 * l;  // >= kPadBases - 2G >= 0
   int64_t dw = (cb >> 3) + 1;  // index of st.dnext
@@ -717,7 +749,7 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
       key_decode(bestk, e.score, e.i, e.j);
       e.flags = 0;
     }
-    if (hmax >= 2047) e.flags |= kEndOverflow;  // 16*H may have saturated: rerun in 32-bit
+    if (hmax >= kFastOverflowScore) e.flags |= kEndOverflow;  // values may have left the exact range: rerun
     if (s2task) {
       if (s2_rerun) {
         e.flags |= kEndScore2;
