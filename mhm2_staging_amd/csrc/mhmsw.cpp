@@ -1022,9 +1022,9 @@ static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
   return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus * 12, st);
 }
 
-// jobs with at most this many packed-form pairs run both band classes of the
+// jobs with at most this many packed-form pairs run every band class of the
 // reverse pass in one launch
-constexpr int64_t kBandBothMaxPairs = 65536;
+constexpr int64_t kBandAllMaxPairs = 65536;
 
 // ev: record phase events (eager launches only; events recorded inside a
 // captured graph cannot be timed)
@@ -1098,12 +1098,11 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->n_fast, 4), st));
       for (int P = rev16; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->n_fast, 16), st));
       // pairs whose begin lies within a narrow band of diagonals: one lane each
-      if (c->n_fast <= kBandBothMaxPairs) {
-        // one launch for both classes: enough wavefronts for every task of both
-        HIP_OR_FAIL(c, launch_band(b, 0, grid_cap(c->n_fast + 128, 256), st));
+      if (c->n_fast <= kBandAllMaxPairs) {
+        // one launch for every class: enough wavefronts for every task of all
+        HIP_OR_FAIL(c, launch_band(b, 0, grid_cap(c->n_fast + 64 * kNumBandClasses, 256), st));
       } else {
-        HIP_OR_FAIL(c, launch_band(b, 8, grid_cap(c->n_fast, 256), st));
-        HIP_OR_FAIL(c, launch_band(b, 4, grid_cap(c->n_fast, 256), st));
+        for (int P = 8; P >= 1; P /= 2) HIP_OR_FAIL(c, launch_band(b, P, grid_cap(c->n_fast, 256), st));
       }
     }
     HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));

@@ -317,10 +317,10 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
     for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 64, P, full, st), "reverse pass");
     for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 16, P, full, st), "reverse pass");
     if (npairs <= 65536) {
-      NA_CHECK(launch_band(b, 0, (int)std::min<int64_t>(full, (npairs + 128 + 255) / 256), st), "reverse band pass");
+      NA_CHECK(launch_band(b, 0, (int)std::min<int64_t>(full, (npairs + 64 * kNumBandClasses + 255) / 256), st),
+               "reverse band pass");
     } else {
-      NA_CHECK(launch_band(b, 8, full, st), "reverse band pass");
-      NA_CHECK(launch_band(b, 4, full, st), "reverse band pass");
+      for (int P = 8; P >= 1; P /= 2) NA_CHECK(launch_band(b, P, full, st), "reverse band pass");
     }
     NA_CHECK(launch_plain(b, plain_grid, kPlainMaxCols, st), "reverse pass (32-bit)");
     NA_CHECK(launch_finalize(b, st), "finalize");

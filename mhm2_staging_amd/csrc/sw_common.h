@@ -21,11 +21,12 @@ constexpr int kClassFast16 = 0;          // + (P-1): 16-lane groups, rows <= 32*
 constexpr int kClassFast64 = kMaxP;      // + (P-1): 64-lane groups, rows <= 128*P
 constexpr int kClassPlain = 2 * kMaxP;   // 32-bit, one wavefront per pair
 constexpr int kNumFwdClasses = 2 * kMaxP + 1;  // classes a forward pass uses
-// Reverse pass only: banded form, one lane per pair, diagonals |l - k| <= 7
-// (kClassBand, 4 packed registers) or <= 15 (kClassBand + 1, 8 registers).
+// Reverse pass only: banded form, one lane per pair, P = 1, 2, 4, 8 packed
+// registers (4P diagonals) for |l - k| <= 1, 3, 7, 15: classes kClassBand + 0..3.
 constexpr int kClassBand = kNumFwdClasses;
+constexpr int kNumBandClasses = 4;
 constexpr int kBandMaxD = 15;
-constexpr int kNumClasses = kNumFwdClasses + 2;
+constexpr int kNumClasses = kNumFwdClasses + kNumBandClasses;
 constexpr int kNumBins = 128;            // column-length bins of 32 columns
 constexpr int kNumBuckets = kNumClasses * kNumBins;
 

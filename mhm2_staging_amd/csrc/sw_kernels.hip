@@ -82,15 +82,13 @@ __device__ __forceinline__ uint32_t subst_sel(uint32_t a, uint32_t b) {
   return __builtin_amdgcn_bitop3_b32(a, b, 0x00030003u, 0x7C);
 }
 
-constexpr uint32_t kKillCode = 0x1010u;  // band kernel: column code outside the pair: selector >= 13 -> s' = 0xFFFF
-constexpr uint32_t kKillRow = 0x1C10u;   // band kernel: row code of a padding row (same effect)
-// Forward kernel: a column outside the pair, or a padding row, compares as N
+// A column outside the pair, or a padding row, compares as N
 // (s' = 16*(gO - ambiguity), a finite non-negative value).  Its cells then
 // hold values no larger than the real cells they follow from, so they can
 // never beat a real cell, and in front of column 0 they stay at the boundary.
 constexpr uint32_t kKillColN = (uint32_t)kCodeN;       // selector 4..7: ambiguity
 constexpr uint32_t kKillRowN = (uint32_t)kCodeN | 0x0C00u;
-// The forward kernel's H, E and F are kept as unsigned 16-bit multiples of
+// The packed kernels keep H, E and F as unsigned 16-bit multiples of
 // 16 below 0x7C00, where the bit patterns of finite non-negative f16 values
 // order as the integers do, so H = max(d, E, F) is one v_pk_maximum3_f16.  A
 // pair whose score reaches this bound (with the largest substitution still
@@ -301,7 +299,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
       t.row_len = ref_len;
       t.col_start = qry_start;
       t.col_len = qry_len;
-      cls = D <= 7 ? kClassBand : kClassBand + 1;
+      cls = kClassBand + (D <= 1 ? 0 : D <= 3 ? 1 : D <= 7 ? 2 : 3);  // 4P diagonals cover |l - k| <= 2P - 1
       const int bn = (ref_len + (int32_t)D) >> b.bin_shift;  // the lane's loop runs K + D steps
       bin = bn < kNumBins ? bn : kNumBins - 1;
       return true;
@@ -512,7 +510,7 @@ struct FastState {
 // s' = 16*(0 + gO) in place of the kill value.
 template <int G, int P, bool EDGE, bool PAD, int S>
 __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, int npad, uint32_t tlo,
-                                          uint32_t thi, uint32_t go, uint32_t ge, uint32_t neg) {
+                                          uint32_t thi, uint32_t go, uint32_t ge) {
   uint32_t code = st.w & 0xFu;
   st.w >>= 4;
   if constexpr (EDGE) {
@@ -565,7 +563,7 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
 template <int G, int P, bool EDGE, bool PAD = false>
 __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
                                            const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh, uint32_t tlo,
-                                           uint32_t thi, uint32_t go, uint32_t ge, uint32_t neg) {
+                                           uint32_t thi, uint32_t go, uint32_t ge) {
   // w takes the 8 codes from dcur:dnext (loaded one refill earlier); the
   // dword after them is requested now and used at the next refill.
 #define MHMSW_REFILL()                                            \
@@ -576,23 +574,23 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
     st.dnext = codes[dw];                                         \
   }
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, PAD, 0>(st, t0 + 0, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 1>(st, t0 + 1, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 2>(st, t0 + 2, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 3>(st, t0 + 3, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 4>(st, t0 + 4, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 5>(st, t0 + 5, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 6>(st, t0 + 6, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 7>(st, t0 + 7, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 0>(st, t0 + 0, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 1>(st, t0 + 1, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 2>(st, t0 + 2, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 3>(st, t0 + 3, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 4>(st, t0 + 4, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 5>(st, t0 + 5, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 6>(st, t0 + 6, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 7>(st, t0 + 7, l, col_len, npad, tlo, thi, go, ge);
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, PAD, 8>(st, t0 + 8, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 9>(st, t0 + 9, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 10>(st, t0 + 10, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 11>(st, t0 + 11, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 12>(st, t0 + 12, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 13>(st, t0 + 13, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 14>(st, t0 + 14, l, col_len, npad, tlo, thi, go, ge, neg);
-  fast_step<G, P, EDGE, PAD, 15>(st, t0 + 15, l, col_len, npad, tlo, thi, go, ge, neg);
+  fast_step<G, P, EDGE, PAD, 8>(st, t0 + 8, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 9>(st, t0 + 9, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 10>(st, t0 + 10, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 11>(st, t0 + 11, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 12>(st, t0 + 12, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 13>(st, t0 + 13, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 14>(st, t0 + 14, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, PAD, 15>(st, t0 + 15, l, col_len, npad, tlo, thi, go, ge);
 #undef MHMSW_REFILL
   // fold the chunk's per-row maxima into the (score, first step) records
   const uint32_t cbase = 0xFFF0u - (uint32_t)t0;  // 0xFFFF - (t0 + 15 - tag) = cbase + tag
@@ -635,7 +633,6 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
   const uint32_t go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
   const uint32_t ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
-  const uint32_t neg = (uint32_t)(uint16_t)(-16 * b.sc.gap_open) * 0x00010001u;
 
   FastState<G, P> st;
   // this lane's 2P row codes are consecutive nibbles: three dword loads
@@ -676,11 +673,11 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   for (int t0 = 0; t0 < nsteps; t0 += 16) {
     const bool middle = t0 >= 2 * G && t0 + 16 <= mincols;
     if (middle) {
-      fast_chunk<G, P, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge, neg);
+      fast_chunk<G, P, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
     } else if (pads) {
-      fast_chunk<G, P, true, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge, neg);
+      fast_chunk<G, P, true, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
     } else {
-      fast_chunk<G, P, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge, neg);
+      fast_chunk<G, P, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
     }
   }
 
@@ -823,24 +820,25 @@ template <int P>
 struct BandState {
   uint32_t rr[P];          // row codes (selector form), slot p at rr[(p - s) mod P]
   uint32_t qr[P];          // even-step column codes, slot p at qr[(p + s) mod P]
-  uint32_t hoe[P], hoo[P]; // 16*(H - gO) of the slot's last even / odd cell
-  uint32_t eo[P], fo[P];   // 16*E, 16*F leaving the last odd step (clamped >= 0)
+  uint32_t hoe[P], hoo[P]; // 16*H of the slot's last even / odd cell
+  uint32_t eo[P], fo[P];   // 16*(E + gO), 16*(F + gO) leaving the last odd step (clamped >= 0)
   uint32_t be[P], bo[P];   // max of 16*H + tag over this chunk's even / odd steps
   uint32_t rcur, rnext, qcur, qnext;
 };
 
-__device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& ho_slot, uint32_t ein, uint32_t fin,
+// One packed cell update in the forward kernel's biased form (11 VALU ops):
+// h_slot holds 16*H of the slot's diagonal predecessor, ein/fin 16*(E + gO)
+// and 16*(F + gO).
+__device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& h_slot, uint32_t ein, uint32_t fin,
                                           uint32_t& eout, uint32_t& fout, uint32_t& best, uint32_t tag, uint32_t tlo,
                                           uint32_t thi, uint32_t go, uint32_t ge) {
   const uint32_t sel = subst_sel(r, q);
   const uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
-  const uint32_t d = pk_adds(ho_slot, sp);
-  uint32_t h = pk_max(d, ein);
-  h = pk_max(h, fin);
-  const uint32_t ho = pk_sub(h, go);
-  ho_slot = ho;
-  eout = pk_max(pk_subu_sat(ein, ge), ho);
-  fout = pk_max(pk_subu_sat(fin, ge), ho);
+  const uint32_t d = pk_addu_sat(h_slot, sp);
+  const uint32_t h = pk_subu_sat(pk_max3_u16f(d, ein, fin), go);
+  h_slot = h;
+  eout = pk_max(pk_subu_sat(ein, ge), h);
+  fout = pk_max(pk_subu_sat(fin, ge), h);
   best = pk_maxu(best, h | tag);
 }
 
@@ -875,7 +873,7 @@ __device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int 
     }
     // column code entering the top slot for the odd step
     uint32_t qn = (qw >> (4 * s)) & 0xFu;
-    if constexpr (EDGE) qn = (u0 + s + 2 * P - D < L) ? qn : kKillCode;
+    if constexpr (EDGE) qn = (u0 + s + 2 * P - D < L) ? qn : kKillColN;
     st.qr[s % P] = __builtin_amdgcn_alignbit(qn, st.qr[s % P], 16);
 #pragma unroll
     for (int p = 0; p < P; p++) {
@@ -885,7 +883,7 @@ __device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int 
     }
     // row code entering slot 0 for the next u-step
     uint32_t rn = ((rw >> (4 * s)) & 0xFu) | kRowHi;
-    if constexpr (EDGE) rn = (u0 + s + 1 < K) ? rn : kKillRow;
+    if constexpr (EDGE) rn = (u0 + s + 1 < K) ? rn : kKillRowN;
     const int ri = (P - 1 - s + 8 * P) % P;
     st.rr[ri] = (st.rr[ri] << 16) | rn;
   }
@@ -934,22 +932,21 @@ __device__ void band_pair(const BatchView& b, const SwTask& task) {
   const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
   const uint32_t go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
   const uint32_t ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
-  const uint32_t neg = (uint32_t)(uint16_t)(-16 * b.sc.gap_open) * 0x00010001u;
   const uint32_t s16 = (uint32_t)(16 * score);
 
   BandState<P> st;
   // u = 0: slot j holds R'[-j] (only slot 0 is on the sequence) and
   // Q'[j - D] for the even step
-  const uint32_t r0 = K > 0 ? (code_at(codes, task.row_start) | kRowHi) : kKillRow;
+  const uint32_t r0 = K > 0 ? (code_at(codes, task.row_start) | kRowHi) : kKillRowN;
 #pragma unroll
   for (int p = 0; p < P; p++) {
-    st.rr[p] = p == 0 ? (r0 | (kKillRow << 16)) : (kKillRow | (kKillRow << 16));
+    st.rr[p] = p == 0 ? (r0 | (kKillRowN << 16)) : (kKillRowN | (kKillRowN << 16));
     const int ilo = p - D, ihi = P + p - D;
-    const uint32_t qlo = (ilo >= 0 && ilo < L) ? code_at(codes, task.col_start + ilo) : kKillCode;
-    const uint32_t qhi = (ihi >= 0 && ihi < L) ? code_at(codes, task.col_start + ihi) : kKillCode;
+    const uint32_t qlo = (ilo >= 0 && ilo < L) ? code_at(codes, task.col_start + ilo) : kKillColN;
+    const uint32_t qhi = (ihi >= 0 && ihi < L) ? code_at(codes, task.col_start + ihi) : kKillColN;
     st.qr[p] = qlo | (qhi << 16);
-    st.hoe[p] = neg;
-    st.hoo[p] = neg;
+    st.hoe[p] = 0;
+    st.hoo[p] = 0;
     st.eo[p] = 0;
     st.fo[p] = 0;
   }
@@ -988,36 +985,18 @@ __device__ void band_pair(const BatchView& b, const SwTask& task) {
   }
 }
 
+// band class of P packed registers per lane (4P diagonals): P = 1, 2, 4, 8
 template <int P>
-__global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
-  const int cls = kClassBand + (P == 4 ? 0 : 1);
-  const int lane = threadIdx.x & 63;
-  PassSched* s = b.sched;
-  const int32_t nwt = s->class_slots[cls] / 64;
-  if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;  // as in sw_fast_kernel
-  const int32_t base = s->class_base[cls];
-  int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
-  SwTask task;
-  if (w < nwt) task = b.tasks[base + w * 64 + lane];
-  while (w < nwt) {
-    const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
-    SwTask next;
-    if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
-    band_pair<P>(b, task);
-    task = next;
-    w = wn;
-  }
+constexpr int band_class_of() {
+  return kClassBand + (P == 1 ? 0 : P == 2 ? 1 : P == 4 ? 2 : 3);
 }
 
-// Both band classes in one launch, for small jobs whose two classes each fill
-// only a few wavefronts: the wide class (D <= 15) is taken by the wavefronts
-// at the start of the grid, the narrow one (D <= 7) by those at its end, so
-// the two run side by side instead of one launch after the other.  (Large
-// jobs keep one launch per class: this kernel holds the wide class's
-// registers, which would cost the narrow class a wave per SIMD.)
+// Take the class's wave tasks from its work counter until they run out.
+// Wavefronts numbered past the class's work leave before touching the shared
+// counter (the rest still take every task: each loops until it runs out).
 template <int P>
 __device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t wave) {
-  const int cls = kClassBand + (P == 4 ? 0 : 1);
+  constexpr int cls = band_class_of<P>();
   PassSched* s = b.sched;
   const int32_t nwt = s->class_slots[cls] / 64;
   if (wave >= nwt) return;
@@ -1035,12 +1014,24 @@ __device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t
   }
 }
 
-__global__ __launch_bounds__(256) void sw_band_both_kernel(BatchView b) {
+template <int P>
+__global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
+  band_class<P>(b, threadIdx.x & 63, (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64));
+}
+
+// Every band class in one launch, for small jobs whose classes each fill only
+// a few wavefronts: each wavefront takes the classes from the widest down, so
+// they run side by side instead of one launch after the other.  (Large jobs
+// keep one launch per class: this kernel holds the widest class's registers,
+// which would cost the narrower classes waves per SIMD.)
+__global__ __launch_bounds__(256) void sw_band_all_kernel(BatchView b) {
   const int lane = threadIdx.x & 63;
   const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
   const int32_t nwaves = (int32_t)(gridDim.x * (blockDim.x / 64));
   band_class<8>(b, lane, wave);
   band_class<4>(b, lane, nwaves - 1 - wave);
+  band_class<2>(b, lane, wave);
+  band_class<1>(b, lane, nwaves - 1 - wave);
 }
 
 // ----------------------------------------------------------------------------
@@ -1278,14 +1269,13 @@ hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream
 }
 
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st) {
-  if (P == 0) {
-    hipLaunchKernelGGL(sw_band_both_kernel, dim3(grid), dim3(256), 0, st, b);
-  } else if (P == 4) {
-    hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b);
-  } else if (P == 8) {
-    hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b);
-  } else {
-    return hipErrorInvalidValue;
+  switch (P) {
+    case 0: hipLaunchKernelGGL(sw_band_all_kernel, dim3(grid), dim3(256), 0, st, b); break;
+    case 1: hipLaunchKernelGGL(sw_band_kernel<1>, dim3(grid), dim3(256), 0, st, b); break;
+    case 2: hipLaunchKernelGGL(sw_band_kernel<2>, dim3(grid), dim3(256), 0, st, b); break;
+    case 4: hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b); break;
+    case 8: hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -76,7 +76,7 @@ hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windo
 hipError_t launch_reverse_copies(const BatchView& b, hipStream_t st);   // reversed copy of every sequence
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
-// reverse pass, banded form: P = 4 (|l - k| <= 7), 8 (<= 15) or 0 (both classes in one launch)
+// reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1) or 0 (every band class in one launch)
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
