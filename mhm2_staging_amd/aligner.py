@@ -174,6 +174,13 @@ class AlignmentResults:
     """int16 arrays as in adept_sw::AlignmentResults (driver.hpp:17-23)."""
 
     def __init__(self, res):
+        # the reference's short fields (its 1024-base limit kept them in range): a value past the int16 range
+        # fails as the reference's errors do
+        for f in ("score", "ctg_end", "read_end"):
+            if res.shape[0] and int(res[f].max()) > np.iinfo(np.int16).max:
+                p = int(np.argmax(res[f]))
+                raise RuntimeError(f"GPU assert pair {p}: {f} {int(res[f][p])} does not fit the short fields of "
+                                   "AlignmentResults")
         self.ref_begin = res["ctg_begin"].astype(np.int16)
         self.query_begin = res["read_begin"].astype(np.int16)
         self.ref_end = res["ctg_end"].astype(np.int16)

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -273,6 +274,17 @@ static void fetch(DriverState *s, AlignmentResults &a) {
   a.top_scores = base + 4 * s->cap;
   for (size_t i = 0; i < n; i++) {
     const mhmsw_result &r = s->res[i];
+    // AlignmentResults holds short (driver.hpp:17-23); the reference kept
+    // its values in range by its 1024-base limit, this engine takes longer
+    // sequences, so a value past SHRT_MAX fails as the reference's errors do
+    // (begins are at most their ends, and at least -1)
+    if (r.score > SHRT_MAX || r.ctg_end > SHRT_MAX || r.read_end > SHRT_MAX) {
+      s->fetched = false;
+      std::ostringstream os;
+      os << "GPU assert pair " << i << ": score " << r.score << ", read end " << r.read_end << ", contig end "
+         << r.ctg_end << " do not fit the short fields of AlignmentResults " << __FILE__ << ":" << __LINE__ << "\n";
+      throw std::runtime_error(os.str());
+    }
     a.ref_begin[i] = (short)r.ctg_begin;
     a.ref_end[i] = (short)r.ctg_end;
     a.query_begin[i] = (short)r.read_begin;

@@ -3,13 +3,48 @@
 // tree's ALN_* scoring and rlen_limit 300, the 18 KAT pairs as (reads =
 // queries, contigs = refs), forwards, block, backwards, block.  Prints
 // "score read_begin read_end ctg_begin ctg_end" per pair.
+//
+// Other modes: "--device-name" prints get_device_name(0) (driver.cpp:35-39:
+// the PCI bus id); "--narrowing" aligns a 40000-base read with itself, whose
+// end coordinates do not fit AlignmentResults' short fields, and prints the
+// error the driver throws.
 #include <cstdio>
+#include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "adept_sw/driver.hpp"
 
-int main() {
+static int narrowing() {
+  adept_sw::GPUDriver drv;
+  drv.init(0, 1, (short)1, (short)-3, (short)-5, (short)-2, 300);
+  std::string seq;  // i.i.d. bases: the window occurs once in the read
+  unsigned x = 12345;
+  for (int i = 0; i < 40000; i++) {
+    x = x * 1103515245u + 12345u;
+    seq += "ACGT"[(x >> 16) & 3];
+  }
+  std::vector<std::string> reads{seq, "ACGTACGT"}, ctgs{seq.substr(33000, 5000), "ACGTACGT"};
+  try {
+    drv.run_kernel_forwards(reads, ctgs, 40000, 5000);
+    drv.kernel_block();
+    drv.run_kernel_backwards(reads, ctgs, 40000, 5000);
+    drv.kernel_block();
+  } catch (const std::runtime_error &e) {
+    std::printf("%s", e.what());
+    return 0;
+  }
+  std::printf("no error\n");
+  return 1;
+}
+
+int main(int argc, char **argv) {
+  if (argc > 1 && !std::strcmp(argv[1], "--device-name")) {
+    std::printf("%s\n", adept_sw::get_device_name(0).c_str());
+    return 0;
+  }
+  if (argc > 1 && !std::strcmp(argv[1], "--narrowing")) return narrowing();
   double t;
   int ndev;
   size_t mem;

@@ -1,5 +1,7 @@
 """The GPUDriver call protocol (Python mirror and the C++ drop-in) on the GPU."""
+import ctypes
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -47,3 +49,33 @@ def test_cpp_dropin_reference_adept_test_sequence(tmp_path):
     want = [[int(r["score"]), int(r["ref_begin"]), int(r["ref_end"]), int(r["query_begin"]), int(r["query_end"])]
             for r in o]
     assert got == want
+
+
+def test_cpp_dropin_get_device_name():
+    """get_device_name returns the device's PCI bus id, as the reference's does (src/adept-sw/driver.cpp:35-39)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "adept_kat")
+    out = subprocess.run([exe, "--device-name"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    name = out.stdout.strip()
+    assert re.fullmatch(r"[0-9a-fA-F]{4}:[0-9a-fA-F]{2}:[0-9a-fA-F]{2}\.[0-7]", name), name
+    buf = ctypes.create_string_buffer(64)
+    assert m.lib().mhmsw_device_name(0, buf, 64) == 0 and buf.value.decode() == name
+
+
+def test_cpp_dropin_short_narrowing_throws():
+    """An end coordinate past 32767 does not fit AlignmentResults' short fields: the driver throws a GPU assert
+    instead of wrapping (the reference's 1024-base limit never produced one)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "adept_kat")
+    out = subprocess.run([exe, "--narrowing"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("GPU assert pair 0:") and "short fields" in out.stdout
+
+
+def test_python_gpudriver_short_narrowing_raises():
+    rng = np.random.default_rng(5)
+    seq = "".join(np.array(list("ACGT"))[rng.integers(0, 4, 40000)])
+    drv = m.GPUDriver()
+    drv.init(0, 1, 1, -3, -5, -2, 300)
+    drv.run_kernel_forwards([seq], [seq[33000:38000]])
+    with pytest.raises(RuntimeError, match="GPU assert pair 0"):
+        drv.kernel_block()
