@@ -533,7 +533,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
     ap.add_argument("--tuning", type=str, default="", metavar="B,G,S",
                     help="fixed scheduling parameters (blocks_per_cu, g64_min_rows, bin_shift) instead of autotuning")
-    ap.add_argument("--serving-qps", type=str, default="50,400,1000,1600,2400", help="comma-separated request rates")
+    ap.add_argument("--serving-qps", type=str, default="50,400,1000,1600,2000,2400", help="comma-separated request rates")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times) to calibrate HBM byte counters")

@@ -86,6 +86,7 @@ typedef struct {
   int32_t packed_scoring;   /* 1: the scoring fits the packed 16-bit form; 0: every pair runs 32-bit,
                                about 10x slower (16*(match+gap_open) > 255 or gap_open below mismatch
                                or ambiguity) */
+  int64_t device_bytes;     /* device memory the context holds (its buffers only grow) */
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,
@@ -232,8 +233,10 @@ int mhmsw_stage_windows(mhmsw_ctx *ctx, const mhmsw_contigs *contigs, int64_t np
 /* --- serving loop: continuous batching of alignment requests ---
  * A server owns `slots` contexts (one HIP stream each) on one device and a
  * dispatcher thread.  Requests are admitted in arrival order as soon as a
- * slot is free and their device-memory estimate fits hbm_budget (bytes,
- * 0 = 80% of the device); requests queued together when a slot frees are
+ * slot is free and their device-memory estimate fits what running jobs leave
+ * of hbm_budget (bytes, 0 = 80% of the device less the resident contigs; a
+ * request whose estimate alone exceeds it fails at submit with MHMSW_E_ARG);
+ * requests queued together when a slot frees are
  * merged into one job (up to 2^20 pairs), and jobs on different slots run
  * concurrently on the device.  Results are identical to aligning the request
  * alone. */
@@ -244,7 +247,9 @@ typedef struct {
   double done_s;
   double device_ms; /* device time of the request's job */
   double launched_s; /* the job's copies and kernels were enqueued */
-  double finished_s; /* the dispatcher saw the job finish (results not yet copied out) */
+  double finished_s; /* the job's slot saw it finish (results not yet copied out) */
+  double job_est_bytes; /* device memory the request's job may take: the admission estimate, counted against
+                           hbm_budget while it runs */
 } mhmsw_req_times;
 int mhmsw_server_create(int device_id, const mhmsw_scoring *scoring, const mhmsw_contigs *contigs, int slots,
                         int64_t hbm_budget, mhmsw_server **out);

@@ -44,7 +44,8 @@ class Stats(ctypes.Structure):
                 ("rev_ms", ctypes.c_double), ("prep_ms", ctypes.c_double),
                 ("class_pairs", ctypes.c_int64 * 17), ("class_cells", ctypes.c_int64 * 17),
                 ("class_ms", ctypes.c_double * 17), ("cigar_ms", ctypes.c_double), ("cigar_reruns", ctypes.c_int64),
-                ("window_ms", ctypes.c_double), ("window_bases", ctypes.c_int64), ("packed_scoring", ctypes.c_int32)]
+                ("window_ms", ctypes.c_double), ("window_bases", ctypes.c_int64), ("packed_scoring", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64)]
 
     def as_dict(self):
         d = {}
@@ -63,7 +64,8 @@ class Tuning(ctypes.Structure):
 
 class ReqTimes(ctypes.Structure):
     _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
-                ("device_ms", ctypes.c_double), ("launched_s", ctypes.c_double), ("finished_s", ctypes.c_double)]
+                ("device_ms", ctypes.c_double), ("launched_s", ctypes.c_double), ("finished_s", ctypes.c_double),
+                ("job_est_bytes", ctypes.c_double)]
 
 
 class ValuPeak(ctypes.Structure):

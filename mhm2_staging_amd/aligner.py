@@ -321,7 +321,8 @@ class Server:
                                          _ptr(batch.read_len), _ptr(batch.ctg_id), _ptr(batch.win_off),
                                          _ptr(batch.win_len), ctypes.byref(rid))
         if rc != 0:
-            raise _native.MhmswError(rc, "submit")
+            msg = self._L.mhmsw_server_last_error(self._h)
+            raise _native.MhmswError(rc, f"submit: {msg.decode() if msg else ''}")
         self._pending[rid.value] = batch.npairs
         self._extra[rid.value] = self._outputs
         return rid.value
@@ -339,7 +340,7 @@ class Server:
             msg = self._L.mhmsw_server_last_error(self._h)
             raise _native.MhmswError(rc, f"request {rid}: {msg.decode() if msg else ''}")
         return out, {"submit_s": t.submit_s, "admit_s": t.admit_s, "done_s": t.done_s, "device_ms": t.device_ms,
-                     "launched_s": t.launched_s, "finished_s": t.finished_s}
+                     "launched_s": t.launched_s, "finished_s": t.finished_s, "job_est_bytes": t.job_est_bytes}
 
     def extras(self, rid, npairs):
         """After results(rid): (score2 or None, mismatches, op_off, ops) or CIGAR fields None when off."""

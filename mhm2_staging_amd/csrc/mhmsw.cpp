@@ -41,7 +41,7 @@ struct DevBuf {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     cap = 0;
-    size_t want = std::max<size_t>(n + n / 4, 64);
+    size_t want = std::max<size_t>(n + n / 4, std::max<size_t>(1, 256 / sizeof(T)));
     hipError_t e = hipMalloc((void**)&ptr, want * sizeof(T));
     if (e == hipSuccess) cap = want;
     return e;
@@ -732,6 +732,42 @@ static hipError_t ensure_job_buffers(mhmsw_ctx* c, int64_t npairs, int64_t total
   return e;
 }
 
+// Device bytes the context holds (every device buffer at its capacity).
+static int64_t device_bytes(const mhmsw_ctx* c) {
+  int64_t b = 0;
+  auto add = [&](const auto& buf) { b += (int64_t)(buf.cap * sizeof(*buf.ptr)); };
+  add(c->d_packed2), add(c->d_nmask), add(c->d_codes), add(c->d_base), add(c->d_len), add(c->d_bucket);
+  add(c->d_rank), add(c->d_errors), add(c->d_counters), add(c->d_win_ctg), add(c->d_win_off), add(c->d_fwd);
+  add(c->d_rev), add(c->d_s2), add(c->d_tasks), add(c->d_sched), add(c->d_results), add(c->d_cig_lists);
+  add(c->d_cig_count), add(c->d_cig_band), add(c->d_cig_rows), add(c->d_cig_nops), add(c->d_cig_mism);
+  add(c->d_cig_off), add(c->d_cig_dirs), add(c->d_cig_gen), add(c->d_cig_ops), add(c->d_cig_used);
+  return b;
+}
+
+constexpr int64_t kCigDirsDwords = 64ll << 20;
+
+// Device bytes the buffers of a job of npairs pairs over total padded bases
+// ask for: ensure_job_buffers, the window tables of a windowed batch, and
+// prepare_cigar's first-pass pools when the CIGAR pass runs (a pass that
+// outgrows them reruns with larger ones).  DevBuf allocates 1.25x what it is
+// asked for (at least 64 elements).
+static int64_t job_buffer_bytes(int64_t npairs, int64_t total, bool windowed, bool cigar) {
+  const int64_t np = std::max<int64_t>(npairs, 1), ns = std::max<int64_t>(2 * npairs, 1);
+  int64_t b = (total / 16 + 4) * 4 + (total / 32 + 4) * 4 + (2 * total / 8 + 64) * 4;  // 2-bit, N bits, nibbles
+  b += np * 4 + ns * 8 + ns * 4 + 2 * np * 4 + 2 * np * (int64_t)sizeof(PairEnd) + np * (int64_t)sizeof(AlnResult) +
+       (np + 64 * kNumClasses + 64) * (int64_t)sizeof(SwTask);
+  if (windowed) b += 2 * np * 4;
+  if (cigar)
+    b += kCigClasses * np * 4 + 2 * kCigClasses * 4 + np * 4 + 4 + 2 * np * 4 + np * 8 + 8 * 8 + (12 * np + 1024) * 4 +
+         std::max<int64_t>(4ll << 20, 32 * np) * 4 + kCigDirsDwords * 4;
+  return b;
+}
+
+// what such a job may allocate on a context that holds no buffers yet
+static int64_t job_alloc_bound(int64_t npairs, int64_t total, bool windowed, bool cigar) {
+  return job_buffer_bytes(npairs, total, windowed, cigar) * 5 / 4 + (1 << 20);
+}
+
 static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t nparts) {
   if (nparts == 0) return fail(c, MHMSW_E_ARG, "no batch");
   const mhmsw_contigs* res = parts[0]->res;
@@ -967,9 +1003,9 @@ static BatchView make_view(mhmsw_ctx* c) {
 
 // Device buffers of the CIGAR pass for the staged batch: op pool and generic
 // pool grow when a pass overflows them (complete() reruns the pass);
-// direction scratch is a fixed 256 MiB (one slot per wavefront, rows of the
-// tallest narrow rectangle; a slot of kCigNarrowMaxRows rows always fits).
-constexpr int64_t kCigDirsDwords = 64ll << 20;
+// direction scratch is a fixed 256 MiB, kCigDirsDwords (one slot per
+// wavefront, rows of the tallest narrow rectangle; a slot of
+// kCigNarrowMaxRows rows always fits).
 static int prepare_cigar(mhmsw_ctx* c) {
   const size_t np = (size_t)std::max<int64_t>(c->npairs, 1);
   c->cig_ops_cap = std::max<int64_t>(c->cig_ops_cap, 12 * (int64_t)np + 1024);
@@ -1308,6 +1344,7 @@ static int complete(mhmsw_ctx* c) {
   if (c->last_job_windows)
     for (int64_t p = 0; p < c->npairs; p++) s.window_bases += c->lens[(size_t)(2 * p + 1)];
   s.packed_scoring = c->sp.fast_ok;
+  s.device_bytes = device_bytes(c);
   s.fwd_fast_ms = ms[2];
   s.fwd_plain_ms = ms[3];
   s.rev_ms = ms[4] + ms[5];
@@ -1615,11 +1652,6 @@ struct Request {
   std::string err;
 };
 
-// device bytes a request's job needs (codes 1 B/base for both copies, 2-bit +
-// bitmap 0.375 B/base, ~140 B per pair of tables)
-int64_t estimate_bytes(const Request& r) {
-  return r.packed.total * 3 / 2 + r.npairs * 140 + (1 << 20);
-}
 
 // Requests queued together run as one job: their packed batches are
 // uploaded one after another (upload_parts) and the results split afterwards.
@@ -1628,7 +1660,8 @@ struct Job {
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<int64_t> first;  // first pair of each request in the job
   int64_t npairs = 0;
-  int64_t est = 0;
+  int64_t total = 0;  // padded bases of the job's layout
+  int64_t est = 0;    // device bytes its buffers may take (job_alloc_bound)
   bool s2 = false, cigar = false;  // the outputs this job computes
   std::vector<mhmsw_result> results;
 };
@@ -1722,11 +1755,8 @@ static void slot_worker(mhmsw_server* s, size_t k) {
     int rc = upload_parts(c, parts.data(), parts.size());
     if (rc == MHMSW_OK) rc = mhmsw_align_staged(c);
     const double launched = steady_now();
-    int q = rc == MHMSW_OK ? 0 : rc;
-    while (q == 0) {
-      q = mhmsw_query(c);
-      if (q == 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
-    }
+    // block on the job's completion event (this thread has nothing else to do)
+    const int q = rc == MHMSW_OK ? (mhmsw_wait(c) == MHMSW_OK ? 1 : MHMSW_E_HIP) : rc;
     const double fin = steady_now();
     mhmsw_stats st{};
     if (q == 1) {
@@ -1749,6 +1779,7 @@ static void slot_worker(mhmsw_server* s, size_t k) {
       r.times.finished_s = fin;
       r.times.done_s = now;
       r.times.device_ms = st.job_ms;
+      r.times.job_est_bytes = (double)j->est;
       if (q < 0 || rc != MHMSW_OK) {
         r.state = q < 0 ? q : rc;
         r.err = mhmsw_last_error(c);
@@ -1772,31 +1803,28 @@ static void server_loop(mhmsw_server* s) {
   while (!s->stop) {
     bool admitted = false;
     while (!s->queue.empty()) {
-      int freek = -1, busy = 0;
-      for (size_t k = 0; k < s->slots.size(); k++) {
-        if (!s->running[k]) {
-          if (freek < 0) freek = (int)k;
-        } else {
-          busy++;
-        }
-      }
+      int freek = -1;
+      for (size_t k = 0; k < s->slots.size() && freek < 0; k++)
+        if (!s->running[k]) freek = (int)k;
       if (freek < 0) break;
       auto j = std::make_shared<Job>();
+      j->s2 = s->score2 != 0;
+      j->cigar = s->cigar != 0;
       while (!s->queue.empty()) {
         const std::shared_ptr<Request>& r = s->queue.front();
         const bool first = j->reqs.empty();
         if (!first && j->npairs + r->npairs > kMergeMaxPairs) break;
-        // wait for memory; a request alone on an idle device always runs
-        if (s->used + j->est + r->est > s->budget && !(first && busy == 0)) break;
+        // wait until the job with this request fits the memory not taken by running jobs
+        const int64_t est = job_alloc_bound(j->npairs + r->npairs, j->total + r->packed.total, true, j->cigar);
+        if (s->used + est > s->budget) break;
         j->first.push_back(j->npairs);
         j->npairs += r->npairs;
-        j->est += r->est;
+        j->total += r->packed.total;
+        j->est = est;
         j->reqs.push_back(r);
         s->queue.pop_front();
       }
       if (j->reqs.empty()) break;
-      j->s2 = s->score2 != 0;
-      j->cigar = s->cigar != 0;
       const double now = steady_now();
       for (auto& r : j->reqs) {
         r->state = 1;
@@ -1809,8 +1837,9 @@ static void server_loop(mhmsw_server* s) {
       admitted = true;
     }
     if (admitted) s->cv_slot.notify_all();
-    // woken by a submission, a finished job or stop
-    s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+    // woken by a submission, a finished job or stop (each changes the state
+    // checked above under this lock, so no wake-up is lost)
+    s->cv_work.wait(lk);
   }
 }
 
@@ -1890,9 +1919,17 @@ int mhmsw_server_submit(mhmsw_server* s, int64_t npairs, const char* reads, cons
     s->err = r->packed.err;
     return rc;
   }
-  r->est = estimate_bytes(*r);
   {
     std::lock_guard<std::mutex> g(s->mu);
+    // a request that alone would not fit the budget could never be admitted
+    r->est = job_alloc_bound(npairs, r->packed.total, true, s->cigar != 0);
+    if (r->est > s->budget) {
+      char m[256];
+      snprintf(m, sizeof(m), "request of %lld pairs may need %lld bytes of device memory, more than the server's HBM "
+               "budget of %lld", (long long)npairs, (long long)r->est, (long long)s->budget);
+      s->err = m;
+      return MHMSW_E_ARG;
+    }
     r->id = s->next_id++;
     s->reqs[r->id] = r;
     s->queue.push_back(r);

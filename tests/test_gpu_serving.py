@@ -154,3 +154,41 @@ def test_server_score2_and_cigars_match_the_batch_api(gpu_aligner, serving_set):
     assert (got == want[0][0]).all()
     assert srv.extras(rid, reqs[0].npairs) == (None, None, None, None)
     srv.close()
+
+
+@pytest.mark.parametrize("npairs,cigar", [(1000, False), (20000, False), (5000, True)])
+def test_server_job_estimate_bounds_its_allocations(serving_set, npairs, cigar):
+    """The admission estimate of a job (the bytes counted against hbm_budget while it runs) bounds what the
+    same batch allocates on a context that holds no buffers yet, and follows the allocation rule closely."""
+    cs, res = serving_set
+    r = m.window_pairs(cs, npairs, seed=300 + npairs)
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv.set_outputs(cigar=cigar)
+    _, t = srv.results(srv.submit(r))
+    srv.close()
+    al = m.Aligner(device=0, scoring=BASELINE_SCORING)
+    al.set_cigar(cigar)
+    al.stage_windows(res, r)
+    al.align_staged()
+    al.wait()
+    st = al.stats()
+    al.close()
+    print(f"[gpu] {npairs} pairs cigar={cigar}: allocated {st['device_bytes']} estimate {t['job_est_bytes']:.0f}",
+          flush=True)
+    assert st["cigar_reruns"] == 0
+    assert 0 < st["device_bytes"] <= t["job_est_bytes"]
+    assert t["job_est_bytes"] <= 1.3 * st["device_bytes"] + (2 << 20)
+
+
+def test_server_rejects_a_request_larger_than_its_budget(serving_set):
+    cs, res = serving_set
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1, hbm_budget=2 << 20)
+    with pytest.raises(m.MhmswError) as e:
+        srv.submit(m.window_pairs(cs, 20000, seed=7))
+    assert e.value.code == -1 and "more than the server's HBM budget" in str(e.value)
+    # a small one still runs
+    small = m.window_pairs(cs, 200, seed=8)
+    got, _ = srv.results(srv.submit(small))
+    srv.close()
+    sb = small.as_strings(cs)
+    assert_same(gpu_table(got), oracle_table(sb, BASELINE_SCORING), sb)
