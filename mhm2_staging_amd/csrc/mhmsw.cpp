@@ -719,7 +719,7 @@ static hipError_t ensure_job_buffers(mhmsw_ctx* c, int64_t npairs, int64_t total
   const size_t np1 = (size_t)std::max<int64_t>(npairs, 1), ns1 = (size_t)std::max<int64_t>(2 * npairs, 1);
   hipError_t e = c->d_packed2.ensure((size_t)(total / 16 + 4));
   if (e == hipSuccess) e = c->d_nmask.ensure((size_t)(total / 32 + 4));
-  if (e == hipSuccess) e = c->d_codes.ensure((size_t)(2 * total / 8 + 64));
+  if (e == hipSuccess) e = c->d_codes.ensure((size_t)(total / 8 + 64));
   if (e == hipSuccess) e = c->d_s2.ensure(np1);
   if (e == hipSuccess) e = c->d_base.ensure(ns1);
   if (e == hipSuccess) e = c->d_len.ensure(ns1);
@@ -753,7 +753,7 @@ constexpr int64_t kCigDirsDwords = 64ll << 20;
 // asked for (at least 64 elements).
 static int64_t job_buffer_bytes(int64_t npairs, int64_t total, bool windowed, bool cigar) {
   const int64_t np = std::max<int64_t>(npairs, 1), ns = std::max<int64_t>(2 * npairs, 1);
-  int64_t b = (total / 16 + 4) * 4 + (total / 32 + 4) * 4 + (2 * total / 8 + 64) * 4;  // 2-bit, N bits, nibbles
+  int64_t b = (total / 16 + 4) * 4 + (total / 32 + 4) * 4 + (total / 8 + 64) * 4;  // 2-bit, N bits, nibbles
   b += np * 4 + ns * 8 + ns * 4 + 2 * np * 4 + 2 * np * (int64_t)sizeof(PairEnd) + np * (int64_t)sizeof(AlnResult) +
        (np + 64 * kNumClasses + 64) * (int64_t)sizeof(SwTask);
   if (windowed) b += 2 * np * 4;
@@ -1086,7 +1086,6 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       HIP_OR_FAIL(c, launch_extract_windows(b, st));
       if (ev) HIP_OR_FAIL(c, hipEventRecord(c->wev[1], st));
     }
-    HIP_OR_FAIL(c, launch_reverse_copies(b, st));
     HIP_OR_FAIL(c, launch_schedule(b, kPassFwd, st));
   }
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[1], st));

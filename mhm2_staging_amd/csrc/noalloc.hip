@@ -3,11 +3,11 @@
 // harnesses that forbid hipMalloc).  No counterpart in the reference.
 //
 // Inputs and outputs are device memory the caller owns.  Everything else the
-// job needs (nibble codes and their reversed copies, the per-sequence layout,
+// job needs (nibble codes, the per-sequence layout,
 // work lists, scheduling state, per-pair end points, counters) lives in one
 // file-scope device array, `scratch`, of fixed size: MHMSW_NOALLOC_MAX_PAIRS
 // pairs and MHMSW_NOALLOC_MAX_BASES padded bases.  Each GPU holds its own copy
-// of the array (about 45 MiB) from the load of this code object until the
+// of the array (about 28 MiB) from the load of this code object until the
 // process ends.  Host code never passes the array's name to a kernel: the
 // device address of each GPU's copy comes from hipGetSymbolAddress, taken
 // once per device with that device current (noalloc_setup, called by
@@ -42,14 +42,14 @@ namespace {
 constexpr int64_t kNaMaxPairs = MHMSW_NOALLOC_MAX_PAIRS;
 constexpr int64_t kNaMaxBases = MHMSW_NOALLOC_MAX_BASES;
 static_assert(kNaMaxBases % 32 == 0, "the forward region stays a multiple of 32 bases");
-// forward code region: front pad, the sequences, tail pad; reversed copies follow it
+// code region: front pad, the sequences, tail pad (reverse tasks read it backwards)
 constexpr int64_t kNaTotal = kPadBases + kNaMaxBases + kPadBases;
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // byte offsets into the scratch array
 constexpr size_t kOffCodes = 0;
-constexpr size_t kCodesBytes = align256((size_t)(2 * kNaTotal / 8 + 64) * 4);  // + 64 dwords of read slack
+constexpr size_t kCodesBytes = align256((size_t)(kNaTotal / 8 + 64) * 4);  // + 64 dwords of read slack
 constexpr size_t kOffBase = kOffCodes + kCodesBytes;
 constexpr size_t kOffLen = kOffBase + align256((size_t)(2 * kNaMaxPairs) * 8);
 constexpr size_t kOffBucket = kOffLen + align256((size_t)(2 * kNaMaxPairs) * 4);
@@ -306,7 +306,6 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
   if (npairs > 0) {
     const int full = std::max(1, ci.cus * ci.tuning.blocks_per_cu);
     const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ci.cus * 16, npairs));
-    NA_CHECK(launch_reverse_copies(b, st), "reverse copies");
     NA_CHECK(launch_schedule(b, kPassFwd, st), "forward schedule");
     for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 64, P, full, st), "forward pass");
     for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 16, P, full, st), "forward pass");

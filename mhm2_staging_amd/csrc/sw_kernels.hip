@@ -100,6 +100,13 @@ __device__ __forceinline__ uint32_t code_at(const uint32_t* __restrict__ codes, 
   return (codes[idx >> 3] >> ((uint32_t)(idx & 7) * 4u)) & 0xFu;
 }
 
+// the 8 nibbles of a dword in reverse order (byte reverse, then swap the
+// nibbles of each byte): reverse tasks stream the forward codes backwards
+__device__ __forceinline__ uint32_t nib_reverse(uint32_t v) {
+  const uint32_t y = __builtin_amdgcn_perm(v, v, 0x00010203u);
+  return ((y >> 4) & 0x0F0F0F0Fu) | ((y << 4) & 0xF0F0F0F0u);
+}
+
 template <int G>
 __device__ __forceinline__ uint32_t shift_in(uint32_t old, uint32_t src) {
   // lane l receives lane l-1 of its group; lane 0 of a group keeps `old`
@@ -156,7 +163,7 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m, int w) {
 }
 
 // ----------------------------------------------------------------------------
-// unpacking: 2-bit + N mask -> nibble codes (forward), then reversed copies
+// unpacking: 2-bit + N mask -> nibble codes
 // ----------------------------------------------------------------------------
 __global__ void unpack_kernel(BatchView b) {
   const int64_t ndw = b.total >> 3;  // total is a multiple of 32
@@ -202,36 +209,6 @@ __global__ void extract_windows_kernel(BatchView b) {
   }
 }
 
-// Reversed copies: sequence s at [total + base, total + base + len).  Output
-// dword w holds reversed positions 8w..8w+7 = source positions len-1-8w down
-// to len-8-8w: one funnel shift of two source dwords, then a nibble reversal
-// (byte permute + nibble swap).  Nibbles past len are 0.  One wavefront per
-// pair: the output dwords of its read and its window form one flat range over
-// the lanes (a 150-base read alone would leave 45 of 64 lanes idle).
-__global__ void reverse_kernel(BatchView b) {
-  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
-  const int lane = threadIdx.x & 63;
-  for (int64_t p = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; p < b.npairs; p += nwaves) {
-    const int64_t base0 = b.seq_base[2 * p], base1 = b.seq_base[2 * p + 1];
-    const int32_t len0 = b.seq_len[2 * p], len1 = b.seq_len[2 * p + 1];
-    const int32_t n0 = (len0 + 7) >> 3, n = n0 + ((len1 + 7) >> 3);
-    for (int32_t k = lane; k < n; k += 64) {
-      const bool first = k < n0;
-      const int64_t base = first ? base0 : base1;
-      const int32_t len = first ? len0 : len1;
-      const int32_t w = first ? k : k - n0;
-      const int64_t s0 = base + len - 8 - 8 * (int64_t)w;  // >= base - 8 >= 0 (front pad)
-      const uint32_t d0 = b.codes[s0 >> 3], d1 = b.codes[(s0 >> 3) + 1];
-      const uint32_t v = __builtin_amdgcn_alignbit(d1, d0, (uint32_t)(s0 & 7) * 4u);
-      const uint32_t y = __builtin_amdgcn_perm(v, v, 0x00010203u);  // byte reverse
-      uint32_t out = ((y >> 4) & 0x0F0F0F0Fu) | ((y << 4) & 0xF0F0F0F0u);
-      const int32_t keep = len - 8 * w;  // valid nibbles in this dword
-      if (keep < 8) out &= (1u << (4 * keep)) - 1u;
-      b.codes[((b.total + base) >> 3) + w] = out;
-    }
-  }
-}
-
 // ----------------------------------------------------------------------------
 // scheduling: classify -> schedule -> scatter
 // ----------------------------------------------------------------------------
@@ -264,7 +241,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
       return false;
     }
     if (fe.flags & kEndWide) wide = true;
-    // reversed prefixes [0..i] and [0..j]: R'[k] = R[i-k] = rev[len-1-i+k]
+    // reversed prefixes [0..i] and [0..j]: R'[k] = R[i-k]
     ref_len = fe.i + 1;
     qry_len = fe.j + 1;
     // Only cells with H' = score matter, and every alignment reaching the
@@ -289,8 +266,9 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
       if (ref_len > rcap) ref_len = (int32_t)rcap;
       if (qry_len > qcap) qry_len = (int32_t)qcap;
     }
-    ref_start = b.total + b.seq_base[sref] + (lref - 1 - fe.i);
-    qry_start = b.total + b.seq_base[sqry] + (lqry - 1 - fe.j);
+    // reverse tasks read the forward codes backwards: R'[k] = codes[ref_start - k]
+    ref_start = b.seq_base[sref] + fe.i;
+    qry_start = b.seq_base[sqry] + fe.j;
     if (!wide && D >= 0 && D <= kBandMaxD && ref_len + D < 0xFFF0) {
       // rows = reversed SSW-ref, columns = reversed SSW-query, one lane per pair
       t.pair = (int32_t)p;
@@ -560,18 +538,29 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   st.vg = vnew;
 }
 
-template <int G, int P, bool EDGE, bool PAD = false>
+template <int G, int P, bool EDGE, bool PAD, bool REV>
 __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
-                                           const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh, uint32_t tlo,
-                                           uint32_t thi, uint32_t go, uint32_t ge) {
-  // w takes the 8 codes from dcur:dnext (loaded one refill earlier); the
-  // dword after them is requested now and used at the next refill.
-#define MHMSW_REFILL()                                            \
-  {                                                               \
-    st.w = __builtin_amdgcn_alignbit(st.dnext, st.dcur, sh);      \
-    st.dcur = st.dnext;                                           \
-    dw++;                                                         \
-    st.dnext = codes[dw];                                         \
+                                           const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
+                                           uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
+  // Forward: w takes the 8 codes from dcur:dnext (loaded one refill
+  // earlier); the dword after them is requested now and used at the next
+  // refill.  Reverse (REV): the window of the next 8 codes lies one dword
+  // lower each time; dcur:dnext hold it (dcur the lower dword), its nibbles
+  // reversed give the codes in column order, and the dword below is
+  // requested now.
+#define MHMSW_REFILL()                                                        \
+  {                                                                           \
+    if constexpr (REV) {                                                      \
+      st.w = nib_reverse(__builtin_amdgcn_alignbit(st.dnext, st.dcur, sh));   \
+      st.dnext = st.dcur;                                                     \
+      dw--;                                                                   \
+      st.dcur = codes[dw];                                                    \
+    } else {                                                                  \
+      st.w = __builtin_amdgcn_alignbit(st.dnext, st.dcur, sh);                \
+      st.dcur = st.dnext;                                                     \
+      dw++;                                                                   \
+      st.dnext = codes[dw];                                                   \
+    }                                                                         \
   }
   MHMSW_REFILL();
   fast_step<G, P, EDGE, PAD, 0>(st, t0 + 0, l, col_len, npad, tlo, thi, go, ge);
@@ -634,17 +623,23 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const uint32_t go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
   const uint32_t ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
 
+  // uniform: a launch holds the tasks of one pass (padding slots carry no
+  // flags); reverse tasks read the forward codes backwards
+  const bool rev = __ballot(task.flags & kTaskReverse) != 0;
   FastState<G, P> st;
-  // this lane's 2P row codes are consecutive nibbles: three dword loads
-  const int64_t r0 = task.row_start + 2 * P * l;
+  // this lane's 2P row codes are consecutive nibbles (descending for a
+  // reverse task): three dword loads, then nibble m of rw1:rw0 is the row
+  // code m (forward) or 2P-1-m (reverse)
+  const int64_t r0 = rev ? task.row_start - 2 * P * l - (2 * P - 1) : task.row_start + 2 * P * l;
   const uint32_t rsh = (uint32_t)(r0 & 7) * 4u;
   const uint32_t a0 = codes[r0 >> 3], a1 = codes[(r0 >> 3) + 1], a2 = codes[(r0 >> 3) + 2];
   const uint32_t rw0 = __builtin_amdgcn_alignbit(a1, a0, rsh), rw1 = __builtin_amdgcn_alignbit(a2, a1, rsh);
 #pragma unroll
   for (int p = 0; p < P; p++) {
     const int rlo = 2 * P * l + p, rhi = 2 * P * l + P + p;
-    const uint32_t nlo = ((p < 8 ? rw0 : rw1) >> (4 * (p & 7))) & 0xFu;
-    const uint32_t nhi = ((P + p < 8 ? rw0 : rw1) >> (4 * ((P + p) & 7))) & 0xFu;
+    const int mlo = rev ? 2 * P - 1 - p : p, mhi = rev ? P - 1 - p : P + p;
+    const uint32_t nlo = ((mlo < 8 ? rw0 : rw1) >> (4 * (mlo & 7))) & 0xFu;
+    const uint32_t nhi = ((mhi < 8 ? rw0 : rw1) >> (4 * (mhi & 7))) & 0xFu;
     const uint32_t clo = rlo < row_len ? (nlo | kRowHi) : kKillRowN;
     const uint32_t chi = rhi < row_len ? (nhi | kRowHi) : kKillRowN;
     st.rb[p] = clo | (chi << 16);
@@ -660,24 +655,34 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   st.xs = 0;
   st.vs = 0;
   st.q = (kKillColN << 16) | kKillColN;
-  // column stream: at step t the low half reads column t - 2l
-  const int64_t cb = task.col_start - 2 * l;  // >= kPadBases - 2G >= 0
-  int64_t dw = (cb >> 3) + 1;  // index of st.dnext
-  const uint32_t sh = (uint32_t)(cb & 7) * 4u;
-  st.dcur = codes[dw - 1];
-  st.dnext = codes[dw];
+  // column stream: at step t the low half reads column t - 2l, i.e. code
+  // index cb + t (forward) or cb - t (reverse).  Forward: dw indexes
+  // st.dnext.  Reverse: the first refill's window starts at cb - 7 and dw
+  // indexes st.dcur, its lower dword.  Either way the stream stays within a
+  // few dwords of the sequence plus its padding (>= kPadBases - 2G - 24 >= 0).
+  const int64_t cb = rev ? task.col_start + 2 * l : task.col_start - 2 * l;
+  const int64_t wa = rev ? cb - 7 : cb;
+  const uint32_t sh = (uint32_t)(wa & 7) * 4u;
+  int64_t dw = rev ? (wa >> 3) : (wa >> 3) + 1;
+  st.dcur = rev ? codes[dw] : codes[dw - 1];
+  st.dnext = rev ? codes[dw + 1] : codes[dw];
   st.w = 0;
 
-  // uniform: a launch holds the tasks of one pass (padding slots carry no flags)
-  const bool pads = b.score2 != 0 && __ballot(task.flags & kTaskReverse) == 0;
+  const bool pads = b.score2 != 0 && !rev;
   for (int t0 = 0; t0 < nsteps; t0 += 16) {
     const bool middle = t0 >= 2 * G && t0 + 16 <= mincols;
-    if (middle) {
-      fast_chunk<G, P, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+    if (rev) {
+      if (middle) {
+        fast_chunk<G, P, false, false, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      } else {
+        fast_chunk<G, P, true, false, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      }
+    } else if (middle) {
+      fast_chunk<G, P, false, false, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
     } else if (pads) {
-      fast_chunk<G, P, true, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      fast_chunk<G, P, true, true, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
     } else {
-      fast_chunk<G, P, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      fast_chunk<G, P, true, false, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
     }
   }
 
@@ -843,19 +848,23 @@ __device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& h_sl
 }
 
 // One chunk of 8 u-steps (16 cell steps).  Row codes R'[u0+1 .. u0+8] and
-// column codes Q'[u0+2P-D .. +7] come from one dword of each stream.
+// column codes Q'[u0+2P-D .. +7] come from one dword window of each stream:
+// the reversed prefixes are the forward codes read backwards, so each
+// window lies one dword below the last one and its nibbles are reversed.
+// rcur:rnext hold the window (rcur the lower dword); the dword below it is
+// requested for the next chunk, never below the lane's own sequence (rfirst).
 template <int P, bool EDGE>
 __device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int L, int D, const uint32_t* __restrict__ codes,
-                                           int64_t& rdw, int64_t rlast, uint32_t rsh, int64_t& qdw, int64_t qlast,
+                                           int64_t& rdw, int64_t rfirst, uint32_t rsh, int64_t& qdw, int64_t qfirst,
                                            uint32_t qsh, uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
-  const uint32_t rw = __builtin_amdgcn_alignbit(st.rnext, st.rcur, rsh);
-  const uint32_t qw = __builtin_amdgcn_alignbit(st.qnext, st.qcur, qsh);
-  st.rcur = st.rnext;
-  st.qcur = st.qnext;
-  rdw++;
-  qdw++;
-  st.rnext = codes[rdw < rlast ? rdw : rlast];
-  st.qnext = codes[qdw < qlast ? qdw : qlast];
+  const uint32_t rw = nib_reverse(__builtin_amdgcn_alignbit(st.rnext, st.rcur, rsh));
+  const uint32_t qw = nib_reverse(__builtin_amdgcn_alignbit(st.qnext, st.qcur, qsh));
+  st.rnext = st.rcur;
+  st.qnext = st.qcur;
+  rdw--;
+  qdw--;
+  st.rcur = codes[rdw > rfirst ? rdw : rfirst];
+  st.qcur = codes[qdw > qfirst ? qdw : qfirst];
 #pragma unroll
   for (int p = 0; p < P; p++) {
     st.be[p] = 0;
@@ -936,40 +945,41 @@ __device__ void band_pair(const BatchView& b, const SwTask& task) {
 
   BandState<P> st;
   // u = 0: slot j holds R'[-j] (only slot 0 is on the sequence) and
-  // Q'[j - D] for the even step
+  // Q'[j - D] for the even step (R'[k] = codes[row_start - k], Q'[l] =
+  // codes[col_start - l])
   const uint32_t r0 = K > 0 ? (code_at(codes, task.row_start) | kRowHi) : kKillRowN;
 #pragma unroll
   for (int p = 0; p < P; p++) {
     st.rr[p] = p == 0 ? (r0 | (kKillRowN << 16)) : (kKillRowN | (kKillRowN << 16));
     const int ilo = p - D, ihi = P + p - D;
-    const uint32_t qlo = (ilo >= 0 && ilo < L) ? code_at(codes, task.col_start + ilo) : kKillColN;
-    const uint32_t qhi = (ihi >= 0 && ihi < L) ? code_at(codes, task.col_start + ihi) : kKillColN;
+    const uint32_t qlo = (ilo >= 0 && ilo < L) ? code_at(codes, task.col_start - ilo) : kKillColN;
+    const uint32_t qhi = (ihi >= 0 && ihi < L) ? code_at(codes, task.col_start - ihi) : kKillColN;
     st.qr[p] = qlo | (qhi << 16);
     st.hoe[p] = 0;
     st.hoo[p] = 0;
     st.eo[p] = 0;
     st.fo[p] = 0;
   }
-  // code streams: rows from R'[1], columns from Q'[2P - D] (2P - D >= 1)
-  const int64_t ra = task.row_start + 1, qa = task.col_start + 2 * P - D;
+  // code streams: rows from R'[1], columns from Q'[2P - D] (2P - D >= 1);
+  // chunk m's window starts 8m codes below the first one (ra, qa)
+  const int64_t ra = task.row_start - 8, qa = task.col_start - (2 * P - D) - 7;
   int64_t rdw = ra >> 3, qdw = qa >> 3;
   const uint32_t rsh = (uint32_t)(ra & 7) * 4u, qsh = (uint32_t)(qa & 7) * 4u;
-  // stream reads stay inside the lane's own sequence (plus one dword)
-  const int64_t rlast = ((task.row_start + K) >> 3) + 1, qlast = ((task.col_start + L) >> 3) + 1;
-  st.rcur = codes[rdw < rlast ? rdw : rlast];
-  st.rnext = codes[rdw + 1 < rlast ? rdw + 1 : rlast];
-  st.qcur = codes[qdw < qlast ? qdw : qlast];
-  st.qnext = codes[qdw + 1 < qlast ? qdw + 1 : qlast];
-  rdw++;
-  qdw++;
+  // stream reads stay within the lane's own sequence (less one dword): its
+  // first code is R'[K-1] = codes[row_start - K + 1]
+  const int64_t rfirst = ((task.row_start - K + 1) >> 3) - 1, qfirst = ((task.col_start - L + 1) >> 3) - 1;
+  st.rcur = codes[rdw > rfirst ? rdw : rfirst];
+  st.rnext = codes[rdw + 1];
+  st.qcur = codes[qdw > qfirst ? qdw : qfirst];
+  st.qnext = codes[qdw + 1];
 
   uint32_t keymin = 0xFFFFFFFFu;
   for (int u0 = 0; u0 < nU; u0 += 8) {
     const bool middle = u0 + 8 < kmin && u0 + 8 <= qmin;
     if (middle) {
-      band_chunk<P, false>(st, u0, K, L, D, codes, rdw, rlast, rsh, qdw, qlast, qsh, tlo, thi, go, ge);
+      band_chunk<P, false>(st, u0, K, L, D, codes, rdw, rfirst, rsh, qdw, qfirst, qsh, tlo, thi, go, ge);
     } else {
-      band_chunk<P, true>(st, u0, K, L, D, codes, rdw, rlast, rsh, qdw, qlast, qsh, tlo, thi, go, ge);
+      band_chunk<P, true>(st, u0, K, L, D, codes, rdw, rfirst, rsh, qdw, qfirst, qsh, tlo, thi, go, ge);
     }
     // scores never exceed the forward score: a hit is a chunk maximum equal to it
     uint32_t m = 0;
@@ -1048,6 +1058,9 @@ __device__ uint64_t plain_sweep(const BatchView& b, const SwTask& task, int k, i
   const int32_t NEG = -(1 << 28);
   const uint32_t* __restrict__ codes = b.codes;
   const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
+  // reverse tasks read the forward codes backwards: row r at row_start - r,
+  // column c at col_start - c
+  const int64_t dir = (task.flags & kTaskReverse) ? -1 : 1;
   const int32_t rows = task.row_len, cols = task.col_len, ncols = cols + npad;
   uint64_t pair_best = 0;
   int32_t s2best = 0;
@@ -1055,11 +1068,11 @@ __device__ uint64_t plain_sweep(const BatchView& b, const SwTask& task, int k, i
   for (int tile = 0; tile < ntiles; tile++) {
     const int32_t r = tile * 64 + k;
     const bool row_ok = r < rows;
-    const uint32_t rc = row_ok ? code_at(codes, task.row_start + r) : 0u;
+    const uint32_t rc = row_ok ? code_at(codes, task.row_start + dir * r) : 0u;
     int32_t hprev = 0, hg = NEG, best = 0, bestc = 0, rowall = 0;
     int32_t up_h_prev = 0;  // H(r-1, c-1)
     int32_t out_h = 0, out_e = NEG;
-    const int64_t cb = task.col_start - k;  // column c = t - k
+    const int64_t cb = task.col_start - dir * k;  // column c = t - k is code cb + dir * t
     for (int t = 0; t < ncols + 63; t++) {
       const int32_t c = t - k;
       const bool col_ok = c >= 0 && c < ncols;
@@ -1073,7 +1086,7 @@ __device__ uint64_t plain_sweep(const BatchView& b, const SwTask& task, int k, i
       }
       const int32_t up_h = __builtin_amdgcn_update_dpp(top_h, out_h, 0x138, 0xF, 0xF, false);
       const int32_t up_e = __builtin_amdgcn_update_dpp(top_e, out_e, 0x138, 0xF, 0xF, false);
-      const uint32_t q = (col_ok && real) ? code_at(codes, cb + t) : 0u;
+      const uint32_t q = (col_ok && real) ? code_at(codes, cb + dir * t) : 0u;
       const int32_t sub = real ? (((rc | q) & 4u) ? -A : (rc == q ? M : -X)) : 0;
       const int32_t e = max(up_e - gE, up_h - gO);     // gap along the rows
       const int32_t f = max(hg - gE, hprev - gO);      // gap along the columns
@@ -1218,10 +1231,6 @@ hipError_t launch_extract_windows(const BatchView& b, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_reverse_copies(const BatchView& b, hipStream_t st) {
-  if (b.npairs > 0) hipLaunchKernelGGL(reverse_kernel, dim3(grid_for(b.npairs, 4)), dim3(256), 0, st, b);
-  return hipGetLastError();
-}
 
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
   hipLaunchKernelGGL(clear_sched_kernel, dim3(8), dim3(256), 0, st, b.sched);

@@ -13,12 +13,12 @@ constexpr int kPassRev = 2;    // reverse pass: begin points
 
 struct BatchView {
   int64_t npairs;
-  int64_t total;            // padded bases of one region (forward codes at [0,total), reversed at [total,2*total))
+  int64_t total;            // padded bases of the layout (codes at [0, total)); reverse tasks read them backwards
   const int64_t* seq_base;  // [2*npairs] first base of sequence s (read of pair p: 2p, contig window: 2p+1)
   const int32_t* seq_len;   // [2*npairs]
   const uint32_t* packed2;  // 2-bit codes, 16 per dword
   const uint32_t* nmask;    // N flags, 32 per dword
-  uint32_t* codes;          // nibble codes, 8 per dword, 2*total nibbles plus slack
+  uint32_t* codes;          // nibble codes, 8 per dword, total nibbles plus slack
   PairEnd* fwd;
   PairEnd* rev;
   PassSched* sched;
@@ -73,7 +73,6 @@ hipError_t launch_cigar(const CigarView& v, int narrow_waves, int wide_waves, hi
 hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
 hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit + N mask -> nibble codes
 hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
-hipError_t launch_reverse_copies(const BatchView& b, hipStream_t st);   // reversed copy of every sequence
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
 // reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1) or 0 (every band class in one launch)

@@ -71,7 +71,11 @@ def test_server_hbm_budget_admission_in_order(serving_set):
     """A budget that fits one request at a time: requests queue, run one by one, in arrival order."""
     cs, res = serving_set
     reqs = [m.window_pairs(cs, 3000, seed=80 + i) for i in range(4)]
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=4, hbm_budget=8 << 20)
+    # the budget: one and a half times one request's admission estimate
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    est = max(srv.results(srv.submit(r))[1]["job_est_bytes"] for r in reqs)
+    srv.close()
+    srv = m.Server(res, scoring=BASELINE_SCORING, slots=4, hbm_budget=int(1.5 * est))
     ids = [srv.submit(r) for r in reqs]
     outs = [srv.results(i) for i in ids]
     srv.close()
