@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -82,6 +83,11 @@ struct mhmsw_ctx {
   int cigar = 0;                // compute SSW's CIGAR and mismatch count
   bool last_job_cigar = false;
   mhmsw_tuning tuning{8, 257, 5};
+  // 256-thread blocks of each persistent kernel that fit one CU at once:
+  // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
+  // (all-class kernel, then P = 1, 2, 4, 8)
+  int occ_fast[2][kMaxP] = {};
+  int occ_band[5] = {};
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
   // staged batch
@@ -281,7 +287,7 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
     mhmsw_destroy(c);
     return MHMSW_E_HIP;
   }
-  if (c->d_sched.ensure(1) != hipSuccess || c->d_errors.ensure(4) != hipSuccess ||
+  if (c->d_sched.ensure(kNumPasses) != hipSuccess || c->d_errors.ensure(4) != hipSuccess ||
       c->d_counters.ensure(2) != hipSuccess ||
       hipHostMalloc((void**)&c->tail, sizeof(mhmsw_ctx::JobTail), hipHostMallocDefault) != hipSuccess) {
     c->tail = nullptr;
@@ -289,6 +295,11 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
     return MHMSW_E_NOMEM;
   }
   std::memset(c->tail, 0, sizeof(mhmsw_ctx::JobTail));
+  for (int g = 0; g < 2; g++)
+    for (int P = 1; P <= kMaxP; P++)
+      if (fast_kernel_occupancy(g ? 64 : 16, P, &c->occ_fast[g][P - 1]) != hipSuccess) c->occ_fast[g][P - 1] = 0;
+  for (int i = 0; i < 5; i++)
+    if (band_kernel_occupancy(i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess) c->occ_band[i] = 0;
   compute_score_params(c);
   // the no-allocation entry point's scratch on this device (a failure here
   // is reported again, with its message, by that entry point's first call)
@@ -476,14 +487,28 @@ struct PackedBatch {
   std::vector<int64_t> base;
   std::vector<int32_t> len;
   std::vector<uint32_t, UninitAlloc<uint32_t>> two, nm;  // every dword is written by pack_batch
+  // the packed codes: in two/nm, or (packed in place) in the context's pinned staging
+  uint32_t* two_p = nullptr;
+  uint32_t* nm_p = nullptr;
+  size_t two_n = 0, nm_n = 0;  // dwords
   std::vector<int32_t> win_ctg, win_off;
   const mhmsw_contigs* res = nullptr;
   std::string err;
+  PackedBatch() = default;
+  PackedBatch(PackedBatch&&) = default;  // moving keeps two_p / nm_p valid (vector storage moves with them)
+  PackedBatch& operator=(PackedBatch&&) = default;
+  PackedBatch(const PackedBatch&) = delete;  // a copy's two_p / nm_p would point into the original
+  PackedBatch& operator=(const PackedBatch&) = delete;
 };
 
 using pack2::pack_seq;
 
-static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& pb, int max_threads) {
+// Where pass 2 of pack_batch writes the 2-bit codes and N bits (two_n and
+// nm_n dwords): false when it cannot provide them.
+using PackDest = std::function<bool(size_t two_n, size_t nm_n, uint32_t** two, uint32_t** nm)>;
+
+static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& pb, int max_threads,
+                      const PackDest* dest = nullptr) {
   const int64_t npairs = in.npairs;
   const bool windowed = in.res != nullptr;
   if (npairs < 0 || (npairs > 0 && (!in.reads || !in.read_off || !in.read_len)) ||
@@ -616,14 +641,25 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   acc += kPadBases;
   pb.total = (acc + 31) / 32 * 32;
   const int64_t packed_end = windowed ? reads_end : pb.total;
-  pb.two.resize((size_t)(packed_end / 16));
-  pb.nm.resize((size_t)(packed_end / 32));
+  pb.two_n = (size_t)(packed_end / 16);
+  pb.nm_n = (size_t)(packed_end / 32);
+  if (dest) {
+    if (!(*dest)(pb.two_n, pb.nm_n, &pb.two_p, &pb.nm_p)) {
+      pb.err = "cannot allocate the pinned staging buffer";
+      return MHMSW_E_NOMEM;
+    }
+  } else {
+    pb.two.resize(pb.two_n);
+    pb.nm.resize(pb.nm_n);
+    pb.two_p = pb.two.data();
+    pb.nm_p = pb.nm.data();
+  }
   // pads before the first and after the last packed sequence
-  std::fill(pb.two.begin(), pb.two.begin() + kPadBases / 16, 0u);
-  std::fill(pb.nm.begin(), pb.nm.begin() + kPadBases / 32, 0u);
+  std::fill(pb.two_p, pb.two_p + kPadBases / 16, 0u);
+  std::fill(pb.nm_p, pb.nm_p + kPadBases / 32, 0u);
   if (!windowed) {
-    std::fill(pb.two.begin() + body_end / 16, pb.two.end(), 0u);
-    std::fill(pb.nm.begin() + body_end / 32, pb.nm.end(), 0u);
+    std::fill(pb.two_p + body_end / 16, pb.two_p + pb.two_n, 0u);
+    std::fill(pb.nm_p + body_end / 32, pb.nm_p + pb.nm_n, 0u);
   }
   // pass 2, per chunk: final offsets, then zero each packed sequence's
   // 32-base-rounded span and pack it (the windows of a windowed batch are not
@@ -638,8 +674,8 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
         if ((s & 1) && windowed) continue;  // extracted on the device
         const int32_t len = pb.len[(size_t)s];
         const int64_t span = ((int64_t)len + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-        uint32_t* two = pb.two.data() + b0 / 16;
-        uint32_t* nb = pb.nm.data() + b0 / 32;
+        uint32_t* two = pb.two_p + b0 / 16;
+        uint32_t* nb = pb.nm_p + b0 / 32;
         std::fill(two, two + span / 16, 0u);
         std::fill(nb, nb + span / 32, 0u);
         const char* src = (s & 1) ? in.ctgs + in.ctg_off[s >> 1] : in.reads + in.read_off[s >> 1];
@@ -700,7 +736,7 @@ static void reroute(mhmsw_ctx* c) {
 // memcpy split over up to 16 threads for large copies (host memory bandwidth
 // is several times one core's copy rate)
 static void par_memcpy(void* dst, const void* src, size_t bytes) {
-  constexpr size_t kChunk = (size_t)8 << 20;
+  constexpr size_t kChunk = (size_t)256 << 10;
   const int nth = (int)std::min<size_t>(std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())),
                                         (bytes + kChunk - 1) / kChunk);
   if (nth <= 1) {
@@ -768,6 +804,33 @@ static int64_t job_alloc_bound(int64_t npairs, int64_t total, bool windowed, boo
   return job_buffer_bytes(npairs, total, windowed, cigar) * 5 / 4 + (1 << 20);
 }
 
+// Where a job's host data sit in the pinned staging: each part's 2-bit codes
+// and N bits, then the per-sequence tables and the window tables of the
+// whole job, each on a 256-byte boundary.
+struct StageLayout {
+  std::vector<size_t> at_two, at_nm;
+  size_t at_base = 0, at_len = 0, at_wc = 0, at_wo = 0, bytes = 0;
+};
+static StageLayout stage_layout(const size_t* two_n, const size_t* nm_n, size_t nparts, int64_t npairs, bool windows) {
+  StageLayout L;
+  auto at = [&](size_t b) {
+    const size_t a = L.bytes;
+    L.bytes += (b + 255) & ~(size_t)255;
+    return a;
+  };
+  L.at_two.resize(nparts);
+  L.at_nm.resize(nparts);
+  for (size_t k = 0; k < nparts; k++) {
+    L.at_two[k] = at(two_n[k] * 4);
+    L.at_nm[k] = at(nm_n[k] * 4);
+  }
+  L.at_base = at((size_t)(2 * npairs) * 8);
+  L.at_len = at((size_t)(2 * npairs) * 4);
+  L.at_wc = at(windows ? (size_t)npairs * 4 : 0);
+  L.at_wo = at(windows ? (size_t)npairs * 4 : 0);
+  return L;
+}
+
 static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t nparts) {
   if (nparts == 0) return fail(c, MHMSW_E_ARG, "no batch");
   const mhmsw_contigs* res = parts[0]->res;
@@ -814,25 +877,21 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
   // everything the job reads from the host, laid out in the pinned staging;
   // the per-sequence tables of several parts are joined there, with their
   // bases shifted by the parts before them
-  size_t stage_bytes = 0;
-  auto stage_at = [&](size_t bytes) {
-    const size_t at = stage_bytes;
-    stage_bytes += (bytes + 255) & ~(size_t)255;
-    return at;
-  };
-  std::vector<size_t> at_two(nparts), at_nm(nparts);
-  for (size_t k = 0; k < nparts; k++) {
-    at_two[k] = stage_at(parts[k]->two.size() * 4);
-    at_nm[k] = stage_at(parts[k]->nm.size() * 4);
-  }
-  const size_t at_base = stage_at((size_t)nseq * 8), at_len = stage_at((size_t)nseq * 4);
   const bool windows = res && npairs > 0;
-  const size_t at_wc = stage_at(windows ? (size_t)npairs * 4 : 0), at_wo = stage_at(windows ? (size_t)npairs * 4 : 0);
+  std::vector<size_t> two_n(nparts), nm_n(nparts);
+  for (size_t k = 0; k < nparts; k++) {
+    two_n[k] = parts[k]->two_n;
+    nm_n[k] = parts[k]->nm_n;
+  }
+  const StageLayout L = stage_layout(two_n.data(), nm_n.data(), nparts, npairs, windows);
+  const std::vector<size_t>& at_two = L.at_two;
+  const std::vector<size_t>& at_nm = L.at_nm;
+  const size_t at_base = L.at_base, at_len = L.at_len, at_wc = L.at_wc, at_wo = L.at_wo;
   if (windows) {
     HIP_OR_FAIL(c, c->d_win_ctg.ensure((size_t)npairs));
     HIP_OR_FAIL(c, c->d_win_off.ensure((size_t)npairs));
   }
-  HIP_OR_FAIL(c, c->h_stage.ensure(stage_bytes));
+  HIP_OR_FAIL(c, c->h_stage.ensure(L.bytes));  // no reallocation when a part was packed in place
   uint8_t* hs = c->h_stage.ptr;
   int64_t* sbase = reinterpret_cast<int64_t*>(hs + at_base);
   int32_t* slen = reinterpret_cast<int32_t*>(hs + at_len);
@@ -849,8 +908,10 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
         std::memcpy(d, src, n);
       }
     };
-    if (!pb.two.empty()) copy(hs + at_two[k], pb.two.data(), pb.two.size() * 4);
-    if (!pb.nm.empty()) copy(hs + at_nm[k], pb.nm.data(), pb.nm.size() * 4);
+    // a part packed in place already sits at its staging offset
+    if (pb.two_n && pb.two_p != reinterpret_cast<uint32_t*>(hs + at_two[k]))
+      copy(hs + at_two[k], pb.two_p, pb.two_n * 4);
+    if (pb.nm_n && pb.nm_p != reinterpret_cast<uint32_t*>(hs + at_nm[k])) copy(hs + at_nm[k], pb.nm_p, pb.nm_n * 4);
     const size_t nq = (size_t)(2 * pb.npairs);
     if (sh == 0) {
       if (nq) copy(sbase + 2 * ps, pb.base.data(), nq * 8);
@@ -888,11 +949,11 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
   }
   for (size_t k = 0; k < nparts; k++) {
     const PackedBatch& pb = *parts[k];
-    if (!pb.two.empty())
-      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr + part_sh[k] / 16, hs + at_two[k], pb.two.size() * 4,
+    if (pb.two_n)
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_packed2.ptr + part_sh[k] / 16, hs + at_two[k], pb.two_n * 4,
                                     hipMemcpyHostToDevice, c->stream));
-    if (!pb.nm.empty())
-      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr + part_sh[k] / 32, hs + at_nm[k], pb.nm.size() * 4,
+    if (pb.nm_n)
+      HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nmask.ptr + part_sh[k] / 32, hs + at_nm[k], pb.nm_n * 4,
                                     hipMemcpyHostToDevice, c->stream));
   }
   c->lens.resize((size_t)nseq);
@@ -918,8 +979,25 @@ static int upload_batch(mhmsw_ctx* c, const PackedBatch& pb) {
 static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
   if (in.res && in.res->device != c->device)
     return fail(c, MHMSW_E_ARG, "resident contigs live on device %d, context on %d", in.res->device, c->device);
+  // The batch is packed straight into the pinned staging, at the offsets the
+  // upload copies from: first nothing may still read it (a job, or the
+  // copies of an earlier stage).
+  if (c->in_flight) {
+    int rc = mhmsw_wait(c);
+    if (rc != MHMSW_OK) return rc;
+  }
+  HIP_OR_FAIL(c, hipSetDevice(c->device));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  const bool windows = in.res != nullptr && in.npairs > 0;
+  const PackDest dest = [&](size_t two_n, size_t nm_n, uint32_t** two, uint32_t** nm) {
+    const StageLayout L = stage_layout(&two_n, &nm_n, 1, in.npairs, windows);
+    if (c->h_stage.ensure(L.bytes) != hipSuccess) return false;
+    *two = reinterpret_cast<uint32_t*>(c->h_stage.ptr + L.at_two[0]);
+    *nm = reinterpret_cast<uint32_t*>(c->h_stage.ptr + L.at_nm[0]);
+    return true;
+  };
   PackedBatch pb;
-  int rc = pack_batch(pack_params(c), in, pb, 16);
+  int rc = pack_batch(pack_params(c), in, pb, 16, &dest);
   if (rc != MHMSW_OK) return fail(c, rc, "%s", pb.err.c_str());
   return upload_batch(c, pb);
 }
@@ -1067,9 +1145,13 @@ constexpr int64_t kBandAllMaxPairs = 65536;
 static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev, int* maxp16_out, int* maxp64_out) {
   // persistent grids: enough wavefronts to fill the device, but no more
   // blocks than a class can use (4 wavefronts per 256-thread block, 4 pairs
-  // per wavefront in 16-lane groups, 1 in 64-lane groups)
-  const int full = std::max(1, c->cus * c->tuning.blocks_per_cu);
-  auto grid_cap = [&](int64_t pairs, int pairs_per_block) {
+  // per wavefront in 16-lane groups, 1 in 64-lane groups, 64 in band
+  // classes) and no more per CU than the kernel keeps resident (occ), so
+  // that a small job's tasks are shared out by the work counter rather than
+  // one per wavefront to blocks that wait for a free CU
+  auto grid_cap = [&](int64_t pairs, int pairs_per_block, int occ) {
+    const int per_cu = occ > 0 ? std::min(c->tuning.blocks_per_cu, occ) : c->tuning.blocks_per_cu;
+    const int64_t full = std::max(1, c->cus * per_cu);
     return (int)std::max<int64_t>(1, std::min<int64_t>(full, (pairs + pairs_per_block - 1) / pairs_per_block));
   };
   // the 32-bit form runs one wavefront per 64-thread block; its grid does not
@@ -1077,8 +1159,19 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // measured shape, 1.7x the rate at 8 on the length-skewed 4M-pair batch)
   const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * 16, c->npairs));
   const int plain_cols = std::max(1, plain_lds_cols(c));
+  // each pass's kernels see that pass's scheduling state
+  const BatchView bf = with_pass(b, kPassFwd), br = with_pass(b, kPassRerun), bv = with_pass(b, kPassRev);
+  // A pair's score is at most match * its shorter length, so when no pair can
+  // reach the packed form's exact range no pair is flagged for a 32-bit rerun
+  // (score2 reruns aside); and when besides no pair runs 32-bit and every
+  // reverse rectangle fits a packed class, the reverse pass has no 32-bit task.
+  const bool rerun_possible =
+      c->score2 || (int64_t)c->sp.match * c->max_plain_cols >= kFastOverflowScore;
+  const bool rev_plain_possible = c->mode != MHMSW_MODE_FAST || !c->sp.fast_ok || c->n_plain > 0 || rerun_possible ||
+                                  c->max_plain_cols > kFast64MaxRows ||
+                                  std::max(c->max_read_len, c->max_ctg_len) > kFastMaxCols;
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
-  HIP_OR_FAIL(c, launch_clear_counters(b, st));
+  HIP_OR_FAIL(c, launch_clear_job(b, st));
   if (c->npairs > 0) {
     HIP_OR_FAIL(c, launch_unpack(b, st));
     if (b.win_ctg) {
@@ -1095,7 +1188,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int k = kClassFast64 + P - 1;
       if (c->fast_class_used[k]) {
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
-        HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->class_pairs[k], 4), st));
+        HIP_OR_FAIL(c, launch_fast(bf, 64, P, grid_cap(c->class_pairs[k], 4, c->occ_fast[1][P - 1]), st));
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp64 = std::max(maxp64, P);
       }
@@ -1104,7 +1197,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int k = kClassFast16 + P - 1;
       if (c->fast_class_used[k]) {
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
-        HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->class_pairs[k], 16), st));
+        HIP_OR_FAIL(c, launch_fast(bf, 16, P, grid_cap(c->class_pairs[k], 16, c->occ_fast[0][P - 1]), st));
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp16 = std::max(maxp16, P);
       }
@@ -1115,12 +1208,12 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   if (c->npairs > 0) {
     if (c->fast_class_used[kClassPlain]) {
       if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][0], st));
-      HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
+      HIP_OR_FAIL(c, launch_plain(bf, plain_grid, plain_cols, st));
       if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[kClassPlain][1], st));
     }
-    if (any_fast) {
+    if (any_fast && rerun_possible) {
       HIP_OR_FAIL(c, launch_schedule(b, kPassRerun, st));
-      HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
+      HIP_OR_FAIL(c, launch_plain(br, plain_grid, plain_cols, st));
     }
   }
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[3], st));
@@ -1130,17 +1223,20 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       // a reverse rectangle is never taller than its forward one, so the
       // reverse classes are bounded by the forward ones
       const int rev16 = maxp64 > 0 ? kMaxP : maxp16;
-      for (int P = maxp64; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 64, P, grid_cap(c->n_fast, 4), st));
-      for (int P = rev16; P >= 1; P--) HIP_OR_FAIL(c, launch_fast(b, 16, P, grid_cap(c->n_fast, 16), st));
+      for (int P = maxp64; P >= 1; P--)
+        HIP_OR_FAIL(c, launch_fast(bv, 64, P, grid_cap(c->n_fast, 4, c->occ_fast[1][P - 1]), st));
+      for (int P = rev16; P >= 1; P--)
+        HIP_OR_FAIL(c, launch_fast(bv, 16, P, grid_cap(c->n_fast, 16, c->occ_fast[0][P - 1]), st));
       // pairs whose begin lies within a narrow band of diagonals: one lane each
       if (c->n_fast <= kBandAllMaxPairs) {
         // one launch for every class: enough wavefronts for every task of all
-        HIP_OR_FAIL(c, launch_band(b, 0, grid_cap(c->n_fast + 64 * kNumBandClasses, 256), st));
+        HIP_OR_FAIL(c, launch_band(bv, 0, grid_cap(c->n_fast + 64 * kNumBandClasses, 256, c->occ_band[0]), st));
       } else {
-        for (int P = 8; P >= 1; P /= 2) HIP_OR_FAIL(c, launch_band(b, P, grid_cap(c->n_fast, 256), st));
+        for (int i = 4; i >= 1; i--)
+          HIP_OR_FAIL(c, launch_band(bv, 1 << (i - 1), grid_cap(c->n_fast, 256, c->occ_band[i]), st));
       }
     }
-    HIP_OR_FAIL(c, launch_plain(b, plain_grid, plain_cols, st));
+    if (rev_plain_possible) HIP_OR_FAIL(c, launch_plain(bv, plain_grid, plain_cols, st));
   }
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
   if (c->npairs > 0) HIP_OR_FAIL(c, launch_finalize(b, st));
@@ -1192,10 +1288,13 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   if (c->use_graphs) {
     // everything the launches depend on: the view, the classes used and
     // their pair counts (grid sizes), the reverse pass's launch choice (from
-    // the packed pair count), the plain kernel's LDS size, mode, grid tuning
-    // and the CIGAR pass
+    // the packed pair count), the plain kernel's LDS size (the longest shorter
+    // side, which with the scoring in the view also decides whether the rerun
+    // pass runs), mode, grid tuning, the CIGAR pass and the longest sequence
+    // (whether reverse rectangles can need the 32-bit form)
     const size_t kv = sizeof(BatchView), kc = sizeof(c->fast_class_used), kq = sizeof(c->class_pairs);
-    const int64_t extra[4] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar, c->n_fast};
+    const int64_t extra[5] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar, c->n_fast,
+                              (int64_t)std::max(c->max_read_len, c->max_ctg_len)};
     key.resize(kv + kc + kq + sizeof(extra) + sizeof(CigarView));
     unsigned char* kp = key.data();
     std::memcpy(kp, &b, kv);

@@ -58,7 +58,7 @@ constexpr size_t kOffFwd = kOffRank + align256((size_t)kNaMaxPairs * 4);
 constexpr size_t kOffRev = kOffFwd + align256((size_t)kNaMaxPairs * sizeof(PairEnd));
 constexpr size_t kOffTasks = kOffRev + align256((size_t)kNaMaxPairs * sizeof(PairEnd));
 constexpr size_t kOffSched = kOffTasks + align256((size_t)(kNaMaxPairs + 64 * kNumClasses + 64) * sizeof(SwTask));
-constexpr size_t kOffErrors = kOffSched + align256(sizeof(PassSched));
+constexpr size_t kOffErrors = kOffSched + align256(kNumPasses * sizeof(PassSched));
 constexpr size_t kOffCounters = kOffErrors + 256;
 constexpr size_t kOffState = kOffCounters + 256;
 constexpr size_t kScratchBytes = kOffState + 256;
@@ -293,7 +293,7 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
     }                                                                  \
   } while (0)
 
-  NA_CHECK(launch_clear_counters(b, st), "clear");
+  NA_CHECK(launch_clear_job(b, st), "clear");
   hipLaunchKernelGGL(na_layout_kernel, dim3(1), dim3(kLayoutThreads), 0, st, npairs, read_len, ctg_len,
                      const_cast<int64_t*>(b.seq_base), const_cast<int32_t*>(b.seq_len), state);
   if (int rc = launched("layout")) return rc;
@@ -306,22 +306,23 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
   if (npairs > 0) {
     const int full = std::max(1, ci.cus * ci.tuning.blocks_per_cu);
     const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ci.cus * 16, npairs));
+    const BatchView bf = with_pass(b, kPassFwd), br = with_pass(b, kPassRerun), bv = with_pass(b, kPassRev);
     NA_CHECK(launch_schedule(b, kPassFwd, st), "forward schedule");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 64, P, full, st), "forward pass");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 16, P, full, st), "forward pass");
-    NA_CHECK(launch_plain(b, plain_grid, kPlainMaxCols, st), "forward pass (32-bit)");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bf, 64, P, full, st), "forward pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bf, 16, P, full, st), "forward pass");
+    NA_CHECK(launch_plain(bf, plain_grid, kPlainMaxCols, st), "forward pass (32-bit)");
     NA_CHECK(launch_schedule(b, kPassRerun, st), "rerun schedule");
-    NA_CHECK(launch_plain(b, plain_grid, kPlainMaxCols, st), "32-bit rerun");
+    NA_CHECK(launch_plain(br, plain_grid, kPlainMaxCols, st), "32-bit rerun");
     NA_CHECK(launch_schedule(b, kPassRev, st), "reverse schedule");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 64, P, full, st), "reverse pass");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(b, 16, P, full, st), "reverse pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bv, 64, P, full, st), "reverse pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bv, 16, P, full, st), "reverse pass");
     if (npairs <= 65536) {
-      NA_CHECK(launch_band(b, 0, (int)std::min<int64_t>(full, (npairs + 64 * kNumBandClasses + 255) / 256), st),
+      NA_CHECK(launch_band(bv, 0, (int)std::min<int64_t>(full, (npairs + 64 * kNumBandClasses + 255) / 256), st),
                "reverse band pass");
     } else {
-      for (int P = 8; P >= 1; P /= 2) NA_CHECK(launch_band(b, P, full, st), "reverse band pass");
+      for (int P = 8; P >= 1; P /= 2) NA_CHECK(launch_band(bv, P, full, st), "reverse band pass");
     }
-    NA_CHECK(launch_plain(b, plain_grid, kPlainMaxCols, st), "reverse pass (32-bit)");
+    NA_CHECK(launch_plain(bv, plain_grid, kPlainMaxCols, st), "reverse pass (32-bit)");
     NA_CHECK(launch_finalize(b, st), "finalize");
   }
   hipLaunchKernelGGL(na_status_kernel, dim3(1), dim3(64), 0, st, state, b.errors, status);

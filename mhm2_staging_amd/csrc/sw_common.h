@@ -33,6 +33,9 @@ constexpr int kNumBuckets = kNumClasses * kNumBins;
 constexpr int kFast16MaxRows = 32 * kMaxP;   // 256
 constexpr int kFast64MaxRows = 128 * kMaxP;  // 1024
 constexpr int kFastMaxCols = 60000;          // step counter stays below 2^16
+// A packed-form pair whose score reaches this is rerun in 32-bit (the packed
+// kernels' values stay exact below 0x7C00 = 16 * 1984).
+constexpr int32_t kFastOverflowScore = 1900;
 constexpr int kPlainMaxCols = 8192;          // LDS boundary row of the 32-bit kernel
 
 // Task flags.

@@ -91,9 +91,8 @@ constexpr uint32_t kKillRowN = (uint32_t)kCodeN | 0x0C00u;
 // The packed kernels keep H, E and F as unsigned 16-bit multiples of
 // 16 below 0x7C00, where the bit patterns of finite non-negative f16 values
 // order as the integers do, so H = max(d, E, F) is one v_pk_maximum3_f16.  A
-// pair whose score reaches this bound (with the largest substitution still
-// below 0x7C00) is rerun exactly in 32-bit.
-constexpr int32_t kFastOverflowScore = 1900;
+// pair whose score reaches kFastOverflowScore (with the largest substitution
+// still below 0x7C00) is rerun exactly in 32-bit.
 constexpr uint32_t kRowHi = 0x0C00u;     // high selector byte of a row code: 0x0C -> byte 0x00
 
 __device__ __forceinline__ uint32_t code_at(const uint32_t* __restrict__ codes, int64_t idx) {
@@ -340,14 +339,13 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   return true;
 }
 
-__global__ void clear_counters_kernel(BatchView b) {
-  if (threadIdx.x < 4) b.errors[threadIdx.x] = 0;
-  if (threadIdx.x < 2) b.counters[threadIdx.x] = 0;
-}
-
-__global__ void clear_sched_kernel(PassSched* s) {
-  int32_t* w = reinterpret_cast<int32_t*>(s);
-  const int n = (int)(sizeof(PassSched) / sizeof(int32_t));
+// At the start of a job: error words, counters and the scheduling state of
+// its three passes (b.sched is the first of kNumPasses).
+__global__ void clear_job_kernel(BatchView b) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) b.errors[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 2) b.counters[threadIdx.x] = 0;
+  int32_t* w = reinterpret_cast<int32_t*>(b.sched);
+  const int n = (int)(kNumPasses * sizeof(PassSched) / sizeof(int32_t));
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0;
 }
 
@@ -1215,8 +1213,8 @@ static int grid_for(int64_t n, int block) {
   return (int)g;
 }
 
-hipError_t launch_clear_counters(const BatchView& b, hipStream_t st) {
-  hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, st, b);
+hipError_t launch_clear_job(const BatchView& b, hipStream_t st) {
+  hipLaunchKernelGGL(clear_job_kernel, dim3(16), dim3(256), 0, st, b);
   return hipGetLastError();
 }
 
@@ -1233,14 +1231,14 @@ hipError_t launch_extract_windows(const BatchView& b, hipStream_t st) {
 
 
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
-  hipLaunchKernelGGL(clear_sched_kernel, dim3(8), dim3(256), 0, st, b.sched);
+  const BatchView v = with_pass(b, pass);  // the pass's own scheduling state, cleared by clear_job_kernel
   {
     const int64_t per_block = 256 * kClassifyPerThread;
     const unsigned nblk = (unsigned)std::max<int64_t>(1, (b.npairs + per_block - 1) / per_block);
-    hipLaunchKernelGGL(classify_kernel, dim3(nblk), dim3(256), 0, st, b, pass);
+    hipLaunchKernelGGL(classify_kernel, dim3(nblk), dim3(256), 0, st, v, pass);
   }
-  hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, st, b);
-  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, b, pass);
+  hipLaunchKernelGGL(schedule_kernel, dim3(1), dim3(64), 0, st, v);
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(b.npairs, 256)), dim3(256), 0, st, v, pass);
   return hipGetLastError();
 }
 
@@ -1300,11 +1298,47 @@ hipError_t launch_finalize(const BatchView& b, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <class K>
+static hipError_t occ(K kernel, int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel, 256, 0);
+}
+
 hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu) {
-  // all fast kernels use 256-thread blocks; P = 8 has the largest register file
-  (void)group;
-  (void)P;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sw_fast_kernel<16, 8>, 256, 0);
+  if (group == 16) {
+    switch (P) {
+      case 1: return occ(sw_fast_kernel<16, 1>, blocks_per_cu);
+      case 2: return occ(sw_fast_kernel<16, 2>, blocks_per_cu);
+      case 3: return occ(sw_fast_kernel<16, 3>, blocks_per_cu);
+      case 4: return occ(sw_fast_kernel<16, 4>, blocks_per_cu);
+      case 5: return occ(sw_fast_kernel<16, 5>, blocks_per_cu);
+      case 6: return occ(sw_fast_kernel<16, 6>, blocks_per_cu);
+      case 7: return occ(sw_fast_kernel<16, 7>, blocks_per_cu);
+      case 8: return occ(sw_fast_kernel<16, 8>, blocks_per_cu);
+    }
+  } else {
+    switch (P) {
+      case 1: return occ(sw_fast_kernel<64, 1>, blocks_per_cu);
+      case 2: return occ(sw_fast_kernel<64, 2>, blocks_per_cu);
+      case 3: return occ(sw_fast_kernel<64, 3>, blocks_per_cu);
+      case 4: return occ(sw_fast_kernel<64, 4>, blocks_per_cu);
+      case 5: return occ(sw_fast_kernel<64, 5>, blocks_per_cu);
+      case 6: return occ(sw_fast_kernel<64, 6>, blocks_per_cu);
+      case 7: return occ(sw_fast_kernel<64, 7>, blocks_per_cu);
+      case 8: return occ(sw_fast_kernel<64, 8>, blocks_per_cu);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t band_kernel_occupancy(int P, int* blocks_per_cu) {
+  switch (P) {
+    case 0: return occ(sw_band_all_kernel, blocks_per_cu);
+    case 1: return occ(sw_band_kernel<1>, blocks_per_cu);
+    case 2: return occ(sw_band_kernel<2>, blocks_per_cu);
+    case 4: return occ(sw_band_kernel<4>, blocks_per_cu);
+    case 8: return occ(sw_band_kernel<8>, blocks_per_cu);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace mhmsw

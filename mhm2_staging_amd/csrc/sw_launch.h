@@ -10,6 +10,7 @@ namespace mhmsw {
 constexpr int kPassFwd = 0;    // forward pass: end points
 constexpr int kPassRerun = 1;  // 32-bit forward rerun of pairs the packed pass flagged
 constexpr int kPassRev = 2;    // reverse pass: begin points
+constexpr int kNumPasses = 3;
 
 struct BatchView {
   int64_t npairs;
@@ -21,7 +22,7 @@ struct BatchView {
   uint32_t* codes;          // nibble codes, 8 per dword, total nibbles plus slack
   PairEnd* fwd;
   PairEnd* rev;
-  PassSched* sched;
+  PassSched* sched;         // kNumPasses of them, one per pass (with_pass selects a pass's)
   SwTask* tasks;
   int32_t* bucket;  // [npairs]
   int32_t* rank;    // [npairs]
@@ -70,7 +71,16 @@ struct CigarView {
 // narrow_waves: wavefronts of each narrow-band launch; wide_waves: of the wide one
 hipError_t launch_cigar(const CigarView& v, int narrow_waves, int wide_waves, hipStream_t st);
 
-hipError_t launch_clear_counters(const BatchView& b, hipStream_t st);
+// The view of pass `pass`: the same batch with that pass's scheduling state.
+// Kernels of a pass (classify, schedule, scatter and the work classes) take it.
+inline BatchView with_pass(const BatchView& b, int pass) {
+  BatchView v = b;
+  v.sched = b.sched + pass;
+  return v;
+}
+
+// error words, counters and the scheduling state of every pass (b: the view with the first PassSched)
+hipError_t launch_clear_job(const BatchView& b, hipStream_t st);
 hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit + N mask -> nibble codes
 hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
@@ -79,6 +89,8 @@ hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
+// 256-thread blocks of a kernel that fit one CU at once (its registers decide)
 hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu);
+hipError_t band_kernel_occupancy(int P, int* blocks_per_cu);  // P as launch_band
 
 }  // namespace mhmsw
