@@ -92,7 +92,8 @@ typedef struct {
 /* Scheduling parameters of the packed form (no counterpart in the reference,
  * whose launch shape is fixed: one block per pair, driver.cpp:347-368).
  *   blocks_per_cu  persistent 256-thread blocks launched per CU (wavefronts
- *                  per CU = 4 x blocks_per_cu), 1..16
+ *                  per CU = 4 x blocks_per_cu), 1..16, for the forward and
+ *                  rectangle kernels (the reverse band kernels run 2)
  *   g64_min_rows   pairs whose shorter side has at least this many bases (and
  *                  at most 1024) use 64-lane groups, shorter ones 16-lane
  *                  groups; 33..257 (257 = 16-lane groups up to 256 rows)

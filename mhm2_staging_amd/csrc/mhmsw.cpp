@@ -640,6 +640,10 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   const int64_t body_end = acc;
   acc += kPadBases;
   pb.total = (acc + 31) / 32 * 32;
+  if (pb.total >= (int64_t)1 << 33) {  // the band kernels index code dwords with 32 bits
+    pb.err = "batch too large (2^33 bases or more with padding)";
+    return MHMSW_E_ARG;
+  }
   const int64_t packed_end = windowed ? reads_end : pb.total;
   pb.two_n = (size_t)(packed_end / 16);
   pb.nm_n = (size_t)(packed_end / 32);
@@ -1158,6 +1162,15 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // follow the packed form's tuning (16 wavefronts per CU: its fastest
   // measured shape, 1.7x the rate at 8 on the length-skewed 4M-pair batch)
   const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * 16, c->npairs));
+  // the band kernels' grids do not follow it either: 2 blocks (8 wavefronts)
+  // per CU, or fewer where a kernel keeps fewer resident.  More waves per CU
+  // only share the L2 among more code streams (the P = 4 class measured
+  // 238 us at 2 blocks per CU and 322 us at 3 on the uneven 1M-pair batch;
+  // no class was faster at 3 or more)
+  auto band_grid = [&](int64_t tasks, int occ) {
+    const int per_cu = occ > 0 ? std::min(2, occ) : 2;
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * per_cu, (tasks + 255) / 256));
+  };
   const int plain_cols = std::max(1, plain_lds_cols(c));
   // each pass's kernels see that pass's scheduling state
   const BatchView bf = with_pass(b, kPassFwd), br = with_pass(b, kPassRerun), bv = with_pass(b, kPassRev);
@@ -1230,10 +1243,10 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       // pairs whose begin lies within a narrow band of diagonals: one lane each
       if (c->n_fast <= kBandAllMaxPairs) {
         // one launch for every class: enough wavefronts for every task of all
-        HIP_OR_FAIL(c, launch_band(bv, 0, grid_cap(c->n_fast + 64 * kNumBandClasses, 256, c->occ_band[0]), st));
+        HIP_OR_FAIL(c, launch_band(bv, 0, band_grid(c->n_fast + 64 * kNumBandClasses, c->occ_band[0]), st));
       } else {
         for (int i = 4; i >= 1; i--)
-          HIP_OR_FAIL(c, launch_band(bv, 1 << (i - 1), grid_cap(c->n_fast, 256, c->occ_band[i]), st));
+          HIP_OR_FAIL(c, launch_band(bv, 1 << (i - 1), band_grid(c->n_fast, c->occ_band[i]), st));
       }
     }
     if (rev_plain_possible) HIP_OR_FAIL(c, launch_plain(bv, plain_grid, plain_cols, st));

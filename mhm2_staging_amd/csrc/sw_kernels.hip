@@ -826,8 +826,45 @@ struct BandState {
   uint32_t hoe[P], hoo[P]; // 16*H of the slot's last even / odd cell
   uint32_t eo[P], fo[P];   // 16*(E + gO), 16*(F + gO) leaving the last odd step (clamped >= 0)
   uint32_t be[P], bo[P];   // max of 16*H + tag over this chunk's even / odd steps
-  uint32_t rcur, rnext, qcur, qnext;
 };
+
+// Code streams of the band kernels.  A lane streams its reversed row prefix
+// and reversed column prefix: one dword window per chunk of 8 u-steps, one
+// dword lower each chunk.  The narrow classes (P <= 2) stage the streams
+// through LDS: per block of S chunks the wavefront loads, for each of its 64
+// lanes and each stream, the segment of S dwords those chunks read (the
+// window's upper dword of the block's first chunk is the previous segment's
+// first dword, kept in a register).  Each load instruction covers 64 / S
+// lanes' segments of S consecutive dwords, so a segment costs one coalesced
+// request instead of one request per dword per lane; segments are S + 1
+// dwords apart in LDS so that the lanes' reads hit distinct banks.  The wide
+// classes spend many more VALU cycles per chunk and hide the per-lane loads,
+// one chunk ahead, behind them; there the segment loads, waited on at once,
+// measured slower, and S = 0 selects the per-lane loads.
+template <int P>
+constexpr int band_seg() {
+  return P <= 2 ? 8 : 0;
+}
+template <int P>
+constexpr int band_lds_wave() {  // dwords per wavefront: row and column streams
+  return 2 * 64 * (band_seg<P>() + 1);
+}
+using band_lds_t = __attribute__((address_space(3))) uint32_t;
+
+// Load segment [base_s, base_s + S) of every lane s's stream (clamped below
+// at that lane's first dword, first_s) into seg[s * (S + 1) ..].
+template <int S>
+__device__ __forceinline__ void band_load_segments(band_lds_t* seg, const uint32_t* __restrict__ codes, int lane,
+                                                   int32_t base, int32_t first) {
+  constexpr int per = 64 / S;  // lanes' segments per load instruction
+#pragma unroll 4
+  for (int j = 0; j < S; j++) {
+    const int sl = per * j + lane / S, i = lane % S;
+    const int32_t bs = __shfl(base, sl, 64), fs = __shfl(first, sl, 64);
+    const int32_t idx = bs + i > fs ? bs + i : fs;
+    seg[sl * (S + 1) + i] = codes[idx];
+  }
+}
 
 // One packed cell update in the forward kernel's biased form (11 VALU ops):
 // h_slot holds 16*H of the slot's diagonal predecessor, ein/fin 16*(E + gO)
@@ -845,24 +882,13 @@ __device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& h_sl
   best = pk_maxu(best, h | tag);
 }
 
-// One chunk of 8 u-steps (16 cell steps).  Row codes R'[u0+1 .. u0+8] and
-// column codes Q'[u0+2P-D .. +7] come from one dword window of each stream:
-// the reversed prefixes are the forward codes read backwards, so each
+// One chunk of 8 u-steps (16 cell steps).  rw holds row codes R'[u0+1 ..
+// u0+8] and qw column codes Q'[u0+2P-D .. +7], lowest nibble first: the
+// reversed prefixes are the forward codes read backwards, so each chunk's
 // window lies one dword below the last one and its nibbles are reversed.
-// rcur:rnext hold the window (rcur the lower dword); the dword below it is
-// requested for the next chunk, never below the lane's own sequence (rfirst).
 template <int P, bool EDGE>
-__device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int L, int D, const uint32_t* __restrict__ codes,
-                                           int64_t& rdw, int64_t rfirst, uint32_t rsh, int64_t& qdw, int64_t qfirst,
-                                           uint32_t qsh, uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
-  const uint32_t rw = nib_reverse(__builtin_amdgcn_alignbit(st.rnext, st.rcur, rsh));
-  const uint32_t qw = nib_reverse(__builtin_amdgcn_alignbit(st.qnext, st.qcur, qsh));
-  st.rnext = st.rcur;
-  st.qnext = st.qcur;
-  rdw--;
-  qdw--;
-  st.rcur = codes[rdw > rfirst ? rdw : rfirst];
-  st.qcur = codes[qdw > qfirst ? qdw : qfirst];
+__device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int L, int D, uint32_t rw, uint32_t qw,
+                                           uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
 #pragma unroll
   for (int p = 0; p < P; p++) {
     st.be[p] = 0;
@@ -923,7 +949,7 @@ __device__ __forceinline__ void band_hits(const BandState<P>& st, int u0, int K,
 }
 
 template <int P>
-__device__ void band_pair(const BatchView& b, const SwTask& task) {
+__device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* lds) {
   const bool active = task.pair >= 0;
   const int K = active ? task.row_len : 0;
   const int L = active ? task.col_len : 0;
@@ -959,25 +985,71 @@ __device__ void band_pair(const BatchView& b, const SwTask& task) {
     st.fo[p] = 0;
   }
   // code streams: rows from R'[1], columns from Q'[2P - D] (2P - D >= 1);
-  // chunk m's window starts 8m codes below the first one (ra, qa)
+  // chunk m's window (8 codes) starts 8m codes below the first one (ra, qa),
+  // so it is dwords rd0 - m and rd0 - m + 1 (qd0 likewise).  Dword indices
+  // are 32-bit (the job's code buffer holds fewer than 2^31 dwords).
   const int64_t ra = task.row_start - 8, qa = task.col_start - (2 * P - D) - 7;
-  int64_t rdw = ra >> 3, qdw = qa >> 3;
+  const int32_t rd0 = (int32_t)(ra >> 3), qd0 = (int32_t)(qa >> 3);
   const uint32_t rsh = (uint32_t)(ra & 7) * 4u, qsh = (uint32_t)(qa & 7) * 4u;
   // stream reads stay within the lane's own sequence (less one dword): its
   // first code is R'[K-1] = codes[row_start - K + 1]
-  const int64_t rfirst = ((task.row_start - K + 1) >> 3) - 1, qfirst = ((task.col_start - L + 1) >> 3) - 1;
-  st.rcur = codes[rdw > rfirst ? rdw : rfirst];
-  st.rnext = codes[rdw + 1];
-  st.qcur = codes[qdw > qfirst ? qdw : qfirst];
-  st.qnext = codes[qdw + 1];
+  const int32_t rfirst = (int32_t)((task.row_start - K + 1) >> 3) - 1;
+  const int32_t qfirst = (int32_t)((task.col_start - L + 1) >> 3) - 1;
+  constexpr int S = band_seg<P>();
+  const int lane = threadIdx.x & 63;
+  band_lds_t* rseg = lds + lane * (S + 1);         // this lane's row segment
+  band_lds_t* qseg = lds + (64 + lane) * (S + 1);  // and column segment
+  // upper dwords of the first chunk's windows (then: of each LDS block's
+  // first chunk, or of the next chunk's window with per-lane loads)
+  uint32_t rcarry = codes[rd0 + 1], qcarry = codes[qd0 + 1];
+  // per-lane loads: lower dwords of the next chunk's windows
+  uint32_t rnow = 0, qnow = 0;
+  if constexpr (S == 0) {
+    rnow = codes[rd0 > rfirst ? rd0 : rfirst];
+    qnow = codes[qd0 > qfirst ? qd0 : qfirst];
+  }
 
   uint32_t keymin = 0xFFFFFFFFu;
   for (int u0 = 0; u0 < nU; u0 += 8) {
+    const int mc = u0 >> 3, c = S > 0 ? mc & (S - 1) : 0;
+    uint32_t rlo, rhi, qlo, qhi;
+    if constexpr (S == 0) {
+      rlo = rnow;
+      rhi = rcarry;
+      qlo = qnow;
+      qhi = qcarry;
+      rcarry = rnow;
+      qcarry = qnow;
+      const int32_t rn = rd0 - mc - 1, qn = qd0 - mc - 1;
+      rnow = codes[rn > rfirst ? rn : rfirst];
+      qnow = codes[qn > qfirst ? qn : qfirst];
+    } else if (c == 0) {
+      // block of chunks mc .. mc + S - 1: dwords d0 - mc - (S - 1) .. d0 - mc
+      // (the LDS reads of the previous block came before these writes: a
+      // wavefront's LDS operations run in order)
+      __builtin_amdgcn_wave_barrier();
+      band_load_segments<S>(lds, codes, lane, rd0 - mc - (S - 1), rfirst);
+      band_load_segments<S>(lds + 64 * (S + 1), codes, lane, qd0 - mc - (S - 1), qfirst);
+      __builtin_amdgcn_wave_barrier();
+      rhi = rcarry;
+      qhi = qcarry;
+      rcarry = rseg[0];
+      qcarry = qseg[0];
+      rlo = rseg[S - 1];
+      qlo = qseg[S - 1];
+    } else {
+      rhi = rseg[S - c];
+      qhi = qseg[S - c];
+      rlo = rseg[S - 1 - c];
+      qlo = qseg[S - 1 - c];
+    }
+    const uint32_t rw = nib_reverse(__builtin_amdgcn_alignbit(rhi, rlo, rsh));
+    const uint32_t qw = nib_reverse(__builtin_amdgcn_alignbit(qhi, qlo, qsh));
     const bool middle = u0 + 8 < kmin && u0 + 8 <= qmin;
     if (middle) {
-      band_chunk<P, false>(st, u0, K, L, D, codes, rdw, rfirst, rsh, qdw, qfirst, qsh, tlo, thi, go, ge);
+      band_chunk<P, false>(st, u0, K, L, D, rw, qw, tlo, thi, go, ge);
     } else {
-      band_chunk<P, true>(st, u0, K, L, D, codes, rdw, rfirst, rsh, qdw, qfirst, qsh, tlo, thi, go, ge);
+      band_chunk<P, true>(st, u0, K, L, D, rw, qw, tlo, thi, go, ge);
     }
     // scores never exceed the forward score: a hit is a chunk maximum equal to it
     uint32_t m = 0;
@@ -1003,7 +1075,7 @@ constexpr int band_class_of() {
 // Wavefronts numbered past the class's work leave before touching the shared
 // counter (the rest still take every task: each loops until it runs out).
 template <int P>
-__device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t wave) {
+__device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t wave, band_lds_t* lds) {
   constexpr int cls = band_class_of<P>();
   PassSched* s = b.sched;
   const int32_t nwt = s->class_slots[cls] / 64;
@@ -1016,15 +1088,26 @@ __device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t
     const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
     SwTask next;
     if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
-    band_pair<P>(b, task);
+    band_pair<P>(b, task, lds);
     task = next;
     w = wn;
   }
 }
 
+// this wavefront's part of the block's band LDS (dw dwords per wavefront)
+__device__ __forceinline__ band_lds_t* band_wave_lds(uint32_t* block_lds, int dw) {
+  return (band_lds_t*)block_lds + (threadIdx.x / 64) * dw;
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
-  band_class<P>(b, threadIdx.x & 63, (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64));
+  const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+  if constexpr (band_seg<P>() > 0) {
+    __shared__ uint32_t lds[4 * band_lds_wave<P>()];
+    band_class<P>(b, threadIdx.x & 63, wave, band_wave_lds(lds, band_lds_wave<P>()));
+  } else {
+    band_class<P>(b, threadIdx.x & 63, wave, nullptr);
+  }
 }
 
 // Every band class in one launch, for small jobs whose classes each fill only
@@ -1033,13 +1116,16 @@ __global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
 // keep one launch per class: this kernel holds the widest class's registers,
 // which would cost the narrower classes waves per SIMD.)
 __global__ __launch_bounds__(256) void sw_band_all_kernel(BatchView b) {
+  static_assert(band_seg<1>() == band_seg<2>() && band_seg<4>() == 0 && band_seg<8>() == 0, "");
+  __shared__ uint32_t lds[4 * band_lds_wave<1>()];  // the LDS-staged classes' segments
+  band_lds_t* wl = band_wave_lds(lds, band_lds_wave<1>());
   const int lane = threadIdx.x & 63;
   const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
   const int32_t nwaves = (int32_t)(gridDim.x * (blockDim.x / 64));
-  band_class<8>(b, lane, wave);
-  band_class<4>(b, lane, nwaves - 1 - wave);
-  band_class<2>(b, lane, wave);
-  band_class<1>(b, lane, nwaves - 1 - wave);
+  band_class<8>(b, lane, wave, wl);
+  band_class<4>(b, lane, nwaves - 1 - wave, wl);
+  band_class<2>(b, lane, wave, wl);
+  band_class<1>(b, lane, nwaves - 1 - wave, wl);
 }
 
 // ----------------------------------------------------------------------------
