@@ -481,38 +481,43 @@ struct FastState {
   uint32_t dcur, dnext;  // code dwords feeding the next refill
 };
 
-// PAD (edge chunks of score2 tasks): the npad columns after the last one
-// score 0 against every row, as SSW's query padding does; their cells get
-// s' = 16*(0 + gO) in place of the kill value.
-template <int G, int P, bool EDGE, bool PAD, int S>
-__device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, int npad, uint32_t tlo,
-                                          uint32_t thi, uint32_t go, uint32_t ge) {
-  uint32_t code = st.w & 0xFu;
-  st.w >>= 4;
-  if constexpr (EDGE) {
-    const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
-    code = kill ? kKillColN : code;
-  }
-  uint32_t pm = 0;    // halves whose column is a padding column
-  uint32_t keep = 0;  // halves whose column is a real column
-  if constexpr (PAD) {
-    const uint32_t clo = (uint32_t)(t - 2 * l - col_len);
-    pm = (clo < (uint32_t)npad ? 0x0000FFFFu : 0u) | (clo - 1u < (uint32_t)npad ? 0xFFFF0000u : 0u);
-    const uint32_t rlo = (uint32_t)(t - 2 * l);
-    keep = (rlo < (uint32_t)col_len ? 0x0000FFFFu : 0u) | (rlo - 1u < (uint32_t)col_len ? 0xFFFF0000u : 0u);
-  }
-  st.q = (st.q << 16) | code;
+// Substitution forms of a step's packed cells.
+//   kSubCodes: rb[p] and q hold codes; sp = perm(thi, tlo, subst_sel(rb[p], q))
+//              over the scoring table (2 ops per packed cell).
+//   kSubPad:   kSubCodes, then score2's padding columns (edge chunks of
+//              score2 tasks): the npad columns after the last one score 0
+//              against every row, as SSW's query padding does, so their cells
+//              get s' = 16*(0 + gO) in place of the kill value.
+//   kSubTab:   pairs without N.  The step turns its column's code into that
+//              column's table of s' against A, C, G, T: one v_alignbyte over
+//              the scoring window wh:wl (bytes x x x m x x x x), which the
+//              code's low two bits shift, so byte 3 - r is m when r is the
+//              column's base and x otherwise.  rb[p] holds byte selectors
+//              (low half: 3 - r over this column's table; high half: 7 - r
+//              over the last column's table, kept in q; rows past the end:
+//              0x0C, which reads 0) and sp = perm(q, tc, rb[p]) (1 op).
+constexpr int kSubCodes = 0, kSubPad = 1, kSubTab = 2;
+
+// The packed cells of one step (substitutions from t1:t0 as above) and the
+// exchange with the neighbouring lanes.
+template <int G, int P, int S, int SUB>
+__device__ __forceinline__ void fast_cells(FastState<G, P>& st, uint32_t t1, uint32_t t0, uint32_t keep, uint32_t pm,
+                                           uint32_t go, uint32_t ge) {
   constexpr uint32_t tag = (uint32_t)(15 - S) * 0x00010001u;
   uint32_t diag = st.xa;
   uint32_t vg = st.vg;
 #pragma unroll
   for (int p = 0; p < P; p++) {
-    const uint32_t sel = subst_sel(st.rb[p], st.q);
-    uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
+    uint32_t sp;
+    if constexpr (SUB == kSubTab) {
+      sp = __builtin_amdgcn_perm(t1, t0, st.rb[p]);
+    } else {
+      sp = __builtin_amdgcn_perm(t1, t0, subst_sel(st.rb[p], st.q));
+    }
     // score2 reads every row's maximum over the real and padding columns, so
     // columns past the padding get s' = 0 (d = Hdiag - gO): below the row's
     // own cells, where N-like values (d = Hdiag - ambiguity) might not be
-    if constexpr (PAD) sp = (sp & keep) | (go & pm);
+    if constexpr (SUB == kSubPad) sp = (sp & keep) | (go & pm);
     // d, E and F carry a bias of 16*gO: d = 16*(Hdiag + s + gO) (sp holds
     // 16*(s + gO) >= 0), E' = 16*(E + gO) >= 0, F' likewise.  Removing the bias
     // with an unsigned saturating subtract is H = max(0, d, E, F).
@@ -536,10 +541,48 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   st.vg = vnew;
 }
 
-template <int G, int P, bool EDGE, bool PAD, bool REV>
+// Scoring constants of a launch: the substitution table (tlo, thi), the
+// kSubTab window (wl, wh) and the all-ambiguity column table (amb4).
+struct StepConst {
+  uint32_t tlo, thi, go, ge, wl, wh, amb4;
+};
+
+template <int G, int P, bool EDGE, int SUB, int S>
+__device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, int npad,
+                                          const StepConst& k) {
+  if constexpr (SUB == kSubTab) {
+    uint32_t tc = __builtin_amdgcn_alignbyte(k.wh, k.wl, st.w);
+    st.w >>= 4;
+    if constexpr (EDGE) {
+      const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
+      tc = kill ? k.amb4 : tc;
+    }
+    fast_cells<G, P, S, kSubTab>(st, st.q, tc, 0, 0, k.go, k.ge);
+    st.q = tc;
+  } else {
+    uint32_t code = st.w & 0xFu;
+    st.w >>= 4;
+    if constexpr (EDGE) {
+      const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
+      code = kill ? kKillColN : code;
+    }
+    uint32_t pm = 0;    // halves whose column is a padding column
+    uint32_t keep = 0;  // halves whose column is a real column
+    if constexpr (SUB == kSubPad) {
+      const uint32_t clo = (uint32_t)(t - 2 * l - col_len);
+      pm = (clo < (uint32_t)npad ? 0x0000FFFFu : 0u) | (clo - 1u < (uint32_t)npad ? 0xFFFF0000u : 0u);
+      const uint32_t rlo = (uint32_t)(t - 2 * l);
+      keep = (rlo < (uint32_t)col_len ? 0x0000FFFFu : 0u) | (rlo - 1u < (uint32_t)col_len ? 0xFFFF0000u : 0u);
+    }
+    st.q = (st.q << 16) | code;
+    fast_cells<G, P, S, SUB>(st, k.thi, k.tlo, keep, pm, k.go, k.ge);
+  }
+}
+
+template <int G, int P, bool EDGE, int SUB, bool REV>
 __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
                                            const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
-                                           uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
+                                           const StepConst& k) {
   // Forward: w takes the 8 codes from dcur:dnext (loaded one refill
   // earlier); the dword after them is requested now and used at the next
   // refill.  Reverse (REV): the window of the next 8 codes lies one dword
@@ -561,23 +604,23 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
     }                                                                         \
   }
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, PAD, 0>(st, t0 + 0, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 1>(st, t0 + 1, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 2>(st, t0 + 2, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 3>(st, t0 + 3, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 4>(st, t0 + 4, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 5>(st, t0 + 5, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 6>(st, t0 + 6, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 7>(st, t0 + 7, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, SUB, 0>(st, t0 + 0, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 1>(st, t0 + 1, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 2>(st, t0 + 2, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 3>(st, t0 + 3, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 4>(st, t0 + 4, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 5>(st, t0 + 5, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 6>(st, t0 + 6, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 7>(st, t0 + 7, l, col_len, npad, k);
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, PAD, 8>(st, t0 + 8, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 9>(st, t0 + 9, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 10>(st, t0 + 10, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 11>(st, t0 + 11, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 12>(st, t0 + 12, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 13>(st, t0 + 13, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 14>(st, t0 + 14, l, col_len, npad, tlo, thi, go, ge);
-  fast_step<G, P, EDGE, PAD, 15>(st, t0 + 15, l, col_len, npad, tlo, thi, go, ge);
+  fast_step<G, P, EDGE, SUB, 8>(st, t0 + 8, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 9>(st, t0 + 9, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 10>(st, t0 + 10, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 11>(st, t0 + 11, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 12>(st, t0 + 12, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 13>(st, t0 + 13, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 14>(st, t0 + 14, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 15>(st, t0 + 15, l, col_len, npad, k);
 #undef MHMSW_REFILL
   // fold the chunk's per-row maxima into the (score, first step) records
   const uint32_t cbase = 0xFFF0u - (uint32_t)t0;  // 0xFFFF - (t0 + 15 - tag) = cbase + tag
@@ -617,9 +660,18 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const int nsteps = (nsteps_need + 15) & ~15;
 
   const uint32_t* __restrict__ codes = b.codes;
-  const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
-  const uint32_t go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
-  const uint32_t ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
+  StepConst k;
+  k.tlo = b.sc.perm_lo;
+  k.thi = b.sc.perm_hi;
+  k.go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
+  k.ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
+  {
+    // table bytes: 0 match, 1-3 mismatch, 4-7 ambiguity (all s' = 16*(s + gO))
+    const uint32_t m = k.tlo & 0xFFu, x = (k.tlo >> 8) & 0xFFu, a = k.thi & 0xFFu;
+    k.wl = x * 0x00010101u | (m << 24);
+    k.wh = x * 0x01010101u;
+    k.amb4 = a * 0x01010101u;
+  }
 
   // uniform: a launch holds the tasks of one pass (padding slots carry no
   // flags); reverse tasks read the forward codes backwards
@@ -632,14 +684,18 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const uint32_t rsh = (uint32_t)(r0 & 7) * 4u;
   const uint32_t a0 = codes[r0 >> 3], a1 = codes[(r0 >> 3) + 1], a2 = codes[(r0 >> 3) + 2];
   const uint32_t rw0 = __builtin_amdgcn_alignbit(a1, a0, rsh), rw1 = __builtin_amdgcn_alignbit(a2, a1, rsh);
+  bool has_n = false;  // an N among this lane's rows, or its group's columns
+  int32_t row0 = 2 * P * l;  // (not hoisted out of the task loop, as below)
+  __asm__ volatile("" : "+v"(row0));
 #pragma unroll
   for (int p = 0; p < P; p++) {
-    const int rlo = 2 * P * l + p, rhi = 2 * P * l + P + p;
+    const int rlo = row0 + p, rhi = row0 + P + p;
     const int mlo = rev ? 2 * P - 1 - p : p, mhi = rev ? P - 1 - p : P + p;
     const uint32_t nlo = ((mlo < 8 ? rw0 : rw1) >> (4 * (mlo & 7))) & 0xFu;
     const uint32_t nhi = ((mhi < 8 ? rw0 : rw1) >> (4 * (mhi & 7))) & 0xFu;
     const uint32_t clo = rlo < row_len ? (nlo | kRowHi) : kKillRowN;
     const uint32_t chi = rhi < row_len ? (nhi | kRowHi) : kKillRowN;
+    has_n = has_n || (rlo < row_len && nlo == (uint32_t)kCodeN) || (rhi < row_len && nhi == (uint32_t)kCodeN);
     st.rb[p] = clo | (chi << 16);
     st.ho[p] = 0;
     st.hg[p] = 0;
@@ -666,25 +722,70 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   st.dnext = rev ? codes[dw + 1] : codes[dw];
   st.w = 0;
 
+  // N among the columns: the group's lanes scan the column sequence's code
+  // dwords (codes [c0, c0 + col_len); nibbles outside it are masked off)
+  if (col_len > 0) {
+    const int64_t c0 = rev ? task.col_start - col_len + 1 : task.col_start;
+    const int64_t c1 = c0 + col_len;
+    for (int64_t d = (c0 >> 3) + l; d <= ((c1 - 1) >> 3); d += G) {
+      uint32_t v = codes[d] & 0x44444444u;  // bit 2 of a nibble: N (codes 0..4)
+      const int64_t n0 = d * 8;
+      if (n0 < c0) v &= 0xFFFFFFFFu << (uint32_t)(4 * (c0 - n0));
+      if (n0 + 8 > c1) v &= 0xFFFFFFFFu >> (uint32_t)(4 * (n0 + 8 - c1));
+      has_n = has_n || v != 0;
+    }
+  }
   const bool pads = b.score2 != 0 && !rev;
+  // the table form serves wavefronts whose pairs hold no N (and no score2
+  // padding): row codes become byte selectors, the last column's table starts
+  // as the kill table
+  const bool tab = !pads && __ballot(has_n) == 0;
+  if (tab) {
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      const uint32_t c = st.rb[p];
+      const uint32_t nlo = c & 0xFu, nhi = (c >> 16) & 0xFu;
+      const uint32_t slo = nlo == (uint32_t)kCodeN ? 0x0C0Cu : ((3u - nlo) | 0x0C00u);
+      const uint32_t shi = nhi == (uint32_t)kCodeN ? 0x0C0Cu : ((7u - nhi) | 0x0C00u);
+      st.rb[p] = slo | (shi << 16);
+    }
+    st.q = k.amb4;
+  }
+
   for (int t0 = 0; t0 < nsteps; t0 += 16) {
     const bool middle = t0 >= 2 * G && t0 + 16 <= mincols;
-    if (rev) {
-      if (middle) {
-        fast_chunk<G, P, false, false, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+    if (tab) {
+      if (rev) {
+        if (middle) {
+          fast_chunk<G, P, false, kSubTab, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
+        } else {
+          fast_chunk<G, P, true, kSubTab, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
+        }
+      } else if (middle) {
+        fast_chunk<G, P, false, kSubTab, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
       } else {
-        fast_chunk<G, P, true, false, true>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+        fast_chunk<G, P, true, kSubTab, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
+      }
+    } else if (rev) {
+      if (middle) {
+        fast_chunk<G, P, false, kSubCodes, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
+      } else {
+        fast_chunk<G, P, true, kSubCodes, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
       }
     } else if (middle) {
-      fast_chunk<G, P, false, false, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      fast_chunk<G, P, false, kSubCodes, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
     } else if (pads) {
-      fast_chunk<G, P, true, true, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      fast_chunk<G, P, true, kSubPad, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
     } else {
-      fast_chunk<G, P, true, false, false>(st, t0, l, col_len, npad, codes, dw, sh, tlo, thi, go, ge);
+      fast_chunk<G, P, true, kSubCodes, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
     }
   }
 
-  // per-lane best (score, i, j), then the group's
+  // per-lane best (score, i, j), then the group's.  The lane's first row
+  // passes through an empty asm so that the compiler does not hoist the 2P
+  // rows' key parts out of the task loop (20 registers held across it).
+  int32_t lrow = 2 * P * l;
+  __asm__ volatile("" : "+v"(lrow));
   uint64_t bestk = 0;
   int32_t hmax = 0;
   const bool rows_ref = (task.flags & kTaskRowsAreRef) != 0;
@@ -696,7 +797,7 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
       const int32_t h = (int32_t)(rec >> 16);
       hmax = h > hmax ? h : hmax;
       const int32_t tf = 0xFFFF - (int32_t)(rec & 0xFFFFu);
-      const int32_t r = 2 * P * l + half * P + p;
+      const int32_t r = lrow + half * P + p;
       const int32_t col = tf - (2 * l + half);
       if (r < row_len && h > 0 && col >= 0 && col < col_len) {
         const uint64_t k = rows_ref ? end_key(h, r, col) : end_key(h, col, r);
@@ -765,8 +866,14 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   }
 }
 
+// Waves per SIMD the compiler must leave room for (168 VGPRs for 3)
 template <int G, int P>
-__global__ __launch_bounds__(256) void sw_fast_kernel(BatchView b) {
+constexpr int fast_min_waves() {
+  return P >= 7 ? 3 : 1;
+}
+
+template <int G, int P>
+__global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(BatchView b) {
   constexpr int GPW = 64 / G;
   const int cls = (G == 16 ? kClassFast16 : kClassFast64) + P - 1;
   const int lane = threadIdx.x & 63;
