@@ -164,18 +164,36 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m, int w) {
 // ----------------------------------------------------------------------------
 // unpacking: 2-bit + N mask -> nibble codes
 // ----------------------------------------------------------------------------
+// Eight 2-bit codes (16 bits) and their N flags (8 bits) -> eight nibble
+// codes.  The packer stores N bases as code 0 with their flag set
+// (pack2bit.h), so a flag moved to bit 2 of its nibble makes the code 4.
+__device__ __forceinline__ uint32_t unpack8(uint32_t two16, uint32_t n8) {
+  uint32_t x = two16;  // code k in bits 2k..2k+1 -> bits 4k..4k+1
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  uint32_t n = n8;     // flag k in bit k -> bit 4k
+  n = (n | (n << 12)) & 0x000F000Fu;
+  n = (n | (n << 6)) & 0x03030303u;
+  n = (n | (n << 3)) & 0x11111111u;
+  return x | (n << 2);
+}
+
+// 32 bases per thread: two dwords of 2-bit codes and one of N flags in, four
+// code dwords out (one 16-byte store)
 __global__ void unpack_kernel(BatchView b) {
-  const int64_t ndw = b.total >> 3;  // total is a multiple of 32
-  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < ndw; d += (int64_t)gridDim.x * blockDim.x) {
-    uint32_t two = (b.packed2[d >> 1] >> ((uint32_t)(d & 1) * 16u)) & 0xFFFFu;
-    uint32_t nm = (b.nmask[d >> 2] >> ((uint32_t)(d & 3) * 8u)) & 0xFFu;
-    uint32_t out = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      uint32_t c = ((nm >> k) & 1u) ? (uint32_t)kCodeN : (two >> (2 * k)) & 3u;
-      out |= c << (4 * k);
-    }
-    b.codes[d] = out;
+  const int64_t ng = b.total >> 5;  // total is a multiple of 32
+  const uint2* __restrict__ two = reinterpret_cast<const uint2*>(b.packed2);
+  uint4* __restrict__ out = reinterpret_cast<uint4*>(b.codes);
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    const uint2 t = two[g];
+    const uint32_t nm = b.nmask[g];
+    uint4 o;
+    o.x = unpack8(t.x & 0xFFFFu, nm & 0xFFu);
+    o.y = unpack8(t.x >> 16, (nm >> 8) & 0xFFu);
+    o.z = unpack8(t.y & 0xFFFFu, (nm >> 16) & 0xFFu);
+    o.w = unpack8(t.y >> 16, nm >> 24);
+    out[g] = o;
   }
 }
 
@@ -1412,7 +1430,7 @@ hipError_t launch_clear_job(const BatchView& b, hipStream_t st) {
 }
 
 hipError_t launch_unpack(const BatchView& b, hipStream_t st) {
-  hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 3, 256)), dim3(256), 0, st, b);
+  hipLaunchKernelGGL(unpack_kernel, dim3(grid_for(b.total >> 5, 256)), dim3(256), 0, st, b);
   return hipGetLastError();
 }
 
