@@ -489,8 +489,8 @@ struct FastState {
   uint32_t ho[P];      // 16*H(r, c_last), packed
   uint32_t hg[P];      // 16*(F(r, c_last + 1) + gO), packed (gap along the columns), clamped >= 0
   uint32_t best[P];    // per-row running max of 16*H + tag, packed u16
-  uint32_t rec_lo[P];  // (H << 16) | (0xFFFF - first step), low-half rows
-  uint32_t rec_hi[P];  // same, high-half rows
+  uint32_t rec_h[P];   // per-row record: 16*H of the row's maximum so far, packed u16
+  uint32_t rec_s[P];   // and the first step at which it was reached, packed u16
   uint32_t xa, xb;     // exchanged 16*H of the row above, 2 steps / 1 step ago
   uint32_t vg;         // exchanged 16*(E + gO) of the first row, clamped >= 0
   uint32_t xs, vs;     // DPP shift results; lane 0 of a group never receives and keeps its boundary
@@ -640,17 +640,41 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
   fast_step<G, P, EDGE, SUB, 14>(st, t0 + 14, l, col_len, npad, k);
   fast_step<G, P, EDGE, SUB, 15>(st, t0 + 15, l, col_len, npad, k);
 #undef MHMSW_REFILL
-  // fold the chunk's per-row maxima into the (score, first step) records
-  const uint32_t cbase = 0xFFF0u - (uint32_t)t0;  // 0xFFFF - (t0 + 15 - tag) = cbase + tag
+  // Fold the per-row maxima into the records: a row whose maximum 16*H + tag
+  // beats its record takes the H and the step of the tag, t0 + (15 - tag) =
+  // t0 | (tag ^ 15).  (best is not cleared between chunks: a maximum carried
+  // over from an earlier chunk is never above the record it already set.)
+  const uint32_t cstep = (uint32_t)t0 * 0x00010001u | 0x000F000Fu;
 #pragma unroll
   for (int p = 0; p < P; p++) {
-    const uint32_t k = st.best[p];
-    const uint32_t klo = k & 0xFFFFu, khi = k >> 16;
-    const uint32_t clo = ((klo & 0xFFF0u) << 12) | ((klo & 0xFu) | cbase);
-    const uint32_t chi = ((khi & 0xFFF0u) << 12) | ((khi & 0xFu) | cbase);
-    st.rec_lo[p] = st.rec_lo[p] > clo ? st.rec_lo[p] : clo;
-    st.rec_hi[p] = st.rec_hi[p] > chi ? st.rec_hi[p] : chi;
+    const uint32_t m = st.best[p];
+    const uint32_t mh = m & 0xFFF0FFF0u;
+    // 0xFFFF in the halves where mh > rec_h (values below 0x7C00: the
+    // signed difference is exact)
+    uint32_t beat = u32(vi(pk_sub(st.rec_h[p], mh)) >> (short)15);
+    // (opaque to the compiler, which would otherwise turn the bit select
+    // below into a compare and a select per half)
+    __asm__("" : "+v"(beat));
+    const uint32_t ms = (m & 0x000F000Fu) ^ cstep;
+    st.rec_s[p] = (ms & beat) | (st.rec_s[p] & ~beat);
+    st.rec_h[p] = pk_maxu(st.rec_h[p], mh);
   }
+}
+
+// A pair's chunks: edge chunks (column checks, substitution EDGE_SUB) while
+// some lane may be left of its column 0 (t0 < 2G), middle chunks while every
+// lane is inside its pair (MID_SUB), edge chunks to the end.  Three loops, so
+// that no chunk chooses its form at run time.
+template <int G, int P, int EDGE_SUB, int MID_SUB, bool REV>
+__device__ __forceinline__ void fast_run(FastState<G, P>& st, int nsteps, int mincols, int l, int col_len, int npad,
+                                         const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
+                                         const StepConst& k) {
+  int t0 = 0;
+  const int head = nsteps < 2 * G ? nsteps : 2 * G;
+  for (; t0 < head; t0 += 16) fast_chunk<G, P, true, EDGE_SUB, REV>(st, t0, l, col_len, npad, codes, dw, sh, k);
+  for (; t0 < nsteps && t0 + 16 <= mincols; t0 += 16)
+    fast_chunk<G, P, false, MID_SUB, REV>(st, t0, l, col_len, npad, codes, dw, sh, k);
+  for (; t0 < nsteps; t0 += 16) fast_chunk<G, P, true, EDGE_SUB, REV>(st, t0, l, col_len, npad, codes, dw, sh, k);
 }
 
 template <int G, int P>
@@ -718,8 +742,8 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
     st.ho[p] = 0;
     st.hg[p] = 0;
     st.best[p] = 0;
-    st.rec_lo[p] = 0;
-    st.rec_hi[p] = 0;
+    st.rec_h[p] = 0;
+    st.rec_s[p] = 0;
   }
   st.xa = 0;  // boundary row: H = 0
   st.xb = 0;
@@ -770,33 +794,18 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
     st.q = k.amb4;
   }
 
-  for (int t0 = 0; t0 < nsteps; t0 += 16) {
-    const bool middle = t0 >= 2 * G && t0 + 16 <= mincols;
-    if (tab) {
-      if (rev) {
-        if (middle) {
-          fast_chunk<G, P, false, kSubTab, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
-        } else {
-          fast_chunk<G, P, true, kSubTab, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
-        }
-      } else if (middle) {
-        fast_chunk<G, P, false, kSubTab, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
-      } else {
-        fast_chunk<G, P, true, kSubTab, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
-      }
-    } else if (rev) {
-      if (middle) {
-        fast_chunk<G, P, false, kSubCodes, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
-      } else {
-        fast_chunk<G, P, true, kSubCodes, true>(st, t0, l, col_len, npad, codes, dw, sh, k);
-      }
-    } else if (middle) {
-      fast_chunk<G, P, false, kSubCodes, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
-    } else if (pads) {
-      fast_chunk<G, P, true, kSubPad, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
+  if (tab) {
+    if (rev) {
+      fast_run<G, P, kSubTab, kSubTab, true>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
     } else {
-      fast_chunk<G, P, true, kSubCodes, false>(st, t0, l, col_len, npad, codes, dw, sh, k);
+      fast_run<G, P, kSubTab, kSubTab, false>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
     }
+  } else if (rev) {
+    fast_run<G, P, kSubCodes, kSubCodes, true>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+  } else if (pads) {
+    fast_run<G, P, kSubPad, kSubCodes, false>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+  } else {
+    fast_run<G, P, kSubCodes, kSubCodes, false>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
   }
 
   // per-lane best (score, i, j), then the group's.  The lane's first row
@@ -811,10 +820,9 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   for (int p = 0; p < P; p++) {
 #pragma unroll
     for (int half = 0; half < 2; half++) {
-      const uint32_t rec = half ? st.rec_hi[p] : st.rec_lo[p];
-      const int32_t h = (int32_t)(rec >> 16);
+      const int32_t h = (int32_t)(((st.rec_h[p] >> (16 * half)) & 0xFFFFu) >> 4);
       hmax = h > hmax ? h : hmax;
-      const int32_t tf = 0xFFFF - (int32_t)(rec & 0xFFFFu);
+      const int32_t tf = (int32_t)((st.rec_s[p] >> (16 * half)) & 0xFFFFu);
       const int32_t r = lrow + half * P + p;
       const int32_t col = tf - (2 * l + half);
       if (r < row_len && h > 0 && col >= 0 && col < col_len) {
@@ -847,7 +855,7 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
 #pragma unroll
       for (int half = 0; half < 2; half++) {
         const int32_t r = 2 * P * l + half * P + p;
-        const int32_t h = (int32_t)((half ? st.rec_hi[p] : st.rec_lo[p]) >> 16);
+        const int32_t h = (int32_t)(((st.rec_h[p] >> (16 * half)) & 0xFFFFu) >> 4);
         if (r < row_len && (r < lo || r > hi)) m = h > m ? h : m;
       }
     }
