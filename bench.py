@@ -342,10 +342,19 @@ def cigar_leg(al, batch, res, scoring, steps, check, cores):
 
 
 def klign_call_leg(calls=40, npairs=20000):
-    """klign-sized calls: new batches of 20k pairs of 150 bp reads vs 350 bp windows, host strings in and
-    results out per call (stage = pack + copy, the job, the result copy), as the drop-in driver is called
-    once per block (klign.cpp:356-388)."""
+    """klign-sized calls: new batches of 20k pairs of 150 bp reads vs 350 bp windows per call, std::string in and
+    short results out through the drop-in adept_sw::GPUDriver (mhm2_staging_amd/klign_call_bench, the call
+    sequence of klign.cpp:356-388); and the same batches through the Python API (stage = pack + copy, the job,
+    the result copy)."""
+    import subprocess
     import mhm2_staging_amd as m
+    prog = os.path.join(ROOT, "mhm2_staging_amd", "klign_call_bench")
+    r = subprocess.run([prog, "--calls", str(calls), "--npairs", str(npairs)], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"klign_call_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out.pop("checksum", None)
     batches = [m.generate(1, npairs=npairs, seed=101 + k) for k in range(4)]
     al = m.Aligner(device=0, scoring=m.MHM2_SCORING, read_is_ssw_ref=True)
     for b in batches:  # every buffer at its size, the worker pool started
@@ -368,10 +377,13 @@ def klign_call_leg(calls=40, npairs=20000):
     al.close()
     cells = float(np.mean([b.cells() for b in batches]))
     w = float(np.median(wall))
-    return {"workload": f"{npairs} pairs per call, 150 bp reads vs 350 bp windows, a new batch every call",
-            "calls": calls, "ms_per_call": round(w * 1e3, 3), "value": round(cells / w / 1e9, 1), "unit": "GCUPS",
-            "stage_ms": round(float(np.median(stage)) * 1e3, 3), "job_ms": round(float(np.median(job)) * 1e3, 3),
-            "results_ms": round(float(np.median(take)) * 1e3, 3)}
+    out["python_api"] = {"workload": "the same batches as flat arrays through the Python API (Aligner.stage, "
+                                     "align_staged, wait, results)",
+                         "ms_per_call": round(w * 1e3, 3), "value": round(cells / w / 1e9, 1), "unit": "GCUPS",
+                         "stage_ms": round(float(np.median(stage)) * 1e3, 3),
+                         "job_ms": round(float(np.median(job)) * 1e3, 3),
+                         "results_ms": round(float(np.median(take)) * 1e3, 3)}
+    return out
 
 
 def locassm_leg(nctgs, steps, warmup, check, cores):
@@ -529,6 +541,7 @@ def main():
     ap.add_argument("--no-klign", action="store_true", help="skip the klign-sized call measurement")
     ap.add_argument("--only-headline", action="store_true", help="the headline batch workload only")
     ap.add_argument("--locassm-only", action="store_true", help="only the local assembly measurement")
+    ap.add_argument("--klign-only", action="store_true", help="only the klign-sized call measurement")
     ap.add_argument("--locassm-contigs", type=int, default=50000)
     ap.add_argument("--no-autotune", action="store_true", help="time the default scheduling parameters")
     ap.add_argument("--tuning", type=str, default="", metavar="B,G,S",
@@ -555,6 +568,12 @@ def main():
         sys.exit(2)
     cores, core_info = host_cores()
     log(f"[bench] host cores used: {cores} ({core_info})")
+
+    if args.klign_only:
+        kc = klign_call_leg()
+        log(f"[bench] klign-sized calls: {kc}")
+        print(json.dumps({"klign_call": kc}), flush=True)
+        return
 
     if args.locassm_only:
         la = locassm_leg(args.locassm_contigs, args.steps, args.warmup, not args.no_cpu, cores)

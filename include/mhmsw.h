@@ -164,13 +164,25 @@ int mhmsw_autotune(mhmsw_ctx *ctx, int repeats, mhmsw_tuning *best);
  * mhmsw_stage packs and copies the batch to the device and returns when the
  * host buffers may be reused; mhmsw_align_staged starts the job on the staged
  * batch (it may be called again on the same staged batch);
- * mhmsw_submit = stage + align_staged.  Results stay valid until the next
- * stage/submit. */
+ * mhmsw_submit = stage + align_staged, and its job also copies the results to
+ * pinned host memory as its last step, so mhmsw_results after it is a host
+ * copy.  Results stay valid until the next stage/submit. */
 int mhmsw_stage(mhmsw_ctx *ctx, int64_t npairs, const char *reads, const int64_t *read_off, const int32_t *read_len,
                 const char *ctgs, const int64_t *ctg_off, const int32_t *ctg_len);
 int mhmsw_align_staged(mhmsw_ctx *ctx);
 int mhmsw_submit(mhmsw_ctx *ctx, int64_t npairs, const char *reads, const int64_t *read_off, const int32_t *read_len,
                  const char *ctgs, const int64_t *ctg_off, const int32_t *ctg_len);
+/* mhmsw_stage / mhmsw_submit with every sequence given by its own pointer:
+ * read p is read_len[p] bytes at reads[p], window p ctg_len[p] bytes at
+ * ctgs[p] (a pointer may be NULL when its length is 0).  This is the form a
+ * caller holding std::string objects has without copying them; it replaces
+ * the flattening of GPUDriver::run_kernel_forwards (src/adept-sw/driver.cpp:
+ * 300-339, two vector copies and a byte memcpy per batch).  The strings are
+ * read before the call returns. */
+int mhmsw_stage_seqs(mhmsw_ctx *ctx, int64_t npairs, const char *const *reads, const int32_t *read_len,
+                     const char *const *ctgs, const int32_t *ctg_len);
+int mhmsw_submit_seqs(mhmsw_ctx *ctx, int64_t npairs, const char *const *reads, const int32_t *read_len,
+                      const char *const *ctgs, const int32_t *ctg_len);
 /* 1 when the job has finished, 0 while it runs, < 0 on error. */
 int mhmsw_query(mhmsw_ctx *ctx);
 int mhmsw_wait(mhmsw_ctx *ctx);

@@ -95,7 +95,7 @@ EXPORTS = [
     "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
     "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_score2", "mhmsw_set_cigar", "mhmsw_set_graphs",
     "mhmsw_set_tuning", "mhmsw_get_tuning", "mhmsw_autotune", "mhmsw_stage",
-    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_results_score2", "mhmsw_results_cigar", "mhmsw_get_stats",
+    "mhmsw_align_staged", "mhmsw_submit", "mhmsw_stage_seqs", "mhmsw_submit_seqs", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_results_score2", "mhmsw_results_cigar", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
     "mhmsw_contigs_load", "mhmsw_contigs_free", "mhmsw_contigs_bytes", "mhmsw_stage_windows",
     "mhmsw_server_create", "mhmsw_server_submit", "mhmsw_server_status", "mhmsw_server_wait",
@@ -136,6 +136,8 @@ def lib():
         "mhmsw_stage": [vp, i64, vp, vp, vp, vp, vp, vp],
         "mhmsw_align_staged": [vp],
         "mhmsw_submit": [vp, i64, vp, vp, vp, vp, vp, vp],
+        "mhmsw_stage_seqs": [vp, i64, vp, vp, vp, vp],
+        "mhmsw_submit_seqs": [vp, i64, vp, vp, vp, vp],
         "mhmsw_query": [vp],
         "mhmsw_wait": [vp],
         "mhmsw_results": [vp, vp, i64],

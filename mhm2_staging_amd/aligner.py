@@ -114,6 +114,26 @@ class Aligner:
     def align_staged(self):
         self._check(self._L.mhmsw_align_staged(self._ctx), "align_staged")
 
+    def submit_seqs(self, reads, ctgs):
+        """Submit pairs given as sequences of bytes objects (read p = reads[p], window p = ctgs[p]; None for an
+        empty one) through mhmsw_submit_seqs, which packs every sequence where it lies; the job also copies its
+        results to pinned host memory."""
+        if len(reads) != len(ctgs):
+            raise ValueError("reads and ctgs differ in count")
+        n = len(reads)
+        rp = (ctypes.c_char_p * max(n, 1))(*reads)
+        cp = (ctypes.c_char_p * max(n, 1))(*ctgs)
+        rl = np.array([len(x) if x is not None else 0 for x in reads], dtype=np.int32)
+        cl = np.array([len(x) if x is not None else 0 for x in ctgs], dtype=np.int32)
+        self._batch = _SeqBatch(n, (rp, cp, rl, cl, reads, ctgs))
+        self._check(self._L.mhmsw_submit_seqs(self._ctx, n, ctypes.cast(rp, ctypes.c_void_p), _ptr(rl),
+                                              ctypes.cast(cp, ctypes.c_void_p), _ptr(cl)), "submit_seqs")
+
+    def align_seqs(self, reads, ctgs):
+        self.submit_seqs(reads, ctgs)
+        self.wait()
+        return self.results()
+
     def submit(self, batch: PairBatch):
         self.stage(batch)
         self.align_staged()
@@ -160,6 +180,14 @@ class Aligner:
         self.submit(batch)
         self.wait()
         return self.results()
+
+
+class _SeqBatch:
+    """The batch of a submit_seqs call: its size, and the host objects it reads."""
+
+    def __init__(self, npairs, keep):
+        self.npairs = npairs
+        self.keep = keep
 
 
 CIGAR_OPS = "MIDNSHP=X"

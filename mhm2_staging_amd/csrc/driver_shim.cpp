@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstdint>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -32,8 +33,7 @@ struct DriverState {
   size_t cap = 0;
   std::vector<short> store;  // 5 * cap
   std::vector<mhmsw_result> res;
-  std::vector<char> rbuf, cbuf;
-  std::vector<int64_t> roff, coff;
+  std::vector<const char *> rptr, cptr;  // each string's bytes, read in place
   std::vector<int32_t> rlen, clen;
   int64_t npairs = 0;
   bool submitted = false;
@@ -187,21 +187,16 @@ void GPUDriver::set_read_is_ssw_ref(bool read_is_ref) {
   MHMSW_CHECK(driver_state, mhmsw_set_orientation(driver_state->ctx, read_is_ref ? 1 : 0));
 }
 
-static void flatten(const std::vector<std::string> &v, std::vector<char> &buf, std::vector<int64_t> &off,
-                    std::vector<int32_t> &len) {
-  size_t tot = 0;
-  for (auto &s : v) tot += s.size();
-  buf.resize(tot + 1);
-  off.resize(v.size());
+// Each sequence by its own pointer and length: the strings are packed where
+// they lie (the reference copies them into one buffer first, driver.cpp:300-339).
+static void point_at(const std::vector<std::string> &v, std::vector<const char *> &ptr, std::vector<int32_t> &len) {
+  ptr.resize(v.size());
   len.resize(v.size());
-  size_t o = 0;
   for (size_t i = 0; i < v.size(); i++) {
-    off[i] = (int64_t)o;
+    if (v[i].size() > (size_t)INT32_MAX) throw std::runtime_error("GPU assert sequence longer than 2^31 bytes\n");
+    ptr[i] = v[i].data();
     len[i] = (int32_t)v[i].size();
-    std::copy(v[i].begin(), v[i].end(), buf.begin() + o);
-    o += v[i].size();
   }
-  buf[tot] = 0;
 }
 
 void GPUDriver::run_kernel_forwards(std::vector<std::string> &reads, std::vector<std::string> &contigs,
@@ -211,11 +206,11 @@ void GPUDriver::run_kernel_forwards(std::vector<std::string> &reads, std::vector
   DriverState *s = driver_state;
   if (!s) throw std::runtime_error("GPU assert run_kernel_forwards before init\n");
   if (reads.size() != contigs.size()) throw std::runtime_error("GPU assert reads and contigs differ in count\n");
-  flatten(reads, s->rbuf, s->roff, s->rlen);
-  flatten(contigs, s->cbuf, s->coff, s->clen);
+  point_at(reads, s->rptr, s->rlen);
+  point_at(contigs, s->cptr, s->clen);
   s->npairs = (int64_t)reads.size();
-  MHMSW_CHECK(s, mhmsw_submit(s->ctx, s->npairs, s->rbuf.data(), s->roff.data(), s->rlen.data(), s->cbuf.data(),
-                              s->coff.data(), s->clen.data()));
+  MHMSW_CHECK(s, mhmsw_submit_seqs(s->ctx, s->npairs, s->rptr.data(), s->rlen.data(), s->cptr.data(),
+                                   s->clen.data()));
   s->submitted = true;
   s->fetched = false;
   s->backwards_called = false;

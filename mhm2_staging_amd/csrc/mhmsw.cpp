@@ -154,6 +154,7 @@ struct mhmsw_ctx {
   // tail_results: the job also copies its results into pinned memory before
   // ev_done (serving slots), so taking them needs no device copy
   bool tail_results = false, last_job_tail_results = false;
+  bool job_host_results = false;  // this job only (mhmsw_submit*): as tail_results
   HostPinned h_results;
 };
 
@@ -419,6 +420,10 @@ struct StageInput {
   const char* ctgs = nullptr;
   const int64_t* ctg_off = nullptr;
   const int32_t* ctg_len = nullptr;
+  // or one pointer per sequence (mhmsw_stage_seqs): read p at read_ptr[p],
+  // window p at ctg_ptr[p]; then reads/read_off and ctgs/ctg_off are unused
+  const char* const* read_ptr = nullptr;
+  const char* const* ctg_ptr = nullptr;
   const mhmsw_contigs* res = nullptr;
   const int32_t* ctg_id = nullptr;
   const int32_t* win_off = nullptr;
@@ -511,8 +516,9 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
                       const PackDest* dest = nullptr) {
   const int64_t npairs = in.npairs;
   const bool windowed = in.res != nullptr;
-  if (npairs < 0 || (npairs > 0 && (!in.reads || !in.read_off || !in.read_len)) ||
-      (npairs > 0 && !windowed && (!in.ctgs || !in.ctg_off || !in.ctg_len)) ||
+  const bool reads_ok = in.read_len && (in.read_ptr || (in.reads && in.read_off));
+  const bool ctgs_ok = in.ctg_len && (in.ctg_ptr || (in.ctgs && in.ctg_off));
+  if (npairs < 0 || (npairs > 0 && !reads_ok) || (npairs > 0 && !windowed && !ctgs_ok) ||
       (npairs > 0 && windowed && (!in.ctg_id || !in.win_off || !in.win_len))) {
     pb.err = "bad batch arguments";
     return MHMSW_E_ARG;
@@ -560,6 +566,8 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
         m[0] = 0;
         if (rl < 0 || cl < 0) {
           snprintf(m, sizeof(m), "negative length at pair %lld", (long long)p);
+        } else if ((in.read_ptr && rl > 0 && !in.read_ptr[p]) || (!windowed && in.ctg_ptr && cl > 0 && !in.ctg_ptr[p])) {
+          snprintf(m, sizeof(m), "pair %lld: null sequence pointer", (long long)p);
         } else if (windowed && (in.ctg_id[p] < 0 || in.ctg_id[p] >= in.res->n || in.win_off[p] < 0 ||
                                 (int64_t)in.win_off[p] + cl > in.res->len[(size_t)in.ctg_id[p]])) {
           snprintf(m, sizeof(m), "pair %lld: window outside its resident contig", (long long)p);
@@ -682,7 +690,9 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
         uint32_t* nb = pb.nm_p + b0 / 32;
         std::fill(two, two + span / 16, 0u);
         std::fill(nb, nb + span / 32, 0u);
-        const char* src = (s & 1) ? in.ctgs + in.ctg_off[s >> 1] : in.reads + in.read_off[s >> 1];
+        const int64_t q = s >> 1;
+        const char* src = (s & 1) ? (in.ctg_ptr ? in.ctg_ptr[q] : in.ctgs + in.ctg_off[q])
+                                  : (in.read_ptr ? in.read_ptr[q] : in.reads + in.read_off[q]);
         pack_seq(src, len, two, nb);
       }
     }
@@ -1022,6 +1032,18 @@ int mhmsw_stage(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* 
   return stage_impl(c, in);
 }
 
+int mhmsw_stage_seqs(mhmsw_ctx* c, int64_t npairs, const char* const* reads, const int32_t* read_len,
+                     const char* const* ctgs, const int32_t* ctg_len) {
+  if (!c) return MHMSW_E_ARG;
+  StageInput in;
+  in.npairs = npairs;
+  in.read_ptr = reads;
+  in.read_len = read_len;
+  in.ctg_ptr = ctgs;
+  in.ctg_len = ctg_len;
+  return stage_impl(c, in);
+}
+
 int mhmsw_stage_windows(mhmsw_ctx* c, const mhmsw_contigs* contigs, int64_t npairs, const char* reads,
                         const int64_t* read_off, const int32_t* read_len, const int32_t* ctg_id, const int32_t* win_off,
                         const int32_t* win_len) {
@@ -1048,6 +1070,13 @@ static int plain_lds_cols(const mhmsw_ctx* c) {
   return c->max_plain_cols;
 }
 
+// Jobs with at most this many packed-form pairs run every band class of the
+// reverse pass in one launch, and their reverse rectangles in one class per
+// group size (the largest the forward pass used: a reverse rectangle is never
+// taller than its forward one).  An empty persistent launch costs about 4.5 us,
+// a few times what running these jobs' few rectangles at the larger P costs.
+constexpr int64_t kBandAllMaxPairs = 65536;
+
 static BatchView make_view(mhmsw_ctx* c) {
   BatchView b;
   std::memset(&b, 0, sizeof(b));  // the graph cache compares views bytewise
@@ -1073,6 +1102,15 @@ static BatchView make_view(mhmsw_ctx* c) {
   b.bin_shift = c->tuning.bin_shift;
   b.score2 = c->score2;
   b.s2 = c->d_s2.ptr;
+  if (c->n_fast <= kBandAllMaxPairs) {
+    int maxp16 = 0, maxp64 = 0;
+    for (int P = 1; P <= kMaxP; P++) {
+      if (c->fast_class_used[kClassFast16 + P - 1]) maxp16 = P;
+      if (c->fast_class_used[kClassFast64 + P - 1]) maxp64 = P;
+    }
+    b.rev_one16 = maxp64 > 0 ? kMaxP : maxp16;
+    b.rev_one64 = maxp64;
+  }
   if (c->res) {
     b.res2 = c->res->d2;
     b.resN = c->res->dN;
@@ -1140,9 +1178,6 @@ static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
   return launch_cigar(make_cigar_view(c), c->cus * 16, c->cus * 12, st);
 }
 
-// jobs with at most this many packed-form pairs run every band class of the
-// reverse pass in one launch
-constexpr int64_t kBandAllMaxPairs = 65536;
 
 // ev: record phase events (eager launches only; events recorded inside a
 // captured graph cannot be timed)
@@ -1153,10 +1188,19 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // classes) and no more per CU than the kernel keeps resident (occ), so
   // that a small job's tasks are shared out by the work counter rather than
   // one per wavefront to blocks that wait for a free CU
+  // A class with few wavefront tasks runs on fewer waves per CU: with at
+  // least kMinTasksPerWave tasks for each wave, the work counter shares them
+  // out evenly, where one task per wave leaves the spread to the order in
+  // which the dispatcher places blocks (a 20k-pair job's 1,250 blocks at up
+  // to 8 per CU: job 0.55 ms; at 2 per CU: 0.45 ms).
+  constexpr int64_t kMinTasksPerWave = 2;
   auto grid_cap = [&](int64_t pairs, int pairs_per_block, int occ) {
-    const int per_cu = occ > 0 ? std::min(c->tuning.blocks_per_cu, occ) : c->tuning.blocks_per_cu;
+    int per_cu = occ > 0 ? std::min(c->tuning.blocks_per_cu, occ) : c->tuning.blocks_per_cu;
+    const int64_t blocks = (pairs + pairs_per_block - 1) / pairs_per_block;  // one task per wave
+    const int64_t fit = blocks / ((int64_t)c->cus * kMinTasksPerWave);      // blocks per CU at that share
+    if (fit < per_cu) per_cu = (int)std::max<int64_t>(std::min(2, per_cu), fit);
     const int64_t full = std::max(1, c->cus * per_cu);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(full, (pairs + pairs_per_block - 1) / pairs_per_block));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(full, blocks));
   };
   // the 32-bit form runs one wavefront per 64-thread block; its grid does not
   // follow the packed form's tuning (16 wavefronts per CU: its fastest
@@ -1236,9 +1280,11 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       // a reverse rectangle is never taller than its forward one, so the
       // reverse classes are bounded by the forward ones
       const int rev16 = maxp64 > 0 ? kMaxP : maxp16;
-      for (int P = maxp64; P >= 1; P--)
+      // a small job's rectangles sit in one class per group size (make_view)
+      const int low64 = b.rev_one64 > 0 ? b.rev_one64 : 1, low16 = b.rev_one16 > 0 ? b.rev_one16 : 1;
+      for (int P = maxp64; P >= low64; P--)
         HIP_OR_FAIL(c, launch_fast(bv, 64, P, grid_cap(c->n_fast, 4, c->occ_fast[1][P - 1]), st));
-      for (int P = rev16; P >= 1; P--)
+      for (int P = rev16; P >= low16; P--)
         HIP_OR_FAIL(c, launch_fast(bv, 16, P, grid_cap(c->n_fast, 16, c->occ_fast[0][P - 1]), st));
       // pairs whose begin lies within a narrow band of diagonals: one lane each
       if (c->n_fast <= kBandAllMaxPairs) {
@@ -1322,7 +1368,7 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
   }
   c->last_job_graph = graph;
   c->last_job_windows = c->res != nullptr && c->npairs > 0;
-  c->last_job_tail_results = c->tail_results && c->npairs > 0;
+  c->last_job_tail_results = (c->tail_results || c->job_host_results) && c->npairs > 0;
   if (c->last_job_tail_results) HIP_OR_FAIL(c, c->h_results.ensure((size_t)c->npairs * sizeof(AlnResult)));
   // after the job: its error words, counters and CIGAR pool use into pinned
   // memory, then ev_done (outside any captured graph)
@@ -1384,11 +1430,27 @@ int mhmsw_set_graphs(mhmsw_ctx* c, int enable) {
   return MHMSW_OK;
 }
 
+// A submitted job's results go to the host: the job copies them into pinned
+// memory as its last step, so mhmsw_results is a host copy.
+static int align_for_host(mhmsw_ctx* c) {
+  c->job_host_results = true;
+  const int rc = mhmsw_align_staged(c);
+  c->job_host_results = false;
+  return rc;
+}
+
 int mhmsw_submit(mhmsw_ctx* c, int64_t npairs, const char* reads, const int64_t* read_off, const int32_t* read_len,
                  const char* ctgs, const int64_t* ctg_off, const int32_t* ctg_len) {
   int rc = mhmsw_stage(c, npairs, reads, read_off, read_len, ctgs, ctg_off, ctg_len);
   if (rc != MHMSW_OK) return rc;
-  return mhmsw_align_staged(c);
+  return align_for_host(c);
+}
+
+int mhmsw_submit_seqs(mhmsw_ctx* c, int64_t npairs, const char* const* reads, const int32_t* read_len,
+                      const char* const* ctgs, const int32_t* ctg_len) {
+  int rc = mhmsw_stage_seqs(c, npairs, reads, read_len, ctgs, ctg_len);
+  if (rc != MHMSW_OK) return rc;
+  return align_for_host(c);
 }
 
 // After a job with the CIGAR pass: a pass whose op pool or generic pool was

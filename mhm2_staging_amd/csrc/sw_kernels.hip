@@ -30,6 +30,12 @@
 //    whose H = max(d, E, F) is one v_pk_maximum3_f16 over biased values).
 //  * plain (32-bit): one wavefront per pair, one row per lane, rows in tiles
 //    of 64, the last row of a tile passed to the next through LDS.
+//
+// The packed kernel's instantiations are compiled in four parts
+// (sw_fast_16_1.hip, sw_fast_16_5.hip, sw_fast_64_1.hip, sw_fast_64_5.hip:
+// this file with MHMSW_FAST_PART defined, one group size and four values of P
+// each), so that they build in parallel; this file alone holds everything
+// else and dispatches to them.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -160,6 +166,8 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m, int w) {
   hi = (uint32_t)__shfl_xor((int)hi, m, w);
   return ((uint64_t)hi << 32) | lo;
 }
+
+#ifndef MHMSW_FAST_PART  // compiled once; the packed kernels' parts skip it
 
 // ----------------------------------------------------------------------------
 // unpacking: 2-bit + N mask -> nibble codes
@@ -333,6 +341,10 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   } else {
     cls = kClassPlain;
   }
+  if (pass == kPassRev) {  // a small job's reverse rectangles: one class per group size
+    if (b.rev_one16 > 0 && cls >= kClassFast16 && cls < kClassFast16 + kMaxP) cls = kClassFast16 + b.rev_one16 - 1;
+    if (b.rev_one64 > 0 && cls >= kClassFast64 && cls < kClassFast64 + kMaxP) cls = kClassFast64 + b.rev_one64 - 1;
+  }
   bool rows_ref;
   if (cls == kClassPlain) {
     rows_ref = !ref_rows_if_short;  // rows = the longer sequence, columns (LDS row) = the shorter
@@ -479,6 +491,8 @@ __global__ void scatter_kernel(BatchView b, int pass) {
     b.tasks[b.sched->bucket_base[bk] + b.rank[p]] = t;
   }
 }
+
+#endif  // MHMSW_FAST_PART
 
 // ----------------------------------------------------------------------------
 // fast packed 16-bit kernel
@@ -926,6 +940,8 @@ __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(
     w = wn;
   }
 }
+
+#ifndef MHMSW_FAST_PART
 
 // ----------------------------------------------------------------------------
 // banded reverse kernel: one lane per pair
@@ -1461,36 +1477,9 @@ hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int G, int P>
-static hipError_t launch_fast_one(const BatchView& b, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((sw_fast_kernel<G, P>), dim3(grid), dim3(256), 0, st, b);
-  return hipGetLastError();
-}
-
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st) {
-  if (group == 16) {
-    switch (P) {
-      case 1: return launch_fast_one<16, 1>(b, grid, st);
-      case 2: return launch_fast_one<16, 2>(b, grid, st);
-      case 3: return launch_fast_one<16, 3>(b, grid, st);
-      case 4: return launch_fast_one<16, 4>(b, grid, st);
-      case 5: return launch_fast_one<16, 5>(b, grid, st);
-      case 6: return launch_fast_one<16, 6>(b, grid, st);
-      case 7: return launch_fast_one<16, 7>(b, grid, st);
-      case 8: return launch_fast_one<16, 8>(b, grid, st);
-    }
-  } else {
-    switch (P) {
-      case 1: return launch_fast_one<64, 1>(b, grid, st);
-      case 2: return launch_fast_one<64, 2>(b, grid, st);
-      case 3: return launch_fast_one<64, 3>(b, grid, st);
-      case 4: return launch_fast_one<64, 4>(b, grid, st);
-      case 5: return launch_fast_one<64, 5>(b, grid, st);
-      case 6: return launch_fast_one<64, 6>(b, grid, st);
-      case 7: return launch_fast_one<64, 7>(b, grid, st);
-      case 8: return launch_fast_one<64, 8>(b, grid, st);
-    }
-  }
+  if (P >= 1 && P <= 4) return group == 16 ? launch_fast_16_1(b, P, grid, st) : launch_fast_64_1(b, P, grid, st);
+  if (P >= 5 && P <= kMaxP) return group == 16 ? launch_fast_16_5(b, P, grid, st) : launch_fast_64_5(b, P, grid, st);
   return hipErrorInvalidValue;
 }
 
@@ -1523,29 +1512,9 @@ static hipError_t occ(K kernel, int* blocks_per_cu) {
 }
 
 hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu) {
-  if (group == 16) {
-    switch (P) {
-      case 1: return occ(sw_fast_kernel<16, 1>, blocks_per_cu);
-      case 2: return occ(sw_fast_kernel<16, 2>, blocks_per_cu);
-      case 3: return occ(sw_fast_kernel<16, 3>, blocks_per_cu);
-      case 4: return occ(sw_fast_kernel<16, 4>, blocks_per_cu);
-      case 5: return occ(sw_fast_kernel<16, 5>, blocks_per_cu);
-      case 6: return occ(sw_fast_kernel<16, 6>, blocks_per_cu);
-      case 7: return occ(sw_fast_kernel<16, 7>, blocks_per_cu);
-      case 8: return occ(sw_fast_kernel<16, 8>, blocks_per_cu);
-    }
-  } else {
-    switch (P) {
-      case 1: return occ(sw_fast_kernel<64, 1>, blocks_per_cu);
-      case 2: return occ(sw_fast_kernel<64, 2>, blocks_per_cu);
-      case 3: return occ(sw_fast_kernel<64, 3>, blocks_per_cu);
-      case 4: return occ(sw_fast_kernel<64, 4>, blocks_per_cu);
-      case 5: return occ(sw_fast_kernel<64, 5>, blocks_per_cu);
-      case 6: return occ(sw_fast_kernel<64, 6>, blocks_per_cu);
-      case 7: return occ(sw_fast_kernel<64, 7>, blocks_per_cu);
-      case 8: return occ(sw_fast_kernel<64, 8>, blocks_per_cu);
-    }
-  }
+  if (P >= 1 && P <= 4) return group == 16 ? occupancy_fast_16_1(P, blocks_per_cu) : occupancy_fast_64_1(P, blocks_per_cu);
+  if (P >= 5 && P <= kMaxP)
+    return group == 16 ? occupancy_fast_16_5(P, blocks_per_cu) : occupancy_fast_64_5(P, blocks_per_cu);
   return hipErrorInvalidValue;
 }
 
@@ -1559,5 +1528,44 @@ hipError_t band_kernel_occupancy(int P, int* blocks_per_cu) {
   }
   return hipErrorInvalidValue;
 }
+
+#else  // MHMSW_FAST_PART: one part of the packed kernels' instantiations
+
+template <int G, int P0, int P1>
+static hipError_t launch_fast_range(const BatchView& b, int P, int grid, hipStream_t st) {
+  if constexpr (P0 > P1) {
+    return hipErrorInvalidValue;
+  } else {
+    if (P == P0) {
+      hipLaunchKernelGGL((sw_fast_kernel<G, P0>), dim3(grid), dim3(256), 0, st, b);
+      return hipGetLastError();
+    }
+    return launch_fast_range<G, P0 + 1, P1>(b, P, grid, st);
+  }
+}
+
+template <int G, int P0, int P1>
+static hipError_t occupancy_fast_range(int P, int* blocks_per_cu) {
+  if constexpr (P0 > P1) {
+    return hipErrorInvalidValue;
+  } else {
+    if (P == P0) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sw_fast_kernel<G, P0>, 256, 0);
+    return occupancy_fast_range<G, P0 + 1, P1>(P, blocks_per_cu);
+  }
+}
+
+#define MHMSW_PART_NAME_(f, g, p) f##_##g##_##p
+#define MHMSW_PART_NAME(f, g, p) MHMSW_PART_NAME_(f, g, p)
+hipError_t MHMSW_PART_NAME(launch_fast, MHMSW_FAST_G, MHMSW_FAST_P0)(const BatchView& b, int P, int grid,
+                                                                     hipStream_t st) {
+  return launch_fast_range<MHMSW_FAST_G, MHMSW_FAST_P0, MHMSW_FAST_P1>(b, P, grid, st);
+}
+hipError_t MHMSW_PART_NAME(occupancy_fast, MHMSW_FAST_G, MHMSW_FAST_P0)(int P, int* blocks_per_cu) {
+  return occupancy_fast_range<MHMSW_FAST_G, MHMSW_FAST_P0, MHMSW_FAST_P1>(P, blocks_per_cu);
+}
+#undef MHMSW_PART_NAME
+#undef MHMSW_PART_NAME_
+
+#endif  // MHMSW_FAST_PART
 
 }  // namespace mhmsw

@@ -43,6 +43,10 @@ struct BatchView {
   // rows are then always SSW-ref positions
   int32_t score2;
   int32_t* s2;           // [npairs]
+  // reverse pass of a small job: every 16-lane (64-lane) rectangle task goes
+  // to the class of P = rev_one16 (rev_one64), so that one launch per group
+  // size runs them; 0: each task's class follows its rows
+  int32_t rev_one16, rev_one64;
 };
 
 // CIGAR pass over the results of a finished job (cigar_kernels.hip).
@@ -91,6 +95,16 @@ hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t 
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
 // 256-thread blocks of a kernel that fit one CU at once (its registers decide)
 hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu);
+// the packed kernel's instantiations, built in parts (sw_fast_<G>_<P0>.hip for
+// P0 .. P0 + 3); launch_fast and fast_kernel_occupancy dispatch to them
+hipError_t launch_fast_16_1(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_16_5(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_64_1(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_64_5(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t occupancy_fast_16_1(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_16_5(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_64_1(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_64_5(int P, int* blocks_per_cu);
 hipError_t band_kernel_occupancy(int P, int* blocks_per_cu);  // P as launch_band
 
 }  // namespace mhmsw
