@@ -355,6 +355,13 @@ def klign_call_leg(calls=40, npairs=20000):
         raise RuntimeError(f"klign_call_bench failed ({r.returncode}): {r.stderr[-2000:]}")
     out = json.loads(r.stdout.strip().splitlines()[-1])
     out.pop("checksum", None)
+    r = subprocess.run([prog, "--calls", str(calls), "--npairs", str(npairs), "--same-geometry", "1"],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"klign_call_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    same = json.loads(r.stdout.strip().splitlines()[-1])
+    same.pop("checksum", None)
+    out["equal_geometry"] = same
     batches = [m.generate(1, npairs=npairs, seed=101 + k) for k in range(4)]
     al = m.Aligner(device=0, scoring=m.MHM2_SCORING, read_is_ssw_ref=True)
     for b in batches:  # every buffer at its size, the worker pool started

@@ -87,6 +87,10 @@ typedef struct {
                                about 10x slower (16*(match+gap_open) > 255 or gap_open below mismatch
                                or ambiguity) */
   int64_t device_bytes;     /* device memory the context holds (its buffers only grow) */
+  /* host time of the batch's staging and launch (steady clock): packing to
+   * 2-bit codes, laying out the pinned staging and enqueueing its copies, and
+   * enqueueing the job */
+  double host_pack_ms, host_upload_ms, host_launch_ms;
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,

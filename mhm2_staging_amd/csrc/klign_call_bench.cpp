@@ -4,9 +4,14 @@
 // kernel_block -> run_kernel_backwards -> kernel_block -> get_aln_results.
 //
 // usage: klign_call_bench [--calls N] [--npairs P] [--batches B] [--warmup W]
+//                         [--same-geometry 0|1]
 // The batches are the seeded plain workload (150-base reads against 350-base
 // windows, 1% substitutions, 0.1% indels; seeds 101, 102, ...), cycled so that
-// every call hands the driver a batch other than the previous one.  Prints one
+// every call hands the driver a batch other than the previous one.  Batch k
+// holds P - k pairs, so that no two batches have the same geometry and every
+// call launches its job afresh, as klign's blocks (of varying lengths) do;
+// with --same-geometry 1 they all hold P pairs of equal lengths, and the
+// driver replays the job it captured as a HIP graph.  Prints one
 // JSON line: the median wall time per call and its parts, and the rate in
 // GCUPS (forward cells per second of wall time).
 #include <algorithm>
@@ -41,7 +46,7 @@ struct Batch {
 }  // namespace
 
 int main(int argc, char** argv) {
-  int calls = 40, batches = 4, warmup = 8;
+  int calls = 40, batches = 4, warmup = 8, same = 0;
   long long npairs = 20000;
   for (int i = 1; i + 1 < argc; i += 2) {
     if (!std::strcmp(argv[i], "--calls")) {
@@ -52,19 +57,21 @@ int main(int argc, char** argv) {
       batches = std::atoi(argv[i + 1]);
     } else if (!std::strcmp(argv[i], "--warmup")) {
       warmup = std::atoi(argv[i + 1]);
+    } else if (!std::strcmp(argv[i], "--same-geometry")) {
+      same = std::atoi(argv[i + 1]);
     } else {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
       return 2;
     }
   }
-  if (calls < 1 || batches < 1 || npairs < 1 || warmup < 0) {
+  if (calls < 1 || batches < 1 || npairs < batches || warmup < 0) {
     std::fprintf(stderr, "bad arguments\n");
     return 2;
   }
   std::vector<Batch> bs((size_t)batches);
   for (int k = 0; k < batches; k++) {
     mhmsw_synth_batch sb;
-    if (mhmsw_synth_generate(1, npairs, (uint64_t)(101 + k), 0, &sb) != 0) {
+    if (mhmsw_synth_generate(1, same ? npairs : npairs - k, (uint64_t)(101 + k), 0, &sb) != 0) {
       std::fprintf(stderr, "generator failed\n");
       return 1;
     }
@@ -120,11 +127,12 @@ int main(int argc, char** argv) {
     for (const Batch& b : bs) cells += b.cells;
     cells /= batches;
     const double w = median(wall);
-    std::printf("{\"workload\": \"%lld pairs per call, 150 bp reads vs 350 bp windows, a new batch every call, "
+    std::printf("{\"workload\": \"%lld pairs per call, 150 bp reads vs 350 bp windows, a new batch every call%s, "
                 "std::string in and short results out through adept_sw::GPUDriver\", \"calls\": %d, "
                 "\"ms_per_call\": %.4f, \"value\": %.1f, \"unit\": \"GCUPS\", \"run_kernel_forwards_ms\": %.4f, "
                 "\"first_kernel_block_ms\": %.4f, \"backwards_and_results_ms\": %.4f, \"checksum\": %lld}\n",
-                npairs, calls, w, cells / (w * 1e-3) / 1e9, median(fwd), median(blk1), median(rest), checksum);
+                npairs, same ? " (equal geometry: graph replays)" : "", calls, w, cells / (w * 1e-3) / 1e9,
+                median(fwd), median(blk1), median(rest), checksum);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s", e.what());
     return 1;

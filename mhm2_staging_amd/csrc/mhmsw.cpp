@@ -8,6 +8,7 @@
 // unpack -> forward pass -> 32-bit rerun of flagged pairs -> reverse pass ->
 // finalize, with no host round trip between the passes (the reference needs
 // one to size its reverse launch, driver.cpp:395).
+#include <chrono>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -155,6 +156,8 @@ struct mhmsw_ctx {
   // ev_done (serving slots), so taking them needs no device copy
   bool tail_results = false, last_job_tail_results = false;
   bool job_host_results = false;  // this job only (mhmsw_submit*): as tail_results
+  // host times of the staged batch and of the job's launch (reported with its stats)
+  double host_pack_ms = 0, host_upload_ms = 0, host_launch_ms = 0;
   HostPinned h_results;
 };
 
@@ -489,12 +492,15 @@ struct PackedBatch {
   bool used[kNumClasses] = {};
   int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
   int64_t nfast = 0, nplain = 0;
-  std::vector<int64_t> base;
-  std::vector<int32_t> len;
+  std::vector<int64_t, UninitAlloc<int64_t>> base;  // [2 * npairs] first base of each sequence
+  std::vector<int32_t, UninitAlloc<int32_t>> len;   // [2 * npairs]
   std::vector<uint32_t, UninitAlloc<uint32_t>> two, nm;  // every dword is written by pack_batch
-  // the packed codes: in two/nm, or (packed in place) in the context's pinned staging
+  // the packed codes and tables: in two/nm/base/len, or (packed in place) in
+  // the context's pinned staging
   uint32_t* two_p = nullptr;
   uint32_t* nm_p = nullptr;
+  int64_t* base_p = nullptr;
+  int32_t* len_p = nullptr;
   size_t two_n = 0, nm_n = 0;  // dwords
   std::vector<int32_t> win_ctg, win_off;
   const mhmsw_contigs* res = nullptr;
@@ -508,10 +514,43 @@ struct PackedBatch {
 
 using pack2::pack_seq;
 
-// Where pass 2 of pack_batch writes the 2-bit codes and N bits (two_n and
-// nm_n dwords): false when it cannot provide them.
-using PackDest = std::function<bool(size_t two_n, size_t nm_n, uint32_t** two, uint32_t** nm)>;
+// Where pack_batch writes the 2-bit codes and N bits (two_n and nm_n dwords)
+// and the per-sequence base and length tables (2 * npairs each): false when
+// it cannot provide them.
+using PackDest = std::function<bool(size_t two_n, size_t nm_n, uint32_t** two, uint32_t** nm, int64_t** base,
+                                    int32_t** len)>;
 
+// The first of pair p's length and layout checks that fails, as a message in
+// m (m[0] == 0: the pair is valid).
+static void pair_error(const PackParams& pp, const StageInput& in, int64_t p, char* m, size_t mn) {
+  const bool windowed = in.res != nullptr;
+  const int32_t rl = in.read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
+  m[0] = 0;
+  if (rl < 0 || cl < 0) {
+    snprintf(m, mn, "negative length at pair %lld", (long long)p);
+  } else if ((in.read_ptr && rl > 0 && !in.read_ptr[p]) || (!windowed && in.ctg_ptr && cl > 0 && !in.ctg_ptr[p])) {
+    snprintf(m, mn, "pair %lld: null sequence pointer", (long long)p);
+  } else if (windowed && (in.ctg_id[p] < 0 || in.ctg_id[p] >= in.res->n || in.win_off[p] < 0 ||
+                          (int64_t)in.win_off[p] + cl > in.res->len[(size_t)in.ctg_id[p]])) {
+    snprintf(m, mn, "pair %lld: window outside its resident contig", (long long)p);
+  } else if (std::min(rl, cl) > kPlainMaxCols) {
+    snprintf(m, mn, "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl, cl);
+  } else if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20)) {
+    snprintf(m, mn, "pair %lld: sequence longer than 2^20 bases", (long long)p);
+  } else if (pp.score2 && (pp.read_is_ref ? cl : rl) + kMaxQueryPad > kPlainMaxCols) {
+    snprintf(m, mn, "pair %lld: score2 needs SSW-query sequences of at most %d bases", (long long)p,
+             kPlainMaxCols - kMaxQueryPad);
+  }
+}
+
+// Packs a batch: validates every pair, lays out its sequences (32-base
+// blocks from kPadBases on; string batches interleave each pair's read and
+// window, windowed batches put every read first and every window after them,
+// and only the reads are packed on the host: the device extracts the windows
+// from the resident contigs), routes each pair to its forward work class and
+// packs the sequences to 2-bit codes plus N bits.  The layout is summed on the
+// calling thread (one pass over the lengths); validation, routing and packing
+// run in one parallel region over contiguous chunks of pairs.
 static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& pb, int max_threads,
                       const PackDest* dest = nullptr) {
   const int64_t npairs = in.npairs;
@@ -531,18 +570,10 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
   pb = PackedBatch();
   pb.npairs = npairs;
   pb.res = in.res;
-  pb.base.resize((size_t)std::max<int64_t>(nseq, 1));
-  pb.len.resize((size_t)std::max<int64_t>(nseq, 1));
   int nth = std::max(1, std::min(max_threads, (int)std::max(1u, std::thread::hardware_concurrency())));
   if (npairs < 4096) nth = 1;
-  auto run = [&](auto&& fn) { hostpool::pool().run(nth, std::function<void(int)>(fn)); };
-  // pass 1, over contiguous chunks of pairs: validate, lay out each chunk from
-  // offset 0, route, and sum
-  // String batches interleave each pair's read and window.  Windowed batches
-  // put every read first and every window after them: only the reads region
-  // is packed on the host and copied, the windows are extracted on the device.
   struct Chunk {
-    int64_t p0 = 0, p1 = 0, bases = 0, wbases = 0, cells = 0, nfast = 0, nplain = 0;
+    int64_t p0 = 0, p1 = 0, cells = 0, nfast = 0, nplain = 0;
     int max_plain_cols = 0, max_rl = 0, max_cl = 0;
     bool used[kNumClasses] = {};
     int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
@@ -555,46 +586,102 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
     ch[(size_t)k].p0 = npairs * k / nchunks;
     ch[(size_t)k].p1 = npairs * (k + 1) / nchunks;
   }
+  // layout: each chunk's first base (reads region, and the windows region of
+  // a windowed batch); lengths outside 0..2^20 count as 0 here and fail the
+  // batch below
+  auto span = [](int32_t len) -> int64_t {
+    return (uint32_t)len < (1u << 20) ? ((int64_t)len + kSeqAlign - 1) / kSeqAlign * kSeqAlign : 0;
+  };
+  const int32_t* clen = windowed ? in.win_len : in.ctg_len;
+  std::vector<int64_t> chunk_base((size_t)nchunks), chunk_wbase((size_t)nchunks);
+  bool suspect = false;
+  int64_t acc = kPadBases;
+  for (int k = 0; k < nchunks; k++) {
+    chunk_base[(size_t)k] = acc;
+    for (int64_t p = ch[(size_t)k].p0; p < ch[(size_t)k].p1; p++) {
+      const int32_t rl = in.read_len[p], cl = clen[p];
+      suspect |= (uint32_t)rl >= (1u << 20) || (uint32_t)cl >= (1u << 20);
+      acc += span(rl) + (windowed ? 0 : span(cl));
+    }
+  }
+  const int64_t reads_end = acc;  // windowed: the packed region ends here
+  if (windowed) {
+    for (int k = 0; k < nchunks; k++) {
+      chunk_wbase[(size_t)k] = acc;
+      for (int64_t p = ch[(size_t)k].p0; p < ch[(size_t)k].p1; p++) acc += span(clen[p]);
+    }
+  }
+  const int64_t body_end = acc;
+  pb.total = (acc + kPadBases + 31) / 32 * 32;
+  if (suspect || pb.total >= (int64_t)1 << 33) {
+    // the first invalid pair fails the batch, as the parallel pass reports it
+    char m[256];
+    for (int64_t p = 0; p < npairs; p++) {
+      pair_error(pp, in, p, m, sizeof(m));
+      if (m[0]) {
+        pb.err = m;
+        return MHMSW_E_ARG;
+      }
+    }
+    if (pb.total >= (int64_t)1 << 33) {  // the band kernels index code dwords with 32 bits
+      pb.err = "batch too large (2^33 bases or more with padding)";
+      return MHMSW_E_ARG;
+    }
+  }
+  const int64_t packed_end = windowed ? reads_end : pb.total;
+  pb.two_n = (size_t)(packed_end / 16);
+  pb.nm_n = (size_t)(packed_end / 32);
+  if (dest) {
+    if (!(*dest)(pb.two_n, pb.nm_n, &pb.two_p, &pb.nm_p, &pb.base_p, &pb.len_p)) {
+      pb.err = "cannot allocate the pinned staging buffer";
+      return MHMSW_E_NOMEM;
+    }
+  } else {
+    pb.two.resize(pb.two_n);
+    pb.nm.resize(pb.nm_n);
+    pb.base.resize((size_t)std::max<int64_t>(nseq, 1));
+    pb.len.resize((size_t)std::max<int64_t>(nseq, 1));
+    pb.two_p = pb.two.data();
+    pb.nm_p = pb.nm.data();
+    pb.base_p = pb.base.data();
+    pb.len_p = pb.len.data();
+  }
+  // pads before the first and after the last packed sequence
+  std::fill(pb.two_p, pb.two_p + kPadBases / 16, 0u);
+  std::fill(pb.nm_p, pb.nm_p + kPadBases / 32, 0u);
+  if (!windowed) {
+    std::fill(pb.two_p + body_end / 16, pb.two_p + pb.two_n, 0u);
+    std::fill(pb.nm_p + body_end / 32, pb.nm_p + pb.nm_n, 0u);
+  }
+  // one parallel region: validate, place, route and pack each chunk's pairs
   std::atomic<int> next_chunk(0);
-  run([&](int) {
+  auto region = [&](int) {
     char m[256];
     for (int k; (k = next_chunk.fetch_add(1)) < nchunks;) {
       Chunk& c = ch[(size_t)k];
-      int64_t acc = 0, accw = 0;
+      int64_t at = chunk_base[(size_t)k], wat = windowed ? chunk_wbase[(size_t)k] : 0;
       for (int64_t p = c.p0; p < c.p1; p++) {
-        const int32_t rl = in.read_len[p], cl = windowed ? in.win_len[p] : in.ctg_len[p];
-        m[0] = 0;
-        if (rl < 0 || cl < 0) {
-          snprintf(m, sizeof(m), "negative length at pair %lld", (long long)p);
-        } else if ((in.read_ptr && rl > 0 && !in.read_ptr[p]) || (!windowed && in.ctg_ptr && cl > 0 && !in.ctg_ptr[p])) {
-          snprintf(m, sizeof(m), "pair %lld: null sequence pointer", (long long)p);
-        } else if (windowed && (in.ctg_id[p] < 0 || in.ctg_id[p] >= in.res->n || in.win_off[p] < 0 ||
-                                (int64_t)in.win_off[p] + cl > in.res->len[(size_t)in.ctg_id[p]])) {
-          snprintf(m, sizeof(m), "pair %lld: window outside its resident contig", (long long)p);
-        } else if (std::min(rl, cl) > kPlainMaxCols) {
-          snprintf(m, sizeof(m), "pair %lld: both sequences exceed %d bases (%d, %d)", (long long)p, kPlainMaxCols, rl,
-                   cl);
-        } else if ((int64_t)rl >= (1 << 20) || (int64_t)cl >= (1 << 20)) {
-          snprintf(m, sizeof(m), "pair %lld: sequence longer than 2^20 bases", (long long)p);
-        } else if (pp.score2 && (pp.read_is_ref ? cl : rl) + kMaxQueryPad > kPlainMaxCols) {
-          snprintf(m, sizeof(m), "pair %lld: score2 needs SSW-query sequences of at most %d bases", (long long)p,
-                   kPlainMaxCols - kMaxQueryPad);
-        }
+        pair_error(pp, in, p, m, sizeof(m));
         if (m[0]) {
           c.bad = p;
           c.err = m;
           break;
         }
-        pb.len[(size_t)(2 * p)] = rl;
-        pb.len[(size_t)(2 * p + 1)] = cl;
-        pb.base[(size_t)(2 * p)] = acc;
-        acc += (rl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+        const int32_t rl = in.read_len[p], cl = clen[p];
+        pb.len_p[2 * p] = rl;
+        pb.len_p[2 * p + 1] = cl;
+        // pack_seq writes every dword of a sequence's 32-base blocks
+        static_assert(kSeqAlign == 32, "sequences start on 32-base blocks");
+        pb.base_p[2 * p] = at;
+        pack_seq(in.read_ptr ? in.read_ptr[p] : in.reads + in.read_off[p], rl, pb.two_p + at / 16, pb.nm_p + at / 32);
+        at += span(rl);
         if (windowed) {
-          pb.base[(size_t)(2 * p + 1)] = accw;
-          accw += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+          pb.base_p[2 * p + 1] = wat;
+          wat += span(cl);
         } else {
-          pb.base[(size_t)(2 * p + 1)] = acc;
-          acc += (cl + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
+          pb.base_p[2 * p + 1] = at;
+          pack_seq(in.ctg_ptr ? in.ctg_ptr[p] : in.ctgs + in.ctg_off[p], cl, pb.two_p + at / 16, pb.nm_p + at / 32);
+          at += span(cl);
         }
         c.cells += (int64_t)rl * cl;
         c.max_plain_cols = std::max(c.max_plain_cols, (int)std::min(rl, cl));
@@ -613,21 +700,16 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
           }
         }
       }
-      c.bases = acc;
-      c.wbases = accw;
     }
-  });
-  // the first invalid pair fails the batch; otherwise chunk offsets and totals
-  int64_t acc = kPadBases;
-  std::vector<int64_t> chunk_base((size_t)nchunks), chunk_wbase((size_t)nchunks);
+  };
+  hostpool::pool().run_claimed(nth, std::function<void(int)>(region));
+  // the first invalid pair fails the batch; otherwise the totals
   for (int k = 0; k < nchunks; k++) {
     const Chunk& c = ch[(size_t)k];
     if (c.bad >= 0) {
       pb.err = c.err;
       return MHMSW_E_ARG;
     }
-    chunk_base[(size_t)k] = acc;
-    acc += c.bases;
     pb.cells += c.cells;
     pb.max_plain_cols = std::max(pb.max_plain_cols, c.max_plain_cols);
     pb.max_read_len = std::max(pb.max_read_len, c.max_rl);
@@ -640,63 +722,6 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
     pb.nfast += c.nfast;
     pb.nplain += c.nplain;
   }
-  const int64_t reads_end = acc;  // windowed: the packed region ends here
-  for (int k = 0; k < nchunks; k++) {
-    chunk_wbase[(size_t)k] = acc;
-    acc += ch[(size_t)k].wbases;
-  }
-  const int64_t body_end = acc;
-  acc += kPadBases;
-  pb.total = (acc + 31) / 32 * 32;
-  if (pb.total >= (int64_t)1 << 33) {  // the band kernels index code dwords with 32 bits
-    pb.err = "batch too large (2^33 bases or more with padding)";
-    return MHMSW_E_ARG;
-  }
-  const int64_t packed_end = windowed ? reads_end : pb.total;
-  pb.two_n = (size_t)(packed_end / 16);
-  pb.nm_n = (size_t)(packed_end / 32);
-  if (dest) {
-    if (!(*dest)(pb.two_n, pb.nm_n, &pb.two_p, &pb.nm_p)) {
-      pb.err = "cannot allocate the pinned staging buffer";
-      return MHMSW_E_NOMEM;
-    }
-  } else {
-    pb.two.resize(pb.two_n);
-    pb.nm.resize(pb.nm_n);
-    pb.two_p = pb.two.data();
-    pb.nm_p = pb.nm.data();
-  }
-  // pads before the first and after the last packed sequence
-  std::fill(pb.two_p, pb.two_p + kPadBases / 16, 0u);
-  std::fill(pb.nm_p, pb.nm_p + kPadBases / 32, 0u);
-  if (!windowed) {
-    std::fill(pb.two_p + body_end / 16, pb.two_p + pb.two_n, 0u);
-    std::fill(pb.nm_p + body_end / 32, pb.nm_p + pb.nm_n, 0u);
-  }
-  // pass 2, per chunk: final offsets, then zero each packed sequence's
-  // 32-base-rounded span and pack it (the windows of a windowed batch are not
-  // packed: the device extracts them from the resident contigs)
-  next_chunk = 0;
-  run([&](int) {
-    for (int k; (k = next_chunk.fetch_add(1)) < nchunks;) {
-      const Chunk& c = ch[(size_t)k];
-      const int64_t off = chunk_base[(size_t)k], woff = windowed ? chunk_wbase[(size_t)k] : off;
-      for (int64_t s = 2 * c.p0; s < 2 * c.p1; s++) {
-        const int64_t b0 = (pb.base[(size_t)s] += (s & 1) ? woff : off);
-        if ((s & 1) && windowed) continue;  // extracted on the device
-        const int32_t len = pb.len[(size_t)s];
-        const int64_t span = ((int64_t)len + kSeqAlign - 1) / kSeqAlign * kSeqAlign;
-        uint32_t* two = pb.two_p + b0 / 16;
-        uint32_t* nb = pb.nm_p + b0 / 32;
-        std::fill(two, two + span / 16, 0u);
-        std::fill(nb, nb + span / 32, 0u);
-        const int64_t q = s >> 1;
-        const char* src = (s & 1) ? (in.ctg_ptr ? in.ctg_ptr[q] : in.ctgs + in.ctg_off[q])
-                                  : (in.read_ptr ? in.read_ptr[q] : in.reads + in.read_off[q]);
-        pack_seq(src, len, two, nb);
-      }
-    }
-  });
   if (windowed) {
     pb.win_ctg.assign(in.ctg_id, in.ctg_id + npairs);
     pb.win_off.assign(in.win_off, in.win_off + npairs);
@@ -928,11 +953,11 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
     if (pb.nm_n && pb.nm_p != reinterpret_cast<uint32_t*>(hs + at_nm[k])) copy(hs + at_nm[k], pb.nm_p, pb.nm_n * 4);
     const size_t nq = (size_t)(2 * pb.npairs);
     if (sh == 0) {
-      if (nq) copy(sbase + 2 * ps, pb.base.data(), nq * 8);
+      if (nq && pb.base_p != sbase + 2 * ps) copy(sbase + 2 * ps, pb.base_p, nq * 8);
     } else {
-      for (size_t q = 0; q < nq; q++) sbase[2 * ps + (int64_t)q] = pb.base[q] + sh;
+      for (size_t q = 0; q < nq; q++) sbase[2 * ps + (int64_t)q] = pb.base_p[q] + sh;
     }
-    if (nq) copy(slen + 2 * ps, pb.len.data(), nq * 4);
+    if (nq && pb.len_p != slen + 2 * ps) copy(slen + 2 * ps, pb.len_p, nq * 4);
     if (windows && pb.npairs > 0) {
       std::memcpy(swc + ps, pb.win_ctg.data(), (size_t)pb.npairs * 4);
       std::memcpy(swo + ps, pb.win_off.data(), (size_t)pb.npairs * 4);
@@ -1003,17 +1028,25 @@ static int stage_impl(mhmsw_ctx* c, const StageInput& in) {
   HIP_OR_FAIL(c, hipSetDevice(c->device));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   const bool windows = in.res != nullptr && in.npairs > 0;
-  const PackDest dest = [&](size_t two_n, size_t nm_n, uint32_t** two, uint32_t** nm) {
+  const PackDest dest = [&](size_t two_n, size_t nm_n, uint32_t** two, uint32_t** nm, int64_t** base,
+                            int32_t** len) {
     const StageLayout L = stage_layout(&two_n, &nm_n, 1, in.npairs, windows);
     if (c->h_stage.ensure(L.bytes) != hipSuccess) return false;
     *two = reinterpret_cast<uint32_t*>(c->h_stage.ptr + L.at_two[0]);
     *nm = reinterpret_cast<uint32_t*>(c->h_stage.ptr + L.at_nm[0]);
+    *base = reinterpret_cast<int64_t*>(c->h_stage.ptr + L.at_base);
+    *len = reinterpret_cast<int32_t*>(c->h_stage.ptr + L.at_len);
     return true;
   };
   PackedBatch pb;
+  const auto t0 = std::chrono::steady_clock::now();
   int rc = pack_batch(pack_params(c), in, pb, 16, &dest);
   if (rc != MHMSW_OK) return fail(c, rc, "%s", pb.err.c_str());
-  return upload_batch(c, pb);
+  const auto t1 = std::chrono::steady_clock::now();
+  rc = upload_batch(c, pb);
+  c->host_pack_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  c->host_upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+  return rc;
 }
 
 extern "C" {
@@ -1316,6 +1349,8 @@ static void drop_graph(mhmsw_ctx* c) {
   c->eager_key.clear();
 }
 
+static int align_staged_impl(mhmsw_ctx* c);
+
 int mhmsw_align_staged(mhmsw_ctx* c) {
   if (!c) return MHMSW_E_ARG;
   if (!c->staged) return fail(c, MHMSW_E_STATE, "no staged batch");
@@ -1323,6 +1358,13 @@ int mhmsw_align_staged(mhmsw_ctx* c) {
     int rc = mhmsw_wait(c);
     if (rc != MHMSW_OK) return rc;
   }
+  const auto t0 = std::chrono::steady_clock::now();
+  const int rc = align_staged_impl(c);
+  c->host_launch_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+static int align_staged_impl(mhmsw_ctx* c) {
   if (c->score2 && plain_lds_cols(c) > kPlainMaxCols)
     return fail(c, MHMSW_E_ARG, "score2 needs SSW-query sequences of at most %d bases (longest: %d)",
                 kPlainMaxCols - kMaxQueryPad, plain_lds_cols(c) - kMaxQueryPad);
@@ -1524,6 +1566,9 @@ static int complete(mhmsw_ctx* c) {
   s.cigar_ms = ms[6];
   s.job_ms = job;
   s.cigar_reruns = 0;
+  s.host_pack_ms = c->host_pack_ms;
+  s.host_upload_ms = c->host_upload_ms;
+  s.host_launch_ms = c->host_launch_ms;
   if (c->last_job_cigar) {
     int rc = settle_cigar(c);
     if (rc != MHMSW_OK) return rc;

@@ -19,7 +19,26 @@ class Pool {
   // fn(t) for t in [0, nth) on nth threads (the caller runs t = 0), returning
   // when all are done.  One region at a time: a caller that finds the pool
   // busy runs its region on threads of its own.
-  void run(int nth, const std::function<void(int)>& fn) {
+  void run(int nth, const std::function<void(int)>& fn) { region(nth, fn, false); }
+
+  // For regions that share their work out through a counter of their own:
+  // fn(t) runs on the caller (t = 0) and on each worker t < nth that wakes
+  // before the caller's fn(0) has returned, i.e. while work may be left.
+  // The call returns once those have finished; a worker that wakes later
+  // skips the region, so a slow wake-up never holds the caller.
+  void run_claimed(int nth, const std::function<void(int)>& fn) { region(nth, fn, true); }
+
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+
+ private:
+  void region(int nth, const std::function<void(int)>& fn, bool claimed) {
     nth = std::max(1, std::min(nth, kMaxThreads));
     if (nth == 1) {
       fn(0);
@@ -41,26 +60,20 @@ class Pool {
       }
       fn_ = &fn;
       active_ = nth;
-      pending_ = nth - 1;
+      open_ = true;
+      running_ = 0;
+      pending_ = claimed ? 0 : nth - 1;
       generation_++;
     }
     cv_work_.notify_all();
     fn(0);
     std::unique_lock<std::mutex> lk(mu_);
-    cv_done_.wait(lk, [&] { return pending_ == 0; });
+    if (claimed) open_ = false;  // workers not yet awake skip the region
+    cv_done_.wait(lk, [&] { return pending_ == 0 && running_ == 0; });
+    open_ = false;
     fn_ = nullptr;
   }
 
-  ~Pool() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
-    }
-    cv_work_.notify_all();
-    for (auto& t : workers_) t.join();
-  }
-
- private:
   void worker(int id) {
     uint64_t seen = 0;
     std::unique_lock<std::mutex> lk(mu_);
@@ -68,12 +81,16 @@ class Pool {
       cv_work_.wait(lk, [&] { return stop_ || generation_ != seen; });
       if (stop_) return;
       seen = generation_;
-      if (id >= active_) continue;  // not part of this region
+      if (id >= active_ || !open_) continue;  // not part of this region, or a claimed region already closed
+      const bool counted = pending_ > 0;       // a region where every index runs
+      running_++;
       const std::function<void(int)>* fn = fn_;
       lk.unlock();
       (*fn)(id);
       lk.lock();
-      if (--pending_ == 0) cv_done_.notify_one();
+      running_--;
+      if (counted) pending_--;
+      if (pending_ == 0 && running_ == 0) cv_done_.notify_one();
     }
   }
 
@@ -82,7 +99,8 @@ class Pool {
   std::condition_variable cv_work_, cv_done_;
   std::vector<std::thread> workers_;
   const std::function<void(int)>* fn_ = nullptr;
-  int active_ = 0, pending_ = 0;
+  int active_ = 0, pending_ = 0, running_ = 0;
+  bool open_ = false;
   uint64_t generation_ = 0;
   bool stop_ = false;
 };

@@ -1,6 +1,8 @@
-// The SSSE3 2-bit packer against the scalar per-base packer (SSW's translation
-// table, src/ssw/ssw.cpp:13-26): random bytes of every value, ACGTU/acgtu text
-// with N, every length 0..300.  Prints "ok <cases> <bases/s simd> <bases/s scalar>".
+// The SSSE3 and AVX-512 2-bit packers against the scalar per-base packer (SSW's
+// translation table, src/ssw/ssw.cpp:13-26): random bytes of every value,
+// ACGTU/acgtu text with N, every length 0..300, and the same dwords written
+// (the buffers start as a fill pattern).  The AVX-512 form is checked where the
+// CPU has it.  Prints "ok <cases> <bases/s pack_seq> <bases/s scalar> <avx512 0|1>".
 #include <chrono>
 #include <cstdio>
 #include <random>
@@ -20,6 +22,8 @@ static void pack_ref(const std::string& s, std::vector<uint32_t>& two, std::vect
 }
 
 int main() {
+  const bool avx512 = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+                      __builtin_cpu_supports("avx512vl");
   std::mt19937_64 rng(7);
   const char alpha[] = "ACGTUacgtuNnRY-*.\\0";
   long cases = 0;
@@ -34,12 +38,17 @@ int main() {
         }
         std::vector<uint32_t> t1, n1, t2, n2;
         pack_ref(s, t1, n1);
-        t2.assign(t1.size(), 0xA5A5A5A5u);
-        n2.assign(n1.size(), 0xA5A5A5A5u);
-        pack_seq(s.data(), len, t2.data(), n2.data());
-        if (t1 != t2 || n1 != n2) {
-          std::printf("MISMATCH kind %d len %d rep %d\n", kind, len, rep);
-          return 1;
+        for (int form = 0; form < (avx512 ? 3 : 2); form++) {
+          t2.assign(t1.size(), 0xA5A5A5A5u);
+          n2.assign(n1.size(), 0xA5A5A5A5u);
+          const unsigned char* u = (const unsigned char*)s.data();
+          if (form == 0) pack_seq(s.data(), len, t2.data(), n2.data());
+          if (form == 1) pack_seq_ssse3(u, len, t2.data(), n2.data());
+          if (form == 2) pack_seq_avx512(u, len, t2.data(), n2.data());
+          if (t1 != t2 || n1 != n2) {
+            std::printf("MISMATCH form %d kind %d len %d rep %d\n", form, kind, len, rep);
+            return 1;
+          }
         }
         cases++;
       }
@@ -56,6 +65,6 @@ int main() {
   for (int32_t x0 = 0; x0 < (int32_t)big.size(); x0 += 32)
     pack_scalar((const unsigned char*)big.data(), x0, 32, two.data(), nb.data());
   double tr = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  std::printf("ok %ld %.3g %.3g\n", cases, big.size() / ts, big.size() / tr);
+  std::printf("ok %ld %.3g %.3g %d\n", cases, big.size() / ts, big.size() / tr, avx512 ? 1 : 0);
   return 0;
 }

@@ -147,8 +147,9 @@ def test_serving_entry_points_fail_cleanly_on_cpu():
 
 
 def test_host_packer_simd_matches_scalar():
-    """The SSSE3 2-bit packer used for staging and resident contigs equals the
-    per-base table packer on every byte value and every length up to 300."""
+    """The SSSE3 and AVX-512 2-bit packers used for staging and resident contigs
+    write what the per-base table packer writes, on every byte value and every
+    length up to 300 (the AVX-512 form where the CPU has it)."""
     exe = os.path.join(ROOT, "tests", "cpp", "pack_test")
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/pack_test is not built (run __graft_entry__.build())")
