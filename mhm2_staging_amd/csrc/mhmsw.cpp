@@ -88,7 +88,7 @@ struct mhmsw_ctx {
   // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
   // (all-class kernel, then P = 1, 2, 4, 8)
   int occ_fast[2][kMaxP] = {};
-  int occ_band[5] = {};
+  int occ_band[6] = {};  // all, P = 1, 2, 4, 8, the diagonal class
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
   // staged batch
@@ -302,8 +302,9 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
   for (int g = 0; g < 2; g++)
     for (int P = 1; P <= kMaxP; P++)
       if (fast_kernel_occupancy(g ? 64 : 16, P, &c->occ_fast[g][P - 1]) != hipSuccess) c->occ_fast[g][P - 1] = 0;
-  for (int i = 0; i < 5; i++)
-    if (band_kernel_occupancy(i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess) c->occ_band[i] = 0;
+  for (int i = 0; i < 6; i++)
+    if (band_kernel_occupancy(i == 5 ? kBandDiag : i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess)
+      c->occ_band[i] = 0;
   compute_score_params(c);
   // the no-allocation entry point's scratch on this device (a failure here
   // is reported again, with its message, by that entry point's first call)
@@ -1109,10 +1110,18 @@ static int plain_lds_cols(const mhmsw_ctx* c) {
 // taller than its forward one).  An empty persistent launch costs about 4.5 us,
 // a few times what running these jobs' few rectangles at the larger P costs.
 constexpr int64_t kBandAllMaxPairs = 65536;
+// A small job's widest band task: 16 diagonals (P <= 4).  Measured on 10k,
+// 20k and 50k pairs of 150 x 350 bases, reverse pass at 1, 3, 7 and 15:
+// 0.150 / 0.134 / 0.113 / 0.143 ms, 0.198 / 0.164 / 0.116 / 0.151 ms and
+// 0.348 / 0.275 / 0.157 / 0.158 ms.
+constexpr int32_t kSmallJobBandMaxD = 7;
+// blocks per CU of the diagonal class's launch in large jobs
+constexpr int kDiagBlocksPerCu = 8;
 
 static BatchView make_view(mhmsw_ctx* c) {
   BatchView b;
   std::memset(&b, 0, sizeof(b));  // the graph cache compares views bytewise
+  b.band_max_d = kBandMaxD;
   b.npairs = c->npairs;
   b.total = c->total;
   b.seq_base = c->d_base.ptr;
@@ -1143,6 +1152,22 @@ static BatchView make_view(mhmsw_ctx* c) {
     }
     b.rev_one16 = maxp64 > 0 ? kMaxP : maxp16;
     b.rev_one64 = maxp64;
+    // A small job's band pass is latency-bound (a wavefront or two per SIMD):
+    // its time is the longest lane's chain, 2P cell updates per u-step.
+    // Bands wider than 7 diagonals each side run as rectangles in 16-lane
+    // groups, whose chain per column is P cell updates for up to 32P rows.
+    b.band_max_d = kSmallJobBandMaxD;
+  } else if (c->mode == MHMSW_MODE_FAST) {
+    // wide-band rectangles probed first; the few full rectangles of pass
+    // kPassRev2 run in the largest class of each group size
+    int maxp16 = 0, maxp64 = 0;
+    for (int P = 1; P <= kMaxP; P++) {
+      if (c->fast_class_used[kClassFast16 + P - 1]) maxp16 = P;
+      if (c->fast_class_used[kClassFast64 + P - 1]) maxp64 = P;
+    }
+    b.rev_probe = 1;
+    b.rev2_one16 = maxp64 > 0 ? kMaxP : maxp16;
+    b.rev2_one64 = maxp64;
   }
   if (c->res) {
     b.res2 = c->res->d2;
@@ -1326,9 +1351,25 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       } else {
         for (int i = 4; i >= 1; i--)
           HIP_OR_FAIL(c, launch_band(bv, 1 << (i - 1), band_grid(c->n_fast, c->occ_band[i]), st));
+        // the diagonal class's lanes each stream two short code runs and do
+        // little arithmetic on them: more waves per CU hide their loads
+        const int diag_cu = c->occ_band[5] > 0 ? std::min(kDiagBlocksPerCu, c->occ_band[5]) : kDiagBlocksPerCu;
+        HIP_OR_FAIL(c, launch_band(bv, kBandDiag,
+                                   (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * diag_cu,
+                                                                               (c->n_fast + 255) / 256)),
+                                   st));
       }
     }
     if (rev_plain_possible) HIP_OR_FAIL(c, launch_plain(bv, plain_grid, plain_cols, st));
+    if (any_fast && b.rev_probe) {
+      // full rectangles of the wide-band pairs whose probe did not reach the score
+      const BatchView bv2 = with_pass(b, kPassRev2);
+      HIP_OR_FAIL(c, launch_schedule(b, kPassRev2, st));
+      if (b.rev2_one64 > 0)
+        HIP_OR_FAIL(c, launch_fast(bv2, 64, b.rev2_one64, grid_cap(c->n_fast, 4, c->occ_fast[1][b.rev2_one64 - 1]), st));
+      if (b.rev2_one16 > 0)
+        HIP_OR_FAIL(c, launch_fast(bv2, 16, b.rev2_one16, grid_cap(c->n_fast, 16, c->occ_fast[0][b.rev2_one16 - 1]), st));
+    }
   }
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));
   if (c->npairs > 0) HIP_OR_FAIL(c, launch_finalize(b, st));

@@ -274,6 +274,7 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
   b.g64_min_rows = ci.tuning.g64_min_rows;
   b.bin_shift = ci.tuning.bin_shift;
   b.score2 = 0;
+  b.band_max_d = kBandMaxD;
   int32_t* state = reinterpret_cast<int32_t*>(base + kOffState);
   static_assert(kStateWords * 4 <= 256, "state words fit their slot");
   static_assert(sizeof(mhmsw_result) == sizeof(AlnResult), "result layout");
@@ -321,6 +322,7 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
                "reverse band pass");
     } else {
       for (int P = 8; P >= 1; P /= 2) NA_CHECK(launch_band(bv, P, full, st), "reverse band pass");
+      NA_CHECK(launch_band(bv, kBandDiag, full, st), "reverse band pass");
     }
     NA_CHECK(launch_plain(bv, plain_grid, kPlainMaxCols, st), "reverse pass (32-bit)");
     NA_CHECK(launch_finalize(b, st), "finalize");

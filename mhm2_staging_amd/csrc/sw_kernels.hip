@@ -143,6 +143,15 @@ __device__ __forceinline__ int wave_min_i32(int v) {
 // Loop bounds and work indices must be uniform so that the wavefront never
 // splits around the persistent work loop.
 __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// A persistent wavefront's next task number from a class's work counter:
+// lane 0 alone adds 1, and the others read its old value.  (Every lane
+// calling atomicAdd with lane 0's 1 and the rest's 0 compiles to a scan over
+// the 64 lanes' values, hundreds of scalar instructions per fetch.)
+__device__ __forceinline__ int32_t next_wave_task(int32_t* counter, int lane) {
+  int32_t old = 0;
+  if (lane == 0) old = atomicAdd(counter, 1);
+  return __builtin_amdgcn_readlane(old, 0);
+}
 __device__ __forceinline__ int32_t from_lane0(int32_t v) { return __builtin_amdgcn_readlane(v, 0); }
 __device__ __forceinline__ int64_t uni64(int64_t v) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
@@ -262,7 +271,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   } else {
     if (fe.score <= 0) return false;
     if (fe.i < 0 || fe.i >= lref || fe.j < 0 || fe.j >= lqry) {
-      atomicAdd(&b.errors[2], 1);  // an end point outside the pair: never reverse from it
+      if (pass == kPassRev) atomicAdd(&b.errors[2], 1);  // an end point outside the pair: never reverse from it
       return false;
     }
     if (fe.flags & kEndWide) wide = true;
@@ -294,23 +303,50 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     // reverse tasks read the forward codes backwards: R'[k] = codes[ref_start - k]
     ref_start = b.seq_base[sref] + fe.i;
     qry_start = b.seq_base[sqry] + fe.j;
-    if (!wide && D >= 0 && D <= kBandMaxD && ref_len + D < 0xFFF0) {
+    bool band = !wide && D >= 0 && D <= b.band_max_d && ref_len + D < 0xFFF0;
+    // A large job probes a wide-band pair whose score is at most
+    // match * kProbeRows on the first kProbeRows rows of its rectangle.  The
+    // values there are the full rectangle's (a cell depends only on cells of
+    // lower rows and columns).  A cell (k, l) holding the score with
+    // k < kProbeRows ends an alignment over at most kProbeRows rows, which
+    // scores at most match * kProbeRows minus the cost of |l - k| gap
+    // positions: so |l - k| <= Dp (D's bound over kProbeRows rows), and the
+    // probe is the band |l - k| <= Dp of those rows.  When it reaches the
+    // score, its least (k, l) is the rectangle's; pass kPassRev2 runs the full
+    // rectangle of the others.  (An unrelated pair's short local alignment
+    // ends within the probe; a gapped real one scores too high to be probed.)
+    const bool probe = b.rev_probe && !wide && !band && D >= 0 && ref_len > kProbeRows &&
+                       (int64_t)b.sc.match * kProbeRows >= fe.score;
+    if (pass == kPassRev2) {
+      if (!probe || b.rev[p].score == fe.score) return false;
+      // the few probes that miss: every one in the largest reverse class
+    } else if (probe) {
+      const int64_t slack = (int64_t)b.sc.match * kProbeRows - fe.score;
+      D = slack >= b.sc.gap_open ? 1 + (slack - b.sc.gap_open) / b.sc.gap_ext : 0;
+      ref_len = kProbeRows;
+      if ((int64_t)qry_len > (int64_t)kProbeRows + D) qry_len = (int32_t)(kProbeRows + D);
+      band = D <= b.band_max_d;
+    }
+    if (pass == kPassRev && band) {
       // rows = reversed SSW-ref, columns = reversed SSW-query, one lane per pair
       t.pair = (int32_t)p;
-      t.flags = kTaskReverse | kTaskRowsAreRef | ((int32_t)D << kTaskBandShift);
+      // the forward score rides in the flags (below 2^19: packed pairs stay
+      // under kFastOverflowScore), so the band kernels need no load of it
+      t.flags = kTaskReverse | kTaskRowsAreRef | ((int32_t)D << kTaskBandShift) | (fe.score << kTaskScoreShift);
       t.row_start = ref_start;
       t.row_len = ref_len;
       t.col_start = qry_start;
       t.col_len = qry_len;
-      cls = kClassBand + (D <= 1 ? 0 : D <= 3 ? 1 : D <= 7 ? 2 : 3);  // 4P diagonals cover |l - k| <= 2P - 1
+      // 4P diagonals cover |l - k| <= 2P - 1; D = 0 is the single diagonal
+      cls = D == 0 ? kClassDiag : kClassBand + (D <= 1 ? 0 : D <= 3 ? 1 : D <= 7 ? 2 : 3);
       const int bn = (ref_len + (int32_t)D) >> b.bin_shift;  // the lane's loop runs K + D steps
       bin = bn < kNumBins ? bn : kNumBins - 1;
       return true;
     }
   }
   t.pair = (int32_t)p;
-  t.flags = pass == kPassRev ? kTaskReverse : 0;
-  if (b.score2 && pass != kPassRev) {
+  t.flags = pass >= kPassRev ? kTaskReverse : 0;
+  if (b.score2 && pass < kPassRev) {
     // score2 reads per-SSW-ref-position maxima: rows are SSW-ref, columns
     // SSW-query plus up to kMaxQueryPad zero-scoring columns
     if (!wide && ref_len <= kFast16MaxRows && ref_len < b.g64_min_rows && qry_len + kMaxQueryPad <= kFastMaxCols) {
@@ -341,9 +377,11 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   } else {
     cls = kClassPlain;
   }
-  if (pass == kPassRev) {  // a small job's reverse rectangles: one class per group size
-    if (b.rev_one16 > 0 && cls >= kClassFast16 && cls < kClassFast16 + kMaxP) cls = kClassFast16 + b.rev_one16 - 1;
-    if (b.rev_one64 > 0 && cls >= kClassFast64 && cls < kClassFast64 + kMaxP) cls = kClassFast64 + b.rev_one64 - 1;
+  if (pass >= kPassRev) {  // a small job's reverse rectangles, and pass kPassRev2's: one class per group size
+    const int one16 = pass == kPassRev2 ? b.rev2_one16 : b.rev_one16;
+    const int one64 = pass == kPassRev2 ? b.rev2_one64 : b.rev_one64;
+    if (one16 > 0 && cls >= kClassFast16 && cls < kClassFast16 + kMaxP) cls = kClassFast16 + one16 - 1;
+    if (one64 > 0 && cls >= kClassFast64 && cls < kClassFast64 + kMaxP) cls = kClassFast64 + one64 - 1;
   }
   bool rows_ref;
   if (cls == kClassPlain) {
@@ -414,11 +452,11 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
     bk[k] = cls * kNumBins + bin;
     lr[k] = atomicAdd(&hist[bk[k]], 1);
     if (pass == kPassRerun) atomicAdd(&b.errors[1], 1);
-    if (pass == kPassRev) {
+    if (pass >= kPassRev) {
       // cells computed: the whole rectangle, or for a band task its 2D+1 diagonals
       int64_t w = t.col_len;
       if (cls >= kClassBand) {
-        const int64_t bw = 2 * (int64_t)(t.flags >> kTaskBandShift) + 1;
+        const int64_t bw = 2 * (int64_t)((t.flags >> kTaskBandShift) & 0xF) + 1;
         w = w < bw ? w : bw;
       }
       my_cells += (unsigned long long)t.row_len * (unsigned long long)w;
@@ -925,14 +963,12 @@ __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(
   // counter (the rest still take every task: each loops until it runs out)
   if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;
   const int32_t base = s->class_base[cls];
-  // every lane takes part in the atomic (lane 0 adds 1, the rest 0): no
-  // divergent branch in the loop, so the wavefront stays whole.  The next
-  // task is fetched before the current one is aligned.
-  int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+  // The next task is fetched before the current one is aligned.
+  int w = next_wave_task(&s->class_next[cls], lane);
   SwTask task;
   if (w < nwt) task = b.tasks[base + w * GPW + grp];
   while (w < nwt) {
-    const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+    const int wn = next_wave_task(&s->class_next[cls], lane);
     SwTask next;
     if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
     fast_pair<G, P>(b, task, l);
@@ -990,9 +1026,13 @@ struct BandState {
 // classes spend many more VALU cycles per chunk and hide the per-lane loads,
 // one chunk ahead, behind them; there the segment loads, waited on at once,
 // measured slower, and S = 0 selects the per-lane loads.
+// A segment of 32 dwords is a 128-byte line: each lane's line is consumed
+// whole by the chunks of one block, before the L2 (shared by about 130k
+// concurrent streams on the device) evicts it; 8-dword segments re-fetched
+// each line up to four times.
 template <int P>
 constexpr int band_seg() {
-  return P <= 2 ? 8 : 0;
+  return P <= 2 ? 32 : 0;
 }
 template <int P>
 constexpr int band_lds_wave() {  // dwords per wavefront: row and column streams
@@ -1000,18 +1040,31 @@ constexpr int band_lds_wave() {  // dwords per wavefront: row and column streams
 }
 using band_lds_t = __attribute__((address_space(3))) uint32_t;
 
-// Load segment [base_s, base_s + S) of every lane s's stream (clamped below
-// at that lane's first dword, first_s) into seg[s * (S + 1) ..].
+// Load segment [base_s, base_s + S) of every lane s's row stream (clamped
+// below at that lane's first dword, first_s) into lds[s * (S + 1) ..], and
+// likewise its column stream into lds[(64 + s) * (S + 1) ..].  Every load is
+// issued before the first LDS write, so the wavefront waits for memory once
+// per block of chunks rather than once per few loads (with one or two
+// wavefronts per SIMD nothing else hides that latency).
 template <int S>
-__device__ __forceinline__ void band_load_segments(band_lds_t* seg, const uint32_t* __restrict__ codes, int lane,
-                                                   int32_t base, int32_t first) {
+__device__ __forceinline__ void band_load_segments(band_lds_t* lds, const uint32_t* __restrict__ codes, int lane,
+                                                   int32_t rbase, int32_t rfirst, int32_t qbase, int32_t qfirst) {
   constexpr int per = 64 / S;  // lanes' segments per load instruction
-#pragma unroll 4
+  const int i = lane % S;
+  uint32_t rv[S], qv[S];
+#pragma unroll
   for (int j = 0; j < S; j++) {
-    const int sl = per * j + lane / S, i = lane % S;
-    const int32_t bs = __shfl(base, sl, 64), fs = __shfl(first, sl, 64);
-    const int32_t idx = bs + i > fs ? bs + i : fs;
-    seg[sl * (S + 1) + i] = codes[idx];
+    const int sl = per * j + lane / S;
+    const int32_t rb = __shfl(rbase, sl, 64), rf = __shfl(rfirst, sl, 64);
+    const int32_t qb = __shfl(qbase, sl, 64), qf = __shfl(qfirst, sl, 64);
+    rv[j] = codes[rb + i > rf ? rb + i : rf];
+    qv[j] = codes[qb + i > qf ? qb + i : qf];
+  }
+#pragma unroll
+  for (int j = 0; j < S; j++) {
+    const int sl = per * j + lane / S;
+    lds[sl * (S + 1) + i] = rv[j];
+    lds[(64 + sl) * (S + 1) + i] = qv[j];
   }
 }
 
@@ -1102,8 +1155,8 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
   const bool active = task.pair >= 0;
   const int K = active ? task.row_len : 0;
   const int L = active ? task.col_len : 0;
-  const int D = active ? (task.flags >> kTaskBandShift) : 0;
-  const int score = active ? b.fwd[task.pair].score : 0;
+  const int D = active ? ((task.flags >> kTaskBandShift) & 0xF) : 0;
+  const int score = active ? (task.flags >> kTaskScoreShift) : 0;
   // wave-uniform loop length and the chunks that need no end-of-sequence checks
   const int nU = uni(wave_max_i32(active ? K + D : 0));
   if (nU <= 0) return;
@@ -1177,8 +1230,7 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
       // (the LDS reads of the previous block came before these writes: a
       // wavefront's LDS operations run in order)
       __builtin_amdgcn_wave_barrier();
-      band_load_segments<S>(lds, codes, lane, rd0 - mc - (S - 1), rfirst);
-      band_load_segments<S>(lds + 64 * (S + 1), codes, lane, qd0 - mc - (S - 1), qfirst);
+      band_load_segments<S>(lds, codes, lane, rd0 - mc - (S - 1), rfirst, qd0 - mc - (S - 1), qfirst);
       __builtin_amdgcn_wave_barrier();
       rhi = rcarry;
       qhi = qcarry;
@@ -1230,14 +1282,125 @@ __device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t
   const int32_t nwt = s->class_slots[cls] / 64;
   if (wave >= nwt) return;
   const int32_t base = s->class_base[cls];
-  int w = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+  int w = next_wave_task(&s->class_next[cls], lane);
   SwTask task;
   if (w < nwt) task = b.tasks[base + w * 64 + lane];
   while (w < nwt) {
-    const int wn = from_lane0(atomicAdd(&s->class_next[cls], lane == 0 ? 1 : 0));
+    const int wn = next_wave_task(&s->class_next[cls], lane);
     SwTask next;
     if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
     band_pair<P>(b, task, lds);
+    task = next;
+    w = wn;
+  }
+}
+
+// Reverse pass of a pair whose band is the single diagonal (D = 0: no
+// alignment with a gap can reach the score).  With the cells off the
+// diagonal absent, the band's cells are H'(k, k) = max(0, H'(k-1, k-1) +
+// s(R'[k], Q'[k])), and the answer is the least k holding the score.  One
+// lane per pair walks the diagonal 8 positions per chunk: the chunk's 8 row
+// and column codes make 8 selector bytes (even and odd positions) and two
+// v_perm turn them into the scoring table's 16*(s + gap_open) bytes; then per
+// position one extract, one add and one max update 16*H, and a shift-or and
+// a max keep the chunk's first maximum.  The code streams are staged through
+// LDS in the narrow band classes' segments, and a wavefront stops when every
+// lane has its answer.
+__device__ void diag_pair(const BatchView& b, const SwTask& task, band_lds_t* lds) {
+  constexpr int S = band_seg<1>();  // the P = 1 segment layout
+  static_assert(band_lds_wave<1>() == 2 * 64 * (S + 1), "the diagonal class uses the P = 1 segment layout");
+  const bool active = task.pair >= 0;
+  const int K = active ? task.row_len : 0;  // rows and columns: with D = 0 the rectangle is K x K
+  const int score = active ? (task.flags >> kTaskScoreShift) : 0;
+  const int nU = uni(wave_max_i32(K));
+  if (nU <= 0) return;
+  const int kmin = uni(wave_min_i32(active ? K : 0x7FFFFFFF));
+  const uint32_t* __restrict__ codes = b.codes;
+  const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
+  const int32_t go = 16 * b.sc.gap_open;
+  const uint32_t s16 = (uint32_t)(16 * score);
+  // streams: R'[k] = codes[row_start - k], Q'[k] = codes[col_start - k]; chunk
+  // m's window of 8 codes starts at row_start - 8m - 7, i.e. dwords rd0 - m and
+  // rd0 - m + 1 (qd0 likewise)
+  const int64_t ra = task.row_start - 7, qa = task.col_start - 7;
+  const int32_t rd0 = (int32_t)(ra >> 3), qd0 = (int32_t)(qa >> 3);
+  const uint32_t rsh = (uint32_t)(ra & 7) * 4u, qsh = (uint32_t)(qa & 7) * 4u;
+  const int32_t rfirst = (int32_t)((task.row_start - K + 1) >> 3) - 1;
+  const int32_t qfirst = (int32_t)((task.col_start - K + 1) >> 3) - 1;
+  const int lane = threadIdx.x & 63;
+  band_lds_t* rseg = lds + lane * (S + 1);
+  band_lds_t* qseg = lds + (64 + lane) * (S + 1);
+  uint32_t rcarry = codes[rd0 + 1], qcarry = codes[qd0 + 1];
+  int32_t h = 0;     // 16 * H'(k - 1, k - 1)
+  int32_t hit = -1;  // least k with H' = score
+  for (int u0 = 0; u0 < nU; u0 += 8) {
+    const int mc = u0 >> 3, c = mc & (S - 1);
+    uint32_t rlo, rhi, qlo, qhi;
+    if (c == 0) {
+      __builtin_amdgcn_wave_barrier();
+      band_load_segments<S>(lds, codes, lane, rd0 - mc - (S - 1), rfirst, qd0 - mc - (S - 1), qfirst);
+      __builtin_amdgcn_wave_barrier();
+      rhi = rcarry;
+      qhi = qcarry;
+      rcarry = rseg[0];
+      qcarry = qseg[0];
+      rlo = rseg[S - 1];
+      qlo = qseg[S - 1];
+    } else {
+      rhi = rseg[S - c];
+      qhi = qseg[S - c];
+      rlo = rseg[S - 1 - c];
+      qlo = qseg[S - 1 - c];
+    }
+    const uint32_t rw = nib_reverse(__builtin_amdgcn_alignbit(rhi, rlo, rsh));
+    const uint32_t qw = nib_reverse(__builtin_amdgcn_alignbit(qhi, qlo, qsh));
+    // selector bytes: bits 0-1 r ^ q, bits 2-3 r | q (the N bit); even
+    // positions from the low nibbles, odd ones from the high nibbles.
+    // Positions past the pair (k >= K) get selector 12, which v_perm turns
+    // into 0 (s = -gap_open): there H only falls, so the chunk's maximum
+    // never passes the score on codes beyond the pair.
+    uint32_t sle = __builtin_amdgcn_bitop3_b32(rw & 0x0F0F0F0Fu, qw & 0x0F0F0F0Fu, 0x03030303u, 0x7C);
+    uint32_t slo = __builtin_amdgcn_bitop3_b32((rw >> 4) & 0x0F0F0F0Fu, (qw >> 4) & 0x0F0F0F0Fu, 0x03030303u, 0x7C);
+    if (u0 + 8 > kmin) {  // some lane's pair ends in this chunk
+      const int lim = K - u0 < 0 ? 0 : K - u0;  // positions kept: i < lim
+      const int je = (lim + 1) >> 1, jo = lim >> 1;  // first killed byte of each half
+      const uint32_t me = je >= 4 ? 0u : 0xFFFFFFFFu << (8 * je);
+      const uint32_t mo = jo >= 4 ? 0u : 0xFFFFFFFFu << (8 * jo);
+      sle = (sle & ~me) | (0x0C0C0C0Cu & me);
+      slo = (slo & ~mo) | (0x0C0C0C0Cu & mo);
+    }
+    const uint32_t se = __builtin_amdgcn_perm(thi, tlo, sle);
+    const uint32_t so = __builtin_amdgcn_perm(thi, tlo, slo);
+    uint32_t best = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int32_t sp = (int32_t)__builtin_amdgcn_ubfe((i & 1) ? so : se, 8 * (i >> 1), 8);
+      h = max(h + sp - go, 0);
+      best = max(best, ((uint32_t)h << 4) | (uint32_t)(7 - i));
+    }
+    if (hit < 0 && (best >> 4) == s16 && score > 0) {
+      const int k = u0 + 7 - (int)(best & 15u);
+      if (k < K) hit = k;
+    }
+    if (__ballot(active && hit < 0 && u0 + 8 < K) == 0) break;
+  }
+  if (active) b.rev[task.pair] = hit >= 0 ? PairEnd{score, hit, hit, 0} : PairEnd{0, 0, 0, 0};
+}
+
+__device__ __forceinline__ void diag_class(const BatchView& b, int lane, int32_t wave, band_lds_t* lds) {
+  constexpr int cls = kClassDiag;
+  PassSched* s = b.sched;
+  const int32_t nwt = s->class_slots[cls] / 64;
+  if (wave >= nwt) return;
+  const int32_t base = s->class_base[cls];
+  int w = next_wave_task(&s->class_next[cls], lane);
+  SwTask task;
+  if (w < nwt) task = b.tasks[base + w * 64 + lane];
+  while (w < nwt) {
+    const int wn = next_wave_task(&s->class_next[cls], lane);
+    SwTask next;
+    if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
+    diag_pair(b, task, lds);
     task = next;
     w = wn;
   }
@@ -1275,6 +1438,13 @@ __global__ __launch_bounds__(256) void sw_band_all_kernel(BatchView b) {
   band_class<4>(b, lane, nwaves - 1 - wave, wl);
   band_class<2>(b, lane, wave, wl);
   band_class<1>(b, lane, nwaves - 1 - wave, wl);
+  diag_class(b, lane, wave, wl);
+}
+
+__global__ __launch_bounds__(256) void sw_diag_kernel(BatchView b) {
+  __shared__ uint32_t lds[4 * band_lds_wave<1>()];
+  const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+  diag_class(b, threadIdx.x & 63, wave, band_wave_lds(lds, band_lds_wave<1>()));
 }
 
 // ----------------------------------------------------------------------------
@@ -1490,6 +1660,7 @@ hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st) {
     case 2: hipLaunchKernelGGL(sw_band_kernel<2>, dim3(grid), dim3(256), 0, st, b); break;
     case 4: hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b); break;
     case 8: hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b); break;
+    case kBandDiag: hipLaunchKernelGGL(sw_diag_kernel, dim3(grid), dim3(256), 0, st, b); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1525,6 +1696,7 @@ hipError_t band_kernel_occupancy(int P, int* blocks_per_cu) {
     case 2: return occ(sw_band_kernel<2>, blocks_per_cu);
     case 4: return occ(sw_band_kernel<4>, blocks_per_cu);
     case 8: return occ(sw_band_kernel<8>, blocks_per_cu);
+    case kBandDiag: return occ(sw_diag_kernel, blocks_per_cu);
   }
   return hipErrorInvalidValue;
 }

@@ -10,7 +10,10 @@ namespace mhmsw {
 constexpr int kPassFwd = 0;    // forward pass: end points
 constexpr int kPassRerun = 1;  // 32-bit forward rerun of pairs the packed pass flagged
 constexpr int kPassRev = 2;    // reverse pass: begin points
-constexpr int kNumPasses = 3;
+constexpr int kPassRev2 = 3;   // large jobs: full wide-band rectangles whose probe did not reach the score
+constexpr int kNumPasses = 4;
+// Rows of a wide-band pair's probe rectangle in the reverse pass of a large job
+constexpr int kProbeRows = 32;
 
 struct BatchView {
   int64_t npairs;
@@ -47,6 +50,14 @@ struct BatchView {
   // to the class of P = rev_one16 (rev_one64), so that one launch per group
   // size runs them; 0: each task's class follows its rows
   int32_t rev_one16, rev_one64;
+  // large jobs: a wide-band pair's reverse rectangle is first probed on its
+  // first kProbeRows rows; pass kPassRev2 runs the full rectangle of the
+  // pairs whose probe did not reach the score, all in the class of P =
+  // rev2_one16 (rev2_one64 for 64-lane groups)
+  int32_t rev_probe, rev2_one16, rev2_one64;
+  // widest band (|l - k| <= band_max_d, at most kBandMaxD) a reverse task runs
+  // in the one-lane band kernels; wider ones run as rectangles
+  int32_t band_max_d;
 };
 
 // CIGAR pass over the results of a finished job (cigar_kernels.hip).
@@ -89,7 +100,9 @@ hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit
 hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
-// reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1) or 0 (every band class in one launch)
+// reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1), kBandDiag (the
+// single-diagonal class, |l - k| = 0) or 0 (every band class in one launch)
+constexpr int kBandDiag = -1;
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
