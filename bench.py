@@ -437,6 +437,22 @@ def locassm_leg(nctgs, steps, warmup, check, cores):
                                    (round(x / max(1, sum(st["phase_cycles"])), 3) for x in st["phase_cycles"]))),
            "extended_sides": int((got.left_len > 0).sum() + (got.right_len > 0).sum()),
            "bases_added": int(got.left_len.sum() + got.right_len.sum())}
+    # roofline: random transactions into per-side tables far larger than the
+    # caches.  Algorithmic bytes = two 64-byte sectors per mer occurrence (its
+    # slot probe and its vote atomic, the count phase's two accesses into the
+    # table); traffic = the whole launch's HBM bytes from the committed PMC
+    # profile of this workload
+    alg = 2 * 64 * st["mers"]
+    rl = {"bound": "hbm", "kernel": "walk_kernel", "unit": "GB/s", "peak": 8000.0,
+          "algorithmic_bytes": int(alg), "bytes_per_mer_occurrence": 128,
+          "achieved": round(alg / (dev_ms * 1e-3) / 1e9, 1)}
+    rl["frac"] = round(rl["achieved"] / rl["peak"], 4)
+    prof = load_traffic("locassm_50k")
+    if prof and nctgs == 50000:
+        rl["traffic"] = prof["per_launch_bytes"]["walk_kernel"]
+        rl["traffic_source"] = prof["source"]
+        rl["traffic_gbs"] = round(rl["traffic"] / (dev_ms * 1e-3) / 1e9, 1)
+    out["roofline"] = rl
     if check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
