@@ -88,7 +88,7 @@ struct mhmsw_ctx {
   // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
   // (all-class kernel, then P = 1, 2, 4, 8)
   int occ_fast[2][kMaxP] = {};
-  int occ_band[6] = {};  // all, P = 1, 2, 4, 8, the diagonal class
+  int occ_band[7] = {};  // all, P = 1, 2, 4, 8, 16 (two-lane pairs), the diagonal class
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
   // staged batch
@@ -302,8 +302,8 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
   for (int g = 0; g < 2; g++)
     for (int P = 1; P <= kMaxP; P++)
       if (fast_kernel_occupancy(g ? 64 : 16, P, &c->occ_fast[g][P - 1]) != hipSuccess) c->occ_fast[g][P - 1] = 0;
-  for (int i = 0; i < 6; i++)
-    if (band_kernel_occupancy(i == 5 ? kBandDiag : i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess)
+  for (int i = 0; i < 7; i++)
+    if (band_kernel_occupancy(i == 6 ? kBandDiag : i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess)
       c->occ_band[i] = 0;
   compute_score_params(c);
   // the no-allocation entry point's scratch on this device (a failure here
@@ -1349,11 +1349,11 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
         // one launch for every class: enough wavefronts for every task of all
         HIP_OR_FAIL(c, launch_band(bv, 0, band_grid(c->n_fast + 64 * kNumBandClasses, c->occ_band[0]), st));
       } else {
-        for (int i = 4; i >= 1; i--)
+        for (int i = 5; i >= 1; i--)
           HIP_OR_FAIL(c, launch_band(bv, 1 << (i - 1), band_grid(c->n_fast, c->occ_band[i]), st));
         // the diagonal class's lanes each stream two short code runs and do
         // little arithmetic on them: more waves per CU hide their loads
-        const int diag_cu = c->occ_band[5] > 0 ? std::min(kDiagBlocksPerCu, c->occ_band[5]) : kDiagBlocksPerCu;
+        const int diag_cu = c->occ_band[6] > 0 ? std::min(kDiagBlocksPerCu, c->occ_band[6]) : kDiagBlocksPerCu;
         HIP_OR_FAIL(c, launch_band(bv, kBandDiag,
                                    (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * diag_cu,
                                                                                (c->n_fast + 255) / 256)),

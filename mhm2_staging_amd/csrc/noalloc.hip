@@ -274,7 +274,9 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
   b.g64_min_rows = ci.tuning.g64_min_rows;
   b.bin_shift = ci.tuning.bin_shift;
   b.score2 = 0;
-  b.band_max_d = kBandMaxD;
+  // one launch runs every band class of a batch of up to 65536 pairs, and it
+  // holds the classes up to P = 8 (|l - k| <= 15)
+  b.band_max_d = npairs <= 65536 ? 15 : kBandMaxD;
   int32_t* state = reinterpret_cast<int32_t*>(base + kOffState);
   static_assert(kStateWords * 4 <= 256, "state words fit their slot");
   static_assert(sizeof(mhmsw_result) == sizeof(AlnResult), "result layout");
@@ -321,7 +323,7 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
       NA_CHECK(launch_band(bv, 0, (int)std::min<int64_t>(full, (npairs + 64 * kNumBandClasses + 255) / 256), st),
                "reverse band pass");
     } else {
-      for (int P = 8; P >= 1; P /= 2) NA_CHECK(launch_band(bv, P, full, st), "reverse band pass");
+      for (int P = 16; P >= 1; P /= 2) NA_CHECK(launch_band(bv, P, full, st), "reverse band pass");
       NA_CHECK(launch_band(bv, kBandDiag, full, st), "reverse band pass");
     }
     NA_CHECK(launch_plain(bv, plain_grid, kPlainMaxCols, st), "reverse pass (32-bit)");

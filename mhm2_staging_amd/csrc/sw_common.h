@@ -24,10 +24,12 @@ constexpr int kNumFwdClasses = 2 * kMaxP + 1;  // classes a forward pass uses
 // Reverse pass only: banded form, one lane per pair, P = 1, 2, 4, 8 packed
 // registers (4P diagonals) for |l - k| <= 1, 3, 7, 15: classes kClassBand + 0..3.
 constexpr int kClassBand = kNumFwdClasses;
-constexpr int kNumBandClasses = 5;
+constexpr int kNumBandClasses = 6;
+// |l - k| <= 31: two lanes of P = 8 per pair (64 diagonals)
+constexpr int kClassBand2 = kClassBand + 4;
 // |l - k| = 0 (D = 0, no gap can reach the score): one diagonal, one lane per pair
-constexpr int kClassDiag = kClassBand + 4;
-constexpr int kBandMaxD = 15;
+constexpr int kClassDiag = kClassBand + 5;
+constexpr int kBandMaxD = 31;
 constexpr int kNumClasses = kNumFwdClasses + kNumBandClasses;
 constexpr int kNumBins = 128;            // column-length bins of 32 columns
 constexpr int kNumBuckets = kNumClasses * kNumBins;
@@ -43,8 +45,8 @@ constexpr int kPlainMaxCols = 8192;          // LDS boundary row of the 32-bit k
 // Task flags.
 constexpr int32_t kTaskRowsAreRef = 1;  // rows hold SSW-ref (outer) positions
 constexpr int32_t kTaskReverse = 2;     // result goes to the reverse (begin) table
-constexpr int kTaskBandShift = 8;       // band tasks: (flags >> kTaskBandShift) & 0xF = D
-constexpr int kTaskScoreShift = 12;     // band tasks: flags >> kTaskScoreShift = the forward score
+constexpr int kTaskBandShift = 8;       // band tasks: (flags >> kTaskBandShift) & 0x1F = D
+constexpr int kTaskScoreShift = 13;     // band tasks: flags >> kTaskScoreShift = the forward score
 
 struct SwTask {
   int32_t pair;       // -1: empty slot

@@ -330,7 +330,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     if (pass == kPassRev && band) {
       // rows = reversed SSW-ref, columns = reversed SSW-query, one lane per pair
       t.pair = (int32_t)p;
-      // the forward score rides in the flags (below 2^19: packed pairs stay
+      // the forward score rides in the flags (below 2^18: packed pairs stay
       // under kFastOverflowScore), so the band kernels need no load of it
       t.flags = kTaskReverse | kTaskRowsAreRef | ((int32_t)D << kTaskBandShift) | (fe.score << kTaskScoreShift);
       t.row_start = ref_start;
@@ -338,7 +338,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
       t.col_start = qry_start;
       t.col_len = qry_len;
       // 4P diagonals cover |l - k| <= 2P - 1; D = 0 is the single diagonal
-      cls = D == 0 ? kClassDiag : kClassBand + (D <= 1 ? 0 : D <= 3 ? 1 : D <= 7 ? 2 : 3);
+      cls = D == 0 ? kClassDiag : kClassBand + (D <= 1 ? 0 : D <= 3 ? 1 : D <= 7 ? 2 : D <= 15 ? 3 : 4);
       const int bn = (ref_len + (int32_t)D) >> b.bin_shift;  // the lane's loop runs K + D steps
       bin = bn < kNumBins ? bn : kNumBins - 1;
       return true;
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
       // cells computed: the whole rectangle, or for a band task its 2D+1 diagonals
       int64_t w = t.col_len;
       if (cls >= kClassBand) {
-        const int64_t bw = 2 * (int64_t)((t.flags >> kTaskBandShift) & 0xF) + 1;
+        const int64_t bw = 2 * (int64_t)((t.flags >> kTaskBandShift) & 0x1F) + 1;
         w = w < bw ? w : bw;
       }
       my_cells += (unsigned long long)t.row_len * (unsigned long long)w;
@@ -492,6 +492,7 @@ __global__ void schedule_kernel(BatchView b) {
     int32_t padded = tot;
     if (c < kClassFast64) padded = (tot + 3) & ~3;        // 4 groups per wavefront
     if (c >= kClassBand) padded = (tot + 63) & ~63;       // one pair per lane
+    if (c == kClassBand2) padded = (tot + 31) & ~31;      // two lanes per pair
     slots[c] = padded;
     s->class_slots[c] = padded;
     s->class_next[c] = 0;
@@ -1088,9 +1089,24 @@ __device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& h_sl
 // u0+8] and qw column codes Q'[u0+2P-D .. +7], lowest nibble first: the
 // reversed prefixes are the forward codes read backwards, so each chunk's
 // window lies one dword below the last one and its nibbles are reversed.
-template <int P, bool EDGE>
-__device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int L, int D, uint32_t rw, uint32_t qw,
-                                           uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
+// Two lanes per pair (NL = 2): lane g of the pair holds the global slots
+// 2P*g .. 2P*g + 2P - 1.  The cells' cross-slot inputs at the lanes' common
+// edge (F into the upper lane's slot 0, E into the lower lane's top slot)
+// and the codes moving between the lanes' rings come from the other lane by
+// a quad_perm DPP move; the lower lane takes the row stream, the upper lane
+// the column stream.
+__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, int g) {  // 0 for the lower lane
+  const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xA0, 0xF, 0xF, false);  // quad_perm 0,0,2,2
+  return g ? x : 0u;
+}
+__device__ __forceinline__ uint32_t from_upper_lane(uint32_t v, int g) {  // 0 for the upper lane
+  const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, false);  // quad_perm 1,1,3,3
+  return g ? 0u : x;
+}
+
+template <int P, int NL, bool EDGE>
+__device__ __forceinline__ void band_chunk(BandState<P>& st, int g, int u0, int K, int L, int D, uint32_t rw,
+                                           uint32_t qw, uint32_t tlo, uint32_t thi, uint32_t go, uint32_t ge) {
 #pragma unroll
   for (int p = 0; p < P; p++) {
     st.be[p] = 0;
@@ -1100,33 +1116,49 @@ __device__ __forceinline__ void band_chunk(BandState<P>& st, int u0, int K, int 
   for (int s = 0; s < 8; s++) {
     const uint32_t tag = (uint32_t)(7 - s) * 0x00010001u;
     uint32_t ee[P], fe[P];
+    // F into slot 0 comes from slot -1: the lower lane's top slot (NL = 2)
+    uint32_t f0 = st.fo[P - 1] << 16;
+    if constexpr (NL == 2) f0 |= from_lower_lane(st.fo[P - 1], g) >> 16;
 #pragma unroll
     for (int p = 0; p < P; p++) {
-      const uint32_t fin = p ? st.fo[p - 1] : (st.fo[P - 1] << 16);
+      const uint32_t fin = p ? st.fo[p - 1] : f0;
       band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s) % P], st.hoe[p], st.eo[p], fin, ee[p], fe[p], st.be[p], tag,
                 tlo, thi, go, ge);
     }
-    // column code entering the top slot for the odd step
+    // column code entering the top slot for the odd step: from the stream, or
+    // (the lower lane of a pair) the code leaving the upper lane's slot 0
     uint32_t qn = (qw >> (4 * s)) & 0xFu;
-    if constexpr (EDGE) qn = (u0 + s + 2 * P - D < L) ? qn : kKillColN;
+    if constexpr (EDGE) qn = (u0 + s + 2 * P * NL - D < L) ? qn : kKillColN;
+    if constexpr (NL == 2) {
+      const uint32_t down = from_upper_lane(st.qr[s % P] & 0xFFFFu, g);
+      qn = g ? qn : down;
+    }
     st.qr[s % P] = __builtin_amdgcn_alignbit(qn, st.qr[s % P], 16);
+    // E into the top slot comes from slot 2P: the upper lane's slot 0 (NL = 2)
+    uint32_t etop = ee[0] >> 16;
+    if constexpr (NL == 2) etop |= from_upper_lane(ee[0], g) << 16;
 #pragma unroll
     for (int p = 0; p < P; p++) {
-      const uint32_t ein = p < P - 1 ? ee[p + 1] : (ee[0] >> 16);
+      const uint32_t ein = p < P - 1 ? ee[p + 1] : etop;
       band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s + 1) % P], st.hoo[p], ein, fe[p], st.eo[p], st.fo[p],
                 st.bo[p], tag, tlo, thi, go, ge);
     }
-    // row code entering slot 0 for the next u-step
+    // row code entering slot 0 for the next u-step: from the stream, or (the
+    // upper lane of a pair) the code leaving the lower lane's top slot
     uint32_t rn = ((rw >> (4 * s)) & 0xFu) | kRowHi;
     if constexpr (EDGE) rn = (u0 + s + 1 < K) ? rn : kKillRowN;
     const int ri = (P - 1 - s + 8 * P) % P;
+    if constexpr (NL == 2) {
+      const uint32_t up = from_lower_lane(st.rr[ri] >> 16, g);
+      rn = g ? up : rn;
+    }
     st.rr[ri] = (st.rr[ri] << 16) | rn;
   }
 }
 
 // Least (k, l) among this chunk's cells holding the score (s16 = 16 * score).
 template <int P>
-__device__ __forceinline__ void band_hits(const BandState<P>& st, int u0, int K, int L, int D, uint32_t s16,
+__device__ __forceinline__ void band_hits(const BandState<P>& st, int g, int u0, int K, int L, int D, uint32_t s16,
                                           uint32_t& keymin) {
 #pragma unroll
   for (int par = 0; par < 2; par++) {
@@ -1136,7 +1168,7 @@ __device__ __forceinline__ void band_hits(const BandState<P>& st, int u0, int K,
       for (int half = 0; half < 2; half++) {
         const uint32_t v = ((par ? st.bo[p] : st.be[p]) >> (16 * half)) & 0xFFFFu;
         if ((v & 0xFFF0u) == s16) {
-          const int j = half * P + p;
+          const int j = 2 * P * g + half * P + p;  // the pair's slot
           const int u = u0 + 7 - (int)(v & 0xFu);
           const int k = u - j;
           const int l = k + 2 * j + par - D;
@@ -1150,18 +1182,21 @@ __device__ __forceinline__ void band_hits(const BandState<P>& st, int u0, int K,
   }
 }
 
-template <int P>
+template <int P, int NL>
 __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* lds) {
+  static_assert(NL == 1 || NL == 2, "one or two lanes per pair");
+  static_assert(NL == 1 || band_seg<P>() == 0, "two-lane pairs load their streams per lane");
+  const int g = (threadIdx.x & 63) & (NL - 1);  // the lane's place in its pair
   const bool active = task.pair >= 0;
   const int K = active ? task.row_len : 0;
   const int L = active ? task.col_len : 0;
-  const int D = active ? ((task.flags >> kTaskBandShift) & 0xF) : 0;
+  const int D = active ? ((task.flags >> kTaskBandShift) & 0x1F) : 0;
   const int score = active ? (task.flags >> kTaskScoreShift) : 0;
   // wave-uniform loop length and the chunks that need no end-of-sequence checks
   const int nU = uni(wave_max_i32(active ? K + D : 0));
   if (nU <= 0) return;
   const int kmin = uni(wave_min_i32(active ? K : 0x7FFFFFFF));
-  const int qmin = uni(wave_min_i32(active ? L - 2 * P + D : 0x7FFFFFFF));
+  const int qmin = uni(wave_min_i32(active ? L - 2 * P * NL + D : 0x7FFFFFFF));
 
   const uint32_t* __restrict__ codes = b.codes;
   const uint32_t tlo = b.sc.perm_lo, thi = b.sc.perm_hi;
@@ -1172,12 +1207,12 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
   BandState<P> st;
   // u = 0: slot j holds R'[-j] (only slot 0 is on the sequence) and
   // Q'[j - D] for the even step (R'[k] = codes[row_start - k], Q'[l] =
-  // codes[col_start - l])
-  const uint32_t r0 = K > 0 ? (code_at(codes, task.row_start) | kRowHi) : kKillRowN;
+  // codes[col_start - l]); j is the pair's slot, 2P*g + the lane's slot
+  const uint32_t r0 = K > 0 && g == 0 ? (code_at(codes, task.row_start) | kRowHi) : kKillRowN;
 #pragma unroll
   for (int p = 0; p < P; p++) {
     st.rr[p] = p == 0 ? (r0 | (kKillRowN << 16)) : (kKillRowN | (kKillRowN << 16));
-    const int ilo = p - D, ihi = P + p - D;
+    const int ilo = 2 * P * g + p - D, ihi = 2 * P * g + P + p - D;
     const uint32_t qlo = (ilo >= 0 && ilo < L) ? code_at(codes, task.col_start - ilo) : kKillColN;
     const uint32_t qhi = (ihi >= 0 && ihi < L) ? code_at(codes, task.col_start - ihi) : kKillColN;
     st.qr[p] = qlo | (qhi << 16);
@@ -1190,7 +1225,7 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
   // chunk m's window (8 codes) starts 8m codes below the first one (ra, qa),
   // so it is dwords rd0 - m and rd0 - m + 1 (qd0 likewise).  Dword indices
   // are 32-bit (the job's code buffer holds fewer than 2^31 dwords).
-  const int64_t ra = task.row_start - 8, qa = task.col_start - (2 * P - D) - 7;
+  const int64_t ra = task.row_start - 8, qa = task.col_start - (2 * P * NL - D) - 7;
   const int32_t rd0 = (int32_t)(ra >> 3), qd0 = (int32_t)(qa >> 3);
   const uint32_t rsh = (uint32_t)(ra & 7) * 4u, qsh = (uint32_t)(qa & 7) * 4u;
   // stream reads stay within the lane's own sequence (less one dword): its
@@ -1248,18 +1283,22 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
     const uint32_t qw = nib_reverse(__builtin_amdgcn_alignbit(qhi, qlo, qsh));
     const bool middle = u0 + 8 < kmin && u0 + 8 <= qmin;
     if (middle) {
-      band_chunk<P, false>(st, u0, K, L, D, rw, qw, tlo, thi, go, ge);
+      band_chunk<P, NL, false>(st, g, u0, K, L, D, rw, qw, tlo, thi, go, ge);
     } else {
-      band_chunk<P, true>(st, u0, K, L, D, rw, qw, tlo, thi, go, ge);
+      band_chunk<P, NL, true>(st, g, u0, K, L, D, rw, qw, tlo, thi, go, ge);
     }
     // scores never exceed the forward score: a hit is a chunk maximum equal to it
     uint32_t m = 0;
 #pragma unroll
     for (int p = 0; p < P; p++) m = pk_maxu(m, pk_maxu(st.be[p], st.bo[p]));
     const uint32_t mh = (m & 0xFFFFu) > (m >> 16) ? (m & 0xFFFFu) : (m >> 16);
-    if (active && score > 0 && (mh & 0xFFF0u) == s16) band_hits<P>(st, u0, K, L, D, s16, keymin);
+    if (active && score > 0 && (mh & 0xFFF0u) == s16) band_hits<P>(st, g, u0, K, L, D, s16, keymin);
   }
-  if (active) {
+  if constexpr (NL == 2) {  // the pair's least hit over both lanes, written by its lower lane
+    const uint32_t other = (uint32_t)__shfl_xor((int)keymin, 1, 64);
+    keymin = other < keymin ? other : keymin;
+  }
+  if (active && g == 0) {
     PairEnd e{0, 0, 0, 0};
     if (keymin != 0xFFFFFFFFu) e = PairEnd{score, (int32_t)(keymin >> 16), (int32_t)(keymin & 0xFFFFu), 0};
     b.rev[task.pair] = e;
@@ -1267,29 +1306,30 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
 }
 
 // band class of P packed registers per lane (4P diagonals): P = 1, 2, 4, 8
-template <int P>
+template <int P, int NL>
 constexpr int band_class_of() {
-  return kClassBand + (P == 1 ? 0 : P == 2 ? 1 : P == 4 ? 2 : 3);
+  return kClassBand + (NL == 2 ? 4 : P == 1 ? 0 : P == 2 ? 1 : P == 4 ? 2 : 3);
 }
 
 // Take the class's wave tasks from its work counter until they run out.
 // Wavefronts numbered past the class's work leave before touching the shared
 // counter (the rest still take every task: each loops until it runs out).
-template <int P>
+template <int P, int NL = 1>
 __device__ __forceinline__ void band_class(const BatchView& b, int lane, int32_t wave, band_lds_t* lds) {
-  constexpr int cls = band_class_of<P>();
+  constexpr int cls = band_class_of<P, NL>();
+  constexpr int per = 64 / NL;  // pairs per wavefront
   PassSched* s = b.sched;
-  const int32_t nwt = s->class_slots[cls] / 64;
+  const int32_t nwt = s->class_slots[cls] / per;
   if (wave >= nwt) return;
   const int32_t base = s->class_base[cls];
   int w = next_wave_task(&s->class_next[cls], lane);
   SwTask task;
-  if (w < nwt) task = b.tasks[base + w * 64 + lane];
+  if (w < nwt) task = b.tasks[base + w * per + lane / NL];
   while (w < nwt) {
     const int wn = next_wave_task(&s->class_next[cls], lane);
     SwTask next;
-    if (wn < nwt) next = b.tasks[base + wn * 64 + lane];
-    band_pair<P>(b, task, lds);
+    if (wn < nwt) next = b.tasks[base + wn * per + lane / NL];
+    band_pair<P, NL>(b, task, lds);
     task = next;
     w = wn;
   }
@@ -1439,6 +1479,12 @@ __global__ __launch_bounds__(256) void sw_band_all_kernel(BatchView b) {
   band_class<2>(b, lane, wave, wl);
   band_class<1>(b, lane, nwaves - 1 - wave, wl);
   diag_class(b, lane, wave, wl);
+}
+
+// Bands of 16..31 diagonals each side: two lanes of 32 diagonals per pair
+__global__ __launch_bounds__(256) void sw_band2_kernel(BatchView b) {
+  const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+  band_class<8, 2>(b, threadIdx.x & 63, wave, nullptr);
 }
 
 __global__ __launch_bounds__(256) void sw_diag_kernel(BatchView b) {
@@ -1660,6 +1706,7 @@ hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st) {
     case 2: hipLaunchKernelGGL(sw_band_kernel<2>, dim3(grid), dim3(256), 0, st, b); break;
     case 4: hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b); break;
     case 8: hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b); break;
+    case 16: hipLaunchKernelGGL(sw_band2_kernel, dim3(grid), dim3(256), 0, st, b); break;
     case kBandDiag: hipLaunchKernelGGL(sw_diag_kernel, dim3(grid), dim3(256), 0, st, b); break;
     default: return hipErrorInvalidValue;
   }
@@ -1696,6 +1743,7 @@ hipError_t band_kernel_occupancy(int P, int* blocks_per_cu) {
     case 2: return occ(sw_band_kernel<2>, blocks_per_cu);
     case 4: return occ(sw_band_kernel<4>, blocks_per_cu);
     case 8: return occ(sw_band_kernel<8>, blocks_per_cu);
+    case 16: return occ(sw_band2_kernel, blocks_per_cu);
     case kBandDiag: return occ(sw_diag_kernel, blocks_per_cu);
   }
   return hipErrorInvalidValue;
