@@ -88,7 +88,7 @@ struct mhmsw_ctx {
   // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
   // (all-class kernel, then P = 1, 2, 4, 8)
   int occ_fast[2][kMaxP] = {};
-  int occ_band[7] = {};  // all, P = 1, 2, 4, 8, 16 (two-lane pairs), the diagonal class
+  int occ_band[7] = {};  // all, P = 1, 2, 4, 8, 16 (four-lane pairs), the diagonal class
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
   // staged batch

@@ -25,7 +25,7 @@ constexpr int kNumFwdClasses = 2 * kMaxP + 1;  // classes a forward pass uses
 // registers (4P diagonals) for |l - k| <= 1, 3, 7, 15: classes kClassBand + 0..3.
 constexpr int kClassBand = kNumFwdClasses;
 constexpr int kNumBandClasses = 6;
-// |l - k| <= 31: two lanes of P = 8 per pair (64 diagonals)
+// |l - k| <= 31: four lanes of P = 4 per pair (64 diagonals)
 constexpr int kClassBand2 = kClassBand + 4;
 // |l - k| = 0 (D = 0, no gap can reach the score): one diagonal, one lane per pair
 constexpr int kClassDiag = kClassBand + 5;

@@ -492,7 +492,7 @@ __global__ void schedule_kernel(BatchView b) {
     int32_t padded = tot;
     if (c < kClassFast64) padded = (tot + 3) & ~3;        // 4 groups per wavefront
     if (c >= kClassBand) padded = (tot + 63) & ~63;       // one pair per lane
-    if (c == kClassBand2) padded = (tot + 31) & ~31;      // two lanes per pair
+    if (c == kClassBand2) padded = (tot + 15) & ~15;      // four lanes per pair
     slots[c] = padded;
     s->class_slots[c] = padded;
     s->class_next[c] = 0;
@@ -1089,19 +1089,23 @@ __device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& h_sl
 // u0+8] and qw column codes Q'[u0+2P-D .. +7], lowest nibble first: the
 // reversed prefixes are the forward codes read backwards, so each chunk's
 // window lies one dword below the last one and its nibbles are reversed.
-// Two lanes per pair (NL = 2): lane g of the pair holds the global slots
-// 2P*g .. 2P*g + 2P - 1.  The cells' cross-slot inputs at the lanes' common
-// edge (F into the upper lane's slot 0, E into the lower lane's top slot)
-// and the codes moving between the lanes' rings come from the other lane by
-// a quad_perm DPP move; the lower lane takes the row stream, the upper lane
-// the column stream.
-__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, int g) {  // 0 for the lower lane
-  const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xA0, 0xF, 0xF, false);  // quad_perm 0,0,2,2
+// NL lanes per pair (NL = 2 or 4): lane g of the pair holds the pair's slots
+// 2P*g .. 2P*g + 2P - 1.  The cells' cross-slot inputs at adjacent lanes'
+// common edges (F into a lane's slot 0 from the lane below, E into its top
+// slot from the lane above) and the codes moving between the lanes' rings
+// come from the neighbouring lane by a quad_perm DPP move; the lowest lane
+// takes the row stream, the highest the column stream.
+template <int NL>
+__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v, int g) {  // 0 for the lowest lane
+  constexpr int ctrl = NL == 2 ? 0xA0 : 0x90;  // quad_perm 0,0,2,2 / 0,0,1,2
+  const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
   return g ? x : 0u;
 }
-__device__ __forceinline__ uint32_t from_upper_lane(uint32_t v, int g) {  // 0 for the upper lane
-  const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xF5, 0xF, 0xF, false);  // quad_perm 1,1,3,3
-  return g ? 0u : x;
+template <int NL>
+__device__ __forceinline__ uint32_t from_upper_lane(uint32_t v, int g) {  // 0 for the highest lane
+  constexpr int ctrl = NL == 2 ? 0xF5 : 0xF9;  // quad_perm 1,1,3,3 / 1,2,3,3
+  const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
+  return g == NL - 1 ? 0u : x;
 }
 
 template <int P, int NL, bool EDGE>
@@ -1116,40 +1120,40 @@ __device__ __forceinline__ void band_chunk(BandState<P>& st, int g, int u0, int 
   for (int s = 0; s < 8; s++) {
     const uint32_t tag = (uint32_t)(7 - s) * 0x00010001u;
     uint32_t ee[P], fe[P];
-    // F into slot 0 comes from slot -1: the lower lane's top slot (NL = 2)
+    // F into slot 0 comes from slot -1: the lane below's top slot
     uint32_t f0 = st.fo[P - 1] << 16;
-    if constexpr (NL == 2) f0 |= from_lower_lane(st.fo[P - 1], g) >> 16;
+    if constexpr (NL > 1) f0 |= from_lower_lane<NL>(st.fo[P - 1], g) >> 16;
 #pragma unroll
     for (int p = 0; p < P; p++) {
       const uint32_t fin = p ? st.fo[p - 1] : f0;
       band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s) % P], st.hoe[p], st.eo[p], fin, ee[p], fe[p], st.be[p], tag,
                 tlo, thi, go, ge);
     }
-    // column code entering the top slot for the odd step: from the stream, or
-    // (the lower lane of a pair) the code leaving the upper lane's slot 0
+    // column code entering the top slot for the odd step: from the stream (the
+    // highest lane of a pair), or the code leaving the next lane's slot 0
     uint32_t qn = (qw >> (4 * s)) & 0xFu;
     if constexpr (EDGE) qn = (u0 + s + 2 * P * NL - D < L) ? qn : kKillColN;
-    if constexpr (NL == 2) {
-      const uint32_t down = from_upper_lane(st.qr[s % P] & 0xFFFFu, g);
-      qn = g ? qn : down;
+    if constexpr (NL > 1) {
+      const uint32_t down = from_upper_lane<NL>(st.qr[s % P] & 0xFFFFu, g);
+      qn = g == NL - 1 ? qn : down;
     }
     st.qr[s % P] = __builtin_amdgcn_alignbit(qn, st.qr[s % P], 16);
-    // E into the top slot comes from slot 2P: the upper lane's slot 0 (NL = 2)
+    // E into the top slot comes from slot 2P: the lane above's slot 0
     uint32_t etop = ee[0] >> 16;
-    if constexpr (NL == 2) etop |= from_upper_lane(ee[0], g) << 16;
+    if constexpr (NL > 1) etop |= from_upper_lane<NL>(ee[0], g) << 16;
 #pragma unroll
     for (int p = 0; p < P; p++) {
       const uint32_t ein = p < P - 1 ? ee[p + 1] : etop;
       band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s + 1) % P], st.hoo[p], ein, fe[p], st.eo[p], st.fo[p],
                 st.bo[p], tag, tlo, thi, go, ge);
     }
-    // row code entering slot 0 for the next u-step: from the stream, or (the
-    // upper lane of a pair) the code leaving the lower lane's top slot
+    // row code entering slot 0 for the next u-step: from the stream (the
+    // lowest lane of a pair), or the code leaving the lane below's top slot
     uint32_t rn = ((rw >> (4 * s)) & 0xFu) | kRowHi;
     if constexpr (EDGE) rn = (u0 + s + 1 < K) ? rn : kKillRowN;
     const int ri = (P - 1 - s + 8 * P) % P;
-    if constexpr (NL == 2) {
-      const uint32_t up = from_lower_lane(st.rr[ri] >> 16, g);
+    if constexpr (NL > 1) {
+      const uint32_t up = from_lower_lane<NL>(st.rr[ri] >> 16, g);
       rn = g ? up : rn;
     }
     st.rr[ri] = (st.rr[ri] << 16) | rn;
@@ -1184,8 +1188,8 @@ __device__ __forceinline__ void band_hits(const BandState<P>& st, int g, int u0,
 
 template <int P, int NL>
 __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* lds) {
-  static_assert(NL == 1 || NL == 2, "one or two lanes per pair");
-  static_assert(NL == 1 || band_seg<P>() == 0, "two-lane pairs load their streams per lane");
+  static_assert(NL == 1 || NL == 2 || NL == 4, "one, two or four lanes per pair");
+  static_assert(NL == 1 || band_seg<P>() == 0, "multi-lane pairs load their streams per lane");
   const int g = (threadIdx.x & 63) & (NL - 1);  // the lane's place in its pair
   const bool active = task.pair >= 0;
   const int K = active ? task.row_len : 0;
@@ -1294,8 +1298,9 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
     const uint32_t mh = (m & 0xFFFFu) > (m >> 16) ? (m & 0xFFFFu) : (m >> 16);
     if (active && score > 0 && (mh & 0xFFF0u) == s16) band_hits<P>(st, g, u0, K, L, D, s16, keymin);
   }
-  if constexpr (NL == 2) {  // the pair's least hit over both lanes, written by its lower lane
-    const uint32_t other = (uint32_t)__shfl_xor((int)keymin, 1, 64);
+#pragma unroll
+  for (int m = 1; m < NL; m <<= 1) {  // the pair's least hit over its lanes, written by its lowest lane
+    const uint32_t other = (uint32_t)__shfl_xor((int)keymin, m, 64);
     keymin = other < keymin ? other : keymin;
   }
   if (active && g == 0) {
@@ -1307,8 +1312,8 @@ __device__ void band_pair(const BatchView& b, const SwTask& task, band_lds_t* ld
 
 // band class of P packed registers per lane (4P diagonals): P = 1, 2, 4, 8
 template <int P, int NL>
-constexpr int band_class_of() {
-  return kClassBand + (NL == 2 ? 4 : P == 1 ? 0 : P == 2 ? 1 : P == 4 ? 2 : 3);
+constexpr int band_class_of() {  // the class a (P, NL) layout serves: 4P * NL diagonals
+  return kClassBand + (4 * P * NL == 4 ? 0 : 4 * P * NL == 8 ? 1 : 4 * P * NL == 16 ? 2 : 4 * P * NL == 32 ? 3 : 4);
 }
 
 // Take the class's wave tasks from its work counter until they run out.
@@ -1451,14 +1456,14 @@ __device__ __forceinline__ band_lds_t* band_wave_lds(uint32_t* block_lds, int dw
   return (band_lds_t*)block_lds + (threadIdx.x / 64) * dw;
 }
 
-template <int P>
+template <int P, int NL = 1>
 __global__ __launch_bounds__(256) void sw_band_kernel(BatchView b) {
   const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
   if constexpr (band_seg<P>() > 0) {
     __shared__ uint32_t lds[4 * band_lds_wave<P>()];
-    band_class<P>(b, threadIdx.x & 63, wave, band_wave_lds(lds, band_lds_wave<P>()));
+    band_class<P, NL>(b, threadIdx.x & 63, wave, band_wave_lds(lds, band_lds_wave<P>()));
   } else {
-    band_class<P>(b, threadIdx.x & 63, wave, nullptr);
+    band_class<P, NL>(b, threadIdx.x & 63, wave, nullptr);
   }
 }
 
@@ -1479,12 +1484,6 @@ __global__ __launch_bounds__(256) void sw_band_all_kernel(BatchView b) {
   band_class<2>(b, lane, wave, wl);
   band_class<1>(b, lane, nwaves - 1 - wave, wl);
   diag_class(b, lane, wave, wl);
-}
-
-// Bands of 16..31 diagonals each side: two lanes of 32 diagonals per pair
-__global__ __launch_bounds__(256) void sw_band2_kernel(BatchView b) {
-  const int32_t wave = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
-  band_class<8, 2>(b, threadIdx.x & 63, wave, nullptr);
 }
 
 __global__ __launch_bounds__(256) void sw_diag_kernel(BatchView b) {
@@ -1706,7 +1705,7 @@ hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st) {
     case 2: hipLaunchKernelGGL(sw_band_kernel<2>, dim3(grid), dim3(256), 0, st, b); break;
     case 4: hipLaunchKernelGGL(sw_band_kernel<4>, dim3(grid), dim3(256), 0, st, b); break;
     case 8: hipLaunchKernelGGL(sw_band_kernel<8>, dim3(grid), dim3(256), 0, st, b); break;
-    case 16: hipLaunchKernelGGL(sw_band2_kernel, dim3(grid), dim3(256), 0, st, b); break;
+    case 16: hipLaunchKernelGGL((sw_band_kernel<4, 4>), dim3(grid), dim3(256), 0, st, b); break;
     case kBandDiag: hipLaunchKernelGGL(sw_diag_kernel, dim3(grid), dim3(256), 0, st, b); break;
     default: return hipErrorInvalidValue;
   }
@@ -1743,7 +1742,7 @@ hipError_t band_kernel_occupancy(int P, int* blocks_per_cu) {
     case 2: return occ(sw_band_kernel<2>, blocks_per_cu);
     case 4: return occ(sw_band_kernel<4>, blocks_per_cu);
     case 8: return occ(sw_band_kernel<8>, blocks_per_cu);
-    case 16: return occ(sw_band2_kernel, blocks_per_cu);
+    case 16: return occ(sw_band_kernel<4, 4>, blocks_per_cu);
     case kBandDiag: return occ(sw_diag_kernel, blocks_per_cu);
   }
   return hipErrorInvalidValue;

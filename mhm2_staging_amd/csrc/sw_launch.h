@@ -100,8 +100,8 @@ hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit
 hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
-// reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1), 16 (two lanes
-// of P = 8 per pair, |l - k| <= 31), kBandDiag (the single-diagonal class,
+// reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1), 16 (four lanes
+// of P = 4 per pair, |l - k| <= 31), kBandDiag (the single-diagonal class,
 // |l - k| = 0) or 0 (every class up to P = 8 in one launch)
 constexpr int kBandDiag = -1;
 hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
