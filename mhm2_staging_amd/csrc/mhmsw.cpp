@@ -87,7 +87,7 @@ struct mhmsw_ctx {
   // 256-thread blocks of each persistent kernel that fit one CU at once:
   // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
   // (all-class kernel, then P = 1, 2, 4, 8)
-  int occ_fast[2][kMaxP] = {};
+  int occ_fast[2][2][kMaxP] = {};  // [16 or 64 lanes per pair][64-lane class][P - 1]
   int occ_band[7] = {};  // all, P = 1, 2, 4, 8, 16 (four-lane pairs), the diagonal class
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
@@ -299,9 +299,14 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
     return MHMSW_E_NOMEM;
   }
   std::memset(c->tail, 0, sizeof(mhmsw_ctx::JobTail));
-  for (int g = 0; g < 2; g++)
-    for (int P = 1; P <= kMaxP; P++)
-      if (fast_kernel_occupancy(g ? 64 : 16, P, &c->occ_fast[g][P - 1]) != hipSuccess) c->occ_fast[g][P - 1] = 0;
+  // every kernel shape a class can run as: the narrow forward shape (a class
+  // of many pairs) and the class's own group size
+  for (int r = 0; r < 2; r++)
+    for (int g = 0; g < 2; g++)
+      for (int P = 1; P <= kMaxP; P++)
+        if (fast_kernel_occupancy(fast_shape(g ? 64 : 16, P, r == 0, INT64_MAX / 64, c->cus),
+                                  &c->occ_fast[r][g][P - 1]) != hipSuccess)
+          c->occ_fast[r][g][P - 1] = 0;
   for (int i = 0; i < 7; i++)
     if (band_kernel_occupancy(i == 6 ? kBandDiag : i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess)
       c->occ_band[i] = 0;
@@ -1264,6 +1269,12 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // follow the packed form's tuning (16 wavefronts per CU: its fastest
   // measured shape, 1.7x the rate at 8 on the length-skewed 4M-pair batch)
   const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cus * 16, c->npairs));
+  // a packed class's launch: the kernel shape of its pass, 4 wavefronts of
+  // 64 / lanes pairs per block
+  auto fast = [&](const BatchView& v, int group, int P, bool forward, int64_t pairs) {
+    const FastShape s = fast_shape(group, P, forward, pairs, c->cus);
+    return launch_fast(v, s, grid_cap(pairs, 256 / s.lanes, c->occ_fast[s.lanes >= 16][group == 64][P - 1]), st);
+  };
   // the band kernels' grids do not follow it either: 2 blocks (8 wavefronts)
   // per CU, or fewer where a kernel keeps fewer resident.  More waves per CU
   // only share the L2 among more code streams (the P = 4 class measured
@@ -1303,7 +1314,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int k = kClassFast64 + P - 1;
       if (c->fast_class_used[k]) {
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
-        HIP_OR_FAIL(c, launch_fast(bf, 64, P, grid_cap(c->class_pairs[k], 4, c->occ_fast[1][P - 1]), st));
+        HIP_OR_FAIL(c, fast(bf, 64, P, true, c->class_pairs[k]));
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp64 = std::max(maxp64, P);
       }
@@ -1312,7 +1323,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const int k = kClassFast16 + P - 1;
       if (c->fast_class_used[k]) {
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][0], st));
-        HIP_OR_FAIL(c, launch_fast(bf, 16, P, grid_cap(c->class_pairs[k], 16, c->occ_fast[0][P - 1]), st));
+        HIP_OR_FAIL(c, fast(bf, 16, P, true, c->class_pairs[k]));
         if (ev) HIP_OR_FAIL(c, hipEventRecord(c->cev[k][1], st));
         maxp16 = std::max(maxp16, P);
       }
@@ -1341,9 +1352,9 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       // a small job's rectangles sit in one class per group size (make_view)
       const int low64 = b.rev_one64 > 0 ? b.rev_one64 : 1, low16 = b.rev_one16 > 0 ? b.rev_one16 : 1;
       for (int P = maxp64; P >= low64; P--)
-        HIP_OR_FAIL(c, launch_fast(bv, 64, P, grid_cap(c->n_fast, 4, c->occ_fast[1][P - 1]), st));
+        HIP_OR_FAIL(c, fast(bv, 64, P, false, c->n_fast));
       for (int P = rev16; P >= low16; P--)
-        HIP_OR_FAIL(c, launch_fast(bv, 16, P, grid_cap(c->n_fast, 16, c->occ_fast[0][P - 1]), st));
+        HIP_OR_FAIL(c, fast(bv, 16, P, false, c->n_fast));
       // pairs whose begin lies within a narrow band of diagonals: one lane each
       if (c->n_fast <= kBandAllMaxPairs) {
         // one launch for every class: enough wavefronts for every task of all
@@ -1366,9 +1377,9 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
       const BatchView bv2 = with_pass(b, kPassRev2);
       HIP_OR_FAIL(c, launch_schedule(b, kPassRev2, st));
       if (b.rev2_one64 > 0)
-        HIP_OR_FAIL(c, launch_fast(bv2, 64, b.rev2_one64, grid_cap(c->n_fast, 4, c->occ_fast[1][b.rev2_one64 - 1]), st));
+        HIP_OR_FAIL(c, fast(bv2, 64, b.rev2_one64, false, c->n_fast));
       if (b.rev2_one16 > 0)
-        HIP_OR_FAIL(c, launch_fast(bv2, 16, b.rev2_one16, grid_cap(c->n_fast, 16, c->occ_fast[0][b.rev2_one16 - 1]), st));
+        HIP_OR_FAIL(c, fast(bv2, 16, b.rev2_one16, false, c->n_fast));
     }
   }
   if (ev) HIP_OR_FAIL(c, hipEventRecord(c->ev[4], st));

@@ -20,20 +20,22 @@
 //    2Pl+P .. 2Pl+2P-1).  The halves are two stages of a systolic array: at
 //    step t the low half of lane l works on column t-2l and the high half on
 //    column t-2l-1, so every dependency crosses one stage per step and is
-//    carried by one DPP row/wave shift.  Scores are kept multiplied by 16; the
-//    low 4 bits of a running per-row maximum hold 15 - (t mod 16), so the
-//    maximum also names the first step that reached it.  Every 16 steps the
-//    per-row maxima are folded into 32-bit (score, first step) records.  Gap
-//    values are clamped at 0 with unsigned saturating subtracts (as SSW's
-//    are), so H = max(d, E, F) needs no extra max with 0: 12 VALU ops per
-//    packed register per step, i.e. 6 per cell (11 in the forward kernel,
-//    whose H = max(d, E, F) is one v_pk_maximum3_f16 over biased values).
+//    carried by one DPP row/wave shift (groups narrower than a 16-lane DPP
+//    row: a row shift and a v_perm that drops what a group's lane 0 got).
+//    Scores are kept multiplied by 16; the low 4 bits of a running per-row
+//    maximum hold 15 - (t mod 16), so the maximum also names the first step
+//    that reached it.  Every 16 steps the per-row maxima are folded into
+//    32-bit (score, first step) records.  Gap values are clamped at 0 with
+//    unsigned saturating subtracts (as SSW's are), so H = max(d, E, F) needs
+//    no extra max with 0: 12 VALU ops per packed register per step, i.e. 6
+//    per cell (10 in the forward kernel, whose H = max(d, E, F) is one
+//    v_pk_maximum3_f16 over biased values).  The forward pass runs 2, 4 or 8
+//    lanes per pair (fast_shape), the reverse pass 16 or 64.
 //  * plain (32-bit): one wavefront per pair, one row per lane, rows in tiles
 //    of 64, the last row of a tile passed to the next through LDS.
 //
-// The packed kernel's instantiations are compiled in four parts
-// (sw_fast_16_1.hip, sw_fast_16_5.hip, sw_fast_64_1.hip, sw_fast_64_5.hip:
-// this file with MHMSW_FAST_PART defined, one group size and four values of P
+// The packed kernel's instantiations are compiled in parts (sw_fast_*.hip:
+// this file with MHMSW_FAST_PART defined, one group size and a list of P
 // each), so that they build in parallel; this file alone holds everything
 // else and dispatches to them.
 #include <hip/hip_runtime.h>
@@ -490,7 +492,7 @@ __global__ void schedule_kernel(BatchView b) {
     for (int bn = kNumBins - 1; bn >= 0; bn--) tot += s->bucket_count[c * kNumBins + bn];
     s->class_count[c] = tot;
     int32_t padded = tot;
-    if (c < kClassFast64) padded = (tot + 3) & ~3;        // 4 groups per wavefront
+    if (c < kClassFast64) padded = (tot + 63) & ~63;      // up to 64 groups per wavefront
     if (c >= kClassBand) padded = (tot + 63) & ~63;       // one pair per lane
     if (c == kClassBand2) padded = (tot + 15) & ~15;      // four lanes per pair
     slots[c] = padded;
@@ -546,7 +548,8 @@ struct FastState {
   uint32_t rec_s[P];   // and the first step at which it was reached, packed u16
   uint32_t xa, xb;     // exchanged 16*H of the row above, 2 steps / 1 step ago
   uint32_t vg;         // exchanged 16*(E + gO) of the first row, clamped >= 0
-  uint32_t xs, vs;     // DPP shift results; lane 0 of a group never receives and keeps its boundary
+  uint32_t xs, vs;     // DPP shift results; lane 0 of a group never receives and keeps its boundary (G >= 16)
+  uint32_t xsel;       // G < 16: v_perm selector joining the received and own halves (see fast_cells)
   uint32_t q;          // column codes (low: this step, high: last step)
   uint32_t w;          // next 8 column codes
   uint32_t dcur, dnext;  // code dwords feeding the next refill
@@ -602,11 +605,25 @@ __device__ __forceinline__ void fast_cells(FastState<G, P>& st, uint32_t t1, uin
     vg = pk_max(pk_subu_sat(vg, ge), h);
     st.best[p] = pk_maxu(st.best[p], h | tag);
   }
+  // The next step's values above the lane's two first rows: the low half
+  // (row 2Pl - 1) is the high half of the lane below's last register, the high
+  // half (row 2Pl + P - 1) the low half of its own.  Lane 0 of a group reads
+  // the boundary (0) instead.
   const uint32_t last = st.ho[P - 1];
-  st.xs = shift_in<G>(st.xs, last);
-  st.vs = shift_in<G>(st.vs, vg);
-  const uint32_t xnew = __builtin_amdgcn_alignbit(last, st.xs, 16);
-  const uint32_t vnew = __builtin_amdgcn_alignbit(vg, st.vs, 16);
+  uint32_t xnew, vnew;
+  if constexpr (G < 16) {
+    // row_shr:1 crosses group edges inside a row; the perm selector of a
+    // group's lane 0 writes zeros in place of what it received
+    const uint32_t xs = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)last, 0x111, 0xF, 0xF, true);
+    const uint32_t vs = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)vg, 0x111, 0xF, 0xF, true);
+    xnew = __builtin_amdgcn_perm(last, xs, st.xsel);
+    vnew = __builtin_amdgcn_perm(vg, vs, st.xsel);
+  } else {
+    st.xs = shift_in<G>(st.xs, last);
+    st.vs = shift_in<G>(st.vs, vg);
+    xnew = __builtin_amdgcn_alignbit(last, st.xs, 16);
+    vnew = __builtin_amdgcn_alignbit(vg, st.vs, 16);
+  }
   st.xa = st.xb;
   st.xb = xnew;
   st.vg = vnew;
@@ -731,7 +748,7 @@ __device__ __forceinline__ void fast_run(FastState<G, P>& st, int nsteps, int mi
 }
 
 template <int G, int P>
-__device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
+__device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const bool active = task.pair >= 0;
   const int32_t row_len = active ? task.row_len : 0;
   const int32_t col_len = active ? task.col_len : 0;
@@ -773,12 +790,19 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const bool rev = __ballot(task.flags & kTaskReverse) != 0;
   FastState<G, P> st;
   // this lane's 2P row codes are consecutive nibbles (descending for a
-  // reverse task): three dword loads, then nibble m of rw1:rw0 is the row
-  // code m (forward) or 2P-1-m (reverse)
+  // reverse task): NRW + 1 dword loads, then nibble m of the rw words is the
+  // row code m (forward) or 2P-1-m (reverse)
+  constexpr int NRW = (2 * P + 7) / 8;
   const int64_t r0 = rev ? task.row_start - 2 * P * l - (2 * P - 1) : task.row_start + 2 * P * l;
   const uint32_t rsh = (uint32_t)(r0 & 7) * 4u;
-  const uint32_t a0 = codes[r0 >> 3], a1 = codes[(r0 >> 3) + 1], a2 = codes[(r0 >> 3) + 2];
-  const uint32_t rw0 = __builtin_amdgcn_alignbit(a1, a0, rsh), rw1 = __builtin_amdgcn_alignbit(a2, a1, rsh);
+  uint32_t rw[NRW];
+  {
+    uint32_t ra[NRW + 1];
+#pragma unroll
+    for (int i = 0; i <= NRW; i++) ra[i] = codes[(r0 >> 3) + i];
+#pragma unroll
+    for (int i = 0; i < NRW; i++) rw[i] = __builtin_amdgcn_alignbit(ra[i + 1], ra[i], rsh);
+  }
   bool has_n = false;  // an N among this lane's rows, or its group's columns
   int32_t row0 = 2 * P * l;  // (not hoisted out of the task loop, as below)
   __asm__ volatile("" : "+v"(row0));
@@ -786,8 +810,8 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   for (int p = 0; p < P; p++) {
     const int rlo = row0 + p, rhi = row0 + P + p;
     const int mlo = rev ? 2 * P - 1 - p : p, mhi = rev ? P - 1 - p : P + p;
-    const uint32_t nlo = ((mlo < 8 ? rw0 : rw1) >> (4 * (mlo & 7))) & 0xFu;
-    const uint32_t nhi = ((mhi < 8 ? rw0 : rw1) >> (4 * (mhi & 7))) & 0xFu;
+    const uint32_t nlo = (rw[mlo >> 3] >> (4 * (mlo & 7))) & 0xFu;
+    const uint32_t nhi = (rw[mhi >> 3] >> (4 * (mhi & 7))) & 0xFu;
     const uint32_t clo = rlo < row_len ? (nlo | kRowHi) : kKillRowN;
     const uint32_t chi = rhi < row_len ? (nhi | kRowHi) : kKillRowN;
     has_n = has_n || (rlo < row_len && nlo == (uint32_t)kCodeN) || (rhi < row_len && nhi == (uint32_t)kCodeN);
@@ -803,6 +827,7 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   st.vg = 0;  // boundary E' (E = -gO: never above 0, so never part of an H)
   st.xs = 0;
   st.vs = 0;
+  st.xsel = l == 0 ? 0x05040C0Cu : 0x05040302u;  // (own low half << 16) | received high half
   st.q = (kKillColN << 16) | kKillColN;
   // column stream: at step t the low half reads column t - 2l, i.e. code
   // index cb + t (forward) or cb - t (reverse).  Forward: dw indexes
@@ -948,13 +973,22 @@ __device__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
 // Waves per SIMD the compiler must leave room for (168 VGPRs for 3)
 template <int G, int P>
 constexpr int fast_min_waves() {
-  return P >= 7 ? 3 : 1;
+  return P > 8 ? 2 : P >= 7 ? 3 : 1;
+}
+
+// The packed class of rows <= 32 Pc (kClassFast16 + Pc - 1) runs G lanes per
+// pair with P = 16 Pc / G packed registers each; 64-lane groups serve the
+// kClassFast64 classes.
+template <int G, int P>
+constexpr int fast_class_of() {
+  return G == 64 ? kClassFast64 + P - 1 : kClassFast16 + G * P / 16 - 1;
 }
 
 template <int G, int P>
 __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(BatchView b) {
   constexpr int GPW = 64 / G;
-  const int cls = (G == 16 ? kClassFast16 : kClassFast64) + P - 1;
+  static_assert(G == 64 || (G >= 2 && 16 % G == 0 && G * P % 16 == 0 && G * P / 16 <= kMaxP), "a packed class's shape");
+  const int cls = fast_class_of<G, P>();
   const int lane = threadIdx.x & 63;
   const int l = lane & (G - 1);
   const int grp = lane / G;
@@ -964,16 +998,23 @@ __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(
   // counter (the rest still take every task: each loops until it runs out)
   if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;
   const int32_t base = s->class_base[cls];
-  // The next task is fetched before the current one is aligned.
+  // The next task's number is taken before the current one is aligned, and
+  // (16- and 64-lane groups) its record loaded; the narrower groups, whose
+  // pairs hold more registers, load it afterwards.
   int w = next_wave_task(&s->class_next[cls], lane);
   SwTask task;
   if (w < nwt) task = b.tasks[base + w * GPW + grp];
   while (w < nwt) {
     const int wn = next_wave_task(&s->class_next[cls], lane);
-    SwTask next;
-    if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
-    fast_pair<G, P>(b, task, l);
-    task = next;
+    if constexpr (G >= 16) {
+      SwTask next;
+      if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
+      fast_pair<G, P>(b, task, l);
+      task = next;
+    } else {
+      fast_pair<G, P>(b, task, l);
+      if (wn < nwt) task = b.tasks[base + wn * GPW + grp];
+    }
     w = wn;
   }
 }
@@ -1692,9 +1733,32 @@ hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st) {
-  if (P >= 1 && P <= 4) return group == 16 ? launch_fast_16_1(b, P, grid, st) : launch_fast_64_1(b, P, grid, st);
-  if (P >= 5 && P <= kMaxP) return group == 16 ? launch_fast_16_5(b, P, grid, st) : launch_fast_64_5(b, P, grid, st);
+// Forward pass, rows <= 32P: G lanes of 16P/G packed registers per pair,
+// the fewest lanes whose registers fit two wavefronts per SIMD.  Fewer lanes
+// per pair mean fewer pipeline fill steps per pair (2G - 1) and the per-step
+// exchange spread over more cells: on the length-skewed 4M-pair batch the
+// 150-row class (4 x 20) runs 11% faster than as 16 x 5, the 250-row class
+// (8 x 16) 7% faster than as 16 x 8.  A class too small to give every SIMD
+// two such wavefronts keeps 16-lane groups (four times the wavefronts): a
+// 10k-pair batch of 150-row reads ran its forward pass in 0.35 ms as 4 x 20
+// and 0.23 ms as 16 x 5.  The reverse pass's rectangles (shorter and less
+// even) keep 16-lane groups, which measured faster there.
+FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus) {
+  if (group == 64 || !forward) return FastShape{group, P};
+  static constexpr int kLanes[kMaxP] = {2, 2, 4, 4, 4, 8, 8, 8};
+  const int G = kLanes[P - 1];
+  if (pairs * G < (int64_t)64 * 2 * 4 * cus) return FastShape{16, P};  // 2 wavefronts on each of 4 SIMDs
+  return FastShape{G, 16 * P / G};
+}
+
+hipError_t launch_fast(const BatchView& b, FastShape s, int grid, hipStream_t st) {
+  switch (s.lanes) {
+    case 2: return launch_fast_2_8(b, s.regs, grid, st);
+    case 4: return launch_fast_4_12(b, s.regs, grid, st);
+    case 8: return launch_fast_8_12(b, s.regs, grid, st);
+    case 16: return s.regs <= 4 ? launch_fast_16_1(b, s.regs, grid, st) : launch_fast_16_5(b, s.regs, grid, st);
+    case 64: return s.regs <= 4 ? launch_fast_64_1(b, s.regs, grid, st) : launch_fast_64_5(b, s.regs, grid, st);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -1728,10 +1792,14 @@ static hipError_t occ(K kernel, int* blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel, 256, 0);
 }
 
-hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu) {
-  if (P >= 1 && P <= 4) return group == 16 ? occupancy_fast_16_1(P, blocks_per_cu) : occupancy_fast_64_1(P, blocks_per_cu);
-  if (P >= 5 && P <= kMaxP)
-    return group == 16 ? occupancy_fast_16_5(P, blocks_per_cu) : occupancy_fast_64_5(P, blocks_per_cu);
+hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu) {
+  switch (s.lanes) {
+    case 2: return occupancy_fast_2_8(s.regs, blocks_per_cu);
+    case 4: return occupancy_fast_4_12(s.regs, blocks_per_cu);
+    case 8: return occupancy_fast_8_12(s.regs, blocks_per_cu);
+    case 16: return s.regs <= 4 ? occupancy_fast_16_1(s.regs, blocks_per_cu) : occupancy_fast_16_5(s.regs, blocks_per_cu);
+    case 64: return s.regs <= 4 ? occupancy_fast_64_1(s.regs, blocks_per_cu) : occupancy_fast_64_5(s.regs, blocks_per_cu);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -1750,37 +1818,43 @@ hipError_t band_kernel_occupancy(int P, int* blocks_per_cu) {
 
 #else  // MHMSW_FAST_PART: one part of the packed kernels' instantiations
 
-template <int G, int P0, int P1>
-static hipError_t launch_fast_range(const BatchView& b, int P, int grid, hipStream_t st) {
-  if constexpr (P0 > P1) {
-    return hipErrorInvalidValue;
-  } else {
-    if (P == P0) {
-      hipLaunchKernelGGL((sw_fast_kernel<G, P0>), dim3(grid), dim3(256), 0, st, b);
-      return hipGetLastError();
-    }
-    return launch_fast_range<G, P0 + 1, P1>(b, P, grid, st);
-  }
+template <int G, int Pk>
+static bool launch_fast_one(int P, const BatchView& b, int grid, hipStream_t st, hipError_t& e) {
+  if (P != Pk) return false;
+  hipLaunchKernelGGL((sw_fast_kernel<G, Pk>), dim3(grid), dim3(256), 0, st, b);
+  e = hipGetLastError();
+  return true;
 }
 
-template <int G, int P0, int P1>
-static hipError_t occupancy_fast_range(int P, int* blocks_per_cu) {
-  if constexpr (P0 > P1) {
-    return hipErrorInvalidValue;
-  } else {
-    if (P == P0) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sw_fast_kernel<G, P0>, 256, 0);
-    return occupancy_fast_range<G, P0 + 1, P1>(P, blocks_per_cu);
-  }
+template <int G, int Pk>
+static bool occupancy_fast_one(int P, int* blocks_per_cu, hipError_t& e) {
+  if (P != Pk) return false;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, sw_fast_kernel<G, Pk>, 256, 0);
+  return true;
+}
+
+template <int G, int... Ps>
+static hipError_t launch_fast_list(const BatchView& b, int P, int grid, hipStream_t st) {
+  hipError_t e = hipErrorInvalidValue;
+  (void)(launch_fast_one<G, Ps>(P, b, grid, st, e) || ...);
+  return e;
+}
+
+template <int G, int... Ps>
+static hipError_t occupancy_fast_list(int P, int* blocks_per_cu) {
+  hipError_t e = hipErrorInvalidValue;
+  (void)(occupancy_fast_one<G, Ps>(P, blocks_per_cu, e) || ...);
+  return e;
 }
 
 #define MHMSW_PART_NAME_(f, g, p) f##_##g##_##p
 #define MHMSW_PART_NAME(f, g, p) MHMSW_PART_NAME_(f, g, p)
 hipError_t MHMSW_PART_NAME(launch_fast, MHMSW_FAST_G, MHMSW_FAST_P0)(const BatchView& b, int P, int grid,
                                                                      hipStream_t st) {
-  return launch_fast_range<MHMSW_FAST_G, MHMSW_FAST_P0, MHMSW_FAST_P1>(b, P, grid, st);
+  return launch_fast_list<MHMSW_FAST_G, MHMSW_FAST_PS>(b, P, grid, st);
 }
 hipError_t MHMSW_PART_NAME(occupancy_fast, MHMSW_FAST_G, MHMSW_FAST_P0)(int P, int* blocks_per_cu) {
-  return occupancy_fast_range<MHMSW_FAST_G, MHMSW_FAST_P0, MHMSW_FAST_P1>(P, blocks_per_cu);
+  return occupancy_fast_list<MHMSW_FAST_G, MHMSW_FAST_PS>(P, blocks_per_cu);
 }
 #undef MHMSW_PART_NAME
 #undef MHMSW_PART_NAME_

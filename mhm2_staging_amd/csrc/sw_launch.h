@@ -99,7 +99,14 @@ hipError_t launch_clear_job(const BatchView& b, hipStream_t st);
 hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit + N mask -> nibble codes
 hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
-hipError_t launch_fast(const BatchView& b, int group, int P, int grid, hipStream_t st);
+// The packed kernel that serves class (group, P) of a pass over `pairs`
+// pairs on `cus` CUs: `lanes` lanes per pair with `regs` packed registers
+// each (lanes * regs = group * P)
+struct FastShape {
+  int lanes, regs;
+};
+FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus);
+hipError_t launch_fast(const BatchView& b, FastShape s, int grid, hipStream_t st);
 // reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1), 16 (four lanes
 // of P = 4 per pair, |l - k| <= 31), kBandDiag (the single-diagonal class,
 // |l - k| = 0) or 0 (every class up to P = 8 in one launch)
@@ -108,13 +115,20 @@ hipError_t launch_band(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_plain(const BatchView& b, int grid, int max_cols, hipStream_t st);
 hipError_t launch_finalize(const BatchView& b, hipStream_t st);
 // 256-thread blocks of a kernel that fit one CU at once (its registers decide)
-hipError_t fast_kernel_occupancy(int group, int P, int* blocks_per_cu);
-// the packed kernel's instantiations, built in parts (sw_fast_<G>_<P0>.hip for
-// P0 .. P0 + 3); launch_fast and fast_kernel_occupancy dispatch to them
+hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu);
+// the packed kernel's instantiations, built in parts (sw_fast_<G>_<P0>.hip:
+// G lanes per pair, a list of P from P0); launch_fast and
+// fast_kernel_occupancy dispatch to them
+hipError_t launch_fast_2_8(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_4_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_8_12(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_5(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_64_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_64_5(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t occupancy_fast_2_8(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_4_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_8_12(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_1(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_5(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_64_1(int P, int* blocks_per_cu);
