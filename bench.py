@@ -121,6 +121,15 @@ def cpu_legs(batch, scoring, res, cores, one_core_seconds=8.0):
     return out
 
 
+def kernel_of(st, k):
+    """The kernel that ran forward class k: sw_fast_kernel<lanes per pair, packed registers per lane>."""
+    lanes = st["class_lanes"][k]
+    if k >= 16 or not lanes:
+        return "sw_plain_kernel"
+    group, P = (16, k + 1) if k < 8 else (64, k - 7)
+    return f"sw_fast_kernel<{lanes},{group * P // lanes}>"
+
+
 def class_roofline(st, class_ms, peak_measured, traffic):
     """Roofline of the forward class with the most device time (HIP events around its launch on the context
     stream): achieved = 10 int16 lane-ops per cell of the class / its launch's duration."""
@@ -130,7 +139,8 @@ def class_roofline(st, class_ms, peak_measured, traffic):
     kcells = st["class_cells"][k]
     achieved = VALU_OPS_PER_CELL * kcells / (cm[k] * 1e-3) / 1e12
     name = _native.class_name(k)
-    rl = {"bound": "valu", "kernel": name, "achieved": round(achieved, 2), "peak": NOMINAL_PEAK,
+    rl = {"bound": "valu", "kernel": kernel_of(st, k), "class": name, "achieved": round(achieved, 2),
+          "peak": NOMINAL_PEAK,
           "unit": "TOP/s (int16 lane-ops)", "frac": round(achieved / NOMINAL_PEAK, 4), "traffic": None,
           "kernel_ms": round(float(cm[k]), 3), "kernel_cells": int(kcells), "kernel_pairs": int(st["class_pairs"][k]),
           "ops_per_cell": VALU_OPS_PER_CELL}
@@ -141,7 +151,7 @@ def class_roofline(st, class_ms, peak_measured, traffic):
         rl["traffic"] = traffic["per_launch_bytes"][name]
         rl["traffic_source"] = traffic["source"]
     classes = {_native.class_name(i): {"pairs": st["class_pairs"][i], "cells": st["class_cells"][i],
-                                       "ms": round(float(cm[i]), 3)}
+                                       "ms": round(float(cm[i]), 3), "kernel": kernel_of(st, i)}
                for i in range(_native.NUM_CLASSES) if st["class_pairs"][i]}
     return rl, classes
 

@@ -91,6 +91,10 @@ typedef struct {
    * 2-bit codes, laying out the pinned staging and enqueueing its copies, and
    * enqueueing the job */
   double host_pack_ms, host_upload_ms, host_launch_ms;
+  /* lanes per pair of the packed kernel that ran each forward class (2, 4,
+   * 8, 16 or 64; 0 for the 32-bit class): a class of many pairs runs
+   * narrower groups with more rows per lane */
+  int32_t class_lanes[MHMSW_NUM_CLASSES];
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,

@@ -46,7 +46,8 @@ class Stats(ctypes.Structure):
                 ("class_ms", ctypes.c_double * 17), ("cigar_ms", ctypes.c_double), ("cigar_reruns", ctypes.c_int64),
                 ("window_ms", ctypes.c_double), ("window_bases", ctypes.c_int64), ("packed_scoring", ctypes.c_int32),
                 ("device_bytes", ctypes.c_int64), ("host_pack_ms", ctypes.c_double),
-                ("host_upload_ms", ctypes.c_double), ("host_launch_ms", ctypes.c_double)]
+                ("host_upload_ms", ctypes.c_double), ("host_launch_ms", ctypes.c_double),
+                ("class_lanes", ctypes.c_int32 * 17)]
 
     def as_dict(self):
         d = {}

@@ -1602,6 +1602,10 @@ static int complete(mhmsw_ctx* c) {
     if (c->class_launched[k] && !c->last_job_graph)
       HIP_OR_FAIL(c, hipEventElapsedTime(&cm, c->cev[k][0], c->cev[k][1]));
     s.class_ms[k] = cm;
+    s.class_lanes[k] = k == kClassPlain ? 0
+                                        : fast_shape(k < kClassFast64 ? 16 : 64, k < kClassFast64 ? k + 1 : k - kClassFast64 + 1,
+                                                     true, c->class_pairs[k], c->cus)
+                                              .lanes;
   }
   s.prep_ms = ms[1];
   float wm = 0.f;
