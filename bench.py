@@ -126,8 +126,7 @@ def kernel_of(st, k):
     lanes = st["class_lanes"][k]
     if k >= 16 or not lanes:
         return "sw_plain_kernel"
-    group, P = (16, k + 1) if k < 8 else (64, k - 7)
-    return f"sw_fast_kernel<{lanes},{group * P // lanes}>"
+    return f"sw_fast_kernel<{lanes},{st['class_regs'][k]}>"
 
 
 def class_roofline(st, class_ms, peak_measured, traffic):

@@ -55,9 +55,11 @@ typedef struct {
   int32_t ctg_begin, ctg_end;
 } mhmsw_result;
 
-/* Work classes of the forward pass: 0..7 packed 16-bit with 16-lane groups
- * and P = 1..8 register pairs per lane (rows <= 32*P), 8..15 packed 16-bit
- * with 64-lane groups (rows <= 128*P), 16 the 32-bit form. */
+/* Work classes of the forward pass: 0..7 packed 16-bit by rows of 16 lanes
+ * of P = 1..8 register pairs (class P-1: rows <= 32*P, except that class 4
+ * ends at 152 rows and class 5 holds 153..192; mhmsw_stats.class_lanes /
+ * class_regs give the kernel shape that ran each), 8..15 packed 16-bit with
+ * 64-lane groups (rows <= 128*P), 16 the 32-bit form. */
 #define MHMSW_NUM_CLASSES 17
 
 /* Per-batch counters of the last completed batch. */
@@ -91,10 +93,12 @@ typedef struct {
    * 2-bit codes, laying out the pinned staging and enqueueing its copies, and
    * enqueueing the job */
   double host_pack_ms, host_upload_ms, host_launch_ms;
-  /* lanes per pair of the packed kernel that ran each forward class (2, 4,
-   * 8, 16 or 64; 0 for the 32-bit class): a class of many pairs runs
-   * narrower groups with more rows per lane */
+  /* the packed kernel that ran each forward class: lanes per pair (2, 4, 8,
+   * 16 or 64) and packed registers (2 rows each) per lane; 0 for the 32-bit
+   * class.  A class of many pairs runs narrower groups with more rows per
+   * lane, and one whose rows are all short enough one register fewer. */
   int32_t class_lanes[MHMSW_NUM_CLASSES];
+  int32_t class_regs[MHMSW_NUM_CLASSES];
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,

@@ -88,6 +88,7 @@ struct mhmsw_ctx {
   // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
   // (all-class kernel, then P = 1, 2, 4, 8)
   int occ_fast[2][2][kMaxP] = {};  // [16 or 64 lanes per pair][64-lane class][P - 1]
+  int occ_trim[kMaxP] = {};        // a 16-lane class's trimmed narrow shape (0: none)
   int occ_band[7] = {};  // all, P = 1, 2, 4, 8, 16 (four-lane pairs), the diagonal class
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
@@ -136,6 +137,7 @@ struct mhmsw_ctx {
   bool class_launched[kNumClasses] = {};
   int64_t class_pairs[kNumClasses] = {};
   int64_t class_cells[kNumClasses] = {};
+  int32_t class_max_rows[kNumClasses] = {};  // longest row side of a class's pairs (route_rows)
   mhmsw_stats stats{};
   int32_t h_errors[4] = {};
   // An upload goes through pinned staging with asynchronous copies on the
@@ -300,13 +302,19 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
   }
   std::memset(c->tail, 0, sizeof(mhmsw_ctx::JobTail));
   // every kernel shape a class can run as: the narrow forward shape (a class
-  // of many pairs) and the class's own group size
+  // of many pairs), its trimmed form (a class of short enough rows) and the
+  // class's own group size
   for (int r = 0; r < 2; r++)
     for (int g = 0; g < 2; g++)
       for (int P = 1; P <= kMaxP; P++)
-        if (fast_kernel_occupancy(fast_shape(g ? 64 : 16, P, r == 0, INT64_MAX / 64, c->cus),
+        if (fast_kernel_occupancy(fast_shape(g ? 64 : 16, P, r == 0, INT64_MAX / 64, c->cus, INT32_MAX),
                                   &c->occ_fast[r][g][P - 1]) != hipSuccess)
           c->occ_fast[r][g][P - 1] = 0;
+  for (int P = 1; P <= kMaxP; P++) {
+    const FastShape full = fast_shape(16, P, true, INT64_MAX / 64, c->cus, INT32_MAX);
+    const FastShape trim = fast_shape(16, P, true, INT64_MAX / 64, c->cus, 1);
+    if (trim.regs == full.regs || fast_kernel_occupancy(trim, &c->occ_trim[P - 1]) != hipSuccess) c->occ_trim[P - 1] = 0;
+  }
   for (int i = 0; i < 7; i++)
     if (band_kernel_occupancy(i == 6 ? kBandDiag : i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess)
       c->occ_band[i] = 0;
@@ -448,20 +456,25 @@ struct PackParams {
   int score2 = 0;
 };
 
+// rows of a pair's forward rectangle: the side route_class sizes its class by
+static int32_t route_rows(const PackParams& pp, int32_t lref, int32_t lqry) {
+  return pp.score2 ? lref : std::min(lref, lqry);
+}
+
 // host copy of make_task's forward routing (sw_kernels.hip)
 static int route_class(const PackParams& pp, int32_t lref, int32_t lqry) {
   if (lref <= 0 || lqry <= 0) return -1;
   const bool wide = pp.mode == MHMSW_MODE_PLAIN || !pp.fast_ok;
   if (pp.score2) {  // rows are SSW-ref
     if (!wide && lref <= kFast16MaxRows && lref < pp.g64_min_rows && lqry + kMaxQueryPad <= kFastMaxCols)
-      return kClassFast16 + (lref + 31) / 32 - 1;
+      return kClassFast16 + fast16_class_of(lref);
     if (!wide && lref <= kFast64MaxRows && lqry + kMaxQueryPad <= kFastMaxCols)
       return kClassFast64 + (lref + 127) / 128 - 1;
     return kClassPlain;
   }
   const int32_t shorter = std::min(lref, lqry), longer = std::max(lref, lqry);
   if (!wide && shorter <= kFast16MaxRows && shorter < pp.g64_min_rows && longer <= kFastMaxCols)
-    return kClassFast16 + (shorter + 31) / 32 - 1;
+    return kClassFast16 + fast16_class_of(shorter);
   if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) return kClassFast64 + (shorter + 127) / 128 - 1;
   return kClassPlain;
 }
@@ -497,6 +510,7 @@ struct PackedBatch {
   int max_read_len = 0, max_ctg_len = 0;
   bool used[kNumClasses] = {};
   int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
+  int32_t crows[kNumClasses] = {};
   int64_t nfast = 0, nplain = 0;
   std::vector<int64_t, UninitAlloc<int64_t>> base;  // [2 * npairs] first base of each sequence
   std::vector<int32_t, UninitAlloc<int32_t>> len;   // [2 * npairs]
@@ -583,6 +597,7 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
     int max_plain_cols = 0, max_rl = 0, max_cl = 0;
     bool used[kNumClasses] = {};
     int64_t cpairs[kNumClasses] = {}, ccells[kNumClasses] = {};
+    int32_t crows[kNumClasses] = {};
     int64_t bad = -1;  // first invalid pair of the chunk
     std::string err;
   };
@@ -699,6 +714,7 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
           c.used[cls] = true;
           c.cpairs[cls]++;
           c.ccells[cls] += (int64_t)rl * cl;
+          c.crows[cls] = std::max(c.crows[cls], route_rows(pp, lref, lqry));
           if (cls == kClassPlain) {
             c.nplain++;
           } else {
@@ -724,6 +740,7 @@ static int pack_batch(const PackParams& pp, const StageInput& in, PackedBatch& p
       pb.used[q] = pb.used[q] || c.used[q];
       pb.cpairs[q] += c.cpairs[q];
       pb.ccells[q] += c.ccells[q];
+      pb.crows[q] = std::max(pb.crows[q], c.crows[q]);
     }
     pb.nfast += c.nfast;
     pb.nplain += c.nplain;
@@ -753,6 +770,7 @@ static void reroute(mhmsw_ctx* c) {
     c->fast_class_used[k] = false;
     c->class_pairs[k] = 0;
     c->class_cells[k] = 0;
+    c->class_max_rows[k] = 0;
   }
   c->n_fast = c->n_plain = 0;
   for (int64_t p = 0; p < c->npairs; p++) {
@@ -763,6 +781,7 @@ static void reroute(mhmsw_ctx* c) {
     c->fast_class_used[cls] = true;
     c->class_pairs[cls]++;
     c->class_cells[cls] += (int64_t)lref * lqry;
+    c->class_max_rows[cls] = std::max(c->class_max_rows[cls], route_rows(pp, lref, lqry));
     if (cls == kClassPlain) {
       c->n_plain++;
     } else {
@@ -899,6 +918,7 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
   std::memset(c->fast_class_used, 0, sizeof(c->fast_class_used));
   std::memset(c->class_pairs, 0, sizeof(c->class_pairs));
   std::memset(c->class_cells, 0, sizeof(c->class_cells));
+  std::memset(c->class_max_rows, 0, sizeof(c->class_max_rows));
   for (size_t k = 0; k < nparts; k++) {
     const PackedBatch& pb = *parts[k];
     npairs += pb.npairs;
@@ -913,6 +933,7 @@ static int upload_parts(mhmsw_ctx* c, const PackedBatch* const* parts, size_t np
       c->fast_class_used[q] = c->fast_class_used[q] || pb.used[q];
       c->class_pairs[q] += pb.cpairs[q];
       c->class_cells[q] += pb.ccells[q];
+      c->class_max_rows[q] = std::max(c->class_max_rows[q], pb.crows[q]);
     }
   }
   const int64_t nseq = 2 * npairs;
@@ -1272,8 +1293,12 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   // a packed class's launch: the kernel shape of its pass, 4 wavefronts of
   // 64 / lanes pairs per block
   auto fast = [&](const BatchView& v, int group, int P, bool forward, int64_t pairs) {
-    const FastShape s = fast_shape(group, P, forward, pairs, c->cus);
-    return launch_fast(v, s, grid_cap(pairs, 256 / s.lanes, c->occ_fast[s.lanes >= 16][group == 64][P - 1]), st);
+    const int k = (group == 64 ? kClassFast64 : kClassFast16) + P - 1;
+    const FastShape s = fast_shape(group, P, forward, pairs, c->cus, c->class_max_rows[k]);
+    const int occ = s.lanes >= 16 ? c->occ_fast[1][group == 64][P - 1]
+                    : s.lanes * s.regs == group * P ? c->occ_fast[0][group == 64][P - 1]
+                                                    : c->occ_trim[P - 1];
+    return launch_fast(v, s, grid_cap(pairs, 256 / s.lanes, occ), st);
   };
   // the band kernels' grids do not follow it either: 2 blocks (8 wavefronts)
   // per CU, or fewer where a kernel keeps fewer resident.  More waves per CU
@@ -1445,18 +1470,20 @@ static int align_staged_impl(mhmsw_ctx* c) {
     // side, which with the scoring in the view also decides whether the rerun
     // pass runs), mode, grid tuning, the CIGAR pass and the longest sequence
     // (whether reverse rectangles can need the 32-bit form)
-    const size_t kv = sizeof(BatchView), kc = sizeof(c->fast_class_used), kq = sizeof(c->class_pairs);
+    const size_t kv = sizeof(BatchView), kc = sizeof(c->fast_class_used), kq = sizeof(c->class_pairs),
+                 kr = sizeof(c->class_max_rows);
     const int64_t extra[5] = {plain_lds_cols(c), c->mode * 64 + c->tuning.blocks_per_cu, c->cigar, c->n_fast,
                               (int64_t)std::max(c->max_read_len, c->max_ctg_len)};
-    key.resize(kv + kc + kq + sizeof(extra) + sizeof(CigarView));
+    key.resize(kv + kc + kq + kr + sizeof(extra) + sizeof(CigarView));
     unsigned char* kp = key.data();
     std::memcpy(kp, &b, kv);
     std::memcpy(kp + kv, c->fast_class_used, kc);
     std::memcpy(kp + kv + kc, c->class_pairs, kq);
-    std::memcpy(kp + kv + kc + kq, extra, sizeof(extra));
+    std::memcpy(kp + kv + kc + kq, c->class_max_rows, kr);
+    std::memcpy(kp + kv + kc + kq + kr, extra, sizeof(extra));
     if (c->cigar) {
       const CigarView cv = make_cigar_view(c);
-      std::memcpy(kp + kv + kc + kq + sizeof(extra), &cv, sizeof(cv));
+      std::memcpy(kp + kv + kc + kq + kr + sizeof(extra), &cv, sizeof(cv));
     }
     graph = (c->graph_exec && key == c->graph_key) || key == c->eager_key;
   }
@@ -1602,10 +1629,12 @@ static int complete(mhmsw_ctx* c) {
     if (c->class_launched[k] && !c->last_job_graph)
       HIP_OR_FAIL(c, hipEventElapsedTime(&cm, c->cev[k][0], c->cev[k][1]));
     s.class_ms[k] = cm;
-    s.class_lanes[k] = k == kClassPlain ? 0
-                                        : fast_shape(k < kClassFast64 ? 16 : 64, k < kClassFast64 ? k + 1 : k - kClassFast64 + 1,
-                                                     true, c->class_pairs[k], c->cus)
-                                              .lanes;
+    const FastShape sh = k == kClassPlain ? FastShape{0, 0}
+                                          : fast_shape(k < kClassFast64 ? 16 : 64,
+                                                       k < kClassFast64 ? k + 1 : k - kClassFast64 + 1, true,
+                                                       c->class_pairs[k], c->cus, c->class_max_rows[k]);
+    s.class_lanes[k] = sh.lanes;
+    s.class_regs[k] = sh.regs;
   }
   s.prep_ms = ms[1];
   float wm = 0.f;

@@ -311,14 +311,14 @@ extern "C" int mhmsw_submit_noalloc(mhmsw_ctx* ctx, int64_t npairs, const char* 
     const int plain_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ci.cus * 16, npairs));
     const BatchView bf = with_pass(b, kPassFwd), br = with_pass(b, kPassRerun), bv = with_pass(b, kPassRev);
     NA_CHECK(launch_schedule(b, kPassFwd, st), "forward schedule");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bf, fast_shape(64, P, true, npairs, ci.cus), full, st), "forward pass");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bf, fast_shape(16, P, true, npairs, ci.cus), full, st), "forward pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bf, fast_shape(64, P, true, npairs, ci.cus, INT32_MAX), full, st), "forward pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bf, fast_shape(16, P, true, npairs, ci.cus, INT32_MAX), full, st), "forward pass");
     NA_CHECK(launch_plain(bf, plain_grid, kPlainMaxCols, st), "forward pass (32-bit)");
     NA_CHECK(launch_schedule(b, kPassRerun, st), "rerun schedule");
     NA_CHECK(launch_plain(br, plain_grid, kPlainMaxCols, st), "32-bit rerun");
     NA_CHECK(launch_schedule(b, kPassRev, st), "reverse schedule");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bv, fast_shape(64, P, false, npairs, ci.cus), full, st), "reverse pass");
-    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bv, fast_shape(16, P, false, npairs, ci.cus), full, st), "reverse pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bv, fast_shape(64, P, false, npairs, ci.cus, INT32_MAX), full, st), "reverse pass");
+    for (int P = kMaxP; P >= 1; P--) NA_CHECK(launch_fast(bv, fast_shape(16, P, false, npairs, ci.cus, INT32_MAX), full, st), "reverse pass");
     if (npairs <= 65536) {
       NA_CHECK(launch_band(bv, 0, (int)std::min<int64_t>(full, (npairs + 64 * kNumBandClasses + 255) / 256), st),
                "reverse band pass");

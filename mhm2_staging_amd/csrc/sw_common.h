@@ -17,7 +17,7 @@ constexpr int64_t kPadBases = 1024;
 
 // Classes of work (one persistent kernel launch each).
 constexpr int kMaxP = 8;                 // packed registers per lane (2 rows each)
-constexpr int kClassFast16 = 0;          // + (P-1): 16-lane groups, rows <= 32*P
+constexpr int kClassFast16 = 0;          // + (P-1): 16-lane groups, rows <= 32*P (see fast16_class_of)
 constexpr int kClassFast64 = kMaxP;      // + (P-1): 64-lane groups, rows <= 128*P
 constexpr int kClassPlain = 2 * kMaxP;   // 32-bit, one wavefront per pair
 constexpr int kNumFwdClasses = 2 * kMaxP + 1;  // classes a forward pass uses
@@ -35,6 +35,16 @@ constexpr int kNumBins = 128;            // column-length bins of 32 columns
 constexpr int kNumBuckets = kNumClasses * kNumBins;
 
 constexpr int kFast16MaxRows = 32 * kMaxP;   // 256
+
+// The 16-lane class (0..7, as an offset from kClassFast16) of a pair with
+// `rows` rows (1..256): rows in (32(k-1), 32k] for class k-1, except that the
+// fifth class ends at 152 rows and the sixth holds 153..192.  The forward pass
+// runs the fifth class as 4 lanes of 19 packed registers (152 rows), which
+// 150/151-base reads fill; a 16-lane group of 5 registers (160 rows, the
+// reverse pass's shape) still holds every row of it.
+constexpr int fast16_class_of(int rows) {
+  return rows <= 128 ? (rows + 31) / 32 - 1 : rows <= 152 ? 4 : rows <= 192 ? 5 : (rows + 31) / 32 - 1;
+}
 constexpr int kFast64MaxRows = 128 * kMaxP;  // 1024
 constexpr int kFastMaxCols = 60000;          // step counter stays below 2^16
 // A packed-form pair whose score reaches this is rerun in 32-bit (the packed

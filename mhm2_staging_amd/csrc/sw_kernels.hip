@@ -352,7 +352,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     // score2 reads per-SSW-ref-position maxima: rows are SSW-ref, columns
     // SSW-query plus up to kMaxQueryPad zero-scoring columns
     if (!wide && ref_len <= kFast16MaxRows && ref_len < b.g64_min_rows && qry_len + kMaxQueryPad <= kFastMaxCols) {
-      cls = kClassFast16 + (ref_len + 31) / 32 - 1;
+      cls = kClassFast16 + fast16_class_of(ref_len);
     } else if (!wide && ref_len <= kFast64MaxRows && qry_len + kMaxQueryPad <= kFastMaxCols) {
       cls = kClassFast64 + (ref_len + 127) / 128 - 1;
     } else {
@@ -371,8 +371,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   const int32_t longer = ref_len <= qry_len ? qry_len : ref_len;
   const bool ref_rows_if_short = ref_len <= qry_len;
   if (!wide && shorter <= kFast16MaxRows && shorter < b.g64_min_rows && longer <= kFastMaxCols) {
-    const int P = (shorter + 31) / 32;
-    cls = kClassFast16 + P - 1;
+    cls = kClassFast16 + fast16_class_of(shorter);
   } else if (!wide && shorter <= kFast64MaxRows && longer <= kFastMaxCols) {
     const int P = (shorter + 127) / 128;
     cls = kClassFast64 + P - 1;
@@ -955,6 +954,9 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
       e.flags = 0;
     }
     if (hmax >= kFastOverflowScore) e.flags |= kEndOverflow;  // values may have left the exact range: rerun
+    // rows past the group's 2GP (a trimmed shape given a taller pair, which
+    // the host's class row bound rules out): rerun exactly
+    if (row_len > 2 * G * P) e.flags |= kEndOverflow;
     if (s2task) {
       if (s2_rerun) {
         e.flags |= kEndScore2;
@@ -976,18 +978,20 @@ constexpr int fast_min_waves() {
   return P > 8 ? 2 : P >= 7 ? 3 : 1;
 }
 
-// The packed class of rows <= 32 Pc (kClassFast16 + Pc - 1) runs G lanes per
-// pair with P = 16 Pc / G packed registers each; 64-lane groups serve the
-// kClassFast64 classes.
+// The 16-lane class Pc (kClassFast16 + Pc - 1, rows up to 32 Pc or the
+// fifth class's 152) runs G lanes per pair with P packed registers each:
+// 16 Pc / G, or fewer (the fifth class's 19, a trimmed shape for a class
+// whose rows are all short enough), 2GP rows still above the class below's
+// 32 (Pc - 1).  64-lane groups serve the kClassFast64 classes.
 template <int G, int P>
 constexpr int fast_class_of() {
-  return G == 64 ? kClassFast64 + P - 1 : kClassFast16 + G * P / 16 - 1;
+  return G == 64 ? kClassFast64 + P - 1 : kClassFast16 + (G * P + 15) / 16 - 1;
 }
 
 template <int G, int P>
 __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(BatchView b) {
   constexpr int GPW = 64 / G;
-  static_assert(G == 64 || (G >= 2 && 16 % G == 0 && G * P % 16 == 0 && G * P / 16 <= kMaxP), "a packed class's shape");
+  static_assert(G == 64 || (G >= 2 && 16 % G == 0 && (G * P + 15) / 16 <= kMaxP), "a packed class's shape");
   const int cls = fast_class_of<G, P>();
   const int lane = threadIdx.x & 63;
   const int l = lane & (G - 1);
@@ -1733,8 +1737,9 @@ hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Forward pass, rows <= 32P: G lanes of 16P/G packed registers per pair,
-// the fewest lanes whose registers fit two wavefronts per SIMD.  Fewer lanes
+// Forward pass, 16-lane class P (rows <= 32P, fast16_class_of): G lanes of
+// 16P/G packed registers per pair (the fifth class, rows <= 152: 19), the
+// fewest lanes whose registers fit two wavefronts per SIMD.  Fewer lanes
 // per pair mean fewer pipeline fill steps per pair (2G - 1) and the per-step
 // exchange spread over more cells: on the length-skewed 4M-pair batch the
 // 150-row class (4 x 20) runs 11% faster than as 16 x 5, the 250-row class
@@ -1743,18 +1748,23 @@ hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
 // 10k-pair batch of 150-row reads ran its forward pass in 0.35 ms as 4 x 20
 // and 0.23 ms as 16 x 5.  The reverse pass's rectangles (shorter and less
 // even) keep 16-lane groups, which measured faster there.
-FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus) {
+// A class whose rows all fit the 2G * kTrim rows of its trimmed shape runs
+// that shape: 100/101-base reads as 4 x 13 (104 rows) instead of 4 x 16 (128).
+FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus, int max_rows) {
   if (group == 64 || !forward) return FastShape{group, P};
   static constexpr int kLanes[kMaxP] = {2, 2, 4, 4, 4, 8, 8, 8};
+  static constexpr int kRegs[kMaxP] = {8, 16, 12, 16, 19, 12, 14, 16};
+  static constexpr int kTrim[kMaxP] = {0, 0, 0, 13, 0, 0, 0, 0};
   const int G = kLanes[P - 1];
   if (pairs * G < (int64_t)64 * 2 * 4 * cus) return FastShape{16, P};  // 2 wavefronts on each of 4 SIMDs
-  return FastShape{G, 16 * P / G};
+  if (kTrim[P - 1] > 0 && max_rows <= 2 * G * kTrim[P - 1]) return FastShape{G, kTrim[P - 1]};
+  return FastShape{G, kRegs[P - 1]};
 }
 
 hipError_t launch_fast(const BatchView& b, FastShape s, int grid, hipStream_t st) {
   switch (s.lanes) {
     case 2: return launch_fast_2_8(b, s.regs, grid, st);
-    case 4: return launch_fast_4_12(b, s.regs, grid, st);
+    case 4: return s.regs <= 16 ? launch_fast_4_12(b, s.regs, grid, st) : launch_fast_4_19(b, s.regs, grid, st);
     case 8: return launch_fast_8_12(b, s.regs, grid, st);
     case 16: return s.regs <= 4 ? launch_fast_16_1(b, s.regs, grid, st) : launch_fast_16_5(b, s.regs, grid, st);
     case 64: return s.regs <= 4 ? launch_fast_64_1(b, s.regs, grid, st) : launch_fast_64_5(b, s.regs, grid, st);
@@ -1795,7 +1805,7 @@ static hipError_t occ(K kernel, int* blocks_per_cu) {
 hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu) {
   switch (s.lanes) {
     case 2: return occupancy_fast_2_8(s.regs, blocks_per_cu);
-    case 4: return occupancy_fast_4_12(s.regs, blocks_per_cu);
+    case 4: return s.regs <= 16 ? occupancy_fast_4_12(s.regs, blocks_per_cu) : occupancy_fast_4_19(s.regs, blocks_per_cu);
     case 8: return occupancy_fast_8_12(s.regs, blocks_per_cu);
     case 16: return s.regs <= 4 ? occupancy_fast_16_1(s.regs, blocks_per_cu) : occupancy_fast_16_5(s.regs, blocks_per_cu);
     case 64: return s.regs <= 4 ? occupancy_fast_64_1(s.regs, blocks_per_cu) : occupancy_fast_64_5(s.regs, blocks_per_cu);

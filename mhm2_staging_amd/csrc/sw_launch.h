@@ -100,12 +100,13 @@ hipError_t launch_unpack(const BatchView& b, hipStream_t st);           // 2-bit
 hipError_t launch_extract_windows(const BatchView& b, hipStream_t st);  // windows of resident contigs (if any)
 hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st);
 // The packed kernel that serves class (group, P) of a pass over `pairs`
-// pairs on `cus` CUs: `lanes` lanes per pair with `regs` packed registers
-// each (lanes * regs = group * P)
+// pairs on `cus` CUs whose rows are at most max_rows: `lanes` lanes per pair
+// with `regs` packed registers each (lanes * regs = group * P, or fewer
+// registers when max_rows allows)
 struct FastShape {
   int lanes, regs;
 };
-FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus);
+FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus, int max_rows);
 hipError_t launch_fast(const BatchView& b, FastShape s, int grid, hipStream_t st);
 // reverse pass, banded form: P = 1, 2, 4, 8 (|l - k| <= 2P - 1), 16 (four lanes
 // of P = 4 per pair, |l - k| <= 31), kBandDiag (the single-diagonal class,
@@ -121,6 +122,7 @@ hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu);
 // fast_kernel_occupancy dispatch to them
 hipError_t launch_fast_2_8(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_4_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_4_19(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_8_12(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_5(const BatchView& b, int P, int grid, hipStream_t st);
@@ -128,6 +130,7 @@ hipError_t launch_fast_64_1(const BatchView& b, int P, int grid, hipStream_t st)
 hipError_t launch_fast_64_5(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t occupancy_fast_2_8(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_4_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_4_19(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_8_12(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_1(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_5(int P, int* blocks_per_cu);

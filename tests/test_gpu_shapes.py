@@ -1,0 +1,90 @@
+"""Every kernel shape of the packed forward pass against the 16-lane one and the oracle.
+
+A class of many pairs runs narrow groups (2, 4 or 8 lanes per pair, 8..19
+packed registers per lane), and the 97..128-row class a trimmed shape (4 x 13)
+when its rows all fit 104; small classes run 16-lane groups.  The 129..152-row
+class runs 4 x 19 and rows 153..160 belong to the 153..192-row class.  Each
+batch below holds one read length, enough pairs for its class to take the
+narrow (or trimmed) shape; the same pairs aligned in small slices take the
+16-lane shape, and the results must agree on every pair, and with the oracle
+on a sample.  Windows carry N bases in a tenth of
+the pairs, so both substitution forms of the kernel run."""
+import numpy as np
+import pytest
+
+import mhm2_staging_amd as m
+from oracle_lib import oracle_batch
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("score", "read_begin", "read_end", "ctg_begin", "ctg_end")
+NPAIRS = 70_000    # above the narrow-shape bound of every class on 256 CUs (65,536 pairs for 2-lane groups)
+SLICE = 4_000      # below the bound of every class: 16-lane groups
+WINDOW = 300
+
+# read length -> (lanes, packed registers) of the forward kernel its class runs
+SHAPES = {30: (2, 8), 60: (2, 16), 90: (4, 12), 100: (4, 13), 104: (4, 13), 105: (4, 16), 120: (4, 16),
+          129: (4, 19), 150: (4, 19), 152: (4, 19), 153: (8, 12), 160: (8, 12), 192: (8, 12), 193: (8, 14),
+          250: (8, 16), 256: (8, 16)}
+
+
+def class_of(rows):
+    """The 16-lane class of a pair with `rows` rows (fast16_class_of in sw_common.h)."""
+    if rows <= 128:
+        return (rows + 31) // 32 - 1
+    return 4 if rows <= 152 else 5 if rows <= 192 else (rows + 31) // 32 - 1
+
+
+def batch_of(read_len, seed):
+    rng = np.random.default_rng(seed)
+    n, L, W = NPAIRS, read_len, WINDOW
+    win = rng.integers(0, 4, (n, W), dtype=np.uint8)
+    start = rng.integers(0, W - L + 1, n)
+    rd = np.take_along_axis(win, start[:, None] + np.arange(L)[None, :], 1)
+    sub = rng.random((n, L)) < 0.02
+    rd[sub] = (rd[sub] + rng.integers(1, 4, int(sub.sum()), dtype=np.uint8)) % 4
+    unrelated = rng.random(n) < 0.1
+    rd[unrelated] = rng.integers(0, 4, (int(unrelated.sum()), L), dtype=np.uint8)
+    letters = np.frombuffer(b"ACGTN", np.uint8)
+    w = letters[win]
+    with_n = np.nonzero(rng.random(n) < 0.1)[0]
+    w[with_n, rng.integers(0, W, len(with_n))] = ord("N")
+    reads = np.concatenate([letters[rd].ravel(), [0]]).astype(np.uint8)
+    ctgs = np.concatenate([w.ravel(), [0]]).astype(np.uint8)
+    return m.PairBatch(reads, np.arange(n, dtype=np.int64) * L, np.full(n, L, np.int32),
+                       ctgs, np.arange(n, dtype=np.int64) * W, np.full(n, W, np.int32))
+
+
+def table(res):
+    return np.stack([res[k] for k in FIELDS], 1)
+
+
+def sliced(b, lo, hi):
+    return m.PairBatch(b.reads, b.read_off[lo:hi], b.read_len[lo:hi], b.ctgs, b.ctg_off[lo:hi], b.ctg_len[lo:hi])
+
+
+@pytest.mark.parametrize("read_len", sorted(SHAPES))
+def test_forward_shape_matches_16_lane_groups_and_oracle(gpu_aligner, read_len):
+    print(f"[gpu] forward shape, reads of {read_len} bases, {NPAIRS} pairs", flush=True)
+    al = gpu_aligner
+    al.set_scoring(m.MHM2_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(False)
+    b = batch_of(read_len, 1000 + read_len)
+    got = table(al.align(b))
+    st = al.stats()
+    cls = class_of(read_len)
+    assert st["class_pairs"][cls] == NPAIRS
+    assert (st["class_lanes"][cls], st["class_regs"][cls]) == SHAPES[read_len]
+    want = np.concatenate([table(al.align(sliced(b, lo, min(lo + SLICE, NPAIRS))))
+                           for lo in range(0, NPAIRS, SLICE)])
+    st = al.stats()
+    assert (st["class_lanes"][cls], st["class_regs"][cls]) == (16, cls + 1)
+    assert np.array_equal(got, want)
+    assert (got[:, 0] > 0).mean() > 0.9
+    idx = np.linspace(0, NPAIRS - 1, 200).astype(np.int64)
+    o = oracle_batch(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
+                     m.MHM2_SCORING)
+    assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
+                                                              "query_end")], 1))
