@@ -87,8 +87,16 @@ struct mhmsw_ctx {
   // 256-thread blocks of each persistent kernel that fit one CU at once:
   // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
   // (all-class kernel, then P = 1, 2, 4, 8)
-  int occ_fast[2][2][kMaxP] = {};  // [16 or 64 lanes per pair][64-lane class][P - 1]
-  int occ_trim[kMaxP] = {};        // a 16-lane class's trimmed narrow shape (0: none)
+  // blocks per CU of every packed kernel shape a class can run as (0: unknown)
+  struct ShapeOcc {
+    int lanes = 0, regs = 0, occ = 0;
+  };
+  ShapeOcc occ_shapes[4 * kMaxP] = {};
+  int occ_of(FastShape s) const {
+    for (const ShapeOcc& o : occ_shapes)
+      if (o.lanes == s.lanes && o.regs == s.regs) return o.occ;
+    return 0;
+  }
   int occ_band[7] = {};  // all, P = 1, 2, 4, 8, 16 (four-lane pairs), the diagonal class
   std::vector<int32_t> lens;  // (read, contig window) length of each staged pair, for re-routing
 
@@ -301,19 +309,22 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
     return MHMSW_E_NOMEM;
   }
   std::memset(c->tail, 0, sizeof(mhmsw_ctx::JobTail));
-  // every kernel shape a class can run as: the narrow forward shape (a class
-  // of many pairs), its trimmed form (a class of short enough rows) and the
-  // class's own group size
-  for (int r = 0; r < 2; r++)
-    for (int g = 0; g < 2; g++)
-      for (int P = 1; P <= kMaxP; P++)
-        if (fast_kernel_occupancy(fast_shape(g ? 64 : 16, P, r == 0, INT64_MAX / 64, c->cus, INT32_MAX),
-                                  &c->occ_fast[r][g][P - 1]) != hipSuccess)
-          c->occ_fast[r][g][P - 1] = 0;
-  for (int P = 1; P <= kMaxP; P++) {
-    const FastShape full = fast_shape(16, P, true, INT64_MAX / 64, c->cus, INT32_MAX);
-    const FastShape trim = fast_shape(16, P, true, INT64_MAX / 64, c->cus, 1);
-    if (trim.regs == full.regs || fast_kernel_occupancy(trim, &c->occ_trim[P - 1]) != hipSuccess) c->occ_trim[P - 1] = 0;
+  // every kernel shape a class can run as: the class's own group size, the
+  // narrow forward shape (a class of many pairs) and its trimmed form (a
+  // class of short enough rows)
+  {
+    int n = 0;
+    for (int P = 1; P <= kMaxP; P++) {
+      const FastShape shapes[4] = {FastShape{16, P}, FastShape{64, P},
+                                   fast_shape(16, P, true, INT64_MAX / 64, c->cus, INT32_MAX),
+                                   fast_shape(16, P, true, INT64_MAX / 64, c->cus, 1)};
+      for (const FastShape& s : shapes) {
+        if (c->occ_of(s) > 0) continue;
+        int occ = 0;
+        if (fast_kernel_occupancy(s, &occ) != hipSuccess) occ = 0;
+        c->occ_shapes[n++] = mhmsw_ctx::ShapeOcc{s.lanes, s.regs, occ};
+      }
+    }
   }
   for (int i = 0; i < 7; i++)
     if (band_kernel_occupancy(i == 6 ? kBandDiag : i ? 1 << (i - 1) : 0, &c->occ_band[i]) != hipSuccess)
@@ -1295,10 +1306,7 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   auto fast = [&](const BatchView& v, int group, int P, bool forward, int64_t pairs) {
     const int k = (group == 64 ? kClassFast64 : kClassFast16) + P - 1;
     const FastShape s = fast_shape(group, P, forward, pairs, c->cus, c->class_max_rows[k]);
-    const int occ = s.lanes >= 16 ? c->occ_fast[1][group == 64][P - 1]
-                    : s.lanes * s.regs == group * P ? c->occ_fast[0][group == 64][P - 1]
-                                                    : c->occ_trim[P - 1];
-    return launch_fast(v, s, grid_cap(pairs, 256 / s.lanes, occ), st);
+    return launch_fast(v, s, grid_cap(pairs, 256 / s.lanes, c->occ_of(s)), st);
   };
   // the band kernels' grids do not follow it either: 2 blocks (8 wavefronts)
   // per CU, or fewer where a kernel keeps fewer resident.  More waves per CU
