@@ -7,13 +7,13 @@ class runs 4 x 19 and rows 153..160 belong to the 153..192-row class.  Each
 batch below holds one read length, enough pairs for its class to take the
 narrow (or trimmed) shape; the same pairs aligned in small slices take the
 16-lane shape, and the results must agree on every pair, and with the oracle
-on a sample.  Windows carry N bases in a tenth of
-the pairs, so both substitution forms of the kernel run."""
+on a sample.  Windows carry N bases in a tenth of the pairs, so both
+substitution forms of the kernel run."""
 import numpy as np
 import pytest
 
 import mhm2_staging_amd as m
-from oracle_lib import oracle_batch
+from oracle_lib import oracle_batch, oracle_batch2
 
 pytestmark = pytest.mark.gpu
 
@@ -88,3 +88,39 @@ def test_forward_shape_matches_16_lane_groups_and_oracle(gpu_aligner, read_len):
                      m.MHM2_SCORING)
     assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
                                                               "query_end")], 1))
+
+
+@pytest.mark.parametrize("read_len", [30, 60, 100, 150, 193, 250])
+def test_forward_shape_score2_matches_16_lane_groups_and_oracle(gpu_aligner, read_len):
+    """SSW's second-best score (score2: the padded edge chunks and the per-row maxima over a
+    group's lanes) on the narrow shapes, against 16-lane groups and the oracle."""
+    print(f"[gpu] forward shape with score2, reads of {read_len} bases, {NPAIRS} pairs", flush=True)
+    al = gpu_aligner
+    al.set_scoring(m.MHM2_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(True)
+    try:
+        b = batch_of(read_len, 2000 + read_len)
+
+        def run(batch):
+            al.stage(batch)
+            al.align_staged()
+            al.wait()
+            return table(al.results()), al.score2()
+
+        got, got2 = run(b)
+        st = al.stats()
+        cls = class_of(read_len)
+        assert (st["class_lanes"][cls], st["class_regs"][cls]) == SHAPES[read_len]
+        parts = [run(sliced(b, lo, min(lo + SLICE, NPAIRS))) for lo in range(0, NPAIRS, SLICE)]
+        assert np.array_equal(got, np.concatenate([p[0] for p in parts]))
+        assert np.array_equal(got2, np.concatenate([p[1] for p in parts]))
+        idx = np.linspace(0, NPAIRS - 1, 200).astype(np.int64)
+        o, o2 = oracle_batch2(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
+                              m.MHM2_SCORING)
+        assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
+                                                                  "query_end")], 1))
+        assert np.array_equal(got2[idx], o2)
+    finally:
+        al.set_score2(False)
