@@ -124,3 +124,31 @@ def test_forward_shape_score2_matches_16_lane_groups_and_oracle(gpu_aligner, rea
         assert np.array_equal(got2[idx], o2)
     finally:
         al.set_score2(False)
+
+
+@pytest.mark.parametrize("read_len", [60, 100, 150, 250])
+def test_forward_shape_contig_as_ssw_ref(gpu_aligner, read_len):
+    """The other orientation (the window is SSW's reference): rows are still the shorter read, but
+    the end point's tie-break runs over columns first."""
+    print(f"[gpu] forward shape, window as SSW-ref, reads of {read_len} bases", flush=True)
+    al = gpu_aligner
+    al.set_scoring(m.MHM2_SCORING)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(False)
+    al.set_orientation(False)
+    try:
+        b = batch_of(read_len, 3000 + read_len)
+        got = table(al.align(b))
+        st = al.stats()
+        cls = class_of(read_len)
+        assert (st["class_lanes"][cls], st["class_regs"][cls]) == SHAPES[read_len]
+        want = np.concatenate([table(al.align(sliced(b, lo, min(lo + SLICE, NPAIRS))))
+                               for lo in range(0, NPAIRS, SLICE)])
+        assert np.array_equal(got, want)
+        idx = np.linspace(0, NPAIRS - 1, 200).astype(np.int64)
+        o = oracle_batch(b.ctgs, b.ctg_off[idx], b.ctg_len[idx], b.reads, b.read_off[idx], b.read_len[idx],
+                         m.MHM2_SCORING)
+        assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "query_begin", "query_end", "ref_begin",
+                                                                  "ref_end")], 1))
+    finally:
+        al.set_orientation(True)
