@@ -91,7 +91,7 @@ struct mhmsw_ctx {
   struct ShapeOcc {
     int lanes = 0, regs = 0, occ = 0;
   };
-  ShapeOcc occ_shapes[4 * kMaxP] = {};
+  ShapeOcc occ_shapes[5 * kMaxP] = {};
   int occ_of(FastShape s) const {
     for (const ShapeOcc& o : occ_shapes)
       if (o.lanes == s.lanes && o.regs == s.regs) return o.occ;
@@ -315,9 +315,10 @@ int mhmsw_create(int device_id, mhmsw_ctx** out) {
   {
     int n = 0;
     for (int P = 1; P <= kMaxP; P++) {
-      const FastShape shapes[4] = {FastShape{16, P}, FastShape{64, P},
+      const FastShape shapes[5] = {FastShape{16, P}, FastShape{64, P},
                                    fast_shape(16, P, true, INT64_MAX / 64, c->cus, INT32_MAX),
-                                   fast_shape(16, P, true, INT64_MAX / 64, c->cus, 1)};
+                                   fast_shape(16, P, true, INT64_MAX / 64, c->cus, 1),
+                                   fast_shape(64, P, true, INT64_MAX / 64, c->cus, INT32_MAX)};
       for (const FastShape& s : shapes) {
         if (c->occ_of(s) > 0) continue;
         int occ = 0;

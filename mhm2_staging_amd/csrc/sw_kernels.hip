@@ -29,7 +29,7 @@
 //    unsigned saturating subtracts (as SSW's are), so H = max(d, E, F) needs
 //    no extra max with 0: 12 VALU ops per packed register per step, i.e. 6
 //    per cell (10 in the forward kernel, whose H = max(d, E, F) is one
-//    v_pk_maximum3_f16 over biased values).  The forward pass runs 2, 4 or 8
+//    v_pk_maximum3_f16 over biased values).  The forward pass runs 2 to 32
 //    lanes per pair (fast_shape), the reverse pass 16 or 64.
 //  * plain (32-bit): one wavefront per pair, one row per lane, rows in tiles
 //    of 64, the last row of a tile passed to the next through LDS.
@@ -491,7 +491,7 @@ __global__ void schedule_kernel(BatchView b) {
     for (int bn = kNumBins - 1; bn >= 0; bn--) tot += s->bucket_count[c * kNumBins + bn];
     s->class_count[c] = tot;
     int32_t padded = tot;
-    if (c < kClassFast64) padded = (tot + 63) & ~63;      // up to 64 groups per wavefront
+    if (c < kClassPlain) padded = (tot + 63) & ~63;       // up to 64 groups per wavefront
     if (c >= kClassBand) padded = (tot + 63) & ~63;       // one pair per lane
     if (c == kClassBand2) padded = (tot + 15) & ~15;      // four lanes per pair
     slots[c] = padded;
@@ -610,11 +610,12 @@ __device__ __forceinline__ void fast_cells(FastState<G, P>& st, uint32_t t1, uin
   // the boundary (0) instead.
   const uint32_t last = st.ho[P - 1];
   uint32_t xnew, vnew;
-  if constexpr (G < 16) {
-    // row_shr:1 crosses group edges inside a row; the perm selector of a
-    // group's lane 0 writes zeros in place of what it received
-    const uint32_t xs = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)last, 0x111, 0xF, 0xF, true);
-    const uint32_t vs = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)vg, 0x111, 0xF, 0xF, true);
+  if constexpr (G < 16 || G == 32) {
+    // row_shr:1 (wave_shr:1 for 32-lane groups) crosses group edges; the perm
+    // selector of a group's lane 0 writes zeros in place of what it received
+    constexpr int shr = G == 32 ? 0x138 : 0x111;
+    const uint32_t xs = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)last, shr, 0xF, 0xF, true);
+    const uint32_t vs = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)vg, shr, 0xF, 0xF, true);
     xnew = __builtin_amdgcn_perm(last, xs, st.xsel);
     vnew = __builtin_amdgcn_perm(vg, vs, st.xsel);
   } else {
@@ -978,20 +979,24 @@ constexpr int fast_min_waves() {
   return P > 8 ? 2 : P >= 7 ? 3 : 1;
 }
 
-// The 16-lane class Pc (kClassFast16 + Pc - 1, rows up to 32 Pc or the
-// fifth class's 152) runs G lanes per pair with P packed registers each:
-// 16 Pc / G, or fewer (the fifth class's 19, a trimmed shape for a class
-// whose rows are all short enough), 2GP rows still above the class below's
-// 32 (Pc - 1).  64-lane groups serve the kClassFast64 classes.
+// The class a shape of 2GP rows serves.  The 16-lane class Pc
+// (kClassFast16 + Pc - 1, rows up to 32 Pc or the fifth class's 152) runs G
+// lanes per pair with P packed registers each: 16 Pc / G, or fewer (the
+// fifth class's 19, a trimmed shape for a class whose rows are all short
+// enough), 2GP rows still above the class below's 32 (Pc - 1).  The 64-lane
+// class Pc (kClassFast64 + Pc - 1, rows up to 128 Pc) runs 64 lanes of Pc
+// registers, or (forward, rows above 256) 16 or 32 lanes of 2GP = 128 Pc rows.
 template <int G, int P>
 constexpr int fast_class_of() {
-  return G == 64 ? kClassFast64 + P - 1 : kClassFast16 + (G * P + 15) / 16 - 1;
+  return G == 64 ? kClassFast64 + P - 1
+         : 2 * G * P > 32 * kMaxP ? kClassFast64 + (2 * G * P + 127) / 128 - 1
+                                  : kClassFast16 + (G * P + 15) / 16 - 1;
 }
 
 template <int G, int P>
 __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(BatchView b) {
   constexpr int GPW = 64 / G;
-  static_assert(G == 64 || (G >= 2 && 16 % G == 0 && (G * P + 15) / 16 <= kMaxP), "a packed class's shape");
+  static_assert(G >= 2 && 64 % G == 0 && 2 * G * P <= 128 * kMaxP, "a packed class's shape");
   const int cls = fast_class_of<G, P>();
   const int lane = threadIdx.x & 63;
   const int l = lane & (G - 1);
@@ -1003,14 +1008,14 @@ __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(
   if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;
   const int32_t base = s->class_base[cls];
   // The next task's number is taken before the current one is aligned, and
-  // (16- and 64-lane groups) its record loaded; the narrower groups, whose
-  // pairs hold more registers, load it afterwards.
+  // (shapes of at most kMaxP registers) its record loaded; the shapes of
+  // more registers load it afterwards.
   int w = next_wave_task(&s->class_next[cls], lane);
   SwTask task;
   if (w < nwt) task = b.tasks[base + w * GPW + grp];
   while (w < nwt) {
     const int wn = next_wave_task(&s->class_next[cls], lane);
-    if constexpr (G >= 16) {
+    if constexpr (G >= 16 && P <= kMaxP) {
       SwTask next;
       if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
       fast_pair<G, P>(b, task, l);
@@ -1750,8 +1755,16 @@ hipError_t launch_schedule(const BatchView& b, int pass, hipStream_t st) {
 // even) keep 16-lane groups, which measured faster there.
 // A class whose rows all fit the 2G * kTrim rows of its trimmed shape runs
 // that shape: 100/101-base reads as 4 x 13 (104 rows) instead of 4 x 16 (128).
+// The 64-lane classes of rows above 256 run forward as 16 lanes of 4P
+// registers (P = 3..5) or 32 lanes of 2P (P = 6..8): a pipeline fill of 31
+// or 63 steps in place of 127, the same rows.
 FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus, int max_rows) {
-  if (group == 64 || !forward) return FastShape{group, P};
+  if (!forward || (group == 64 && P <= 2)) return FastShape{group, P};
+  if (group == 64) {
+    const int G = P <= 5 ? 16 : 32;
+    if (pairs * G < (int64_t)64 * 2 * 4 * cus) return FastShape{64, P};
+    return FastShape{G, 64 * P / G};
+  }
   static constexpr int kLanes[kMaxP] = {2, 2, 4, 4, 4, 8, 8, 8};
   static constexpr int kRegs[kMaxP] = {8, 16, 12, 16, 19, 12, 14, 16};
   static constexpr int kTrim[kMaxP] = {0, 0, 0, 13, 0, 0, 0, 0};
@@ -1766,7 +1779,11 @@ hipError_t launch_fast(const BatchView& b, FastShape s, int grid, hipStream_t st
     case 2: return launch_fast_2_8(b, s.regs, grid, st);
     case 4: return s.regs <= 16 ? launch_fast_4_12(b, s.regs, grid, st) : launch_fast_4_19(b, s.regs, grid, st);
     case 8: return launch_fast_8_12(b, s.regs, grid, st);
-    case 16: return s.regs <= 4 ? launch_fast_16_1(b, s.regs, grid, st) : launch_fast_16_5(b, s.regs, grid, st);
+    case 16:
+      return s.regs <= 4   ? launch_fast_16_1(b, s.regs, grid, st)
+             : s.regs <= 8 ? launch_fast_16_5(b, s.regs, grid, st)
+                           : launch_fast_16_12(b, s.regs, grid, st);
+    case 32: return launch_fast_32_12(b, s.regs, grid, st);
     case 64: return s.regs <= 4 ? launch_fast_64_1(b, s.regs, grid, st) : launch_fast_64_5(b, s.regs, grid, st);
   }
   return hipErrorInvalidValue;
@@ -1807,7 +1824,11 @@ hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu) {
     case 2: return occupancy_fast_2_8(s.regs, blocks_per_cu);
     case 4: return s.regs <= 16 ? occupancy_fast_4_12(s.regs, blocks_per_cu) : occupancy_fast_4_19(s.regs, blocks_per_cu);
     case 8: return occupancy_fast_8_12(s.regs, blocks_per_cu);
-    case 16: return s.regs <= 4 ? occupancy_fast_16_1(s.regs, blocks_per_cu) : occupancy_fast_16_5(s.regs, blocks_per_cu);
+    case 16:
+      return s.regs <= 4   ? occupancy_fast_16_1(s.regs, blocks_per_cu)
+             : s.regs <= 8 ? occupancy_fast_16_5(s.regs, blocks_per_cu)
+                           : occupancy_fast_16_12(s.regs, blocks_per_cu);
+    case 32: return occupancy_fast_32_12(s.regs, blocks_per_cu);
     case 64: return s.regs <= 4 ? occupancy_fast_64_1(s.regs, blocks_per_cu) : occupancy_fast_64_5(s.regs, blocks_per_cu);
   }
   return hipErrorInvalidValue;

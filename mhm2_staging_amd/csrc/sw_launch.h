@@ -126,6 +126,8 @@ hipError_t launch_fast_4_19(const BatchView& b, int P, int grid, hipStream_t st)
 hipError_t launch_fast_8_12(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_5(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_16_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_32_12(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_64_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_64_5(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t occupancy_fast_2_8(int P, int* blocks_per_cu);
@@ -134,6 +136,8 @@ hipError_t occupancy_fast_4_19(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_8_12(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_1(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_5(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_16_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_32_12(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_64_1(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_64_5(int P, int* blocks_per_cu);
 hipError_t band_kernel_occupancy(int P, int* blocks_per_cu);  // P as launch_band

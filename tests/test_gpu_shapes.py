@@ -29,15 +29,18 @@ SHAPES = {30: (2, 8), 60: (2, 16), 90: (4, 12), 100: (4, 13), 104: (4, 13), 105:
 
 
 def class_of(rows):
-    """The 16-lane class of a pair with `rows` rows (fast16_class_of in sw_common.h)."""
+    """The packed class of a pair with `rows` rows (fast16_class_of in sw_common.h; the 64-lane
+    classes, 8..15, above 256 rows)."""
+    if rows > 256:
+        return 8 + (rows + 127) // 128 - 1
     if rows <= 128:
         return (rows + 31) // 32 - 1
     return 4 if rows <= 152 else 5 if rows <= 192 else (rows + 31) // 32 - 1
 
 
-def batch_of(read_len, seed):
+def batch_of(read_len, seed, n=NPAIRS, W=WINDOW):
     rng = np.random.default_rng(seed)
-    n, L, W = NPAIRS, read_len, WINDOW
+    L = read_len
     win = rng.integers(0, 4, (n, W), dtype=np.uint8)
     start = rng.integers(0, W - L + 1, n)
     rd = np.take_along_axis(win, start[:, None] + np.arange(L)[None, :], 1)
@@ -152,3 +155,36 @@ def test_forward_shape_contig_as_ssw_ref(gpu_aligner, read_len):
                                                                   "ref_end")], 1))
     finally:
         al.set_orientation(True)
+
+
+# rows above 256 (the 64-lane classes): read length -> forward shape
+SHAPES64 = {300: (16, 12), 450: (16, 16), 600: (16, 20), 700: (32, 12), 850: (32, 14), 1000: (32, 16)}
+
+
+@pytest.mark.parametrize("read_len", sorted(SHAPES64))
+def test_forward_shape_64_lane_classes(gpu_aligner, read_len):
+    """Reads of 257..1024 bases: 16 or 32 lanes per pair forward, against 64-lane groups (slices
+    of 4,000 pairs) and the oracle."""
+    n, W = 20_000, 1024
+    print(f"[gpu] forward shape, reads of {read_len} bases against {W}-base windows, {n} pairs", flush=True)
+    al = gpu_aligner
+    al.set_scoring(m.MHM2_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(False)
+    b = batch_of(read_len, 4000 + read_len, n, W)
+    got = table(al.align(b))
+    st = al.stats()
+    cls = class_of(read_len)
+    assert st["class_pairs"][cls] == n
+    assert (st["class_lanes"][cls], st["class_regs"][cls]) == SHAPES64[read_len]
+    want = np.concatenate([table(al.align(sliced(b, lo, min(lo + SLICE, n)))) for lo in range(0, n, SLICE)])
+    st = al.stats()
+    assert (st["class_lanes"][cls], st["class_regs"][cls]) == (64, cls - 7)
+    assert np.array_equal(got, want)
+    assert (got[:, 0] > 0).mean() > 0.85
+    idx = np.linspace(0, n - 1, 100).astype(np.int64)
+    o = oracle_batch(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
+                     m.MHM2_SCORING)
+    assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
+                                                              "query_end")], 1))
