@@ -14,7 +14,7 @@ import numpy as np
 from . import _native
 from .synth import ContigSet, PairBatch, WindowBatch, from_strings
 
-# mhm2's alignment scoring, BASELINE "previous defaults" (CMakeDefinitions.txt:118-122)
+# mhm2's alignment scoring, its "previous defaults" (CMakeDefinitions.txt:118-122)
 MHM2_SCORING = (1, 3, 5, 2, 2)
 # the values active in the reference tree (CMakeDefinitions.txt:113-117)
 MHM2_TREE_SCORING = (1, 1, 1, 1, 1)

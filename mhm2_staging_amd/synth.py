@@ -54,7 +54,7 @@ def from_strings(reads, ctgs):
 
 
 def generate(config, npairs=0, seed=0, threads=0):
-    """Workload of BASELINE config 1..5 (plain, uneven, skewed, repeats, serving pairs)."""
+    """Seeded workload 1..5 (plain, uneven, skewed, repeats, serving pairs)."""
     L = _native.lib()
     b = _native.SynthBatch()
     rc = L.mhmsw_synth_generate(int(config), int(npairs), int(seed), int(threads), ctypes.byref(b))

@@ -33,5 +33,5 @@ KAT_SCORINGS = {
     "mhm2_tree": (1, 1, 1, 1, 1),     # ALN_* as set in CMakeDefinitions.txt:113-117
     "cigar": (2, 4, 4, 2, 1),         # cigar_aln_scoring, ssw-test.cpp:103
 }
-# BASELINE scoring (CMakeDefinitions.txt:118-122); the reference test does not use it
-BASELINE_SCORING = (1, 3, 5, 2, 2)
+# mhm2's default scoring (CMakeDefinitions.txt:118-122); the reference test does not use it
+MHM2_DEFAULT_SCORING = (1, 3, 5, 2, 2)

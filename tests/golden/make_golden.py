@@ -4,8 +4,9 @@ Run in a container that has /root/reference (``make -C oracle`` builds the
 reference from its own sources first) and libmhmsw.so (for the config-1
 generator).  Output: tests/golden/golden_small.npz (seeded small corpora at
 four scorings, both roles), tests/golden/golden_kat.json and
-tests/golden/golden_config1.npz (10,000 pairs of config 1 under BASELINE
-scoring in klign's orientation, with a sha256 of the result table).
+tests/golden/golden_config1.npz (10,000 pairs of the plain 150 x 350 bp
+workload under mhm2's default scoring in klign's orientation, with a sha256
+of the result table).
 """
 import hashlib
 import json
@@ -20,10 +21,10 @@ sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle_lib import have_ref, pack_strings, ref_batch  # noqa: E402
-from golden.kat_cases import BASELINE_SCORING, KAT_CASES, KAT_SCORINGS  # noqa: E402
+from golden.kat_cases import MHM2_DEFAULT_SCORING, KAT_CASES, KAT_SCORINGS  # noqa: E402
 
 SCORINGS = dict(KAT_SCORINGS)
-SCORINGS["baseline"] = BASELINE_SCORING
+SCORINGS["baseline"] = MHM2_DEFAULT_SCORING
 
 
 def result_hash(res):
@@ -117,10 +118,10 @@ def main():
     batch = m.generate(1)
     # klign orientation: SSW-ref = read, SSW-query = contig window (klign.cpp:257)
     res = ref_batch(batch.reads, batch.read_off, batch.read_len, batch.ctgs, batch.ctg_off, batch.ctg_len,
-                    BASELINE_SCORING, 8)
+                    MHM2_DEFAULT_SCORING, 8)
     table = np.stack([res[k] for k in res.dtype.names], 1).astype(np.int32)
     np.savez_compressed(os.path.join(HERE, "golden_config1.npz"), results=table,
-                        sha256=np.array(result_hash(res)), scoring=np.array(BASELINE_SCORING, dtype=np.int32))
+                        sha256=np.array(result_hash(res)), scoring=np.array(MHM2_DEFAULT_SCORING, dtype=np.int32))
     print("wrote golden_small.npz, golden_kat.json, golden_config1.npz; config1 sha256", result_hash(res))
 
 

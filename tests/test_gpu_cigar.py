@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import mhm2_staging_amd as m
-from golden.kat_cases import BASELINE_SCORING
+from golden.kat_cases import MHM2_DEFAULT_SCORING
 from oracle_lib import cigar_string, oracle_cigar_batch
 from test_gpu_parity import assert_same, gpu_table
 
@@ -100,14 +100,14 @@ def test_cigar_reference_fixtures(gpu_aligner, mode):
 @pytest.mark.parametrize("config,npairs", [(1, 6000), (2, 6000), (3, 6000), (4, 1500), (5, 4000)])
 def test_cigar_configs_vs_oracle(gpu_aligner, config, npairs):
     b = m.generate(config, npairs=npairs, seed=21)
-    for sc in (BASELINE_SCORING, (1, 1, 1, 1, 1), (10, 2, 3, 1, 2)):
+    for sc in (MHM2_DEFAULT_SCORING, (1, 1, 1, 1, 1), (10, 2, 3, 1, 2)):
         check_vs_oracle(gpu_aligner, b, sc)
 
 
 def test_cigar_plain_form_and_other_orientation(gpu_aligner):
     b = m.generate(2, npairs=3000, seed=22)
-    check_vs_oracle(gpu_aligner, b, BASELINE_SCORING, mode=m.MODE_PLAIN)
-    check_vs_oracle(gpu_aligner, b, BASELINE_SCORING, read_is_ref=False)
+    check_vs_oracle(gpu_aligner, b, MHM2_DEFAULT_SCORING, mode=m.MODE_PLAIN)
+    check_vs_oracle(gpu_aligner, b, MHM2_DEFAULT_SCORING, read_is_ref=False)
     check_vs_oracle(gpu_aligner, b, (2, 2, 3, 1, 2), read_is_ref=False)
 
 
@@ -139,7 +139,7 @@ def test_cigar_edge_cases(gpu_aligner):
     reads.append(w[:1500] + w[2100:3600])
     ctgs.append(w[:3600])
     b = m.from_strings(reads, ctgs)
-    for sc in (BASELINE_SCORING, (1, 1, 1, 1, 1), (10, 2, 3, 1, 2)):
+    for sc in (MHM2_DEFAULT_SCORING, (1, 1, 1, 1, 1), (10, 2, 3, 1, 2)):
         for mode in (m.MODE_FAST, m.MODE_PLAIN):
             check_vs_oracle(gpu_aligner, b, sc, mode=mode)
 
@@ -165,12 +165,12 @@ def test_cigar_results_unchanged_and_graph_replays(gpu_aligner):
     the captured job (CIGAR pass inside) match an eager run every time."""
     b = m.generate(3, npairs=5000, seed=23)
     al = gpu_aligner
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     al.set_mode(m.MODE_FAST)
     al.set_orientation(True)
     base = al.align(b)
     al.set_graphs(False)
-    eager = run_cigar(al, b, BASELINE_SCORING)
+    eager = run_cigar(al, b, MHM2_DEFAULT_SCORING)
     al.set_graphs(True)
     assert np.array_equal(gpu_table(eager[0]), gpu_table(base))
     al.set_cigar(True)

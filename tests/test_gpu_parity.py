@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import mhm2_staging_amd as m
-from golden.kat_cases import BASELINE_SCORING, KAT_CASES
+from golden.kat_cases import MHM2_DEFAULT_SCORING, KAT_CASES
 from oracle_lib import oracle_batch
 
 pytestmark = pytest.mark.gpu
@@ -85,14 +85,14 @@ def test_small_corpus_vs_reference_golden(gpu_aligner, mode, name):
 def test_config1_vs_reference_golden(gpu_aligner, mode):
     z = np.load(os.path.join(GOLD, "golden_config1.npz"))
     b = m.generate(1)
-    assert_same(run(gpu_aligner, b, BASELINE_SCORING, mode), z["results"], b)
+    assert_same(run(gpu_aligner, b, MHM2_DEFAULT_SCORING, mode), z["results"], b)
 
 
 @pytest.mark.parametrize("config,npairs", [(2, 6000), (3, 6000), (4, 1500), (5, 4000)])
 @pytest.mark.parametrize("mode", MODES)
 def test_configs_vs_oracle(gpu_aligner, config, npairs, mode):
     b = m.generate(config, npairs=npairs)
-    assert_same(run(gpu_aligner, b, BASELINE_SCORING, mode), oracle_table(b, BASELINE_SCORING), b)
+    assert_same(run(gpu_aligner, b, MHM2_DEFAULT_SCORING, mode), oracle_table(b, MHM2_DEFAULT_SCORING), b)
 
 
 @pytest.mark.parametrize("scoring", [(1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1)])
@@ -105,9 +105,9 @@ def test_other_scorings_vs_oracle(gpu_aligner, scoring):
 
 def test_orientation_ctg_as_ssw_ref_vs_oracle(gpu_aligner):
     b = m.generate(3, npairs=3000, seed=5)
-    want = oracle_table(b, BASELINE_SCORING, read_is_ref=False)
+    want = oracle_table(b, MHM2_DEFAULT_SCORING, read_is_ref=False)
     for mode in MODES:
-        assert_same(run(gpu_aligner, b, BASELINE_SCORING, mode, read_is_ref=False), want, b)
+        assert_same(run(gpu_aligner, b, MHM2_DEFAULT_SCORING, mode, read_is_ref=False), want, b)
 
 
 def test_overflow_rerun_exact(gpu_aligner):
@@ -148,14 +148,14 @@ def test_edge_cases_vs_oracle(gpu_aligner):
     reads.append(rs(300, "AC"))   # many equal-score ends
     ctgs.append(rs(900, "AC"))
     b = m.from_strings(reads, ctgs)
-    want = oracle_table(b, BASELINE_SCORING)
+    want = oracle_table(b, MHM2_DEFAULT_SCORING)
     for mode in MODES:
-        assert_same(run(gpu_aligner, b, BASELINE_SCORING, mode), want, b)
+        assert_same(run(gpu_aligner, b, MHM2_DEFAULT_SCORING, mode), want, b)
 
 
 def test_repeat_staged_runs_identical(gpu_aligner):
     b = m.generate(3, npairs=4000, seed=9)
-    gpu_aligner.set_scoring(BASELINE_SCORING)
+    gpu_aligner.set_scoring(MHM2_DEFAULT_SCORING)
     gpu_aligner.set_orientation(True)
     gpu_aligner.set_mode(m.MODE_FAST)
     gpu_aligner.stage(b)
@@ -170,10 +170,10 @@ def test_repeat_staged_runs_identical(gpu_aligner):
 def test_full_size_config2_properties(gpu_aligner):
     """1M-pair uneven batch: fast == plain on every pair, sample == oracle, coordinate invariants."""
     b = m.generate(2)
-    fast = run(gpu_aligner, b, BASELINE_SCORING, m.MODE_FAST)
+    fast = run(gpu_aligner, b, MHM2_DEFAULT_SCORING, m.MODE_FAST)
     st = gpu_aligner.stats()
     assert st["npairs"] == b.npairs and st["cells"] == b.cells()
-    plain = run(gpu_aligner, b, BASELINE_SCORING, m.MODE_PLAIN)
+    plain = run(gpu_aligner, b, MHM2_DEFAULT_SCORING, m.MODE_PLAIN)
     assert_same(fast, plain, b)
     pos = fast[:, 0] > 0
     assert (fast[pos, 1] <= fast[pos, 2]).all() and (fast[pos, 3] <= fast[pos, 4]).all()
@@ -181,7 +181,7 @@ def test_full_size_config2_properties(gpu_aligner):
     assert (fast[:, 0] <= np.minimum(b.read_len, b.ctg_len)).all()
     idx = np.random.default_rng(0).choice(b.npairs, 3000, replace=False)
     sub = b.subset(idx)
-    assert_same(fast[idx], oracle_table(sub, BASELINE_SCORING), sub)
+    assert_same(fast[idx], oracle_table(sub, MHM2_DEFAULT_SCORING), sub)
 
 
 def test_graph_replays_match_eager_launches(gpu_aligner):
@@ -189,7 +189,7 @@ def test_graph_replays_match_eager_launches(gpu_aligner):
     a = m.generate(3, npairs=3000, seed=31)
     bb = m.generate(2, npairs=2000, seed=32)
     al = gpu_aligner
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     al.set_orientation(True)
     al.set_mode(m.MODE_FAST)
     try:
@@ -214,10 +214,10 @@ def test_graph_replays_match_eager_launches(gpu_aligner):
             al.align_staged()
             al.wait()
             outs.append(gpu_table(al.results()))
-        want = oracle_table(a, BASELINE_SCORING)
+        want = oracle_table(a, MHM2_DEFAULT_SCORING)
         for o in outs:
             assert_same(o, want, a)
-        assert_same(other, oracle_table(bb, BASELINE_SCORING), bb)
+        assert_same(other, oracle_table(bb, MHM2_DEFAULT_SCORING), bb)
     finally:
         al.set_graphs(True)
 
@@ -225,9 +225,9 @@ def test_graph_replays_match_eager_launches(gpu_aligner):
 def test_tunings_give_identical_results_and_autotune(gpu_aligner):
     """Scheduling parameters change speed only: every tuning matches the oracle."""
     b = m.generate(2, npairs=4000, seed=41)
-    want = oracle_table(b, BASELINE_SCORING)
+    want = oracle_table(b, MHM2_DEFAULT_SCORING)
     al = gpu_aligner
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     al.set_orientation(True)
     al.set_mode(m.MODE_FAST)
     al.stage(b)

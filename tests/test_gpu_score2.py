@@ -11,13 +11,13 @@ import numpy as np
 import pytest
 
 import mhm2_staging_amd as m
-from golden.kat_cases import BASELINE_SCORING
+from golden.kat_cases import MHM2_DEFAULT_SCORING
 from oracle_lib import oracle_batch2
 from test_gpu_parity import assert_same, gpu_table
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-SCORINGS = [BASELINE_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1)]
+SCORINGS = [MHM2_DEFAULT_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1)]
 
 
 def strings_batch(refs_buf, refs_off, refs_len, qry_buf, qry_off, qry_len):
@@ -94,7 +94,7 @@ def test_score2_configs_against_oracle(gpu_aligner, cfg):
 
 def test_score2_plain_form_and_other_orientation(gpu_aligner):
     batch = m.generate(2, npairs=3000, seed=97)
-    check(gpu_aligner, batch, BASELINE_SCORING, mode=m.MODE_PLAIN)
+    check(gpu_aligner, batch, MHM2_DEFAULT_SCORING, mode=m.MODE_PLAIN)
     check(gpu_aligner, batch, (2, 2, 3, 1, 2), read_is_ref=False)
 
 
@@ -107,7 +107,7 @@ def test_score2_overflow_reruns(gpu_aligner):
 def test_score2_leaves_other_results_unchanged(gpu_aligner):
     batch = m.generate(3, npairs=5000, seed=99)
     al = gpu_aligner
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     al.set_mode(m.MODE_FAST)
     al.set_orientation(True)
     al.set_score2(False)
@@ -115,7 +115,7 @@ def test_score2_leaves_other_results_unchanged(gpu_aligner):
     al.align_staged()
     al.wait()
     plain_res = al.results()
-    res, _ = run_score2(al, batch, BASELINE_SCORING)
+    res, _ = run_score2(al, batch, MHM2_DEFAULT_SCORING)
     assert (res == plain_res).all()
 
 
@@ -124,7 +124,7 @@ def test_score2_graph_replays(gpu_aligner):
     al = gpu_aligner
     al.set_graphs(True)
     try:
-        res0, s20 = run_score2(al, batch, BASELINE_SCORING)
+        res0, s20 = run_score2(al, batch, MHM2_DEFAULT_SCORING)
         al.set_score2(True)
         for _ in range(3):
             al.align_staged()
@@ -134,5 +134,5 @@ def test_score2_graph_replays(gpu_aligner):
         al.set_score2(False)
     finally:
         al.set_graphs(False)
-    _, want = oracle_s2(batch, BASELINE_SCORING)
+    _, want = oracle_s2(batch, MHM2_DEFAULT_SCORING)
     assert (s20 == want).all()

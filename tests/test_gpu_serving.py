@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import mhm2_staging_amd as m
-from golden.kat_cases import BASELINE_SCORING
+from golden.kat_cases import MHM2_DEFAULT_SCORING
 from test_gpu_parity import assert_same, gpu_table, oracle_table
 
 pytestmark = pytest.mark.gpu
@@ -21,9 +21,9 @@ def test_resident_windows_equal_string_windows_and_oracle(gpu_aligner, serving_s
     cs, res = serving_set
     wb = m.window_pairs(cs, 3000, seed=52)
     sb = wb.as_strings(cs)
-    want = oracle_table(sb, BASELINE_SCORING)
+    want = oracle_table(sb, MHM2_DEFAULT_SCORING)
     al = gpu_aligner
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     al.set_orientation(True)
     for mode in (m.MODE_FAST, m.MODE_PLAIN):
         al.set_mode(mode)
@@ -45,13 +45,13 @@ def test_server_results_same_alone_and_in_the_mix(serving_set):
     reqs = [m.window_pairs(cs, n, seed=60 + i) for i, n in enumerate(sizes)]
     # alone: a fresh server, one request at a time
     alone = []
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=1)
     for r in reqs:
         rid = srv.submit(r)
         alone.append(srv.results(rid)[0])
     srv.close()
     # in the mix: all submitted back to back on 4 slots, overlapping on the device
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=4)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=4)
     ids = [srv.submit(r) for r in reqs]
     mixed = [srv.results(i) for i in ids]
     srv.close()
@@ -64,7 +64,7 @@ def test_server_results_same_alone_and_in_the_mix(serving_set):
     # and they are the SSW answers
     for r, a in zip(reqs[:3], alone[:3]):
         sb = r.as_strings(cs)
-        assert_same(gpu_table(a), oracle_table(sb, BASELINE_SCORING), sb)
+        assert_same(gpu_table(a), oracle_table(sb, MHM2_DEFAULT_SCORING), sb)
 
 
 def test_server_hbm_budget_admission_in_order(serving_set):
@@ -72,10 +72,10 @@ def test_server_hbm_budget_admission_in_order(serving_set):
     cs, res = serving_set
     reqs = [m.window_pairs(cs, 3000, seed=80 + i) for i in range(4)]
     # the budget: one and a half times one request's admission estimate
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=1)
     est = max(srv.results(srv.submit(r))[1]["job_est_bytes"] for r in reqs)
     srv.close()
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=4, hbm_budget=int(1.5 * est))
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=4, hbm_budget=int(1.5 * est))
     ids = [srv.submit(r) for r in reqs]
     outs = [srv.results(i) for i in ids]
     srv.close()
@@ -86,7 +86,7 @@ def test_server_hbm_budget_admission_in_order(serving_set):
         assert admits[k] >= dones[k - 1] - 1e-6  # serialized by the budget
     for r, (o, _) in zip(reqs[:2], outs[:2]):
         sb = r.as_strings(cs)
-        assert_same(gpu_table(o), oracle_table(sb, BASELINE_SCORING), sb)
+        assert_same(gpu_table(o), oracle_table(sb, MHM2_DEFAULT_SCORING), sb)
 
 
 def test_server_merges_queued_requests(serving_set):
@@ -96,7 +96,7 @@ def test_server_merges_queued_requests(serving_set):
     sizes = [20000] + [300 + 97 * i for i in range(15)]
     reqs = [m.window_pairs(cs, n, seed=100 + i) for i, n in enumerate(sizes)]
     al = m.Aligner(device=0)
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     alone = []
     for r in reqs:
         al.stage_windows(res, r)
@@ -104,7 +104,7 @@ def test_server_merges_queued_requests(serving_set):
         al.wait()
         alone.append(al.results())
     al.close()
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=1)
     ids = [srv.submit(r) for r in reqs]
     outs = [srv.results(i) for i in ids]
     srv.close()
@@ -117,7 +117,7 @@ def test_server_merges_queued_requests(serving_set):
         assert (gpu_table(a) == gpu_table(b)).all()
     for r, (o, _) in list(zip(reqs, outs))[1:4]:
         sb = r.as_strings(cs)
-        assert_same(gpu_table(o), oracle_table(sb, BASELINE_SCORING), sb)
+        assert_same(gpu_table(o), oracle_table(sb, MHM2_DEFAULT_SCORING), sb)
 
 
 def test_server_score2_and_cigars_match_the_batch_api(gpu_aligner, serving_set):
@@ -127,7 +127,7 @@ def test_server_score2_and_cigars_match_the_batch_api(gpu_aligner, serving_set):
     cs, res = serving_set
     reqs = [m.window_pairs(cs, n, seed=90 + i) for i, n in enumerate([800, 3000, 150, 2200])]
     al = gpu_aligner
-    al.set_scoring(BASELINE_SCORING)
+    al.set_scoring(MHM2_DEFAULT_SCORING)
     al.set_mode(m.MODE_FAST)
     al.set_orientation(True)
     al.set_score2(True)
@@ -142,7 +142,7 @@ def test_server_score2_and_cigars_match_the_batch_api(gpu_aligner, serving_set):
     finally:
         al.set_score2(False)
         al.set_cigar(False)
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=1)
     srv.set_outputs(score2=True, cigar=True)
     ids = [srv.submit(r) for r in reqs]
     for r, rid, (wres, ws2, (wm, woff, wops)) in zip(reqs, ids, want):
@@ -166,11 +166,11 @@ def test_server_job_estimate_bounds_its_allocations(serving_set, npairs, cigar):
     same batch allocates on a context that holds no buffers yet, and follows the allocation rule closely."""
     cs, res = serving_set
     r = m.window_pairs(cs, npairs, seed=300 + npairs)
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=1)
     srv.set_outputs(cigar=cigar)
     _, t = srv.results(srv.submit(r))
     srv.close()
-    al = m.Aligner(device=0, scoring=BASELINE_SCORING)
+    al = m.Aligner(device=0, scoring=MHM2_DEFAULT_SCORING)
     al.set_cigar(cigar)
     al.stage_windows(res, r)
     al.align_staged()
@@ -186,7 +186,7 @@ def test_server_job_estimate_bounds_its_allocations(serving_set, npairs, cigar):
 
 def test_server_rejects_a_request_larger_than_its_budget(serving_set):
     cs, res = serving_set
-    srv = m.Server(res, scoring=BASELINE_SCORING, slots=1, hbm_budget=2 << 20)
+    srv = m.Server(res, scoring=MHM2_DEFAULT_SCORING, slots=1, hbm_budget=2 << 20)
     with pytest.raises(m.MhmswError) as e:
         srv.submit(m.window_pairs(cs, 20000, seed=7))
     assert e.value.code == -1 and "more than the server's HBM budget" in str(e.value)
@@ -195,4 +195,4 @@ def test_server_rejects_a_request_larger_than_its_budget(serving_set):
     got, _ = srv.results(srv.submit(small))
     srv.close()
     sb = small.as_strings(cs)
-    assert_same(gpu_table(got), oracle_table(sb, BASELINE_SCORING), sb)
+    assert_same(gpu_table(got), oracle_table(sb, MHM2_DEFAULT_SCORING), sb)

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from golden.kat_cases import BASELINE_SCORING, KAT_CASES, KAT_SCORINGS
+from golden.kat_cases import MHM2_DEFAULT_SCORING, KAT_CASES, KAT_SCORINGS
 from oracle_lib import have_ref, oracle_batch, oracle_batch2, oracle_pairs, pack_strings, ref_batch, ref_batch2
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -63,7 +63,7 @@ def test_oracle_satisfies_reference_kat_expectations(name):
 
 def test_oracle_kat_one_deletion_under_baseline_scoring():
     """Under 1/3/5/2/2 the '1 del' case prefers the ungapped 11-base run (score 11 > 15-5)."""
-    res = oracle_pairs(["AAAATTTTCCCCGGGG"], ["AAAATTTCCCCGGGG"], BASELINE_SCORING)
+    res = oracle_pairs(["AAAATTTTCCCCGGGG"], ["AAAATTTCCCCGGGG"], MHM2_DEFAULT_SCORING)
     assert res["score"][0] == 11
     assert (res["ref_begin"][0], res["ref_end"][0]) == (5, 15)
     assert (res["query_begin"][0], res["query_end"][0]) == (4, 14)
@@ -85,7 +85,7 @@ def test_oracle_config1_subset_matches_golden():
     b = m.generate(1)
     idx = np.arange(0, b.npairs, 7)
     res = oracle_batch(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
-                       BASELINE_SCORING)
+                       MHM2_DEFAULT_SCORING)
     np.testing.assert_array_equal(table(res), z["results"][idx])
 
 
@@ -99,7 +99,7 @@ def test_oracle_matches_reference_fresh_corpus(seed):
         refs.append("".join(rng.choice(list(alph), int(rng.integers(1, 120)))))
         qrys.append("".join(rng.choice(list(alph), int(rng.integers(1, 120)))))
     a, b = pack_strings(refs), pack_strings(qrys)
-    for sc in [BASELINE_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1), (3, 2, 6, 1, 2)]:
+    for sc in [MHM2_DEFAULT_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1), (3, 2, 6, 1, 2)]:
         np.testing.assert_array_equal(table(oracle_batch(*a, *b, sc)), table(ref_batch(*a, *b, sc)))
 
 
@@ -145,7 +145,7 @@ def test_oracle_score2_first_principles():
     """Two copies of a motif in the SSW-ref: the second copy's score is the
     second best when it lies outside the mask window, and is masked (score2
     from elsewhere, here 0-score rows) when it lies inside."""
-    sc = BASELINE_SCORING
+    sc = MHM2_DEFAULT_SCORING
     motif = "ACGTTGCAAGTCCGATGCAT"                     # 20 bases, scores 20 against itself
     far = motif + "T" * 60 + motif[:12]                # second copy (12 bases) 60 bases later
     near = motif + "T" * 5 + motif[:12]                # inside max(m/2, 15) of the end
@@ -165,7 +165,7 @@ def test_oracle_score2_matches_reference_fresh_corpus(seed):
         refs.append("".join(rng.choice(list(alph), int(rng.integers(1, 300)))))
         qrys.append("".join(rng.choice(list(alph), int(rng.integers(1, 300)))))
     a, b = pack_strings(refs), pack_strings(qrys)
-    for sc in [BASELINE_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1)]:
+    for sc in [MHM2_DEFAULT_SCORING, (1, 1, 1, 1, 1), (2, 2, 3, 1, 2), (2, 4, 4, 2, 1)]:
         r1, s1 = oracle_batch2(*a, *b, sc)
         r2, s2 = ref_batch2(*a, *b, sc)
         np.testing.assert_array_equal(table(r1), table(r2))
