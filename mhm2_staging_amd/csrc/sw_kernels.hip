@@ -249,7 +249,10 @@ __global__ void extract_windows_kernel(BatchView b) {
 // scheduling: classify -> schedule -> scatter
 // ----------------------------------------------------------------------------
 // Geometry of pair p in pass `pass`; false when the pair has no task.
-__device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, int& cls, int& bin) {
+// `direct`, when given, receives the reverse pass's answer for a pair that
+// needs no task (make_task then returns false).
+__device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, int& cls, int& bin,
+                          PairEnd* direct = nullptr) {
   const int64_t sref = b.sc.read_is_ref ? 2 * p : 2 * p + 1;  // SSW-ref sequence
   const int64_t sqry = b.sc.read_is_ref ? 2 * p + 1 : 2 * p;  // SSW-query sequence
   const int32_t lref = b.seq_len[sref], lqry = b.seq_len[sqry];
@@ -328,6 +331,22 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
       ref_len = kProbeRows;
       if ((int64_t)qry_len > (int64_t)kProbeRows + D) qry_len = (int32_t)(kProbeRows + D);
       band = D <= b.band_max_d;
+    }
+    // A single-diagonal pair (D = 0: no gap reaches the score) whose score
+    // leaves no room for a mismatch or an N either is score/match matches in
+    // a row: each non-match costs at least c = match + min(mismatch,
+    // ambiguity) against a match, and an alignment of b non-matches within
+    // the rectangle's L = min(rows, columns) positions scores at most
+    // match*L - b*c.  Along its diagonal H' = (k+1)*match, so the least cell
+    // holding the score is k = l = score/match - 1, with no codes to read.
+    if (pass == kPassRev && band && D == 0 && direct && b.sc.match > 0) {
+      const int32_t L = ref_len < qry_len ? ref_len : qry_len;
+      const int64_t c = (int64_t)b.sc.match + (b.sc.mismatch < b.sc.ambiguity ? b.sc.mismatch : b.sc.ambiguity);
+      const int32_t a = fe.score / b.sc.match;
+      if ((int64_t)fe.score + c > (int64_t)b.sc.match * L && fe.score % b.sc.match == 0 && a <= L) {
+        *direct = PairEnd{fe.score, a - 1, a - 1, 0};
+        return false;
+      }
     }
     if (pass == kPassRev && band) {
       // rows = reversed SSW-ref, columns = reversed SSW-query, one lane per pair
@@ -441,13 +460,14 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
     if (p >= b.npairs) continue;
     SwTask t;
     int cls = 0, bin = 0;
-    if (!make_task(b, pass, p, t, cls, bin)) {
+    PairEnd direct{0, 0, 0, 0};
+    if (!make_task(b, pass, p, t, cls, bin, &direct)) {
       b.bucket[p] = -1;
       if (pass == kPassFwd) {
         b.fwd[p] = PairEnd{0, -1, 0, 0};
         if (b.score2) b.s2[p] = 0;
       }
-      if (pass == kPassRev) b.rev[p] = PairEnd{0, 0, 0, 0};
+      if (pass == kPassRev) b.rev[p] = direct;
       continue;
     }
     bk[k] = cls * kNumBins + bin;
