@@ -55,11 +55,12 @@ typedef struct {
   int32_t ctg_begin, ctg_end;
 } mhmsw_result;
 
-/* Work classes of the forward pass: 0..7 packed 16-bit by rows of 16 lanes
- * of P = 1..8 register pairs (class P-1: rows <= 32*P, except that class 4
- * ends at 152 rows and class 5 holds 153..192; mhmsw_stats.class_lanes /
- * class_regs give the kernel shape that ran each), 8..15 packed 16-bit with
- * 64-lane groups (rows <= 128*P), 16 the 32-bit form. */
+/* Work classes of the forward pass: 0..7 packed 16-bit, rows <= 32*P for
+ * P = 1..8 (class P-1; class 4 ends at 152 rows and class 5 holds
+ * 153..192), 8..15 packed 16-bit, rows <= 128*P, 16 the 32-bit form.  A
+ * packed class runs 16 (classes 0..7) or 64 (8..15) lanes per pair of P
+ * register pairs each, or, when it holds enough pairs, fewer lanes of more
+ * registers: mhmsw_stats.class_lanes / class_regs give the shape that ran. */
 #define MHMSW_NUM_CLASSES 17
 
 /* Per-batch counters of the last completed batch. */
@@ -107,8 +108,8 @@ typedef struct {
  *                  per CU = 4 x blocks_per_cu), 1..16, for the forward and
  *                  rectangle kernels (the reverse band kernels run 2)
  *   g64_min_rows   pairs whose shorter side has at least this many bases (and
- *                  at most 1024) use 64-lane groups, shorter ones 16-lane
- *                  groups; 33..257 (257 = 16-lane groups up to 256 rows)
+ *                  at most 1024) take the classes 8..15, shorter ones the
+ *                  classes 0..7; 33..257 (257: classes 0..7 up to 256 rows)
  *   bin_shift      column-length bucket width is 1 << bin_shift bases, 3..7 */
 typedef struct {
   int32_t blocks_per_cu;

@@ -84,10 +84,8 @@ struct mhmsw_ctx {
   int cigar = 0;                // compute SSW's CIGAR and mismatch count
   bool last_job_cigar = false;
   mhmsw_tuning tuning{8, 257, 5};
-  // 256-thread blocks of each persistent kernel that fit one CU at once:
-  // packed classes (16-lane, then 64-lane groups, P = 1..8) and band classes
-  // (all-class kernel, then P = 1, 2, 4, 8)
-  // blocks per CU of every packed kernel shape a class can run as (0: unknown)
+  // 256-thread blocks of each persistent kernel that fit one CU at once: every
+  // packed kernel shape a class can run as (0: unknown), and the band classes
   struct ShapeOcc {
     int lanes = 0, regs = 0, occ = 0;
   };
@@ -1279,9 +1277,9 @@ static hipError_t enqueue_cigar(mhmsw_ctx* c, hipStream_t st) {
 // captured graph cannot be timed)
 static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev, int* maxp16_out, int* maxp64_out) {
   // persistent grids: enough wavefronts to fill the device, but no more
-  // blocks than a class can use (4 wavefronts per 256-thread block, 4 pairs
-  // per wavefront in 16-lane groups, 1 in 64-lane groups, 64 in band
-  // classes) and no more per CU than the kernel keeps resident (occ), so
+  // blocks than a class can use (4 wavefronts per 256-thread block, 64 / G
+  // pairs per wavefront in G-lane groups, 64 in band classes) and no more per
+  // CU than the kernel keeps resident (occ), so
   // that a small job's tasks are shared out by the work counter rather than
   // one per wavefront to blocks that wait for a free CU
   // A class with few wavefront tasks runs on fewer waves per CU: with at
