@@ -188,3 +188,69 @@ def test_forward_shape_64_lane_classes(gpu_aligner, read_len):
                      m.MHM2_SCORING)
     assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
                                                               "query_end")], 1))
+
+
+def mixed_batch(n, seed):
+    """n pairs of mixed lengths: reads of 1..300 bases, windows of 1..700, reads copied from their
+    window (2% substitutions) where it is long enough, else random; N in a few windows."""
+    rng = np.random.default_rng(seed)
+    rl = rng.integers(1, 301, n)
+    wl = rng.integers(1, 701, n)
+    letters = np.frombuffer(b"ACGT", np.uint8)
+    win = letters[rng.integers(0, 4, int(wl.sum()))]
+    wo = np.concatenate([[0], np.cumsum(wl)[:-1]])
+    reads = letters[rng.integers(0, 4, int(rl.sum()))]
+    ro = np.concatenate([[0], np.cumsum(rl)[:-1]])
+    fits = np.nonzero((wl >= rl) & (rng.random(n) < 0.9))[0]
+    for p in fits:
+        s = wo[p] + rng.integers(0, wl[p] - rl[p] + 1)
+        reads[ro[p]:ro[p] + rl[p]] = win[s:s + rl[p]]
+    sub = np.nonzero(rng.random(len(reads)) < 0.02)[0]
+    reads[sub] = letters[rng.integers(0, 4, len(sub))]
+    nw = rng.integers(0, len(win), len(win) // 2000)
+    win[nw] = ord("N")
+    return m.PairBatch(np.concatenate([reads, [0]]).astype(np.uint8), ro.astype(np.int64), rl.astype(np.int32),
+                       np.concatenate([win, [0]]).astype(np.uint8), wo.astype(np.int64), wl.astype(np.int32))
+
+
+@pytest.mark.parametrize("score2", [False, True])
+def test_mixed_lengths_every_class_in_one_job(gpu_aligner, score2):
+    """600k pairs of mixed lengths: every 16-lane class and a 64-lane one in one job, each large
+    enough for its narrow shape, the task ranges of the classes side by side; against the same
+    pairs in slices of 4,000 (16- and 64-lane groups) and the oracle."""
+    n = 600_000
+    print(f"[gpu] mixed lengths, {n} pairs, score2={score2}", flush=True)
+    al = gpu_aligner
+    al.set_scoring(m.MHM2_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(score2)
+    try:
+        b = mixed_batch(n, 77 + score2)
+
+        def run(batch):
+            al.stage(batch)
+            al.align_staged()
+            al.wait()
+            return table(al.results()), (al.score2() if score2 else None)
+
+        got, got2 = run(b)
+        st = al.stats()
+        narrow = [k for k in range(8) if st["class_pairs"][k] and st["class_lanes"][k] < 16]
+        assert len(narrow) >= 4, (st["class_pairs"][:8], st["class_lanes"][:8])
+        parts = [run(sliced(b, lo, min(lo + SLICE, n))) for lo in range(0, n, SLICE)]
+        assert np.array_equal(got, np.concatenate([p[0] for p in parts]))
+        if score2:
+            assert np.array_equal(got2, np.concatenate([p[1] for p in parts]))
+        idx = np.linspace(0, n - 1, 300).astype(np.int64)
+        if score2:
+            o, o2 = oracle_batch2(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx],
+                                  b.ctg_len[idx], m.MHM2_SCORING)
+            assert np.array_equal(got2[idx], o2)
+        else:
+            o = oracle_batch(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
+                             m.MHM2_SCORING)
+        assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
+                                                                  "query_end")], 1))
+    finally:
+        al.set_score2(False)
