@@ -1,0 +1,6 @@
+// Instantiation of sw_fast_kernel<16, 16> (see sw_kernels.hip).
+#define MHMSW_FAST_PART 1
+#define MHMSW_FAST_G 16
+#define MHMSW_FAST_P0 16
+#define MHMSW_FAST_PS 16
+#include "sw_kernels.hip"

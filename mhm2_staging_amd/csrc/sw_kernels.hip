@@ -1797,13 +1797,18 @@ FastShape fast_shape(int group, int P, bool forward, int64_t pairs, int cus, int
 hipError_t launch_fast(const BatchView& b, FastShape s, int grid, hipStream_t st) {
   switch (s.lanes) {
     case 2: return launch_fast_2_8(b, s.regs, grid, st);
-    case 4: return s.regs <= 16 ? launch_fast_4_12(b, s.regs, grid, st) : launch_fast_4_19(b, s.regs, grid, st);
-    case 8: return launch_fast_8_12(b, s.regs, grid, st);
+    case 4:
+      return s.regs <= 13   ? launch_fast_4_12(b, s.regs, grid, st)
+             : s.regs <= 16 ? launch_fast_4_16(b, s.regs, grid, st)
+                            : launch_fast_4_19(b, s.regs, grid, st);
+    case 8: return s.regs <= 12 ? launch_fast_8_12(b, s.regs, grid, st) : launch_fast_8_14(b, s.regs, grid, st);
     case 16:
-      return s.regs <= 4   ? launch_fast_16_1(b, s.regs, grid, st)
-             : s.regs <= 8 ? launch_fast_16_5(b, s.regs, grid, st)
-                           : launch_fast_16_12(b, s.regs, grid, st);
-    case 32: return launch_fast_32_12(b, s.regs, grid, st);
+      return s.regs <= 4    ? launch_fast_16_1(b, s.regs, grid, st)
+             : s.regs <= 8  ? launch_fast_16_5(b, s.regs, grid, st)
+             : s.regs <= 12 ? launch_fast_16_12(b, s.regs, grid, st)
+             : s.regs <= 16 ? launch_fast_16_16(b, s.regs, grid, st)
+                            : launch_fast_16_20(b, s.regs, grid, st);
+    case 32: return s.regs <= 12 ? launch_fast_32_12(b, s.regs, grid, st) : launch_fast_32_14(b, s.regs, grid, st);
     case 64: return s.regs <= 4 ? launch_fast_64_1(b, s.regs, grid, st) : launch_fast_64_5(b, s.regs, grid, st);
   }
   return hipErrorInvalidValue;
@@ -1842,13 +1847,18 @@ static hipError_t occ(K kernel, int* blocks_per_cu) {
 hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu) {
   switch (s.lanes) {
     case 2: return occupancy_fast_2_8(s.regs, blocks_per_cu);
-    case 4: return s.regs <= 16 ? occupancy_fast_4_12(s.regs, blocks_per_cu) : occupancy_fast_4_19(s.regs, blocks_per_cu);
-    case 8: return occupancy_fast_8_12(s.regs, blocks_per_cu);
+    case 4:
+      return s.regs <= 13   ? occupancy_fast_4_12(s.regs, blocks_per_cu)
+             : s.regs <= 16 ? occupancy_fast_4_16(s.regs, blocks_per_cu)
+                            : occupancy_fast_4_19(s.regs, blocks_per_cu);
+    case 8: return s.regs <= 12 ? occupancy_fast_8_12(s.regs, blocks_per_cu) : occupancy_fast_8_14(s.regs, blocks_per_cu);
     case 16:
-      return s.regs <= 4   ? occupancy_fast_16_1(s.regs, blocks_per_cu)
-             : s.regs <= 8 ? occupancy_fast_16_5(s.regs, blocks_per_cu)
-                           : occupancy_fast_16_12(s.regs, blocks_per_cu);
-    case 32: return occupancy_fast_32_12(s.regs, blocks_per_cu);
+      return s.regs <= 4    ? occupancy_fast_16_1(s.regs, blocks_per_cu)
+             : s.regs <= 8  ? occupancy_fast_16_5(s.regs, blocks_per_cu)
+             : s.regs <= 12 ? occupancy_fast_16_12(s.regs, blocks_per_cu)
+             : s.regs <= 16 ? occupancy_fast_16_16(s.regs, blocks_per_cu)
+                            : occupancy_fast_16_20(s.regs, blocks_per_cu);
+    case 32: return s.regs <= 12 ? occupancy_fast_32_12(s.regs, blocks_per_cu) : occupancy_fast_32_14(s.regs, blocks_per_cu);
     case 64: return s.regs <= 4 ? occupancy_fast_64_1(s.regs, blocks_per_cu) : occupancy_fast_64_5(s.regs, blocks_per_cu);
   }
   return hipErrorInvalidValue;

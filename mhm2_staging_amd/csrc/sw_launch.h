@@ -122,22 +122,32 @@ hipError_t fast_kernel_occupancy(FastShape s, int* blocks_per_cu);
 // fast_kernel_occupancy dispatch to them
 hipError_t launch_fast_2_8(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_4_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_4_16(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_4_19(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_8_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_8_14(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_5(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_16_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_16_16(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_16_20(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_32_12(const BatchView& b, int P, int grid, hipStream_t st);
+hipError_t launch_fast_32_14(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_64_1(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t launch_fast_64_5(const BatchView& b, int P, int grid, hipStream_t st);
 hipError_t occupancy_fast_2_8(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_4_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_4_16(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_4_19(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_8_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_8_14(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_1(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_5(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_16_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_16_16(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_16_20(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_32_12(int P, int* blocks_per_cu);
+hipError_t occupancy_fast_32_14(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_64_1(int P, int* blocks_per_cu);
 hipError_t occupancy_fast_64_5(int P, int* blocks_per_cu);
 hipError_t band_kernel_occupancy(int P, int* blocks_per_cu);  // P as launch_band
