@@ -250,9 +250,10 @@ __global__ void extract_windows_kernel(BatchView b) {
 // ----------------------------------------------------------------------------
 // Geometry of pair p in pass `pass`; false when the pair has no task.
 // `direct`, when given, receives the reverse pass's answer for a pair that
-// needs no task (make_task then returns false).
+// needs no task (make_task then returns false); `probe_out`, when given, is set
+// for a probe task of pass kPassRev.
 __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, int& cls, int& bin,
-                          PairEnd* direct = nullptr) {
+                          PairEnd* direct = nullptr, bool* probe_out = nullptr) {
   const int64_t sref = b.sc.read_is_ref ? 2 * p : 2 * p + 1;  // SSW-ref sequence
   const int64_t sqry = b.sc.read_is_ref ? 2 * p + 1 : 2 * p;  // SSW-query sequence
   const int32_t lref = b.seq_len[sref], lqry = b.seq_len[sqry];
@@ -322,6 +323,7 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
     // ends within the probe; a gapped real one scores too high to be probed.)
     const bool probe = b.rev_probe && !wide && !band && D >= 0 && ref_len > kProbeRows &&
                        (int64_t)b.sc.match * kProbeRows >= fe.score;
+    if (pass == kPassRev && probe && probe_out) *probe_out = true;  // a probe kPassRev2 checks
     if (pass == kPassRev2) {
       if (!probe || b.rev[p].score == fe.score) return false;
       // the few probes that miss: every one in the largest reverse class
@@ -427,8 +429,8 @@ __device__ bool make_task(const BatchView& b, int pass, int64_t p, SwTask& t, in
   return true;
 }
 
-// At the start of a job: error words, counters and the scheduling state of
-// its three passes (b.sched is the first of kNumPasses).
+// At the start of a job: error words, counters (reverse cells, a probe flag) and the
+// scheduling state of its passes (b.sched is the first of kNumPasses).
 __global__ void clear_job_kernel(BatchView b) {
   if (blockIdx.x == 0 && threadIdx.x < 4) b.errors[threadIdx.x] = 0;
   if (blockIdx.x == 0 && threadIdx.x < 2) b.counters[threadIdx.x] = 0;
@@ -444,10 +446,17 @@ __global__ void clear_job_kernel(BatchView b) {
 constexpr int kClassifyPerThread = 8;
 
 __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
+  // pass kPassRev2 checks the probes of pass kPassRev: with none issued it has
+  // no task, and its scatter_kernel reads none of the buckets left unwritten
+  if (pass == kPassRev2 && b.counters[1] == 0) return;
   __shared__ int32_t hist[kNumBuckets];
   __shared__ unsigned long long blk_cells;  // reverse cells of the block's pairs: one global atomic per block
+  __shared__ int blk_probe;                 // a probe among the block's pairs: one global store per block
   for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) hist[i] = 0;
-  if (threadIdx.x == 0) blk_cells = 0;
+  if (threadIdx.x == 0) {
+    blk_cells = 0;
+    blk_probe = 0;
+  }
   __syncthreads();
   unsigned long long my_cells = 0;
   const int64_t p0 = (int64_t)blockIdx.x * blockDim.x * kClassifyPerThread;
@@ -461,7 +470,10 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
     SwTask t;
     int cls = 0, bin = 0;
     PairEnd direct{0, 0, 0, 0};
-    if (!make_task(b, pass, p, t, cls, bin, &direct)) {
+    bool probe = false;
+    const bool task = make_task(b, pass, p, t, cls, bin, &direct, &probe);
+    if (probe) blk_probe = 1;
+    if (!task) {
       b.bucket[p] = -1;
       if (pass == kPassFwd) {
         b.fwd[p] = PairEnd{0, -1, 0, 0};
@@ -486,6 +498,7 @@ __global__ __launch_bounds__(256) void classify_kernel(BatchView b, int pass) {
   if (my_cells) atomicAdd(&blk_cells, my_cells);
   __syncthreads();
   if (threadIdx.x == 0 && blk_cells) atomicAdd(&b.counters[0], blk_cells);
+  if (threadIdx.x == 0 && blk_probe) b.counters[1] = 1;
   for (int i = threadIdx.x; i < kNumBuckets; i += blockDim.x) {
     const int32_t n = hist[i];
     hist[i] = n ? atomicAdd(&b.sched->bucket_count[i], n) : 0;
@@ -542,6 +555,7 @@ __global__ void schedule_kernel(BatchView b) {
 }
 
 __global__ void scatter_kernel(BatchView b, int pass) {
+  if (pass == kPassRev2 && b.counters[1] == 0) return;  // no probes: no tasks (classify_kernel)
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < b.npairs; p += (int64_t)gridDim.x * blockDim.x) {
     const int bk = b.bucket[p];
     if (bk < 0) continue;
