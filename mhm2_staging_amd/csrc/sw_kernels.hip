@@ -70,9 +70,9 @@ __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) { return u32(
 __device__ __forceinline__ uint32_t pk_addu_sat(uint32_t a, uint32_t b) {
   return u32u(__builtin_elementwise_add_sat(vu(a), vu(b)));
 }
-// max of three packed u16 values, each a multiple of 16 below 0x7C00: as
-// f16 bit patterns these are finite and non-negative (denormals included,
-// which gfx950 kernels keep), and f16 maximum returns one of its inputs
+// max of three packed u16 values, each below 0x7C00: as f16 bit patterns
+// these are finite and non-negative (denormals included, which gfx950 kernels
+// keep), and f16 maximum returns one of its inputs
 typedef _Float16 v2f16 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pk_max3_u16f(uint32_t a, uint32_t b, uint32_t c) {
   const v2f16 x = __builtin_bit_cast(v2f16, a), y = __builtin_bit_cast(v2f16, b), z = __builtin_bit_cast(v2f16, c);
@@ -577,7 +577,7 @@ struct FastState {
   uint32_t ho[P];      // 16*H(r, c_last), packed
   uint32_t hg[P];      // 16*(F(r, c_last + 1) + gO), packed (gap along the columns), clamped >= 0
   uint32_t best[P];    // per-row running max of 16*H + tag, packed u16
-  uint32_t rec_h[P];   // per-row record: 16*H of the row's maximum so far, packed u16
+  uint32_t rec_h[P];   // per-row record: 16*H | 15 of the row's maximum so far, packed u16
   uint32_t rec_s[P];   // and the first step at which it was reached, packed u16
   uint32_t xa, xb;     // exchanged 16*H of the row above, 2 steps / 1 step ago
   uint32_t vg;         // exchanged 16*(E + gO) of the first row, clamped >= 0
@@ -636,7 +636,13 @@ __device__ __forceinline__ void fast_cells(FastState<G, P>& st, uint32_t t1, uin
     // E' and F' update: max(E' - gE, H) is 16*(max(E - gE, H - gO) + gO)
     st.hg[p] = pk_max(pk_subu_sat(st.hg[p], ge), h);
     vg = pk_max(pk_subu_sat(vg, ge), h);
-    st.best[p] = pk_maxu(st.best[p], h | tag);
+    // The row maxima take two steps at a time: at an odd step, diag still
+    // holds the register's H of the step before (tag one higher), so one
+    // three-way max covers both (16*H + tag stays below 0x7C00).
+    if constexpr ((S & 1) != 0) {
+      constexpr uint32_t tag_prev = (uint32_t)(16 - S) * 0x00010001u;
+      st.best[p] = pk_max3_u16f(st.best[p], h | tag, diag | tag_prev);
+    }
   }
   // The next step's values above the lane's two first rows: the low half
   // (row 2Pl - 1) is the high half of the lane below's last register, the high
@@ -745,23 +751,23 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
   fast_step<G, P, EDGE, SUB, 15>(st, t0 + 15, l, col_len, npad, k);
 #undef MHMSW_REFILL
   // Fold the per-row maxima into the records: a row whose maximum 16*H + tag
-  // beats its record takes the H and the step of the tag, t0 + (15 - tag) =
-  // t0 | (tag ^ 15).  (best is not cleared between chunks: a maximum carried
-  // over from an earlier chunk is never above the record it already set.)
+  // has a higher H than its record takes the step of the tag, t0 + (15 - tag)
+  // = t0 | (tag ^ 15).  best is never cleared, so it never falls: a maximum
+  // carried over from an earlier chunk is the record itself, and the new
+  // record is best with its tag bits set.
   const uint32_t cstep = (uint32_t)t0 * 0x00010001u | 0x000F000Fu;
 #pragma unroll
   for (int p = 0; p < P; p++) {
     const uint32_t m = st.best[p];
-    const uint32_t mh = m & 0xFFF0FFF0u;
-    // 0xFFFF in the halves where mh > rec_h (values below 0x7C00: the
-    // signed difference is exact)
-    uint32_t beat = u32(vi(pk_sub(st.rec_h[p], mh)) >> (short)15);
+    // 0xFFFF in the halves where m > rec_h = 16*Hrec | 15, i.e. where m's H
+    // is above Hrec (values below 0x7C00: the signed difference is exact)
+    uint32_t beat = u32(vi(pk_sub(st.rec_h[p], m)) >> (short)15);
     // (opaque to the compiler, which would otherwise turn the bit select
     // below into a compare and a select per half)
     __asm__("" : "+v"(beat));
     const uint32_t ms = (m & 0x000F000Fu) ^ cstep;
     st.rec_s[p] = (ms & beat) | (st.rec_s[p] & ~beat);
-    st.rec_h[p] = pk_maxu(st.rec_h[p], mh);
+    st.rec_h[p] = m | 0x000F000Fu;
   }
 }
 
@@ -853,7 +859,7 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
     st.ho[p] = 0;
     st.hg[p] = 0;
     st.best[p] = 0;
-    st.rec_h[p] = 0;
+    st.rec_h[p] = 0x000F000Fu;  // H = 0
     st.rec_s[p] = 0;
   }
   st.xa = 0;  // boundary row: H = 0
