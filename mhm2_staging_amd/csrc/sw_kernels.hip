@@ -1159,20 +1159,24 @@ __device__ __forceinline__ void band_load_segments(band_lds_t* lds, const uint32
   }
 }
 
-// One packed cell update in the forward kernel's biased form (11 VALU ops):
-// h_slot holds 16*H of the slot's diagonal predecessor, ein/fin 16*(E + gO)
-// and 16*(F + gO).
+// One packed cell update in the forward kernel's biased form: h_slot holds
+// 16*H of the slot's diagonal predecessor (the slot's cell one u-step
+// earlier), ein/fin 16*(E + gO) and 16*(F + gO).  The chunk maxima take two
+// u-steps at a time: at an odd u-step (TWO) one three-way max adds this cell
+// (tagged tag) and the predecessor (tagged tag + 1).
+template <bool TWO>
 __device__ __forceinline__ void band_cell(uint32_t r, uint32_t q, uint32_t& h_slot, uint32_t ein, uint32_t fin,
                                           uint32_t& eout, uint32_t& fout, uint32_t& best, uint32_t tag, uint32_t tlo,
                                           uint32_t thi, uint32_t go, uint32_t ge) {
   const uint32_t sel = subst_sel(r, q);
   const uint32_t sp = __builtin_amdgcn_perm(thi, tlo, sel);
-  const uint32_t d = pk_addu_sat(h_slot, sp);
+  const uint32_t prev = h_slot;
+  const uint32_t d = pk_addu_sat(prev, sp);
   const uint32_t h = pk_subu_sat(pk_max3_u16f(d, ein, fin), go);
   h_slot = h;
   eout = pk_max(pk_subu_sat(ein, ge), h);
   fout = pk_max(pk_subu_sat(fin, ge), h);
-  best = pk_maxu(best, h | tag);
+  if constexpr (TWO) best = pk_max3_u16f(best, h | tag, prev | (tag + 0x00010001u));
 }
 
 // One chunk of 8 u-steps (16 cell steps).  rw holds row codes R'[u0+1 ..
@@ -1216,8 +1220,13 @@ __device__ __forceinline__ void band_chunk(BandState<P>& st, int g, int u0, int 
 #pragma unroll
     for (int p = 0; p < P; p++) {
       const uint32_t fin = p ? st.fo[p - 1] : f0;
-      band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s) % P], st.hoe[p], st.eo[p], fin, ee[p], fe[p], st.be[p], tag,
-                tlo, thi, go, ge);
+      if (s & 1) {  // (s is a constant after unrolling)
+        band_cell<true>(st.rr[(p - s + 8 * P) % P], st.qr[(p + s) % P], st.hoe[p], st.eo[p], fin, ee[p], fe[p],
+                        st.be[p], tag, tlo, thi, go, ge);
+      } else {
+        band_cell<false>(st.rr[(p - s + 8 * P) % P], st.qr[(p + s) % P], st.hoe[p], st.eo[p], fin, ee[p], fe[p],
+                         st.be[p], tag, tlo, thi, go, ge);
+      }
     }
     // column code entering the top slot for the odd step: from the stream (the
     // highest lane of a pair), or the code leaving the next lane's slot 0
@@ -1234,8 +1243,13 @@ __device__ __forceinline__ void band_chunk(BandState<P>& st, int g, int u0, int 
 #pragma unroll
     for (int p = 0; p < P; p++) {
       const uint32_t ein = p < P - 1 ? ee[p + 1] : etop;
-      band_cell(st.rr[(p - s + 8 * P) % P], st.qr[(p + s + 1) % P], st.hoo[p], ein, fe[p], st.eo[p], st.fo[p],
-                st.bo[p], tag, tlo, thi, go, ge);
+      if (s & 1) {
+        band_cell<true>(st.rr[(p - s + 8 * P) % P], st.qr[(p + s + 1) % P], st.hoo[p], ein, fe[p], st.eo[p],
+                        st.fo[p], st.bo[p], tag, tlo, thi, go, ge);
+      } else {
+        band_cell<false>(st.rr[(p - s + 8 * P) % P], st.qr[(p + s + 1) % P], st.hoo[p], ein, fe[p], st.eo[p],
+                         st.fo[p], st.bo[p], tag, tlo, thi, go, ge);
+      }
     }
     // row code entering slot 0 for the next u-step: from the stream (the
     // lowest lane of a pair), or the code leaving the lane below's top slot
