@@ -100,6 +100,10 @@ typedef struct {
    * lane, and one whose rows are all short enough one register fewer. */
   int32_t class_lanes[MHMSW_NUM_CLASSES];
   int32_t class_regs[MHMSW_NUM_CLASSES];
+  /* the forward kernel's scale of H (H << class_tag_bits, the low bits
+   * tagging the step of a row's maximum): 4, or 5 for the narrow shapes when
+   * the scoring and the class's rows allow it; 0 for the 32-bit class */
+  int32_t class_tag_bits[MHMSW_NUM_CLASSES];
 } mhmsw_stats;
 
 /* Scheduling parameters of the packed form (no counterpart in the reference,

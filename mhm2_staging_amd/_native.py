@@ -47,7 +47,8 @@ class Stats(ctypes.Structure):
                 ("window_ms", ctypes.c_double), ("window_bases", ctypes.c_int64), ("packed_scoring", ctypes.c_int32),
                 ("device_bytes", ctypes.c_int64), ("host_pack_ms", ctypes.c_double),
                 ("host_upload_ms", ctypes.c_double), ("host_launch_ms", ctypes.c_double),
-                ("class_lanes", ctypes.c_int32 * 17), ("class_regs", ctypes.c_int32 * 17)]
+                ("class_lanes", ctypes.c_int32 * 17), ("class_regs", ctypes.c_int32 * 17),
+                ("class_tag_bits", ctypes.c_int32 * 17)]
 
     def as_dict(self):
         d = {}

@@ -1154,6 +1154,16 @@ constexpr int32_t kSmallJobBandMaxD = 7;
 // blocks per CU of the diagonal class's launch in large jobs
 constexpr int kDiagBlocksPerCu = 8;
 
+// Forward launches keep H as 32 H in the narrow shapes (at most 8 lanes per
+// pair; a fold of the row maxima every 32 steps instead of 16) when the
+// scoring table's bytes still fit ((match + gap open) * 32 <= 255) and no pair
+// of class k can reach kFastOverflowScore32 (a score is at most match times
+// the rows).
+static bool fold32_ok(const mhmsw_ctx* c, int k) {
+  return c->sp.fast_ok && c->sp.match + c->sp.gap_open <= 7 &&
+         (int64_t)c->class_max_rows[k] * c->sp.match < kFastOverflowScore32;
+}
+
 static BatchView make_view(mhmsw_ctx* c) {
   BatchView b;
   std::memset(&b, 0, sizeof(b));  // the graph cache compares views bytewise
@@ -1305,7 +1315,9 @@ static int enqueue_job(mhmsw_ctx* c, const BatchView& b, hipStream_t st, bool ev
   auto fast = [&](const BatchView& v, int group, int P, bool forward, int64_t pairs) {
     const int k = (group == 64 ? kClassFast64 : kClassFast16) + P - 1;
     const FastShape s = fast_shape(group, P, forward, pairs, c->cus, c->class_max_rows[k]);
-    return launch_fast(v, s, grid_cap(pairs, 256 / s.lanes, c->occ_of(s)), st);
+    BatchView w = v;
+    w.fold32 = forward && fold32_ok(c, k);
+    return launch_fast(w, s, grid_cap(pairs, 256 / s.lanes, c->occ_of(s)), st);
   };
   // the band kernels' grids do not follow it either: 2 blocks (8 wavefronts)
   // per CU, or fewer where a kernel keeps fewer resident.  More waves per CU
@@ -1642,6 +1654,7 @@ static int complete(mhmsw_ctx* c) {
                                                        c->class_pairs[k], c->cus, c->class_max_rows[k]);
     s.class_lanes[k] = sh.lanes;
     s.class_regs[k] = sh.regs;
+    s.class_tag_bits[k] = k == kClassPlain ? 0 : sh.lanes <= 8 && fold32_ok(c, k) ? 5 : 4;
   }
   s.prep_ms = ms[1];
   float wm = 0.f;

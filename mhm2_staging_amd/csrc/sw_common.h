@@ -50,6 +50,10 @@ constexpr int kFastMaxCols = 60000;          // step counter stays below 2^16
 // A packed-form pair whose score reaches this is rerun in 32-bit (the packed
 // kernels' values stay exact below 0x7C00 = 16 * 1984).
 constexpr int32_t kFastOverflowScore = 1900;
+// The same for the narrow forward shapes that keep 32 H (values exact below
+// 0x7C00 = 32 * 992 while match + gap open <= 7).  The host gives them only
+// classes whose rows times match stay below it, so none is ever rerun.
+constexpr int32_t kFastOverflowScore32 = 960;
 constexpr int kPlainMaxCols = 8192;          // LDS boundary row of the 32-bit kernel
 
 // Task flags.

@@ -607,10 +607,10 @@ constexpr int kSubCodes = 0, kSubPad = 1, kSubTab = 2;
 
 // The packed cells of one step (substitutions from t1:t0 as above) and the
 // exchange with the neighbouring lanes.
-template <int G, int P, int S, int SUB>
+template <int G, int P, int S, int SUB, int T0>
 __device__ __forceinline__ void fast_cells(FastState<G, P>& st, uint32_t t1, uint32_t t0, uint32_t keep, uint32_t pm,
                                            uint32_t go, uint32_t ge) {
-  constexpr uint32_t tag = (uint32_t)(15 - S) * 0x00010001u;
+  constexpr uint32_t tag = (uint32_t)(T0 - S) * 0x00010001u;
   uint32_t diag = st.xa;
   uint32_t vg = st.vg;
 #pragma unroll
@@ -640,7 +640,7 @@ __device__ __forceinline__ void fast_cells(FastState<G, P>& st, uint32_t t1, uin
     // holds the register's H of the step before (tag one higher), so one
     // three-way max covers both (16*H + tag stays below 0x7C00).
     if constexpr ((S & 1) != 0) {
-      constexpr uint32_t tag_prev = (uint32_t)(16 - S) * 0x00010001u;
+      constexpr uint32_t tag_prev = (uint32_t)(T0 + 1 - S) * 0x00010001u;
       st.best[p] = pk_max3_u16f(st.best[p], h | tag, diag | tag_prev);
     }
   }
@@ -675,7 +675,7 @@ struct StepConst {
   uint32_t tlo, thi, go, ge, wl, wh, amb4;
 };
 
-template <int G, int P, bool EDGE, int SUB, int S>
+template <int G, int P, bool EDGE, int SUB, int S, int T0>
 __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int col_len, int npad,
                                           const StepConst& k) {
   if constexpr (SUB == kSubTab) {
@@ -685,7 +685,7 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
       const bool kill = (uint32_t)(t - 2 * l) >= (uint32_t)col_len;
       tc = kill ? k.amb4 : tc;
     }
-    fast_cells<G, P, S, kSubTab>(st, st.q, tc, 0, 0, k.go, k.ge);
+    fast_cells<G, P, S, kSubTab, T0>(st, st.q, tc, 0, 0, k.go, k.ge);
     st.q = tc;
   } else {
     uint32_t code = st.w & 0xFu;
@@ -703,14 +703,14 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
       keep = (rlo < (uint32_t)col_len ? 0x0000FFFFu : 0u) | (rlo - 1u < (uint32_t)col_len ? 0xFFFF0000u : 0u);
     }
     st.q = (st.q << 16) | code;
-    fast_cells<G, P, S, SUB>(st, k.thi, k.tlo, keep, pm, k.go, k.ge);
+    fast_cells<G, P, S, SUB, T0>(st, k.thi, k.tlo, keep, pm, k.go, k.ge);
   }
 }
 
-template <int G, int P, bool EDGE, int SUB, bool REV>
+template <int G, int P, bool EDGE, int SUB, bool REV, int TB, int T0>
 __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
                                            const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
-                                           const StepConst& k) {
+                                           const StepConst& k, bool fold) {
   // Forward: w takes the 8 codes from dcur:dnext (loaded one refill
   // earlier); the dword after them is requested now and used at the next
   // refill.  Reverse (REV): the window of the next 8 codes lies one dword
@@ -732,42 +732,46 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
     }                                                                         \
   }
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, SUB, 0>(st, t0 + 0, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 1>(st, t0 + 1, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 2>(st, t0 + 2, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 3>(st, t0 + 3, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 4>(st, t0 + 4, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 5>(st, t0 + 5, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 6>(st, t0 + 6, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 7>(st, t0 + 7, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 0, T0>(st, t0 + 0, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 1, T0>(st, t0 + 1, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 2, T0>(st, t0 + 2, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 3, T0>(st, t0 + 3, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 4, T0>(st, t0 + 4, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 5, T0>(st, t0 + 5, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 6, T0>(st, t0 + 6, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 7, T0>(st, t0 + 7, l, col_len, npad, k);
   MHMSW_REFILL();
-  fast_step<G, P, EDGE, SUB, 8>(st, t0 + 8, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 9>(st, t0 + 9, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 10>(st, t0 + 10, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 11>(st, t0 + 11, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 12>(st, t0 + 12, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 13>(st, t0 + 13, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 14>(st, t0 + 14, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 15>(st, t0 + 15, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 8, T0>(st, t0 + 8, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 9, T0>(st, t0 + 9, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 10, T0>(st, t0 + 10, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 11, T0>(st, t0 + 11, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 12, T0>(st, t0 + 12, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 13, T0>(st, t0 + 13, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 14, T0>(st, t0 + 14, l, col_len, npad, k);
+  fast_step<G, P, EDGE, SUB, 15, T0>(st, t0 + 15, l, col_len, npad, k);
 #undef MHMSW_REFILL
-  // Fold the per-row maxima into the records: a row whose maximum 16*H + tag
-  // has a higher H than its record takes the step of the tag, t0 + (15 - tag)
-  // = t0 | (tag ^ 15).  best is never cleared, so it never falls: a maximum
-  // carried over from an earlier chunk is the record itself, and the new
-  // record is best with its tag bits set.
-  const uint32_t cstep = (uint32_t)t0 * 0x00010001u | 0x000F000Fu;
+  // Fold the per-row maxima into the records (at the end of a tag period, or
+  // of the pair): a row whose maximum H' + tag (H' = H << TB) has a higher H
+  // than its record takes the step of the tag, the period's first step plus
+  // (2^TB - 1 - tag), i.e. its first step | (tag ^ (2^TB - 1)).  best is never
+  // cleared, so it never falls: a maximum carried over from an earlier period
+  // is the record itself, and the new record is best with its tag bits set.
+  if (fold) {
+    constexpr uint32_t low = (uint32_t)((1 << TB) - 1) * 0x00010001u;
+    const uint32_t cstep = (uint32_t)(t0 & -(1 << TB)) * 0x00010001u | low;
 #pragma unroll
-  for (int p = 0; p < P; p++) {
-    const uint32_t m = st.best[p];
-    // 0xFFFF in the halves where m > rec_h = 16*Hrec | 15, i.e. where m's H
-    // is above Hrec (values below 0x7C00: the signed difference is exact)
-    uint32_t beat = u32(vi(pk_sub(st.rec_h[p], m)) >> (short)15);
-    // (opaque to the compiler, which would otherwise turn the bit select
-    // below into a compare and a select per half)
-    __asm__("" : "+v"(beat));
-    const uint32_t ms = (m & 0x000F000Fu) ^ cstep;
-    st.rec_s[p] = (ms & beat) | (st.rec_s[p] & ~beat);
-    st.rec_h[p] = m | 0x000F000Fu;
+    for (int p = 0; p < P; p++) {
+      const uint32_t m = st.best[p];
+      // 0xFFFF in the halves where m > rec_h = H'rec | low, i.e. where m's H
+      // is above Hrec (values below 0x7C00: the signed difference is exact)
+      uint32_t beat = u32(vi(pk_sub(st.rec_h[p], m)) >> (short)15);
+      // (opaque to the compiler, which would otherwise turn the bit select
+      // below into a compare and a select per half)
+      __asm__("" : "+v"(beat));
+      const uint32_t ms = (m & low) ^ cstep;
+      st.rec_s[p] = (ms & beat) | (st.rec_s[p] & ~beat);
+      st.rec_h[p] = m | low;
+    }
   }
 }
 
@@ -775,19 +779,47 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
 // some lane may be left of its column 0 (t0 < 2G), middle chunks while every
 // lane is inside its pair (MID_SUB), edge chunks to the end.  Three loops, so
 // that no chunk chooses its form at run time.
-template <int G, int P, int EDGE_SUB, int MID_SUB, bool REV>
+//   TB = 4: H is kept as 16 H; each chunk's 16 steps carry tags 15..0 and the
+//   chunk folds.
+//   TB = 5 (G <= 8, so 2G <= 16): 32 H; a tag period is two chunks (tags
+//   31..16, then 15..0) and folds after the second, or after the first when
+//   that is the pair's last.  The first period is edge chunks, so that the
+//   middle loop runs whole periods.
+template <int G, int P, int EDGE_SUB, int MID_SUB, bool REV, int TB>
 __device__ __forceinline__ void fast_run(FastState<G, P>& st, int nsteps, int mincols, int l, int col_len, int npad,
                                          const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
                                          const StepConst& k) {
   int t0 = 0;
-  const int head = nsteps < 2 * G ? nsteps : 2 * G;
-  for (; t0 < head; t0 += 16) fast_chunk<G, P, true, EDGE_SUB, REV>(st, t0, l, col_len, npad, codes, dw, sh, k);
-  for (; t0 < nsteps && t0 + 16 <= mincols; t0 += 16)
-    fast_chunk<G, P, false, MID_SUB, REV>(st, t0, l, col_len, npad, codes, dw, sh, k);
-  for (; t0 < nsteps; t0 += 16) fast_chunk<G, P, true, EDGE_SUB, REV>(st, t0, l, col_len, npad, codes, dw, sh, k);
+  if constexpr (TB == 4) {
+    const int head = nsteps < 2 * G ? nsteps : 2 * G;
+    for (; t0 < head; t0 += 16)
+      fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+    for (; t0 < nsteps && t0 + 16 <= mincols; t0 += 16)
+      fast_chunk<G, P, false, MID_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+    for (; t0 < nsteps; t0 += 16)
+      fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+  } else {
+    static_assert(2 * G <= 16, "the first chunk holds every lane's column 0");
+    for (; t0 < nsteps && t0 < 32; t0 += 32) {
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 >= nsteps);
+      if (t0 + 16 < nsteps)
+        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+    }
+    for (; t0 + 32 <= nsteps && t0 + 32 <= mincols; t0 += 32) {
+      fast_chunk<G, P, false, MID_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, false);
+      fast_chunk<G, P, false, MID_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+    }
+    for (; t0 < nsteps; t0 += 32) {
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 >= nsteps);
+      if (t0 + 16 < nsteps)
+        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+    }
+  }
 }
 
-template <int G, int P>
+// One task of every group of the wavefront.  TB: H is kept as H << TB (4,
+// or 5 for forward launches whose scoring and rows allow it: see the kernel).
+template <int G, int P, int TB>
 __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task, int l) {
   const bool active = task.pair >= 0;
   const int32_t row_len = active ? task.row_len : 0;
@@ -813,12 +845,14 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
 
   const uint32_t* __restrict__ codes = b.codes;
   StepConst k;
-  k.tlo = b.sc.perm_lo;
-  k.thi = b.sc.perm_hi;
-  k.go = (uint32_t)(16 * b.sc.gap_open) * 0x00010001u;
-  k.ge = (uint32_t)(16 * b.sc.gap_ext) * 0x00010001u;
+  // the host's table holds 16*(s + gO) per byte; at TB = 5 every byte is
+  // below 0x80 (match + gO <= 7), so doubling the words doubles each byte
+  k.tlo = b.sc.perm_lo << (TB - 4);
+  k.thi = b.sc.perm_hi << (TB - 4);
+  k.go = (uint32_t)(b.sc.gap_open << TB) * 0x00010001u;
+  k.ge = (uint32_t)(b.sc.gap_ext << TB) * 0x00010001u;
   {
-    // table bytes: 0 match, 1-3 mismatch, 4-7 ambiguity (all s' = 16*(s + gO))
+    // table bytes: 0 match, 1-3 mismatch, 4-7 ambiguity (all s' = (s + gO) << TB)
     const uint32_t m = k.tlo & 0xFFu, x = (k.tlo >> 8) & 0xFFu, a = k.thi & 0xFFu;
     k.wl = x * 0x00010101u | (m << 24);
     k.wh = x * 0x01010101u;
@@ -826,8 +860,9 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
   }
 
   // uniform: a launch holds the tasks of one pass (padding slots carry no
-  // flags); reverse tasks read the forward codes backwards
-  const bool rev = __ballot(task.flags & kTaskReverse) != 0;
+  // flags); reverse tasks read the forward codes backwards.  (TB = 5 launches
+  // are forward launches.)
+  const bool rev = TB == 4 && __ballot(task.flags & kTaskReverse) != 0;
   FastState<G, P> st;
   // this lane's 2P row codes are consecutive nibbles (descending for a
   // reverse task): NRW + 1 dword loads, then nibble m of the rw words is the
@@ -859,7 +894,7 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
     st.ho[p] = 0;
     st.hg[p] = 0;
     st.best[p] = 0;
-    st.rec_h[p] = 0x000F000Fu;  // H = 0
+    st.rec_h[p] = (uint32_t)((1 << TB) - 1) * 0x00010001u;  // H = 0
     st.rec_s[p] = 0;
   }
   st.xa = 0;  // boundary row: H = 0
@@ -914,16 +949,16 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
 
   if (tab) {
     if (rev) {
-      fast_run<G, P, kSubTab, kSubTab, true>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+      fast_run<G, P, kSubTab, kSubTab, true, TB>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
     } else {
-      fast_run<G, P, kSubTab, kSubTab, false>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+      fast_run<G, P, kSubTab, kSubTab, false, TB>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
     }
   } else if (rev) {
-    fast_run<G, P, kSubCodes, kSubCodes, true>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+    fast_run<G, P, kSubCodes, kSubCodes, true, TB>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
   } else if (pads) {
-    fast_run<G, P, kSubPad, kSubCodes, false>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+    fast_run<G, P, kSubPad, kSubCodes, false, TB>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
   } else {
-    fast_run<G, P, kSubCodes, kSubCodes, false>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
+    fast_run<G, P, kSubCodes, kSubCodes, false, TB>(st, nsteps, mincols, l, col_len, npad, codes, dw, sh, k);
   }
 
   // per-lane best (score, i, j), then the group's.  The lane's first row
@@ -938,7 +973,7 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
   for (int p = 0; p < P; p++) {
 #pragma unroll
     for (int half = 0; half < 2; half++) {
-      const int32_t h = (int32_t)(((st.rec_h[p] >> (16 * half)) & 0xFFFFu) >> 4);
+      const int32_t h = (int32_t)(((st.rec_h[p] >> (16 * half)) & 0xFFFFu) >> TB);
       hmax = h > hmax ? h : hmax;
       const int32_t tf = (int32_t)((st.rec_s[p] >> (16 * half)) & 0xFFFFu);
       const int32_t r = lrow + half * P + p;
@@ -973,7 +1008,7 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
 #pragma unroll
       for (int half = 0; half < 2; half++) {
         const int32_t r = 2 * P * l + half * P + p;
-        const int32_t h = (int32_t)(((st.rec_h[p] >> (16 * half)) & 0xFFFFu) >> 4);
+        const int32_t h = (int32_t)(((st.rec_h[p] >> (16 * half)) & 0xFFFFu) >> TB);
         if (r < row_len && (r < lo || r > hi)) m = h > m ? h : m;
       }
     }
@@ -994,7 +1029,8 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
       key_decode(bestk, e.score, e.i, e.j);
       e.flags = 0;
     }
-    if (hmax >= kFastOverflowScore) e.flags |= kEndOverflow;  // values may have left the exact range: rerun
+    // values may have left the exact range: rerun
+    if (hmax >= (TB == 5 ? kFastOverflowScore32 : kFastOverflowScore)) e.flags |= kEndOverflow;
     // rows past the group's 2GP (a trimmed shape given a taller pair, which
     // the host's class row bound rules out): rerun exactly
     if (row_len > 2 * G * P) e.flags |= kEndOverflow;
@@ -1033,6 +1069,33 @@ constexpr int fast_class_of() {
                                   : kClassFast16 + (G * P + 15) / 16 - 1;
 }
 
+// The class's wave tasks from its work counter.  The next task's number is
+// taken before the current one is aligned, and (shapes of at most kMaxP
+// registers) its record loaded; the shapes of more registers load it
+// afterwards.
+template <int G, int P, int TB>
+__device__ __forceinline__ void fast_tasks(const BatchView& b, int cls, int32_t base, int32_t nwt, int lane, int l,
+                                           int grp) {
+  constexpr int GPW = 64 / G;
+  PassSched* s = b.sched;
+  int w = next_wave_task(&s->class_next[cls], lane);
+  SwTask task;
+  if (w < nwt) task = b.tasks[base + w * GPW + grp];
+  while (w < nwt) {
+    const int wn = next_wave_task(&s->class_next[cls], lane);
+    if constexpr (G >= 16 && P <= kMaxP) {
+      SwTask next;
+      if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
+      fast_pair<G, P, TB>(b, task, l);
+      task = next;
+    } else {
+      fast_pair<G, P, TB>(b, task, l);
+      if (wn < nwt) task = b.tasks[base + wn * GPW + grp];
+    }
+    w = wn;
+  }
+}
+
 template <int G, int P>
 __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(BatchView b) {
   constexpr int GPW = 64 / G;
@@ -1047,25 +1110,17 @@ __global__ __launch_bounds__(256, (fast_min_waves<G, P>())) void sw_fast_kernel(
   // counter (the rest still take every task: each loops until it runs out)
   if ((int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) >= nwt) return;
   const int32_t base = s->class_base[cls];
-  // The next task's number is taken before the current one is aligned, and
-  // (shapes of at most kMaxP registers) its record loaded; the shapes of
-  // more registers load it afterwards.
-  int w = next_wave_task(&s->class_next[cls], lane);
-  SwTask task;
-  if (w < nwt) task = b.tasks[base + w * GPW + grp];
-  while (w < nwt) {
-    const int wn = next_wave_task(&s->class_next[cls], lane);
-    if constexpr (G >= 16 && P <= kMaxP) {
-      SwTask next;
-      if (wn < nwt) next = b.tasks[base + wn * GPW + grp];
-      fast_pair<G, P>(b, task, l);
-      task = next;
-    } else {
-      fast_pair<G, P>(b, task, l);
-      if (wn < nwt) task = b.tasks[base + wn * GPW + grp];
+  // Narrow groups (G <= 8) of forward launches keep H as 32 H when the host
+  // allows it (b.fold32: match + gap open <= 7 and the class's rows times
+  // match below kFastOverflowScore32), which folds the row maxima every 32
+  // steps instead of 16.
+  if constexpr (G <= 8) {
+    if (b.fold32) {
+      fast_tasks<G, P, 5>(b, cls, base, nwt, lane, l, grp);
+      return;
     }
-    w = wn;
   }
+  fast_tasks<G, P, 4>(b, cls, base, nwt, lane, l, grp);
 }
 
 #ifndef MHMSW_FAST_PART

@@ -58,6 +58,11 @@ struct BatchView {
   // widest band (|l - k| <= band_max_d, at most kBandMaxD) a reverse task runs
   // in the one-lane band kernels; wider ones run as rectangles
   int32_t band_max_d;
+  // forward launches of the packed kernel whose narrow shapes (at most 8
+  // lanes per pair) keep H as 32 H and fold their row maxima every 32 steps:
+  // set per launch when match + gap open <= 7 and the class's rows times
+  // match stay below kFastOverflowScore32; 0 otherwise
+  int32_t fold32;
 };
 
 // CIGAR pass over the results of a finished job (cigar_kernels.hip).

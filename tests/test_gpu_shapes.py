@@ -80,10 +80,12 @@ def test_forward_shape_matches_16_lane_groups_and_oracle(gpu_aligner, read_len):
     cls = class_of(read_len)
     assert st["class_pairs"][cls] == NPAIRS
     assert (st["class_lanes"][cls], st["class_regs"][cls]) == SHAPES[read_len]
+    assert st["class_tag_bits"][cls] == 5  # 32 H: match + gap open = 6, rows * match < 960
     want = np.concatenate([table(al.align(sliced(b, lo, min(lo + SLICE, NPAIRS))))
                            for lo in range(0, NPAIRS, SLICE)])
     st = al.stats()
     assert (st["class_lanes"][cls], st["class_regs"][cls]) == (16, cls + 1)
+    assert st["class_tag_bits"][cls] == 4
     assert np.array_equal(got, want)
     assert (got[:, 0] > 0).mean() > 0.9
     idx = np.linspace(0, NPAIRS - 1, 200).astype(np.int64)
@@ -91,6 +93,42 @@ def test_forward_shape_matches_16_lane_groups_and_oracle(gpu_aligner, read_len):
                      m.MHM2_SCORING)
     assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
                                                               "query_end")], 1))
+
+
+# (scoring, read length, H scale of the narrow shape: 32 H needs match + gap open <= 7 and
+# rows * match < 960)
+TAG_BITS_CASES = [((2, 3, 6, 2, 2), 150, 4), ((2, 3, 6, 2, 2), 250, 4), ((4, 3, 3, 2, 2), 150, 5),
+                  ((4, 3, 3, 2, 2), 250, 4), ((1, 2, 6, 1, 1), 60, 5), ((3, 1, 4, 2, 1), 193, 5)]
+
+
+@pytest.mark.parametrize("scoring,read_len,tag_bits", TAG_BITS_CASES)
+def test_forward_shape_h_scale_by_scoring(gpu_aligner, scoring, read_len, tag_bits):
+    """The narrow shapes keep 16 H or 32 H as the scoring and the class's rows allow: both scales
+    against 16-lane groups (16 H) and the oracle under the same scoring."""
+    print(f"[gpu] forward shape, scoring {scoring}, reads of {read_len} bases", flush=True)
+    al = gpu_aligner
+    al.set_scoring(scoring)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(False)
+    try:
+        b = batch_of(read_len, 5000 + read_len + 7 * scoring[0])
+        got = table(al.align(b))
+        st = al.stats()
+        cls = class_of(read_len)
+        assert (st["class_lanes"][cls], st["class_regs"][cls]) == SHAPES[read_len]
+        assert st["class_tag_bits"][cls] == tag_bits
+        want = np.concatenate([table(al.align(sliced(b, lo, min(lo + SLICE, NPAIRS))))
+                               for lo in range(0, NPAIRS, SLICE)])
+        assert np.array_equal(got, want)
+        assert (got[:, 0] > 0).mean() > 0.9
+        idx = np.linspace(0, NPAIRS - 1, 200).astype(np.int64)
+        o = oracle_batch(b.reads, b.read_off[idx], b.read_len[idx], b.ctgs, b.ctg_off[idx], b.ctg_len[idx],
+                         scoring)
+        assert np.array_equal(got[idx], np.stack([o[k] for k in ("score", "ref_begin", "ref_end", "query_begin",
+                                                                  "query_end")], 1))
+    finally:
+        al.set_scoring(m.MHM2_SCORING)
 
 
 @pytest.mark.parametrize("read_len", [30, 60, 100, 150, 193, 250])
