@@ -142,7 +142,7 @@ def class_roofline(st, class_ms, peak_measured, traffic):
           "peak": NOMINAL_PEAK,
           "unit": "TOP/s (int16 lane-ops)", "frac": round(achieved / NOMINAL_PEAK, 4), "traffic": None,
           "kernel_ms": round(float(cm[k]), 3), "kernel_cells": int(kcells), "kernel_pairs": int(st["class_pairs"][k]),
-          "ops_per_cell": VALU_OPS_PER_CELL}
+          "ops_per_cell": VALU_OPS_PER_CELL, "h_scale_bits": int(st["class_tag_bits"][k])}
     if peak_measured:
         rl["peak_measured"] = round(peak_measured["lane_ops_per_s"] / 1e12, 2)
         rl["frac_of_measured"] = round(achieved * 1e12 / peak_measured["lane_ops_per_s"], 4)
@@ -150,7 +150,8 @@ def class_roofline(st, class_ms, peak_measured, traffic):
         rl["traffic"] = traffic["per_launch_bytes"][name]
         rl["traffic_source"] = traffic["source"]
     classes = {_native.class_name(i): {"pairs": st["class_pairs"][i], "cells": st["class_cells"][i],
-                                       "ms": round(float(cm[i]), 3), "kernel": kernel_of(st, i)}
+                                       "ms": round(float(cm[i]), 3), "kernel": kernel_of(st, i),
+                                       "h_scale_bits": int(st["class_tag_bits"][i])}
                for i in range(_native.NUM_CLASSES) if st["class_pairs"][i]}
     return rl, classes
 
