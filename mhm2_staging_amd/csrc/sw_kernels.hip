@@ -707,7 +707,7 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   }
 }
 
-template <int G, int P, bool EDGE, int SUB, bool REV, int TB, int T0>
+template <int G, int P, bool EDGE, int SUB, bool REV, int TB, int T0, int NS = 16>
 __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
                                            const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
                                            const StepConst& k, bool fold) {
@@ -740,15 +740,17 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
   fast_step<G, P, EDGE, SUB, 5, T0>(st, t0 + 5, l, col_len, npad, k);
   fast_step<G, P, EDGE, SUB, 6, T0>(st, t0 + 6, l, col_len, npad, k);
   fast_step<G, P, EDGE, SUB, 7, T0>(st, t0 + 7, l, col_len, npad, k);
-  MHMSW_REFILL();
-  fast_step<G, P, EDGE, SUB, 8, T0>(st, t0 + 8, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 9, T0>(st, t0 + 9, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 10, T0>(st, t0 + 10, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 11, T0>(st, t0 + 11, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 12, T0>(st, t0 + 12, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 13, T0>(st, t0 + 13, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 14, T0>(st, t0 + 14, l, col_len, npad, k);
-  fast_step<G, P, EDGE, SUB, 15, T0>(st, t0 + 15, l, col_len, npad, k);
+  if constexpr (NS == 16) {  // (a pair's last chunk may be 8 steps)
+    MHMSW_REFILL();
+    fast_step<G, P, EDGE, SUB, 8, T0>(st, t0 + 8, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 9, T0>(st, t0 + 9, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 10, T0>(st, t0 + 10, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 11, T0>(st, t0 + 11, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 12, T0>(st, t0 + 12, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 13, T0>(st, t0 + 13, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 14, T0>(st, t0 + 14, l, col_len, npad, k);
+    fast_step<G, P, EDGE, SUB, 15, T0>(st, t0 + 15, l, col_len, npad, k);
+  }
 #undef MHMSW_REFILL
   // Fold the per-row maxima into the records (at the end of a tag period, or
   // of the pair): a row whose maximum H' + tag (H' = H << TB) has a higher H
@@ -796,8 +798,9 @@ __device__ __forceinline__ void fast_run(FastState<G, P>& st, int nsteps, int mi
       fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
     for (; t0 < nsteps && t0 + 16 <= mincols; t0 += 16)
       fast_chunk<G, P, false, MID_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
-    for (; t0 < nsteps; t0 += 16)
+    for (; t0 + 16 <= nsteps; t0 += 16)
       fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+    if (t0 < nsteps) fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15, 8>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
   } else {
     static_assert(2 * G <= 16, "the first chunk holds every lane's column 0");
     for (; t0 < nsteps && t0 < 32; t0 += 32) {
@@ -809,10 +812,17 @@ __device__ __forceinline__ void fast_run(FastState<G, P>& st, int nsteps, int mi
       fast_chunk<G, P, false, MID_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, false);
       fast_chunk<G, P, false, MID_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
     }
-    for (; t0 < nsteps; t0 += 32) {
-      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 >= nsteps);
+    for (; t0 + 32 <= nsteps; t0 += 32) {
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, false);
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+    }
+    // the last 8, 16 or 24 steps
+    if (t0 + 16 <= nsteps) {
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 == nsteps);
       if (t0 + 16 < nsteps)
-        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15, 8>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+    } else if (t0 < nsteps) {
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31, 8>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
     }
   }
 }
@@ -841,7 +851,9 @@ __device__ __forceinline__ void fast_pair(const BatchView& b, const SwTask& task
     }
   }
   if (nsteps_need <= 0) return;
-  const int nsteps = (nsteps_need + 15) & ~15;
+  // whole 8-step halves of a chunk (the column stream refills 8 codes at a
+  // time); every chunk but a pair's last runs 16 steps
+  const int nsteps = (nsteps_need + 7) & ~7;
 
   const uint32_t* __restrict__ codes = b.codes;
   StepConst k;
