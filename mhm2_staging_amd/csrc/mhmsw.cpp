@@ -1850,10 +1850,10 @@ extern "C" int mhmsw_autotune(mhmsw_ctx* c, int repeats, mhmsw_tuning* best_out)
   int rc = time_job(c, repeats, &best_ms);
   static const int kBlocks[] = {2, 4, 6, 8, 12};
   static const int kG64[] = {257, 225, 193, 161, 129};
-  static const int kBins[] = {4, 5, 6};
+  static const int kBins[] = {3, 4, 5, 6};
   for (int round = 0; rc == MHMSW_OK && round < 3; round++) {
     const int* vals = round == 0 ? kBlocks : round == 1 ? kG64 : kBins;
-    const int nvals = round == 0 ? 5 : round == 1 ? 5 : 3;
+    const int nvals = round == 0 ? 5 : round == 1 ? 5 : 4;
     for (int i = 0; i < nvals && rc == MHMSW_OK; i++) {
       mhmsw_tuning t = best;
       if (round == 0) t.blocks_per_cu = vals[i];
