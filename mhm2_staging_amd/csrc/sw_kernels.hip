@@ -707,10 +707,10 @@ __device__ __forceinline__ void fast_step(FastState<G, P>& st, int t, int l, int
   }
 }
 
-template <int G, int P, bool EDGE, int SUB, bool REV, int TB, int T0, int NS = 16>
+template <int G, int P, bool EDGE, int SUB, bool REV, int TB, int T0>
 __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, int col_len, int npad,
                                            const uint32_t* __restrict__ codes, int64_t& dw, uint32_t sh,
-                                           const StepConst& k, bool fold) {
+                                           const StepConst& k, bool fold, bool half = false) {
   // Forward: w takes the 8 codes from dcur:dnext (loaded one refill
   // earlier); the dword after them is requested now and used at the next
   // refill.  Reverse (REV): the window of the next 8 codes lies one dword
@@ -740,7 +740,8 @@ __device__ __forceinline__ void fast_chunk(FastState<G, P>& st, int t0, int l, i
   fast_step<G, P, EDGE, SUB, 5, T0>(st, t0 + 5, l, col_len, npad, k);
   fast_step<G, P, EDGE, SUB, 6, T0>(st, t0 + 6, l, col_len, npad, k);
   fast_step<G, P, EDGE, SUB, 7, T0>(st, t0 + 7, l, col_len, npad, k);
-  if constexpr (NS == 16) {  // (a pair's last chunk may be 8 steps)
+  // a pair's last chunk (an edge chunk) may end after 8 steps (half; uniform)
+  if (!EDGE || !half) {
     MHMSW_REFILL();
     fast_step<G, P, EDGE, SUB, 8, T0>(st, t0 + 8, l, col_len, npad, k);
     fast_step<G, P, EDGE, SUB, 9, T0>(st, t0 + 9, l, col_len, npad, k);
@@ -795,34 +796,30 @@ __device__ __forceinline__ void fast_run(FastState<G, P>& st, int nsteps, int mi
   if constexpr (TB == 4) {
     const int head = nsteps < 2 * G ? nsteps : 2 * G;
     for (; t0 < head; t0 += 16)
-      fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+      fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true, t0 + 8 >= nsteps);
     for (; t0 < nsteps && t0 + 16 <= mincols; t0 += 16)
       fast_chunk<G, P, false, MID_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
-    for (; t0 + 16 <= nsteps; t0 += 16)
-      fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
-    if (t0 < nsteps) fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15, 8>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+    for (; t0 < nsteps; t0 += 16)
+      fast_chunk<G, P, true, EDGE_SUB, REV, 4, 15>(st, t0, l, col_len, npad, codes, dw, sh, k, true, t0 + 8 >= nsteps);
   } else {
     static_assert(2 * G <= 16, "the first chunk holds every lane's column 0");
     for (; t0 < nsteps && t0 < 32; t0 += 32) {
-      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 >= nsteps);
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 >= nsteps,
+                                                   t0 + 8 >= nsteps);
       if (t0 + 16 < nsteps)
-        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
+        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true,
+                                                     t0 + 24 >= nsteps);
     }
     for (; t0 + 32 <= nsteps && t0 + 32 <= mincols; t0 += 32) {
       fast_chunk<G, P, false, MID_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, false);
       fast_chunk<G, P, false, MID_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
     }
-    for (; t0 + 32 <= nsteps; t0 += 32) {
-      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, false);
-      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
-    }
-    // the last 8, 16 or 24 steps
-    if (t0 + 16 <= nsteps) {
-      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 == nsteps);
+    for (; t0 < nsteps; t0 += 32) {
+      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31>(st, t0, l, col_len, npad, codes, dw, sh, k, t0 + 16 >= nsteps,
+                                                   t0 + 8 >= nsteps);
       if (t0 + 16 < nsteps)
-        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15, 8>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true);
-    } else if (t0 < nsteps) {
-      fast_chunk<G, P, true, EDGE_SUB, REV, 5, 31, 8>(st, t0, l, col_len, npad, codes, dw, sh, k, true);
+        fast_chunk<G, P, true, EDGE_SUB, REV, 5, 15>(st, t0 + 16, l, col_len, npad, codes, dw, sh, k, true,
+                                                     t0 + 24 >= nsteps);
     }
   }
 }
