@@ -167,6 +167,12 @@ int mhmsw_set_cigar(mhmsw_ctx *ctx, int enable);
  * replayed while it stays the same (batches that change every call never pay
  * for a capture).  0: launches are always issued one by one. */
 int mhmsw_set_graphs(mhmsw_ctx *ctx, int enable);
+/* No counterpart in the reference (its driver times nothing).  1 (default):
+ * an eagerly launched job records events around each phase and class, which
+ * mhmsw_get_stats reports.  0: only the job's own two events, as for a
+ * replayed graph (job_ms only), which saves their host time in every call;
+ * the drop-in adept_sw::GPUDriver turns it off. */
+int mhmsw_set_phase_timing(mhmsw_ctx *ctx, int enable);
 int mhmsw_set_tuning(mhmsw_ctx *ctx, const mhmsw_tuning *tuning);
 int mhmsw_get_tuning(const mhmsw_ctx *ctx, mhmsw_tuning *tuning);
 /* Times candidate tunings on the staged batch (coordinate descent over the

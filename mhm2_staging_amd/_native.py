@@ -96,7 +96,7 @@ class SynthBatch(ctypes.Structure):
 # every symbol include/mhmsw.h and include/mhmsw_synth.h declare
 EXPORTS = [
     "mhmsw_device_count", "mhmsw_device_total_mem", "mhmsw_device_name", "mhmsw_create", "mhmsw_destroy",
-    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_score2", "mhmsw_set_cigar", "mhmsw_set_graphs",
+    "mhmsw_last_error", "mhmsw_set_scoring", "mhmsw_set_orientation", "mhmsw_set_mode", "mhmsw_set_score2", "mhmsw_set_cigar", "mhmsw_set_graphs", "mhmsw_set_phase_timing",
     "mhmsw_set_tuning", "mhmsw_get_tuning", "mhmsw_autotune", "mhmsw_stage",
     "mhmsw_align_staged", "mhmsw_submit", "mhmsw_stage_seqs", "mhmsw_submit_seqs", "mhmsw_query", "mhmsw_wait", "mhmsw_results", "mhmsw_results_score2", "mhmsw_results_cigar", "mhmsw_get_stats",
     "mhmsw_synth_generate", "mhmsw_synth_free",
@@ -133,6 +133,7 @@ def lib():
         "mhmsw_set_score2": [vp, ctypes.c_int],
         "mhmsw_set_cigar": [vp, ctypes.c_int],
         "mhmsw_set_graphs": [vp, ctypes.c_int],
+        "mhmsw_set_phase_timing": [vp, ctypes.c_int],
         "mhmsw_set_tuning": [vp, ctypes.POINTER(Tuning)],
         "mhmsw_get_tuning": [vp, ctypes.POINTER(Tuning)],
         "mhmsw_autotune": [vp, ctypes.c_int, ctypes.POINTER(Tuning)],

@@ -131,6 +131,10 @@ struct mhmsw_ctx {
   // hipGraph of the whole job, replayed while the staged geometry is unchanged
   bool use_graphs = true;
   bool last_job_graph = false;
+  // per-phase and per-class events of eager jobs (mhmsw_set_phase_timing);
+  // last_job_phases: the last job recorded them
+  bool phase_timing = true;
+  bool last_job_phases = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_key;
@@ -1507,6 +1511,7 @@ static int align_staged_impl(mhmsw_ctx* c) {
     graph = (c->graph_exec && key == c->graph_key) || key == c->eager_key;
   }
   c->last_job_graph = graph;
+  c->last_job_phases = !graph && c->phase_timing;
   c->last_job_windows = c->res != nullptr && c->npairs > 0;
   c->last_job_tail_results = (c->tail_results || c->job_host_results) && c->npairs > 0;
   if (c->last_job_tail_results) HIP_OR_FAIL(c, c->h_results.ensure((size_t)c->npairs * sizeof(AlnResult)));
@@ -1525,8 +1530,12 @@ static int align_staged_impl(mhmsw_ctx* c) {
     return e;
   };
   if (!graph) {
-    int rc = enqueue_job(c, b, st, true, &maxp16, &maxp64);
+    // without phase timing only the job's two events are recorded, as for a
+    // replayed graph (each event record is host time in every call)
+    if (!c->phase_timing) HIP_OR_FAIL(c, hipEventRecord(c->ev[0], st));
+    int rc = enqueue_job(c, b, st, c->phase_timing, &maxp16, &maxp64);
     if (rc != MHMSW_OK) return rc;
+    if (!c->phase_timing) HIP_OR_FAIL(c, hipEventRecord(c->ev[kEvents - 1], st));
     HIP_OR_FAIL(c, enqueue_tail());
     if (c->use_graphs) c->eager_key = key;
   } else {
@@ -1559,6 +1568,13 @@ static int align_staged_impl(mhmsw_ctx* c) {
   }
   c->in_flight = true;
   c->have_results = false;
+  return MHMSW_OK;
+}
+
+int mhmsw_set_phase_timing(mhmsw_ctx* c, int enable) {
+  if (!c) return MHMSW_E_ARG;
+  if (c->in_flight) return fail(c, MHMSW_E_STATE, "phase timing changed while a batch is in flight");
+  c->phase_timing = enable != 0;
   return MHMSW_OK;
 }
 
@@ -1627,7 +1643,7 @@ static int complete(mhmsw_ctx* c) {
   std::memcpy(counters, c->tail->counters, sizeof(counters));
   float ms[kEvents] = {};
   float job = 0.f;
-  if (c->last_job_graph) {
+  if (!c->last_job_phases) {
     HIP_OR_FAIL(c, hipEventElapsedTime(&job, c->ev[0], c->ev[kEvents - 1]));
   } else {
     for (int i = 1; i < kEvents; i++) HIP_OR_FAIL(c, hipEventElapsedTime(&ms[i], c->ev[i - 1], c->ev[i]));
@@ -1645,7 +1661,7 @@ static int complete(mhmsw_ctx* c) {
     s.class_pairs[k] = c->class_pairs[k];
     s.class_cells[k] = c->class_cells[k];
     float cm = 0.f;
-    if (c->class_launched[k] && !c->last_job_graph)
+    if (c->class_launched[k] && c->last_job_phases)
       HIP_OR_FAIL(c, hipEventElapsedTime(&cm, c->cev[k][0], c->cev[k][1]));
     s.class_ms[k] = cm;
     const FastShape sh = k == kClassPlain ? FastShape{0, 0}
@@ -1658,7 +1674,7 @@ static int complete(mhmsw_ctx* c) {
   }
   s.prep_ms = ms[1];
   float wm = 0.f;
-  if (c->last_job_windows && !c->last_job_graph) HIP_OR_FAIL(c, hipEventElapsedTime(&wm, c->wev[0], c->wev[1]));
+  if (c->last_job_windows && c->last_job_phases) HIP_OR_FAIL(c, hipEventElapsedTime(&wm, c->wev[0], c->wev[1]));
   s.window_ms = wm;
   s.window_bases = 0;
   if (c->last_job_windows)

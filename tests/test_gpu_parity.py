@@ -245,3 +245,28 @@ def test_tunings_give_identical_results_and_autotune(gpu_aligner):
         assert_same(gpu_table(al.results()), want, b)
     finally:
         al.set_tuning()
+
+
+def test_phase_timing_off_same_results_job_time_only(gpu_aligner):
+    """mhmsw_set_phase_timing(ctx, 0): an eager job records only its own two events (the drop-in
+    driver's setting); results are unchanged and the stats report job_ms without per-phase times."""
+    print("[gpu] phase timing off, 10k pairs", flush=True)
+    al = gpu_aligner
+    al.set_scoring(m.MHM2_SCORING)
+    al.set_orientation(True)
+    al.set_mode(m.MODE_FAST)
+    al.set_score2(False)
+    b = m.generate(1)
+    al.set_graphs(False)
+    try:
+        want = al.align(b)
+        assert al.stats()["fwd_fast_ms"] > 0
+        al._check(al._L.mhmsw_set_phase_timing(al._ctx, 0), "set_phase_timing")
+        got = al.align(b)
+        st = al.stats()
+    finally:
+        al._check(al._L.mhmsw_set_phase_timing(al._ctx, 1), "set_phase_timing")
+        al.set_graphs(True)
+    for k in ("score", "read_begin", "read_end", "ctg_begin", "ctg_end"):
+        assert np.array_equal(got[k], want[k])
+    assert st["job_ms"] > 0 and st["fwd_fast_ms"] == 0 and st["rev_ms"] == 0
